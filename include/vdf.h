@@ -1,0 +1,164 @@
+/*
+ * vdf.h -- C ABI of the MI355X (gfx950) radix-argsort / group-by library
+ *          (libvdf_hip.so).
+ *
+ * Conventions
+ *   - Every entry point returns vdf_status (0 = VDF_OK).  On failure,
+ *     vdf_last_error() returns a message for the calling host thread.
+ *   - Data pointers named d_* are device pointers (hipMalloc'ed or equivalent)
+ *     owned by the caller.  Scratch memory is owned by the handle.
+ *   - `stream` is a hipStream_t passed as void* (NULL = the null stream).
+ *     All work is stream-ordered; an entry point returns once its work is
+ *     enqueued.  Results are ready when the stream has reached that point.
+ *   - A handle serialises its own calls (internal mutex) and owns one scratch
+ *     arena; use one handle per concurrently used stream.
+ *   - Rows per single sort: n <= VDF_MAX_ROWS (2^30 - 1); row ids are 32 bit
+ *     unless VDF_SORT_PERM64 asks for 64-bit (size_t) ids.
+ *
+ * Reference interfaces replaced (paths relative to include/vison/ of VisonDF):
+ *   vdf_argsort     sort_{uintegers,integers,char}(std::vector<size_t>& idx,
+ *                   const T* col, size_t nrow)
+ *                     inlines/transform/sort_by/sort_uintegers.inl:8-13
+ *                     inlines/transform/sort_by/sort_integers.inl:8-13
+ *                     inlines/transform/sort_by/sort_char.inl:9-15
+ *                   and the radix kernels they call,
+ *                     inlines/radix_sorts/radix_sort_uint32{,_mt}.inl:5-7 / 6-8
+ *                     inlines/radix_sorts/radix_sort_uint64{,_mt}.inl:5-7 / 6-8
+ *                     inlines/radix_sorts/radix_sort_charbuf_flat.inl:4-8
+ *   vdf_gather      permute_block_mt(storage, ..., idx, nrow)
+ *                     inlines/transform/sort_by/permute_block_mt.inl:5-10,39-41
+ *   vdf_groupby     transform_group_by<..., Function, ...>(key_cols, n_col, ...)
+ *                     transform/transform_group_by/transform_group_by.hpp:11-14
+ *                     per-row loops inlines/transform/transform_group_by/
+ *                     onecol_functions.inl:4-130, create_value_col.inl:36-61,
+ *                     group-contiguous rows hard/create_value_col_hard.inl:15-106
+ */
+#ifndef VDF_H
+#define VDF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct vdf_handle vdf_handle;
+typedef int vdf_status;
+
+/* status codes */
+#define VDF_OK               0
+#define VDF_ERR_INVALID      1 /* bad argument: null pointer, unknown type, size limit */
+#define VDF_ERR_UNSUPPORTED  2 /* combination not implemented */
+#define VDF_ERR_OOM          3 /* device allocation failed */
+#define VDF_ERR_HIP          4 /* a HIP runtime call failed */
+#define VDF_ERR_BUDGET       5 /* request exceeds the handle's HBM budget */
+
+/* key column types (sort_by_mt.hpp:72-115 type letters u / i / c) */
+#define VDF_KEY_U32     0
+#define VDF_KEY_I32     1
+#define VDF_KEY_U64     2
+#define VDF_KEY_I64     3
+#define VDF_KEY_CHAR16  4 /* fixed-width 16-byte char[16] cells (CharT = char[16]) */
+
+/* value column types */
+#define VDF_VAL_F64 0
+#define VDF_VAL_I64 1
+#define VDF_VAL_I32 2
+#define VDF_VAL_U32 3
+#define VDF_VAL_U64 4
+
+/* vdf_argsort flags */
+#define VDF_SORT_DESC          0x1u /* stable descending (INTERNALS.doc:1610-1630)          */
+#define VDF_SORT_DESC_REVERSE  0x2u /* ascending, then reversed: the reference code's        */
+                                    /* descending path (sort_uintegers.inl:216-218)         */
+#define VDF_SORT_PERM64        0x4u /* d_perm_out is uint64_t[n] (size_t ids, as            */
+                                    /* sort_by_mt.hpp:21-24), else uint32_t[n]              */
+
+/* vdf_groupby aggregate mask (GroupFunction, types/supported_group_methods.inl:1-7;
+ * MIN / MAX replace Gather + a user min/max functor, create_value_col.inl:48-58) */
+#define VDF_AGG_COUNT 0x1u
+#define VDF_AGG_SUM   0x2u
+#define VDF_AGG_MEAN  0x4u
+#define VDF_AGG_MIN   0x8u
+#define VDF_AGG_MAX   0x10u
+#define VDF_AGG_ALL   0x1Fu
+
+#define VDF_MAX_ROWS        ((1ull << 30) - 1ull)
+#define VDF_MAX_VALUE_COLS  4
+
+/* ---- library / handle ---- */
+const char* vdf_last_error(void);
+const char* vdf_version(void);
+
+/* Create a handle on `device`.  hbm_budget_bytes caps the handle's scratch
+ * arena (0 = no cap beyond the device's free memory). */
+vdf_status vdf_create(vdf_handle** out, int device, uint64_t hbm_budget_bytes);
+vdf_status vdf_destroy(vdf_handle* h);
+
+/* Scratch bytes vdf_argsort / vdf_groupby need for n rows (for budgeting). */
+uint64_t vdf_argsort_scratch_bytes(int key_type, uint64_t n, uint32_t flags, int want_keys);
+uint64_t vdf_groupby_scratch_bytes(int key_type, uint64_t n);
+
+/* ---- stable LSD radix argsort ----
+ * d_perm_out[r] = row id of the r-th smallest key (stable: equal keys keep
+ * row order).  d_sorted_keys_out (nullable) receives the keys in that order.
+ * Replaces sort_{uintegers,integers,char} + radix_sort_* (see header). */
+vdf_status vdf_argsort(vdf_handle* h, int key_type, const void* d_keys, uint64_t n,
+                       uint32_t flags, void* d_perm_out, void* d_sorted_keys_out,
+                       void* stream);
+
+/* ---- permutation gather: d_dst[r] = d_src[d_perm[r]] ----
+ * elem_bytes in {4, 8, 16}; perm is uint32_t[n] or (perm64) uint64_t[n].
+ * Replaces permute_block_mt.inl:39-41 for one column. */
+vdf_status vdf_gather(vdf_handle* h, uint32_t elem_bytes, const void* d_src,
+                      const void* d_perm, int perm64, uint64_t n, void* d_dst,
+                      void* stream);
+
+/* ---- sort-based group-by ---- */
+typedef struct vdf_value_col {
+    int type;          /* VDF_VAL_* */
+    const void* data;  /* device, n elements */
+} vdf_value_col;
+
+typedef struct vdf_agg_out {
+    void* sum;    /* f64 column: double[G]; integer column: int64_t[G], mod 2^64 */
+    double* mean; /* double[G]: sum / count (integer columns: exact sum / count) */
+    void* min;    /* value type [G]; first of equal values in row order */
+    void* max;    /* value type [G] */
+} vdf_agg_out;
+
+typedef struct vdf_groupby_out {
+    uint64_t* n_groups;  /* device, 1 element: G                                    */
+    void* group_keys;    /* device, key type, capacity n: keys in ascending order    */
+    uint32_t* offsets;   /* device, capacity n + 1: group g is perm[offsets[g] ..   */
+                         /* offsets[g+1]) ; offsets[G] = n                          */
+    uint32_t* counts;    /* device, capacity n (nullable)                           */
+    uint32_t* perm;      /* device, capacity n (nullable): group-contiguous row ids */
+                         /* in row order within each group (row_view_idx analogue) */
+    vdf_agg_out aggs[VDF_MAX_VALUE_COLS]; /* per value column; null members skipped */
+} vdf_groupby_out;
+
+/* Group rows by one key column and aggregate up to VDF_MAX_VALUE_COLS value
+ * columns.  Groups come out in ascending key order (the reference emits
+ * first-occurrence order; compare after ordering by key).  agg_mask selects
+ * which members of out->aggs are written (null members are skipped too).
+ * Replaces transform_group_by (see header). */
+vdf_status vdf_groupby(vdf_handle* h, int key_type, const void* d_keys, uint64_t n,
+                       const vdf_value_col* vals, int nvals, uint32_t agg_mask,
+                       vdf_groupby_out* out, void* stream);
+
+/* ---- per-kernel timing (HIP events on the launch stream) ----
+ * While enabled, every kernel launch of this handle is bracketed by events.
+ * vdf_profile_read synchronises the recorded events and returns, for one
+ * kernel name, total milliseconds, launch count and algorithmic HBM bytes
+ * summed over launches that did work. */
+vdf_status vdf_profile_enable(vdf_handle* h, int on);
+vdf_status vdf_profile_reset(vdf_handle* h);
+vdf_status vdf_profile_read(vdf_handle* h, const char* kernel, double* total_ms,
+                            uint64_t* launches, uint64_t* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VDF_H */

@@ -1,0 +1,156 @@
+"""ctypes access to the oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: the checker for the HIP library and the CPU
+baseline.  Only tests/, the smoke check in __graft_entry__.py and the
+cpu_baseline leg of bench.py import this package; the product never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+REF_LIB_PATH = os.path.join(_HERE, "_ref", "libvref.so")
+
+KEY_U32, KEY_I32, KEY_U64, KEY_I64, KEY_CHAR16 = 0, 1, 2, 3, 4
+VAL_F64, VAL_I64, VAL_I32, VAL_U32, VAL_U64 = 0, 1, 2, 3, 4
+
+_lib = None
+_ref = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} missing: run `make -C oracle`")
+        L = C.CDLL(LIB_PATH)
+        vp, sz, i32 = C.c_void_p, C.c_size_t, C.c_int
+        L.orc_argsort.restype = i32
+        L.orc_argsort.argtypes = [i32, vp, sz, i32, i32, vp]
+        L.orc_gather.restype = None
+        L.orc_gather.argtypes = [sz, vp, vp, sz, vp, i32]
+        L.orc_exact_sum.restype = C.c_double
+        L.orc_exact_sum.argtypes = [vp, sz]
+        L.orc_groupby.restype = C.c_uint64
+        L.orc_groupby.argtypes = [i32, vp, sz, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, i32]
+        L.orc_hash_groupby_f64.restype = C.c_uint64
+        L.orc_hash_groupby_f64.argtypes = [i32, vp, sz, vp, i32, i32, vp, vp, vp, vp, vp, vp]
+        L.orc_hash_bytes16.restype = C.c_uint64
+        L.orc_hash_bytes16.argtypes = [vp, sz]
+        for name in ("orc_radix_sort_u32", "orc_radix_sort_u64"):
+            getattr(L, name).restype = None
+            getattr(L, name).argtypes = [vp, vp, sz]
+        for name in ("orc_radix_sort_u32_mt", "orc_radix_sort_u64_mt"):
+            getattr(L, name).restype = None
+            getattr(L, name).argtypes = [vp, vp, sz, i32]
+        L.orc_radix_sort_charbuf.restype = None
+        L.orc_radix_sort_charbuf.argtypes = [vp, sz, sz, vp]
+        _lib = L
+    return _lib
+
+
+def ref_lib():
+    """The reference's own radix .inl files compiled in place (None if unbuilt)."""
+    global _ref
+    if _ref is None and os.path.exists(REF_LIB_PATH):
+        L = C.CDLL(REF_LIB_PATH)
+        vp, sz = C.c_void_p, C.c_size_t
+        for name in ("vref_radix_sort_u32", "vref_radix_sort_u32_mt8", "vref_radix_sort_u64",
+                     "vref_radix_sort_u64_mt8"):
+            getattr(L, name).restype = None
+            getattr(L, name).argtypes = [vp, vp, sz]
+        L.vref_radix_sort_charbuf.restype = None
+        L.vref_radix_sort_charbuf.argtypes = [vp, sz, sz, vp]
+        _ref = L
+    return _ref
+
+
+def _p(a: np.ndarray):
+    return C.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def key_type_of(a: np.ndarray) -> int:
+    if a.dtype == np.uint8 and a.ndim == 2 and a.shape[1] == 16:
+        return KEY_CHAR16
+    return {np.dtype(np.uint32): KEY_U32, np.dtype(np.int32): KEY_I32,
+            np.dtype(np.uint64): KEY_U64, np.dtype(np.int64): KEY_I64}[a.dtype]
+
+
+def val_type_of(a: np.ndarray) -> int:
+    return {np.dtype(np.float64): VAL_F64, np.dtype(np.int64): VAL_I64, np.dtype(np.int32): VAL_I32,
+            np.dtype(np.uint32): VAL_U32, np.dtype(np.uint64): VAL_U64}[a.dtype]
+
+
+def argsort(keys: np.ndarray, mode: int = 0, threads: int = 1) -> np.ndarray:
+    """Stable argsort as the reference computes it; mode 0 asc, 1 stable desc, 2 reversed asc."""
+    keys = np.ascontiguousarray(keys)
+    n = keys.shape[0]
+    perm = np.empty(n, dtype=np.uint64)
+    rc = lib().orc_argsort(key_type_of(keys), _p(keys), n, mode, threads, _p(perm))
+    assert rc == 0
+    return perm
+
+
+def gather(src: np.ndarray, perm: np.ndarray, threads: int = 1) -> np.ndarray:
+    src = np.ascontiguousarray(src)
+    perm = np.ascontiguousarray(perm, dtype=np.uint64)
+    out = np.empty((perm.shape[0],) + src.shape[1:], dtype=src.dtype)
+    eb = src.dtype.itemsize * (src.shape[1] if src.ndim == 2 else 1)
+    lib().orc_gather(eb, _p(src), _p(perm), perm.shape[0], _p(out), threads)
+    return out
+
+
+def exact_sum(x: np.ndarray) -> float:
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    return lib().orc_exact_sum(_p(x), x.shape[0])
+
+
+def groupby(keys: np.ndarray, vals=(), threads: int = 1) -> dict:
+    """Sort-based group-by (ascending keys): keys, offsets, counts, perm and per
+    value column sum / mean / min / max."""
+    keys = np.ascontiguousarray(keys)
+    n = keys.shape[0]
+    nv = len(vals)
+    vals = [np.ascontiguousarray(v) for v in vals]
+    vt = np.array([val_type_of(v) for v in vals] or [0], dtype=np.int32)
+    vptr = (C.c_void_p * max(nv, 1))(*[v.ctypes.data for v in vals])
+    gk = np.empty_like(keys) if n else np.empty((0,) + keys.shape[1:], dtype=keys.dtype)
+    off = np.empty(n + 1, dtype=np.uint64)
+    cnt = np.empty(max(n, 1), dtype=np.uint64)
+    perm = np.empty(max(n, 1), dtype=np.uint64)
+    sums, means, mins, maxs = [], [], [], []
+    for v in vals:
+        sums.append(np.empty(max(n, 1), dtype=np.float64 if v.dtype == np.float64 else np.int64))
+        means.append(np.empty(max(n, 1), dtype=np.float64))
+        mins.append(np.empty(max(n, 1), dtype=v.dtype))
+        maxs.append(np.empty(max(n, 1), dtype=v.dtype))
+
+    def arr(lst):
+        return (C.c_void_p * max(nv, 1))(*[a.ctypes.data for a in lst])
+
+    g = lib().orc_groupby(key_type_of(keys), _p(keys), n, nv, _p(vt), vptr, _p(gk), _p(off), _p(cnt), _p(perm),
+                          arr(sums), arr(means), arr(mins), arr(maxs), threads)
+    g = int(g)
+    return {
+        "n_groups": g, "keys": gk[:g], "offsets": off[:g + 1], "counts": cnt[:g], "perm": perm[:n],
+        "aggs": [{"sum": sums[c][:g], "mean": means[c][:g], "min": mins[c][:g], "max": maxs[c][:g]}
+                 for c in range(nv)],
+    }
+
+
+def hash_groupby_f64(keys: np.ndarray, val: np.ndarray, threads: int = 1, refapi: bool = False) -> dict:
+    """The reference's hash group-by restated (first-occurrence group order)."""
+    keys = np.ascontiguousarray(keys)
+    val = np.ascontiguousarray(val, dtype=np.float64)
+    n = keys.shape[0]
+    gk = np.empty_like(keys)
+    cnt = np.empty(max(n, 1), dtype=np.uint64)
+    s, m, mn, mx = (np.empty(max(n, 1), dtype=np.float64) for _ in range(4))
+    g = int(lib().orc_hash_groupby_f64(key_type_of(keys), _p(keys), n, _p(val), threads, 1 if refapi else 0,
+                                       _p(gk), _p(cnt), _p(s), _p(m), _p(mn), _p(mx)))
+    return {"n_groups": g, "keys": gk[:g], "counts": cnt[:g], "sum": s[:g], "mean": m[:g], "min": mn[:g],
+            "max": mx[:g]}

@@ -1,0 +1,25 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU and the built libvdf_hip.so")
+
+
+@pytest.fixture(scope="session")
+def vdf():
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from visondf_amd.device import Vdf
+
+    h = Vdf(0)
+    yield h
+    h.close()

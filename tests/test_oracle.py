@@ -1,0 +1,254 @@
+"""Pins the oracle (CPU restatement) against checks that are not the oracle itself:
+the reference's own radix output (golden vectors), the INTERNALS.doc sort_by
+example, numpy/pandas/math.fsum library routines and brute force."""
+from __future__ import annotations
+
+import hashlib
+import json
+import math
+import os
+from fractions import Fraction
+
+import numpy as np
+import pandas as pd
+import pytest
+
+import oracle
+from tests.golden import cases
+from visondf_amd import datagen
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "radix_golden.json")
+
+
+def _golden():
+    with open(GOLDEN) as f:
+        return json.load(f)["cases"]
+
+
+def _order_key(keys: np.ndarray) -> np.ndarray:
+    """The unsigned order-preserving image the reference sorts (sort_integers.inl:46)."""
+    if keys.dtype == np.int32:
+        return keys.view(np.uint32) ^ np.uint32(0x80000000)
+    if keys.dtype == np.int64:
+        return keys.view(np.uint64) ^ np.uint64(0x8000000000000000)
+    return keys
+
+
+def _numpy_stable(keys: np.ndarray, mode: int) -> np.ndarray:
+    if keys.ndim == 2:  # char[16]: lexicographic over signed bytes (bytes ^ 0x80)
+        k = keys ^ np.uint8(0x80)
+        if mode == 1:
+            k = ~k
+        perm = np.lexsort([k[:, j] for j in range(15, -1, -1)])
+    else:
+        k = _order_key(keys)
+        if mode == 1:
+            k = ~k
+        perm = np.argsort(k, kind="stable")
+    if mode == 2:
+        perm = perm[::-1]
+    return perm.astype(np.uint64)
+
+
+# ---------------------------------------------------------------- golden vectors
+@pytest.mark.parametrize("case", _golden(), ids=lambda c: c["name"])
+def test_oracle_matches_reference_golden(case):
+    keys = cases.make_keys(case)
+    assert hashlib.sha256(np.ascontiguousarray(keys).tobytes()).hexdigest() == case["key_sha256"]
+    for threads in (1, 4):
+        if case["kind"] == "char16" and threads > 1:
+            continue
+        perm = oracle.argsort(keys, 0, threads).astype(np.uint32)
+        assert hashlib.sha256(perm.tobytes()).hexdigest() == case["perm_sha256"]
+        if "perm" in case:
+            assert perm.tolist() == case["perm"]
+
+
+def test_golden_cover_all_cases():
+    names = {c["name"] for c in _golden()}
+    assert names == {c["name"] for c in cases.CASES}
+
+
+# -------------------------------------------------------- INTERNALS.doc example
+# 15-row frame sorted by col3 (INTERNALS.doc:1554-1631): ids are 1-based rows.
+DOC_COL3 = np.array([2, 7, 2, 7, 2, 7, 2, 7, 7, 2, 7, 7, 7, 2, 7], dtype=np.uint32)
+DOC_ASC = [1, 3, 5, 7, 10, 14, 2, 4, 6, 8, 9, 11, 12, 13, 15]       # :1594-1608
+DOC_DESC = [2, 4, 6, 8, 9, 11, 12, 13, 15, 1, 3, 5, 7, 10, 14]      # :1616-1630
+
+
+def test_internals_doc_sort_by_example():
+    assert (oracle.argsort(DOC_COL3, 0) + 1).tolist() == DOC_ASC
+    assert (oracle.argsort(DOC_COL3, 1) + 1).tolist() == DOC_DESC
+    # the reference code's descending path (std::reverse) differs from the doc
+    assert (oracle.argsort(DOC_COL3, 2) + 1).tolist() == DOC_ASC[::-1]
+
+
+# ---------------------------------------------------------- numpy stable argsort
+def _inputs(rng):
+    n = 5003
+    yield rng.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32)
+    yield rng.integers(0, 3, n).astype(np.uint32)
+    yield rng.integers(-2**31, 2**31, n).astype(np.int32)
+    yield rng.integers(-5, 5, n).astype(np.int32)
+    yield rng.integers(0, 2**63, n, dtype=np.uint64) * np.uint64(2) + rng.integers(0, 2, n, dtype=np.uint64)
+    yield rng.integers(-2**63, 2**63 - 1, n, dtype=np.int64)
+    yield rng.integers(-3, 3, n).astype(np.int64)
+    yield datagen.char16_keys(5, 0, n)
+    yield datagen.char16_keys(5, 0, n, distinct=11)
+    yield np.zeros(0, dtype=np.uint32)
+    yield np.array([7], dtype=np.int64)
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+@pytest.mark.parametrize("threads", [1, 3])
+def test_oracle_vs_numpy_stable(mode, threads):
+    rng = np.random.default_rng(100 + mode)
+    for keys in _inputs(rng):
+        got = oracle.argsort(keys, mode, threads)
+        assert np.array_equal(got, _numpy_stable(keys, mode)), (keys.dtype, keys.shape)
+
+
+def test_char16_vs_python_sorted():
+    keys = datagen.char16_keys(9, 0, 700, distinct=50)
+    ref = sorted(range(700), key=lambda i: bytes(keys[i]))  # stable; ASCII < 0x80
+    assert oracle.argsort(keys, 0).tolist() == ref
+
+
+def test_radix_planted_mistake_is_caught():
+    """A sort that ignores stability must disagree with the pinned answer."""
+    keys = DOC_COL3
+    unstable = np.argsort(keys, kind="quicksort")[::-1]
+    unstable = unstable[np.argsort(keys[unstable], kind="stable")]
+    assert (unstable + 1).tolist() != DOC_ASC
+
+
+# --------------------------------------------------------------------- gather
+def test_gather_vs_numpy_take():
+    rng = np.random.default_rng(3)
+    src = rng.standard_normal(10_001)
+    perm = rng.permutation(10_001).astype(np.uint64)
+    assert np.array_equal(oracle.gather(src, perm, 4), np.take(src, perm.astype(np.int64)))
+    src16 = datagen.char16_keys(1, 0, 999)
+    p16 = rng.permutation(999).astype(np.uint64)
+    assert np.array_equal(oracle.gather(src16, p16), src16[p16.astype(np.int64)])
+
+
+# ------------------------------------------------------------------ exact sum
+def test_exact_sum_vs_fsum():
+    rng = np.random.default_rng(5)
+    for x in (rng.standard_normal(100_000),
+              rng.standard_normal(5_000) * 10.0 ** rng.integers(-200, 200, 5_000),
+              np.array([1e100, 1.0, -1e100, 1e-100] * 100),
+              np.array([]), np.array([0.1] * 10)):
+        assert oracle.exact_sum(x) == math.fsum(x)
+
+
+def test_exact_sum_closed_forms():
+    # sum of k * 2^-20 for k in 1..n is n(n+1)/2 * 2^-20 exactly
+    n = 200_000
+    x = np.arange(1, n + 1, dtype=np.float64) * 2.0 ** -20
+    assert oracle.exact_sum(x) == (n * (n + 1) // 2) * 2.0 ** -20
+    # cancellation that defeats a sequential sum
+    y = np.array([2.0 ** 60, 1.0, -(2.0 ** 60)] * 1000)
+    assert oracle.exact_sum(y) == 1000.0
+    assert float(np.sum(y)) != 1000.0 or float(sum(y)) != 1000.0
+    # random data: compare with rational arithmetic rounded once
+    rng = np.random.default_rng(6)
+    z = rng.standard_normal(3000)
+    exact = sum(Fraction(float(v)) for v in z)
+    assert oracle.exact_sum(z) == float(exact)
+
+
+# ----------------------------------------------------------- sorted group-by
+def _pandas_groupby(keys, vals):
+    df = pd.DataFrame({"k": keys.astype(np.int64) if keys.dtype != np.uint64 else keys})
+    for c, v in enumerate(vals):
+        df[f"v{c}"] = v
+    g = df.groupby("k", sort=True)
+    return g
+
+
+@pytest.mark.parametrize("n", [0, 1, 37, 20_000])
+def test_groupby_vs_pandas(n):
+    key, vf, vi = datagen.groupby_small(n, seed=7)
+    key = key % 97
+    r = oracle.groupby(key, [vf, vi])
+    if n == 0:
+        assert r["n_groups"] == 0
+        return
+    g = _pandas_groupby(key, [vf, vi])
+    assert np.array_equal(r["keys"], g.size().index.values)
+    assert np.array_equal(r["counts"], g.size().values)
+    assert np.array_equal(r["aggs"][0]["min"], g["v0"].min().values)
+    assert np.array_equal(r["aggs"][0]["max"], g["v0"].max().values)
+    assert np.array_equal(r["aggs"][1]["min"], g["v1"].min().values)
+    assert np.array_equal(r["aggs"][1]["max"], g["v1"].max().values)
+    # integer sums exact; f64 sums correctly rounded (checked against fsum)
+    assert np.array_equal(r["aggs"][1]["sum"], g["v1"].sum().values.astype(np.int64))
+    fs = np.array([math.fsum(vf[key == k]) for k in r["keys"]])
+    assert np.array_equal(r["aggs"][0]["sum"], fs)
+    assert np.array_equal(r["aggs"][0]["mean"], fs / r["counts"])
+    exact_mean_i = np.array([float(Fraction(int(vi[key == k].astype(np.int64).sum()), int(c)))
+                             for k, c in zip(r["keys"], r["counts"])])
+    assert np.array_equal(r["aggs"][1]["mean"], exact_mean_i)
+    # group-contiguous permutation, rows in row order within a group
+    perm = r["perm"]
+    off = r["offsets"]
+    for gi in range(r["n_groups"]):
+        rows = perm[off[gi]:off[gi + 1]]
+        assert np.all(key[rows] == r["keys"][gi])
+        assert np.all(np.diff(rows.astype(np.int64)) > 0)
+
+
+def test_groupby_brute_force_tiny():
+    keys = np.array([5, -1, 5, 3, -1, 5], dtype=np.int32)
+    vals = np.array([1.5, -0.0, 0.0, 2.0, 7.0, -3.0])
+    r = oracle.groupby(keys, [vals])
+    assert r["keys"].tolist() == [-1, 3, 5]
+    assert r["counts"].tolist() == [2, 1, 3]
+    assert r["aggs"][0]["sum"].tolist() == [7.0, 2.0, -1.5]
+    # min: first of equal values in row order wins (-0.0 seen before 0.0 here)
+    assert r["aggs"][0]["min"].tolist() == [-0.0, 2.0, -3.0]
+    assert math.copysign(1.0, r["aggs"][0]["min"][0]) == -1.0
+    assert r["offsets"].tolist() == [0, 2, 3, 6]
+    assert r["perm"].tolist() == [1, 4, 3, 0, 2, 5]
+
+
+def test_groupby_int_sum_wraps_mod_2_64():
+    keys = np.zeros(4, dtype=np.int64)
+    vals = np.array([2**62, 2**62, 2**62, 2**62], dtype=np.int64)  # 2^64 total
+    r = oracle.groupby(keys, [vals])
+    assert r["aggs"][0]["sum"].tolist() == [0]
+    assert r["aggs"][0]["mean"].tolist() == [float(2**62)]
+
+
+def test_groupby_unsigned_and_char16_keys():
+    rng = np.random.default_rng(8)
+    k = datagen.char16_keys(3, 0, 3000, distinct=40)
+    v = rng.integers(0, 2**32, 3000, dtype=np.uint64).astype(np.uint32)
+    r = oracle.groupby(k, [v])
+    d = {}
+    for i in range(3000):
+        d.setdefault(bytes(k[i]), []).append(int(v[i]))
+    ks = sorted(d)
+    assert [bytes(x) for x in r["keys"]] == ks
+    assert r["aggs"][0]["sum"].tolist() == [sum(d[x]) for x in ks]
+    assert r["aggs"][0]["min"].tolist() == [min(d[x]) for x in ks]
+    assert r["aggs"][0]["max"].tolist() == [max(d[x]) for x in ks]
+
+
+# -------------------------------------------------------- hash group-by (CPU baseline)
+@pytest.mark.parametrize("threads,refapi", [(1, False), (4, False), (1, True), (3, True)])
+def test_hash_groupby_matches_sorted(threads, refapi):
+    key, vf, _ = datagen.groupby_small(30_000, seed=9)
+    h = oracle.hash_groupby_f64(key, vf, threads, refapi)
+    s = oracle.groupby(key, [vf])
+    order = np.argsort(h["keys"], kind="stable")
+    assert np.array_equal(h["keys"][order], s["keys"])
+    assert np.array_equal(h["counts"][order], s["counts"])
+    assert np.array_equal(h["min"][order], s["aggs"][0]["min"])
+    assert np.array_equal(h["max"][order], s["aggs"][0]["max"])
+    np.testing.assert_allclose(h["sum"][order], s["aggs"][0]["sum"], rtol=1e-12)
+    # first-occurrence order (ankerl iterates in insertion order)
+    _, first = np.unique(key, return_index=True)
+    assert np.array_equal(h["keys"], key[np.sort(first)])

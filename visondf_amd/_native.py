@@ -1,0 +1,112 @@
+"""ctypes binding of libvdf_hip.so (the C ABI declared in include/vdf.h).
+
+The library is built in-tree (``python -c "import __graft_entry__ as g; g.build()"``
+or ``make -C visondf_amd/csrc``).  There is no fallback: if the shared
+library is missing, loading raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libvdf_hip.so")
+
+# status codes
+VDF_OK = 0
+VDF_ERR_INVALID = 1
+VDF_ERR_UNSUPPORTED = 2
+VDF_ERR_OOM = 3
+VDF_ERR_HIP = 4
+VDF_ERR_BUDGET = 5
+
+# key types
+KEY_U32, KEY_I32, KEY_U64, KEY_I64, KEY_CHAR16 = 0, 1, 2, 3, 4
+# value types
+VAL_F64, VAL_I64, VAL_I32, VAL_U32, VAL_U64 = 0, 1, 2, 3, 4
+# argsort flags
+SORT_DESC, SORT_DESC_REVERSE, SORT_PERM64 = 0x1, 0x2, 0x4
+# aggregates
+AGG_COUNT, AGG_SUM, AGG_MEAN, AGG_MIN, AGG_MAX, AGG_ALL = 0x1, 0x2, 0x4, 0x8, 0x10, 0x1F
+
+MAX_ROWS = (1 << 30) - 1
+MAX_VALUE_COLS = 4
+
+# every symbol include/vdf.h declares
+EXPORTS = (
+    "vdf_last_error", "vdf_version", "vdf_create", "vdf_destroy",
+    "vdf_argsort_scratch_bytes", "vdf_groupby_scratch_bytes",
+    "vdf_argsort", "vdf_gather", "vdf_groupby",
+    "vdf_profile_enable", "vdf_profile_reset", "vdf_profile_read",
+)
+
+
+class VdfValueCol(C.Structure):
+    _fields_ = [("type", C.c_int), ("data", C.c_void_p)]
+
+
+class VdfAggOut(C.Structure):
+    _fields_ = [("sum", C.c_void_p), ("mean", C.c_void_p), ("min", C.c_void_p), ("max", C.c_void_p)]
+
+
+class VdfGroupbyOut(C.Structure):
+    _fields_ = [
+        ("n_groups", C.c_void_p),
+        ("group_keys", C.c_void_p),
+        ("offsets", C.c_void_p),
+        ("counts", C.c_void_p),
+        ("perm", C.c_void_p),
+        ("aggs", VdfAggOut * MAX_VALUE_COLS),
+    ]
+
+
+class VdfError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"vdf status {status}: {msg}")
+        self.status = status
+
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load libvdf_hip.so once; raise if it was not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+    lib = C.CDLL(LIB_PATH)
+    vp, u64, i32, u32 = C.c_void_p, C.c_uint64, C.c_int, C.c_uint32
+    lib.vdf_last_error.restype = C.c_char_p
+    lib.vdf_last_error.argtypes = []
+    lib.vdf_version.restype = C.c_char_p
+    lib.vdf_version.argtypes = []
+    lib.vdf_create.restype = i32
+    lib.vdf_create.argtypes = [C.POINTER(vp), i32, u64]
+    lib.vdf_destroy.restype = i32
+    lib.vdf_destroy.argtypes = [vp]
+    lib.vdf_argsort_scratch_bytes.restype = u64
+    lib.vdf_argsort_scratch_bytes.argtypes = [i32, u64, u32, i32]
+    lib.vdf_groupby_scratch_bytes.restype = u64
+    lib.vdf_groupby_scratch_bytes.argtypes = [i32, u64]
+    lib.vdf_argsort.restype = i32
+    lib.vdf_argsort.argtypes = [vp, i32, vp, u64, u32, vp, vp, vp]
+    lib.vdf_gather.restype = i32
+    lib.vdf_gather.argtypes = [vp, u32, vp, vp, i32, u64, vp, vp]
+    lib.vdf_groupby.restype = i32
+    lib.vdf_groupby.argtypes = [vp, i32, vp, u64, C.POINTER(VdfValueCol), i32, u32,
+                                C.POINTER(VdfGroupbyOut), vp]
+    lib.vdf_profile_enable.restype = i32
+    lib.vdf_profile_enable.argtypes = [vp, i32]
+    lib.vdf_profile_reset.restype = i32
+    lib.vdf_profile_reset.argtypes = [vp]
+    lib.vdf_profile_read.restype = i32
+    lib.vdf_profile_read.argtypes = [vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(u64), C.POINTER(u64)]
+    _lib = lib
+    return lib
+
+
+def check(status: int) -> None:
+    if status != VDF_OK:
+        raise VdfError(status, load().vdf_last_error().decode())

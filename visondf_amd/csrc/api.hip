@@ -1,0 +1,297 @@
+// api.hip -- the extern "C" boundary of libvdf_hip.so (declared in include/vdf.h).
+//
+// Every entry point: validate arguments, take the handle's lock, make the
+// handle's device current (restored on return), enqueue on the caller's
+// stream, and turn exceptions into a status code plus a thread-local message.
+#include "vdf_internal.h"
+
+#include <algorithm>
+#include <cstring>
+#include <exception>
+
+namespace vdf {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+void* Handle::ensure_scratch(size_t bytes)
+{
+    if (bytes <= scratch_cap) return scratch;
+    if (budget && bytes > budget)
+        throw Error{VDF_ERR_BUDGET, "scratch of " + std::to_string(bytes) + " bytes exceeds the handle budget of " +
+                                        std::to_string(budget) + " bytes"};
+    // Grow: earlier work of this handle may still use the old arena.
+    VDF_HIP_CHECK(hipDeviceSynchronize());
+    if (scratch) VDF_HIP_CHECK(hipFree(scratch));
+    scratch = nullptr;
+    scratch_cap = 0;
+    size_t want = bytes + bytes / 8;
+    if (budget) want = std::min<size_t>(want, budget);
+    void* p = nullptr;
+    if (hipMalloc(&p, want) != hipSuccess) {
+        (void)hipGetLastError();
+        want = bytes;
+        if (hipMalloc(&p, want) != hipSuccess) {
+            (void)hipGetLastError();
+            throw Error{VDF_ERR_OOM, "hipMalloc of " + std::to_string(want) + " scratch bytes failed"};
+        }
+    }
+    scratch = p;
+    scratch_cap = want;
+    return scratch;
+}
+
+hipEvent_t Handle::get_event()
+{
+    if (!free_events.empty()) {
+        hipEvent_t e = free_events.back();
+        free_events.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    VDF_HIP_CHECK(hipEventCreate(&e));
+    return e;
+}
+
+void Handle::prof_collect()
+{
+    for (ProfRec& r : recs) {
+        VDF_HIP_CHECK(hipEventSynchronize(r.end));
+        float ms = 0.f;
+        VDF_HIP_CHECK(hipEventElapsedTime(&ms, r.beg, r.end));
+        uint64_t bytes = r.bytes;
+        bool active = true;
+        if (r.plan_slot >= 0 && r.passes_host) {
+            const uint32_t D = *r.passes_host;
+            const uint32_t s = static_cast<uint32_t>(r.plan_slot);
+            active = s < D;
+            bytes = D == 1 ? r.pass_bytes[0] : s == 0 ? r.pass_bytes[1] : s == D - 1 ? r.pass_bytes[3] : r.pass_bytes[2];
+        }
+        if (active) {
+            auto it = std::find_if(totals.begin(), totals.end(), [&](const ProfTotal& t) { return t.name == r.name; });
+            if (it == totals.end()) {
+                totals.push_back(ProfTotal{r.name, 0.0, 0, 0});
+                it = totals.end() - 1;
+            }
+            it->ms += ms;
+            it->launches += 1;
+            it->bytes += bytes;
+        }
+        free_events.push_back(r.beg);
+        free_events.push_back(r.end);
+    }
+    recs.clear();
+}
+
+// RAII: lock the handle and make its device current for the call
+struct CallScope {
+    std::lock_guard<std::mutex> lk;
+    int prev = -1;
+    explicit CallScope(Handle* h) : lk(h->mu)
+    {
+        VDF_HIP_CHECK(hipGetDevice(&prev));
+        if (prev != h->device) VDF_HIP_CHECK(hipSetDevice(h->device));
+    }
+    ~CallScope()
+    {
+        int cur = -1;
+        if (hipGetDevice(&cur) == hipSuccess && prev >= 0 && cur != prev) (void)hipSetDevice(prev);
+    }
+};
+
+template <typename F>
+static vdf_status guarded(F&& f)
+{
+    try {
+        f();
+        return VDF_OK;
+    } catch (const Error& e) {
+        set_error(e.msg);
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        set_error("host allocation failed");
+        return VDF_ERR_OOM;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return VDF_ERR_INVALID;
+    }
+}
+
+static void require(bool ok, const char* msg)
+{
+    if (!ok) throw Error{VDF_ERR_INVALID, msg};
+}
+
+static int val_bytes(int t)
+{
+    switch (t) {
+    case VDF_VAL_F64:
+    case VDF_VAL_I64:
+    case VDF_VAL_U64: return 8;
+    case VDF_VAL_I32:
+    case VDF_VAL_U32: return 4;
+    default: return 0;
+    }
+}
+
+}  // namespace vdf
+
+using namespace vdf;
+
+struct vdf_handle : Handle {};
+
+extern "C" {
+
+const char* vdf_last_error(void) { return g_last_error.c_str(); }
+
+const char* vdf_version(void) { return "vdf_hip 0.1 (gfx950)"; }
+
+vdf_status vdf_create(vdf_handle** out, int device, uint64_t hbm_budget_bytes)
+{
+    return guarded([&] {
+        require(out != nullptr, "vdf_create: out is null");
+        *out = nullptr;
+        int ndev = 0;
+        VDF_HIP_CHECK(hipGetDeviceCount(&ndev));
+        require(device >= 0 && device < ndev, "vdf_create: no such device");
+        auto* h = new vdf_handle();
+        h->device = device;
+        h->budget = hbm_budget_bytes;
+        *out = h;
+    });
+}
+
+vdf_status vdf_destroy(vdf_handle* h)
+{
+    return guarded([&] {
+        if (!h) return;
+        {
+            CallScope cs(h);
+            VDF_HIP_CHECK(hipDeviceSynchronize());
+            for (ProfRec& r : h->recs) {
+                (void)hipEventDestroy(r.beg);
+                (void)hipEventDestroy(r.end);
+            }
+            for (hipEvent_t e : h->free_events) (void)hipEventDestroy(e);
+            if (h->scratch) (void)hipFree(h->scratch);
+            if (h->passes_ring) (void)hipHostFree(h->passes_ring);
+        }
+        delete h;
+    });
+}
+
+uint64_t vdf_argsort_scratch_bytes(int key_type, uint64_t n, uint32_t flags, int want_keys)
+{
+    (void)want_keys;
+    if (key_bytes(key_type) == 0) return 0;
+    return argsort_scratch(key_type, n, flags);
+}
+
+uint64_t vdf_groupby_scratch_bytes(int key_type, uint64_t n)
+{
+    if (key_bytes(key_type) == 0) return 0;
+    return groupby_scratch(key_type, n);
+}
+
+vdf_status vdf_argsort(vdf_handle* h, int key_type, const void* d_keys, uint64_t n, uint32_t flags,
+                       void* d_perm_out, void* d_sorted_keys_out, void* stream)
+{
+    return guarded([&] {
+        require(h != nullptr, "vdf_argsort: null handle");
+        require(key_bytes(key_type) != 0, "vdf_argsort: unknown key type");
+        require((flags & ~(VDF_SORT_DESC | VDF_SORT_DESC_REVERSE | VDF_SORT_PERM64)) == 0,
+                "vdf_argsort: unknown flag bits");
+        require(!((flags & VDF_SORT_DESC) && (flags & VDF_SORT_DESC_REVERSE)),
+                "vdf_argsort: VDF_SORT_DESC and VDF_SORT_DESC_REVERSE are exclusive");
+        require(n <= VDF_MAX_ROWS, "vdf_argsort: n exceeds VDF_MAX_ROWS (2^30 - 1)");
+        if (n == 0) return;
+        require(d_keys != nullptr && d_perm_out != nullptr, "vdf_argsort: null device pointer");
+        CallScope cs(h);
+        void* scr = h->ensure_scratch(argsort_scratch(key_type, n, flags));
+        argsort_enqueue(h, key_type, d_keys, n, flags, d_perm_out, d_sorted_keys_out, scr,
+                        static_cast<hipStream_t>(stream));
+    });
+}
+
+vdf_status vdf_gather(vdf_handle* h, uint32_t elem_bytes, const void* d_src, const void* d_perm, int perm64,
+                      uint64_t n, void* d_dst, void* stream)
+{
+    return guarded([&] {
+        require(h != nullptr, "vdf_gather: null handle");
+        require(elem_bytes == 4 || elem_bytes == 8 || elem_bytes == 16, "vdf_gather: elem_bytes must be 4, 8 or 16");
+        if (n == 0) return;
+        require(d_src && d_perm && d_dst, "vdf_gather: null device pointer");
+        CallScope cs(h);
+        gather_enqueue(h, elem_bytes, d_src, d_perm, perm64, n, d_dst, static_cast<hipStream_t>(stream));
+    });
+}
+
+vdf_status vdf_groupby(vdf_handle* h, int key_type, const void* d_keys, uint64_t n, const vdf_value_col* vals,
+                       int nvals, uint32_t agg_mask, vdf_groupby_out* out, void* stream)
+{
+    return guarded([&] {
+        require(h != nullptr, "vdf_groupby: null handle");
+        require(key_bytes(key_type) != 0, "vdf_groupby: unknown key type");
+        require(out != nullptr, "vdf_groupby: null output descriptor");
+        require(nvals >= 0 && nvals <= VDF_MAX_VALUE_COLS, "vdf_groupby: nvals out of range");
+        require(nvals == 0 || vals != nullptr, "vdf_groupby: null value columns");
+        require((agg_mask & ~VDF_AGG_ALL) == 0, "vdf_groupby: unknown aggregate bits");
+        require(n <= VDF_MAX_ROWS, "vdf_groupby: n exceeds VDF_MAX_ROWS (2^30 - 1)");
+        require(out->n_groups != nullptr && out->offsets != nullptr, "vdf_groupby: n_groups and offsets are required");
+        for (int c = 0; c < nvals; ++c) {
+            require(val_bytes(vals[c].type) != 0, "vdf_groupby: unknown value type");
+            require(n == 0 || vals[c].data != nullptr, "vdf_groupby: null value column data");
+        }
+        require(n == 0 || d_keys != nullptr, "vdf_groupby: null keys");
+        CallScope cs(h);
+        void* scr = h->ensure_scratch(groupby_scratch(key_type, n));
+        groupby_enqueue(h, key_type, d_keys, n, vals, nvals, agg_mask, out, scr, static_cast<hipStream_t>(stream));
+    });
+}
+
+vdf_status vdf_profile_enable(vdf_handle* h, int on)
+{
+    return guarded([&] {
+        require(h != nullptr, "vdf_profile_enable: null handle");
+        CallScope cs(h);
+        if (on && !h->passes_ring) {
+            void* p = nullptr;
+            VDF_HIP_CHECK(hipHostMalloc(&p, Handle::kRing * sizeof(uint32_t), hipHostMallocDefault));
+            h->passes_ring = static_cast<uint32_t*>(p);
+        }
+        h->prof = on != 0;
+    });
+}
+
+vdf_status vdf_profile_reset(vdf_handle* h)
+{
+    return guarded([&] {
+        require(h != nullptr, "vdf_profile_reset: null handle");
+        CallScope cs(h);
+        h->prof_collect();
+        h->totals.clear();
+    });
+}
+
+vdf_status vdf_profile_read(vdf_handle* h, const char* kernel, double* total_ms, uint64_t* launches, uint64_t* bytes)
+{
+    return guarded([&] {
+        require(h != nullptr && kernel != nullptr, "vdf_profile_read: null argument");
+        CallScope cs(h);
+        h->prof_collect();
+        double ms = 0;
+        uint64_t l = 0, b = 0;
+        for (const ProfTotal& t : h->totals)
+            if (t.name == kernel) {
+                ms = t.ms;
+                l = t.launches;
+                b = t.bytes;
+            }
+        if (total_ms) *total_ms = ms;
+        if (launches) *launches = l;
+        if (bytes) *bytes = b;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,169 @@
+// vdf_internal.h -- shared host/device definitions of libvdf_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/vdf.h"
+
+namespace vdf {
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+void set_error(const std::string& msg);
+
+struct Error {
+    vdf_status code;
+    std::string msg;
+};
+
+#define VDF_HIP_CHECK(expr)                                                              \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            throw ::vdf::Error{VDF_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)}; \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// radix constants: 8-bit digits, 256 buckets
+// ---------------------------------------------------------------------------
+constexpr int kRadixBits = 8;
+constexpr int kRadix = 1 << kRadixBits;
+constexpr int kBlock = 256;            // threads per sort / group-by block (4 waves)
+constexpr int kWaves = kBlock / 64;
+constexpr int kMaxDigits = 16;         // 128-bit keys
+constexpr uint32_t kFlagA = 1u << 30;  // look-back word: tile aggregate available
+constexpr uint32_t kFlagP = 2u << 30;  // look-back word: inclusive prefix available
+constexpr uint32_t kValMask = (1u << 30) - 1u;
+
+// Pass plan written on the device by the plan kernel: which digits need a
+// scatter pass (digits whose histogram puts all rows in one bucket are
+// skipped) and each digit's exclusive bucket offsets.
+struct Plan {
+    uint32_t passes;                     // D >= 1
+    uint32_t digit_of[kMaxDigits];       // pass slot -> digit index
+    uint32_t base[kMaxDigits][kRadix];   // digit -> bucket -> first output row
+};
+
+// ---------------------------------------------------------------------------
+// handle: scratch arena + timing
+// ---------------------------------------------------------------------------
+struct ProfRec {
+    std::string name;
+    hipEvent_t beg, end;
+    uint64_t bytes;          // algorithmic bytes of the launch
+    int plan_slot;           // >= 0: radix pass slot, active iff slot < *passes_host
+    uint32_t* passes_host;   // pinned copy of Plan::passes for that sort (or null)
+    uint64_t pass_bytes[4];  // slot bytes when it is the only / first / middle / last pass
+};
+
+struct ProfTotal {
+    std::string name;
+    double ms = 0;
+    uint64_t launches = 0;
+    uint64_t bytes = 0;
+};
+
+struct Handle {
+    int device = 0;
+    uint64_t budget = 0;
+    std::mutex mu;
+    // scratch arena (grown on demand, never shrunk)
+    void* scratch = nullptr;
+    size_t scratch_cap = 0;
+    // timing
+    bool prof = false;
+    std::vector<ProfRec> recs;
+    std::vector<hipEvent_t> free_events;
+    std::vector<ProfTotal> totals;
+    uint32_t* passes_ring = nullptr;   // pinned host ring for Plan::passes copies
+    size_t ring_pos = 0;
+    static constexpr size_t kRing = 4096;
+
+    void* ensure_scratch(size_t bytes);
+    hipEvent_t get_event();
+    void prof_collect();
+};
+
+// scoped launch timer: records events around a launch when profiling is on
+struct LaunchTimer {
+    Handle* h;
+    hipStream_t s;
+    const char* name;
+    uint64_t bytes;
+    int slot;
+    uint32_t* passes_host;
+    const uint64_t* pass_bytes;
+    hipEvent_t b = nullptr;
+    LaunchTimer(Handle* h_, hipStream_t s_, const char* n_, uint64_t bytes_, int slot_ = -1,
+                uint32_t* ph = nullptr, const uint64_t* pb = nullptr)
+        : h(h_), s(s_), name(n_), bytes(bytes_), slot(slot_), passes_host(ph), pass_bytes(pb)
+    {
+        if (h->prof) {
+            b = h->get_event();
+            VDF_HIP_CHECK(hipEventRecord(b, s));
+        }
+    }
+    ~LaunchTimer() noexcept(false)
+    {
+        if (h->prof) {
+            hipEvent_t e = h->get_event();
+            VDF_HIP_CHECK(hipEventRecord(e, s));
+            ProfRec r{name, b, e, bytes, slot, passes_host, {0, 0, 0, 0}};
+            if (pass_bytes)
+                for (int i = 0; i < 4; ++i) r.pass_bytes[i] = pass_bytes[i];
+            h->recs.push_back(r);
+        }
+    }
+};
+
+inline void check_launch(const char* what)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) throw Error{VDF_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e)};
+}
+
+inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+// Bump allocator over the handle's scratch arena.
+struct Carve {
+    char* base;
+    size_t off = 0;
+    explicit Carve(void* b) : base(static_cast<char*>(b)) {}
+    template <typename T>
+    T* take(size_t count)
+    {
+        off = align_up(off, 256);
+        T* p = reinterpret_cast<T*>(base + off);
+        off += count * sizeof(T);
+        return p;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// entry points implemented by the kernel files
+// ---------------------------------------------------------------------------
+size_t key_bytes(int key_type);
+size_t argsort_scratch(int key_type, uint64_t n, uint32_t flags);
+
+// Sorts into d_perm_out / d_keys_out using scratch at `scr` (argsort_scratch bytes).
+void argsort_enqueue(Handle* h, int key_type, const void* d_keys, uint64_t n, uint32_t flags,
+                     void* d_perm_out, void* d_keys_out, void* scr, hipStream_t s);
+
+void gather_enqueue(Handle* h, uint32_t elem_bytes, const void* d_src, const void* d_perm,
+                    int perm64, uint64_t n, void* d_dst, hipStream_t s);
+
+size_t groupby_scratch(int key_type, uint64_t n);
+void groupby_enqueue(Handle* h, int key_type, const void* d_keys, uint64_t n,
+                     const vdf_value_col* vals, int nvals, uint32_t agg_mask,
+                     const vdf_groupby_out* out, void* scr, hipStream_t s);
+
+// fill kernel (graph-safe replacement for hipMemsetAsync)
+void fill_u32(uint32_t* p, size_t count, uint32_t value, hipStream_t s);
+
+}  // namespace vdf
