@@ -1,0 +1,361 @@
+// vison_gpu.hpp -- header-only C++ façade over libvdf_hip.so (include/vdf.h).
+//
+// Mirrors the sort / group-by surface of the reference's vison::Dataframe
+// (include/vison/vison.hpp:187-616) for device-resident columns:
+//   sort_by<ASC, Soft>(n)                 transform/sort_by/sort_by.hpp:9-19,
+//                                         sort_by_mt.hpp:10-180
+//   transform_group_by<Function>(x, n_col, colname)
+//                                         transform/transform_group_by/
+//                                         transform_group_by.hpp:11-14
+//   group_by(key_col, value_cols)         per-group results (keys ascending)
+//   materialize()                         materialize/materialize_mt.hpp:10
+// Columns live in HBM; every operation runs in the HIP library.  Errors from
+// the library are thrown as vison_gpu::Error (the reference prints to
+// std::cerr and returns; a thrown error keeps the frame unchanged instead).
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <variant>
+#include <vector>
+
+#include "vdf.h"
+
+namespace vison_gpu {
+
+struct Error : std::runtime_error {
+    int status;
+    Error(int s, const std::string& m) : std::runtime_error(m), status(s) {}
+};
+
+inline void check(vdf_status s)
+{
+    if (s != VDF_OK) throw Error(s, vdf_last_error());
+}
+inline void hip_check(hipError_t e, const char* what)
+{
+    if (e != hipSuccess) throw Error(VDF_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// GroupFunction of types/supported_group_methods.inl:1-7; Gather + a min /
+// max functor (create_value_col.inl:48-58) becomes Min / Max.
+enum class GroupFunction { Occurence, Sum, Mean, Min, Max };
+
+struct Char16 {
+    char c[16];
+};
+
+// owning device buffer
+class DeviceBuffer {
+  public:
+    DeviceBuffer() = default;
+    explicit DeviceBuffer(size_t bytes) : bytes_(bytes)
+    {
+        if (bytes) hip_check(hipMalloc(&p_, bytes), "hipMalloc");
+    }
+    DeviceBuffer(const DeviceBuffer&) = delete;
+    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+    DeviceBuffer(DeviceBuffer&& o) noexcept : p_(o.p_), bytes_(o.bytes_)
+    {
+        o.p_ = nullptr;
+        o.bytes_ = 0;
+    }
+    DeviceBuffer& operator=(DeviceBuffer&& o) noexcept
+    {
+        if (this != &o) {
+            release();
+            p_ = o.p_;
+            bytes_ = o.bytes_;
+            o.p_ = nullptr;
+            o.bytes_ = 0;
+        }
+        return *this;
+    }
+    ~DeviceBuffer() { release(); }
+    void* get() const { return p_; }
+    size_t bytes() const { return bytes_; }
+
+  private:
+    void release()
+    {
+        if (p_) (void)hipFree(p_);
+        p_ = nullptr;
+    }
+    void* p_ = nullptr;
+    size_t bytes_ = 0;
+};
+
+// one library handle (scratch arena) per frame
+class Handle {
+  public:
+    explicit Handle(int device = 0, uint64_t budget = 0) { check(vdf_create(&h_, device, budget)); }
+    Handle(const Handle&) = delete;
+    Handle& operator=(const Handle&) = delete;
+    ~Handle() { (void)vdf_destroy(h_); }
+    vdf_handle* get() const { return h_; }
+
+  private:
+    vdf_handle* h_ = nullptr;
+};
+
+template <typename T>
+struct TypeCode;
+template <> struct TypeCode<uint32_t> { static constexpr int key = VDF_KEY_U32, val = VDF_VAL_U32; static constexpr char letter = 'u'; };
+template <> struct TypeCode<int32_t> { static constexpr int key = VDF_KEY_I32, val = VDF_VAL_I32; static constexpr char letter = 'i'; };
+template <> struct TypeCode<uint64_t> { static constexpr int key = VDF_KEY_U64, val = VDF_VAL_U64; static constexpr char letter = 'u'; };
+template <> struct TypeCode<int64_t> { static constexpr int key = VDF_KEY_I64, val = VDF_VAL_I64; static constexpr char letter = 'i'; };
+template <> struct TypeCode<double> { static constexpr int key = -1, val = VDF_VAL_F64; static constexpr char letter = 'd'; };
+template <> struct TypeCode<Char16> { static constexpr int key = VDF_KEY_CHAR16, val = -1; static constexpr char letter = 'c'; };
+
+// A device column: element size, key / value type codes, name.
+struct Column {
+    std::string name;
+    size_t elem = 0;
+    int key_type = -1;
+    int val_type = -1;
+    char letter = '?';
+    DeviceBuffer data;
+};
+
+template <typename T>
+struct GroupResult {
+    size_t n_groups = 0;
+    std::vector<T> keys;                 // ascending
+    std::vector<uint32_t> offsets;       // n_groups + 1, into row_view
+    std::vector<uint32_t> counts;
+    std::vector<uint32_t> row_view;      // group-contiguous row ids
+    std::vector<std::vector<double>> sum_f, mean, min_f, max_f;   // per f64 value column
+    std::vector<std::vector<int64_t>> sum_i, min_i, max_i;        // per integer value column
+};
+
+class Dataframe {
+  public:
+    explicit Dataframe(int device = 0) : h_(device) {}
+
+    unsigned int get_nrow() const { return nrow_; }
+    unsigned int get_ncol() const { return static_cast<unsigned int>(cols_.size()); }
+    bool in_view() const { return in_view_; }
+    const std::vector<size_t>& get_row_view_idx() const { return row_view_idx_; }
+
+    template <typename T>
+    void add_col(const std::vector<T>& host, const std::string& name)
+    {
+        if (!cols_.empty() && host.size() != nrow_) throw Error(VDF_ERR_INVALID, "add_col: row count mismatch");
+        Column c;
+        c.name = name;
+        c.elem = sizeof(T);
+        c.key_type = TypeCode<T>::key;
+        c.val_type = TypeCode<T>::val;
+        c.letter = TypeCode<T>::letter;
+        c.data = DeviceBuffer(host.size() * sizeof(T));
+        if (!host.empty())
+            hip_check(hipMemcpy(c.data.get(), host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice),
+                      "hipMemcpy H2D");
+        nrow_ = static_cast<unsigned int>(host.size());
+        cols_.push_back(std::move(c));
+    }
+
+    template <typename T>
+    std::vector<T> get_col(unsigned int n) const
+    {
+        const Column& c = col(n);
+        if (c.elem != sizeof(T)) throw Error(VDF_ERR_INVALID, "get_col: element size mismatch");
+        std::vector<T> out(nrow_);
+        if (nrow_)
+            hip_check(hipMemcpy(out.data(), c.data.get(), nrow_ * sizeof(T), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+        return out;
+    }
+
+    // Reorder rows by column n (sort_by.hpp:9-19).  ASC=false is the stable
+    // descending order of INTERNALS.doc:1610-1630.  Soft=true keeps the
+    // permutation as a row view (sort_by_mt.hpp:160-177), Soft=false gathers
+    // every column (permute_block_mt.inl:39-41).
+    template <bool ASC = true, bool Soft = true>
+    void sort_by(unsigned int& n)
+    {
+        if (!Soft && in_view_) throw Error(VDF_ERR_INVALID, "sort_by: materialize() the view first");
+        const Column& k = col(n);
+        if (k.key_type < 0) throw Error(VDF_ERR_UNSUPPORTED, "sort_by: column type has no radix key path");
+        if (nrow_ == 0) return;
+        DeviceBuffer perm(static_cast<size_t>(nrow_) * 4);
+        check(vdf_argsort(h_.get(), k.key_type, k.data.get(), nrow_, ASC ? 0u : VDF_SORT_DESC, perm.get(), nullptr,
+                          nullptr));
+        if constexpr (Soft) {
+            std::vector<uint32_t> p(nrow_);
+            hip_check(hipMemcpy(p.data(), perm.get(), nrow_ * 4ull, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+            if (in_view_) {  // compose with the existing view
+                std::vector<size_t> r(nrow_);
+                for (size_t i = 0; i < nrow_; ++i) r[i] = row_view_idx_[p[i]];
+                row_view_idx_.swap(r);
+            } else {
+                row_view_idx_.assign(p.begin(), p.end());
+            }
+            in_view_ = true;
+        } else {
+            permute_all(perm.get());
+        }
+    }
+
+    // Apply the row view to every column (materialize_mt.hpp:10).
+    void materialize()
+    {
+        if (!in_view_) return;
+        std::vector<uint32_t> p(row_view_idx_.begin(), row_view_idx_.end());
+        DeviceBuffer perm(p.size() * 4);
+        if (!p.empty())
+            hip_check(hipMemcpy(perm.get(), p.data(), p.size() * 4, hipMemcpyHostToDevice), "hipMemcpy H2D");
+        permute_all(perm.get());
+        row_view_idx_.clear();
+        in_view_ = false;
+    }
+
+    // Group by key column `key` and aggregate value columns (ascending keys).
+    template <typename K>
+    GroupResult<K> group_by(unsigned int key, const std::vector<unsigned int>& vals, uint32_t aggs = VDF_AGG_ALL)
+    {
+        const Column& k = col(key);
+        if (k.key_type < 0 || k.elem != sizeof(K)) throw Error(VDF_ERR_INVALID, "group_by: bad key column");
+        if (vals.size() > VDF_MAX_VALUE_COLS) throw Error(VDF_ERR_INVALID, "group_by: too many value columns");
+        const size_t n = nrow_;
+        GroupResult<K> R;
+        DeviceBuffer ng(8), gkeys(std::max<size_t>(n, 1) * sizeof(K)), off((n + 1) * 4),
+            cnt(std::max<size_t>(n, 1) * 4), perm(std::max<size_t>(n, 1) * 4);
+        std::vector<DeviceBuffer> bufs;
+        vdf_groupby_out out{};
+        out.n_groups = static_cast<uint64_t*>(ng.get());
+        out.group_keys = gkeys.get();
+        out.offsets = static_cast<uint32_t*>(off.get());
+        out.counts = static_cast<uint32_t*>(cnt.get());
+        out.perm = static_cast<uint32_t*>(perm.get());
+        std::vector<vdf_value_col> vc;
+        for (size_t c = 0; c < vals.size(); ++c) {
+            const Column& v = col(vals[c]);
+            if (v.val_type < 0) throw Error(VDF_ERR_UNSUPPORTED, "group_by: value column type");
+            vc.push_back(vdf_value_col{v.val_type, v.data.get()});
+            for (int a = 0; a < 4; ++a) bufs.emplace_back(std::max<size_t>(n, 1) * 8);
+            out.aggs[c].sum = bufs[bufs.size() - 4].get();
+            out.aggs[c].mean = static_cast<double*>(bufs[bufs.size() - 3].get());
+            out.aggs[c].min = bufs[bufs.size() - 2].get();
+            out.aggs[c].max = bufs[bufs.size() - 1].get();
+        }
+        check(vdf_groupby(h_.get(), k.key_type, k.data.get(), n, vc.data(), static_cast<int>(vc.size()), aggs, &out,
+                          nullptr));
+        uint64_t g = 0;
+        hip_check(hipMemcpy(&g, ng.get(), 8, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+        R.n_groups = g;
+        R.keys.resize(g);
+        R.offsets.resize(g + 1);
+        R.counts.resize(g);
+        R.row_view.resize(n);
+        auto d2h = [](void* dst, const void* src, size_t bytes) {
+            if (bytes) hip_check(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+        };
+        d2h(R.keys.data(), gkeys.get(), g * sizeof(K));
+        d2h(R.offsets.data(), off.get(), (g + 1) * 4);
+        d2h(R.counts.data(), cnt.get(), g * 4);
+        d2h(R.row_view.data(), perm.get(), n * 4);
+        for (size_t c = 0; c < vals.size(); ++c) {
+            const Column& v = col(vals[c]);
+            if (v.val_type == VDF_VAL_F64) {
+                std::vector<double> s(g), m(g), lo(g), hi(g);
+                d2h(s.data(), out.aggs[c].sum, g * 8);
+                d2h(m.data(), out.aggs[c].mean, g * 8);
+                d2h(lo.data(), out.aggs[c].min, g * 8);
+                d2h(hi.data(), out.aggs[c].max, g * 8);
+                R.sum_f.push_back(std::move(s));
+                R.mean.push_back(std::move(m));
+                R.min_f.push_back(std::move(lo));
+                R.max_f.push_back(std::move(hi));
+            } else {
+                std::vector<int64_t> s(g), lo(g), hi(g);
+                std::vector<double> m(g);
+                d2h(s.data(), out.aggs[c].sum, g * 8);
+                d2h(m.data(), out.aggs[c].mean, g * 8);
+                const bool w4 = v.elem == 4;
+                std::vector<uint32_t> l4(w4 ? g : 0), h4(w4 ? g : 0);
+                if (w4) {
+                    d2h(l4.data(), out.aggs[c].min, g * 4);
+                    d2h(h4.data(), out.aggs[c].max, g * 4);
+                    for (size_t i = 0; i < g; ++i) {
+                        lo[i] = v.val_type == VDF_VAL_I32 ? static_cast<int32_t>(l4[i]) : static_cast<int64_t>(l4[i]);
+                        hi[i] = v.val_type == VDF_VAL_I32 ? static_cast<int32_t>(h4[i]) : static_cast<int64_t>(h4[i]);
+                    }
+                } else {
+                    d2h(lo.data(), out.aggs[c].min, g * 8);
+                    d2h(hi.data(), out.aggs[c].max, g * 8);
+                }
+                R.sum_i.push_back(std::move(s));
+                R.mean.push_back(std::move(m));
+                R.min_i.push_back(std::move(lo));
+                R.max_i.push_back(std::move(hi));
+            }
+        }
+        return R;
+    }
+
+    // transform_group_by (transform_group_by.hpp:11): append a per-row column
+    // holding the aggregate of the row's group, as CreateValueCol does
+    // (create_value_col.inl:36-61).  One key column; Mean = sum / count.
+    template <GroupFunction F>
+    void transform_group_by(const std::vector<unsigned int>& x, unsigned int n_col, const std::string& colname = "n")
+    {
+        if (x.size() != 1) throw Error(VDF_ERR_UNSUPPORTED, "transform_group_by: one key column");
+        const Column& k = col(x[0]);
+        std::vector<double> per_row(nrow_);
+        auto fill = [&](auto R) {
+            for (size_t g = 0; g < R.n_groups; ++g) {
+                double v = 0;
+                if constexpr (F == GroupFunction::Occurence) v = static_cast<double>(R.counts[g]);
+                else {
+                    const bool f = !R.sum_f.empty();
+                    if (F == GroupFunction::Sum) v = f ? R.sum_f[0][g] : static_cast<double>(R.sum_i[0][g]);
+                    if (F == GroupFunction::Mean) v = R.mean[0][g];
+                    if (F == GroupFunction::Min) v = f ? R.min_f[0][g] : static_cast<double>(R.min_i[0][g]);
+                    if (F == GroupFunction::Max) v = f ? R.max_f[0][g] : static_cast<double>(R.max_i[0][g]);
+                }
+                for (uint32_t p = R.offsets[g]; p < R.offsets[g + 1]; ++p) per_row[R.row_view[p]] = v;
+            }
+        };
+        const std::vector<unsigned int> vals = (F == GroupFunction::Occurence) ? std::vector<unsigned int>{}
+                                                                               : std::vector<unsigned int>{n_col};
+        switch (k.key_type) {
+        case VDF_KEY_U32: fill(group_by<uint32_t>(x[0], vals)); break;
+        case VDF_KEY_I32: fill(group_by<int32_t>(x[0], vals)); break;
+        case VDF_KEY_U64: fill(group_by<uint64_t>(x[0], vals)); break;
+        case VDF_KEY_I64: fill(group_by<int64_t>(x[0], vals)); break;
+        default: fill(group_by<Char16>(x[0], vals)); break;
+        }
+        add_col(per_row, colname);
+    }
+
+  private:
+    const Column& col(unsigned int n) const
+    {
+        if (n >= cols_.size()) throw Error(VDF_ERR_INVALID, "column index out of range");
+        return cols_[n];
+    }
+
+    void permute_all(void* perm)
+    {
+        for (Column& c : cols_) {
+            DeviceBuffer dst(c.data.bytes());
+            check(vdf_gather(h_.get(), static_cast<uint32_t>(c.elem), c.data.get(), perm, 0, nrow_, dst.get(),
+                             nullptr));
+            c.data = std::move(dst);
+        }
+        hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    }
+
+    Handle h_;
+    unsigned int nrow_ = 0;
+    std::vector<Column> cols_;
+    bool in_view_ = false;
+    std::vector<size_t> row_view_idx_;
+};
+
+}  // namespace vison_gpu

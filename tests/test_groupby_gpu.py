@@ -1,0 +1,167 @@
+"""GPU parity of the sort-based group-by against the oracle.
+
+Bar: group keys, offsets, counts, the group-contiguous permutation, integer
+sums (mod 2^64), integer means (exact sum / count), and min / max are
+bit-exact; f64 sums and means agree with the correctly rounded exact sum
+within a relative 1e-12 (F64_RTOL)."""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from visondf_amd import _native as N
+from visondf_amd import datagen
+
+pytestmark = pytest.mark.gpu
+
+F64_RTOL = 1e-12
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _dev(a: np.ndarray):
+    import torch
+
+    a = np.ascontiguousarray(a)
+    if a.dtype == np.uint32:
+        a = a.view(np.int32)
+    elif a.dtype == np.uint64:
+        a = a.view(np.int64)
+    return torch.from_numpy(a).cuda()
+
+
+def _host(t, dtype):
+    a = t.cpu().numpy()
+    return a.view(dtype) if a.ndim == 1 else a
+
+
+def _check(vdf, keys, vals, aggs=N.AGG_ALL, label=""):
+    _say("groupby", label, keys.dtype, keys.shape, [v.dtype for v in vals])
+    kt = oracle.key_type_of(keys)
+    vts = [oracle.val_type_of(v) for v in vals]
+    r = vdf.groupby(_dev(keys), [_dev(v) for v in vals], aggs=aggs, key_type=kt, val_types=vts)
+    o = oracle.groupby(keys, vals, threads=8)
+    g = o["n_groups"]
+    assert r.n_groups == g, label
+    if g == 0:
+        return r, o
+    assert np.array_equal(_host(r.keys, keys.dtype), o["keys"]), label
+    assert np.array_equal(_host(r.offsets, np.uint32).astype(np.uint64), o["offsets"]), label
+    assert np.array_equal(_host(r.perm, np.uint32).astype(np.uint64), o["perm"]), label
+    if aggs & N.AGG_COUNT:
+        assert np.array_equal(_host(r.counts, np.uint32).astype(np.uint64), o["counts"]), label
+    for c, v in enumerate(vals):
+        ra, oa = r.aggs[c], o["aggs"][c]
+        if aggs & N.AGG_MIN:
+            assert np.array_equal(_host(ra["min"], v.dtype).view(np.uint8), oa["min"].view(np.uint8)), (label, c)
+        if aggs & N.AGG_MAX:
+            assert np.array_equal(_host(ra["max"], v.dtype).view(np.uint8), oa["max"].view(np.uint8)), (label, c)
+        if v.dtype == np.float64:
+            for k in ("sum", "mean"):
+                if (aggs & (N.AGG_SUM if k == "sum" else N.AGG_MEAN)):
+                    got, exp = _host(ra[k], np.float64), oa[k]
+                    err = np.abs(got - exp)
+                    assert np.all(err <= F64_RTOL * np.abs(exp)), (label, c, k, float(err.max()))
+        else:
+            if aggs & N.AGG_SUM:
+                assert np.array_equal(_host(ra["sum"], np.int64), oa["sum"].view(np.int64)), (label, c)
+            if aggs & N.AGG_MEAN:
+                assert np.array_equal(_host(ra["mean"], np.float64), oa["mean"]), (label, c)
+    return r, o
+
+
+def test_groupby_small_config(vdf):
+    """1M rows, int64 key in [0, 1000), f64 and int32 values: all aggregates."""
+    key, vf, vi = datagen.groupby_small(1_000_000, seed=1)
+    _check(vdf, key, [vf, vi], label="c1")
+
+
+@pytest.mark.parametrize("kt", ["u32", "i32", "u64", "i64", "char16"])
+def test_groupby_key_types(vdf, kt):
+    rng = np.random.default_rng(31)
+    n = 300_007
+    if kt == "u32":
+        keys = (rng.integers(0, 5000, n) * 858_993).astype(np.uint32)
+    elif kt == "i32":
+        keys = rng.integers(-3000, 3000, n).astype(np.int32)
+    elif kt == "u64":
+        keys = rng.integers(0, 7000, n).astype(np.uint64) * np.uint64(0x9E3779B97F4A7C15)
+    elif kt == "i64":
+        keys = rng.integers(-2**62, 2**62, 4000)[rng.integers(0, 4000, n)].astype(np.int64)
+    else:
+        keys = datagen.char16_keys(5, 0, n, distinct=2000)
+    vals = [rng.standard_normal(n), rng.integers(-2**40, 2**40, n), rng.integers(-2**31, 2**31, n).astype(np.int32),
+            rng.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32)]
+    _check(vdf, keys, vals, label=kt)
+
+
+def test_groupby_u64_values_and_wraparound(vdf):
+    n = 50_000
+    keys = (np.arange(n) % 3).astype(np.int32)
+    big = np.full(n, 2**62 + 12345, dtype=np.int64)  # group sums wrap mod 2^64
+    u = np.full(n, 2**64 - 7, dtype=np.uint64)
+    _check(vdf, keys, [big, u], label="wrap")
+
+
+@pytest.mark.parametrize("n", [1, 2, 4095, 4096, 4097, 8191, 8193, 100_000])
+def test_groupby_tile_edges(vdf, n):
+    rng = np.random.default_rng(n)
+    keys = rng.integers(0, max(n // 5, 1), n).astype(np.int32)
+    _check(vdf, keys, [rng.standard_normal(n), rng.integers(-100, 100, n)], label=f"edge{n}")
+
+
+def test_groupby_single_group_spans_tiles(vdf):
+    n = 3_000_001  # ~730 tiles, one group: the fix-up walks every tile
+    vals = datagen.normal01(3, 1, n)
+    _check(vdf, np.zeros(n, dtype=np.int64), [vals, np.arange(n, dtype=np.int64)], label="one-group")
+
+
+def test_groupby_every_row_distinct(vdf):
+    n = 262_147
+    keys = np.random.default_rng(1).permutation(n).astype(np.uint32)
+    _check(vdf, keys, [np.random.default_rng(2).standard_normal(n)], label="distinct")
+
+
+def test_groupby_group_sizes_near_tile(vdf):
+    """Runs of 4095..4097 equal keys so heads land on every tile offset."""
+    sizes = np.array([4095, 4096, 4097, 1, 8192, 12289, 3, 4096] * 6)
+    keys = np.repeat(np.arange(sizes.size, dtype=np.int64), sizes)
+    perm = np.random.default_rng(5).permutation(keys.size)
+    keys = keys[perm]
+    _check(vdf, keys, [np.random.default_rng(6).standard_normal(keys.size)], label="near-tile")
+
+
+def test_groupby_counts_only_and_masks(vdf):
+    key, vf, vi = datagen.groupby_small(200_000, seed=2)
+    _check(vdf, key, [], aggs=N.AGG_COUNT, label="counts-only")
+    _check(vdf, key, [vf, vi], aggs=N.AGG_SUM | N.AGG_MIN, label="sum+min")
+
+
+def test_groupby_empty(vdf):
+    import torch
+
+    r = vdf.groupby(torch.empty(0, dtype=torch.int32, device="cuda"), [])
+    assert r.n_groups == 0
+
+
+def test_groupby_zipf_20m(vdf):
+    """Zipf(1.1) keys over 2^24 values (the skewed configuration at 20M rows)."""
+    key, vf, vi = datagen.zipf_groupby(20_000_000, seed=3)
+    _check(vdf, key, [vf, vi], label="zipf20m")
+
+
+def test_groupby_deterministic_repeat(vdf):
+    """Same table twice (and after other work) gives bit-identical results."""
+    key, vf, _ = datagen.zipf_groupby(2_000_000, seed=8)
+    r1 = vdf.groupby(_dev(key), [_dev(vf)])
+    other = datagen.groupby_small(300_000, seed=9)
+    vdf.groupby(_dev(other[0]), [_dev(other[1])])
+    r2 = vdf.groupby(_dev(key), [_dev(vf)])
+    for k in ("sum", "mean", "min", "max"):
+        assert np.array_equal(r1.aggs[0][k].cpu().numpy().view(np.uint64), r2.aggs[0][k].cpu().numpy().view(np.uint64))

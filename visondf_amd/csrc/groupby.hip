@@ -1,15 +1,566 @@
-// groupby.hip -- sort-based group-by for gfx950 (placeholder until the
-// segmented-reduction kernels land).
+// groupby.hip -- sort-based group-by for gfx950 (MI355X).
+//
+// Semantics restate the reference's hash group-by aggregates
+// (inlines/transform/transform_group_by/onecol_functions.inl:4-130 Occurence /
+// Sum / Gather, create_value_col.inl:36-61) with these stated contracts:
+//   groups in ascending key order; count; sum (f64: double-double accumulation
+//   rounded once; integers: two's complement mod 2^64); mean = sum / count
+//   (integers: exact 128-bit sum / count); min / max = first of equal values
+//   in row order (std::min_element over the row-ordered Gather vector).
+//
+// Pipeline on one stream:
+//   1. radix argsort of the keys (radix.hip) -> perm (row ids in key order,
+//      rows in row order within a key) and the sorted keys
+//   2. k_gb_tile: one 256-thread block per 4096-row tile (dynamic tile ids):
+//        head flags (key != previous key) by wave ballots; head count of the
+//        tile published to a decoupled look-back array -> global group ids;
+//        group keys and offsets written at heads; per value column the
+//        values are gathered through perm (striped, coalesced row ids),
+//        staged in padded LDS, reduced sequentially over 16 consecutive rows
+//        per thread, then a block segmented scan joins the threads.  Groups
+//        that start and end inside the tile are final here; each tile leaves
+//        two partials (before its first head, after its last head).
+//   3. k_gb_fixup: one block per tile owning the group that starts at the
+//        tile's last head; joins the following tiles' leading partials in
+//        tile order with a fixed reduction tree.
+// The reduction order is a function of the row count and tile size only, so a
+// table's results do not depend on what else runs with it.
 #include "vdf_internal.h"
+
+#include <algorithm>
 
 namespace vdf {
 
-size_t groupby_scratch(int key_type, uint64_t n) { return argsort_scratch(key_type, n, 0); }
+namespace {
 
-void groupby_enqueue(Handle*, int, const void*, uint64_t, const vdf_value_col*, int, uint32_t,
-                     const vdf_groupby_out*, void*, hipStream_t)
+constexpr int kGbRows = 16;                 // consecutive rows per thread
+constexpr int kGbTile = kBlock * kGbRows;   // 4096 rows per tile
+constexpr int kGbPad = kGbTile + kGbTile / 16;
+
+enum : int { KIND_F64 = 0, KIND_SIGNED = 1, KIND_UNSIGNED = 2, KIND_COUNT = 3 };
+
+struct K16 {
+    uint64_t lo, hi;
+};
+__device__ inline bool keq(uint32_t a, uint32_t b) { return a == b; }
+__device__ inline bool keq(uint64_t a, uint64_t b) { return a == b; }
+__device__ inline bool keq(const K16& a, const K16& b) { return a.lo == b.lo && a.hi == b.hi; }
+
+// Running aggregate of one segment.  F64: a = hi, b = lo of a double-double
+// sum; integers: a = low, b = high word of a 128-bit sum.  mn / mx hold the raw
+// 64-bit value (f64 bits or the sign / zero extended integer).
+struct Agg {
+    uint64_t cnt;
+    uint64_t a, b;
+    uint64_t mn, mx;
+};
+
+__device__ inline Agg agg_empty() { return Agg{0, 0, 0, 0, 0}; }
+__device__ inline double as_d(uint64_t x) { return __longlong_as_double(static_cast<long long>(x)); }
+__device__ inline uint64_t as_u(double d) { return static_cast<uint64_t>(__double_as_longlong(d)); }
+
+__device__ inline void two_sum(double x, double y, double& s, double& e)
 {
-    throw Error{VDF_ERR_UNSUPPORTED, "vdf_groupby: not built yet"};
+    s = x + y;
+    const double bp = s - x;
+    e = (x - (s - bp)) + (y - bp);
+}
+
+__device__ inline void agg_add(Agg& s, uint64_t x, int kind)
+{
+    if (kind == KIND_F64) {
+        const double v = as_d(x);
+        if (s.cnt == 0) {
+            s.mn = x;
+            s.mx = x;
+        } else {
+            if (v < as_d(s.mn)) s.mn = x;
+            if (v > as_d(s.mx)) s.mx = x;
+        }
+        double t, e;
+        two_sum(as_d(s.a), v, t, e);
+        s.a = as_u(t);
+        s.b = as_u(as_d(s.b) + e);
+    } else if (kind == KIND_SIGNED) {
+        const int64_t v = static_cast<int64_t>(x);
+        if (s.cnt == 0) {
+            s.mn = x;
+            s.mx = x;
+        } else {
+            if (v < static_cast<int64_t>(s.mn)) s.mn = x;
+            if (v > static_cast<int64_t>(s.mx)) s.mx = x;
+        }
+        const uint64_t lo = s.a + x;
+        s.b += static_cast<uint64_t>(v >> 63) + (lo < s.a ? 1u : 0u);
+        s.a = lo;
+    } else if (kind == KIND_UNSIGNED) {
+        if (s.cnt == 0) {
+            s.mn = x;
+            s.mx = x;
+        } else {
+            if (x < s.mn) s.mn = x;
+            if (x > s.mx) s.mx = x;
+        }
+        const uint64_t lo = s.a + x;
+        s.b += (lo < s.a ? 1u : 0u);
+        s.a = lo;
+    }
+    s.cnt += 1;
+}
+
+// A precedes B in row order.
+__device__ inline Agg agg_join(const Agg& A, const Agg& B, int kind)
+{
+    if (B.cnt == 0) return A;
+    if (A.cnt == 0) return B;
+    Agg r;
+    r.cnt = A.cnt + B.cnt;
+    if (kind == KIND_F64) {
+        double t, e;
+        two_sum(as_d(A.a), as_d(B.a), t, e);
+        r.a = as_u(t);
+        r.b = as_u((as_d(A.b) + as_d(B.b)) + e);
+        r.mn = (as_d(B.mn) < as_d(A.mn)) ? B.mn : A.mn;
+        r.mx = (as_d(B.mx) > as_d(A.mx)) ? B.mx : A.mx;
+    } else {
+        const uint64_t lo = A.a + B.a;
+        r.b = A.b + B.b + (lo < A.a ? 1u : 0u);
+        r.a = lo;
+        if (kind == KIND_SIGNED) {
+            r.mn = (static_cast<int64_t>(B.mn) < static_cast<int64_t>(A.mn)) ? B.mn : A.mn;
+            r.mx = (static_cast<int64_t>(B.mx) > static_cast<int64_t>(A.mx)) ? B.mx : A.mx;
+        } else {
+            r.mn = (B.mn < A.mn) ? B.mn : A.mn;
+            r.mx = (B.mx > A.mx) ? B.mx : A.mx;
+        }
+    }
+    return r;
+}
+
+// Correctly rounded conversion of an unsigned 128-bit integer to double.
+__device__ inline double u128_to_double(uint64_t lo, uint64_t hi)
+{
+    if (hi == 0) return static_cast<double>(lo);
+    const int lz = __clzll(static_cast<long long>(hi));
+    uint64_t m = lz ? ((hi << lz) | (lo >> (64 - lz))) : hi;
+    const uint64_t rest = lz ? (lo << lz) : lo;
+    m |= (rest != 0) ? 1u : 0u;  // sticky bit below the rounding position
+    return ldexp(static_cast<double>(m), 64 - lz);
+}
+
+__device__ inline double i128_to_double(uint64_t lo, uint64_t hi)
+{
+    if (static_cast<int64_t>(hi) >= 0) return u128_to_double(lo, hi);
+    const uint64_t nlo = ~lo + 1u;
+    const uint64_t nhi = ~hi + (nlo == 0 ? 1u : 0u);
+    return -u128_to_double(nlo, nhi);
+}
+
+struct GbCol {
+    const void* data;
+    int type;   // VDF_VAL_*
+    int kind;   // KIND_*
+    void* sum;
+    double* mean;
+    void* mn;
+    void* mx;
+};
+
+struct GbArgs {
+    const void* skeys;     // sorted keys (caller's key type)
+    const uint32_t* perm;  // row ids in key order
+    uint64_t n;
+    uint32_t ntiles;
+    int ncols;             // >= 1 (a count-only column when no values are aggregated)
+    GbCol cols[VDF_MAX_VALUE_COLS];
+    uint32_t* counts;      // nullable
+    uint32_t* offsets;
+    void* group_keys;      // nullable
+    uint64_t* n_groups;
+    uint32_t* status;      // [ntiles] look-back words for head counts
+    uint32_t* ticket;
+    uint32_t* tile_gbase;  // [ntiles] first group id of the tile's heads
+    uint32_t* tile_heads;  // [ntiles] heads in the tile
+    Agg* prefix;           // [ncols][ntiles] rows before the tile's first head (or the whole tile)
+    Agg* suffix;           // [ncols][ntiles] rows from the tile's last head to its end
+};
+
+__device__ inline uint64_t load_val(const GbCol& c, uint64_t row)
+{
+    switch (c.type) {
+    case VDF_VAL_I32: return static_cast<uint64_t>(static_cast<int64_t>(static_cast<const int32_t*>(c.data)[row]));
+    case VDF_VAL_U32: return static_cast<uint64_t>(static_cast<const uint32_t*>(c.data)[row]);
+    default: return static_cast<const uint64_t*>(c.data)[row];
+    }
+}
+
+__device__ inline void write_final(const GbArgs& a, int c, uint32_t gid, const Agg& s)
+{
+    const GbCol& col = a.cols[c];
+    if (c == 0 && a.counts) a.counts[gid] = static_cast<uint32_t>(s.cnt);
+    if (col.kind == KIND_COUNT) return;
+    const bool w4 = (col.type == VDF_VAL_I32 || col.type == VDF_VAL_U32);
+    if (col.kind == KIND_F64) {
+        const double sum = as_d(s.a) + as_d(s.b);
+        if (col.sum) static_cast<double*>(col.sum)[gid] = sum;
+        if (col.mean) col.mean[gid] = sum / static_cast<double>(s.cnt);
+    } else {
+        if (col.sum) static_cast<uint64_t*>(col.sum)[gid] = s.a;
+        if (col.mean) {
+            const double tot = (col.kind == KIND_SIGNED) ? i128_to_double(s.a, s.b) : u128_to_double(s.a, s.b);
+            col.mean[gid] = tot / static_cast<double>(s.cnt);
+        }
+    }
+    if (col.mn) {
+        if (w4) static_cast<uint32_t*>(col.mn)[gid] = static_cast<uint32_t>(s.mn);
+        else static_cast<uint64_t*>(col.mn)[gid] = s.mn;
+    }
+    if (col.mx) {
+        if (w4) static_cast<uint32_t*>(col.mx)[gid] = static_cast<uint32_t>(s.mx);
+        else static_cast<uint64_t*>(col.mx)[gid] = s.mx;
+    }
+}
+
+__device__ inline void store_relaxed(uint32_t* p, uint32_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline uint32_t load_relaxed(uint32_t* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ inline int pad(int i) { return i + (i >> 4); }
+
+template <typename K>
+__global__ __launch_bounds__(kBlock) void k_gb_tile(GbArgs a)
+{
+    __shared__ uint64_t s_val[kGbPad];
+    __shared__ uint64_t s_hmask[kGbRows * kWaves];  // head bits of 64 striped rows
+    __shared__ Agg s_scan[kBlock];
+    __shared__ uint32_t s_flag[kBlock];
+    __shared__ uint32_t s_wsum[kWaves];
+    __shared__ uint32_t s_tile, s_gbase;
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) s_tile = atomicAdd(a.ticket, 1u);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const uint64_t tbase = static_cast<uint64_t>(tile) * kGbTile;
+    const K* keys = static_cast<const K*>(a.skeys);
+
+    // ---- head flags, striped rows (row = j*256 + tid), packed by ballot ----
+#pragma unroll
+    for (int j = 0; j < kGbRows; ++j) {
+        const uint64_t r = tbase + static_cast<uint64_t>(j) * kBlock + tid;
+        bool hd = false;
+        if (r < a.n) hd = (r == 0) || !keq(keys[r], keys[r - 1]);
+        const uint64_t m = __ballot(hd);
+        if (lane == 0) s_hmask[j * kWaves + wave] = m;
+    }
+    __syncthreads();
+
+    // ---- this thread's 16 consecutive rows: local rows tid*16 .. tid*16+15 ----
+    const int lr0 = tid * kGbRows;
+    const uint32_t hb = static_cast<uint32_t>(
+        (s_hmask[(lr0 >> 8) * kWaves + ((lr0 & 255) >> 6)] >> (lr0 & 63)) & 0xFFFFull);
+    const uint32_t hcount = __popc(hb);
+    const uint32_t hx = [&] {  // exclusive scan of head counts over the block
+        uint32_t inc = hcount;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(inc, off, 64);
+            if (lane >= off) inc += y;
+        }
+        if (lane == 63) s_wsum[wave] = inc;
+        __syncthreads();
+        uint32_t pre = 0;
+#pragma unroll
+        for (int i = 0; i < kWaves; ++i) pre += (i < wave) ? s_wsum[i] : 0u;
+        return pre + inc - hcount;
+    }();
+    const uint32_t H = s_wsum[0] + s_wsum[1] + s_wsum[2] + s_wsum[3];
+
+    // ---- decoupled look-back over the tiles' head counts -> first group id ----
+    if (tid == 0) {
+        uint32_t* my = a.status + tile;
+        store_relaxed(my, (tile == 0 ? kFlagP : kFlagA) | H);
+        uint32_t excl = 0;
+        if (tile > 0) {
+            uint32_t k = tile - 1;
+            for (;;) {
+                const uint32_t w = load_relaxed(a.status + k);
+                const uint32_t f = w & ~kValMask;
+                if (f == 0u) {
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                excl += w & kValMask;
+                if (f == kFlagP) break;
+                --k;
+            }
+            store_relaxed(my, kFlagP | (excl + H));
+        }
+        s_gbase = excl;
+        a.tile_gbase[tile] = excl;
+        a.tile_heads[tile] = H;
+        if (tile == a.ntiles - 1) {
+            *a.n_groups = static_cast<uint64_t>(excl) + H;
+            a.offsets[excl + H] = static_cast<uint32_t>(a.n);
+        }
+    }
+    __syncthreads();
+    const uint32_t gbase = s_gbase;
+
+    // ---- group keys and offsets at the heads ----
+    {
+        uint32_t bits = hb;
+        uint32_t g = gbase + hx;
+        while (bits) {
+            const int k = __ffs(bits) - 1;
+            bits &= bits - 1;
+            const uint64_t r = tbase + lr0 + k;
+            a.offsets[g] = static_cast<uint32_t>(r);
+            if (a.group_keys) static_cast<K*>(a.group_keys)[g] = keys[r];
+            ++g;
+        }
+    }
+
+    // ---- per value column: gather, sequential reduce, block segmented scan ----
+    const uint32_t tn = static_cast<uint32_t>(std::min<uint64_t>(kGbTile, a.n - tbase));
+    for (int c = 0; c < a.ncols; ++c) {
+        const GbCol col = a.cols[c];
+        const int kind = col.kind;
+        if (kind != KIND_COUNT) {
+#pragma unroll
+            for (int j = 0; j < kGbRows; ++j) {
+                const int li = j * kBlock + tid;
+                const uint64_t r = tbase + li;
+                if (r < a.n) s_val[pad(li)] = load_val(col, a.perm[r]);
+            }
+        }
+        __syncthreads();
+
+        Agg cur = agg_empty(), first = agg_empty();
+        bool seen = false;
+        uint32_t g = gbase + hx;  // id of the next head's group
+#pragma unroll
+        for (int j = 0; j < kGbRows; ++j) {
+            const int li = lr0 + j;
+            if (static_cast<uint32_t>(li) < tn) {
+                if ((hb >> j) & 1u) {
+                    if (!seen) {
+                        first = cur;
+                        seen = true;
+                    } else {
+                        write_final(a, c, g - 1, cur);
+                    }
+                    cur = agg_empty();
+                    ++g;
+                }
+                agg_add(cur, kind == KIND_COUNT ? 0ull : s_val[pad(li)], kind);
+            }
+        }
+
+        // segmented inclusive scan of (has head, tail partial) over threads
+        Agg x = cur;
+        uint32_t f = seen ? 1u : 0u;
+        s_scan[tid] = x;
+        s_flag[tid] = f;
+        __syncthreads();
+        for (int off = 1; off < kBlock; off <<= 1) {
+            Agg y = agg_empty();
+            uint32_t fy = 0;
+            if (tid >= off) {
+                y = s_scan[tid - off];
+                fy = s_flag[tid - off];
+            }
+            __syncthreads();
+            if (tid >= off) {
+                if (!f) x = agg_join(y, x, kind);
+                f |= fy;
+                s_scan[tid] = x;
+                s_flag[tid] = f;
+            }
+            __syncthreads();
+        }
+        // exclusive carry into this thread
+        const Agg carry = tid > 0 ? s_scan[tid - 1] : agg_empty();
+        const uint32_t carry_f = tid > 0 ? s_flag[tid - 1] : 0u;
+        Agg* pre = a.prefix + static_cast<uint64_t>(c) * a.ntiles;
+        Agg* suf = a.suffix + static_cast<uint64_t>(c) * a.ntiles;
+        if (seen) {
+            const Agg seg = agg_join(carry, first, kind);
+            if (carry_f) write_final(a, c, gbase + hx - 1, seg);
+            else pre[tile] = seg;  // the tile's first head: rows before it
+        }
+        if (tid == kBlock - 1) {
+            if (f) suf[tile] = x;
+            else pre[tile] = x;
+        }
+        __syncthreads();
+    }
+}
+
+// Joins the group that starts at tile b's last head with the leading
+// partials of the following tiles up to (and including) the next tile that
+// has a head.
+__global__ __launch_bounds__(kBlock) void k_gb_fixup(GbArgs a)
+{
+    __shared__ Agg s_red[kBlock];
+    __shared__ uint32_t s_first;
+    const uint32_t b = blockIdx.x;
+    const int tid = threadIdx.x;
+    const uint32_t heads = a.tile_heads[b];
+    if (heads == 0) return;
+    const uint32_t gid = a.tile_gbase[b] + heads - 1;
+    const bool short_chain = (b + 1 >= a.ntiles) || a.tile_heads[b + 1] > 0;
+    for (int c = 0; c < a.ncols; ++c) {
+        const int kind = a.cols[c].kind;
+        const Agg* pre = a.prefix + static_cast<uint64_t>(c) * a.ntiles;
+        Agg acc = a.suffix[static_cast<uint64_t>(c) * a.ntiles + b];
+        if (short_chain) {
+            if (tid == 0) {
+                if (b + 1 < a.ntiles) acc = agg_join(acc, pre[b + 1], kind);
+                write_final(a, c, gid, acc);
+            }
+            continue;
+        }
+        for (uint32_t t0 = b + 1; t0 < a.ntiles; t0 += kBlock) {
+            const uint32_t t = t0 + tid;
+            if (tid == 0) s_first = kBlock;
+            __syncthreads();
+            const bool stop = (t >= a.ntiles) || a.tile_heads[t] > 0;
+            if (stop) atomicMin(&s_first, static_cast<uint32_t>(tid));
+            __syncthreads();
+            const uint32_t first = s_first;
+            s_red[tid] = (t < a.ntiles && static_cast<uint32_t>(tid) <= first) ? pre[t] : agg_empty();
+            __syncthreads();
+            for (int s = 1; s < kBlock; s <<= 1) {
+                if ((tid & (2 * s - 1)) == 0) s_red[tid] = agg_join(s_red[tid], s_red[tid + s], kind);
+                __syncthreads();
+            }
+            acc = agg_join(acc, s_red[0], kind);
+            __syncthreads();
+            if (first < static_cast<uint32_t>(kBlock)) break;
+        }
+        if (tid == 0) write_final(a, c, gid, acc);
+    }
+}
+
+__global__ void k_gb_empty(uint64_t* n_groups, uint32_t* offsets)
+{
+    *n_groups = 0;
+    offsets[0] = 0;
+}
+
+int kind_of(int type)
+{
+    switch (type) {
+    case VDF_VAL_F64: return KIND_F64;
+    case VDF_VAL_I64:
+    case VDF_VAL_I32: return KIND_SIGNED;
+    default: return KIND_UNSIGNED;
+    }
+}
+
+struct GbLayout {
+    uint64_t ntiles;
+    size_t sort_bytes, bytes;
+};
+
+GbLayout gb_layout(int key_type, uint64_t n)
+{
+    GbLayout L;
+    L.ntiles = (n + kGbTile - 1) / kGbTile;
+    L.sort_bytes = align_up(argsort_scratch(key_type, n, 0), 256);
+    size_t b = L.sort_bytes;
+    b += align_up(n * key_bytes(key_type), 256);   // sorted keys
+    b += align_up(n * 4, 256);                     // perm when the caller passes none
+    b += align_up((L.ntiles + 1) * 4, 256);        // status + ticket
+    b += 2 * align_up(L.ntiles * 4, 256);          // tile_gbase, tile_heads
+    b += 2 * align_up(VDF_MAX_VALUE_COLS * L.ntiles * sizeof(Agg), 256);
+    L.bytes = b + 256;
+    return L;
+}
+
+}  // namespace
+
+size_t groupby_scratch(int key_type, uint64_t n) { return gb_layout(key_type, n).bytes; }
+
+void groupby_enqueue(Handle* h, int key_type, const void* d_keys, uint64_t n, const vdf_value_col* vals,
+                     int nvals, uint32_t agg_mask, const vdf_groupby_out* out, void* scr, hipStream_t s)
+{
+    if (n == 0) {
+        LaunchTimer t(h, s, "groupby_empty", 0);
+        hipLaunchKernelGGL(k_gb_empty, dim3(1), dim3(1), 0, s, out->n_groups, out->offsets);
+        check_launch("k_gb_empty");
+        return;
+    }
+    const GbLayout L = gb_layout(key_type, n);
+    const size_t kb = key_bytes(key_type);
+    Carve cv(scr);
+    void* sort_scr = cv.take<char>(L.sort_bytes);
+    void* skeys = cv.take<char>(n * kb);
+    uint32_t* perm_scr = cv.take<uint32_t>(n);
+    uint32_t* status = cv.take<uint32_t>(L.ntiles + 1);
+    GbArgs a{};
+    a.tile_gbase = cv.take<uint32_t>(L.ntiles);
+    a.tile_heads = cv.take<uint32_t>(L.ntiles);
+    a.prefix = cv.take<Agg>(VDF_MAX_VALUE_COLS * L.ntiles);
+    a.suffix = cv.take<Agg>(VDF_MAX_VALUE_COLS * L.ntiles);
+    uint32_t* perm = out->perm ? out->perm : perm_scr;
+
+    argsort_enqueue(h, key_type, d_keys, n, 0, perm, skeys, sort_scr, s);
+
+    a.skeys = skeys;
+    a.perm = perm;
+    a.n = n;
+    a.ntiles = static_cast<uint32_t>(L.ntiles);
+    a.counts = (agg_mask & VDF_AGG_COUNT) ? out->counts : nullptr;
+    a.offsets = out->offsets;
+    a.group_keys = out->group_keys;
+    a.n_groups = out->n_groups;
+    a.status = status;
+    a.ticket = status + L.ntiles;
+    uint64_t val_bytes = 0;
+    if (nvals == 0) {
+        a.ncols = 1;
+        a.cols[0] = GbCol{nullptr, VDF_VAL_U32, KIND_COUNT, nullptr, nullptr, nullptr, nullptr};
+    } else {
+        a.ncols = nvals;
+        for (int c = 0; c < nvals; ++c) {
+            const vdf_agg_out& o = out->aggs[c];
+            GbCol& g = a.cols[c];
+            g.data = vals[c].data;
+            g.type = vals[c].type;
+            g.kind = kind_of(vals[c].type);
+            g.sum = (agg_mask & VDF_AGG_SUM) ? o.sum : nullptr;
+            g.mean = (agg_mask & VDF_AGG_MEAN) ? o.mean : nullptr;
+            g.mn = (agg_mask & VDF_AGG_MIN) ? o.min : nullptr;
+            g.mx = (agg_mask & VDF_AGG_MAX) ? o.max : nullptr;
+            val_bytes += 4 + ((vals[c].type == VDF_VAL_I32 || vals[c].type == VDF_VAL_U32) ? 4 : 8);
+        }
+    }
+    {
+        LaunchTimer t(h, s, "fill", 0);
+        fill_u32(status, L.ntiles + 1, 0u, s);
+    }
+    {
+        // algorithmic bytes: sorted keys (+ the previous key) + per column row id and value
+        LaunchTimer t(h, s, "groupby_tile", n * (kb + val_bytes));
+        switch (kb) {
+        case 4: hipLaunchKernelGGL(k_gb_tile<uint32_t>, dim3(a.ntiles), dim3(kBlock), 0, s, a); break;
+        case 8: hipLaunchKernelGGL(k_gb_tile<uint64_t>, dim3(a.ntiles), dim3(kBlock), 0, s, a); break;
+        default: hipLaunchKernelGGL(k_gb_tile<K16>, dim3(a.ntiles), dim3(kBlock), 0, s, a); break;
+        }
+        check_launch("k_gb_tile");
+    }
+    {
+        LaunchTimer t(h, s, "groupby_fixup", 0);
+        hipLaunchKernelGGL(k_gb_fixup, dim3(a.ntiles), dim3(kBlock), 0, s, a);
+        check_launch("k_gb_fixup");
+    }
 }
 
 }  // namespace vdf

@@ -1,8 +1,10 @@
 """Seeded synthetic inputs for the radix-sort / group-by workloads.
 
 All columns come from SplitMix64 streams keyed by (seed, column id), generated
-on the host with numpy, so the same arrays feed the GPU path and the CPU
-baseline bit-for-bit.
+on the host (libvdf_datagen.so, OpenMP; numpy restatements below when it is
+not built), so the same arrays feed the GPU path and the CPU baseline
+bit-for-bit.  The library and the numpy code agree bit-for-bit except for
+normal01, where libm and numpy's log / cos may differ by a couple of ulps.
 
 Workloads (row counts as used by the benchmark and the parity tests):
   groupby_small   n rows, int64 key uniform in [0, 1000), f64 value uniform
@@ -15,7 +17,31 @@ Workloads (row counts as used by the benchmark and the parity tests):
 """
 from __future__ import annotations
 
+import ctypes as C
+import os
+
 import numpy as np
+
+_LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libvdf_datagen.so")
+_lib = None
+
+
+def _gen():
+    """The OpenMP generator library (libvdf_datagen.so) or None when unbuilt."""
+    global _lib
+    if _lib is None and os.path.exists(_LIB_PATH):
+        L = C.CDLL(_LIB_PATH)
+        u64, vp, i32, dbl = C.c_uint64, C.c_void_p, C.c_int, C.c_double
+        L.vdg_splitmix64.argtypes = [u64, u64, u64, u64, vp, i32]
+        L.vdg_uniform01.argtypes = [u64, u64, u64, vp, i32]
+        L.vdg_normal01.argtypes = [u64, u64, u64, vp, i32]
+        L.vdg_zipf_ranks.argtypes = [u64, u64, u64, dbl, u64, vp, i32]
+        _lib = L
+    return _lib
+
+
+def _threads() -> int:
+    return max(1, min(16, os.cpu_count() or 1))
 
 _GOLDEN = np.uint64(0x9E3779B97F4A7C15)
 _M1 = np.uint64(0xBF58476D1CE4E5B9)
@@ -28,8 +54,13 @@ def _mix(z: np.ndarray) -> np.ndarray:
     return z ^ (z >> np.uint64(31))
 
 
-def splitmix64(seed: int, col: int, n: int, start: int = 0) -> np.ndarray:
+def splitmix64(seed: int, col: int, n: int, start: int = 0, use_lib: bool = True) -> np.ndarray:
     """n outputs of the SplitMix64 stream for (seed, col), from index `start`."""
+    L = _gen() if use_lib else None
+    if L is not None:
+        out = np.empty(n, dtype=np.uint64)
+        L.vdg_splitmix64(seed & 0xFFFFFFFFFFFFFFFF, col, start, n, out.ctypes.data, _threads())
+        return out
     with np.errstate(over="ignore"):
         s0 = _mix(np.array([np.uint64(seed & 0xFFFFFFFFFFFFFFFF) * _GOLDEN + np.uint64(col & 0xFFFF)],
                            dtype=np.uint64))[0]
@@ -42,8 +73,13 @@ def splitmix64(seed: int, col: int, n: int, start: int = 0) -> np.ndarray:
     return out
 
 
-def uniform01(seed: int, col: int, n: int) -> np.ndarray:
+def uniform01(seed: int, col: int, n: int, use_lib: bool = True) -> np.ndarray:
     """Doubles uniform in [0, 1): top 53 bits of each draw times 2^-53."""
+    L = _gen() if use_lib else None
+    if L is not None:
+        out = np.empty(n, dtype=np.float64)
+        L.vdg_uniform01(seed & 0xFFFFFFFFFFFFFFFF, col, n, out.ctypes.data, _threads())
+        return out
     return (splitmix64(seed, col, n) >> np.uint64(11)).astype(np.float64) * (2.0 ** -53)
 
 
@@ -53,20 +89,31 @@ def uniform_int(seed: int, col: int, n: int, lo: int, hi: int) -> np.ndarray:
     return (splitmix64(seed, col, n) % span).astype(np.int64) + lo
 
 
-def normal01(seed: int, col: int, n: int) -> np.ndarray:
+def normal01(seed: int, col: int, n: int, use_lib: bool = True) -> np.ndarray:
     """N(0, 1) by the Box-Muller transform of two uniform streams."""
-    u1 = uniform01(seed, col, n)
-    u2 = uniform01(seed, col + 1000, n)
+    L = _gen() if use_lib else None
+    if L is not None:
+        out = np.empty(n, dtype=np.float64)
+        L.vdg_normal01(seed & 0xFFFFFFFFFFFFFFFF, col, n, out.ctypes.data, _threads())
+        return out
+    u1 = uniform01(seed, col, n, use_lib=False)
+    u2 = uniform01(seed, col + 1000, n, use_lib=False)
     u1 = np.where(u1 <= 0.0, 2.0 ** -53, u1)
     return np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u2)
 
 
-def zipf_ranks(seed: int, col: int, n: int, s: float = 1.1, domain: int = 1 << 24) -> np.ndarray:
+def zipf_ranks(seed: int, col: int, n: int, s: float = 1.1, domain: int = 1 << 24,
+               use_lib: bool = True) -> np.ndarray:
     """Zipf(s) ranks in [0, domain) by exact inverse-CDF table lookup."""
+    L = _gen() if use_lib else None
+    if L is not None:
+        out = np.empty(n, dtype=np.int64)
+        L.vdg_zipf_ranks(seed & 0xFFFFFFFFFFFFFFFF, col, n, s, domain, out.ctypes.data, _threads())
+        return out
     w = np.arange(1, domain + 1, dtype=np.float64) ** (-s)
     cdf = np.cumsum(w)
     cdf /= cdf[-1]
-    u = uniform01(seed, col, n)
+    u = uniform01(seed, col, n, use_lib=False)
     r = np.searchsorted(cdf, u, side="right")
     return np.minimum(r, domain - 1).astype(np.int64)
 
