@@ -66,6 +66,9 @@ def main():
     ap.add_argument("--rows", type=int, default=100_000_000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--calibrate", action="store_true",
+                    help="first run one plain 1 GiB device copy (the known-size kernel that scales "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU sharding is a design "
@@ -87,6 +90,12 @@ def main():
     perm = torch.empty(n, dtype=torch.int32, device="cuda")
     out = torch.empty(n, dtype=torch.float64, device="cuda")
     v = Vdf(0)
+    if args.calibrate:
+        a = torch.empty(1 << 27, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
 
     def step():
         v.argsort(d_keys, key_type=N.KEY_U32, perm_out=perm)

@@ -148,6 +148,19 @@ vdf_status vdf_groupby(vdf_handle* h, int key_type, const void* d_keys, uint64_t
                        const vdf_value_col* vals, int nvals, uint32_t agg_mask,
                        vdf_groupby_out* out, void* stream);
 
+/* ---- tuning table (per handle) ----
+ * Knobs (unknown names or unsupported values -> VDF_ERR_INVALID):
+ *   "sort_items_32"  rows per thread of a radix pass tile, 4-byte keys: 12, 16, 24
+ *   "sort_items_64"  same for 8-byte keys: 8, 12, 16   (16-byte keys use 8)
+ *   "sort_chunk"     consecutive tiles one tile group (one XCD) claims at a time: 1 .. 1024
+ *   "hist_grid"      blocks of the digit-histogram kernel: 64 .. 65535
+ *   "gather_rows"    rows per thread of the gather kernel: 4, 8, 16
+ *   "gather_nt"      1 = non-temporal loads of the gathered column
+ * No reference counterpart: the reference fixes its digit width and thread
+ * count at compile time (radix_sort_uint32_mt.inl:3-5). */
+vdf_status vdf_tune_set(vdf_handle* h, const char* name, int64_t value);
+vdf_status vdf_tune_get(vdf_handle* h, const char* name, int64_t* value);
+
 /* ---- per-kernel timing (HIP events on the launch stream) ----
  * While enabled, every kernel launch of this handle is bracketed by events.
  * vdf_profile_read synchronises the recorded events and returns, for one

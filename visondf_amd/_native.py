@@ -38,6 +38,7 @@ EXPORTS = (
     "vdf_argsort_scratch_bytes", "vdf_groupby_scratch_bytes",
     "vdf_argsort", "vdf_gather", "vdf_groupby",
     "vdf_profile_enable", "vdf_profile_reset", "vdf_profile_read",
+    "vdf_tune_set", "vdf_tune_get",
 )
 
 
@@ -103,6 +104,10 @@ def load() -> C.CDLL:
     lib.vdf_profile_reset.argtypes = [vp]
     lib.vdf_profile_read.restype = i32
     lib.vdf_profile_read.argtypes = [vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(u64), C.POINTER(u64)]
+    lib.vdf_tune_set.restype = i32
+    lib.vdf_tune_set.argtypes = [vp, C.c_char_p, C.c_int64]
+    lib.vdf_tune_get.restype = i32
+    lib.vdf_tune_get.argtypes = [vp, C.c_char_p, C.POINTER(C.c_int64)]
     _lib = lib
     return lib
 

@@ -250,6 +250,40 @@ vdf_status vdf_groupby(vdf_handle* h, int key_type, const void* d_keys, uint64_t
     });
 }
 
+static int* tune_slot(Tuning& t, const std::string& n)
+{
+    if (n == "sort_items_32") return &t.sort_items_32;
+    if (n == "sort_items_64") return &t.sort_items_64;
+    if (n == "sort_chunk") return &t.sort_chunk;
+    if (n == "hist_grid") return &t.hist_grid;
+    if (n == "gather_rows") return &t.gather_rows;
+    if (n == "gather_nt") return &t.gather_nt;
+    return nullptr;
+}
+
+vdf_status vdf_tune_set(vdf_handle* h, const char* name, int64_t value)
+{
+    return guarded([&] {
+        require(h != nullptr && name != nullptr, "vdf_tune_set: null argument");
+        std::lock_guard<std::mutex> lk(h->mu);
+        int* slot = tune_slot(h->tune, name);
+        require(slot != nullptr, "vdf_tune_set: unknown tuning knob");
+        require(tune_supported(name, value), "vdf_tune_set: value not supported for this knob");
+        *slot = static_cast<int>(value);
+    });
+}
+
+vdf_status vdf_tune_get(vdf_handle* h, const char* name, int64_t* value)
+{
+    return guarded([&] {
+        require(h != nullptr && name != nullptr && value != nullptr, "vdf_tune_get: null argument");
+        std::lock_guard<std::mutex> lk(h->mu);
+        int* slot = tune_slot(h->tune, name);
+        require(slot != nullptr, "vdf_tune_get: unknown tuning knob");
+        *value = *slot;
+    });
+}
+
 vdf_status vdf_profile_enable(vdf_handle* h, int on)
 {
     return guarded([&] {

@@ -10,11 +10,9 @@
 
 namespace vdf {
 
-struct alignas(16) V16 {
-    uint64_t a, b;
-};
+typedef uint64_t V16 __attribute__((ext_vector_type(2)));  // 16-byte cells (char[16] keys)
 
-template <typename V, typename P, int ROWS>
+template <typename V, typename P, int ROWS, bool NT>
 __global__ __launch_bounds__(256) void k_gather(const V* __restrict__ src, const P* __restrict__ perm, uint64_t n,
                                                 V* __restrict__ dst)
 {
@@ -29,7 +27,7 @@ __global__ __launch_bounds__(256) void k_gather(const V* __restrict__ src, const
 #pragma unroll
     for (int u = 0; u < ROWS; ++u) {
         const uint64_t i = base + static_cast<uint64_t>(u) * 256;
-        if (i < n) v[u] = src[idx[u]];
+        if (i < n) v[u] = NT ? __builtin_nontemporal_load(src + idx[u]) : src[idx[u]];
     }
 #pragma unroll
     for (int u = 0; u < ROWS; ++u) {
@@ -38,15 +36,27 @@ __global__ __launch_bounds__(256) void k_gather(const V* __restrict__ src, const
     }
 }
 
-template <typename V, typename P>
-static void launch_gather(const void* src, const void* perm, uint64_t n, void* dst, hipStream_t s)
+template <typename V, typename P, int ROWS, bool NT>
+static void launch_rows(const void* src, const void* perm, uint64_t n, void* dst, hipStream_t s)
 {
-    constexpr int ROWS = 8;
     const uint64_t per_block = 256 * ROWS;
     const unsigned grid = static_cast<unsigned>((n + per_block - 1) / per_block);
-    hipLaunchKernelGGL((k_gather<V, P, ROWS>), dim3(grid), dim3(256), 0, s, static_cast<const V*>(src),
+    hipLaunchKernelGGL((k_gather<V, P, ROWS, NT>), dim3(grid), dim3(256), 0, s, static_cast<const V*>(src),
                        static_cast<const P*>(perm), n, static_cast<V*>(dst));
     check_launch("k_gather");
+}
+
+template <typename V, typename P>
+static void launch_gather(const Tuning& t, const void* src, const void* perm, uint64_t n, void* dst, hipStream_t s)
+{
+    if (t.gather_nt) {
+        if (t.gather_rows == 4) return launch_rows<V, P, 4, true>(src, perm, n, dst, s);
+        if (t.gather_rows == 16) return launch_rows<V, P, 16, true>(src, perm, n, dst, s);
+        return launch_rows<V, P, 8, true>(src, perm, n, dst, s);
+    }
+    if (t.gather_rows == 4) return launch_rows<V, P, 4, false>(src, perm, n, dst, s);
+    if (t.gather_rows == 16) return launch_rows<V, P, 16, false>(src, perm, n, dst, s);
+    return launch_rows<V, P, 8, false>(src, perm, n, dst, s);
 }
 
 void gather_enqueue(Handle* h, uint32_t elem_bytes, const void* d_src, const void* d_perm, int perm64,
@@ -57,16 +67,16 @@ void gather_enqueue(Handle* h, uint32_t elem_bytes, const void* d_src, const voi
     LaunchTimer t(h, s, "gather", n * (pb + 2ull * elem_bytes));
     switch (elem_bytes) {
     case 4:
-        perm64 ? launch_gather<uint32_t, uint64_t>(d_src, d_perm, n, d_dst, s)
-               : launch_gather<uint32_t, uint32_t>(d_src, d_perm, n, d_dst, s);
+        perm64 ? launch_gather<uint32_t, uint64_t>(h->tune, d_src, d_perm, n, d_dst, s)
+               : launch_gather<uint32_t, uint32_t>(h->tune, d_src, d_perm, n, d_dst, s);
         break;
     case 8:
-        perm64 ? launch_gather<uint64_t, uint64_t>(d_src, d_perm, n, d_dst, s)
-               : launch_gather<uint64_t, uint32_t>(d_src, d_perm, n, d_dst, s);
+        perm64 ? launch_gather<uint64_t, uint64_t>(h->tune, d_src, d_perm, n, d_dst, s)
+               : launch_gather<uint64_t, uint32_t>(h->tune, d_src, d_perm, n, d_dst, s);
         break;
     case 16:
-        perm64 ? launch_gather<V16, uint64_t>(d_src, d_perm, n, d_dst, s)
-               : launch_gather<V16, uint32_t>(d_src, d_perm, n, d_dst, s);
+        perm64 ? launch_gather<V16, uint64_t>(h->tune, d_src, d_perm, n, d_dst, s)
+               : launch_gather<V16, uint32_t>(h->tune, d_src, d_perm, n, d_dst, s);
         break;
     default: throw Error{VDF_ERR_INVALID, "vdf_gather: elem_bytes must be 4, 8 or 16"};
     }

@@ -6,7 +6,7 @@
 // inlines/transform/sort_by/sort_{uintegers,integers,char}.inl.
 //
 // Launch sequence for one sort of n rows (all on one stream, no host sync):
-//   1. k_fill        zero the digit histograms and pass tickets
+//   1. k_fill        zero the digit histograms and tile counters
 //   2. k_histogram   one read of the keys -> 256-bucket histogram of every
 //                    8-bit digit (LDS-privatised), clears look-back buffer 0
 //   3. k_plan        exclusive bucket offsets per digit; digits whose
@@ -95,20 +95,6 @@ struct KeyOps<K128> {
                      : static_cast<uint32_t>(k.hi >> (8 * (d - 8))) & 0xFFu;
     }
     __device__ static K128 zero() { return K128{0ull, 0ull}; }
-};
-
-// rows per thread in a scatter tile, by key width (LDS and VGPR budget)
-template <typename K>
-struct PassItems {
-    static constexpr int value = 16;
-};
-template <>
-struct PassItems<uint64_t> {
-    static constexpr int value = 12;
-};
-template <>
-struct PassItems<K128> {
-    static constexpr int value = 8;
 };
 
 __device__ inline void store_relaxed(uint32_t* p, uint32_t v)
@@ -226,14 +212,53 @@ struct PassArgs {
     uint32_t* vbuf[2];     // row ids between passes
     void* perm_out;        // caller's permutation, written by the last pass
     uint32_t* lb[2];       // look-back words, [tiles][256], alternating by slot parity
-    uint32_t* tickets;     // [kMaxDigits] dynamic tile ids
+    uint32_t* tickets;     // per pass slot: one claim counter per tile group
     const Plan* plan;
     uint64_t n;
     uint64_t mlo, mhi;     // key transform masks
+    uint32_t ntiles;
+    uint32_t groups;       // tile groups (one per XCD when the grid allows)
+    uint32_t chunk;        // consecutive tiles a group claims at a time
     uint32_t perm64;
     uint32_t reverse;
 };
 
+// LDS-only block barrier: orders LDS traffic between the block's waves
+// without waiting for the block's outstanding global stores.
+__device__ inline void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+__device__ inline uint32_t block_excl_scan_lds(uint32_t x, uint32_t* s_wsum)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t inc = x;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(inc, off, 64);
+        if (lane >= off) inc += y;
+    }
+    if (lane == 63) s_wsum[w] = inc;
+    lds_barrier();
+    uint32_t pre = 0;
+#pragma unroll
+    for (int i = 0; i < kWaves; ++i) pre += (i < w) ? s_wsum[i] : 0u;
+    lds_barrier();
+    return pre + inc - x;
+}
+
+// One radix scatter pass, persistent and software-pipelined.
+//
+// Per tile: load (coalesced, striped rows), stable rank inside each wave,
+// publish the tile's per-bucket counts (flag A), stage the tile in LDS in
+// bucket order.  The block then claims, loads and ranks its next tile and
+// publishes that aggregate before it resolves the staged tile's decoupled
+// look-back (flag P) and writes it out, so look-back waits overlap work.
+//
+// Ranking of item j in a wave: every lane ORs its lane bit into the wave's
+// 64-bit mask of its digit; reading the mask back gives the lanes sharing
+// the digit (peers).  The lowest peer adds the group size to the wave's
+// bucket counter and clears the mask; every peer reads the counter back.
+// LDS operations of one wave complete in program order, so each read sees
+// exactly the items before j, and the result is the stable rank.
 template <typename K, int ITEMS>
 __global__ __launch_bounds__(kBlock) void k_pass(PassArgs a, uint32_t slot)
 {
@@ -247,148 +272,198 @@ __global__ __launch_bounds__(kBlock) void k_pass(PassArgs a, uint32_t slot)
 
     __shared__ K s_keys[TILE];
     __shared__ uint32_t s_vals[TILE];
+    __shared__ uint64_t s_mask[kWaves][kRadix];
     __shared__ uint32_t s_wc[kWaves][kRadix];
     __shared__ uint32_t s_gbase[kRadix];
     __shared__ uint32_t s_wsum[kWaves];
     __shared__ uint32_t s_tile;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    if (tid == 0) s_tile = atomicAdd(&a.tickets[slot], 1u);
-#pragma unroll
-    for (int w = 0; w < kWaves; ++w) s_wc[w][tid] = 0;
-    __syncthreads();
-    const uint32_t tile = s_tile;
-    const uint64_t tbase = static_cast<uint64_t>(tile) * TILE;
-
     uint32_t* lb = a.lb[slot & 1];
-    a.lb[(slot + 1) & 1][static_cast<uint64_t>(tile) * kRadix + tid] = 0u;  // next pass's row
-
-    // ---- load: wave w owns rows [w*64*ITEMS, (w+1)*64*ITEMS) of the tile,
-    //      item j of lane l is row j*64 + l (coalesced, row order = (j, l)) ----
+    uint32_t* lbn = a.lb[(slot + 1) & 1];
+    const K* kin = static_cast<const K*>(a.kbuf[(slot - 1) & 1]);
+    const uint32_t* vin = a.vbuf[(slot - 1) & 1];
     K key[ITEMS];
     uint32_t val[ITEMS];
     uint32_t rnk[ITEMS];
-    const uint64_t wbase = tbase + static_cast<uint64_t>(wave) * 64 * ITEMS + lane;
-    if (first) {
-#pragma unroll
-        for (int j = 0; j < ITEMS; ++j) {
-            const uint64_t i = wbase + static_cast<uint64_t>(j) * 64;
-            key[j] = (i < a.n) ? KO::load(a.keys_in, i, a.mlo, a.mhi) : KO::zero();
-            val[j] = static_cast<uint32_t>(i);
-        }
-    } else {
-        const K* kin = static_cast<const K*>(a.kbuf[(slot - 1) & 1]);
-        const uint32_t* vin = a.vbuf[(slot - 1) & 1];
-#pragma unroll
-        for (int j = 0; j < ITEMS; ++j) {
-            const uint64_t i = wbase + static_cast<uint64_t>(j) * 64;
-            const bool v = i < a.n;
-            key[j] = v ? kin[i] : KO::zero();
-            val[j] = v ? vin[i] : 0u;
-        }
-    }
 
-    // ---- stable rank inside the wave.  For item j, `dif` collects the lanes
-    //      whose digit differs from this lane's in any bit (8 ballots of the
-    //      sign-extended bit); peers = valid lanes outside `dif`.  The lowest
-    //      peer adds the group size to the wave's bucket counter and every peer
-    //      reads the counter back right after; LDS operations of one wave
-    //      complete in program order, so these reads see exactly the items
-    //      before j and the 2*ITEMS LDS operations pipeline without waits. ----
-    uint32_t* wc = s_wc[wave];
-#pragma unroll
-    for (int j = 0; j < ITEMS; ++j) {
-        const bool v = (wbase + static_cast<uint64_t>(j) * 64) < a.n;
-        const uint32_t d = KO::digit(key[j], dig);
-        uint32_t dif_lo = 0u, dif_hi = 0u;
-#pragma unroll
-        for (int b = 0; b < kRadixBits; ++b) {
-            const uint32_t m = static_cast<uint32_t>(static_cast<int32_t>(d << (31 - b)) >> 31);
-            const uint64_t bb = __ballot(m != 0u);
-            dif_lo |= static_cast<uint32_t>(bb) ^ m;
-            dif_hi |= static_cast<uint32_t>(bb >> 32) ^ m;
+    // Tile claiming.  Block b belongs to group g = b % groups; dispatch puts
+    // consecutive blocks on consecutive XCDs, so a group is one XCD.  The
+    // k-th claim of group g gets tile (k / chunk * groups + g) * chunk +
+    // k % chunk: runs of `chunk` consecutive tiles are written by CUs sharing
+    // one L2, which merges the partial lines of adjacent bucket runs.  The
+    // kernel is launched cooperatively, so every block is resident; each group
+    // claims its tiles in increasing order and every claimed tile publishes
+    // its aggregate before its block waits, so the smallest unfinished tile
+    // always completes: no deadlock.
+    auto claim = [&]() -> uint32_t {
+        if (tid == 0) {
+            const uint32_t g = blockIdx.x % a.groups;
+            const uint32_t k = atomicAdd(a.tickets + slot * kTicketStride + g, 1u);
+            const uint64_t t = (static_cast<uint64_t>(k / a.chunk) * a.groups + g) * a.chunk + (k % a.chunk);
+            s_tile = static_cast<uint32_t>(std::min<uint64_t>(t, 0xFFFFFFFFull));
         }
-        const uint64_t vm = __ballot(v);
-        const uint32_t p_lo = ~dif_lo & static_cast<uint32_t>(vm);
-        const uint32_t p_hi = ~dif_hi & static_cast<uint32_t>(vm >> 32);
-        const uint32_t lt = __builtin_amdgcn_mbcnt_hi(p_hi, __builtin_amdgcn_mbcnt_lo(p_lo, 0u));
-        const uint32_t cnt = static_cast<uint32_t>(__popc(p_lo) + __popc(p_hi));
-        if (v && lt == 0u) atomicAdd(&wc[d], cnt);
-        uint32_t after = 0u;
-        if (v) after = wc[d];
-        rnk[j] = after - cnt + lt;
-    }
-    __syncthreads();
+        lds_barrier();
+        const uint32_t t = s_tile;
+        lds_barrier();
+        return t;
+    };
 
-    // ---- thread tid owns bucket tid: wave-exclusive prefixes, tile total ----
-    uint32_t tot = 0;
+    // load tile t, rank it, publish its aggregate; returns the bucket total and
+    // tile-local bucket start of bucket `tid`; s_wc then holds each wave's
+    // first LDS slot per bucket
+    auto load_rank = [&](uint32_t t, uint32_t& tot, uint32_t& lstart) {
 #pragma unroll
-    for (int w = 0; w < kWaves; ++w) {
-        const uint32_t c = s_wc[w][tid];
-        s_wc[w][tid] = tot;
-        tot += c;
-    }
-    uint32_t* my = lb + static_cast<uint64_t>(tile) * kRadix + tid;
-    store_relaxed(my, (tile == 0 ? kFlagP : kFlagA) | tot);
-    const uint32_t lstart = block_excl_scan(tot, s_wsum);
+        for (int w = 0; w < kWaves; ++w) {
+            s_wc[w][tid] = 0;
+            s_mask[w][tid] = 0;
+        }
+        lbn[static_cast<uint64_t>(t) * kRadix + tid] = 0u;  // the next pass's row of this tile
+        const uint64_t wbase = static_cast<uint64_t>(t) * TILE + static_cast<uint64_t>(wave) * 64 * ITEMS + lane;
+        if (first) {
 #pragma unroll
-    for (int w = 0; w < kWaves; ++w) s_wc[w][tid] += lstart;
-
-    // ---- decoupled look-back over earlier tiles of this bucket ----
-    uint32_t excl = 0;
-    if (tile > 0) {
-        uint32_t k = tile - 1;
-        for (;;) {
-            const uint32_t w = load_relaxed(lb + static_cast<uint64_t>(k) * kRadix + tid);
-            const uint32_t f = w & ~kValMask;
-            if (f == 0u) {
-                __builtin_amdgcn_s_sleep(1);
-                continue;
+            for (int j = 0; j < ITEMS; ++j) {
+                const uint64_t i = wbase + static_cast<uint64_t>(j) * 64;
+                key[j] = (i < a.n) ? KO::load(a.keys_in, i, a.mlo, a.mhi) : KO::zero();
+                val[j] = static_cast<uint32_t>(i);
             }
-            excl += w & kValMask;
-            if (f == kFlagP) break;
-            --k;
-        }
-        store_relaxed(my, kFlagP | (excl + tot));
-    }
-    s_gbase[tid] = a.plan->base[dig][tid] + excl - lstart;
-    __syncthreads();
-
-    // ---- stage the tile in LDS in bucket order ----
+        } else {
 #pragma unroll
-    for (int j = 0; j < ITEMS; ++j) {
-        if (wbase + static_cast<uint64_t>(j) * 64 < a.n) {
+            for (int j = 0; j < ITEMS; ++j) {
+                const uint64_t i = wbase + static_cast<uint64_t>(j) * 64;
+                const bool v = i < a.n;
+                key[j] = v ? kin[i] : KO::zero();
+                val[j] = v ? vin[i] : 0u;
+            }
+        }
+        lds_barrier();
+        uint32_t* wc = s_wc[wave];
+        unsigned long long* mk = reinterpret_cast<unsigned long long*>(s_mask[wave]);
+        const unsigned long long lanebit = 1ull << lane;
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) {
+            const bool v = (wbase + static_cast<uint64_t>(j) * 64) < a.n;
             const uint32_t d = KO::digit(key[j], dig);
-            const uint32_t p = s_wc[wave][d] + rnk[j];
-            s_keys[p] = key[j];
-            s_vals[p] = val[j];
+            if (v) atomicOr(&mk[d], lanebit);
+            unsigned long long peers = 0;
+            if (v) peers = mk[d];
+            const uint32_t pl = static_cast<uint32_t>(peers), ph = static_cast<uint32_t>(peers >> 32);
+            const uint32_t lt = __builtin_amdgcn_mbcnt_hi(ph, __builtin_amdgcn_mbcnt_lo(pl, 0u));
+            const uint32_t cnt = static_cast<uint32_t>(__popc(pl) + __popc(ph));
+            if (v && lt == 0u) {
+                atomicAdd(&wc[d], cnt);
+                mk[d] = 0ull;
+            }
+            uint32_t after = 0;
+            if (v) after = wc[d];
+            rnk[j] = after - cnt + lt;
         }
-    }
-    __syncthreads();
+        lds_barrier();
+        tot = 0;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) {
+            const uint32_t c = s_wc[w][tid];
+            s_wc[w][tid] = tot;
+            tot += c;
+        }
+        store_relaxed(lb + static_cast<uint64_t>(t) * kRadix + tid, (t == 0 ? kFlagP : kFlagA) | tot);
+        lstart = block_excl_scan_lds(tot, s_wsum);
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) s_wc[w][tid] += lstart;
+        lds_barrier();
+    };
 
-    // ---- write: consecutive LDS slots of one bucket go to consecutive rows ----
-    const uint32_t tn = static_cast<uint32_t>(std::min<uint64_t>(TILE, a.n - tbase));
-    if (!last) {
-        K* kout = static_cast<K*>(a.kbuf[slot & 1]);
-        uint32_t* vout = a.vbuf[slot & 1];
-        for (uint32_t i = tid; i < tn; i += kBlock) {
-            const K k = s_keys[i];
-            const uint32_t p = s_gbase[KO::digit(k, dig)] + i;
-            kout[p] = k;
-            vout[p] = s_vals[i];
+    auto stage = [&](uint32_t t) {
+        const uint64_t wbase = static_cast<uint64_t>(t) * TILE + static_cast<uint64_t>(wave) * 64 * ITEMS + lane;
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) {
+            if (wbase + static_cast<uint64_t>(j) * 64 < a.n) {
+                const uint32_t p = s_wc[wave][KO::digit(key[j], dig)] + rnk[j];
+                s_keys[p] = key[j];
+                s_vals[p] = val[j];
+            }
         }
-    } else {
-        const uint32_t nm1 = static_cast<uint32_t>(a.n - 1);
-        for (uint32_t i = tid; i < tn; i += kBlock) {
-            const K k = s_keys[i];
-            uint32_t p = s_gbase[KO::digit(k, dig)] + i;
-            if (a.reverse) p = nm1 - p;
-            if (a.keys_out) KO::store(a.keys_out, p, k, a.mlo, a.mhi);
-            if (a.perm64)
-                static_cast<uint64_t*>(a.perm_out)[p] = s_vals[i];
-            else
-                static_cast<uint32_t*>(a.perm_out)[p] = s_vals[i];
+        lds_barrier();
+    };
+
+    // decoupled look-back for the staged tile t (8 predecessor words per
+    // step, consumed in tile order until an inclusive prefix), then write it
+    auto resolve_write = [&](uint32_t t, uint32_t tot, uint32_t lstart) {
+        uint32_t excl = 0;
+        if (t > 0) {
+            uint32_t k = t - 1;
+            for (;;) {
+                uint32_t w[8];
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+                    w[q] = (k >= static_cast<uint32_t>(q)) ? load_relaxed(lb + static_cast<uint64_t>(k - q) * kRadix + tid)
+                                                           : kFlagP;
+                uint32_t consumed = 0;
+                bool done = false, stall = false;
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    if (!done && !stall) {
+                        const uint32_t f = w[q] & ~kValMask;
+                        if (f == 0u) {
+                            stall = true;
+                        } else {
+                            excl += w[q] & kValMask;
+                            ++consumed;
+                            done = (f == kFlagP);
+                        }
+                    }
+                }
+                if (done) break;
+                k -= consumed;  // tile 0 always holds P, so k never passes below 0
+                if (stall) __builtin_amdgcn_s_sleep(1);
+            }
+            store_relaxed(lb + static_cast<uint64_t>(t) * kRadix + tid, kFlagP | (excl + tot));
         }
+        s_gbase[tid] = a.plan->base[dig][tid] + excl - lstart;
+        lds_barrier();
+        const uint64_t tbase = static_cast<uint64_t>(t) * TILE;
+        const uint32_t tn = static_cast<uint32_t>(std::min<uint64_t>(TILE, a.n - tbase));
+        if (!last) {
+            K* kout = static_cast<K*>(a.kbuf[slot & 1]);
+            uint32_t* vout = a.vbuf[slot & 1];
+            for (uint32_t i = tid; i < tn; i += kBlock) {
+                const K kk = s_keys[i];
+                const uint32_t p = s_gbase[KO::digit(kk, dig)] + i;
+                kout[p] = kk;
+                vout[p] = s_vals[i];
+            }
+        } else {
+            const uint32_t nm1 = static_cast<uint32_t>(a.n - 1);
+            for (uint32_t i = tid; i < tn; i += kBlock) {
+                const K kk = s_keys[i];
+                uint32_t p = s_gbase[KO::digit(kk, dig)] + i;
+                if (a.reverse) p = nm1 - p;
+                if (a.keys_out) KO::store(a.keys_out, p, kk, a.mlo, a.mhi);
+                if (a.perm64)
+                    static_cast<uint64_t*>(a.perm_out)[p] = s_vals[i];
+                else
+                    static_cast<uint32_t*>(a.perm_out)[p] = s_vals[i];
+            }
+        }
+        lds_barrier();
+    };
+
+    uint32_t t0 = claim();
+    if (t0 >= a.ntiles) return;
+    uint32_t tot0, ls0;
+    load_rank(t0, tot0, ls0);
+    stage(t0);
+    for (;;) {
+        const uint32_t t1 = claim();
+        const bool has1 = t1 < a.ntiles;
+        uint32_t tot1 = 0, ls1 = 0;
+        if (has1) load_rank(t1, tot1, ls1);
+        resolve_write(t0, tot0, ls0);
+        if (!has1) break;
+        stage(t1);
+        t0 = t1;
+        tot0 = tot1;
+        ls0 = ls1;
     }
 }
 
@@ -407,30 +482,20 @@ size_t key_bytes(int key_type)
     }
 }
 
-static int pass_items(int key_type)
-{
-    switch (key_type) {
-    case VDF_KEY_U32:
-    case VDF_KEY_I32: return PassItems<uint32_t>::value;
-    case VDF_KEY_U64:
-    case VDF_KEY_I64: return PassItems<uint64_t>::value;
-    default: return PassItems<K128>::value;
-    }
-}
-
-
 struct SortLayout {
     uint64_t ntiles;
     size_t zero_words;  // hist + tickets
     size_t bytes;
 };
 
-static SortLayout sort_layout(int key_type, uint64_t n)
+// Scratch is sized for the smallest supported tile (8 rows per thread), so it
+// holds every tuning choice.
+static SortLayout sort_layout(int key_type, uint64_t n, int items = 8)
 {
     SortLayout L;
-    const uint64_t tile = static_cast<uint64_t>(kBlock) * pass_items(key_type);
+    const uint64_t tile = static_cast<uint64_t>(kBlock) * items;
     L.ntiles = (n + tile - 1) / tile;
-    L.zero_words = kMaxDigits * kRadix + kMaxDigits;
+    L.zero_words = kMaxDigits * kRadix + kMaxDigits * kTicketStride;
     const size_t kb = key_bytes(key_type);
     size_t b = 0;
     b += align_up(L.zero_words * 4, 256);
@@ -444,14 +509,83 @@ static SortLayout sort_layout(int key_type, uint64_t n)
 
 size_t argsort_scratch(int key_type, uint64_t n, uint32_t) { return sort_layout(key_type, n).bytes; }
 
+using PassFn = void (*)(PassArgs, uint32_t);
+
+// Pass kernel instantiations the tuning table can select.
+template <typename K>
+static PassFn pick_pass(int items);
+
+template <>
+PassFn pick_pass<uint32_t>(int items)
+{
+    if (items == 12) return k_pass<uint32_t, 12>;
+    if (items == 16) return k_pass<uint32_t, 16>;
+    if (items == 24) return k_pass<uint32_t, 24>;
+    return nullptr;
+}
+
+template <>
+PassFn pick_pass<uint64_t>(int items)
+{
+    if (items == 8) return k_pass<uint64_t, 8>;
+    if (items == 12) return k_pass<uint64_t, 12>;
+    if (items == 16) return k_pass<uint64_t, 16>;
+    return nullptr;
+}
+
+template <>
+PassFn pick_pass<K128>(int items)
+{
+    return items == 8 ? k_pass<K128, 8> : nullptr;
+}
+
+bool tune_supported(const char* name, int64_t v)
+{
+    const std::string n(name);
+    if (n == "sort_items_32") return pick_pass<uint32_t>(static_cast<int>(v)) != nullptr;
+    if (n == "sort_items_64") return pick_pass<uint64_t>(static_cast<int>(v)) != nullptr;
+    if (n == "sort_chunk") return v >= 1 && v <= 1024;
+    if (n == "hist_grid") return v >= 64 && v <= 65535;
+    if (n == "gather_rows") return v == 4 || v == 8 || v == 16;
+    if (n == "gather_nt") return v == 0 || v == 1;
+    return false;
+}
+
+// Resident blocks of a pass kernel on the whole device (persistent grid).
+static unsigned resident_blocks(PassFn fn, int device)
+{
+    struct Entry {
+        PassFn fn;
+        int device;
+        unsigned blocks;
+    };
+    static std::mutex mu;
+    static std::vector<Entry> cache;
+    std::lock_guard<std::mutex> lk(mu);
+    for (const Entry& e : cache)
+        if (e.fn == fn && e.device == device) return e.blocks;
+    int per_cu = 0, cus = 0;
+    VDF_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(fn), kBlock, 0));
+    VDF_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    const unsigned blocks = static_cast<unsigned>(std::max(1, per_cu) * std::max(1, cus));
+    cache.push_back(Entry{fn, device, blocks});
+    return blocks;
+}
+
 template <typename K>
 static void launch_sort(Handle* h, const void* d_keys, uint64_t n, uint64_t mlo, uint64_t mhi,
                         uint32_t flags, void* d_perm_out, void* d_keys_out, void* scr, hipStream_t s,
                         int key_type)
 {
-    constexpr int ITEMS = PassItems<K>::value;
     constexpr int D = KeyOps<K>::kDigits;
-    const SortLayout L = sort_layout(key_type, n);
+    int items = sizeof(K) == 4 ? h->tune.sort_items_32 : (sizeof(K) == 8 ? h->tune.sort_items_64 : 8);
+    PassFn pass = pick_pass<K>(items);
+    if (!pass) {  // tuning value without an instantiation for this key width
+        items = sizeof(K) == 4 ? 16 : 8;
+        pass = pick_pass<K>(items);
+    }
+    const SortLayout L = sort_layout(key_type, n);  // scratch carve (smallest tile)
+    const uint64_t ntiles = (n + static_cast<uint64_t>(kBlock) * items - 1) / (static_cast<uint64_t>(kBlock) * items);
     Carve c(scr);
     uint32_t* zero = c.take<uint32_t>(L.zero_words);
     uint32_t* hist = zero;
@@ -472,8 +606,16 @@ static void launch_sort(Handle* h, const void* d_keys, uint64_t n, uint64_t mlo,
     a.n = n;
     a.mlo = mlo;
     a.mhi = mhi;
+    a.ntiles = static_cast<uint32_t>(ntiles);
     a.perm64 = (flags & VDF_SORT_PERM64) ? 1u : 0u;
     a.reverse = (flags & VDF_SORT_DESC_REVERSE) ? 1u : 0u;
+    // persistent grid: every resident block, a multiple of the group count
+    unsigned grid = static_cast<unsigned>(std::min<uint64_t>(resident_blocks(pass, h->device), ntiles));
+    a.groups = std::min(8u, grid);
+    grid = grid / a.groups * a.groups;
+    // enough chunks that every group gets several
+    a.chunk = static_cast<uint32_t>(std::max<uint64_t>(
+        1, std::min<uint64_t>(static_cast<uint64_t>(h->tune.sort_chunk), ntiles / (static_cast<uint64_t>(a.groups) * 8))));
 
     const size_t kb = sizeof(K);
     {
@@ -481,10 +623,11 @@ static void launch_sort(Handle* h, const void* d_keys, uint64_t n, uint64_t mlo,
         fill_u32(zero, L.zero_words, 0u, s);
     }
     {
-        const unsigned grid = static_cast<unsigned>(std::min<uint64_t>(std::max<uint64_t>(L.ntiles, 1), 2048));
+        const unsigned hgrid = static_cast<unsigned>(
+            std::min<uint64_t>(std::max<uint64_t>((n + 1023) / 1024, 1), static_cast<uint64_t>(h->tune.hist_grid)));
         LaunchTimer t(h, s, "radix_histogram", n * kb);
-        hipLaunchKernelGGL(k_histogram<K>, dim3(grid), dim3(kBlock), 0, s, d_keys, n, mlo, mhi, hist, a.lb[0],
-                           L.ntiles * kRadix);
+        hipLaunchKernelGGL(k_histogram<K>, dim3(hgrid), dim3(kBlock), 0, s, d_keys, n, mlo, mhi, hist, a.lb[0],
+                           ntiles * kRadix);
         check_launch("k_histogram");
     }
     uint32_t* passes_host = nullptr;
@@ -505,9 +648,10 @@ static void launch_sort(Handle* h, const void* d_keys, uint64_t n, uint64_t mlo,
                                     n * (kb + 4 + kout + pb_out)};
     for (int slot = 0; slot < D; ++slot) {
         LaunchTimer t(h, s, "radix_pass", 0, slot, passes_host, pass_bytes);
-        hipLaunchKernelGGL((k_pass<K, ITEMS>), dim3(static_cast<unsigned>(L.ntiles)), dim3(kBlock), 0, s, a,
-                           static_cast<uint32_t>(slot));
-        check_launch("k_pass");
+        uint32_t sl = static_cast<uint32_t>(slot);
+        void* args[] = {&a, &sl};
+        VDF_HIP_CHECK(
+            hipLaunchCooperativeKernel(reinterpret_cast<const void*>(pass), dim3(grid), dim3(kBlock), args, 0, s));
     }
 }
 

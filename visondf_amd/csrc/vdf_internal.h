@@ -40,6 +40,7 @@ constexpr int kMaxDigits = 16;         // 128-bit keys
 constexpr uint32_t kFlagA = 1u << 30;  // look-back word: tile aggregate available
 constexpr uint32_t kFlagP = 2u << 30;  // look-back word: inclusive prefix available
 constexpr uint32_t kValMask = (1u << 30) - 1u;
+constexpr uint32_t kTicketStride = 16;   // words of tile-claim counters per pass slot (one per tile group)
 
 // Pass plan written on the device by the plan kernel: which digits need a
 // scatter pass (digits whose histogram puts all rows in one bucket are
@@ -69,8 +70,21 @@ struct ProfTotal {
     uint64_t bytes = 0;
 };
 
+// Tuning table of one handle (set by vdf_tune_set or the autotuner).
+struct Tuning {
+    int sort_items_32 = 16;  // rows per thread of a radix pass tile, 4-byte keys
+    int sort_items_64 = 12;  // same, 8-byte keys
+    int sort_chunk = 64;     // consecutive tiles a tile group (one XCD) claims at a time
+    int hist_grid = 2048;    // blocks of the histogram kernel
+    int gather_rows = 8;     // rows per thread of the gather kernel
+    int gather_nt = 0;       // non-temporal loads of the gathered column
+};
+
+bool tune_supported(const char* name, int64_t value);
+
 struct Handle {
     int device = 0;
+    Tuning tune;
     uint64_t budget = 0;
     std::mutex mu;
     // scratch arena (grown on demand, never shrunk)

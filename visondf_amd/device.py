@@ -161,6 +161,15 @@ class Vdf:
                              perm[:n] if perm is not None else None,
                              [{k: t[:g] for k, t in d.items()} for d in agg_t])
 
+    # ----------------------------------------------------------------- tuning
+    def tune(self, name: str, value: int):
+        N.check(self.lib.vdf_tune_set(self.h, name.encode(), int(value)))
+
+    def tune_get(self, name: str) -> int:
+        v = C.c_int64()
+        N.check(self.lib.vdf_tune_get(self.h, name.encode(), C.byref(v)))
+        return v.value
+
     # -------------------------------------------------------------- profiling
     def profile(self, on: bool = True):
         N.check(self.lib.vdf_profile_enable(self.h, 1 if on else 0))
