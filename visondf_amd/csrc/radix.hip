@@ -143,6 +143,33 @@ void fill_u32(uint32_t* p, size_t count, uint32_t value, hipStream_t s)
 // One read of the keys; per-block LDS histograms of every digit, folded into
 // the global histogram with one atomic per non-empty bucket.  Also clears the
 // first look-back buffer.
+// Adds one key per lane to the block histogram.  Digits flagged in `shared`
+// (wave-uniform) were constant across a sample of the wave's keys; for those
+// the wave first checks whether every lane has the same digit and then adds
+// the count once, instead of 64 atomics on one LDS word.
+template <typename K>
+__device__ inline void hist_add(uint32_t* sh, const K& k, bool valid, uint32_t shared)
+{
+    using KO = KeyOps<K>;
+    const uint64_t vm = __ballot(valid);
+#pragma unroll
+    for (int d = 0; d < KO::kDigits; ++d) {
+        const uint32_t b = KO::digit(k, d);
+        if ((shared >> d) & 1u) {
+            const uint32_t b0 = __builtin_amdgcn_readfirstlane(b);
+            if (__ballot(valid && b == b0) == vm) {
+                if (__lane_id() == static_cast<uint32_t>(__ffsll(static_cast<long long>(vm)) - 1))
+                    atomicAdd(&sh[d * kRadix + b0], static_cast<uint32_t>(__popcll(vm)));
+                continue;
+            }
+        }
+        if (valid) atomicAdd(&sh[d * kRadix + b], 1u);
+    }
+}
+
+// One read of the keys; per-block LDS histograms of every digit, folded into
+// the global histogram with one atomic per non-empty bucket.  Also clears the
+// first look-back buffer.
 template <typename K>
 __global__ __launch_bounds__(kBlock) void k_histogram(const void* keys, uint64_t n, uint64_t mlo,
                                                       uint64_t mhi, uint32_t* hist, uint32_t* lb0,
@@ -154,28 +181,42 @@ __global__ __launch_bounds__(kBlock) void k_histogram(const void* keys, uint64_t
     for (int j = threadIdx.x; j < D * kRadix; j += kBlock) sh[j] = 0;
     __syncthreads();
     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
-    uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
-    for (; i + 3 * stride < n; i += 4 * stride) {
+    // every lane of a wave runs the same number of iterations (ballots below)
+    const uint64_t start = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    const uint64_t wave_start = start - (threadIdx.x & 63);
+    uint32_t shared = 0;  // digits constant over the wave's first 64 keys
+    {
+        const bool v = start < n;
+        const K k = v ? KO::load(keys, start, mlo, mhi) : KO::zero();
+        const uint64_t vm = __ballot(v);
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const uint32_t b = KO::digit(k, d);
+            const uint32_t b0 = __builtin_amdgcn_readfirstlane(b);
+            if (vm != 0 && __ballot(v && b == b0) == vm) shared |= 1u << d;
+        }
+    }
+    const uint32_t lane = threadIdx.x & 63;
+    uint64_t i0 = wave_start;
+    // full rounds: every lane of the wave has 4 valid keys
+    for (; i0 + 63 + 3 * stride < n; i0 += 4 * stride) {
         K k[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) k[u] = KO::load(keys, i + u * stride, mlo, mhi);
+        for (int u = 0; u < 4; ++u) k[u] = KO::load(keys, i0 + lane + u * stride, mlo, mhi);
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
-#pragma unroll
-            for (int d = 0; d < D; ++d) atomicAdd(&sh[d * kRadix + KO::digit(k[u], d)], 1u);
+        for (int u = 0; u < 4; ++u) hist_add<K>(sh, k[u], true, shared);
     }
-    for (; i < n; i += stride) {
-        const K k = KO::load(keys, i, mlo, mhi);
-#pragma unroll
-        for (int d = 0; d < D; ++d) atomicAdd(&sh[d * kRadix + KO::digit(k, d)], 1u);
+    for (; i0 < n; i0 += stride) {
+        const uint64_t i = i0 + lane;
+        const bool v = i < n;
+        hist_add<K>(sh, v ? KO::load(keys, i, mlo, mhi) : KO::zero(), v, shared);
     }
     __syncthreads();
     for (int j = threadIdx.x; j < D * kRadix; j += kBlock) {
         const uint32_t c = sh[j];
         if (c) atomicAdd(&hist[j], c);
     }
-    for (uint64_t w = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; w < lb_words; w += stride)
-        lb0[w] = 0u;
+    for (uint64_t w = start; w < lb_words; w += stride) lb0[w] = 0u;
 }
 
 // Single block: exclusive bucket offsets for every digit and the list of
