@@ -148,12 +148,41 @@ vdf_status vdf_groupby(vdf_handle* h, int key_type, const void* d_keys, uint64_t
                        const vdf_value_col* vals, int nvals, uint32_t agg_mask,
                        vdf_groupby_out* out, void* stream);
 
+/* ---- batches of independent tables (segments) ----
+ * One launch sequence sorts / groups every table of the batch: one histogram
+ * launch, one plan launch and one persistent launch per radix pass cover all
+ * tables; each table gets its own digit plan (digits constant within the table
+ * are skipped for it), its own look-back chain and its own outputs.  A table's
+ * results are identical to a call with that table alone.
+ * No reference counterpart: the reference sorts one frame per call
+ * (sort_by_mt.hpp:10); this is the batched form of vdf_argsort / vdf_groupby. */
+typedef struct vdf_sort_seg {
+    const void* keys;       /* device, n keys */
+    uint64_t n;             /* <= VDF_MAX_ROWS */
+    void* perm_out;         /* device, n row ids (table-local), u32 or u64 (VDF_SORT_PERM64) */
+    void* sorted_keys_out;  /* device, n keys (nullable) */
+} vdf_sort_seg;
+
+vdf_status vdf_argsort_batch(vdf_handle* h, int key_type, const vdf_sort_seg* segs, int nsegs,
+                             uint32_t flags, void* stream);
+
+typedef struct vdf_table {
+    const void* keys;                       /* device, n keys */
+    uint64_t n;                             /* <= VDF_MAX_ROWS */
+    const void* vals[VDF_MAX_VALUE_COLS];   /* device value columns, types given per call */
+} vdf_table;
+
+/* outs[i] receives table i's groups (same layout as vdf_groupby). */
+vdf_status vdf_groupby_batch(vdf_handle* h, int key_type, const vdf_table* tables, int ntables,
+                             const int* val_types, int nvals, uint32_t agg_mask,
+                             vdf_groupby_out* outs, void* stream);
+
 /* ---- tuning table (per handle) ----
  * Knobs (unknown names or unsupported values -> VDF_ERR_INVALID):
  *   "sort_items_32"  rows per thread of a radix pass tile, 4-byte keys: 12, 16, 24
  *   "sort_items_64"  same for 8-byte keys: 8, 12, 16   (16-byte keys use 8)
  *   "sort_chunk"     consecutive tiles one tile group (one XCD) claims at a time: 1 .. 1024
- *   "hist_grid"      blocks of the digit-histogram kernel: 64 .. 65535
+ *   "hist_rows"      rows per digit-histogram work item: 4096 .. 2^24
  *   "gather_rows"    rows per thread of the gather kernel: 4, 8, 16
  *   "gather_nt"      1 = non-temporal loads of the gathered column
  * No reference counterpart: the reference fixes its digit width and thread

@@ -39,6 +39,7 @@ EXPORTS = (
     "vdf_argsort", "vdf_gather", "vdf_groupby",
     "vdf_profile_enable", "vdf_profile_reset", "vdf_profile_read",
     "vdf_tune_set", "vdf_tune_get",
+    "vdf_argsort_batch", "vdf_groupby_batch",
 )
 
 
@@ -59,6 +60,14 @@ class VdfGroupbyOut(C.Structure):
         ("perm", C.c_void_p),
         ("aggs", VdfAggOut * MAX_VALUE_COLS),
     ]
+
+
+class VdfSortSeg(C.Structure):
+    _fields_ = [("keys", C.c_void_p), ("n", C.c_uint64), ("perm_out", C.c_void_p), ("sorted_keys_out", C.c_void_p)]
+
+
+class VdfTable(C.Structure):
+    _fields_ = [("keys", C.c_void_p), ("n", C.c_uint64), ("vals", C.c_void_p * MAX_VALUE_COLS)]
 
 
 class VdfError(RuntimeError):
@@ -104,6 +113,11 @@ def load() -> C.CDLL:
     lib.vdf_profile_reset.argtypes = [vp]
     lib.vdf_profile_read.restype = i32
     lib.vdf_profile_read.argtypes = [vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(u64), C.POINTER(u64)]
+    lib.vdf_argsort_batch.restype = i32
+    lib.vdf_argsort_batch.argtypes = [vp, i32, C.POINTER(VdfSortSeg), i32, u32, vp]
+    lib.vdf_groupby_batch.restype = i32
+    lib.vdf_groupby_batch.argtypes = [vp, i32, C.POINTER(VdfTable), i32, C.POINTER(C.c_int), i32, u32,
+                                      C.POINTER(VdfGroupbyOut), vp]
     lib.vdf_tune_set.restype = i32
     lib.vdf_tune_set.argtypes = [vp, C.c_char_p, C.c_int64]
     lib.vdf_tune_get.restype = i32

@@ -42,6 +42,45 @@ void* Handle::ensure_scratch(size_t bytes)
     return scratch;
 }
 
+void Handle::staging_begin(size_t bytes)
+{
+    const int slot = static_cast<int>(staging_next++ % kStaging);
+    if (staging_pending[slot]) {
+        VDF_HIP_CHECK(hipEventSynchronize(staging_done[slot]));
+        staging_pending[slot] = false;
+    }
+    if (bytes > staging_cap[slot]) {
+        if (staging[slot]) VDF_HIP_CHECK(hipHostFree(staging[slot]));
+        staging[slot] = nullptr;
+        staging_cap[slot] = 0;
+        const size_t want = std::max<size_t>(bytes + bytes / 4, 1 << 16);
+        VDF_HIP_CHECK(hipHostMalloc(&staging[slot], want, hipHostMallocDefault));
+        staging_cap[slot] = want;
+    }
+    staging_slot = slot;
+    staging_off = 0;
+}
+
+void* Handle::staging_take(size_t bytes)
+{
+    if (staging_slot < 0) throw Error{VDF_ERR_HIP, "staging_take outside a call"};
+    staging_off = align_up(staging_off, 256);
+    if (staging_off + bytes > staging_cap[staging_slot]) throw Error{VDF_ERR_HIP, "staging region too small"};
+    void* p = static_cast<char*>(staging[staging_slot]) + staging_off;
+    staging_off += bytes;
+    return p;
+}
+
+void Handle::staging_end(hipStream_t s)
+{
+    if (staging_slot < 0) return;
+    if (!staging_done[staging_slot])
+        VDF_HIP_CHECK(hipEventCreateWithFlags(&staging_done[staging_slot], hipEventDisableTiming));
+    VDF_HIP_CHECK(hipEventRecord(staging_done[staging_slot], s));
+    staging_pending[staging_slot] = true;
+    staging_slot = -1;
+}
+
 hipEvent_t Handle::get_event()
 {
     if (!free_events.empty()) {
@@ -97,6 +136,20 @@ struct CallScope {
     {
         int cur = -1;
         if (hipGetDevice(&cur) == hipSuccess && prev >= 0 && cur != prev) (void)hipSetDevice(prev);
+    }
+};
+
+// ends the call's staging region on every exit path
+struct StagingScope {
+    Handle* h;
+    hipStream_t s;
+    StagingScope(Handle* h_, hipStream_t s_) : h(h_), s(s_) {}
+    ~StagingScope()
+    {
+        try {
+            h->staging_end(s);
+        } catch (...) {
+        }
     }
 };
 
@@ -176,6 +229,10 @@ vdf_status vdf_destroy(vdf_handle* h)
             for (hipEvent_t e : h->free_events) (void)hipEventDestroy(e);
             if (h->scratch) (void)hipFree(h->scratch);
             if (h->passes_ring) (void)hipHostFree(h->passes_ring);
+            for (int i = 0; i < Handle::kStaging; ++i) {
+                if (h->staging[i]) (void)hipHostFree(h->staging[i]);
+                if (h->staging_done[i]) (void)hipEventDestroy(h->staging_done[i]);
+            }
         }
         delete h;
     });
@@ -208,7 +265,10 @@ vdf_status vdf_argsort(vdf_handle* h, int key_type, const void* d_keys, uint64_t
         if (n == 0) return;
         require(d_keys != nullptr && d_perm_out != nullptr, "vdf_argsort: null device pointer");
         CallScope cs(h);
-        void* scr = h->ensure_scratch(argsort_scratch(key_type, n, flags));
+        const uint64_t hr = static_cast<uint64_t>(h->tune.hist_rows);
+        void* scr = h->ensure_scratch(argsort_batch_scratch(key_type, &n, 1, hr));
+        h->staging_begin(argsort_batch_staging(key_type, &n, 1, hr));
+        StagingScope ss(h, static_cast<hipStream_t>(stream));
         argsort_enqueue(h, key_type, d_keys, n, flags, d_perm_out, d_sorted_keys_out, scr,
                         static_cast<hipStream_t>(stream));
     });
@@ -245,8 +305,81 @@ vdf_status vdf_groupby(vdf_handle* h, int key_type, const void* d_keys, uint64_t
         }
         require(n == 0 || d_keys != nullptr, "vdf_groupby: null keys");
         CallScope cs(h);
-        void* scr = h->ensure_scratch(groupby_scratch(key_type, n));
+        const uint64_t hr = static_cast<uint64_t>(h->tune.hist_rows);
+        void* scr = h->ensure_scratch(groupby_batch_scratch(key_type, &n, 1, hr));
+        h->staging_begin(groupby_batch_staging(key_type, &n, 1, hr));
+        StagingScope ss(h, static_cast<hipStream_t>(stream));
         groupby_enqueue(h, key_type, d_keys, n, vals, nvals, agg_mask, out, scr, static_cast<hipStream_t>(stream));
+    });
+}
+
+vdf_status vdf_argsort_batch(vdf_handle* h, int key_type, const vdf_sort_seg* segs, int nsegs, uint32_t flags,
+                             void* stream)
+{
+    return guarded([&] {
+        require(h != nullptr, "vdf_argsort_batch: null handle");
+        require(key_bytes(key_type) != 0, "vdf_argsort_batch: unknown key type");
+        require((flags & ~(VDF_SORT_DESC | VDF_SORT_DESC_REVERSE | VDF_SORT_PERM64)) == 0,
+                "vdf_argsort_batch: unknown flag bits");
+        require(!((flags & VDF_SORT_DESC) && (flags & VDF_SORT_DESC_REVERSE)),
+                "vdf_argsort_batch: VDF_SORT_DESC and VDF_SORT_DESC_REVERSE are exclusive");
+        require(nsegs >= 0 && nsegs <= (1 << 22), "vdf_argsort_batch: nsegs out of range");
+        if (nsegs == 0) return;
+        require(segs != nullptr, "vdf_argsort_batch: null segment list");
+        std::vector<SortSeg> ss(nsegs);
+        std::vector<uint64_t> ns(nsegs);
+        uint64_t total = 0;
+        for (int i = 0; i < nsegs; ++i) {
+            require(segs[i].n <= VDF_MAX_ROWS, "vdf_argsort_batch: a table exceeds VDF_MAX_ROWS (2^30 - 1)");
+            require(segs[i].n == 0 || (segs[i].keys && segs[i].perm_out), "vdf_argsort_batch: null device pointer");
+            ss[i] = SortSeg{segs[i].keys, segs[i].n, segs[i].perm_out, segs[i].sorted_keys_out};
+            ns[i] = segs[i].n;
+            total += segs[i].n;
+        }
+        if (total == 0) return;
+        CallScope cs(h);
+        const uint64_t hr = static_cast<uint64_t>(h->tune.hist_rows);
+        void* scr = h->ensure_scratch(argsort_batch_scratch(key_type, ns.data(), nsegs, hr));
+        h->staging_begin(argsort_batch_staging(key_type, ns.data(), nsegs, hr));
+        StagingScope st(h, static_cast<hipStream_t>(stream));
+        argsort_batch_enqueue(h, key_type, ss.data(), nsegs, flags, scr, static_cast<hipStream_t>(stream));
+    });
+}
+
+vdf_status vdf_groupby_batch(vdf_handle* h, int key_type, const vdf_table* tables, int ntables, const int* val_types,
+                             int nvals, uint32_t agg_mask, vdf_groupby_out* outs, void* stream)
+{
+    return guarded([&] {
+        require(h != nullptr, "vdf_groupby_batch: null handle");
+        require(key_bytes(key_type) != 0, "vdf_groupby_batch: unknown key type");
+        require(ntables >= 0 && ntables <= (1 << 22), "vdf_groupby_batch: ntables out of range");
+        require(nvals >= 0 && nvals <= VDF_MAX_VALUE_COLS, "vdf_groupby_batch: nvals out of range");
+        require((agg_mask & ~VDF_AGG_ALL) == 0, "vdf_groupby_batch: unknown aggregate bits");
+        if (ntables == 0) return;
+        require(tables != nullptr && outs != nullptr, "vdf_groupby_batch: null table or output list");
+        require(nvals == 0 || val_types != nullptr, "vdf_groupby_batch: null value types");
+        for (int c = 0; c < nvals; ++c) require(val_bytes(val_types[c]) != 0, "vdf_groupby_batch: unknown value type");
+        std::vector<GbTable> ts(ntables);
+        std::vector<uint64_t> ns(ntables);
+        for (int i = 0; i < ntables; ++i) {
+            require(tables[i].n <= VDF_MAX_ROWS, "vdf_groupby_batch: a table exceeds VDF_MAX_ROWS (2^30 - 1)");
+            require(outs[i].n_groups != nullptr && outs[i].offsets != nullptr,
+                    "vdf_groupby_batch: n_groups and offsets are required");
+            require(tables[i].n == 0 || tables[i].keys != nullptr, "vdf_groupby_batch: null keys");
+            for (int c = 0; c < nvals; ++c)
+                require(tables[i].n == 0 || tables[i].vals[c] != nullptr, "vdf_groupby_batch: null value column");
+            ts[i].keys = tables[i].keys;
+            ts[i].n = tables[i].n;
+            for (int c = 0; c < VDF_MAX_VALUE_COLS; ++c) ts[i].vals[c] = c < nvals ? tables[i].vals[c] : nullptr;
+            ns[i] = tables[i].n;
+        }
+        CallScope cs(h);
+        const uint64_t hr = static_cast<uint64_t>(h->tune.hist_rows);
+        void* scr = h->ensure_scratch(groupby_batch_scratch(key_type, ns.data(), ntables, hr));
+        h->staging_begin(groupby_batch_staging(key_type, ns.data(), ntables, hr));
+        StagingScope st(h, static_cast<hipStream_t>(stream));
+        groupby_batch_enqueue(h, key_type, ts.data(), ntables, val_types, nvals, agg_mask, outs, scr,
+                              static_cast<hipStream_t>(stream));
     });
 }
 
@@ -255,7 +388,7 @@ static int* tune_slot(Tuning& t, const std::string& n)
     if (n == "sort_items_32") return &t.sort_items_32;
     if (n == "sort_items_64") return &t.sort_items_64;
     if (n == "sort_chunk") return &t.sort_chunk;
-    if (n == "hist_grid") return &t.hist_grid;
+    if (n == "hist_rows") return &t.hist_rows;
     if (n == "gather_rows") return &t.gather_rows;
     if (n == "gather_nt") return &t.gather_nt;
     return nullptr;

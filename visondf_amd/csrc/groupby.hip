@@ -8,10 +8,13 @@
 //   (integers: exact 128-bit sum / count); min / max = first of equal values
 //   in row order (std::min_element over the row-ordered Gather vector).
 //
+// Every group-by is a batch of tables (segments); one table is a batch of one.
 // Pipeline on one stream:
-//   1. radix argsort of the keys (radix.hip) -> perm (row ids in key order,
-//      rows in row order within a key) and the sorted keys
-//   2. k_gb_tile: one 256-thread block per 4096-row tile (dynamic tile ids):
+//   1. radix argsort of every table (radix.hip, one launch sequence for the
+//      batch) -> perm (row ids in key order, rows in row order within a key)
+//      and the sorted keys, concatenated in scratch
+//   2. k_gb_tile: one 256-thread block per 4096-row tile of a table (dynamic
+//      tile ids over the tiles of all tables):
 //        head flags (key != previous key) by wave ballots; head count of the
 //        tile published to a decoupled look-back array -> global group ids;
 //        group keys and offsets written at heads; per value column the
@@ -156,68 +159,77 @@ __device__ inline double i128_to_double(uint64_t lo, uint64_t hi)
     return -u128_to_double(nlo, nhi);
 }
 
-struct GbCol {
-    const void* data;
-    int type;   // VDF_VAL_*
-    int kind;   // KIND_*
-    void* sum;
-    double* mean;
-    void* mn;
-    void* mx;
+// One value column of the batch (types are shared by all tables).
+struct GbColType {
+    int type;  // VDF_VAL_*
+    int kind;  // KIND_*
+};
+
+// One table: its rows in the concatenated scratch and the caller's outputs.
+struct GbSeg {
+    uint64_t n;
+    uint64_t row_off;        // first row of the table's sorted keys in scratch
+    const uint32_t* perm;    // the table's permutation (local row ids)
+    uint32_t gtile_off;      // first group-by tile
+    uint32_t ntiles;         // group-by tiles of the table
+    const void* vals[VDF_MAX_VALUE_COLS];
+    vdf_groupby_out out;
 };
 
 struct GbArgs {
-    const void* skeys;     // sorted keys (caller's key type)
-    const uint32_t* perm;  // row ids in key order
-    uint64_t n;
+    const GbSeg* segs;
+    const uint32_t* tile_seg;  // tile -> table (null: one table)
+    const void* skeys;         // sorted keys of all tables (caller's key type)
     uint32_t ntiles;
-    int ncols;             // >= 1 (a count-only column when no values are aggregated)
-    GbCol cols[VDF_MAX_VALUE_COLS];
-    uint32_t* counts;      // nullable
-    uint32_t* offsets;
-    void* group_keys;      // nullable
-    uint64_t* n_groups;
-    uint32_t* status;      // [ntiles] look-back words for head counts
+    int ncols;                 // >= 1 (a count-only column when no values are aggregated)
+    GbColType cols[VDF_MAX_VALUE_COLS];
+    uint32_t agg_mask;
+    uint32_t* status;          // [ntiles] look-back words for head counts
     uint32_t* ticket;
-    uint32_t* tile_gbase;  // [ntiles] first group id of the tile's heads
-    uint32_t* tile_heads;  // [ntiles] heads in the tile
-    Agg* prefix;           // [ncols][ntiles] rows before the tile's first head (or the whole tile)
-    Agg* suffix;           // [ncols][ntiles] rows from the tile's last head to its end
+    uint32_t* tile_gbase;      // [ntiles] table-local id of the tile's first head's group
+    uint32_t* tile_heads;      // [ntiles] heads in the tile
+    Agg* prefix;               // [ncols][ntiles] rows before the tile's first head (or the whole tile)
+    Agg* suffix;               // [ncols][ntiles] rows from the tile's last head to its end
 };
 
-__device__ inline uint64_t load_val(const GbCol& c, uint64_t row)
+__device__ inline uint64_t load_val(int type, const void* data, uint64_t row)
 {
-    switch (c.type) {
-    case VDF_VAL_I32: return static_cast<uint64_t>(static_cast<int64_t>(static_cast<const int32_t*>(c.data)[row]));
-    case VDF_VAL_U32: return static_cast<uint64_t>(static_cast<const uint32_t*>(c.data)[row]);
-    default: return static_cast<const uint64_t*>(c.data)[row];
+    switch (type) {
+    case VDF_VAL_I32: return static_cast<uint64_t>(static_cast<int64_t>(static_cast<const int32_t*>(data)[row]));
+    case VDF_VAL_U32: return static_cast<uint64_t>(static_cast<const uint32_t*>(data)[row]);
+    default: return static_cast<const uint64_t*>(data)[row];
     }
 }
 
-__device__ inline void write_final(const GbArgs& a, int c, uint32_t gid, const Agg& s)
+__device__ inline void write_final(const GbArgs& a, const GbSeg& sg, int c, uint32_t gid, const Agg& s)
 {
-    const GbCol& col = a.cols[c];
-    if (c == 0 && a.counts) a.counts[gid] = static_cast<uint32_t>(s.cnt);
+    const GbColType col = a.cols[c];
+    if (c == 0 && (a.agg_mask & VDF_AGG_COUNT) && sg.out.counts) sg.out.counts[gid] = static_cast<uint32_t>(s.cnt);
     if (col.kind == KIND_COUNT) return;
+    const vdf_agg_out& o = sg.out.aggs[c];
+    void* osum = (a.agg_mask & VDF_AGG_SUM) ? o.sum : nullptr;
+    double* omean = (a.agg_mask & VDF_AGG_MEAN) ? o.mean : nullptr;
+    void* omin = (a.agg_mask & VDF_AGG_MIN) ? o.min : nullptr;
+    void* omax = (a.agg_mask & VDF_AGG_MAX) ? o.max : nullptr;
     const bool w4 = (col.type == VDF_VAL_I32 || col.type == VDF_VAL_U32);
     if (col.kind == KIND_F64) {
         const double sum = as_d(s.a) + as_d(s.b);
-        if (col.sum) static_cast<double*>(col.sum)[gid] = sum;
-        if (col.mean) col.mean[gid] = sum / static_cast<double>(s.cnt);
+        if (osum) static_cast<double*>(osum)[gid] = sum;
+        if (omean) omean[gid] = sum / static_cast<double>(s.cnt);
     } else {
-        if (col.sum) static_cast<uint64_t*>(col.sum)[gid] = s.a;
-        if (col.mean) {
+        if (osum) static_cast<uint64_t*>(osum)[gid] = s.a;
+        if (omean) {
             const double tot = (col.kind == KIND_SIGNED) ? i128_to_double(s.a, s.b) : u128_to_double(s.a, s.b);
-            col.mean[gid] = tot / static_cast<double>(s.cnt);
+            omean[gid] = tot / static_cast<double>(s.cnt);
         }
     }
-    if (col.mn) {
-        if (w4) static_cast<uint32_t*>(col.mn)[gid] = static_cast<uint32_t>(s.mn);
-        else static_cast<uint64_t*>(col.mn)[gid] = s.mn;
+    if (omin) {
+        if (w4) static_cast<uint32_t*>(omin)[gid] = static_cast<uint32_t>(s.mn);
+        else static_cast<uint64_t*>(omin)[gid] = s.mn;
     }
-    if (col.mx) {
-        if (w4) static_cast<uint32_t*>(col.mx)[gid] = static_cast<uint32_t>(s.mx);
-        else static_cast<uint64_t*>(col.mx)[gid] = s.mx;
+    if (omax) {
+        if (w4) static_cast<uint32_t*>(omax)[gid] = static_cast<uint32_t>(s.mx);
+        else static_cast<uint64_t*>(omax)[gid] = s.mx;
     }
 }
 
@@ -245,16 +257,20 @@ __global__ __launch_bounds__(kBlock) void k_gb_tile(GbArgs a)
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     if (tid == 0) s_tile = atomicAdd(a.ticket, 1u);
     __syncthreads();
-    const uint32_t tile = s_tile;
-    const uint64_t tbase = static_cast<uint64_t>(tile) * kGbTile;
-    const K* keys = static_cast<const K*>(a.skeys);
+    const uint32_t tile = __builtin_amdgcn_readfirstlane(s_tile);
+    const uint32_t sgi = a.tile_seg ? __builtin_amdgcn_readfirstlane(a.tile_seg[tile]) : 0u;
+    const GbSeg& sg = a.segs[sgi];
+    const uint64_t n = sg.n;
+    const uint32_t lt = tile - sg.gtile_off;  // tile index inside the table
+    const uint64_t tbase = static_cast<uint64_t>(lt) * kGbTile;
+    const K* keys = static_cast<const K*>(a.skeys) + sg.row_off;
 
     // ---- head flags, striped rows (row = j*256 + tid), packed by ballot ----
 #pragma unroll
     for (int j = 0; j < kGbRows; ++j) {
         const uint64_t r = tbase + static_cast<uint64_t>(j) * kBlock + tid;
         bool hd = false;
-        if (r < a.n) hd = (r == 0) || !keq(keys[r], keys[r - 1]);
+        if (r < n) hd = (r == 0) || !keq(keys[r], keys[r - 1]);
         const uint64_t m = __ballot(hd);
         if (lane == 0) s_hmask[j * kWaves + wave] = m;
     }
@@ -281,12 +297,12 @@ __global__ __launch_bounds__(kBlock) void k_gb_tile(GbArgs a)
     }();
     const uint32_t H = s_wsum[0] + s_wsum[1] + s_wsum[2] + s_wsum[3];
 
-    // ---- decoupled look-back over the tiles' head counts -> first group id ----
+    // ---- decoupled look-back over the table's tiles' head counts -> first group id ----
     if (tid == 0) {
         uint32_t* my = a.status + tile;
-        store_relaxed(my, (tile == 0 ? kFlagP : kFlagA) | H);
+        store_relaxed(my, (lt == 0 ? kFlagP : kFlagA) | H);
         uint32_t excl = 0;
-        if (tile > 0) {
+        if (lt > 0) {
             uint32_t k = tile - 1;
             for (;;) {
                 const uint32_t w = load_relaxed(a.status + k);
@@ -297,16 +313,16 @@ __global__ __launch_bounds__(kBlock) void k_gb_tile(GbArgs a)
                 }
                 excl += w & kValMask;
                 if (f == kFlagP) break;
-                --k;
+                --k;  // the table's first tile holds P
             }
             store_relaxed(my, kFlagP | (excl + H));
         }
         s_gbase = excl;
         a.tile_gbase[tile] = excl;
         a.tile_heads[tile] = H;
-        if (tile == a.ntiles - 1) {
-            *a.n_groups = static_cast<uint64_t>(excl) + H;
-            a.offsets[excl + H] = static_cast<uint32_t>(a.n);
+        if (lt == sg.ntiles - 1) {
+            *sg.out.n_groups = static_cast<uint64_t>(excl) + H;
+            sg.out.offsets[excl + H] = static_cast<uint32_t>(n);
         }
     }
     __syncthreads();
@@ -320,23 +336,24 @@ __global__ __launch_bounds__(kBlock) void k_gb_tile(GbArgs a)
             const int k = __ffs(bits) - 1;
             bits &= bits - 1;
             const uint64_t r = tbase + lr0 + k;
-            a.offsets[g] = static_cast<uint32_t>(r);
-            if (a.group_keys) static_cast<K*>(a.group_keys)[g] = keys[r];
+            sg.out.offsets[g] = static_cast<uint32_t>(r);
+            if (sg.out.group_keys) static_cast<K*>(sg.out.group_keys)[g] = keys[r];
             ++g;
         }
     }
 
     // ---- per value column: gather, sequential reduce, block segmented scan ----
-    const uint32_t tn = static_cast<uint32_t>(std::min<uint64_t>(kGbTile, a.n - tbase));
+    const uint32_t tn = static_cast<uint32_t>(std::min<uint64_t>(kGbTile, n - tbase));
     for (int c = 0; c < a.ncols; ++c) {
-        const GbCol col = a.cols[c];
+        const GbColType col = a.cols[c];
         const int kind = col.kind;
         if (kind != KIND_COUNT) {
+            const void* vdata = sg.vals[c];
 #pragma unroll
             for (int j = 0; j < kGbRows; ++j) {
                 const int li = j * kBlock + tid;
                 const uint64_t r = tbase + li;
-                if (r < a.n) s_val[pad(li)] = load_val(col, a.perm[r]);
+                if (r < n) s_val[pad(li)] = load_val(col.type, vdata, sg.perm[r]);
             }
         }
         __syncthreads();
@@ -353,7 +370,7 @@ __global__ __launch_bounds__(kBlock) void k_gb_tile(GbArgs a)
                         first = cur;
                         seen = true;
                     } else {
-                        write_final(a, c, g - 1, cur);
+                        write_final(a, sg, c, g - 1, cur);
                     }
                     cur = agg_empty();
                     ++g;
@@ -391,7 +408,7 @@ __global__ __launch_bounds__(kBlock) void k_gb_tile(GbArgs a)
         Agg* suf = a.suffix + static_cast<uint64_t>(c) * a.ntiles;
         if (seen) {
             const Agg seg = agg_join(carry, first, kind);
-            if (carry_f) write_final(a, c, gbase + hx - 1, seg);
+            if (carry_f) write_final(a, sg, c, gbase + hx - 1, seg);
             else pre[tile] = seg;  // the tile's first head: rows before it
         }
         if (tid == kBlock - 1) {
@@ -403,8 +420,8 @@ __global__ __launch_bounds__(kBlock) void k_gb_tile(GbArgs a)
 }
 
 // Joins the group that starts at tile b's last head with the leading
-// partials of the following tiles up to (and including) the next tile that
-// has a head.
+// partials of the table's following tiles up to (and including) the next tile
+// that has a head.
 __global__ __launch_bounds__(kBlock) void k_gb_fixup(GbArgs a)
 {
     __shared__ Agg s_red[kBlock];
@@ -413,28 +430,31 @@ __global__ __launch_bounds__(kBlock) void k_gb_fixup(GbArgs a)
     const int tid = threadIdx.x;
     const uint32_t heads = a.tile_heads[b];
     if (heads == 0) return;
+    const uint32_t sgi = a.tile_seg ? a.tile_seg[b] : 0u;
+    const GbSeg& sg = a.segs[sgi];
+    const uint32_t tend = sg.gtile_off + sg.ntiles;  // one past the table's last tile
     const uint32_t gid = a.tile_gbase[b] + heads - 1;
-    const bool short_chain = (b + 1 >= a.ntiles) || a.tile_heads[b + 1] > 0;
+    const bool short_chain = (b + 1 >= tend) || a.tile_heads[b + 1] > 0;
     for (int c = 0; c < a.ncols; ++c) {
         const int kind = a.cols[c].kind;
         const Agg* pre = a.prefix + static_cast<uint64_t>(c) * a.ntiles;
         Agg acc = a.suffix[static_cast<uint64_t>(c) * a.ntiles + b];
         if (short_chain) {
             if (tid == 0) {
-                if (b + 1 < a.ntiles) acc = agg_join(acc, pre[b + 1], kind);
-                write_final(a, c, gid, acc);
+                if (b + 1 < tend) acc = agg_join(acc, pre[b + 1], kind);
+                write_final(a, sg, c, gid, acc);
             }
             continue;
         }
-        for (uint32_t t0 = b + 1; t0 < a.ntiles; t0 += kBlock) {
+        for (uint32_t t0 = b + 1; t0 < tend; t0 += kBlock) {
             const uint32_t t = t0 + tid;
             if (tid == 0) s_first = kBlock;
             __syncthreads();
-            const bool stop = (t >= a.ntiles) || a.tile_heads[t] > 0;
+            const bool stop = (t >= tend) || a.tile_heads[t] > 0;
             if (stop) atomicMin(&s_first, static_cast<uint32_t>(tid));
             __syncthreads();
             const uint32_t first = s_first;
-            s_red[tid] = (t < a.ntiles && static_cast<uint32_t>(tid) <= first) ? pre[t] : agg_empty();
+            s_red[tid] = (t < tend && static_cast<uint32_t>(tid) <= first) ? pre[t] : agg_empty();
             __syncthreads();
             for (int s = 1; s < kBlock; s <<= 1) {
                 if ((tid & (2 * s - 1)) == 0) s_red[tid] = agg_join(s_red[tid], s_red[tid + s], kind);
@@ -444,14 +464,18 @@ __global__ __launch_bounds__(kBlock) void k_gb_fixup(GbArgs a)
             __syncthreads();
             if (first < static_cast<uint32_t>(kBlock)) break;
         }
-        if (tid == 0) write_final(a, c, gid, acc);
+        if (tid == 0) write_final(a, sg, c, gid, acc);
     }
 }
 
-__global__ void k_gb_empty(uint64_t* n_groups, uint32_t* offsets)
+// Tables without rows: zero groups.
+__global__ void k_gb_empty(const GbSeg* segs, uint32_t nsegs)
 {
-    *n_groups = 0;
-    offsets[0] = 0;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nsegs && segs[i].n == 0) {
+        *segs[i].out.n_groups = 0;
+        segs[i].out.offsets[0] = 0;
+    }
 }
 
 int kind_of(int type)
@@ -465,90 +489,140 @@ int kind_of(int type)
 }
 
 struct GbLayout {
-    uint64_t ntiles;
-    size_t sort_bytes, bytes;
+    uint64_t rows = 0, tiles = 0;
+    size_t sort_bytes = 0, bytes = 0;
 };
 
-GbLayout gb_layout(int key_type, uint64_t n)
+GbLayout gb_layout(int key_type, const uint64_t* n, int ntables, uint64_t hist_rows)
 {
     GbLayout L;
-    L.ntiles = (n + kGbTile - 1) / kGbTile;
-    L.sort_bytes = align_up(argsort_scratch(key_type, n, 0), 256);
+    for (int i = 0; i < ntables; ++i) {
+        L.rows += n[i];
+        L.tiles += (n[i] + kGbTile - 1) / kGbTile;
+    }
+    L.sort_bytes = align_up(argsort_batch_scratch(key_type, n, ntables, hist_rows), 256);
     size_t b = L.sort_bytes;
-    b += align_up(n * key_bytes(key_type), 256);   // sorted keys
-    b += align_up(n * 4, 256);                     // perm when the caller passes none
-    b += align_up((L.ntiles + 1) * 4, 256);        // status + ticket
-    b += 2 * align_up(L.ntiles * 4, 256);          // tile_gbase, tile_heads
-    b += 2 * align_up(VDF_MAX_VALUE_COLS * L.ntiles * sizeof(Agg), 256);
+    b += align_up(L.rows * key_bytes(key_type), 256);            // sorted keys
+    b += align_up(L.rows * 4, 256);                              // perms the caller does not take
+    b += align_up(static_cast<size_t>(ntables) * sizeof(GbSeg), 256);
+    b += align_up(std::max<uint64_t>(L.tiles, 1) * 4, 256);      // tile -> table
+    b += align_up((L.tiles + 1) * 4, 256);                       // status + ticket
+    b += 2 * align_up(std::max<uint64_t>(L.tiles, 1) * 4, 256);  // tile_gbase, tile_heads
+    b += 2 * align_up(VDF_MAX_VALUE_COLS * std::max<uint64_t>(L.tiles, 1) * sizeof(Agg), 256);
     L.bytes = b + 256;
     return L;
 }
 
 }  // namespace
 
-size_t groupby_scratch(int key_type, uint64_t n) { return gb_layout(key_type, n).bytes; }
+size_t groupby_scratch(int key_type, uint64_t n) { return gb_layout(key_type, &n, 1, kDefaultHistRows).bytes; }
 
-void groupby_enqueue(Handle* h, int key_type, const void* d_keys, uint64_t n, const vdf_value_col* vals,
-                     int nvals, uint32_t agg_mask, const vdf_groupby_out* out, void* scr, hipStream_t s)
+size_t groupby_batch_scratch(int key_type, const uint64_t* n, int ntables, uint64_t hist_rows)
 {
-    if (n == 0) {
-        LaunchTimer t(h, s, "groupby_empty", 0);
-        hipLaunchKernelGGL(k_gb_empty, dim3(1), dim3(1), 0, s, out->n_groups, out->offsets);
-        check_launch("k_gb_empty");
-        return;
-    }
-    const GbLayout L = gb_layout(key_type, n);
+    return gb_layout(key_type, n, ntables, hist_rows).bytes;
+}
+
+size_t groupby_batch_staging(int key_type, const uint64_t* n, int ntables, uint64_t hist_rows)
+{
+    const GbLayout L = gb_layout(key_type, n, ntables, hist_rows);
+    return argsort_batch_staging(key_type, n, ntables, hist_rows) + 512 +
+           align_up(static_cast<size_t>(ntables) * sizeof(GbSeg), 256) + std::max<uint64_t>(L.tiles, 1) * 4;
+}
+
+void groupby_batch_enqueue(Handle* h, int key_type, const GbTable* tables, int ntables, const int* val_types,
+                           int nvals, uint32_t agg_mask, const vdf_groupby_out* outs, void* scr, hipStream_t s)
+{
+    std::vector<uint64_t> ns(ntables);
+    for (int i = 0; i < ntables; ++i) ns[i] = tables[i].n;
+    const uint64_t hist_rows = static_cast<uint64_t>(h->tune.hist_rows);
+    const GbLayout L = gb_layout(key_type, ns.data(), ntables, hist_rows);
     const size_t kb = key_bytes(key_type);
     Carve cv(scr);
     void* sort_scr = cv.take<char>(L.sort_bytes);
-    void* skeys = cv.take<char>(n * kb);
-    uint32_t* perm_scr = cv.take<uint32_t>(n);
-    uint32_t* status = cv.take<uint32_t>(L.ntiles + 1);
+    char* skeys = cv.take<char>(L.rows * kb);
+    uint32_t* perm_scr = cv.take<uint32_t>(L.rows);
+    GbSeg* dsegs = cv.take<GbSeg>(ntables);
+    uint32_t* dtile = cv.take<uint32_t>(std::max<uint64_t>(L.tiles, 1));
+    uint32_t* status = cv.take<uint32_t>(L.tiles + 1);
     GbArgs a{};
-    a.tile_gbase = cv.take<uint32_t>(L.ntiles);
-    a.tile_heads = cv.take<uint32_t>(L.ntiles);
-    a.prefix = cv.take<Agg>(VDF_MAX_VALUE_COLS * L.ntiles);
-    a.suffix = cv.take<Agg>(VDF_MAX_VALUE_COLS * L.ntiles);
-    uint32_t* perm = out->perm ? out->perm : perm_scr;
+    a.tile_gbase = cv.take<uint32_t>(std::max<uint64_t>(L.tiles, 1));
+    a.tile_heads = cv.take<uint32_t>(std::max<uint64_t>(L.tiles, 1));
+    a.prefix = cv.take<Agg>(VDF_MAX_VALUE_COLS * std::max<uint64_t>(L.tiles, 1));
+    a.suffix = cv.take<Agg>(VDF_MAX_VALUE_COLS * std::max<uint64_t>(L.tiles, 1));
 
-    argsort_enqueue(h, key_type, d_keys, n, 0, perm, skeys, sort_scr, s);
+    // 1. sort every table: sorted keys into scratch, perms to the caller or scratch
+    std::vector<SortSeg> ss(ntables);
+    std::vector<const uint32_t*> perms(ntables);
+    uint64_t row = 0;
+    for (int i = 0; i < ntables; ++i) {
+        uint32_t* p = outs[i].perm ? outs[i].perm : perm_scr + row;
+        perms[i] = p;
+        ss[i] = SortSeg{tables[i].keys, tables[i].n, p, skeys + row * kb};
+        row += tables[i].n;
+    }
+    argsort_batch_enqueue(h, key_type, ss.data(), ntables, 0, sort_scr, s);
 
+    // 2. table descriptions and the tile map (pinned staging -> device)
+    const size_t seg_bytes = align_up(static_cast<size_t>(ntables) * sizeof(GbSeg), 256);
+    char* stg = static_cast<char*>(h->staging_take(seg_bytes + std::max<uint64_t>(L.tiles, 1) * 4));
+    GbSeg* hs = reinterpret_cast<GbSeg*>(stg);
+    uint32_t* ht = reinterpret_cast<uint32_t*>(stg + seg_bytes);
+    row = 0;
+    uint64_t tiles = 0;
+    for (int i = 0; i < ntables; ++i) {
+        const uint64_t n = tables[i].n;
+        const uint64_t nt = (n + kGbTile - 1) / kGbTile;
+        GbSeg g{};
+        g.n = n;
+        g.row_off = row;
+        g.perm = perms[i];
+        g.gtile_off = static_cast<uint32_t>(tiles);
+        g.ntiles = static_cast<uint32_t>(nt);
+        for (int c = 0; c < nvals; ++c) g.vals[c] = tables[i].vals[c];
+        g.out = outs[i];
+        hs[i] = g;
+        for (uint64_t t = 0; t < nt; ++t) ht[tiles + t] = static_cast<uint32_t>(i);
+        row += n;
+        tiles += nt;
+    }
+    // dsegs and dtile are consecutive carves with the same alignment as the staging layout
+    VDF_HIP_CHECK(hipMemcpyAsync(dsegs, stg, seg_bytes + tiles * 4, hipMemcpyHostToDevice, s));
+    if (static_cast<char*>(static_cast<void*>(dtile)) != reinterpret_cast<char*>(dsegs) + seg_bytes)
+        throw Error{VDF_ERR_HIP, "group-by scratch layout mismatch"};
+
+    {
+        LaunchTimer t(h, s, "groupby_empty", 0);
+        hipLaunchKernelGGL(k_gb_empty, dim3((ntables + 255) / 256), dim3(256), 0, s, dsegs,
+                           static_cast<uint32_t>(ntables));
+        check_launch("k_gb_empty");
+    }
+    if (tiles == 0) return;
+
+    a.segs = dsegs;
+    a.tile_seg = ntables > 1 ? dtile : nullptr;
     a.skeys = skeys;
-    a.perm = perm;
-    a.n = n;
-    a.ntiles = static_cast<uint32_t>(L.ntiles);
-    a.counts = (agg_mask & VDF_AGG_COUNT) ? out->counts : nullptr;
-    a.offsets = out->offsets;
-    a.group_keys = out->group_keys;
-    a.n_groups = out->n_groups;
+    a.ntiles = static_cast<uint32_t>(tiles);
+    a.agg_mask = agg_mask;
     a.status = status;
-    a.ticket = status + L.ntiles;
+    a.ticket = status + tiles;
     uint64_t val_bytes = 0;
     if (nvals == 0) {
         a.ncols = 1;
-        a.cols[0] = GbCol{nullptr, VDF_VAL_U32, KIND_COUNT, nullptr, nullptr, nullptr, nullptr};
+        a.cols[0] = GbColType{VDF_VAL_U32, KIND_COUNT};
     } else {
         a.ncols = nvals;
         for (int c = 0; c < nvals; ++c) {
-            const vdf_agg_out& o = out->aggs[c];
-            GbCol& g = a.cols[c];
-            g.data = vals[c].data;
-            g.type = vals[c].type;
-            g.kind = kind_of(vals[c].type);
-            g.sum = (agg_mask & VDF_AGG_SUM) ? o.sum : nullptr;
-            g.mean = (agg_mask & VDF_AGG_MEAN) ? o.mean : nullptr;
-            g.mn = (agg_mask & VDF_AGG_MIN) ? o.min : nullptr;
-            g.mx = (agg_mask & VDF_AGG_MAX) ? o.max : nullptr;
-            val_bytes += 4 + ((vals[c].type == VDF_VAL_I32 || vals[c].type == VDF_VAL_U32) ? 4 : 8);
+            a.cols[c] = GbColType{val_types[c], kind_of(val_types[c])};
+            val_bytes += 4 + ((val_types[c] == VDF_VAL_I32 || val_types[c] == VDF_VAL_U32) ? 4 : 8);
         }
     }
     {
         LaunchTimer t(h, s, "fill", 0);
-        fill_u32(status, L.ntiles + 1, 0u, s);
+        fill_u32(status, tiles + 1, 0u, s);
     }
     {
         // algorithmic bytes: sorted keys (+ the previous key) + per column row id and value
-        LaunchTimer t(h, s, "groupby_tile", n * (kb + val_bytes));
+        LaunchTimer t(h, s, "groupby_tile", L.rows * (kb + val_bytes));
         switch (kb) {
         case 4: hipLaunchKernelGGL(k_gb_tile<uint32_t>, dim3(a.ntiles), dim3(kBlock), 0, s, a); break;
         case 8: hipLaunchKernelGGL(k_gb_tile<uint64_t>, dim3(a.ntiles), dim3(kBlock), 0, s, a); break;
@@ -561,6 +635,20 @@ void groupby_enqueue(Handle* h, int key_type, const void* d_keys, uint64_t n, co
         hipLaunchKernelGGL(k_gb_fixup, dim3(a.ntiles), dim3(kBlock), 0, s, a);
         check_launch("k_gb_fixup");
     }
+}
+
+void groupby_enqueue(Handle* h, int key_type, const void* d_keys, uint64_t n, const vdf_value_col* vals,
+                     int nvals, uint32_t agg_mask, const vdf_groupby_out* out, void* scr, hipStream_t s)
+{
+    GbTable t{};
+    t.keys = d_keys;
+    t.n = n;
+    int types[VDF_MAX_VALUE_COLS] = {0, 0, 0, 0};
+    for (int c = 0; c < nvals; ++c) {
+        t.vals[c] = vals[c].data;
+        types[c] = vals[c].type;
+    }
+    groupby_batch_enqueue(h, key_type, &t, 1, types, nvals, agg_mask, out, scr, s);
 }
 
 }  // namespace vdf

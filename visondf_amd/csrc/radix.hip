@@ -5,19 +5,19 @@
 // radix_sort_charbuf_flat.inl) and the key transforms of
 // inlines/transform/sort_by/sort_{uintegers,integers,char}.inl.
 //
-// Launch sequence for one sort of n rows (all on one stream, no host sync):
-//   1. k_fill        zero the digit histograms and tile counters
-//   2. k_histogram   one read of the keys -> 256-bucket histogram of every
-//                    8-bit digit (LDS-privatised), clears look-back buffer 0
-//   3. k_plan        exclusive bucket offsets per digit; digits whose
-//                    histogram holds all n rows in one bucket are skipped
-//   4. k_pass x slots  one scatter pass per digit slot; slots beyond the
-//                    plan's pass count exit at once.  Each 256-thread block
-//                    takes a tile id from an atomic ticket, ranks its tile
-//                    stably with wave64 ballots, publishes per-bucket counts
-//                    to a decoupled look-back array (Merrill & Garland 2016;
-//                    Adinets & Merrill 2022 "Onesweep"), stages the tile in
-//                    LDS in bucket order and writes coalesced runs.
+// Every sort is a batch of segments (independent tables; a plain argsort is
+// one segment).  Launch sequence (one stream, no host synchronisation):
+//   1. k_fill        zero the per-segment digit histograms and tile counters
+//   2. k_histogram   work items of up to `hist_rows` rows of one segment: one
+//                    read of the keys -> 256-bucket LDS histogram of every
+//                    8-bit digit, flushed into the segment's histogram; also
+//                    clears look-back buffer 0
+//   3. k_plan        one block per segment: exclusive bucket offsets per
+//                    digit; digits whose histogram holds all the segment's
+//                    rows in one bucket are skipped for that segment
+//   4. k_pass x slots  one scatter pass per digit slot over the tiles of all
+//                    segments (segments with fewer passes skip their tiles);
+//                    persistent, see k_pass.
 //
 // Keys are mapped to an unsigned order-preserving form on first load
 // (u32/u64: identity, i32/i64: sign bit flipped, char[16]: bytes ^ 0x80 read
@@ -140,9 +140,48 @@ void fill_u32(uint32_t* p, size_t count, uint32_t value, hipStream_t s)
     check_launch("k_fill");
 }
 
-// One read of the keys; per-block LDS histograms of every digit, folded into
-// the global histogram with one atomic per non-empty bucket.  Also clears the
-// first look-back buffer.
+// ---------------------------------------------------------------------------
+// device-side batch description
+// ---------------------------------------------------------------------------
+struct SegDesc {
+    const void* keys;   // the segment's keys in the caller's form
+    void* perm_out;     // the segment's permutation (local row ids), u32 or u64
+    void* keys_out;     // the segment's sorted keys (nullable)
+    uint64_t n;         // rows
+    uint64_t row_off;   // first row in the concatenated scratch buffers
+    uint32_t tile_off;  // first pass tile of the segment
+    uint32_t pad;
+};
+
+struct HistItem {
+    uint32_t seg;
+    uint32_t pad;
+    uint64_t begin, end;  // local rows
+};
+
+constexpr int kPlanHdr = 1 + kMaxDigits;  // passes + digit_of[]
+
+struct SortArgs {
+    const SegDesc* segs;
+    const uint32_t* tile_seg;  // pass tile -> segment (null: one segment)
+    const HistItem* items;
+    uint32_t* hist;            // [S][D][256]
+    uint32_t* plan_hdr;        // [S][kPlanHdr]
+    uint32_t* plan_base;       // [S][D][256] segment-local bucket starts
+    void* kbuf[2];             // transformed keys between passes (concatenated)
+    uint32_t* vbuf[2];         // local row ids between passes (concatenated)
+    uint32_t* lb[2];           // look-back words [tiles][256], alternating by slot parity
+    uint32_t* tickets;         // [kMaxDigits][kTicketStride] tile-claim counters
+    uint64_t mlo, mhi;         // key transform masks
+    uint64_t lb_words;         // words of one look-back buffer
+    uint32_t nsegs;
+    uint32_t ntiles;
+    uint32_t groups;           // tile groups (one per XCD when the grid allows)
+    uint32_t chunk;            // consecutive tiles a group claims at a time
+    uint32_t perm64;
+    uint32_t reverse;
+};
+
 // Adds one key per lane to the block histogram.  Digits flagged in `shared`
 // (wave-uniform) were constant across a sample of the wave's keys; for those
 // the wave first checks whether every lane has the same digit and then adds
@@ -167,27 +206,26 @@ __device__ inline void hist_add(uint32_t* sh, const K& k, bool valid, uint32_t s
     }
 }
 
-// One read of the keys; per-block LDS histograms of every digit, folded into
-// the global histogram with one atomic per non-empty bucket.  Also clears the
-// first look-back buffer.
+// One work item per block: rows [begin, end) of one segment, one read of the
+// keys, LDS histogram of every digit, flushed with one atomic per non-empty
+// bucket into the segment's histogram.  Also clears look-back buffer 0.
 template <typename K>
-__global__ __launch_bounds__(kBlock) void k_histogram(const void* keys, uint64_t n, uint64_t mlo,
-                                                      uint64_t mhi, uint32_t* hist, uint32_t* lb0,
-                                                      uint64_t lb_words)
+__global__ __launch_bounds__(kBlock) void k_histogram(SortArgs a)
 {
     using KO = KeyOps<K>;
     constexpr int D = KO::kDigits;
     __shared__ uint32_t sh[D * kRadix];
     for (int j = threadIdx.x; j < D * kRadix; j += kBlock) sh[j] = 0;
     __syncthreads();
-    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
-    // every lane of a wave runs the same number of iterations (ballots below)
-    const uint64_t start = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
-    const uint64_t wave_start = start - (threadIdx.x & 63);
+    const HistItem it = a.items[blockIdx.x];
+    const void* keys = a.segs[it.seg].keys;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t wave0 = it.begin + (threadIdx.x & ~63u);  // first row of this wave
     uint32_t shared = 0;  // digits constant over the wave's first 64 keys
     {
-        const bool v = start < n;
-        const K k = v ? KO::load(keys, start, mlo, mhi) : KO::zero();
+        const uint64_t i = wave0 + lane;
+        const bool v = i < it.end;
+        const K k = v ? KO::load(keys, i, a.mlo, a.mhi) : KO::zero();
         const uint64_t vm = __ballot(v);
 #pragma unroll
         for (int d = 0; d < D; ++d) {
@@ -196,73 +234,63 @@ __global__ __launch_bounds__(kBlock) void k_histogram(const void* keys, uint64_t
             if (vm != 0 && __ballot(v && b == b0) == vm) shared |= 1u << d;
         }
     }
-    const uint32_t lane = threadIdx.x & 63;
-    uint64_t i0 = wave_start;
-    // full rounds: every lane of the wave has 4 valid keys
-    for (; i0 + 63 + 3 * stride < n; i0 += 4 * stride) {
+    // waves step through the item in strides of the block; full rounds first
+    uint64_t i0 = wave0;
+    for (; i0 + 63 + 3 * kBlock < it.end; i0 += 4 * kBlock) {
         K k[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) k[u] = KO::load(keys, i0 + lane + u * stride, mlo, mhi);
+        for (int u = 0; u < 4; ++u) k[u] = KO::load(keys, i0 + lane + u * kBlock, a.mlo, a.mhi);
 #pragma unroll
         for (int u = 0; u < 4; ++u) hist_add<K>(sh, k[u], true, shared);
     }
-    for (; i0 < n; i0 += stride) {
+    for (; i0 < it.end; i0 += kBlock) {
         const uint64_t i = i0 + lane;
-        const bool v = i < n;
-        hist_add<K>(sh, v ? KO::load(keys, i, mlo, mhi) : KO::zero(), v, shared);
+        const bool v = i < it.end;
+        hist_add<K>(sh, v ? KO::load(keys, i, a.mlo, a.mhi) : KO::zero(), v, shared);
     }
     __syncthreads();
+    uint32_t* h = a.hist + static_cast<uint64_t>(it.seg) * D * kRadix;
     for (int j = threadIdx.x; j < D * kRadix; j += kBlock) {
         const uint32_t c = sh[j];
-        if (c) atomicAdd(&hist[j], c);
+        if (c) atomicAdd(&h[j], c);
     }
-    for (uint64_t w = start; w < lb_words; w += stride) lb0[w] = 0u;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
+    for (uint64_t w = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; w < a.lb_words; w += stride)
+        a.lb[0][w] = 0u;
 }
 
-// Single block: exclusive bucket offsets for every digit and the list of
-// digits that need a pass.
-__global__ __launch_bounds__(kBlock) void k_plan(const uint32_t* hist, int ndig, uint32_t n, Plan* plan)
+// One block per segment: exclusive bucket offsets of every digit and the list
+// of digits that need a pass (all-in-one-bucket digits are skipped).
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_plan(SortArgs a)
 {
     __shared__ uint32_t s_wsum[kWaves];
     __shared__ uint32_t s_triv[kMaxDigits];
+    const uint32_t sgi = blockIdx.x;
     const int t = threadIdx.x;
+    const uint64_t n = a.segs[sgi].n;
+    const uint32_t* h = a.hist + static_cast<uint64_t>(sgi) * D * kRadix;
+    uint32_t* base = a.plan_base + static_cast<uint64_t>(sgi) * D * kRadix;
     if (t < kMaxDigits) s_triv[t] = 0;
     __syncthreads();
-    for (int d = 0; d < ndig; ++d) {
-        const uint32_t c = hist[d * kRadix + t];
+    for (int d = 0; d < D; ++d) {
+        const uint32_t c = h[d * kRadix + t];
         if (c == n) s_triv[d] = 1;
-        plan->base[d][t] = block_excl_scan(c, s_wsum);
+        base[d * kRadix + t] = block_excl_scan(c, s_wsum);
     }
     __syncthreads();
     if (t == 0) {
-        uint32_t D = 0;
-        for (int d = 0; d < ndig; ++d)
-            if (!s_triv[d]) plan->digit_of[D++] = static_cast<uint32_t>(d);
-        if (D == 0) {  // every digit constant: one pass still writes perm and keys
-            plan->digit_of[0] = 0;
-            D = 1;
+        uint32_t* hdr = a.plan_hdr + static_cast<uint64_t>(sgi) * kPlanHdr;
+        uint32_t P = 0;
+        for (int d = 0; d < D; ++d)
+            if (!s_triv[d]) hdr[1 + P++] = static_cast<uint32_t>(d);
+        if (P == 0 && n > 0) {  // every digit constant: one pass still writes perm and keys
+            hdr[1] = 0;
+            P = 1;
         }
-        plan->passes = D;
+        hdr[0] = P;
     }
 }
-
-struct PassArgs {
-    const void* keys_in;   // caller's keys (read by the first pass, transformed on load)
-    void* kbuf[2];         // transformed keys between passes
-    void* keys_out;        // caller's sorted-keys output (nullable), written by the last pass
-    uint32_t* vbuf[2];     // row ids between passes
-    void* perm_out;        // caller's permutation, written by the last pass
-    uint32_t* lb[2];       // look-back words, [tiles][256], alternating by slot parity
-    uint32_t* tickets;     // per pass slot: one claim counter per tile group
-    const Plan* plan;
-    uint64_t n;
-    uint64_t mlo, mhi;     // key transform masks
-    uint32_t ntiles;
-    uint32_t groups;       // tile groups (one per XCD when the grid allows)
-    uint32_t chunk;        // consecutive tiles a group claims at a time
-    uint32_t perm64;
-    uint32_t reverse;
-};
 
 // LDS-only block barrier: orders LDS traffic between the block's waves
 // without waiting for the block's outstanding global stores.
@@ -286,30 +314,54 @@ __device__ inline uint32_t block_excl_scan_lds(uint32_t x, uint32_t* s_wsum)
     return pre + inc - x;
 }
 
-// One radix scatter pass, persistent and software-pipelined.
+// What a block knows about one claimed tile.
+struct TileCtx {
+    uint32_t t;         // global pass tile
+    uint32_t t0;        // the segment's first tile
+    uint32_t dig;       // digit of this pass for the segment
+    uint32_t first, last;
+    uint64_t n;         // segment rows
+    uint64_t row_off;   // segment's first row in the scratch buffers
+    uint64_t lrow0;     // first local row of the tile
+    const void* keys;   // segment's caller keys (first pass)
+    void* perm_out;
+    void* keys_out;
+    const uint32_t* base;  // segment's bucket starts for this digit
+};
+
+// One radix scatter pass over the tiles of every segment, persistent and
+// software-pipelined.
+//
+// Tile claiming.  Block b belongs to group g = b % groups; dispatch puts
+// consecutive blocks on consecutive XCDs, so a group is one XCD.  The k-th
+// claim of group g gets tile (k / chunk * groups + g) * chunk + k % chunk:
+// runs of `chunk` consecutive tiles are written by CUs sharing one L2, which
+// merges the partial lines of adjacent bucket runs.  The kernel is launched
+// cooperatively, so every block is resident; each group claims its tiles in
+// increasing order and every claimed tile publishes its aggregate before its
+// block waits, so the smallest unfinished tile always completes: no deadlock.
 //
 // Per tile: load (coalesced, striped rows), stable rank inside each wave,
-// publish the tile's per-bucket counts (flag A), stage the tile in LDS in
-// bucket order.  The block then claims, loads and ranks its next tile and
-// publishes that aggregate before it resolves the staged tile's decoupled
-// look-back (flag P) and writes it out, so look-back waits overlap work.
+// publish the tile's per-bucket counts (flag A; a segment's first tile
+// publishes its inclusive prefix, flag P), stage the tile in LDS in bucket
+// order.  The block then claims, loads and ranks its next tile and publishes
+// that aggregate before it resolves the staged tile's decoupled look-back
+// (Merrill & Garland 2016; onesweep, Adinets & Merrill 2022) and writes it
+// out, so look-back waits overlap work.  Look-back never crosses the start of
+// the tile's segment.
 //
 // Ranking of item j in a wave: every lane ORs its lane bit into the wave's
-// 64-bit mask of its digit; reading the mask back gives the lanes sharing
-// the digit (peers).  The lowest peer adds the group size to the wave's
-// bucket counter and clears the mask; every peer reads the counter back.
-// LDS operations of one wave complete in program order, so each read sees
-// exactly the items before j, and the result is the stable rank.
+// 64-bit mask of its digit; reading the mask back gives the lanes sharing the
+// digit (peers).  The lowest peer adds the group size to the wave's bucket
+// counter and clears the mask; every peer reads the counter back.  LDS
+// operations of one wave complete in program order, so each read sees exactly
+// the items before j, and the result is the stable rank.
 template <typename K, int ITEMS>
-__global__ __launch_bounds__(kBlock) void k_pass(PassArgs a, uint32_t slot)
+__global__ __launch_bounds__(kBlock) void k_pass(SortArgs a, uint32_t slot)
 {
     using KO = KeyOps<K>;
     constexpr int TILE = kBlock * ITEMS;
-    const uint32_t D = a.plan->passes;
-    if (slot >= D) return;
-    const int dig = static_cast<int>(a.plan->digit_of[slot]);
-    const bool first = (slot == 0);
-    const bool last = (slot == D - 1);
+    constexpr int D = KO::kDigits;
 
     __shared__ K s_keys[TILE];
     __shared__ uint32_t s_vals[TILE];
@@ -328,15 +380,6 @@ __global__ __launch_bounds__(kBlock) void k_pass(PassArgs a, uint32_t slot)
     uint32_t val[ITEMS];
     uint32_t rnk[ITEMS];
 
-    // Tile claiming.  Block b belongs to group g = b % groups; dispatch puts
-    // consecutive blocks on consecutive XCDs, so a group is one XCD.  The
-    // k-th claim of group g gets tile (k / chunk * groups + g) * chunk +
-    // k % chunk: runs of `chunk` consecutive tiles are written by CUs sharing
-    // one L2, which merges the partial lines of adjacent bucket runs.  The
-    // kernel is launched cooperatively, so every block is resident; each group
-    // claims its tiles in increasing order and every claimed tile publishes
-    // its aggregate before its block waits, so the smallest unfinished tile
-    // always completes: no deadlock.
     auto claim = [&]() -> uint32_t {
         if (tid == 0) {
             const uint32_t g = blockIdx.x % a.groups;
@@ -345,45 +388,74 @@ __global__ __launch_bounds__(kBlock) void k_pass(PassArgs a, uint32_t slot)
             s_tile = static_cast<uint32_t>(std::min<uint64_t>(t, 0xFFFFFFFFull));
         }
         lds_barrier();
-        const uint32_t t = s_tile;
+        const uint32_t t = __builtin_amdgcn_readfirstlane(s_tile);
         lds_barrier();
         return t;
     };
 
-    // load tile t, rank it, publish its aggregate; returns the bucket total and
+    // claims until a tile whose segment needs this pass; false at the end
+    auto next_tile = [&](TileCtx& c) -> bool {
+        for (;;) {
+            const uint32_t t = claim();
+            if (t >= a.ntiles) return false;
+            const uint32_t sg = a.tile_seg ? __builtin_amdgcn_readfirstlane(a.tile_seg[t]) : 0u;
+            const uint32_t* hdr = a.plan_hdr + static_cast<uint64_t>(sg) * kPlanHdr;
+            const uint32_t P = hdr[0];
+            if (slot >= P) continue;
+            const SegDesc& sd = a.segs[sg];
+            c.t = t;
+            c.t0 = sd.tile_off;
+            c.dig = hdr[1 + slot];
+            c.first = (slot == 0);
+            c.last = (slot == P - 1);
+            c.n = sd.n;
+            c.row_off = sd.row_off;
+            c.lrow0 = static_cast<uint64_t>(t - sd.tile_off) * TILE;
+            c.keys = sd.keys;
+            c.perm_out = sd.perm_out;
+            c.keys_out = sd.keys_out;
+            c.base = a.plan_base + (static_cast<uint64_t>(sg) * D + c.dig) * kRadix;
+            return true;
+        }
+    };
+
+    // load tile c, rank it, publish its aggregate; returns the bucket total and
     // tile-local bucket start of bucket `tid`; s_wc then holds each wave's
     // first LDS slot per bucket
-    auto load_rank = [&](uint32_t t, uint32_t& tot, uint32_t& lstart) {
+    auto load_rank = [&](const TileCtx& c, uint32_t& tot, uint32_t& lstart) {
 #pragma unroll
         for (int w = 0; w < kWaves; ++w) {
             s_wc[w][tid] = 0;
             s_mask[w][tid] = 0;
         }
-        lbn[static_cast<uint64_t>(t) * kRadix + tid] = 0u;  // the next pass's row of this tile
-        const uint64_t wbase = static_cast<uint64_t>(t) * TILE + static_cast<uint64_t>(wave) * 64 * ITEMS + lane;
-        if (first) {
+        lbn[static_cast<uint64_t>(c.t) * kRadix + tid] = 0u;  // the next pass's row of this tile
+        const uint64_t wl = c.lrow0 + static_cast<uint64_t>(wave) * 64 * ITEMS + lane;  // local row of item 0
+        if (c.first) {
 #pragma unroll
             for (int j = 0; j < ITEMS; ++j) {
-                const uint64_t i = wbase + static_cast<uint64_t>(j) * 64;
-                key[j] = (i < a.n) ? KO::load(a.keys_in, i, a.mlo, a.mhi) : KO::zero();
+                const uint64_t i = wl + static_cast<uint64_t>(j) * 64;
+                key[j] = (i < c.n) ? KO::load(c.keys, i, a.mlo, a.mhi) : KO::zero();
                 val[j] = static_cast<uint32_t>(i);
             }
         } else {
+            const K* kp = kin + c.row_off;
+            const uint32_t* vp = vin + c.row_off;
 #pragma unroll
             for (int j = 0; j < ITEMS; ++j) {
-                const uint64_t i = wbase + static_cast<uint64_t>(j) * 64;
-                const bool v = i < a.n;
-                key[j] = v ? kin[i] : KO::zero();
-                val[j] = v ? vin[i] : 0u;
+                const uint64_t i = wl + static_cast<uint64_t>(j) * 64;
+                const bool v = i < c.n;
+                key[j] = v ? kp[i] : KO::zero();
+                val[j] = v ? vp[i] : 0u;
             }
         }
         lds_barrier();
         uint32_t* wc = s_wc[wave];
         unsigned long long* mk = reinterpret_cast<unsigned long long*>(s_mask[wave]);
         const unsigned long long lanebit = 1ull << lane;
+        const int dig = static_cast<int>(c.dig);
 #pragma unroll
         for (int j = 0; j < ITEMS; ++j) {
-            const bool v = (wbase + static_cast<uint64_t>(j) * 64) < a.n;
+            const bool v = (wl + static_cast<uint64_t>(j) * 64) < c.n;
             const uint32_t d = KO::digit(key[j], dig);
             if (v) atomicOr(&mk[d], lanebit);
             unsigned long long peers = 0;
@@ -403,22 +475,23 @@ __global__ __launch_bounds__(kBlock) void k_pass(PassArgs a, uint32_t slot)
         tot = 0;
 #pragma unroll
         for (int w = 0; w < kWaves; ++w) {
-            const uint32_t c = s_wc[w][tid];
+            const uint32_t cw = s_wc[w][tid];
             s_wc[w][tid] = tot;
-            tot += c;
+            tot += cw;
         }
-        store_relaxed(lb + static_cast<uint64_t>(t) * kRadix + tid, (t == 0 ? kFlagP : kFlagA) | tot);
+        store_relaxed(lb + static_cast<uint64_t>(c.t) * kRadix + tid, (c.t == c.t0 ? kFlagP : kFlagA) | tot);
         lstart = block_excl_scan_lds(tot, s_wsum);
 #pragma unroll
         for (int w = 0; w < kWaves; ++w) s_wc[w][tid] += lstart;
         lds_barrier();
     };
 
-    auto stage = [&](uint32_t t) {
-        const uint64_t wbase = static_cast<uint64_t>(t) * TILE + static_cast<uint64_t>(wave) * 64 * ITEMS + lane;
+    auto stage = [&](const TileCtx& c) {
+        const uint64_t wl = c.lrow0 + static_cast<uint64_t>(wave) * 64 * ITEMS + lane;
+        const int dig = static_cast<int>(c.dig);
 #pragma unroll
         for (int j = 0; j < ITEMS; ++j) {
-            if (wbase + static_cast<uint64_t>(j) * 64 < a.n) {
+            if (wl + static_cast<uint64_t>(j) * 64 < c.n) {
                 const uint32_t p = s_wc[wave][KO::digit(key[j], dig)] + rnk[j];
                 s_keys[p] = key[j];
                 s_vals[p] = val[j];
@@ -427,18 +500,19 @@ __global__ __launch_bounds__(kBlock) void k_pass(PassArgs a, uint32_t slot)
         lds_barrier();
     };
 
-    // decoupled look-back for the staged tile t (8 predecessor words per
-    // step, consumed in tile order until an inclusive prefix), then write it
-    auto resolve_write = [&](uint32_t t, uint32_t tot, uint32_t lstart) {
+    // decoupled look-back for the staged tile (8 predecessor words per step,
+    // consumed in tile order until an inclusive prefix), then write it out
+    auto resolve_write = [&](const TileCtx& c, uint32_t tot, uint32_t lstart) {
         uint32_t excl = 0;
-        if (t > 0) {
-            uint32_t k = t - 1;
+        if (c.t > c.t0) {
+            uint32_t k = c.t - 1;
             for (;;) {
                 uint32_t w[8];
 #pragma unroll
                 for (int q = 0; q < 8; ++q)
-                    w[q] = (k >= static_cast<uint32_t>(q)) ? load_relaxed(lb + static_cast<uint64_t>(k - q) * kRadix + tid)
-                                                           : kFlagP;
+                    w[q] = (k >= c.t0 + static_cast<uint32_t>(q))
+                               ? load_relaxed(lb + static_cast<uint64_t>(k - q) * kRadix + tid)
+                               : kFlagP;
                 uint32_t consumed = 0;
                 bool done = false, stall = false;
 #pragma unroll
@@ -455,18 +529,18 @@ __global__ __launch_bounds__(kBlock) void k_pass(PassArgs a, uint32_t slot)
                     }
                 }
                 if (done) break;
-                k -= consumed;  // tile 0 always holds P, so k never passes below 0
+                k -= consumed;  // the segment's first tile holds P, so k stays >= t0
                 if (stall) __builtin_amdgcn_s_sleep(1);
             }
-            store_relaxed(lb + static_cast<uint64_t>(t) * kRadix + tid, kFlagP | (excl + tot));
+            store_relaxed(lb + static_cast<uint64_t>(c.t) * kRadix + tid, kFlagP | (excl + tot));
         }
-        s_gbase[tid] = a.plan->base[dig][tid] + excl - lstart;
+        s_gbase[tid] = c.base[tid] + excl - lstart;
         lds_barrier();
-        const uint64_t tbase = static_cast<uint64_t>(t) * TILE;
-        const uint32_t tn = static_cast<uint32_t>(std::min<uint64_t>(TILE, a.n - tbase));
-        if (!last) {
-            K* kout = static_cast<K*>(a.kbuf[slot & 1]);
-            uint32_t* vout = a.vbuf[slot & 1];
+        const uint32_t tn = static_cast<uint32_t>(std::min<uint64_t>(TILE, c.n - c.lrow0));
+        const int dig = static_cast<int>(c.dig);
+        if (!c.last) {
+            K* kout = static_cast<K*>(a.kbuf[slot & 1]) + c.row_off;
+            uint32_t* vout = a.vbuf[slot & 1] + c.row_off;
             for (uint32_t i = tid; i < tn; i += kBlock) {
                 const K kk = s_keys[i];
                 const uint32_t p = s_gbase[KO::digit(kk, dig)] + i;
@@ -474,35 +548,34 @@ __global__ __launch_bounds__(kBlock) void k_pass(PassArgs a, uint32_t slot)
                 vout[p] = s_vals[i];
             }
         } else {
-            const uint32_t nm1 = static_cast<uint32_t>(a.n - 1);
+            const uint32_t nm1 = static_cast<uint32_t>(c.n - 1);
             for (uint32_t i = tid; i < tn; i += kBlock) {
                 const K kk = s_keys[i];
                 uint32_t p = s_gbase[KO::digit(kk, dig)] + i;
                 if (a.reverse) p = nm1 - p;
-                if (a.keys_out) KO::store(a.keys_out, p, kk, a.mlo, a.mhi);
+                if (c.keys_out) KO::store(c.keys_out, p, kk, a.mlo, a.mhi);
                 if (a.perm64)
-                    static_cast<uint64_t*>(a.perm_out)[p] = s_vals[i];
+                    static_cast<uint64_t*>(c.perm_out)[p] = s_vals[i];
                 else
-                    static_cast<uint32_t*>(a.perm_out)[p] = s_vals[i];
+                    static_cast<uint32_t*>(c.perm_out)[p] = s_vals[i];
             }
         }
         lds_barrier();
     };
 
-    uint32_t t0 = claim();
-    if (t0 >= a.ntiles) return;
+    TileCtx c0, c1;
+    if (!next_tile(c0)) return;
     uint32_t tot0, ls0;
-    load_rank(t0, tot0, ls0);
-    stage(t0);
+    load_rank(c0, tot0, ls0);
+    stage(c0);
     for (;;) {
-        const uint32_t t1 = claim();
-        const bool has1 = t1 < a.ntiles;
+        const bool has1 = next_tile(c1);
         uint32_t tot1 = 0, ls1 = 0;
-        if (has1) load_rank(t1, tot1, ls1);
-        resolve_write(t0, tot0, ls0);
+        if (has1) load_rank(c1, tot1, ls1);
+        resolve_write(c0, tot0, ls0);
         if (!has1) break;
-        stage(t1);
-        t0 = t1;
+        stage(c1);
+        c0 = c1;
         tot0 = tot1;
         ls0 = ls1;
     }
@@ -523,34 +596,72 @@ size_t key_bytes(int key_type)
     }
 }
 
+static int key_digits(int key_type) { return static_cast<int>(key_bytes(key_type)); }
+
+constexpr uint64_t kMinTileRows = static_cast<uint64_t>(kBlock) * 8;  // smallest pass tile
+
+// Scratch of a sort batch.  Tile-indexed arrays are sized for the smallest
+// pass tile, so they hold every tuning choice.
 struct SortLayout {
-    uint64_t ntiles;
-    size_t zero_words;  // hist + tickets
-    size_t bytes;
+    uint64_t rows = 0, tiles = 0, items = 0;
+    size_t zero_words = 0;
+    size_t bytes = 0;
 };
 
-// Scratch is sized for the smallest supported tile (8 rows per thread), so it
-// holds every tuning choice.
-static SortLayout sort_layout(int key_type, uint64_t n, int items = 8)
+static uint64_t hist_items(uint64_t n, uint64_t rows_per_item)
+{
+    return n == 0 ? 0 : (n + rows_per_item - 1) / rows_per_item;
+}
+
+static SortLayout sort_layout(int key_type, const uint64_t* n, int nsegs, uint64_t hist_rows)
 {
     SortLayout L;
-    const uint64_t tile = static_cast<uint64_t>(kBlock) * items;
-    L.ntiles = (n + tile - 1) / tile;
-    L.zero_words = kMaxDigits * kRadix + kMaxDigits * kTicketStride;
+    for (int s = 0; s < nsegs; ++s) {
+        L.rows += n[s];
+        L.tiles += (n[s] + kMinTileRows - 1) / kMinTileRows;
+        L.items += hist_items(n[s], hist_rows);
+    }
+    const int D = key_digits(key_type);
     const size_t kb = key_bytes(key_type);
+    L.zero_words = kMaxDigits * kTicketStride + static_cast<size_t>(nsegs) * D * kRadix;
     size_t b = 0;
     b += align_up(L.zero_words * 4, 256);
-    b += align_up(sizeof(Plan), 256);
-    b += 2 * align_up(L.ntiles * kRadix * 4, 256);
-    b += 2 * align_up(n * kb, 256);
-    b += 2 * align_up(n * 4, 256);
+    b += align_up(static_cast<size_t>(nsegs) * kPlanHdr * 4, 256);
+    b += align_up(static_cast<size_t>(nsegs) * D * kRadix * 4, 256);
+    b += align_up(static_cast<size_t>(nsegs) * sizeof(SegDesc), 256);
+    b += align_up(std::max<uint64_t>(L.items, 1) * sizeof(HistItem), 256);
+    b += align_up(std::max<uint64_t>(L.tiles, 1) * 4, 256);
+    b += 2 * align_up(std::max<uint64_t>(L.tiles, 1) * kRadix * 4, 256);
+    b += 2 * align_up(L.rows * kb, 256);
+    b += 2 * align_up(L.rows * 4, 256);
     L.bytes = b + 256;
     return L;
 }
 
-size_t argsort_scratch(int key_type, uint64_t n, uint32_t) { return sort_layout(key_type, n).bytes; }
+// host staging bytes of a sort batch (descriptors, work items, tile map)
+static size_t staging_bytes(const SortLayout& L, int nsegs)
+{
+    return align_up(static_cast<size_t>(nsegs) * sizeof(SegDesc), 256) +
+           align_up(std::max<uint64_t>(L.items, 1) * sizeof(HistItem), 256) +
+           align_up(std::max<uint64_t>(L.tiles, 1) * 4, 256);
+}
 
-using PassFn = void (*)(PassArgs, uint32_t);
+size_t argsort_scratch(int key_type, uint64_t n, uint32_t)
+{
+    return sort_layout(key_type, &n, 1, kDefaultHistRows).bytes;
+}
+
+size_t argsort_batch_scratch(int key_type, const uint64_t* n, int nsegs, uint64_t hist_rows)
+{
+    return sort_layout(key_type, n, nsegs, hist_rows).bytes;
+}
+
+size_t argsort_batch_staging(int key_type, const uint64_t* n, int nsegs, uint64_t hist_rows)
+{
+    return staging_bytes(sort_layout(key_type, n, nsegs, hist_rows), nsegs);
+}
+
+using PassFn = void (*)(SortArgs, uint32_t);
 
 // Pass kernel instantiations the tuning table can select.
 template <typename K>
@@ -586,7 +697,7 @@ bool tune_supported(const char* name, int64_t v)
     if (n == "sort_items_32") return pick_pass<uint32_t>(static_cast<int>(v)) != nullptr;
     if (n == "sort_items_64") return pick_pass<uint64_t>(static_cast<int>(v)) != nullptr;
     if (n == "sort_chunk") return v >= 1 && v <= 1024;
-    if (n == "hist_grid") return v >= 64 && v <= 65535;
+    if (n == "hist_rows") return v >= 4096 && v <= (1 << 24);
     if (n == "gather_rows") return v == 4 || v == 8 || v == 16;
     if (n == "gather_nt") return v == 0 || v == 1;
     return false;
@@ -614,9 +725,8 @@ static unsigned resident_blocks(PassFn fn, int device)
 }
 
 template <typename K>
-static void launch_sort(Handle* h, const void* d_keys, uint64_t n, uint64_t mlo, uint64_t mhi,
-                        uint32_t flags, void* d_perm_out, void* d_keys_out, void* scr, hipStream_t s,
-                        int key_type)
+static void launch_sort(Handle* h, const SortSeg* segs, int nsegs, uint64_t mlo, uint64_t mhi, uint32_t flags,
+                        void* scr, hipStream_t s)
 {
     constexpr int D = KeyOps<K>::kDigits;
     int items = sizeof(K) == 4 ? h->tune.sort_items_32 : (sizeof(K) == 8 ? h->tune.sort_items_64 : 8);
@@ -625,28 +735,58 @@ static void launch_sort(Handle* h, const void* d_keys, uint64_t n, uint64_t mlo,
         items = sizeof(K) == 4 ? 16 : 8;
         pass = pick_pass<K>(items);
     }
-    const SortLayout L = sort_layout(key_type, n);  // scratch carve (smallest tile)
-    const uint64_t ntiles = (n + static_cast<uint64_t>(kBlock) * items - 1) / (static_cast<uint64_t>(kBlock) * items);
+    const uint64_t tile_rows = static_cast<uint64_t>(kBlock) * items;
+    const uint64_t hist_rows = static_cast<uint64_t>(h->tune.hist_rows);
+    std::vector<uint64_t> ns(nsegs);
+    for (int i = 0; i < nsegs; ++i) ns[i] = segs[i].n;
+    const SortLayout L = sort_layout(sizeof(K) == 4 ? VDF_KEY_U32 : (sizeof(K) == 8 ? VDF_KEY_U64 : VDF_KEY_CHAR16),
+                                     ns.data(), nsegs, hist_rows);
+
+    // ---- host-built batch description, uploaded through pinned staging ----
+    char* stg = static_cast<char*>(h->staging_take(staging_bytes(L, nsegs)));
+    Carve hc(stg);
+    SegDesc* hs = hc.take<SegDesc>(nsegs);
+    HistItem* hi = hc.take<HistItem>(std::max<uint64_t>(L.items, 1));
+    uint32_t* ht = hc.take<uint32_t>(std::max<uint64_t>(L.tiles, 1));
+    uint64_t row = 0, tiles = 0, nitems = 0;
+    for (int i = 0; i < nsegs; ++i) {
+        const uint64_t n = segs[i].n;
+        const uint64_t nt = (n + tile_rows - 1) / tile_rows;
+        hs[i] = SegDesc{segs[i].keys, segs[i].perm_out, segs[i].keys_out, n, row, static_cast<uint32_t>(tiles), 0};
+        for (uint64_t t = 0; t < nt; ++t) ht[tiles + t] = static_cast<uint32_t>(i);
+        for (uint64_t b = 0; b < n; b += hist_rows) hi[nitems++] = HistItem{static_cast<uint32_t>(i), 0, b, std::min(n, b + hist_rows)};
+        row += n;
+        tiles += nt;
+    }
+    const uint64_t ntiles = tiles;
+    if (ntiles == 0) return;  // every segment empty
+
     Carve c(scr);
     uint32_t* zero = c.take<uint32_t>(L.zero_words);
-    uint32_t* hist = zero;
-    uint32_t* tickets = zero + kMaxDigits * kRadix;
-    Plan* plan = c.take<Plan>(1);
-    PassArgs a{};
-    a.lb[0] = c.take<uint32_t>(L.ntiles * kRadix);
-    a.lb[1] = c.take<uint32_t>(L.ntiles * kRadix);
-    a.kbuf[0] = c.take<char>(n * sizeof(K));
-    a.kbuf[1] = c.take<char>(n * sizeof(K));
-    a.vbuf[0] = c.take<uint32_t>(n);
-    a.vbuf[1] = c.take<uint32_t>(n);
-    a.keys_in = d_keys;
-    a.keys_out = d_keys_out;
-    a.perm_out = d_perm_out;
-    a.tickets = tickets;
-    a.plan = plan;
-    a.n = n;
+    SortArgs a{};
+    a.tickets = zero;
+    a.hist = zero + kMaxDigits * kTicketStride;
+    a.plan_hdr = c.take<uint32_t>(static_cast<size_t>(nsegs) * kPlanHdr);
+    a.plan_base = c.take<uint32_t>(static_cast<size_t>(nsegs) * D * kRadix);
+    SegDesc* dsegs = c.take<SegDesc>(nsegs);
+    HistItem* ditems = c.take<HistItem>(std::max<uint64_t>(L.items, 1));
+    uint32_t* dtiles = c.take<uint32_t>(std::max<uint64_t>(L.tiles, 1));
+    a.lb[0] = c.take<uint32_t>(std::max<uint64_t>(L.tiles, 1) * kRadix);
+    a.lb[1] = c.take<uint32_t>(std::max<uint64_t>(L.tiles, 1) * kRadix);
+    a.kbuf[0] = c.take<char>(L.rows * sizeof(K));
+    a.kbuf[1] = c.take<char>(L.rows * sizeof(K));
+    a.vbuf[0] = c.take<uint32_t>(L.rows);
+    a.vbuf[1] = c.take<uint32_t>(L.rows);
+    const size_t up = reinterpret_cast<char*>(ht + ntiles) - stg;
+    VDF_HIP_CHECK(hipMemcpyAsync(dsegs, stg, up, hipMemcpyHostToDevice, s));  // descs, items, tile map
+    // the host carve mirrors the device carve (same order, same alignment)
+    a.segs = dsegs;
+    a.items = ditems;
+    a.tile_seg = nsegs > 1 ? dtiles : nullptr;
     a.mlo = mlo;
     a.mhi = mhi;
+    a.lb_words = ntiles * kRadix;
+    a.nsegs = static_cast<uint32_t>(nsegs);
     a.ntiles = static_cast<uint32_t>(ntiles);
     a.perm64 = (flags & VDF_SORT_PERM64) ? 1u : 0u;
     a.reverse = (flags & VDF_SORT_DESC_REVERSE) ? 1u : 0u;
@@ -664,31 +804,34 @@ static void launch_sort(Handle* h, const void* d_keys, uint64_t n, uint64_t mlo,
         fill_u32(zero, L.zero_words, 0u, s);
     }
     {
-        const unsigned hgrid = static_cast<unsigned>(
-            std::min<uint64_t>(std::max<uint64_t>((n + 1023) / 1024, 1), static_cast<uint64_t>(h->tune.hist_grid)));
-        LaunchTimer t(h, s, "radix_histogram", n * kb);
-        hipLaunchKernelGGL(k_histogram<K>, dim3(hgrid), dim3(kBlock), 0, s, d_keys, n, mlo, mhi, hist, a.lb[0],
-                           ntiles * kRadix);
+        LaunchTimer t(h, s, "radix_histogram", L.rows * kb);
+        hipLaunchKernelGGL(k_histogram<K>, dim3(static_cast<unsigned>(nitems)), dim3(kBlock), 0, s, a);
         check_launch("k_histogram");
     }
-    uint32_t* passes_host = nullptr;
     {
         LaunchTimer t(h, s, "radix_plan", 0);
-        hipLaunchKernelGGL(k_plan, dim3(1), dim3(kBlock), 0, s, hist, D, static_cast<uint32_t>(n), plan);
+        hipLaunchKernelGGL(k_plan<D>, dim3(static_cast<unsigned>(nsegs)), dim3(kBlock), 0, s, a);
         check_launch("k_plan");
     }
-    if (h->prof) {
+    // Active passes per segment are decided on the device; for timing, one
+    // segment's pass count is copied back so inactive launches can be told
+    // apart (bytes of batched launches are counted as if every segment ran
+    // the slot).
+    uint32_t* passes_host = nullptr;
+    if (h->prof && nsegs == 1) {
         passes_host = h->passes_ring + (h->ring_pos++ % Handle::kRing);
-        VDF_HIP_CHECK(hipMemcpyAsync(passes_host, &plan->passes, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        VDF_HIP_CHECK(hipMemcpyAsync(passes_host, a.plan_hdr, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     }
     // algorithmic bytes of a slot when it is the only / first / middle / last pass:
     // keys read + keys written + row ids read + row ids written
+    const uint64_t n = L.rows;
     const uint64_t pb_out = a.perm64 ? 8 : 4;
-    const uint64_t kout = d_keys_out ? kb : 0;
+    const uint64_t kout = segs[0].keys_out ? kb : 0;
     const uint64_t pass_bytes[4] = {n * (kb + kout + pb_out), n * (kb + kb + 4), n * (2 * kb + 8),
                                     n * (kb + 4 + kout + pb_out)};
     for (int slot = 0; slot < D; ++slot) {
-        LaunchTimer t(h, s, "radix_pass", 0, slot, passes_host, pass_bytes);
+        LaunchTimer t(h, s, "radix_pass", passes_host ? 0 : pass_bytes[2], passes_host ? slot : -1, passes_host,
+                      pass_bytes);
         uint32_t sl = static_cast<uint32_t>(slot);
         void* args[] = {&a, &sl};
         VDF_HIP_CHECK(
@@ -696,30 +839,44 @@ static void launch_sort(Handle* h, const void* d_keys, uint64_t n, uint64_t mlo,
     }
 }
 
-void argsort_enqueue(Handle* h, int key_type, const void* d_keys, uint64_t n, uint32_t flags,
-                     void* d_perm_out, void* d_keys_out, void* scr, hipStream_t s)
+static void masks_for(int key_type, uint32_t flags, uint64_t& mlo, uint64_t& mhi)
 {
     const bool desc = (flags & VDF_SORT_DESC) != 0;
     switch (key_type) {
     case VDF_KEY_U32:
-    case VDF_KEY_I32: {
-        uint64_t m = (key_type == VDF_KEY_I32 ? 0x80000000ull : 0ull) ^ (desc ? 0xFFFFFFFFull : 0ull);
-        launch_sort<uint32_t>(h, d_keys, n, m, 0, flags, d_perm_out, d_keys_out, scr, s, key_type);
+    case VDF_KEY_I32:
+        mlo = (key_type == VDF_KEY_I32 ? 0x80000000ull : 0ull) ^ (desc ? 0xFFFFFFFFull : 0ull);
+        mhi = 0;
         break;
-    }
     case VDF_KEY_U64:
-    case VDF_KEY_I64: {
-        uint64_t m = (key_type == VDF_KEY_I64 ? 0x8000000000000000ull : 0ull) ^ (desc ? ~0ull : 0ull);
-        launch_sort<uint64_t>(h, d_keys, n, m, 0, flags, d_perm_out, d_keys_out, scr, s, key_type);
+    case VDF_KEY_I64:
+        mlo = (key_type == VDF_KEY_I64 ? 0x8000000000000000ull : 0ull) ^ (desc ? ~0ull : 0ull);
+        mhi = 0;
         break;
-    }
-    case VDF_KEY_CHAR16: {
-        uint64_t m = 0x8080808080808080ull ^ (desc ? ~0ull : 0ull);
-        launch_sort<K128>(h, d_keys, n, m, m, flags, d_perm_out, d_keys_out, scr, s, key_type);
+    case VDF_KEY_CHAR16:
+        mlo = mhi = 0x8080808080808080ull ^ (desc ? ~0ull : 0ull);
         break;
-    }
     default: throw Error{VDF_ERR_INVALID, "unknown key type"};
     }
+}
+
+void argsort_batch_enqueue(Handle* h, int key_type, const SortSeg* segs, int nsegs, uint32_t flags, void* scr,
+                           hipStream_t s)
+{
+    uint64_t mlo = 0, mhi = 0;
+    masks_for(key_type, flags, mlo, mhi);
+    switch (key_bytes(key_type)) {
+    case 4: launch_sort<uint32_t>(h, segs, nsegs, mlo, mhi, flags, scr, s); break;
+    case 8: launch_sort<uint64_t>(h, segs, nsegs, mlo, mhi, flags, scr, s); break;
+    default: launch_sort<K128>(h, segs, nsegs, mlo, mhi, flags, scr, s); break;
+    }
+}
+
+void argsort_enqueue(Handle* h, int key_type, const void* d_keys, uint64_t n, uint32_t flags, void* d_perm_out,
+                     void* d_keys_out, void* scr, hipStream_t s)
+{
+    const SortSeg seg{d_keys, n, d_perm_out, d_keys_out};
+    argsort_batch_enqueue(h, key_type, &seg, 1, flags, scr, s);
 }
 
 }  // namespace vdf

@@ -42,13 +42,14 @@ constexpr uint32_t kFlagP = 2u << 30;  // look-back word: inclusive prefix avail
 constexpr uint32_t kValMask = (1u << 30) - 1u;
 constexpr uint32_t kTicketStride = 16;   // words of tile-claim counters per pass slot (one per tile group)
 
-// Pass plan written on the device by the plan kernel: which digits need a
-// scatter pass (digits whose histogram puts all rows in one bucket are
-// skipped) and each digit's exclusive bucket offsets.
-struct Plan {
-    uint32_t passes;                     // D >= 1
-    uint32_t digit_of[kMaxDigits];       // pass slot -> digit index
-    uint32_t base[kMaxDigits][kRadix];   // digit -> bucket -> first output row
+constexpr uint64_t kDefaultHistRows = 65536;  // rows per histogram work item
+
+// One segment (table) of a sort batch: caller's keys and outputs.
+struct SortSeg {
+    const void* keys;
+    uint64_t n;
+    void* perm_out;   // local row ids, u32 or (VDF_SORT_PERM64) u64
+    void* keys_out;   // nullable
 };
 
 // ---------------------------------------------------------------------------
@@ -75,7 +76,7 @@ struct Tuning {
     int sort_items_32 = 16;  // rows per thread of a radix pass tile, 4-byte keys
     int sort_items_64 = 12;  // same, 8-byte keys
     int sort_chunk = 64;     // consecutive tiles a tile group (one XCD) claims at a time
-    int hist_grid = 2048;    // blocks of the histogram kernel
+    int hist_rows = 65536;   // rows per histogram work item
     int gather_rows = 8;     // rows per thread of the gather kernel
     int gather_nt = 0;       // non-temporal loads of the gathered column
 };
@@ -99,7 +100,21 @@ struct Handle {
     size_t ring_pos = 0;
     static constexpr size_t kRing = 4096;
 
+    // pinned host staging for batch descriptions: a ring of buffers, one per
+    // call; a buffer is reused once the upload of its previous call finished
+    static constexpr int kStaging = 4;
+    void* staging[kStaging] = {};
+    size_t staging_cap[kStaging] = {};
+    hipEvent_t staging_done[kStaging] = {};
+    bool staging_pending[kStaging] = {};
+    int staging_slot = -1;
+    size_t staging_off = 0;
+    unsigned staging_next = 0;
+
     void* ensure_scratch(size_t bytes);
+    void staging_begin(size_t bytes);      // start a call's staging region
+    void* staging_take(size_t bytes);      // carve from it (256-byte aligned)
+    void staging_end(hipStream_t s);       // the region's uploads are enqueued on s
     hipEvent_t get_event();
     void prof_collect();
 };
@@ -165,17 +180,34 @@ struct Carve {
 size_t key_bytes(int key_type);
 size_t argsort_scratch(int key_type, uint64_t n, uint32_t flags);
 
+size_t argsort_batch_scratch(int key_type, const uint64_t* n, int nsegs, uint64_t hist_rows);
+size_t argsort_batch_staging(int key_type, const uint64_t* n, int nsegs, uint64_t hist_rows);
+
 // Sorts into d_perm_out / d_keys_out using scratch at `scr` (argsort_scratch bytes).
 void argsort_enqueue(Handle* h, int key_type, const void* d_keys, uint64_t n, uint32_t flags,
                      void* d_perm_out, void* d_keys_out, void* scr, hipStream_t s);
+// Sorts every segment of a batch (argsort_batch_scratch bytes of scratch).
+void argsort_batch_enqueue(Handle* h, int key_type, const SortSeg* segs, int nsegs, uint32_t flags, void* scr,
+                           hipStream_t s);
 
 void gather_enqueue(Handle* h, uint32_t elem_bytes, const void* d_src, const void* d_perm,
                     int perm64, uint64_t n, void* d_dst, hipStream_t s);
 
+// One table of a group-by batch.
+struct GbTable {
+    const void* keys;
+    uint64_t n;
+    const void* vals[VDF_MAX_VALUE_COLS];
+};
+
 size_t groupby_scratch(int key_type, uint64_t n);
+size_t groupby_batch_scratch(int key_type, const uint64_t* n, int ntables, uint64_t hist_rows);
+size_t groupby_batch_staging(int key_type, const uint64_t* n, int ntables, uint64_t hist_rows);
 void groupby_enqueue(Handle* h, int key_type, const void* d_keys, uint64_t n,
                      const vdf_value_col* vals, int nvals, uint32_t agg_mask,
                      const vdf_groupby_out* out, void* scr, hipStream_t s);
+void groupby_batch_enqueue(Handle* h, int key_type, const GbTable* tables, int ntables, const int* val_types,
+                           int nvals, uint32_t agg_mask, const vdf_groupby_out* outs, void* scr, hipStream_t s);
 
 // fill kernel (graph-safe replacement for hipMemsetAsync)
 void fill_u32(uint32_t* p, size_t count, uint32_t value, hipStream_t s);

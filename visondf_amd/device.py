@@ -115,28 +115,24 @@ class Vdf:
         return out
 
     # ---------------------------------------------------------------- groupby
-    def groupby(self, keys: torch.Tensor, vals=(), aggs: int = N.AGG_ALL, want_perm: bool = True,
-                stream=None, trim: bool = True, key_type: int | None = None, val_types=None):
-        kt = key_type_of(keys) if key_type is None else key_type
+    @staticmethod
+    def _alloc_out(keys: torch.Tensor, vts, dtypes, aggs: int, want_perm: bool):
+        """Output buffers of one table (capacity n groups) and the C descriptor."""
         n = keys.shape[0]
         dev = keys.device
         out = N.VdfGroupbyOut()
-        ng = torch.zeros(1, dtype=torch.int64, device=dev)
-        gkeys = torch.empty_like(keys)
-        offsets = torch.empty(n + 1, dtype=torch.int32, device=dev)
-        counts = torch.empty(max(n, 1), dtype=torch.int32, device=dev)
-        perm = torch.empty(max(n, 1), dtype=torch.int32, device=dev) if want_perm else None
-        out.n_groups = ng.data_ptr()
-        out.group_keys = gkeys.data_ptr()
-        out.offsets = offsets.data_ptr()
-        out.counts = counts.data_ptr()
-        out.perm = perm.data_ptr() if perm is not None else None
-        cols = (N.VdfValueCol * max(len(vals), 1))()
-        agg_t = []
-        for c, v in enumerate(vals):
-            vt = val_type_of(v) if val_types is None else val_types[c]
-            cols[c].type = vt
-            cols[c].data = v.data_ptr()
+        t = {"ng": torch.zeros(1, dtype=torch.int64, device=dev),
+             "keys": torch.empty_like(keys),
+             "offsets": torch.empty(n + 1, dtype=torch.int32, device=dev),
+             "counts": torch.empty(max(n, 1), dtype=torch.int32, device=dev),
+             "perm": torch.empty(max(n, 1), dtype=torch.int32, device=dev) if want_perm else None,
+             "aggs": []}
+        out.n_groups = t["ng"].data_ptr()
+        out.group_keys = t["keys"].data_ptr() if n else None
+        out.offsets = t["offsets"].data_ptr()
+        out.counts = t["counts"].data_ptr()
+        out.perm = t["perm"].data_ptr() if want_perm else None
+        for c, (vt, dt) in enumerate(zip(vts, dtypes)):
             d = {}
             sum_dt = torch.float64 if vt == N.VAL_F64 else torch.int64
             if aggs & N.AGG_SUM:
@@ -146,20 +142,83 @@ class Vdf:
                 d["mean"] = torch.empty(max(n, 1), dtype=torch.float64, device=dev)
                 out.aggs[c].mean = d["mean"].data_ptr()
             if aggs & N.AGG_MIN:
-                d["min"] = torch.empty(max(n, 1), dtype=v.dtype, device=dev)
+                d["min"] = torch.empty(max(n, 1), dtype=dt, device=dev)
                 out.aggs[c].min = d["min"].data_ptr()
             if aggs & N.AGG_MAX:
-                d["max"] = torch.empty(max(n, 1), dtype=v.dtype, device=dev)
+                d["max"] = torch.empty(max(n, 1), dtype=dt, device=dev)
                 out.aggs[c].max = d["max"].data_ptr()
-            agg_t.append(d)
+            t["aggs"].append(d)
+        return out, t
+
+    @staticmethod
+    def _result(t, n: int, trim: bool) -> GroupbyResult:
+        if not trim:
+            return GroupbyResult(t["ng"], t["keys"], t["offsets"], t["counts"], t["perm"], t["aggs"])
+        g = int(t["ng"].item())
+        return GroupbyResult(g, t["keys"][:g], t["offsets"][:g + 1], t["counts"][:g],
+                             t["perm"][:n] if t["perm"] is not None else None,
+                             [{k: x[:g] for k, x in d.items()} for d in t["aggs"]])
+
+    def groupby(self, keys: torch.Tensor, vals=(), aggs: int = N.AGG_ALL, want_perm: bool = True,
+                stream=None, trim: bool = True, key_type: int | None = None, val_types=None):
+        kt = key_type_of(keys) if key_type is None else key_type
+        n = keys.shape[0]
+        vts = [val_type_of(v) if val_types is None else val_types[c] for c, v in enumerate(vals)]
+        out, t = self._alloc_out(keys, vts, [v.dtype for v in vals], aggs, want_perm)
+        cols = (N.VdfValueCol * max(len(vals), 1))()
+        for c, v in enumerate(vals):
+            cols[c].type = vts[c]
+            cols[c].data = v.data_ptr()
         N.check(self.lib.vdf_groupby(self.h, kt, _ptr(keys), n, cols, len(vals), aggs, C.byref(out),
                                      C.c_void_p(_stream_ptr(stream))))
-        if not trim:
-            return GroupbyResult(ng, gkeys, offsets, counts, perm, agg_t)
-        g = int(ng.item())
-        return GroupbyResult(g, gkeys[:g], offsets[:g + 1], counts[:g],
-                             perm[:n] if perm is not None else None,
-                             [{k: t[:g] for k, t in d.items()} for d in agg_t])
+        return self._result(t, n, trim)
+
+    def groupby_batch(self, tables, aggs: int = N.AGG_ALL, want_perm: bool = True, stream=None,
+                      trim: bool = True, key_type: int | None = None, val_types=None):
+        """tables: list of (keys, [value columns]); all tables share key and value types."""
+        if not tables:
+            return []
+        k0, v0 = tables[0]
+        kt = key_type_of(k0) if key_type is None else key_type
+        vts = [val_type_of(v) if val_types is None else val_types[c] for c, v in enumerate(v0)]
+        nv = len(v0)
+        tabs = (N.VdfTable * len(tables))()
+        outs = (N.VdfGroupbyOut * len(tables))()
+        keep = []
+        for i, (k, vs) in enumerate(tables):
+            tabs[i].keys = k.data_ptr() if k.shape[0] else None
+            tabs[i].n = k.shape[0]
+            for c, v in enumerate(vs):
+                tabs[i].vals[c] = v.data_ptr() if k.shape[0] else None
+            o, t = self._alloc_out(k, vts, [v.dtype for v in vs], aggs, want_perm)
+            outs[i] = o
+            keep.append(t)
+        vt_arr = (C.c_int * max(nv, 1))(*vts)
+        N.check(self.lib.vdf_groupby_batch(self.h, kt, tabs, len(tables), vt_arr, nv, aggs, outs,
+                                           C.c_void_p(_stream_ptr(stream))))
+        return [self._result(t, k.shape[0], trim) for t, (k, _) in zip(keep, tables)]
+
+    def argsort_batch(self, keys_list, *, key_type: int | None = None, descending: bool = False,
+                      reverse_desc: bool = False, perm64: bool = False, want_keys: bool = False, stream=None):
+        if not keys_list:
+            return []
+        kt = key_type_of(keys_list[0]) if key_type is None else key_type
+        flags = (N.SORT_DESC if descending else 0) | (N.SORT_DESC_REVERSE if reverse_desc else 0) \
+            | (N.SORT_PERM64 if perm64 else 0)
+        segs = (N.VdfSortSeg * len(keys_list))()
+        res = []
+        for i, k in enumerate(keys_list):
+            n = k.shape[0]
+            p = torch.empty(n, dtype=torch.int64 if perm64 else torch.int32, device=k.device)
+            ko = torch.empty_like(k) if want_keys else None
+            segs[i].keys = k.data_ptr() if n else None
+            segs[i].n = n
+            segs[i].perm_out = p.data_ptr() if n else None
+            segs[i].sorted_keys_out = ko.data_ptr() if (want_keys and n) else None
+            res.append((p, ko) if want_keys else p)
+        N.check(self.lib.vdf_argsort_batch(self.h, kt, segs, len(keys_list), flags,
+                                           C.c_void_p(_stream_ptr(stream))))
+        return res
 
     # ----------------------------------------------------------------- tuning
     def tune(self, name: str, value: int):
