@@ -1,18 +1,36 @@
-"""Benchmark: radix argsort + permutation gather on one MI355X.
+"""Benchmark of the radix-sort group-by path on one MI355X.
 
-Default workload: 100M rows, uint32 keys uniform over 2^32, stable LSD radix
-argsort (8-bit digits, 4 passes) and the gather of one float64 column through
-the permutation.  One step = argsort + gather of the whole table, inputs
-resident in HBM.  Prints one JSON line.
+Workloads (inputs resident in HBM before timing; synthetic seeded data):
+  argsort_u32   headline: 100M rows, uint32 keys uniform over 2^32; stable LSD
+                radix argsort (8-bit digits, 4 passes) + gather of one float64
+                column through the permutation.  One step = argsort + gather.
+  zipf_groupby  200M rows, int32 keys Zipf(1.1) over 2^24 values, float64
+                N(0,1) and int64 uniform [0, 2^40) values; group-by
+                sum/count/mean/min/max.  One step = the whole group-by.
+  batched       4096 tables, rows log-uniform in [1K, 4M], int32 keys uniform
+                in [0, rows/8), float64 values; every table sorted and grouped
+                by one segmented launch sequence, against one call per table.
+  served        Poisson arrivals (2000 requests/s) of group-bys over 32
+                device-resident tables of 10K-50M rows (uint64 or 16-byte
+                string keys, float64 values), continuous batching over several
+                streams against one launch per request; p50/p99 latency.
 
-    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--rows 100000000]
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3]          # all workloads
+    python bench.py --workload argsort_u32 [--steps ...]          # one, in-process
+
+Without --workload the parent process touches no GPU: it runs each workload
+in a child process with its own time limit and prints one JSON line whose
+top-level fields are the headline workload's, with the others under
+"workloads".
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import subprocess
 import sys
+import threading
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -20,13 +38,85 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0
 METRIC = "group-by rows/sec + HBM GB/s (% peak) at 100M rows; served p50/p99 latency"
+KERNELS = ("radix_histogram", "radix_plan", "radix_pass", "gather", "fill", "groupby_empty", "groupby_tile",
+           "groupby_fixup")
+WORKLOADS = ("argsort_u32", "zipf_groupby", "batched", "served")
+CHILD_TIMEOUT = {"argsort_u32": 600, "zipf_groupby": 400, "batched": 500, "served": 500}
 
 
 def _log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(keys, val, threads: int, reps: int = 3) -> dict:
+def _threads(args) -> int:
+    return args.cpu_threads or min(16, os.cpu_count() or 1)
+
+
+def _load_traffic(name: str):
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    return d.get(name)
+
+
+def _kernel_table(v) -> dict:
+    kernels = {}
+    for name in KERNELS:
+        ms, launches, nbytes = v.profile_read(name)
+        if launches:
+            kernels[name] = {"launches": launches, "total_ms": ms, "avg_us": 1e3 * ms / launches,
+                             "bytes_per_launch": nbytes / launches,
+                             "gbps": (nbytes / (ms * 1e-3)) / 1e9 if ms > 0 and nbytes else None}
+    return kernels
+
+
+def _roofline(kernels: dict, name: str | None = None) -> dict:
+    """HBM roofline of one kernel (default: the one with the most time that
+    moves algorithmic bytes)."""
+    if name is None:
+        cands = [(k["total_ms"], n) for n, k in kernels.items() if k["gbps"]]
+        name = max(cands)[1]
+    k = kernels[name]
+    return {"bound": "hbm", "kernel": name, "achieved": k["gbps"], "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+            "frac": k["gbps"] / HBM_PEAK_GBPS, "traffic": _load_traffic(name)}
+
+
+def _time_steps(step, steps: int, warmup: int, v=None):
+    import torch
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    if v is not None:
+        v.profile(True)
+        v.profile_reset()
+    torch.cuda.synchronize()
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    t0.record()
+    for _ in range(steps):
+        step()
+    t1.record()
+    torch.cuda.synchronize()
+    kernels = None
+    if v is not None:
+        kernels = _kernel_table(v)
+        v.profile(False)
+    return t0.elapsed_time(t1), kernels
+
+
+def _base_record(args, value, unit, ms_step, dtype, config, higher=True) -> dict:
+    return {"metric": METRIC, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": higher, "scaling": "weak",
+            "vs_baseline": None, "dtype": dtype, "data": "synthetic", "config": config}
+
+
+# ---------------------------------------------------------------------------
+# headline: 100M uint32 argsort + f64 gather
+# ---------------------------------------------------------------------------
+def cpu_baseline_argsort(keys, val, threads: int, reps: int = 3) -> dict:
     """The oracle's restatement of radix_sort_uint32_mt (16-bit digits) + gather,
     timed on the host cores (median of reps after one warm-up)."""
     import numpy as np
@@ -49,32 +139,7 @@ def cpu_baseline(keys, val, threads: int, reps: int = 3) -> dict:
                       f"oracle radix_sort_uint32_mt restatement, 16-bit digits, {threads} OpenMP threads"}
 
 
-def _load_traffic(name: str):
-    path = os.path.join(ROOT, "profiles", "traffic.json")
-    if not os.path.exists(path):
-        return None
-    with open(path) as f:
-        d = json.load(f)
-    return d.get(name)
-
-
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--rows", type=int, default=100_000_000)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--calibrate", action="store_true",
-                    help="first run one plain 1 GiB device copy (the known-size kernel that scales "
-                         "rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU sharding is a design "
-              f"note, not part of this build", file=sys.stderr)
-        sys.exit(2)
-
+def run_argsort_u32(args) -> dict:
     import numpy as np
     import torch
 
@@ -82,8 +147,8 @@ def main():
     from visondf_amd import datagen
     from visondf_amd.device import Vdf
 
-    n = args.rows
-    _log(f"generating {n} rows")
+    n = args.rows or 100_000_000
+    _log(f"argsort_u32: generating {n} rows")
     keys, val = datagen.argsort_u32(n)
     d_keys = torch.from_numpy(keys.view(np.int32)).cuda()
     d_val = torch.from_numpy(val).cuda()
@@ -101,69 +166,360 @@ def main():
         v.argsort(d_keys, key_type=N.KEY_U32, perm_out=perm)
         v.gather(d_val, perm, out=out)
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    v.profile(True)
-    v.profile_reset()
-    torch.cuda.synchronize()
-    t0 = torch.cuda.Event(enable_timing=True)
-    t1 = torch.cuda.Event(enable_timing=True)
-    t0.record()
-    for _ in range(args.steps):
-        step()
-    t1.record()
-    torch.cuda.synchronize()
-    ms_total = t0.elapsed_time(t1)
+    ms_total, kernels = _time_steps(step, args.steps, args.warmup, v)
     ms_step = ms_total / args.steps
-    kernels = {}
-    for name in ("radix_histogram", "radix_plan", "radix_pass", "gather", "fill"):
-        ms, launches, nbytes = v.profile_read(name)
-        if launches:
-            kernels[name] = {"launches": launches, "avg_us": 1e3 * ms / launches,
-                             "bytes_per_launch": nbytes / launches,
-                             "gbps": (nbytes / (ms * 1e-3)) / 1e9 if ms > 0 else None}
-    v.profile(False)
-    p = kernels["radix_pass"]
-    achieved = p["gbps"]
-    traffic = _load_traffic("radix_pass")
     rows_per_s = n * args.steps / (ms_total * 1e-3)
     step_bytes = sum(k["bytes_per_launch"] * k["launches"] for k in kernels.values()) / args.steps
-    # correctness spot check of the last step (cheap: sortedness through the permutation)
+    # spot check of the last step: the permutation sorts the keys and the gather follows it
     ks = d_keys.to(torch.int64) & 0xFFFFFFFF
     sk = ks[perm.to(torch.int64)]
     assert bool((sk[1:] >= sk[:-1]).all()), "permutation does not sort the keys"
+    assert torch.equal(out, d_val[perm.to(torch.int64)]), "gather does not follow the permutation"
 
-    rec = {
-        "metric": METRIC,
-        "value": rows_per_s,
-        "unit": "rows/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "u32",
-        "data": "synthetic",
-        "config": {"workload": "argsort_u32_gather_f64", "rows": n, "key": "uint32 uniform over 2^32",
-                   "value_column": "float64", "digit_bits": 8, "radix_passes": 4,
-                   "parallelism": "single GPU"},
-        "hbm_gbps_step": step_bytes / (ms_step * 1e-3) / 1e9,
-        "hbm_pct_peak_step": 100.0 * step_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBPS,
-        "roofline": {"bound": "hbm", "kernel": "radix_pass", "achieved": achieved, "peak": HBM_PEAK_GBPS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic},
-        "kernels": kernels,
-    }
+    rec = _base_record(args, rows_per_s, "rows/s", ms_step, "u32",
+                       {"workload": "argsort_u32_gather_f64", "rows": n, "key": "uint32 uniform over 2^32",
+                        "value_column": "float64", "digit_bits": 8, "radix_passes": 4,
+                        "parallelism": "single GPU"})
+    rec["hbm_gbps_step"] = step_bytes / (ms_step * 1e-3) / 1e9
+    rec["hbm_pct_peak_step"] = 100.0 * rec["hbm_gbps_step"] / HBM_PEAK_GBPS
+    rec["roofline"] = _roofline(kernels, "radix_pass")
+    rec["kernels"] = kernels
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        rec["cpu_baseline"] = cpu_baseline(keys, val, threads)
+        rec["cpu_baseline"] = cpu_baseline_argsort(keys, val, _threads(args))
         rec["speedup_vs_cpu"] = rows_per_s / rec["cpu_baseline"]["value"]
     else:
         rec["cpu_baseline"] = None
     v.close()
-    print(json.dumps(rec), flush=True)
+    return rec
+
+
+# ---------------------------------------------------------------------------
+# 200M-row Zipf group-by
+# ---------------------------------------------------------------------------
+def _cpu_groupby(tables, threads: int, what: str) -> dict:
+    """The oracle's sort-based group-by over a list of (keys, [values]) tables."""
+    import oracle
+
+    rows = sum(k.shape[0] for k, _ in tables)
+    oracle.groupby(tables[0][0][:1000], [x[:1000] for x in tables[0][1]], threads)  # warm-up
+    t0 = time.perf_counter()
+    for k, vals in tables:
+        oracle.groupby(k, vals, threads)
+    dt = time.perf_counter() - t0
+    _log(f"cpu baseline ({what}): {rows} rows in {dt:.3f} s")
+    return {"value": rows / dt, "unit": "rows/s", "cores": threads, "kind": "port",
+            "sample": f"{what}; oracle restatement of the reference's radix sort + group-by loops, "
+                      f"{threads} OpenMP threads"}
+
+
+def run_zipf_groupby(args) -> dict:
+    import torch
+
+    from visondf_amd import _native as N
+    from visondf_amd import datagen
+    from visondf_amd.device import Vdf
+
+    n = args.rows or 200_000_000
+    _log(f"zipf_groupby: generating {n} rows")
+    key, vf, vi = datagen.zipf_groupby(n)
+    d_key = torch.from_numpy(key).cuda()
+    d_vf = torch.from_numpy(vf).cuda()
+    d_vi = torch.from_numpy(vi).cuda()
+    v = Vdf(0)
+    res = {}
+
+    def step():
+        res["r"] = v.groupby(d_key, [d_vf, d_vi], aggs=N.AGG_ALL, want_perm=False, trim=False)
+
+    ms_total, kernels = _time_steps(step, args.steps, args.warmup, v)
+    ms_step = ms_total / args.steps
+    rows_per_s = n * args.steps / (ms_total * 1e-3)
+    g = int(res["r"].n_groups.item())
+    cnt = res["r"].counts[:g].to(torch.int64).sum().item()
+    assert cnt == n, "group counts do not add up to the rows"
+    rec = _base_record(args, rows_per_s, "rows/s", ms_step, "f64+i64",
+                       {"workload": "zipf_groupby", "rows": n, "key": "int32 Zipf(1.1) over 2^24 values",
+                        "values": ["float64 N(0,1)", "int64 uniform [0,2^40)"],
+                        "aggregates": "count,sum,mean,min,max", "groups": g, "parallelism": "single GPU"})
+    rec["roofline"] = _roofline(kernels)
+    rec["kernels"] = kernels
+    if not args.no_cpu:
+        m = min(n, 20_000_000)
+        rec["cpu_baseline"] = _cpu_groupby([(key[:m], [vf[:m], vi[:m]])], _threads(args),
+                                           f"first {m} rows of the workload")
+        rec["speedup_vs_cpu"] = rows_per_s / rec["cpu_baseline"]["value"]
+    v.close()
+    return rec
+
+
+# ---------------------------------------------------------------------------
+# batched: 4096 tables in one segmented launch sequence
+# ---------------------------------------------------------------------------
+def run_batched(args) -> dict:
+    import ctypes as C
+
+    import numpy as np
+    import torch
+
+    from visondf_amd import _native as N
+    from visondf_amd import datagen
+    from visondf_amd.device import Vdf, _stream_ptr
+
+    ntab = args.tables or 4096
+    sizes = datagen.batched_table_sizes(ntab, 1_000, 4_000_000, seed=4)
+    total = int(sizes.sum())
+    _log(f"batched: generating {ntab} tables, {total} rows")
+    keys = np.empty(total, dtype=np.int32)
+    vals = np.empty(total, dtype=np.float64)
+    offs = np.zeros(ntab + 1, dtype=np.int64)
+    offs[1:] = np.cumsum(sizes)
+    for i, n in enumerate(sizes):
+        k, x = datagen.batched_table(int(n), i)
+        keys[offs[i]:offs[i + 1]] = k
+        vals[offs[i]:offs[i + 1]] = x
+    d_keys = torch.from_numpy(keys).cuda()
+    d_vals = torch.from_numpy(vals).cuda()
+    v = Vdf(0)
+    caps = [max(int(n) // 8, 1) for n in sizes]  # keys are drawn from [0, rows/8)
+    tables = [(d_keys[offs[i]:offs[i + 1]], [d_vals[offs[i]:offs[i + 1]]]) for i in range(ntab)]
+
+    # descriptors built once: the timed region is the library call
+    tabs = (N.VdfTable * ntab)()
+    outs = (N.VdfGroupbyOut * ntab)()
+    keep = []
+    for i, (k, vs) in enumerate(tables):
+        tabs[i].keys = k.data_ptr()
+        tabs[i].n = k.shape[0]
+        tabs[i].vals[0] = vs[0].data_ptr()
+        o, t = Vdf._alloc_out(k, [N.VAL_F64], [torch.float64], N.AGG_ALL, False, caps[i])
+        outs[i] = o
+        keep.append(t)
+    vt = (C.c_int * 1)(N.VAL_F64)
+    stream = C.c_void_p(_stream_ptr(None))
+
+    def step():
+        N.check(v.lib.vdf_groupby_batch(v.h, N.KEY_I32, tabs, ntab, vt, 1, N.AGG_ALL, outs, stream))
+
+    ms_total, kernels = _time_steps(step, args.steps, args.warmup, v)
+    ms_step = ms_total / args.steps
+    rows_per_s = total * args.steps / (ms_total * 1e-3)
+    ng = torch.cat([t["ng"] for t in keep]).cpu().numpy()
+    assert np.all(ng <= np.array(caps)), "a table has more groups than its capacity"
+    cnt = sum(int(t["counts"][:g].to(torch.int64).sum().item()) for t, g in zip(keep[:64], ng[:64]))
+    assert cnt == int(sizes[:64].sum()), "group counts do not add up to the rows"
+
+    # one vdf_groupby call per table, same outputs, descriptors also built beforehand
+    cols = []
+    for i, (k, vs) in enumerate(tables):
+        c = (N.VdfValueCol * 1)()
+        c[0].type = N.VAL_F64
+        c[0].data = vs[0].data_ptr()
+        cols.append(c)
+
+    def per_table():
+        for i, (k, _) in enumerate(tables):
+            N.check(v.lib.vdf_groupby(v.h, N.KEY_I32, k.data_ptr(), k.shape[0], cols[i], 1, N.AGG_ALL,
+                                      C.byref(outs[i]), stream))
+
+    per_table()
+    torch.cuda.synchronize()
+    ms_single, _ = _time_steps(per_table, 1, 0)
+    rec = _base_record(args, rows_per_s, "rows/s", ms_step, "i32+f64",
+                       {"workload": "batched_groupby", "tables": ntab, "rows": total,
+                        "rows_per_table": "log-uniform [1K, 4M]", "key": "int32 uniform [0, rows/8)",
+                        "values": ["float64 uniform [0,1)"], "aggregates": "count,sum,mean,min,max",
+                        "parallelism": "single GPU"})
+    rec["one_call_per_table"] = {"ms": ms_single, "rows_per_s": total / (ms_single * 1e-3),
+                                 "batched_speedup": ms_single / ms_step}
+    rec["roofline"] = _roofline(kernels)
+    rec["kernels"] = kernels
+    if not args.no_cpu:
+        sample = [(keys[offs[i]:offs[i + 1]], [vals[offs[i]:offs[i + 1]]]) for i in range(min(ntab, 96))]
+        rec["cpu_baseline"] = _cpu_groupby(sample, _threads(args), f"the first {len(sample)} tables, one call each")
+        rec["speedup_vs_cpu"] = rows_per_s / rec["cpu_baseline"]["value"]
+    v.close()
+    return rec
+
+
+# ---------------------------------------------------------------------------
+# served: Poisson arrivals, continuous batching vs one launch per request
+# ---------------------------------------------------------------------------
+def _replay(srv, ids, rows, arrivals, picks):
+    """Submits requests at their arrival times (busy-wait pacing) while a
+    second thread waits for each result in order and releases it.  Returns
+    per-request (latency s, queue s, batch size) and the makespan."""
+    import numpy as np
+
+    R = len(arrivals)
+    rids = [None] * R
+    ready = [threading.Event() for _ in range(R)]
+    out = [None] * R
+
+    def collector():
+        for i in range(R):
+            ready[i].wait()
+            s = srv.wait(rids[i], timeout_s=300.0)
+            if s is None:
+                raise RuntimeError("served request did not finish in 300 s")
+            out[i] = s
+            srv.release(rids[i])
+
+    th = threading.Thread(target=collector)
+    th.start()
+    t0 = time.perf_counter()
+    for i in range(R):
+        while time.perf_counter() - t0 < arrivals[i]:
+            pass
+        rids[i] = srv.submit(ids[picks[i]], want_perm=False)
+        ready[i].set()
+    th.join()
+    if any(s is None for s in out):
+        raise RuntimeError("collector failed")
+    lat = np.array([(s.done_ns - s.submit_ns) * 1e-9 for s in out])
+    qs = np.array([(s.start_ns - s.submit_ns) * 1e-9 for s in out])
+    bt = np.array([s.batch_tables for s in out])
+    bad = [s.status for s in out if s.status != 0]
+    if bad:
+        raise RuntimeError(f"{len(bad)} served requests failed (status {bad[0]})")
+    span = (max(s.done_ns for s in out) - min(s.submit_ns for s in out)) * 1e-9
+    return lat, qs, bt, span
+
+
+def run_served(args) -> dict:
+    import numpy as np
+    import torch
+
+    from visondf_amd import _native as N
+    from visondf_amd import datagen
+    from visondf_amd.server import Server
+
+    T = args.tables or 32
+    R = args.requests
+    rate = args.rate
+    sizes = datagen.served_table_sizes(T, 10_000, 50_000_000, seed=5)
+    _log(f"served: generating {T} tables, {int(sizes.sum())} rows")
+    tables = []
+    for i, n in enumerate(sizes):
+        kind = "u64" if i % 2 == 0 else "char16"
+        k, x = datagen.served_table(int(n), i, kind)
+        dk = torch.from_numpy(k.view(np.int64) if kind == "u64" else k).cuda()
+        tables.append((kind, int(n), dk, torch.from_numpy(x).cuda(), k, x))
+    torch.cuda.synchronize()
+    rng = np.random.default_rng(6)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, R))
+    picks = rng.integers(0, T, R)
+    rows = np.array([t[1] for t in tables])
+    req_rows = int(rows[picks].sum())
+
+    def serve(workers, max_batch):
+        with Server(0, arena_bytes=args.arena_gb << 30, workers=workers, max_batch_tables=max_batch) as srv:
+            ids = [srv.add_table(dk, [dx], key_type=N.KEY_U64 if kind == "u64" else N.KEY_CHAR16,
+                                 max_groups=n // 16 + 1) for kind, n, dk, dx, _, _ in tables]
+            for tid in ids:  # warm-up: every table once
+                r = srv.submit(tid)
+                s = srv.wait(r, timeout_s=120.0)
+                if s is None or s.status != 0:
+                    raise RuntimeError("warm-up request failed")
+                srv.release(r)
+            lat, qs, bt, span = _replay(srv, ids, rows, arrivals, picks)
+            st = srv.stats()
+        return {"workers": workers, "max_batch_tables": max_batch,
+                "p50_ms": 1e3 * float(np.percentile(lat, 50)), "p99_ms": 1e3 * float(np.percentile(lat, 99)),
+                "mean_ms": 1e3 * float(lat.mean()), "queue_p50_ms": 1e3 * float(np.percentile(qs, 50)),
+                "rows_per_s": req_rows / span, "requests_per_s": R / span, "makespan_s": span,
+                "mean_batch": float(bt.mean()), "max_batch": int(bt.max()), "launches": st["launches"],
+                "arena_peak_gb": st["arena_peak"] / 2**30}
+
+    _log("served: continuous batching")
+    batched = serve(args.workers, 64)
+    _log(f"served: batched p50 {batched['p50_ms']:.2f} ms p99 {batched['p99_ms']:.2f} ms")
+    _log("served: one launch per request")
+    single = serve(1, 1)
+    _log(f"served: single p50 {single['p50_ms']:.2f} ms p99 {single['p99_ms']:.2f} ms")
+    rec = _base_record(args, batched["rows_per_s"], "rows/s", 1e3 * batched["makespan_s"], "u64/char16+f64",
+                       {"workload": "served_groupby", "tables": T, "rows_per_table": "log-uniform [10K, 50M]",
+                        "keys": "uint64 (even tables) / 16-byte ASCII (odd tables), rows/16 distinct",
+                        "values": ["float64 uniform [0,1)"], "arrivals": f"Poisson {rate:g}/s",
+                        "requests": R, "request_rows": req_rows, "parallelism": "single GPU"})
+    rec["steps"] = R
+    rec["warmup"] = T
+    rec["served"] = {"continuous_batching": batched, "one_launch_per_request": single,
+                     "p99_speedup": single["p99_ms"] / batched["p99_ms"],
+                     "p50_speedup": single["p50_ms"] / batched["p50_ms"]}
+    if not args.no_cpu:
+        sample = [(t[4], [t[5]]) for t in tables if t[1] <= 2_000_000][:8]
+        rec["cpu_baseline"] = _cpu_groupby(sample, _threads(args),
+                                           f"{len(sample)} served tables of <= 2M rows, one call each")
+    return rec
+
+
+RUNNERS = {"argsort_u32": run_argsort_u32, "zipf_groupby": run_zipf_groupby, "batched": run_batched,
+           "served": run_served}
+
+
+def _child(name: str, args) -> dict:
+    steps, warmup = args.steps, args.warmup
+    if name in ("zipf_groupby", "batched"):  # long steps: fewer of them
+        steps, warmup = max(3, args.steps // 4), min(args.warmup, 2)
+    cmd = [sys.executable, os.path.abspath(__file__), "--workload", name, "--steps", str(steps),
+           "--warmup", str(warmup)]
+    if args.no_cpu:
+        cmd.append("--no-cpu")
+    if args.calibrate and name == "argsort_u32":
+        cmd.append("--calibrate")
+    _log(f"running {name}: {' '.join(cmd[1:])}")
+    t0 = time.time()
+    p = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=None, text=True, timeout=CHILD_TIMEOUT[name])
+    lines = [ln for ln in p.stdout.splitlines() if ln.startswith("{")]
+    if p.returncode != 0 or not lines:
+        raise RuntimeError(f"{name} exited with status {p.returncode}")
+    rec = json.loads(lines[-1])
+    rec["wall_s"] = time.time() - t0
+    return rec
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", choices=WORKLOADS, default=None,
+                    help="run one workload in this process (default: all, each in a child process)")
+    ap.add_argument("--workloads", default=",".join(WORKLOADS), help="workloads of the default run")
+    ap.add_argument("--rows", type=int, default=0, help="rows of argsort_u32 / zipf_groupby (0 = the workload's)")
+    ap.add_argument("--tables", type=int, default=0, help="tables of batched / served (0 = the workload's)")
+    ap.add_argument("--requests", type=int, default=1000, help="served: requests replayed")
+    ap.add_argument("--rate", type=float, default=2000.0, help="served: Poisson arrival rate, requests/s")
+    ap.add_argument("--workers", type=int, default=3, help="served: launch sequences in flight")
+    ap.add_argument("--arena-gb", type=int, default=48, help="served: server arena (admission budget), GiB")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--calibrate", action="store_true",
+                    help="first run one plain 1 GiB device copy (the known-size kernel that scales "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU sharding is a design "
+              f"note, not part of this build", file=sys.stderr)
+        sys.exit(2)
+
+    if args.workload:
+        print(json.dumps(RUNNERS[args.workload](args)), flush=True)
+        return
+
+    names = [w for w in args.workloads.split(",") if w]
+    head = _child("argsort_u32", args)
+    others = {}
+    for name in names:
+        if name == "argsort_u32":
+            continue
+        try:
+            others[name] = _child(name, args)
+        except Exception as e:  # a failed secondary workload is reported, not fatal
+            _log(f"{name} failed: {e}")
+            others[name] = {"error": str(e)}
+    head["workloads"] = others
+    print(json.dumps(head), flush=True)
 
 
 if __name__ == "__main__":

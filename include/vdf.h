@@ -53,6 +53,7 @@ typedef int vdf_status;
 #define VDF_ERR_OOM          3 /* device allocation failed */
 #define VDF_ERR_HIP          4 /* a HIP runtime call failed */
 #define VDF_ERR_BUDGET       5 /* request exceeds the handle's HBM budget */
+#define VDF_ERR_TIMEOUT      6 /* vdf_server_wait: the request is not done yet */
 
 /* key column types (sort_by_mt.hpp:72-115 type letters u / i / c) */
 #define VDF_KEY_U32     0
@@ -129,14 +130,19 @@ typedef struct vdf_agg_out {
 } vdf_agg_out;
 
 typedef struct vdf_groupby_out {
-    uint64_t* n_groups;  /* device, 1 element: G                                    */
-    void* group_keys;    /* device, key type, capacity n: keys in ascending order    */
-    uint32_t* offsets;   /* device, capacity n + 1: group g is perm[offsets[g] ..   */
+    uint64_t* n_groups;  /* device, 1 element: G (always the true group count)      */
+    void* group_keys;    /* device, key type, C groups: keys in ascending order      */
+    uint32_t* offsets;   /* device, C + 1 entries: group g is perm[offsets[g] ..     */
                          /* offsets[g+1]) ; offsets[G] = n                          */
-    uint32_t* counts;    /* device, capacity n (nullable)                           */
-    uint32_t* perm;      /* device, capacity n (nullable): group-contiguous row ids */
+    uint32_t* counts;    /* device, C groups (nullable)                             */
+    uint32_t* perm;      /* device, n row ids (nullable): group-contiguous row ids  */
                          /* in row order within each group (row_view_idx analogue) */
-    vdf_agg_out aggs[VDF_MAX_VALUE_COLS]; /* per value column; null members skipped */
+    vdf_agg_out aggs[VDF_MAX_VALUE_COLS]; /* per value column, C groups; null      */
+                                          /* members skipped                       */
+    uint64_t capacity;   /* C: groups the per-group outputs hold; 0 means n.        */
+                         /* When G > C only groups 0 .. C-1 are written (offsets    */
+                         /* up to C, no sentinel) and *n_groups still reports G, so */
+                         /* the caller sees the truncation and can retry larger.    */
 } vdf_groupby_out;
 
 /* Group rows by one key column and aggregate up to VDF_MAX_VALUE_COLS value
@@ -182,6 +188,9 @@ vdf_status vdf_groupby_batch(vdf_handle* h, int key_type, const vdf_table* table
  *   "sort_items_32"  rows per thread of a radix pass tile, 4-byte keys: 12, 16, 24
  *   "sort_items_64"  same for 8-byte keys: 8, 12, 16   (16-byte keys use 8)
  *   "sort_chunk"     consecutive tiles one tile group (one XCD) claims at a time: 1 .. 1024
+ *   "sort_xcd"       1 (default): XCD-grouped tile order, passes launched cooperatively;
+ *                    0: global tile order, plain launches (for handles whose sorts run
+ *                    concurrently with other persistent work on the device)
  *   "hist_rows"      rows per digit-histogram work item: 4096 .. 2^24
  *   "gather_rows"    rows per thread of the gather kernel: 4, 8, 16
  *   "gather_nt"      1 = non-temporal loads of the gathered column
@@ -199,6 +208,75 @@ vdf_status vdf_profile_enable(vdf_handle* h, int on);
 vdf_status vdf_profile_reset(vdf_handle* h);
 vdf_status vdf_profile_read(vdf_handle* h, const char* kernel, double* total_ms,
                             uint64_t* launches, uint64_t* bytes);
+
+/* ---- group-by request server: device-resident tables, continuous batching ----
+ * A server owns one HBM arena of arena_bytes (allocated at create) and
+ * `workers` host threads, each with its own stream.  Tables are registered
+ * once (their columns stay in the caller's device memory and must outlive the
+ * server).  Submitted requests wait in one FIFO queue; whenever a worker is
+ * free it takes the oldest request plus later requests of the same key type,
+ * value types and aggregate mask (up to max_batch_tables requests and
+ * max_batch_rows rows, the oldest always taken), reserves each request's
+ * result block and the batch's scratch in the arena, and runs them as one
+ * vdf_groupby_batch launch sequence.  Requests that arrive while batches run
+ * are admitted by the next free worker as soon as arena memory allows, so
+ * batches of different workers overlap on the device.  A request that does
+ * not fit waits (the oldest request is never passed for memory); one that can
+ * never fit fails with VDF_ERR_BUDGET.  Results stay in the arena until
+ * vdf_server_release.  A request's results are identical to vdf_groupby on
+ * its table alone.
+ * No reference counterpart: VisonDF has no request loop; each group-by runs
+ * on the calling thread (transform_group_by.hpp:11-14). */
+typedef struct vdf_server vdf_server;
+
+typedef struct vdf_server_config {
+    int device;
+    int workers;              /* launch sequences in flight at once: 1 .. 16            */
+    int max_batch_tables;     /* requests per launch sequence: 1 .. 65536               */
+    int reserved;
+    uint64_t arena_bytes;     /* HBM for scratch and results (the admission budget)     */
+    uint64_t max_batch_rows;  /* rows per launch sequence (0 = no limit)                */
+} vdf_server_config;
+
+typedef struct vdf_served {
+    vdf_status status;        /* of the request (VDF_OK or why it failed)               */
+    uint32_t batch_tables;    /* requests in the launch sequence that ran it            */
+    uint64_t n_groups;
+    uint64_t submit_ns;       /* steady-clock stamps: submitted, launch started, done   */
+    uint64_t start_ns;
+    uint64_t done_ns;
+    vdf_groupby_out out;      /* device pointers into the arena (valid until release);  */
+                              /* out.n_groups is NULL (the count is n_groups above)     */
+} vdf_served;
+
+typedef struct vdf_server_stats {
+    uint64_t submitted, completed, failed;
+    uint64_t launches;        /* launch sequences (batches) run                         */
+    uint64_t retries;         /* requests rerun with a larger group capacity            */
+    uint64_t rows;            /* rows of completed requests                             */
+    uint64_t queued;          /* waiting now                                            */
+    uint64_t arena_bytes, arena_used, arena_peak;
+} vdf_server_stats;
+
+vdf_status vdf_server_create(vdf_server** out, const vdf_server_config* cfg);
+/* Finishes every queued request, then frees the arena and the workers. */
+vdf_status vdf_server_destroy(vdf_server* s);
+/* Registers a device-resident table.  max_groups sizes its per-group result
+ * arrays (0 = n); a request whose table has more groups is rerun once with
+ * the exact count. */
+vdf_status vdf_server_add_table(vdf_server* s, int key_type, const void* d_keys, uint64_t n,
+                                const int* val_types, const void* const* d_vals, int nvals,
+                                uint64_t max_groups, uint32_t* table_id);
+/* Queues a group-by of a registered table (want_perm: also the group-contiguous
+ * row ids).  Returns at once. */
+vdf_status vdf_server_submit(vdf_server* s, uint32_t table_id, uint32_t agg_mask, int want_perm,
+                             uint64_t* request_id);
+/* Waits up to timeout_us (< 0: no limit) for a request; VDF_ERR_TIMEOUT if it
+ * is not done.  On VDF_OK, *out describes the request (its own status too). */
+vdf_status vdf_server_wait(vdf_server* s, uint64_t request_id, int64_t timeout_us, vdf_served* out);
+/* Frees a finished request's results (their arena memory admits new work). */
+vdf_status vdf_server_release(vdf_server* s, uint64_t request_id);
+vdf_status vdf_server_stats_get(vdf_server* s, vdf_server_stats* out);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,158 @@
+// Unit test of the request server's host logic (visondf_amd/csrc/serve_core.h):
+// the arena range allocator against a byte-occupancy model, and the batch
+// planner's admission rules.  Built with g++ by tests/test_serve_core.py.
+#include "../visondf_amd/csrc/serve_core.h"
+
+#include <cstdio>
+#include <cstdlib>
+#include <random>
+#include <vector>
+
+using namespace vdf;
+
+static int g_fail = 0;
+#define CHECK(c)                                                            \
+    do {                                                                    \
+        if (!(c)) {                                                         \
+            std::printf("FAIL %s:%d: %s\n", __FILE__, __LINE__, #c);        \
+            ++g_fail;                                                       \
+        }                                                                   \
+    } while (0)
+
+static void allocator_random(uint64_t seed)
+{
+    const uint64_t A = 256, SIZE = 1 << 20;
+    RangeAllocator ra(SIZE, A);
+    std::vector<int> owner(SIZE / A, -1);  // model: which live range holds each 256-byte unit
+    struct Live {
+        uint64_t off, len;
+    };
+    std::vector<Live> live;
+    std::mt19937_64 rng(seed);
+    for (int step = 0; step < 20000; ++step) {
+        const bool do_alloc = live.empty() || (rng() % 100) < 55;
+        if (do_alloc) {
+            const uint64_t bytes = 1 + rng() % (SIZE / 16);
+            const uint64_t off = ra.alloc(bytes);
+            const uint64_t len = (bytes + A - 1) / A * A;
+            // model: first fit over free units
+            uint64_t want = RangeAllocator::kNone;
+            for (uint64_t u = 0, run = 0; u < SIZE / A; ++u) {
+                run = owner[u] < 0 ? run + 1 : 0;
+                if (run * A >= len) {
+                    want = (u + 1) * A - len;
+                    break;
+                }
+            }
+            CHECK(off == want);
+            if (off == RangeAllocator::kNone) continue;
+            CHECK(off % A == 0 && off + len <= SIZE);
+            for (uint64_t u = off / A; u < (off + len) / A; ++u) {
+                CHECK(owner[u] < 0);
+                owner[u] = step;
+            }
+            live.push_back({off, len});
+        } else {
+            const size_t k = rng() % live.size();
+            CHECK(ra.release(live[k].off));
+            for (uint64_t u = live[k].off / A; u < (live[k].off + live[k].len) / A; ++u) owner[u] = -1;
+            live.erase(live.begin() + static_cast<long>(k));
+        }
+        uint64_t used = 0;
+        for (const Live& l : live) used += l.len;
+        CHECK(ra.used() == used);
+        CHECK(ra.live_ranges() == live.size());
+    }
+    CHECK(!ra.release(12345 * A + 7));  // not a live range start
+    for (const Live& l : live) CHECK(ra.release(l.off));
+    CHECK(ra.used() == 0);
+    CHECK(ra.free_ranges() == 1);  // fully merged
+    CHECK(ra.largest_free() == SIZE);
+}
+
+static uint64_t scratch_of(const std::vector<uint64_t>& ns)
+{
+    uint64_t s = 1000;
+    for (uint64_t n : ns) s += 10 * n;
+    return s;
+}
+
+static void planner_rules()
+{
+    const uint64_t A = 256;
+    // FIFO head, signature filter, table limit
+    {
+        RangeAllocator ra(1 << 24, A);
+        std::vector<PlanItem> q = {{100, 1, 1000}, {200, 2, 1000}, {300, 1, 1000}, {400, 1, 1000}, {500, 1, 1000}};
+        PlanLimits lim;
+        lim.max_tables = 3;
+        Plan p = plan_batch(q, lim, ra, scratch_of);
+        CHECK(p.picked.size() == 3);
+        CHECK(p.picked[0] == 0 && p.picked[1] == 2 && p.picked[2] == 3);
+        CHECK(p.scr_bytes == scratch_of({100, 300, 400}));
+        CHECK(p.scr_off != RangeAllocator::kNone);
+        CHECK(ra.live_ranges() == 4);
+    }
+    // row limit skips large later requests but always takes the head
+    {
+        RangeAllocator ra(1 << 24, A);
+        std::vector<PlanItem> q = {{5000, 7, 10}, {100, 7, 10}, {9000, 7, 10}, {200, 7, 10}};
+        PlanLimits lim;
+        lim.max_rows = 5400;
+        Plan p = plan_batch(q, lim, ra, scratch_of);
+        CHECK(p.picked.size() == 3);
+        CHECK(p.picked[0] == 0 && p.picked[1] == 1 && p.picked[2] == 3);
+    }
+    // memory: the scan stops at the first request whose result block does not fit
+    {
+        RangeAllocator ra(64 * A, A);
+        std::vector<PlanItem> q = {{1, 0, 8 * A}, {1, 0, 60 * A}, {1, 0, 2 * A}};
+        PlanLimits lim;
+        Plan p = plan_batch(q, lim, ra, [](const std::vector<uint64_t>&) { return uint64_t(4 * 256); });
+        CHECK(p.picked.size() == 1 && p.picked[0] == 0);
+        CHECK(ra.used() == 12 * A);
+    }
+    // scratch does not fit with every pick: newest picks are dropped until it does
+    {
+        RangeAllocator ra(40 * A, A);
+        std::vector<PlanItem> q = {{10, 0, 10 * A}, {10, 0, 10 * A}, {10, 0, 10 * A}};
+        PlanLimits lim;
+        auto scr = [](const std::vector<uint64_t>& ns) { return uint64_t(ns.size() * 5 * 256); };
+        Plan p = plan_batch(q, lim, ra, scr);
+        CHECK(p.picked.size() == 2);  // 3 blocks = 30 units + 15 scratch > 40; 2 blocks + 10 = 30
+        CHECK(ra.used() == 30 * A);
+        CHECK(p.scr_bytes == 10 * A);
+    }
+    // nothing fits now (memory held elsewhere): empty plan, not head_never_fits
+    {
+        RangeAllocator ra(40 * A, A);
+        const uint64_t held = ra.alloc(35 * A);
+        std::vector<PlanItem> q = {{10, 0, 10 * A}};
+        Plan p = plan_batch(q, PlanLimits{}, ra, scratch_of);
+        CHECK(p.picked.empty() && !p.head_never_fits);
+        CHECK(ra.used() == 35 * A);
+        ra.release(held);
+        Plan p2 = plan_batch(q, PlanLimits{}, ra, scratch_of);
+        CHECK(p2.picked.size() == 1);
+    }
+    // the head alone exceeds the arena
+    {
+        RangeAllocator ra(40 * A, A);
+        std::vector<PlanItem> q = {{10, 0, 39 * A}, {1, 0, A}};
+        Plan p = plan_batch(q, PlanLimits{}, ra, [](const std::vector<uint64_t>&) { return uint64_t(2 * 256); });
+        CHECK(p.head_never_fits && p.picked.empty());
+        CHECK(ra.used() == 0);
+    }
+}
+
+int main()
+{
+    for (uint64_t s = 1; s <= 4; ++s) allocator_random(s);
+    planner_rules();
+    if (g_fail) {
+        std::printf("%d failures\n", g_fail);
+        return 1;
+    }
+    std::printf("serve_core ok\n");
+    return 0;
+}

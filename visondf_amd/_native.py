@@ -19,6 +19,7 @@ VDF_ERR_UNSUPPORTED = 2
 VDF_ERR_OOM = 3
 VDF_ERR_HIP = 4
 VDF_ERR_BUDGET = 5
+VDF_ERR_TIMEOUT = 6
 
 # key types
 KEY_U32, KEY_I32, KEY_U64, KEY_I64, KEY_CHAR16 = 0, 1, 2, 3, 4
@@ -40,6 +41,8 @@ EXPORTS = (
     "vdf_profile_enable", "vdf_profile_reset", "vdf_profile_read",
     "vdf_tune_set", "vdf_tune_get",
     "vdf_argsort_batch", "vdf_groupby_batch",
+    "vdf_server_create", "vdf_server_destroy", "vdf_server_add_table", "vdf_server_submit",
+    "vdf_server_wait", "vdf_server_release", "vdf_server_stats_get",
 )
 
 
@@ -59,7 +62,24 @@ class VdfGroupbyOut(C.Structure):
         ("counts", C.c_void_p),
         ("perm", C.c_void_p),
         ("aggs", VdfAggOut * MAX_VALUE_COLS),
+        ("capacity", C.c_uint64),
     ]
+
+
+class VdfServerConfig(C.Structure):
+    _fields_ = [("device", C.c_int), ("workers", C.c_int), ("max_batch_tables", C.c_int), ("reserved", C.c_int),
+                ("arena_bytes", C.c_uint64), ("max_batch_rows", C.c_uint64)]
+
+
+class VdfServed(C.Structure):
+    _fields_ = [("status", C.c_int), ("batch_tables", C.c_uint32), ("n_groups", C.c_uint64),
+                ("submit_ns", C.c_uint64), ("start_ns", C.c_uint64), ("done_ns", C.c_uint64),
+                ("out", VdfGroupbyOut)]
+
+
+class VdfServerStats(C.Structure):
+    _fields_ = [(k, C.c_uint64) for k in ("submitted", "completed", "failed", "launches", "retries", "rows",
+                                          "queued", "arena_bytes", "arena_used", "arena_peak")]
 
 
 class VdfSortSeg(C.Structure):
@@ -118,6 +138,21 @@ def load() -> C.CDLL:
     lib.vdf_groupby_batch.restype = i32
     lib.vdf_groupby_batch.argtypes = [vp, i32, C.POINTER(VdfTable), i32, C.POINTER(C.c_int), i32, u32,
                                       C.POINTER(VdfGroupbyOut), vp]
+    lib.vdf_server_create.restype = i32
+    lib.vdf_server_create.argtypes = [C.POINTER(vp), C.POINTER(VdfServerConfig)]
+    lib.vdf_server_destroy.restype = i32
+    lib.vdf_server_destroy.argtypes = [vp]
+    lib.vdf_server_add_table.restype = i32
+    lib.vdf_server_add_table.argtypes = [vp, i32, vp, u64, C.POINTER(C.c_int), C.POINTER(vp), i32, u64,
+                                         C.POINTER(u32)]
+    lib.vdf_server_submit.restype = i32
+    lib.vdf_server_submit.argtypes = [vp, u32, u32, i32, C.POINTER(u64)]
+    lib.vdf_server_wait.restype = i32
+    lib.vdf_server_wait.argtypes = [vp, u64, C.c_int64, C.POINTER(VdfServed)]
+    lib.vdf_server_release.restype = i32
+    lib.vdf_server_release.argtypes = [vp, u64]
+    lib.vdf_server_stats_get.restype = i32
+    lib.vdf_server_stats_get.argtypes = [vp, C.POINTER(VdfServerStats)]
     lib.vdf_tune_set.restype = i32
     lib.vdf_tune_set.argtypes = [vp, C.c_char_p, C.c_int64]
     lib.vdf_tune_get.restype = i32

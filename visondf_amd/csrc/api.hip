@@ -81,6 +81,30 @@ void Handle::staging_end(hipStream_t s)
     staging_slot = -1;
 }
 
+void Handle::release_resources()
+{
+    for (ProfRec& r : recs) {
+        (void)hipEventDestroy(r.beg);
+        (void)hipEventDestroy(r.end);
+    }
+    recs.clear();
+    for (hipEvent_t e : free_events) (void)hipEventDestroy(e);
+    free_events.clear();
+    if (scratch) (void)hipFree(scratch);
+    scratch = nullptr;
+    scratch_cap = 0;
+    if (passes_ring) (void)hipHostFree(passes_ring);
+    passes_ring = nullptr;
+    for (int i = 0; i < kStaging; ++i) {
+        if (staging[i]) (void)hipHostFree(staging[i]);
+        if (staging_done[i]) (void)hipEventDestroy(staging_done[i]);
+        staging[i] = nullptr;
+        staging_done[i] = nullptr;
+        staging_cap[i] = 0;
+        staging_pending[i] = false;
+    }
+}
+
 hipEvent_t Handle::get_event()
 {
     if (!free_events.empty()) {
@@ -153,41 +177,6 @@ struct StagingScope {
     }
 };
 
-template <typename F>
-static vdf_status guarded(F&& f)
-{
-    try {
-        f();
-        return VDF_OK;
-    } catch (const Error& e) {
-        set_error(e.msg);
-        return e.code;
-    } catch (const std::bad_alloc&) {
-        set_error("host allocation failed");
-        return VDF_ERR_OOM;
-    } catch (const std::exception& e) {
-        set_error(e.what());
-        return VDF_ERR_INVALID;
-    }
-}
-
-static void require(bool ok, const char* msg)
-{
-    if (!ok) throw Error{VDF_ERR_INVALID, msg};
-}
-
-static int val_bytes(int t)
-{
-    switch (t) {
-    case VDF_VAL_F64:
-    case VDF_VAL_I64:
-    case VDF_VAL_U64: return 8;
-    case VDF_VAL_I32:
-    case VDF_VAL_U32: return 4;
-    default: return 0;
-    }
-}
-
 }  // namespace vdf
 
 using namespace vdf;
@@ -222,17 +211,7 @@ vdf_status vdf_destroy(vdf_handle* h)
         {
             CallScope cs(h);
             VDF_HIP_CHECK(hipDeviceSynchronize());
-            for (ProfRec& r : h->recs) {
-                (void)hipEventDestroy(r.beg);
-                (void)hipEventDestroy(r.end);
-            }
-            for (hipEvent_t e : h->free_events) (void)hipEventDestroy(e);
-            if (h->scratch) (void)hipFree(h->scratch);
-            if (h->passes_ring) (void)hipHostFree(h->passes_ring);
-            for (int i = 0; i < Handle::kStaging; ++i) {
-                if (h->staging[i]) (void)hipHostFree(h->staging[i]);
-                if (h->staging_done[i]) (void)hipEventDestroy(h->staging_done[i]);
-            }
+            h->release_resources();
         }
         delete h;
     });
@@ -388,6 +367,7 @@ static int* tune_slot(Tuning& t, const std::string& n)
     if (n == "sort_items_32") return &t.sort_items_32;
     if (n == "sort_items_64") return &t.sort_items_64;
     if (n == "sort_chunk") return &t.sort_chunk;
+    if (n == "sort_xcd") return &t.sort_xcd;
     if (n == "hist_rows") return &t.hist_rows;
     if (n == "gather_rows") return &t.gather_rows;
     if (n == "gather_nt") return &t.gather_nt;

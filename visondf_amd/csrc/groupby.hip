@@ -172,6 +172,7 @@ struct GbSeg {
     const uint32_t* perm;    // the table's permutation (local row ids)
     uint32_t gtile_off;      // first group-by tile
     uint32_t ntiles;         // group-by tiles of the table
+    uint64_t cap;            // groups the outputs hold: ids >= cap are not written
     const void* vals[VDF_MAX_VALUE_COLS];
     vdf_groupby_out out;
 };
@@ -203,6 +204,7 @@ __device__ inline uint64_t load_val(int type, const void* data, uint64_t row)
 
 __device__ inline void write_final(const GbArgs& a, const GbSeg& sg, int c, uint32_t gid, const Agg& s)
 {
+    if (gid >= sg.cap) return;
     const GbColType col = a.cols[c];
     if (c == 0 && (a.agg_mask & VDF_AGG_COUNT) && sg.out.counts) sg.out.counts[gid] = static_cast<uint32_t>(s.cnt);
     if (col.kind == KIND_COUNT) return;
@@ -322,7 +324,7 @@ __global__ __launch_bounds__(kBlock) void k_gb_tile(GbArgs a)
         a.tile_heads[tile] = H;
         if (lt == sg.ntiles - 1) {
             *sg.out.n_groups = static_cast<uint64_t>(excl) + H;
-            sg.out.offsets[excl + H] = static_cast<uint32_t>(n);
+            if (excl + H <= sg.cap) sg.out.offsets[excl + H] = static_cast<uint32_t>(n);
         }
     }
     __syncthreads();
@@ -336,8 +338,10 @@ __global__ __launch_bounds__(kBlock) void k_gb_tile(GbArgs a)
             const int k = __ffs(bits) - 1;
             bits &= bits - 1;
             const uint64_t r = tbase + lr0 + k;
-            sg.out.offsets[g] = static_cast<uint32_t>(r);
-            if (sg.out.group_keys) static_cast<K*>(sg.out.group_keys)[g] = keys[r];
+            if (g < sg.cap) {
+                sg.out.offsets[g] = static_cast<uint32_t>(r);
+                if (sg.out.group_keys) static_cast<K*>(sg.out.group_keys)[g] = keys[r];
+            }
             ++g;
         }
     }
@@ -578,6 +582,7 @@ void groupby_batch_enqueue(Handle* h, int key_type, const GbTable* tables, int n
         g.perm = perms[i];
         g.gtile_off = static_cast<uint32_t>(tiles);
         g.ntiles = static_cast<uint32_t>(nt);
+        g.cap = (outs[i].capacity == 0 || outs[i].capacity > n) ? n : outs[i].capacity;
         for (int c = 0; c < nvals; ++c) g.vals[c] = tables[i].vals[c];
         g.out = outs[i];
         hs[i] = g;

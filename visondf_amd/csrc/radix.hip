@@ -336,10 +336,13 @@ struct TileCtx {
 // consecutive blocks on consecutive XCDs, so a group is one XCD.  The k-th
 // claim of group g gets tile (k / chunk * groups + g) * chunk + k % chunk:
 // runs of `chunk` consecutive tiles are written by CUs sharing one L2, which
-// merges the partial lines of adjacent bucket runs.  The kernel is launched
-// cooperatively, so every block is resident; each group claims its tiles in
-// increasing order and every claimed tile publishes its aggregate before its
-// block waits, so the smallest unfinished tile always completes: no deadlock.
+// merges the partial lines of adjacent bucket runs.  With several groups the
+// kernel is launched cooperatively, so every block is resident; each group
+// claims its tiles in increasing order and every claimed tile publishes its
+// aggregate before its block waits, so the smallest unfinished tile always
+// completes: no deadlock.  With one group and chunk 1 the claimed tiles are
+// always a prefix of the tile order held by running blocks, which is
+// deadlock-free at any residency (plain launch).
 //
 // Per tile: load (coalesced, striped rows), stable rank inside each wave,
 // publish the tile's per-bucket counts (flag A; a segment's first tile
@@ -697,6 +700,7 @@ bool tune_supported(const char* name, int64_t v)
     if (n == "sort_items_32") return pick_pass<uint32_t>(static_cast<int>(v)) != nullptr;
     if (n == "sort_items_64") return pick_pass<uint64_t>(static_cast<int>(v)) != nullptr;
     if (n == "sort_chunk") return v >= 1 && v <= 1024;
+    if (n == "sort_xcd") return v == 0 || v == 1;
     if (n == "hist_rows") return v >= 4096 && v <= (1 << 24);
     if (n == "gather_rows") return v == 4 || v == 8 || v == 16;
     if (n == "gather_nt") return v == 0 || v == 1;
@@ -792,11 +796,21 @@ static void launch_sort(Handle* h, const SortSeg* segs, int nsegs, uint64_t mlo,
     a.reverse = (flags & VDF_SORT_DESC_REVERSE) ? 1u : 0u;
     // persistent grid: every resident block, a multiple of the group count
     unsigned grid = static_cast<unsigned>(std::min<uint64_t>(resident_blocks(pass, h->device), ntiles));
-    a.groups = std::min(8u, grid);
-    grid = grid / a.groups * a.groups;
-    // enough chunks that every group gets several
-    a.chunk = static_cast<uint32_t>(std::max<uint64_t>(
-        1, std::min<uint64_t>(static_cast<uint64_t>(h->tune.sort_chunk), ntiles / (static_cast<uint64_t>(a.groups) * 8))));
+    const bool xcd = h->tune.sort_xcd != 0;
+    if (xcd) {
+        a.groups = std::min(8u, grid);
+        grid = grid / a.groups * a.groups;
+        // enough chunks that every group gets several
+        a.chunk = static_cast<uint32_t>(std::max<uint64_t>(
+            1, std::min<uint64_t>(static_cast<uint64_t>(h->tune.sort_chunk),
+                                  ntiles / (static_cast<uint64_t>(a.groups) * 8))));
+    } else {
+        // one group, chunk 1: tiles are claimed in global order, so the claimed
+        // tiles are always a prefix held by running blocks and no co-residency
+        // is needed
+        a.groups = 1;
+        a.chunk = 1;
+    }
 
     const size_t kb = sizeof(K);
     {
@@ -834,8 +848,11 @@ static void launch_sort(Handle* h, const SortSeg* segs, int nsegs, uint64_t mlo,
                       pass_bytes);
         uint32_t sl = static_cast<uint32_t>(slot);
         void* args[] = {&a, &sl};
-        VDF_HIP_CHECK(
-            hipLaunchCooperativeKernel(reinterpret_cast<const void*>(pass), dim3(grid), dim3(kBlock), args, 0, s));
+        if (xcd)
+            VDF_HIP_CHECK(
+                hipLaunchCooperativeKernel(reinterpret_cast<const void*>(pass), dim3(grid), dim3(kBlock), args, 0, s));
+        else
+            VDF_HIP_CHECK(hipLaunchKernel(reinterpret_cast<const void*>(pass), dim3(grid), dim3(kBlock), args, 0, s));
     }
 }
 

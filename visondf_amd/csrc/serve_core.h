@@ -1,0 +1,174 @@
+// serve_core.h -- host-only parts of the group-by request server: the range
+// allocator that carves the server's HBM arena, and the planner that picks
+// the next batch of queued requests.  No HIP here; the same code is compiled
+// into libvdf_hip.so and into a plain g++ unit test.
+//
+// No reference counterpart: VisonDF runs one group-by per call on the calling
+// thread (transform_group_by.hpp:11-14) and has no request loop.
+#pragma once
+
+#include <algorithm>
+#include <cstdint>
+#include <iterator>
+#include <map>
+#include <vector>
+
+namespace vdf {
+
+// First-fit allocator of byte ranges inside one arena of `size` bytes.  Every
+// range starts on an `align` boundary and its length is rounded up to one.
+// Adjacent free ranges are merged on free, so a drained arena is one range.
+class RangeAllocator {
+public:
+    static constexpr uint64_t kNone = ~0ull;
+
+    explicit RangeAllocator(uint64_t size = 0, uint64_t align = 256) : align_(align) { reset(size); }
+
+    void reset(uint64_t size)
+    {
+        size_ = size / align_ * align_;
+        free_.clear();
+        live_.clear();
+        used_ = peak_ = 0;
+        if (size_) free_[0] = size_;
+    }
+
+    uint64_t round(uint64_t bytes) const { return (std::max<uint64_t>(bytes, 1) + align_ - 1) / align_ * align_; }
+
+    // Offset of a new range of at least `bytes`, or kNone when no free range is
+    // large enough.
+    uint64_t alloc(uint64_t bytes)
+    {
+        const uint64_t need = round(bytes);
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < need) continue;
+            const uint64_t off = it->first;
+            const uint64_t rest = it->second - need;
+            free_.erase(it);
+            if (rest) free_[off + need] = rest;
+            live_[off] = need;
+            used_ += need;
+            peak_ = std::max(peak_, used_);
+            return off;
+        }
+        return kNone;
+    }
+
+    // Returns false for an offset that is not the start of a live range.
+    bool release(uint64_t off)
+    {
+        auto it = live_.find(off);
+        if (it == live_.end()) return false;
+        uint64_t beg = off, len = it->second;
+        live_.erase(it);
+        used_ -= len;
+        auto nx = free_.lower_bound(beg);
+        if (nx != free_.end() && nx->first == beg + len) {
+            len += nx->second;
+            nx = free_.erase(nx);
+        }
+        if (nx != free_.begin()) {
+            auto pv = std::prev(nx);
+            if (pv->first + pv->second == beg) {
+                beg = pv->first;
+                len += pv->second;
+                free_.erase(pv);
+            }
+        }
+        free_[beg] = len;
+        return true;
+    }
+
+    uint64_t size() const { return size_; }
+    uint64_t used() const { return used_; }
+    uint64_t peak() const { return peak_; }
+    size_t live_ranges() const { return live_.size(); }
+    size_t free_ranges() const { return free_.size(); }
+    uint64_t largest_free() const
+    {
+        uint64_t m = 0;
+        for (const auto& f : free_) m = std::max(m, f.second);
+        return m;
+    }
+
+private:
+    uint64_t align_;
+    uint64_t size_ = 0;
+    uint64_t used_ = 0, peak_ = 0;
+    std::map<uint64_t, uint64_t> free_;  // offset -> length
+    std::map<uint64_t, uint64_t> live_;  // offset -> length
+};
+
+// One queued request as the planner sees it.
+struct PlanItem {
+    uint64_t rows;       // rows of the request's table
+    uint64_t sig;        // batch signature: requests of one batch share it
+    uint64_t out_bytes;  // bytes of the request's result block
+};
+
+struct PlanLimits {
+    size_t max_tables = 64;          // requests per batch
+    uint64_t max_rows = 1ull << 30;  // rows per batch (the oldest request is always taken)
+};
+
+struct Plan {
+    std::vector<size_t> picked;    // queue positions, increasing
+    std::vector<uint64_t> out_off; // arena offset of each picked request's result block
+    uint64_t scr_off = RangeAllocator::kNone;
+    uint64_t scr_bytes = 0;
+    bool head_never_fits = false;  // the oldest request alone exceeds the whole arena
+};
+
+// Picks the next batch: the oldest queued request and, in queue order, later
+// requests with its signature while the batch stays within the limits; each
+// picked request gets its result block, then the batch gets one scratch block
+// of scratch_bytes(rows of the picked requests).  Memory is taken in queue
+// order and the scan stops at the first request whose block does not fit, so
+// a large old request is never passed by newer ones for memory; when the
+// scratch block does not fit, the newest picks are dropped until it does.  An
+// empty plan means nothing can start until memory is released (or, with
+// head_never_fits, that the oldest request can never run in this arena).
+template <typename ScratchFn>
+Plan plan_batch(const std::vector<PlanItem>& queue, const PlanLimits& lim, RangeAllocator& arena,
+                ScratchFn scratch_bytes)
+{
+    Plan p;
+    if (queue.empty()) return p;
+    const PlanItem& head = queue[0];
+    {
+        std::vector<uint64_t> one{head.rows};
+        if (arena.round(head.out_bytes) + arena.round(scratch_bytes(one)) > arena.size()) {
+            p.head_never_fits = true;
+            return p;
+        }
+    }
+    uint64_t rows = 0;
+    std::vector<uint64_t> ns;
+    for (size_t i = 0; i < queue.size() && p.picked.size() < lim.max_tables; ++i) {
+        const PlanItem& q = queue[i];
+        if (q.sig != head.sig) continue;
+        if (i > 0 && rows + q.rows > lim.max_rows) continue;
+        const uint64_t off = arena.alloc(q.out_bytes);
+        if (off == RangeAllocator::kNone) break;
+        p.picked.push_back(i);
+        p.out_off.push_back(off);
+        ns.push_back(q.rows);
+        rows += q.rows;
+    }
+    while (!p.picked.empty()) {
+        const uint64_t sb = scratch_bytes(ns);
+        const uint64_t off = arena.alloc(sb);
+        if (off != RangeAllocator::kNone) {
+            p.scr_off = off;
+            p.scr_bytes = sb;
+            return p;
+        }
+        arena.release(p.out_off.back());
+        p.out_off.pop_back();
+        p.picked.pop_back();
+        ns.pop_back();
+    }
+    return p;
+}
+
+}  // namespace vdf

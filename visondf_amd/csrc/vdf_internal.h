@@ -4,7 +4,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <exception>
 #include <mutex>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -21,6 +23,43 @@ struct Error {
     vdf_status code;
     std::string msg;
 };
+
+// Runs f, turning exceptions into a status code plus the thread's message.
+template <typename F>
+vdf_status guarded(F&& f)
+{
+    try {
+        f();
+        return VDF_OK;
+    } catch (const Error& e) {
+        set_error(e.msg);
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        set_error("host allocation failed");
+        return VDF_ERR_OOM;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return VDF_ERR_INVALID;
+    }
+}
+
+inline void require(bool ok, const char* msg)
+{
+    if (!ok) throw Error{VDF_ERR_INVALID, msg};
+}
+
+// bytes of one cell of a value column type (0: unknown type)
+inline int val_bytes(int t)
+{
+    switch (t) {
+    case VDF_VAL_F64:
+    case VDF_VAL_I64:
+    case VDF_VAL_U64: return 8;
+    case VDF_VAL_I32:
+    case VDF_VAL_U32: return 4;
+    default: return 0;
+    }
+}
 
 #define VDF_HIP_CHECK(expr)                                                              \
     do {                                                                                 \
@@ -76,6 +115,8 @@ struct Tuning {
     int sort_items_32 = 16;  // rows per thread of a radix pass tile, 4-byte keys
     int sort_items_64 = 12;  // same, 8-byte keys
     int sort_chunk = 64;     // consecutive tiles a tile group (one XCD) claims at a time
+    int sort_xcd = 1;        // 1: XCD-grouped tile order, cooperative launch; 0: global tile
+                             //    order, plain launch (safe beside other persistent work)
     int hist_rows = 65536;   // rows per histogram work item
     int gather_rows = 8;     // rows per thread of the gather kernel
     int gather_nt = 0;       // non-temporal loads of the gathered column
@@ -117,6 +158,7 @@ struct Handle {
     void staging_end(hipStream_t s);       // the region's uploads are enqueued on s
     hipEvent_t get_event();
     void prof_collect();
+    void release_resources();  // events, scratch, pinned buffers (work must be finished)
 };
 
 // scoped launch timer: records events around a launch when profiling is on

@@ -14,6 +14,8 @@ Workloads (row counts as used by the benchmark and the parity tests):
                   2^24 values, f64 value N(0, 1), int64 value uniform [0, 2^40)
   batched_tables  table row counts log-uniform in [lo, hi], int32 keys uniform
                   in [0, rows / 8), f64 values uniform [0, 1)
+  served_tables   row counts log-uniform in [10K, 50M]; uint64 or 16-byte ASCII
+                  keys from rows / 16 distinct values, f64 values uniform [0, 1)
 """
 from __future__ import annotations
 
@@ -155,6 +157,29 @@ def batched_table(rows: int, table_id: int, seed: int = 4):
     key = uniform_int(seed * 1_000_003 + table_id, 1, rows, 0, max(rows // 8, 1) - 1).astype(np.int32)
     val = uniform01(seed * 1_000_003 + table_id, 2, rows)
     return key, val
+
+
+def served_table_sizes(ntables: int = 64, lo: int = 10_000, hi: int = 50_000_000, seed: int = 5):
+    """Row counts of the served tables, log-uniform in [lo, hi]."""
+    u = uniform01(seed, 0, ntables)
+    return np.floor(np.exp(np.log(lo) + u * (np.log(hi) - np.log(lo)))).astype(np.int64)
+
+
+def served_table(rows: int, table_id: int, key_kind: str, seed: int = 5):
+    """One served table: `rows` keys drawn from rows / 16 distinct values
+    (uint64 spread over all 64 bits, or 16-byte printable ASCII) and a f64
+    value column uniform in [0, 1)."""
+    s = seed * 1_000_003 + table_id
+    distinct = max(rows // 16, 1)
+    if key_kind == "u64":
+        ids = splitmix64(s, 1, rows) % np.uint64(distinct)
+        with np.errstate(over="ignore"):
+            key = _mix(ids * _GOLDEN + np.uint64(s))
+    elif key_kind == "char16":
+        key = char16_keys(s, 1, rows, distinct=distinct)
+    else:
+        raise ValueError(key_kind)
+    return key, uniform01(s, 2, rows)
 
 
 def char16_keys(seed: int, col: int, n: int, distinct: int | None = None) -> np.ndarray:

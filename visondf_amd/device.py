@@ -116,19 +116,22 @@ class Vdf:
 
     # ---------------------------------------------------------------- groupby
     @staticmethod
-    def _alloc_out(keys: torch.Tensor, vts, dtypes, aggs: int, want_perm: bool):
-        """Output buffers of one table (capacity n groups) and the C descriptor."""
+    def _alloc_out(keys: torch.Tensor, vts, dtypes, aggs: int, want_perm: bool, capacity: int = 0):
+        """Output buffers of one table (room for `capacity` groups, 0 = n) and the
+        C descriptor."""
         n = keys.shape[0]
+        cap = n if capacity <= 0 else min(int(capacity), n)
         dev = keys.device
         out = N.VdfGroupbyOut()
-        t = {"ng": torch.zeros(1, dtype=torch.int64, device=dev),
-             "keys": torch.empty_like(keys),
-             "offsets": torch.empty(n + 1, dtype=torch.int32, device=dev),
-             "counts": torch.empty(max(n, 1), dtype=torch.int32, device=dev),
+        out.capacity = cap
+        t = {"cap": cap, "ng": torch.zeros(1, dtype=torch.int64, device=dev),
+             "keys": torch.empty((max(cap, 1),) + tuple(keys.shape[1:]), dtype=keys.dtype, device=dev),
+             "offsets": torch.empty(cap + 1, dtype=torch.int32, device=dev),
+             "counts": torch.empty(max(cap, 1), dtype=torch.int32, device=dev),
              "perm": torch.empty(max(n, 1), dtype=torch.int32, device=dev) if want_perm else None,
              "aggs": []}
         out.n_groups = t["ng"].data_ptr()
-        out.group_keys = t["keys"].data_ptr() if n else None
+        out.group_keys = t["keys"].data_ptr() if cap else None
         out.offsets = t["offsets"].data_ptr()
         out.counts = t["counts"].data_ptr()
         out.perm = t["perm"].data_ptr() if want_perm else None
@@ -136,16 +139,16 @@ class Vdf:
             d = {}
             sum_dt = torch.float64 if vt == N.VAL_F64 else torch.int64
             if aggs & N.AGG_SUM:
-                d["sum"] = torch.empty(max(n, 1), dtype=sum_dt, device=dev)
+                d["sum"] = torch.empty(max(cap, 1), dtype=sum_dt, device=dev)
                 out.aggs[c].sum = d["sum"].data_ptr()
             if aggs & N.AGG_MEAN:
-                d["mean"] = torch.empty(max(n, 1), dtype=torch.float64, device=dev)
+                d["mean"] = torch.empty(max(cap, 1), dtype=torch.float64, device=dev)
                 out.aggs[c].mean = d["mean"].data_ptr()
             if aggs & N.AGG_MIN:
-                d["min"] = torch.empty(max(n, 1), dtype=dt, device=dev)
+                d["min"] = torch.empty(max(cap, 1), dtype=dt, device=dev)
                 out.aggs[c].min = d["min"].data_ptr()
             if aggs & N.AGG_MAX:
-                d["max"] = torch.empty(max(n, 1), dtype=dt, device=dev)
+                d["max"] = torch.empty(max(cap, 1), dtype=dt, device=dev)
                 out.aggs[c].max = d["max"].data_ptr()
             t["aggs"].append(d)
         return out, t
@@ -155,16 +158,19 @@ class Vdf:
         if not trim:
             return GroupbyResult(t["ng"], t["keys"], t["offsets"], t["counts"], t["perm"], t["aggs"])
         g = int(t["ng"].item())
+        if g > t["cap"]:
+            raise N.VdfError(N.VDF_ERR_INVALID, f"{g} groups exceed the output capacity {t['cap']}")
         return GroupbyResult(g, t["keys"][:g], t["offsets"][:g + 1], t["counts"][:g],
                              t["perm"][:n] if t["perm"] is not None else None,
                              [{k: x[:g] for k, x in d.items()} for d in t["aggs"]])
 
     def groupby(self, keys: torch.Tensor, vals=(), aggs: int = N.AGG_ALL, want_perm: bool = True,
-                stream=None, trim: bool = True, key_type: int | None = None, val_types=None):
+                stream=None, trim: bool = True, key_type: int | None = None, val_types=None,
+                capacity: int = 0):
         kt = key_type_of(keys) if key_type is None else key_type
         n = keys.shape[0]
         vts = [val_type_of(v) if val_types is None else val_types[c] for c, v in enumerate(vals)]
-        out, t = self._alloc_out(keys, vts, [v.dtype for v in vals], aggs, want_perm)
+        out, t = self._alloc_out(keys, vts, [v.dtype for v in vals], aggs, want_perm, capacity)
         cols = (N.VdfValueCol * max(len(vals), 1))()
         for c, v in enumerate(vals):
             cols[c].type = vts[c]
@@ -174,8 +180,9 @@ class Vdf:
         return self._result(t, n, trim)
 
     def groupby_batch(self, tables, aggs: int = N.AGG_ALL, want_perm: bool = True, stream=None,
-                      trim: bool = True, key_type: int | None = None, val_types=None):
-        """tables: list of (keys, [value columns]); all tables share key and value types."""
+                      trim: bool = True, key_type: int | None = None, val_types=None, capacities=None):
+        """tables: list of (keys, [value columns]); all tables share key and value types.
+        capacities: per-table group capacity of the outputs (0 / None = rows)."""
         if not tables:
             return []
         k0, v0 = tables[0]
@@ -190,7 +197,8 @@ class Vdf:
             tabs[i].n = k.shape[0]
             for c, v in enumerate(vs):
                 tabs[i].vals[c] = v.data_ptr() if k.shape[0] else None
-            o, t = self._alloc_out(k, vts, [v.dtype for v in vs], aggs, want_perm)
+            o, t = self._alloc_out(k, vts, [v.dtype for v in vs], aggs, want_perm,
+                                   capacities[i] if capacities is not None else 0)
             outs[i] = o
             keep.append(t)
         vt_arr = (C.c_int * max(nv, 1))(*vts)
