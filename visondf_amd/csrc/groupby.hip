@@ -14,15 +14,17 @@
 //      batch) -> perm (row ids in key order, rows in row order within a key)
 //      and the sorted keys, concatenated in scratch
 //   2. k_gb_tile: one 256-thread block per 4096-row tile of a table (dynamic
-//      tile ids over the tiles of all tables):
-//        head flags (key != previous key) by wave ballots; head count of the
-//        tile published to a decoupled look-back array -> global group ids;
-//        group keys and offsets written at heads; per value column the
-//        values are gathered through perm (striped, coalesced row ids),
-//        staged in padded LDS, reduced sequentially over 16 consecutive rows
-//        per thread, then a block segmented scan joins the threads.  Groups
-//        that start and end inside the tile are final here; each tile leaves
-//        two partials (before its first head, after its last head).
+//      tile ids over the tiles of all tables); each lane owns 16 consecutive
+//      sorted rows (16-byte vector loads; table starts are 16-row aligned in
+//      scratch):
+//        head flags (key != previous key) per lane; head count of the tile
+//        (wave shuffles + 4 LDS words) published to a decoupled look-back
+//        array -> global group ids; group keys and offsets written at heads;
+//        per value column the lane gathers its 16 values through perm,
+//        reduces them sequentially, and a wave segmented scan by shuffles
+//        (cross-wave carry through LDS) joins the lanes.  Groups that start
+//        and end inside the tile are final here; each tile leaves two
+//        partials (before its first head, after its last head).
 //   3. k_gb_fixup: one block per tile owning the group that starts at the
 //        tile's last head; joins the following tiles' leading partials in
 //        tile order with a fixed reduction tree.
@@ -38,7 +40,6 @@ namespace {
 
 constexpr int kGbRows = 16;                 // consecutive rows per thread
 constexpr int kGbTile = kBlock * kGbRows;   // 4096 rows per tile
-constexpr int kGbPad = kGbTile + kGbTile / 16;
 
 enum : int { KIND_F64 = 0, KIND_SIGNED = 1, KIND_UNSIGNED = 2, KIND_COUNT = 3 };
 
@@ -244,15 +245,193 @@ __device__ inline uint32_t load_relaxed(uint32_t* p)
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__device__ inline int pad(int i) { return i + (i >> 4); }
+// 64-bit wave shuffle as two 32-bit ones
+__device__ inline uint64_t shfl_up64(uint64_t x, int off)
+{
+    const uint32_t lo = __shfl_up(static_cast<uint32_t>(x), off, 64);
+    const uint32_t hi = __shfl_up(static_cast<uint32_t>(x >> 32), off, 64);
+    return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
+template <int KIND>
+__device__ inline Agg shfl_up_agg(const Agg& x, int off)
+{
+    Agg y = agg_empty();
+    y.cnt = shfl_up64(x.cnt, off);
+    if (KIND != KIND_COUNT) {
+        y.a = shfl_up64(x.a, off);
+        y.b = shfl_up64(x.b, off);
+        y.mn = shfl_up64(x.mn, off);
+        y.mx = shfl_up64(x.mx, off);
+    }
+    return y;
+}
+
+// The segmented-scan operator: (f1, x1) then (f2, x2) -> (f1 | f2, f2 ? x2 : x1 + x2).
+template <int KIND>
+__device__ inline void seg_combine(Agg& x, uint32_t& f, const Agg& y, uint32_t fy)
+{
+    // (y, fy) precedes (x, f)
+    if (!f) x = agg_join(y, x, KIND);
+    f |= fy;
+}
+
+// Loads this lane's rows p[0 .. cnt) (cnt <= kGbRows) of 4-byte words; 16-byte
+// vector loads when the lane has all 16 rows and p is 16-byte aligned.
+__device__ inline void load_rows_u32(const uint32_t* p, uint32_t cnt, uint32_t (&o)[kGbRows])
+{
+    if (cnt == kGbRows && (reinterpret_cast<uintptr_t>(p) & 15u) == 0) {
+        const uint4* q = reinterpret_cast<const uint4*>(p);
+#pragma unroll
+        for (int i = 0; i < kGbRows / 4; ++i) {
+            const uint4 t = q[i];
+            o[4 * i + 0] = t.x;
+            o[4 * i + 1] = t.y;
+            o[4 * i + 2] = t.z;
+            o[4 * i + 3] = t.w;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < kGbRows; ++j) o[j] = (static_cast<uint32_t>(j) < cnt) ? p[j] : 0u;
+    }
+}
+
+// Head bits of this lane's rows (bit j: row r0 + j starts a group).  Rows are
+// read in 4-row chunks; a chunk is one or more 16-byte vector loads when the
+// lane has all 16 rows (table starts are 16-row aligned in scratch).
+template <typename K>
+__device__ inline uint32_t head_bits(const K* keys, uint64_t r0, uint32_t cnt)
+{
+    if (cnt == 0) return 0u;
+    K prev = keys[r0 == 0 ? 0 : r0 - 1];
+    uint32_t hb = 0;
+    if (cnt == kGbRows) {
+        constexpr int V = static_cast<int>(4 * sizeof(K) / 16);  // uint4 per 4 rows
+        const uint4* q = reinterpret_cast<const uint4*>(keys + r0);
+#pragma unroll
+        for (int c = 0; c < kGbRows / 4; ++c) {
+            uint4 t[V];
+#pragma unroll
+            for (int i = 0; i < V; ++i) t[i] = q[c * V + i];
+            K k[4];
+            __builtin_memcpy(k, t, sizeof(k));
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int j = 4 * c + i;
+                if (!keq(k[i], prev) || (r0 + j == 0)) hb |= 1u << j;
+                prev = k[i];
+            }
+        }
+    } else {
+        for (uint32_t j = 0; j < cnt; ++j) {
+            const K k = keys[r0 + j];
+            if (!keq(k, prev) || (r0 + j == 0)) hb |= 1u << j;
+            prev = k;
+        }
+    }
+    return hb;
+}
+
+// One value column of a tile: gather through perm, sequential reduce over the
+// lane's 16 rows, wave segmented scan by shuffles, cross-wave carry through LDS.
+template <int KIND>
+__device__ __attribute__((always_inline)) inline void gb_column(const GbArgs& a, const GbSeg& sg, int c, uint32_t tile, uint64_t tbase, uint32_t cnt,
+                          uint32_t hb, uint32_t gfirst, Agg* s_wagg, uint32_t* s_wflag)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint64_t r0 = tbase + static_cast<uint64_t>(tid) * kGbRows;
+    uint64_t v[kGbRows];
+    if (KIND != KIND_COUNT) {
+        uint32_t p[kGbRows];
+        load_rows_u32(sg.perm + r0, cnt, p);
+        const void* vd = sg.vals[c];
+        const int type = a.cols[c].type;
+#pragma unroll
+        for (int j = 0; j < kGbRows; ++j) v[j] = (static_cast<uint32_t>(j) < cnt) ? load_val(type, vd, p[j]) : 0ull;
+    }
+    Agg cur = agg_empty(), first = agg_empty();
+    bool seen = false;
+    uint32_t g = gfirst;  // id of the next head's group
+#pragma unroll
+    for (int j = 0; j < kGbRows; ++j) {
+        if (static_cast<uint32_t>(j) < cnt) {
+            if ((hb >> j) & 1u) {
+                if (!seen) {
+                    first = cur;
+                    seen = true;
+                } else {
+                    write_final(a, sg, c, g - 1, cur);
+                }
+                cur = agg_empty();
+                ++g;
+            }
+            agg_add(cur, KIND == KIND_COUNT ? 0ull : v[j], KIND);
+        }
+    }
+
+    // inclusive segmented scan over the wave's lanes
+    Agg x = cur;
+    uint32_t f = seen ? 1u : 0u;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const Agg y = shfl_up_agg<KIND>(x, off);
+        const uint32_t fy = __shfl_up(f, off, 64);
+        if (lane >= off) seg_combine<KIND>(x, f, y, fy);
+    }
+    if (lane == 63) {
+        s_wagg[wave] = x;
+        s_wflag[wave] = f;
+    }
+    Agg ex = shfl_up_agg<KIND>(x, 1);
+    uint32_t exf = __shfl_up(f, 1, 64);
+    if (lane == 0) {
+        ex = agg_empty();
+        exf = 0;
+    }
+    __syncthreads();
+    // carry from the waves before this one, in wave order
+    Agg wc = agg_empty();
+    uint32_t wcf = 0;
+    for (int i = 0; i < wave; ++i) {
+        if (s_wflag[i]) {
+            wc = s_wagg[i];
+            wcf = 1;
+        } else {
+            wc = agg_join(wc, s_wagg[i], KIND);
+        }
+    }
+    // exclusive carry into this lane
+    Agg carry = ex;
+    uint32_t cf = exf;
+    if (!exf) {
+        carry = agg_join(wc, ex, KIND);
+        cf = wcf;
+    }
+    Agg* pre = a.prefix + static_cast<uint64_t>(c) * a.ntiles;
+    Agg* suf = a.suffix + static_cast<uint64_t>(c) * a.ntiles;
+    if (seen) {
+        const Agg seg = agg_join(carry, first, KIND);
+        if (cf) write_final(a, sg, c, gfirst - 1, seg);
+        else pre[tile] = seg;  // the tile's first head: rows before it
+    }
+    if (tid == kBlock - 1) {
+        Agg bx = x;
+        uint32_t bf = f;
+        if (!f) {
+            bx = agg_join(wc, x, KIND);
+            bf = wcf;
+        }
+        if (bf) suf[tile] = bx;
+        else pre[tile] = bx;
+    }
+    __syncthreads();  // s_wagg is reused by the next column
+}
 
 template <typename K>
 __global__ __launch_bounds__(kBlock) void k_gb_tile(GbArgs a)
 {
-    __shared__ uint64_t s_val[kGbPad];
-    __shared__ uint64_t s_hmask[kGbRows * kWaves];  // head bits of 64 striped rows
-    __shared__ Agg s_scan[kBlock];
-    __shared__ uint32_t s_flag[kBlock];
+    __shared__ Agg s_wagg[kWaves];
+    __shared__ uint32_t s_wflag[kWaves];
     __shared__ uint32_t s_wsum[kWaves];
     __shared__ uint32_t s_tile, s_gbase;
 
@@ -267,36 +446,25 @@ __global__ __launch_bounds__(kBlock) void k_gb_tile(GbArgs a)
     const uint64_t tbase = static_cast<uint64_t>(lt) * kGbTile;
     const K* keys = static_cast<const K*>(a.skeys) + sg.row_off;
 
-    // ---- head flags, striped rows (row = j*256 + tid), packed by ballot ----
-#pragma unroll
-    for (int j = 0; j < kGbRows; ++j) {
-        const uint64_t r = tbase + static_cast<uint64_t>(j) * kBlock + tid;
-        bool hd = false;
-        if (r < n) hd = (r == 0) || !keq(keys[r], keys[r - 1]);
-        const uint64_t m = __ballot(hd);
-        if (lane == 0) s_hmask[j * kWaves + wave] = m;
-    }
-    __syncthreads();
-
-    // ---- this thread's 16 consecutive rows: local rows tid*16 .. tid*16+15 ----
-    const int lr0 = tid * kGbRows;
-    const uint32_t hb = static_cast<uint32_t>(
-        (s_hmask[(lr0 >> 8) * kWaves + ((lr0 & 255) >> 6)] >> (lr0 & 63)) & 0xFFFFull);
+    // ---- this lane's rows: tbase + tid*16 .. +15 (cnt of them inside the table) ----
+    const uint32_t tn = static_cast<uint32_t>(std::min<uint64_t>(kGbTile, n - tbase));
+    const uint32_t lr0 = static_cast<uint32_t>(tid) * kGbRows;
+    const uint32_t cnt = lr0 < tn ? std::min<uint32_t>(kGbRows, tn - lr0) : 0u;
+    const uint64_t r0 = tbase + lr0;
+    const uint32_t hb = head_bits<K>(keys, r0, cnt);
     const uint32_t hcount = __popc(hb);
-    const uint32_t hx = [&] {  // exclusive scan of head counts over the block
-        uint32_t inc = hcount;
+    uint32_t inc = hcount;  // inclusive scan of head counts over the wave
 #pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const uint32_t y = __shfl_up(inc, off, 64);
-            if (lane >= off) inc += y;
-        }
-        if (lane == 63) s_wsum[wave] = inc;
-        __syncthreads();
-        uint32_t pre = 0;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(inc, off, 64);
+        if (lane >= off) inc += y;
+    }
+    if (lane == 63) s_wsum[wave] = inc;
+    __syncthreads();
+    uint32_t pre_w = 0;
 #pragma unroll
-        for (int i = 0; i < kWaves; ++i) pre += (i < wave) ? s_wsum[i] : 0u;
-        return pre + inc - hcount;
-    }();
+    for (int i = 0; i < kWaves; ++i) pre_w += (i < wave) ? s_wsum[i] : 0u;
+    const uint32_t hx = pre_w + inc - hcount;  // heads before this lane in the tile
     const uint32_t H = s_wsum[0] + s_wsum[1] + s_wsum[2] + s_wsum[3];
 
     // ---- decoupled look-back over the table's tiles' head counts -> first group id ----
@@ -328,16 +496,16 @@ __global__ __launch_bounds__(kBlock) void k_gb_tile(GbArgs a)
         }
     }
     __syncthreads();
-    const uint32_t gbase = s_gbase;
+    const uint32_t gfirst = s_gbase + hx;
 
     // ---- group keys and offsets at the heads ----
     {
         uint32_t bits = hb;
-        uint32_t g = gbase + hx;
+        uint32_t g = gfirst;
         while (bits) {
             const int k = __ffs(bits) - 1;
             bits &= bits - 1;
-            const uint64_t r = tbase + lr0 + k;
+            const uint64_t r = r0 + k;
             if (g < sg.cap) {
                 sg.out.offsets[g] = static_cast<uint32_t>(r);
                 if (sg.out.group_keys) static_cast<K*>(sg.out.group_keys)[g] = keys[r];
@@ -346,80 +514,16 @@ __global__ __launch_bounds__(kBlock) void k_gb_tile(GbArgs a)
         }
     }
 
-    // ---- per value column: gather, sequential reduce, block segmented scan ----
-    const uint32_t tn = static_cast<uint32_t>(std::min<uint64_t>(kGbTile, n - tbase));
+    // ---- per value column ----
     for (int c = 0; c < a.ncols; ++c) {
-        const GbColType col = a.cols[c];
-        const int kind = col.kind;
-        if (kind != KIND_COUNT) {
-            const void* vdata = sg.vals[c];
-#pragma unroll
-            for (int j = 0; j < kGbRows; ++j) {
-                const int li = j * kBlock + tid;
-                const uint64_t r = tbase + li;
-                if (r < n) s_val[pad(li)] = load_val(col.type, vdata, sg.perm[r]);
-            }
+        switch (a.cols[c].kind) {
+        case KIND_F64: gb_column<KIND_F64>(a, sg, c, tile, tbase, cnt, hb, gfirst, s_wagg, s_wflag); break;
+        case KIND_SIGNED: gb_column<KIND_SIGNED>(a, sg, c, tile, tbase, cnt, hb, gfirst, s_wagg, s_wflag); break;
+        case KIND_UNSIGNED:
+            gb_column<KIND_UNSIGNED>(a, sg, c, tile, tbase, cnt, hb, gfirst, s_wagg, s_wflag);
+            break;
+        default: gb_column<KIND_COUNT>(a, sg, c, tile, tbase, cnt, hb, gfirst, s_wagg, s_wflag); break;
         }
-        __syncthreads();
-
-        Agg cur = agg_empty(), first = agg_empty();
-        bool seen = false;
-        uint32_t g = gbase + hx;  // id of the next head's group
-#pragma unroll
-        for (int j = 0; j < kGbRows; ++j) {
-            const int li = lr0 + j;
-            if (static_cast<uint32_t>(li) < tn) {
-                if ((hb >> j) & 1u) {
-                    if (!seen) {
-                        first = cur;
-                        seen = true;
-                    } else {
-                        write_final(a, sg, c, g - 1, cur);
-                    }
-                    cur = agg_empty();
-                    ++g;
-                }
-                agg_add(cur, kind == KIND_COUNT ? 0ull : s_val[pad(li)], kind);
-            }
-        }
-
-        // segmented inclusive scan of (has head, tail partial) over threads
-        Agg x = cur;
-        uint32_t f = seen ? 1u : 0u;
-        s_scan[tid] = x;
-        s_flag[tid] = f;
-        __syncthreads();
-        for (int off = 1; off < kBlock; off <<= 1) {
-            Agg y = agg_empty();
-            uint32_t fy = 0;
-            if (tid >= off) {
-                y = s_scan[tid - off];
-                fy = s_flag[tid - off];
-            }
-            __syncthreads();
-            if (tid >= off) {
-                if (!f) x = agg_join(y, x, kind);
-                f |= fy;
-                s_scan[tid] = x;
-                s_flag[tid] = f;
-            }
-            __syncthreads();
-        }
-        // exclusive carry into this thread
-        const Agg carry = tid > 0 ? s_scan[tid - 1] : agg_empty();
-        const uint32_t carry_f = tid > 0 ? s_flag[tid - 1] : 0u;
-        Agg* pre = a.prefix + static_cast<uint64_t>(c) * a.ntiles;
-        Agg* suf = a.suffix + static_cast<uint64_t>(c) * a.ntiles;
-        if (seen) {
-            const Agg seg = agg_join(carry, first, kind);
-            if (carry_f) write_final(a, sg, c, gbase + hx - 1, seg);
-            else pre[tile] = seg;  // the tile's first head: rows before it
-        }
-        if (tid == kBlock - 1) {
-            if (f) suf[tile] = x;
-            else pre[tile] = x;
-        }
-        __syncthreads();
     }
 }
 
@@ -501,7 +605,7 @@ GbLayout gb_layout(int key_type, const uint64_t* n, int ntables, uint64_t hist_r
 {
     GbLayout L;
     for (int i = 0; i < ntables; ++i) {
-        L.rows += n[i];
+        L.rows += align_up(n[i], kGbRows);  // table starts 16-row aligned (vector loads)
         L.tiles += (n[i] + kGbTile - 1) / kGbTile;
     }
     L.sort_bytes = align_up(argsort_batch_scratch(key_type, n, ntables, hist_rows), 256);
@@ -562,7 +666,7 @@ void groupby_batch_enqueue(Handle* h, int key_type, const GbTable* tables, int n
         uint32_t* p = outs[i].perm ? outs[i].perm : perm_scr + row;
         perms[i] = p;
         ss[i] = SortSeg{tables[i].keys, tables[i].n, p, skeys + row * kb};
-        row += tables[i].n;
+        row += align_up(tables[i].n, kGbRows);
     }
     argsort_batch_enqueue(h, key_type, ss.data(), ntables, 0, sort_scr, s);
 
@@ -587,7 +691,7 @@ void groupby_batch_enqueue(Handle* h, int key_type, const GbTable* tables, int n
         g.out = outs[i];
         hs[i] = g;
         for (uint64_t t = 0; t < nt; ++t) ht[tiles + t] = static_cast<uint32_t>(i);
-        row += n;
+        row += align_up(n, kGbRows);
         tiles += nt;
     }
     // dsegs and dtile are consecutive carves with the same alignment as the staging layout
