@@ -191,6 +191,7 @@ vdf_status vdf_groupby_batch(vdf_handle* h, int key_type, const vdf_table* table
  *   "sort_xcd"       1 (default): XCD-grouped tile order, passes launched cooperatively;
  *                    0: global tile order, plain launches (for handles whose sorts run
  *                    concurrently with other persistent work on the device)
+ *   "sort_blocks_per_cu"  radix pass blocks per CU: 0 (default: every resident block) .. 32
  *   "hist_rows"      rows per digit-histogram work item: 4096 .. 2^24
  *   "gather_rows"    rows per thread of the gather kernel: 4, 8, 16
  *   "gather_nt"      1 = non-temporal loads of the gathered column
@@ -198,6 +199,54 @@ vdf_status vdf_groupby_batch(vdf_handle* h, int key_type, const vdf_table* table
  * count at compile time (radix_sort_uint32_mt.inl:3-5). */
 vdf_status vdf_tune_set(vdf_handle* h, const char* name, int64_t value);
 vdf_status vdf_tune_get(vdf_handle* h, const char* name, int64_t* value);
+
+/* ---- autotuner ----
+ * Times the caller's workload (one ordinary call first validates the
+ * arguments) under candidate launch shapes and stores the fastest in the
+ * handle's tuning table: rows per thread of a radix pass tile (tile size),
+ * radix pass blocks per CU (block count), XCD tile order and chunk, rows per
+ * histogram work item.  The digit width is fixed at 8 bits (256 buckets, one
+ * per thread of a 256-thread block).  Coordinate descent, each candidate timed
+ * as the minimum of `reps` runs; outputs are written by every run and are the
+ * same for every candidate.
+ * No reference counterpart (radix_sort_uint32_mt.inl:3-5 fixes the shape). */
+typedef struct vdf_tune_report {
+    int trials;
+    int sort_items;          /* rows per thread of a pass tile for this key width */
+    int sort_blocks_per_cu;  /* 0 = every resident block */
+    int sort_xcd;
+    int sort_chunk;
+    int hist_rows;
+    int digit_bits;
+    int reserved;
+    double default_ms;       /* the workload under the tuning table before the call */
+    double best_ms;
+} vdf_tune_report;
+
+vdf_status vdf_autotune_argsort(vdf_handle* h, int key_type, const void* d_keys, uint64_t n, uint32_t flags,
+                                void* d_perm_out, int reps, void* stream, vdf_tune_report* report);
+vdf_status vdf_autotune_groupby_batch(vdf_handle* h, int key_type, const vdf_table* tables, int ntables,
+                                      const int* val_types, int nvals, uint32_t agg_mask,
+                                      vdf_groupby_out* outs, int reps, void* stream,
+                                      vdf_tune_report* report);
+
+/* ---- HIP-graph-captured batch plans ----
+ * vdf_plan_groupby_batch captures the whole launch sequence of a
+ * vdf_groupby_batch call for one batch shape (the tables' row counts, key and
+ * value types, aggregates) into HIP graphs with their own scratch (it runs
+ * nothing).  vdf_plan_launch replays it for any tables and outputs of that
+ * shape: the host rewrites the batch description the graph uploads first,
+ * then launches the graph on `stream`.  Replays of one plan share its scratch:
+ * launch them on one stream or order them.  The tuning table is the handle's
+ * at capture, with radix passes in global tile order (plain launches).
+ * No reference counterpart. */
+typedef struct vdf_plan vdf_plan;
+vdf_status vdf_plan_groupby_batch(vdf_handle* h, int key_type, const vdf_table* tables, int ntables,
+                                  const int* val_types, int nvals, uint32_t agg_mask, vdf_plan** out);
+vdf_status vdf_plan_launch(vdf_plan* p, const vdf_table* tables, int ntables, vdf_groupby_out* outs,
+                           void* stream);
+vdf_status vdf_plan_scratch_bytes(vdf_plan* p, uint64_t* bytes);
+vdf_status vdf_plan_destroy(vdf_plan* p);
 
 /* ---- per-kernel timing (HIP events on the launch stream) ----
  * While enabled, every kernel launch of this handle is bracketed by events.

@@ -43,6 +43,8 @@ EXPORTS = (
     "vdf_argsort_batch", "vdf_groupby_batch",
     "vdf_server_create", "vdf_server_destroy", "vdf_server_add_table", "vdf_server_submit",
     "vdf_server_wait", "vdf_server_release", "vdf_server_stats_get",
+    "vdf_autotune_argsort", "vdf_autotune_groupby_batch",
+    "vdf_plan_groupby_batch", "vdf_plan_launch", "vdf_plan_scratch_bytes", "vdf_plan_destroy",
 )
 
 
@@ -80,6 +82,12 @@ class VdfServed(C.Structure):
 class VdfServerStats(C.Structure):
     _fields_ = [(k, C.c_uint64) for k in ("submitted", "completed", "failed", "launches", "retries", "rows",
                                           "queued", "arena_bytes", "arena_used", "arena_peak")]
+
+
+class VdfTuneReport(C.Structure):
+    _fields_ = [("trials", C.c_int), ("sort_items", C.c_int), ("sort_blocks_per_cu", C.c_int), ("sort_xcd", C.c_int),
+                ("sort_chunk", C.c_int), ("hist_rows", C.c_int), ("digit_bits", C.c_int), ("reserved", C.c_int),
+                ("default_ms", C.c_double), ("best_ms", C.c_double)]
 
 
 class VdfSortSeg(C.Structure):
@@ -153,6 +161,20 @@ def load() -> C.CDLL:
     lib.vdf_server_release.argtypes = [vp, u64]
     lib.vdf_server_stats_get.restype = i32
     lib.vdf_server_stats_get.argtypes = [vp, C.POINTER(VdfServerStats)]
+    lib.vdf_autotune_argsort.restype = i32
+    lib.vdf_autotune_argsort.argtypes = [vp, i32, vp, u64, u32, vp, i32, vp, C.POINTER(VdfTuneReport)]
+    lib.vdf_autotune_groupby_batch.restype = i32
+    lib.vdf_autotune_groupby_batch.argtypes = [vp, i32, C.POINTER(VdfTable), i32, C.POINTER(C.c_int), i32, u32,
+                                               C.POINTER(VdfGroupbyOut), i32, vp, C.POINTER(VdfTuneReport)]
+    lib.vdf_plan_groupby_batch.restype = i32
+    lib.vdf_plan_groupby_batch.argtypes = [vp, i32, C.POINTER(VdfTable), i32, C.POINTER(C.c_int), i32, u32,
+                                           C.POINTER(vp)]
+    lib.vdf_plan_launch.restype = i32
+    lib.vdf_plan_launch.argtypes = [vp, C.POINTER(VdfTable), i32, C.POINTER(VdfGroupbyOut), vp]
+    lib.vdf_plan_scratch_bytes.restype = i32
+    lib.vdf_plan_scratch_bytes.argtypes = [vp, C.POINTER(u64)]
+    lib.vdf_plan_destroy.restype = i32
+    lib.vdf_plan_destroy.argtypes = [vp]
     lib.vdf_tune_set.restype = i32
     lib.vdf_tune_set.argtypes = [vp, C.c_char_p, C.c_int64]
     lib.vdf_tune_get.restype = i32

@@ -44,6 +44,12 @@ void* Handle::ensure_scratch(size_t bytes)
 
 void Handle::staging_begin(size_t bytes)
 {
+    if (staging_fixed) {
+        if (bytes > staging_fixed_cap) throw Error{VDF_ERR_HIP, "plan staging buffer too small"};
+        staging_slot = kStaging;  // marks the fixed buffer
+        staging_off = 0;
+        return;
+    }
     const int slot = static_cast<int>(staging_next++ % kStaging);
     if (staging_pending[slot]) {
         VDF_HIP_CHECK(hipEventSynchronize(staging_done[slot]));
@@ -65,8 +71,10 @@ void* Handle::staging_take(size_t bytes)
 {
     if (staging_slot < 0) throw Error{VDF_ERR_HIP, "staging_take outside a call"};
     staging_off = align_up(staging_off, 256);
-    if (staging_off + bytes > staging_cap[staging_slot]) throw Error{VDF_ERR_HIP, "staging region too small"};
-    void* p = static_cast<char*>(staging[staging_slot]) + staging_off;
+    const bool fixed = staging_slot == kStaging;
+    if (staging_off + bytes > (fixed ? staging_fixed_cap : staging_cap[staging_slot]))
+        throw Error{VDF_ERR_HIP, "staging region too small"};
+    void* p = static_cast<char*>(fixed ? staging_fixed : staging[staging_slot]) + staging_off;
     staging_off += bytes;
     return p;
 }
@@ -74,6 +82,10 @@ void* Handle::staging_take(size_t bytes)
 void Handle::staging_end(hipStream_t s)
 {
     if (staging_slot < 0) return;
+    if (staging_slot == kStaging) {  // a plan's buffer: the plan tracks its reuse
+        staging_slot = -1;
+        return;
+    }
     if (!staging_done[staging_slot])
         VDF_HIP_CHECK(hipEventCreateWithFlags(&staging_done[staging_slot], hipEventDisableTiming));
     VDF_HIP_CHECK(hipEventRecord(staging_done[staging_slot], s));
@@ -180,8 +192,6 @@ struct StagingScope {
 }  // namespace vdf
 
 using namespace vdf;
-
-struct vdf_handle : Handle {};
 
 extern "C" {
 
@@ -368,6 +378,7 @@ static int* tune_slot(Tuning& t, const std::string& n)
     if (n == "sort_items_64") return &t.sort_items_64;
     if (n == "sort_chunk") return &t.sort_chunk;
     if (n == "sort_xcd") return &t.sort_xcd;
+    if (n == "sort_blocks_per_cu") return &t.sort_bpc;
     if (n == "hist_rows") return &t.hist_rows;
     if (n == "gather_rows") return &t.gather_rows;
     if (n == "gather_nt") return &t.gather_nt;

@@ -694,6 +694,7 @@ void groupby_batch_enqueue(Handle* h, int key_type, const GbTable* tables, int n
         row += align_up(n, kGbRows);
         tiles += nt;
     }
+    if (h->dry) return;  // a plan refresh only wants the staging
     // dsegs and dtile are consecutive carves with the same alignment as the staging layout
     VDF_HIP_CHECK(hipMemcpyAsync(dsegs, stg, seg_bytes + tiles * 4, hipMemcpyHostToDevice, s));
     if (static_cast<char*>(static_cast<void*>(dtile)) != reinterpret_cast<char*>(dsegs) + seg_bytes)

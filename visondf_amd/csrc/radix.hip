@@ -701,10 +701,25 @@ bool tune_supported(const char* name, int64_t v)
     if (n == "sort_items_64") return pick_pass<uint64_t>(static_cast<int>(v)) != nullptr;
     if (n == "sort_chunk") return v >= 1 && v <= 1024;
     if (n == "sort_xcd") return v == 0 || v == 1;
+    if (n == "sort_blocks_per_cu") return v >= 0 && v <= 32;
     if (n == "hist_rows") return v >= 4096 && v <= (1 << 24);
     if (n == "gather_rows") return v == 4 || v == 8 || v == 16;
     if (n == "gather_nt") return v == 0 || v == 1;
     return false;
+}
+
+// Compute units of a device (cached).
+static unsigned device_cus(int device)
+{
+    static std::mutex mu;
+    static std::vector<std::pair<int, unsigned>> cache;
+    std::lock_guard<std::mutex> lk(mu);
+    for (const auto& e : cache)
+        if (e.first == device) return e.second;
+    int cus = 0;
+    VDF_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    cache.emplace_back(device, static_cast<unsigned>(std::max(1, cus)));
+    return cache.back().second;
 }
 
 // Resident blocks of a pass kernel on the whole device (persistent grid).
@@ -763,7 +778,7 @@ static void launch_sort(Handle* h, const SortSeg* segs, int nsegs, uint64_t mlo,
         tiles += nt;
     }
     const uint64_t ntiles = tiles;
-    if (ntiles == 0) return;  // every segment empty
+    if (ntiles == 0 || h->dry) return;  // every segment empty, or only the staging was wanted
 
     Carve c(scr);
     uint32_t* zero = c.take<uint32_t>(L.zero_words);
@@ -794,8 +809,11 @@ static void launch_sort(Handle* h, const SortSeg* segs, int nsegs, uint64_t mlo,
     a.ntiles = static_cast<uint32_t>(ntiles);
     a.perm64 = (flags & VDF_SORT_PERM64) ? 1u : 0u;
     a.reverse = (flags & VDF_SORT_DESC_REVERSE) ? 1u : 0u;
-    // persistent grid: every resident block, a multiple of the group count
-    unsigned grid = static_cast<unsigned>(std::min<uint64_t>(resident_blocks(pass, h->device), ntiles));
+    // persistent grid: every resident block (or the tuned blocks per CU), a
+    // multiple of the group count
+    uint64_t cap = resident_blocks(pass, h->device);
+    if (h->tune.sort_bpc > 0) cap = std::min<uint64_t>(cap, static_cast<uint64_t>(h->tune.sort_bpc) * device_cus(h->device));
+    unsigned grid = static_cast<unsigned>(std::min<uint64_t>(cap, ntiles));
     const bool xcd = h->tune.sort_xcd != 0;
     if (xcd) {
         a.groups = std::min(8u, grid);

@@ -117,6 +117,7 @@ struct Tuning {
     int sort_chunk = 64;     // consecutive tiles a tile group (one XCD) claims at a time
     int sort_xcd = 1;        // 1: XCD-grouped tile order, cooperative launch; 0: global tile
                              //    order, plain launch (safe beside other persistent work)
+    int sort_bpc = 0;        // radix pass blocks per CU (persistent grid); 0 = every resident block
     int hist_rows = 65536;   // rows per histogram work item
     int gather_rows = 8;     // rows per thread of the gather kernel
     int gather_nt = 0;       // non-temporal loads of the gathered column
@@ -151,6 +152,12 @@ struct Handle {
     int staging_slot = -1;
     size_t staging_off = 0;
     unsigned staging_next = 0;
+    // a captured plan's own pinned buffer replaces the ring (staging_fixed != null)
+    void* staging_fixed = nullptr;
+    size_t staging_fixed_cap = 0;
+    // dry run: enqueue functions write the host staging of a batch and return
+    // before uploading or launching anything (refreshes a captured plan)
+    bool dry = false;
 
     void* ensure_scratch(size_t bytes);
     void staging_begin(size_t bytes);      // start a call's staging region
@@ -255,3 +262,6 @@ void groupby_batch_enqueue(Handle* h, int key_type, const GbTable* tables, int n
 void fill_u32(uint32_t* p, size_t count, uint32_t value, hipStream_t s);
 
 }  // namespace vdf
+
+// the C ABI's opaque handle is the library's Handle
+struct vdf_handle : vdf::Handle {};

@@ -228,6 +228,19 @@ class Vdf:
                                            C.c_void_p(_stream_ptr(stream))))
         return res
 
+    def prepare_groupby_batch(self, tables, aggs: int = N.AGG_ALL, want_perm: bool = True, key_type=None,
+                              val_types=None, capacities=None) -> "PreparedBatch":
+        """Descriptors and outputs of one batch, built once for repeated runs."""
+        return PreparedBatch(self, tables, aggs, want_perm, key_type, val_types, capacities)
+
+    def autotune_argsort(self, keys: torch.Tensor, perm_out: torch.Tensor, *, key_type: int | None = None,
+                         flags: int = 0, reps: int = 3, stream=None) -> dict:
+        kt = key_type_of(keys) if key_type is None else key_type
+        rep = N.VdfTuneReport()
+        N.check(self.lib.vdf_autotune_argsort(self.h, kt, _ptr(keys), keys.shape[0], flags, _ptr(perm_out), reps,
+                                              C.c_void_p(_stream_ptr(stream)), C.byref(rep)))
+        return {k: getattr(rep, k) for k, _ in N.VdfTuneReport._fields_ if k != "reserved"}
+
     # ----------------------------------------------------------------- tuning
     def tune(self, name: str, value: int):
         N.check(self.lib.vdf_tune_set(self.h, name.encode(), int(value)))
@@ -251,3 +264,86 @@ class Vdf:
         N.check(self.lib.vdf_profile_read(self.h, kernel.encode(), C.byref(ms), C.byref(launches),
                                           C.byref(nbytes)))
         return ms.value, launches.value, nbytes.value
+
+
+class PreparedBatch:
+    """A group-by batch whose C descriptors and output buffers are built once.
+
+    run() calls vdf_groupby_batch; autotune() tunes the handle on this batch;
+    capture() records the launch sequence as a HIP-graph plan and replay()
+    launches it (the same tables and outputs, or set_tables() of the same
+    shape first)."""
+
+    def __init__(self, vdf: Vdf, tables, aggs, want_perm, key_type, val_types, capacities):
+        self.v = vdf
+        k0, v0 = tables[0]
+        self.kt = key_type_of(k0) if key_type is None else key_type
+        self.vts = [val_type_of(x) if val_types is None else val_types[c] for c, x in enumerate(v0)]
+        self.nv = len(v0)
+        self.aggs = aggs
+        self.ntab = len(tables)
+        self.tabs = (N.VdfTable * self.ntab)()
+        self.outs = (N.VdfGroupbyOut * self.ntab)()
+        self.keep = []
+        self.sizes = []
+        for i, (k, vs) in enumerate(tables):
+            o, t = Vdf._alloc_out(k, self.vts, [x.dtype for x in vs], aggs, want_perm,
+                                  capacities[i] if capacities is not None else 0)
+            self.outs[i] = o
+            self.keep.append(t)
+            self.sizes.append(k.shape[0])
+        self.set_tables(tables)
+        self.vt_arr = (C.c_int * max(self.nv, 1))(*self.vts)
+        self.plan = None
+
+    def set_tables(self, tables):
+        """Points the batch at other tables with the same row counts."""
+        if len(tables) != self.ntab:
+            raise ValueError("table count differs")
+        self.tables = list(tables)
+        for i, (k, vs) in enumerate(tables):
+            if k.shape[0] != self.sizes[i]:
+                raise ValueError(f"table {i}: {k.shape[0]} rows, batch has {self.sizes[i]}")
+            self.tabs[i].keys = k.data_ptr() if k.shape[0] else None
+            self.tabs[i].n = k.shape[0]
+            for c, x in enumerate(vs):
+                self.tabs[i].vals[c] = x.data_ptr() if k.shape[0] else None
+
+    def run(self, stream=None):
+        N.check(self.v.lib.vdf_groupby_batch(self.v.h, self.kt, self.tabs, self.ntab, self.vt_arr, self.nv,
+                                             self.aggs, self.outs, C.c_void_p(_stream_ptr(stream))))
+
+    def autotune(self, reps: int = 3, stream=None) -> dict:
+        rep = N.VdfTuneReport()
+        N.check(self.v.lib.vdf_autotune_groupby_batch(self.v.h, self.kt, self.tabs, self.ntab, self.vt_arr, self.nv,
+                                                      self.aggs, self.outs, reps, C.c_void_p(_stream_ptr(stream)),
+                                                      C.byref(rep)))
+        return {k: getattr(rep, k) for k, _ in N.VdfTuneReport._fields_ if k != "reserved"}
+
+    def capture(self):
+        if self.plan is None:
+            p = C.c_void_p()
+            N.check(self.v.lib.vdf_plan_groupby_batch(self.v.h, self.kt, self.tabs, self.ntab, self.vt_arr, self.nv,
+                                                      self.aggs, C.byref(p)))
+            self.plan = p
+        return self
+
+    def replay(self, stream=None):
+        if self.plan is None:
+            raise RuntimeError("capture() first")
+        N.check(self.v.lib.vdf_plan_launch(self.plan, self.tabs, self.ntab, self.outs,
+                                           C.c_void_p(_stream_ptr(stream))))
+
+    def release_plan(self):
+        if self.plan is not None:
+            N.check(self.v.lib.vdf_plan_destroy(self.plan))
+            self.plan = None
+
+    def __del__(self):
+        try:
+            self.release_plan()
+        except Exception:
+            pass
+
+    def result(self, i: int, trim: bool = True) -> GroupbyResult:
+        return Vdf._result(self.keep[i], self.sizes[i], trim)
