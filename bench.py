@@ -282,33 +282,28 @@ def run_batched(args) -> dict:
     v = Vdf(0)
     caps = [max(int(n) // 8, 1) for n in sizes]  # keys are drawn from [0, rows/8)
     tables = [(d_keys[offs[i]:offs[i + 1]], [d_vals[offs[i]:offs[i + 1]]]) for i in range(ntab)]
-
-    # descriptors built once: the timed region is the library call
-    tabs = (N.VdfTable * ntab)()
-    outs = (N.VdfGroupbyOut * ntab)()
-    keep = []
-    for i, (k, vs) in enumerate(tables):
-        tabs[i].keys = k.data_ptr()
-        tabs[i].n = k.shape[0]
-        tabs[i].vals[0] = vs[0].data_ptr()
-        o, t = Vdf._alloc_out(k, [N.VAL_F64], [torch.float64], N.AGG_ALL, False, caps[i])
-        outs[i] = o
-        keep.append(t)
-    vt = (C.c_int * 1)(N.VAL_F64)
-    stream = C.c_void_p(_stream_ptr(None))
-
-    def step():
-        N.check(v.lib.vdf_groupby_batch(v.h, N.KEY_I32, tabs, ntab, vt, 1, N.AGG_ALL, outs, stream))
-
-    ms_total, kernels = _time_steps(step, args.steps, args.warmup, v)
+    # descriptors and outputs built once: the timed region is the library call
+    pb = v.prepare_groupby_batch(tables, aggs=N.AGG_ALL, want_perm=False, key_type=N.KEY_I32, capacities=caps)
+    tune = None
+    if not args.no_tune:
+        _log("batched: autotuning the launch shape")
+        tune = pb.autotune(reps=1)
+        _log(f"batched: autotune {tune}")
+    ms_total, kernels = _time_steps(pb.run, args.steps, args.warmup, v)
     ms_step = ms_total / args.steps
     rows_per_s = total * args.steps / (ms_total * 1e-3)
-    ng = torch.cat([t["ng"] for t in keep]).cpu().numpy()
+    ng = torch.cat([t["ng"] for t in pb.keep]).cpu().numpy()
     assert np.all(ng <= np.array(caps)), "a table has more groups than its capacity"
-    cnt = sum(int(t["counts"][:g].to(torch.int64).sum().item()) for t, g in zip(keep[:64], ng[:64]))
+    cnt = sum(int(t["counts"][:g].to(torch.int64).sum().item()) for t, g in zip(pb.keep[:64], ng[:64]))
     assert cnt == int(sizes[:64].sum()), "group counts do not add up to the rows"
 
+    # the same batch as a captured HIP-graph plan
+    pb.capture()
+    ms_graph, _ = _time_steps(pb.replay, args.steps, 1)
+    pb.release_plan()
+
     # one vdf_groupby call per table, same outputs, descriptors also built beforehand
+    stream = C.c_void_p(_stream_ptr(None))
     cols = []
     for i, (k, vs) in enumerate(tables):
         c = (N.VdfValueCol * 1)()
@@ -319,7 +314,7 @@ def run_batched(args) -> dict:
     def per_table():
         for i, (k, _) in enumerate(tables):
             N.check(v.lib.vdf_groupby(v.h, N.KEY_I32, k.data_ptr(), k.shape[0], cols[i], 1, N.AGG_ALL,
-                                      C.byref(outs[i]), stream))
+                                      C.byref(pb.outs[i]), stream))
 
     per_table()
     torch.cuda.synchronize()
@@ -331,6 +326,9 @@ def run_batched(args) -> dict:
                         "parallelism": "single GPU"})
     rec["one_call_per_table"] = {"ms": ms_single, "rows_per_s": total / (ms_single * 1e-3),
                                  "batched_speedup": ms_single / ms_step}
+    rec["graph_replay"] = {"ms_per_step": ms_graph / args.steps,
+                           "rows_per_s": total * args.steps / (ms_graph * 1e-3)}
+    rec["autotune"] = tune
     rec["roofline"] = _roofline(kernels)
     rec["kernels"] = kernels
     if not args.no_cpu:
@@ -465,6 +463,8 @@ def _child(name: str, args) -> dict:
            "--warmup", str(warmup)]
     if args.no_cpu:
         cmd.append("--no-cpu")
+    if args.no_tune:
+        cmd.append("--no-tune")
     if args.calibrate and name == "argsort_u32":
         cmd.append("--calibrate")
     _log(f"running {name}: {' '.join(cmd[1:])}")
@@ -493,6 +493,7 @@ def main():
     ap.add_argument("--workers", type=int, default=3, help="served: launch sequences in flight")
     ap.add_argument("--arena-gb", type=int, default=48, help="served: server arena (admission budget), GiB")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
+    ap.add_argument("--no-tune", action="store_true", help="batched: skip the launch-shape autotuner")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--calibrate", action="store_true",
                     help="first run one plain 1 GiB device copy (the known-size kernel that scales "

@@ -96,7 +96,20 @@ class Handle {
     explicit Handle(int device = 0, uint64_t budget = 0) { check(vdf_create(&h_, device, budget)); }
     Handle(const Handle&) = delete;
     Handle& operator=(const Handle&) = delete;
-    ~Handle() { (void)vdf_destroy(h_); }
+    Handle(Handle&& o) noexcept : h_(o.h_) { o.h_ = nullptr; }
+    Handle& operator=(Handle&& o) noexcept
+    {
+        if (this != &o) {
+            if (h_) (void)vdf_destroy(h_);
+            h_ = o.h_;
+            o.h_ = nullptr;
+        }
+        return *this;
+    }
+    ~Handle()
+    {
+        if (h_) (void)vdf_destroy(h_);
+    }
     vdf_handle* get() const { return h_; }
 
   private:
@@ -129,8 +142,11 @@ struct GroupResult {
     std::vector<uint32_t> offsets;       // n_groups + 1, into row_view
     std::vector<uint32_t> counts;
     std::vector<uint32_t> row_view;      // group-contiguous row ids
-    std::vector<std::vector<double>> sum_f, mean, min_f, max_f;   // per f64 value column
-    std::vector<std::vector<int64_t>> sum_i, min_i, max_i;        // per integer value column
+    // Indexed by value-column position: for a float64 column c, sum_f[c],
+    // min_f[c], max_f[c] hold its groups (sum_i[c] ... empty); for an integer
+    // column the *_i vectors do.  mean[c] is filled for every column.
+    std::vector<std::vector<double>> sum_f, mean, min_f, max_f;
+    std::vector<std::vector<int64_t>> sum_i, min_i, max_i;
 };
 
 class Dataframe {
@@ -171,8 +187,9 @@ class Dataframe {
         return out;
     }
 
-    // Reorder rows by column n (sort_by.hpp:9-19).  ASC=false is the stable
-    // descending order of INTERNALS.doc:1610-1630.  Soft=true keeps the
+    // Reorder rows by column n (sort_by.hpp:9-19).  ASC=false is the
+    // reference's descending path: the ascending order reversed, so equal keys
+    // come out in reverse row order (sort_uintegers.inl:216-218).  Soft=true keeps the
     // permutation as a row view (sort_by_mt.hpp:160-177), Soft=false gathers
     // every column (permute_block_mt.inl:39-41).
     template <bool ASC = true, bool Soft = true>
@@ -183,8 +200,8 @@ class Dataframe {
         if (k.key_type < 0) throw Error(VDF_ERR_UNSUPPORTED, "sort_by: column type has no radix key path");
         if (nrow_ == 0) return;
         DeviceBuffer perm(static_cast<size_t>(nrow_) * 4);
-        check(vdf_argsort(h_.get(), k.key_type, k.data.get(), nrow_, ASC ? 0u : VDF_SORT_DESC, perm.get(), nullptr,
-                          nullptr));
+        check(vdf_argsort(h_.get(), k.key_type, k.data.get(), nrow_, ASC ? 0u : VDF_SORT_DESC_REVERSE, perm.get(),
+                          nullptr, nullptr));
         if constexpr (Soft) {
             std::vector<uint32_t> p(nrow_);
             hip_check(hipMemcpy(p.data(), perm.get(), nrow_ * 4ull, hipMemcpyDeviceToHost), "hipMemcpy D2H");
@@ -252,6 +269,8 @@ class Dataframe {
         R.offsets.resize(g + 1);
         R.counts.resize(g);
         R.row_view.resize(n);
+        for (auto* v : {&R.sum_f, &R.mean, &R.min_f, &R.max_f}) v->resize(vals.size());
+        for (auto* v : {&R.sum_i, &R.min_i, &R.max_i}) v->resize(vals.size());
         auto d2h = [](void* dst, const void* src, size_t bytes) {
             if (bytes) hip_check(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hipMemcpy D2H");
         };
@@ -267,10 +286,10 @@ class Dataframe {
                 d2h(m.data(), out.aggs[c].mean, g * 8);
                 d2h(lo.data(), out.aggs[c].min, g * 8);
                 d2h(hi.data(), out.aggs[c].max, g * 8);
-                R.sum_f.push_back(std::move(s));
-                R.mean.push_back(std::move(m));
-                R.min_f.push_back(std::move(lo));
-                R.max_f.push_back(std::move(hi));
+                R.sum_f[c] = std::move(s);
+                R.mean[c] = std::move(m);
+                R.min_f[c] = std::move(lo);
+                R.max_f[c] = std::move(hi);
             } else {
                 std::vector<int64_t> s(g), lo(g), hi(g);
                 std::vector<double> m(g);
@@ -289,10 +308,10 @@ class Dataframe {
                     d2h(lo.data(), out.aggs[c].min, g * 8);
                     d2h(hi.data(), out.aggs[c].max, g * 8);
                 }
-                R.sum_i.push_back(std::move(s));
-                R.mean.push_back(std::move(m));
-                R.min_i.push_back(std::move(lo));
-                R.max_i.push_back(std::move(hi));
+                R.sum_i[c] = std::move(s);
+                R.mean[c] = std::move(m);
+                R.min_i[c] = std::move(lo);
+                R.max_i[c] = std::move(hi);
             }
         }
         return R;
@@ -312,7 +331,7 @@ class Dataframe {
                 double v = 0;
                 if constexpr (F == GroupFunction::Occurence) v = static_cast<double>(R.counts[g]);
                 else {
-                    const bool f = !R.sum_f.empty();
+                    const bool f = !R.sum_f[0].empty() || !R.min_f[0].empty();
                     if (F == GroupFunction::Sum) v = f ? R.sum_f[0][g] : static_cast<double>(R.sum_i[0][g]);
                     if (F == GroupFunction::Mean) v = R.mean[0][g];
                     if (F == GroupFunction::Min) v = f ? R.min_f[0][g] : static_cast<double>(R.min_i[0][g]);

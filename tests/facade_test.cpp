@@ -1,0 +1,228 @@
+// GPU test program of the header-only C++ façade (include/vison_gpu.hpp):
+// the reference-shaped Dataframe calls (add_col, sort_by<ASC, Soft>,
+// materialize, group_by, transform_group_by) checked against host
+// computations on the same seeded frame.  Built by visondf_amd/csrc/Makefile
+// (target facade) and run by tests/test_facade_gpu.py.
+#include "vison_gpu.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <map>
+#include <numeric>
+#include <random>
+
+using vison_gpu::Char16;
+using vison_gpu::Dataframe;
+using vison_gpu::GroupFunction;
+
+static int g_fail = 0;
+#define CHECK(c)                                                             \
+    do {                                                                     \
+        if (!(c)) {                                                          \
+            std::printf("FAIL %s:%d: %s\n", __FILE__, __LINE__, #c);         \
+            ++g_fail;                                                        \
+        }                                                                    \
+    } while (0)
+
+struct Frame {
+    std::vector<uint32_t> k32;
+    std::vector<int64_t> k64;
+    std::vector<double> vf;
+    std::vector<int32_t> vi;
+    std::vector<Char16> kc;
+};
+
+static Frame make(size_t n, uint64_t seed)
+{
+    std::mt19937_64 rng(seed);
+    Frame f;
+    f.k32.resize(n);
+    f.k64.resize(n);
+    f.vf.resize(n);
+    f.vi.resize(n);
+    f.kc.resize(n);
+    std::normal_distribution<double> nd(0.0, 1.0);
+    for (size_t i = 0; i < n; ++i) {
+        f.k32[i] = static_cast<uint32_t>(rng() % 5000) * 2654435761u;
+        f.k64[i] = static_cast<int64_t>(rng() % 201) - 100;
+        f.vf[i] = nd(rng);
+        f.vi[i] = static_cast<int32_t>(rng() % 2000001) - 1000000;
+        const uint64_t id = rng() % 700;
+        for (int j = 0; j < 16; ++j) f.kc[i].c[j] = static_cast<char>(32 + (id * 2654435761ull >> (j * 3)) % 95);
+    }
+    return f;
+}
+
+static Dataframe load(const Frame& f)
+{
+    Dataframe df(0);
+    df.add_col(f.k32, "k32");
+    df.add_col(f.k64, "k64");
+    df.add_col(f.vf, "vf");
+    df.add_col(f.vi, "vi");
+    df.add_col(f.kc, "kc");
+    return df;
+}
+
+template <typename K>
+static std::vector<size_t> stable_order(const std::vector<K>& k)
+{
+    std::vector<size_t> idx(k.size());
+    std::iota(idx.begin(), idx.end(), 0);
+    std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return k[a] < k[b]; });
+    return idx;
+}
+
+static bool same_bits(double a, double b) { return std::memcmp(&a, &b, 8) == 0; }
+
+int main()
+{
+    std::setvbuf(stdout, nullptr, _IONBF, 0);
+    const size_t n = 200003;
+    const Frame f = make(n, 7);
+
+    // hard ascending sort by the uint32 column: every column follows
+    std::printf("sort_by<true, false> uint32\n");
+    {
+        Dataframe df = load(f);
+        unsigned int c = 0;
+        df.sort_by<true, false>(c);
+        const auto idx = stable_order(f.k32);
+        const auto k = df.get_col<uint32_t>(0);
+        const auto vf = df.get_col<double>(2);
+        const auto vi = df.get_col<int32_t>(3);
+        bool ok = true;
+        for (size_t i = 0; i < n && ok; ++i)
+            ok = k[i] == f.k32[idx[i]] && same_bits(vf[i], f.vf[idx[i]]) && vi[i] == f.vi[idx[i]];
+        CHECK(ok);
+    }
+    // hard descending sort by the int64 column: the reference's ascending-then-reverse order
+    std::printf("sort_by<false, false> int64\n");
+    {
+        Dataframe df = load(f);
+        unsigned int c = 1;
+        df.sort_by<false, false>(c);
+        auto idx = stable_order(f.k64);
+        std::reverse(idx.begin(), idx.end());
+        const auto vi = df.get_col<int32_t>(3);
+        bool ok = true;
+        for (size_t i = 0; i < n && ok; ++i) ok = vi[i] == f.vi[idx[i]];
+        CHECK(ok);
+    }
+    // soft sort (row view) then materialize == hard sort
+    std::printf("soft sort + materialize\n");
+    {
+        Dataframe a = load(f), b = load(f);
+        unsigned int c = 4;
+        a.sort_by<true, true>(c);
+        CHECK(a.in_view());
+        a.materialize();
+        CHECK(!a.in_view());
+        b.sort_by<true, false>(c);
+        CHECK(a.get_col<double>(2) == b.get_col<double>(2));
+    }
+    // group_by on the int64 key: counts, integer sums, min / max exact; f64 within 1e-12
+    std::printf("group_by int64\n");
+    {
+        Dataframe df = load(f);
+        const auto R = df.group_by<int64_t>(1, {2, 3});
+        struct Acc {
+            uint64_t cnt = 0;
+            long double sf = 0, sabs = 0;
+            double mnf = 0, mxf = 0;
+            int64_t si = 0, mni = 0, mxi = 0;
+        };
+        std::map<int64_t, Acc> m;
+        for (size_t i = 0; i < n; ++i) {
+            Acc& a = m[f.k64[i]];
+            if (a.cnt == 0) {
+                a.mnf = a.mxf = f.vf[i];
+                a.mni = a.mxi = f.vi[i];
+            }
+            a.mnf = std::min(a.mnf, f.vf[i]);
+            a.mxf = std::max(a.mxf, f.vf[i]);
+            a.mni = std::min<int64_t>(a.mni, f.vi[i]);
+            a.mxi = std::max<int64_t>(a.mxi, f.vi[i]);
+            a.sf += f.vf[i];
+            a.sabs += std::fabs(f.vf[i]);
+            a.si += f.vi[i];
+            ++a.cnt;
+        }
+        CHECK(R.n_groups == m.size());
+        size_t g = 0;
+        bool ok = true;
+        for (const auto& [key, a] : m) {
+            if (g >= R.n_groups) break;
+            ok = ok && R.keys[g] == key && R.counts[g] == a.cnt;
+            ok = ok && R.sum_i[1][g] == a.si && R.min_i[1][g] == a.mni && R.max_i[1][g] == a.mxi;
+            ok = ok && R.min_f[0][g] == a.mnf && R.max_f[0][g] == a.mxf;
+            const double exact = static_cast<double>(a.sf);
+            // 1e-12 relative, plus the long-double reference's own rounding bound
+            ok = ok && std::fabs(R.sum_f[0][g] - exact) <= 1e-12 * std::fabs(exact) + 1e-15 * static_cast<double>(a.sabs);
+            ok = ok && std::fabs(R.mean[1][g] - static_cast<double>(a.si) / a.cnt) <=
+                           1e-15 * std::fabs(static_cast<double>(a.si) / a.cnt);
+            // the row view lists the group's rows in row order
+            for (uint32_t p = R.offsets[g]; p < R.offsets[g + 1] && ok; ++p) {
+                ok = f.k64[R.row_view[p]] == key;
+                if (p > R.offsets[g]) ok = ok && R.row_view[p] > R.row_view[p - 1];
+            }
+            ++g;
+        }
+        CHECK(ok);
+    }
+    // transform_group_by: per-row group sum and count columns
+    std::printf("transform_group_by\n");
+    {
+        Dataframe df = load(f);
+        df.transform_group_by<GroupFunction::Sum>({0}, 3, "sum_vi");
+        df.transform_group_by<GroupFunction::Occurence>({0}, 0, "count");
+        CHECK(df.get_ncol() == 7);
+        std::map<uint32_t, std::pair<int64_t, uint64_t>> m;
+        for (size_t i = 0; i < n; ++i) {
+            m[f.k32[i]].first += f.vi[i];
+            m[f.k32[i]].second += 1;
+        }
+        const auto s = df.get_col<double>(5);
+        const auto c = df.get_col<double>(6);
+        bool ok = true;
+        for (size_t i = 0; i < n && ok; ++i)
+            ok = s[i] == static_cast<double>(m[f.k32[i]].first) && c[i] == static_cast<double>(m[f.k32[i]].second);
+        CHECK(ok);
+    }
+    // 16-byte string keys: group count and ascending byte order
+    std::printf("group_by char16\n");
+    {
+        Dataframe df = load(f);
+        const auto R = df.group_by<Char16>(4, {});
+        std::map<std::string, uint64_t> m;
+        for (size_t i = 0; i < n; ++i) m[std::string(f.kc[i].c, 16)] += 1;
+        CHECK(R.n_groups == m.size());
+        size_t g = 0;
+        bool ok = true;
+        for (const auto& [key, cnt] : m) {
+            if (g >= R.n_groups) break;
+            ok = ok && std::memcmp(R.keys[g].c, key.data(), 16) == 0 && R.counts[g] == cnt;
+            ++g;
+        }
+        CHECK(ok);
+    }
+    // errors surface as exceptions with the library's message
+    {
+        Dataframe df = load(f);
+        bool thrown = false;
+        try {
+            unsigned int c = 2;  // a float64 column has no radix key path here
+            df.sort_by<true, false>(c);
+        } catch (const vison_gpu::Error& e) {
+            thrown = e.status == VDF_ERR_UNSUPPORTED;
+        }
+        CHECK(thrown);
+    }
+    if (g_fail) {
+        std::printf("%d failures\n", g_fail);
+        return 1;
+    }
+    std::printf("facade ok\n");
+    return 0;
+}

@@ -145,10 +145,34 @@ static void planner_rules()
     }
 }
 
+// A caller that consumes results in submission order: newer same-signature
+// requests must not take the memory an older request of another signature
+// needs, or the older one could never be admitted.
+static void planner_keeps_room_for_skipped()
+{
+    const uint64_t A = 256;
+    auto scr = [](const std::vector<uint64_t>& ns) { return uint64_t(ns.size() * 2 * 256); };
+    // arena 40 units; queue: head (sig 0, 6 units), older skipped (sig 1, 20 units), newer sig 0 requests
+    RangeAllocator ra(40 * A, A);
+    std::vector<PlanItem> q = {{1, 0, 6 * A}, {1, 1, 20 * A}, {1, 0, 6 * A}, {1, 0, 6 * A}, {1, 0, 6 * A}};
+    Plan p = plan_batch(q, PlanLimits{}, ra, scr);
+    // head (6) + scratch; the skipped request needs 20 + 2 = 22 contiguous units
+    CHECK(!p.picked.empty() && p.picked[0] == 0);
+    CHECK(ra.largest_free() >= 22 * A);
+    for (size_t k : p.picked) CHECK(k != 1);
+    // the in-order consumer: results of the batch stay held; the skipped request is now the head
+    std::vector<PlanItem> q2(q.begin() + 1, q.end());
+    for (size_t k = p.picked.size(); k-- > 1;) q2.erase(q2.begin() + static_cast<long>(p.picked[k] - 1));
+    ra.release(p.scr_off);  // the batch finished; its results (out blocks) are still held
+    Plan p2 = plan_batch(q2, PlanLimits{}, ra, scr);
+    CHECK(!p2.picked.empty() && p2.picked[0] == 0);  // the older request runs
+}
+
 int main()
 {
     for (uint64_t s = 1; s <= 4; ++s) allocator_random(s);
     planner_rules();
+    planner_keeps_room_for_skipped();
     if (g_fail) {
         std::printf("%d failures\n", g_fail);
         return 1;

@@ -125,8 +125,17 @@ struct Plan {
 // of scratch_bytes(rows of the picked requests).  Memory is taken in queue
 // order and the scan stops at the first request whose block does not fit, so
 // a large old request is never passed by newer ones for memory; when the
-// scratch block does not fit, the newest picks are dropped until it does.  An
-// empty plan means nothing can start until memory is released (or, with
+// scratch block does not fit, the newest picks are dropped until it does.
+//
+// Requests of another signature are skipped (they wait for a batch of their
+// own), but memory is kept for them: a newer request is picked only if, with
+// its block (and finally the batch scratch) taken, one free range still holds
+// the result block plus single-request scratch of every older skipped
+// request.  Without this, newer results held by a caller that consumes
+// results in submission order could fill the arena while the older request
+// waits for memory that only the caller can free.
+//
+// An empty plan means nothing can start until memory is released (or, with
 // head_never_fits, that the oldest request can never run in this arena).
 template <typename ScratchFn>
 Plan plan_batch(const std::vector<PlanItem>& queue, const PlanLimits& lim, RangeAllocator& arena,
@@ -143,30 +152,45 @@ Plan plan_batch(const std::vector<PlanItem>& queue, const PlanLimits& lim, Range
         }
     }
     uint64_t rows = 0;
+    uint64_t keep = 0;            // largest need of an older skipped request so far
     std::vector<uint64_t> ns;
+    std::vector<uint64_t> keep_at;  // `keep` when each pick was made
     for (size_t i = 0; i < queue.size() && p.picked.size() < lim.max_tables; ++i) {
         const PlanItem& q = queue[i];
-        if (q.sig != head.sig) continue;
+        if (q.sig != head.sig) {
+            std::vector<uint64_t> one{q.rows};
+            keep = std::max(keep, arena.round(q.out_bytes) + arena.round(scratch_bytes(one)));
+            continue;
+        }
         if (i > 0 && rows + q.rows > lim.max_rows) continue;
         const uint64_t off = arena.alloc(q.out_bytes);
         if (off == RangeAllocator::kNone) break;
+        if (keep && arena.largest_free() < keep) {  // would starve an older request
+            arena.release(off);
+            break;
+        }
         p.picked.push_back(i);
         p.out_off.push_back(off);
         ns.push_back(q.rows);
+        keep_at.push_back(keep);
         rows += q.rows;
     }
     while (!p.picked.empty()) {
         const uint64_t sb = scratch_bytes(ns);
         const uint64_t off = arena.alloc(sb);
         if (off != RangeAllocator::kNone) {
-            p.scr_off = off;
-            p.scr_bytes = sb;
-            return p;
+            if (keep_at.back() == 0 || arena.largest_free() >= keep_at.back()) {
+                p.scr_off = off;
+                p.scr_bytes = sb;
+                return p;
+            }
+            arena.release(off);
         }
         arena.release(p.out_off.back());
         p.out_off.pop_back();
         p.picked.pop_back();
         ns.pop_back();
+        keep_at.pop_back();
     }
     return p;
 }
