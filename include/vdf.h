@@ -21,6 +21,8 @@
  *                     inlines/transform/sort_by/sort_uintegers.inl:8-13
  *                     inlines/transform/sort_by/sort_integers.inl:8-13
  *                     inlines/transform/sort_by/sort_char.inl:9-15
+ *                   and sort_flt(idx, col, nrow) for float keys
+ *                     inlines/transform/sort_by/sort_flt.inl:8-13 (transform :44-50)
  *                   and the radix kernels they call,
  *                     inlines/radix_sorts/radix_sort_uint32{,_mt}.inl:5-7 / 6-8
  *                     inlines/radix_sorts/radix_sort_uint64{,_mt}.inl:5-7 / 6-8
@@ -55,12 +57,14 @@ typedef int vdf_status;
 #define VDF_ERR_BUDGET       5 /* request exceeds the handle's HBM budget */
 #define VDF_ERR_TIMEOUT      6 /* vdf_server_wait: the request is not done yet */
 
-/* key column types (sort_by_mt.hpp:72-115 type letters u / i / c) */
+/* key column types (sort_by_mt.hpp:72-115 type letters u / i / c / d) */
 #define VDF_KEY_U32     0
 #define VDF_KEY_I32     1
 #define VDF_KEY_U64     2
 #define VDF_KEY_I64     3
 #define VDF_KEY_CHAR16  4 /* fixed-width 16-byte char[16] cells (CharT = char[16]) */
+#define VDF_KEY_F32     5 /* IEEE float, total order of sort_flt.inl:44-50:              */
+#define VDF_KEY_F64     6 /* -NaN < -inf < ... < -0 < +0 < ... < +inf < +NaN; groups by bits */
 
 /* value column types */
 #define VDF_VAL_F64 0
@@ -87,6 +91,7 @@ typedef int vdf_status;
 
 #define VDF_MAX_ROWS        ((1ull << 30) - 1ull)
 #define VDF_MAX_VALUE_COLS  4
+#define VDF_MAX_KEY_COLS    4
 
 /* ---- library / handle ---- */
 const char* vdf_last_error(void);
@@ -153,6 +158,41 @@ typedef struct vdf_groupby_out {
 vdf_status vdf_groupby(vdf_handle* h, int key_type, const void* d_keys, uint64_t n,
                        const vdf_value_col* vals, int nvals, uint32_t agg_mask,
                        vdf_groupby_out* out, void* stream);
+
+/* ---- multi-column keys ----
+ * Rows are keyed by several columns (their byte widths summing to at most 16);
+ * equal keys are equal in every column, as the reference's concatenated key
+ * bytes (transform_group_by_sametype_mt.hpp:13, transform_group_by_difftype_mt.hpp:13,
+ * key_build.inl:3-110).  Order: lexicographic over the columns, first column
+ * most significant, each column in its own order (as vdf_argsort).  The
+ * columns are packed into one composite key on the device, so the sort moves
+ * 4, 8 or 16 key bytes per row. */
+typedef struct vdf_key_col {
+    int type;           /* VDF_KEY_* */
+    const void* data;   /* device, n cells */
+} vdf_key_col;
+
+/* Group by several key columns.  out->group_keys is ignored; group_keys[c]
+ * (device, capacity C cells of column c's type; nullable) receives column c's
+ * group keys. */
+vdf_status vdf_groupby_multi(vdf_handle* h, const vdf_key_col* keys, int nkeys, uint64_t n,
+                             const vdf_value_col* vals, int nvals, uint32_t agg_mask,
+                             vdf_groupby_out* out, void* const* group_keys, void* stream);
+/* Stable argsort by several key columns (flags as vdf_argsort; descending
+ * applies to the whole key). */
+vdf_status vdf_argsort_multi(vdf_handle* h, const vdf_key_col* keys, int nkeys, uint64_t n, uint32_t flags,
+                             void* d_perm_out, void* stream);
+
+/* ---- per-row broadcast of group results ----
+ * d_out[perm[p]] = value of the group that sorted position p belongs to, for
+ * the outputs of one group-by (values d_group_vals of src_type, offsets,
+ * n_groups, perm = the group-contiguous row ids); dst_type is src_type or
+ * VDF_VAL_F64 (converted).  Replaces CreateValueCol (create_value_col.inl:36-61),
+ * the per-row column of transform_group_by.  Writes nothing when the group
+ * count exceeds `capacity`. */
+vdf_status vdf_broadcast(vdf_handle* h, int src_type, int dst_type, const void* d_group_vals,
+                         const uint32_t* d_offsets, const uint64_t* d_n_groups, uint64_t capacity,
+                         const uint32_t* d_perm, uint64_t n, void* d_out, void* stream);
 
 /* ---- batches of independent tables (segments) ----
  * One launch sequence sorts / groups every table of the batch: one histogram

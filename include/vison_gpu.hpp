@@ -16,6 +16,7 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstring>
 #include <stdexcept>
@@ -122,7 +123,8 @@ template <> struct TypeCode<uint32_t> { static constexpr int key = VDF_KEY_U32, 
 template <> struct TypeCode<int32_t> { static constexpr int key = VDF_KEY_I32, val = VDF_VAL_I32; static constexpr char letter = 'i'; };
 template <> struct TypeCode<uint64_t> { static constexpr int key = VDF_KEY_U64, val = VDF_VAL_U64; static constexpr char letter = 'u'; };
 template <> struct TypeCode<int64_t> { static constexpr int key = VDF_KEY_I64, val = VDF_VAL_I64; static constexpr char letter = 'i'; };
-template <> struct TypeCode<double> { static constexpr int key = -1, val = VDF_VAL_F64; static constexpr char letter = 'd'; };
+template <> struct TypeCode<double> { static constexpr int key = VDF_KEY_F64, val = VDF_VAL_F64; static constexpr char letter = 'd'; };
+template <> struct TypeCode<float> { static constexpr int key = VDF_KEY_F32, val = -1; static constexpr char letter = 'd'; };
 template <> struct TypeCode<Char16> { static constexpr int key = VDF_KEY_CHAR16, val = -1; static constexpr char letter = 'c'; };
 
 // A device column: element size, key / value type codes, name.
@@ -231,41 +233,156 @@ class Dataframe {
         in_view_ = false;
     }
 
+    // Device-resident results of one group-by (per-group arrays sized n).
+    struct DeviceGroups {
+        size_t n = 0;
+        DeviceBuffer ng, off, cnt, perm;
+        std::vector<DeviceBuffer> keys;  // one per key column
+        std::vector<DeviceBuffer> agg;   // 4 per value column: sum, mean, min, max
+        vdf_groupby_out out{};
+        uint64_t n_groups() const
+        {
+            uint64_t g = 0;
+            hip_check(hipMemcpy(&g, ng.get(), 8, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+            return g;
+        }
+    };
+
+    // Group by one or more key columns (lexicographic, first column most
+    // significant) and aggregate value columns; results stay on the device.
+    DeviceGroups group_by_device(const std::vector<unsigned int>& key_cols, const std::vector<unsigned int>& vals,
+                                 uint32_t aggs = VDF_AGG_ALL)
+    {
+        if (key_cols.empty() || key_cols.size() > VDF_MAX_KEY_COLS)
+            throw Error(VDF_ERR_INVALID, "group_by: 1 .. VDF_MAX_KEY_COLS key columns");
+        if (vals.size() > VDF_MAX_VALUE_COLS) throw Error(VDF_ERR_INVALID, "group_by: too many value columns");
+        const size_t n = nrow_;
+        DeviceGroups D;
+        D.n = n;
+        const size_t cap = std::max<size_t>(n, 1);
+        D.ng = DeviceBuffer(8);
+        D.off = DeviceBuffer((n + 1) * 4);
+        D.cnt = DeviceBuffer(cap * 4);
+        D.perm = DeviceBuffer(cap * 4);
+        std::vector<vdf_key_col> kc;
+        std::vector<void*> kout;
+        for (unsigned int k : key_cols) {
+            const Column& c = col(k);
+            if (c.key_type < 0) throw Error(VDF_ERR_UNSUPPORTED, "group_by: column type has no key path");
+            kc.push_back(vdf_key_col{c.key_type, c.data.get()});
+            D.keys.emplace_back(cap * c.elem);
+            kout.push_back(D.keys.back().get());
+        }
+        vdf_groupby_out& out = D.out;
+        out.n_groups = static_cast<uint64_t*>(D.ng.get());
+        out.group_keys = kout[0];
+        out.offsets = static_cast<uint32_t*>(D.off.get());
+        out.counts = static_cast<uint32_t*>(D.cnt.get());
+        out.perm = static_cast<uint32_t*>(D.perm.get());
+        std::vector<vdf_value_col> vc;
+        for (size_t c = 0; c < vals.size(); ++c) {
+            const Column& v = col(vals[c]);
+            if (v.val_type < 0) throw Error(VDF_ERR_UNSUPPORTED, "group_by: value column type");
+            vc.push_back(vdf_value_col{v.val_type, v.data.get()});
+            for (int a = 0; a < 4; ++a) D.agg.emplace_back(cap * 8);
+            out.aggs[c].sum = D.agg[4 * c + 0].get();
+            out.aggs[c].mean = static_cast<double*>(D.agg[4 * c + 1].get());
+            out.aggs[c].min = D.agg[4 * c + 2].get();
+            out.aggs[c].max = D.agg[4 * c + 3].get();
+        }
+        if (kc.size() == 1)
+            check(vdf_groupby(h_.get(), kc[0].type, kc[0].data, n, vc.data(), static_cast<int>(vc.size()), aggs, &out,
+                              nullptr));
+        else
+            check(vdf_groupby_multi(h_.get(), kc.data(), static_cast<int>(kc.size()), n, vc.data(),
+                                    static_cast<int>(vc.size()), aggs, &out, kout.data(), nullptr));
+        return D;
+    }
+
     // Group by key column `key` and aggregate value columns (ascending keys).
     template <typename K>
     GroupResult<K> group_by(unsigned int key, const std::vector<unsigned int>& vals, uint32_t aggs = VDF_AGG_ALL)
     {
         const Column& k = col(key);
         if (k.key_type < 0 || k.elem != sizeof(K)) throw Error(VDF_ERR_INVALID, "group_by: bad key column");
-        if (vals.size() > VDF_MAX_VALUE_COLS) throw Error(VDF_ERR_INVALID, "group_by: too many value columns");
-        const size_t n = nrow_;
-        GroupResult<K> R;
-        DeviceBuffer ng(8), gkeys(std::max<size_t>(n, 1) * sizeof(K)), off((n + 1) * 4),
-            cnt(std::max<size_t>(n, 1) * 4), perm(std::max<size_t>(n, 1) * 4);
-        std::vector<DeviceBuffer> bufs;
-        vdf_groupby_out out{};
-        out.n_groups = static_cast<uint64_t*>(ng.get());
-        out.group_keys = gkeys.get();
-        out.offsets = static_cast<uint32_t*>(off.get());
-        out.counts = static_cast<uint32_t*>(cnt.get());
-        out.perm = static_cast<uint32_t*>(perm.get());
-        std::vector<vdf_value_col> vc;
-        for (size_t c = 0; c < vals.size(); ++c) {
-            const Column& v = col(vals[c]);
-            if (v.val_type < 0) throw Error(VDF_ERR_UNSUPPORTED, "group_by: value column type");
-            vc.push_back(vdf_value_col{v.val_type, v.data.get()});
-            for (int a = 0; a < 4; ++a) bufs.emplace_back(std::max<size_t>(n, 1) * 8);
-            out.aggs[c].sum = bufs[bufs.size() - 4].get();
-            out.aggs[c].mean = static_cast<double*>(bufs[bufs.size() - 3].get());
-            out.aggs[c].min = bufs[bufs.size() - 2].get();
-            out.aggs[c].max = bufs[bufs.size() - 1].get();
+        DeviceGroups D = group_by_device({key}, vals, aggs);
+        return to_host<K>(D, vals);
+    }
+
+    // Group by several key columns; group keys of column i in keys_of[i] (raw bytes).
+    struct MultiGroupResult {
+        GroupResult<uint8_t> r;                    // r.keys unused
+        std::vector<std::vector<uint8_t>> keys_of;  // per key column, n_groups cells
+    };
+    MultiGroupResult group_by_multi(const std::vector<unsigned int>& key_cols, const std::vector<unsigned int>& vals,
+                                    uint32_t aggs = VDF_AGG_ALL)
+    {
+        DeviceGroups D = group_by_device(key_cols, vals, aggs);
+        MultiGroupResult M;
+        M.r = to_host<uint8_t>(D, vals, /*with_keys=*/false);
+        for (size_t c = 0; c < key_cols.size(); ++c) {
+            const size_t eb = col(key_cols[c]).elem;
+            M.keys_of.emplace_back(M.r.n_groups * eb);
+            if (M.r.n_groups)
+                hip_check(hipMemcpy(M.keys_of.back().data(), D.keys[c].get(), M.r.n_groups * eb,
+                                    hipMemcpyDeviceToHost),
+                          "hipMemcpy D2H");
         }
-        check(vdf_groupby(h_.get(), k.key_type, k.data.get(), n, vc.data(), static_cast<int>(vc.size()), aggs, &out,
-                          nullptr));
-        uint64_t g = 0;
-        hip_check(hipMemcpy(&g, ng.get(), 8, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+        return M;
+    }
+
+    // transform_group_by (transform_group_by.hpp:11): append a per-row float64
+    // column holding the aggregate of the row's group, as CreateValueCol does
+    // (create_value_col.inl:36-61), for one or more key columns x.  The
+    // group-by and the broadcast both run on the device.  Mean = sum / count.
+    template <GroupFunction F>
+    void transform_group_by(const std::vector<unsigned int>& x, unsigned int n_col, const std::string& colname = "n")
+    {
+        const std::vector<unsigned int> vals = (F == GroupFunction::Occurence) ? std::vector<unsigned int>{}
+                                                                               : std::vector<unsigned int>{n_col};
+        DeviceGroups D = group_by_device(x, vals);
+        const void* src = nullptr;
+        int src_type = VDF_VAL_U32;
+        if constexpr (F == GroupFunction::Occurence) {
+            src = D.cnt.get();
+        } else {
+            const Column& v = col(n_col);
+            const bool f = v.val_type == VDF_VAL_F64;
+            const bool u = v.val_type == VDF_VAL_U32 || v.val_type == VDF_VAL_U64;
+            if (F == GroupFunction::Sum) {
+                src = D.agg[0].get();
+                src_type = f ? VDF_VAL_F64 : (u ? VDF_VAL_U64 : VDF_VAL_I64);
+            } else if (F == GroupFunction::Mean) {
+                src = D.agg[1].get();
+                src_type = VDF_VAL_F64;
+            } else {
+                src = D.agg[F == GroupFunction::Min ? 2 : 3].get();
+                src_type = v.val_type;
+            }
+        }
+        Column c;
+        c.name = colname;
+        c.elem = 8;
+        c.key_type = VDF_KEY_F64;
+        c.val_type = VDF_VAL_F64;
+        c.letter = 'd';
+        c.data = DeviceBuffer(static_cast<size_t>(nrow_) * 8);
+        check(vdf_broadcast(h_.get(), src_type, VDF_VAL_F64, src, static_cast<const uint32_t*>(D.off.get()),
+                            static_cast<const uint64_t*>(D.ng.get()), std::max<size_t>(nrow_, 1),
+                            static_cast<const uint32_t*>(D.perm.get()), nrow_, c.data.get(), nullptr));
+        hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        cols_.push_back(std::move(c));
+    }
+
+  private:
+    template <typename K>
+    GroupResult<K> to_host(const DeviceGroups& D, const std::vector<unsigned int>& vals, bool with_keys = true) const
+    {
+        const size_t n = D.n;
+        GroupResult<K> R;
+        const uint64_t g = D.n_groups();
         R.n_groups = g;
-        R.keys.resize(g);
+        if (with_keys) R.keys.resize(g);
         R.offsets.resize(g + 1);
         R.counts.resize(g);
         R.row_view.resize(n);
@@ -274,18 +391,19 @@ class Dataframe {
         auto d2h = [](void* dst, const void* src, size_t bytes) {
             if (bytes) hip_check(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hipMemcpy D2H");
         };
-        d2h(R.keys.data(), gkeys.get(), g * sizeof(K));
-        d2h(R.offsets.data(), off.get(), (g + 1) * 4);
-        d2h(R.counts.data(), cnt.get(), g * 4);
-        d2h(R.row_view.data(), perm.get(), n * 4);
+        if (with_keys) d2h(R.keys.data(), D.keys[0].get(), g * sizeof(K));
+        d2h(R.offsets.data(), D.off.get(), (g + 1) * 4);
+        d2h(R.counts.data(), D.cnt.get(), g * 4);
+        d2h(R.row_view.data(), D.perm.get(), n * 4);
         for (size_t c = 0; c < vals.size(); ++c) {
             const Column& v = col(vals[c]);
+            const vdf_agg_out& a = D.out.aggs[c];
             if (v.val_type == VDF_VAL_F64) {
                 std::vector<double> s(g), m(g), lo(g), hi(g);
-                d2h(s.data(), out.aggs[c].sum, g * 8);
-                d2h(m.data(), out.aggs[c].mean, g * 8);
-                d2h(lo.data(), out.aggs[c].min, g * 8);
-                d2h(hi.data(), out.aggs[c].max, g * 8);
+                d2h(s.data(), a.sum, g * 8);
+                d2h(m.data(), a.mean, g * 8);
+                d2h(lo.data(), a.min, g * 8);
+                d2h(hi.data(), a.max, g * 8);
                 R.sum_f[c] = std::move(s);
                 R.mean[c] = std::move(m);
                 R.min_f[c] = std::move(lo);
@@ -293,20 +411,20 @@ class Dataframe {
             } else {
                 std::vector<int64_t> s(g), lo(g), hi(g);
                 std::vector<double> m(g);
-                d2h(s.data(), out.aggs[c].sum, g * 8);
-                d2h(m.data(), out.aggs[c].mean, g * 8);
+                d2h(s.data(), a.sum, g * 8);
+                d2h(m.data(), a.mean, g * 8);
                 const bool w4 = v.elem == 4;
                 std::vector<uint32_t> l4(w4 ? g : 0), h4(w4 ? g : 0);
                 if (w4) {
-                    d2h(l4.data(), out.aggs[c].min, g * 4);
-                    d2h(h4.data(), out.aggs[c].max, g * 4);
+                    d2h(l4.data(), a.min, g * 4);
+                    d2h(h4.data(), a.max, g * 4);
                     for (size_t i = 0; i < g; ++i) {
                         lo[i] = v.val_type == VDF_VAL_I32 ? static_cast<int32_t>(l4[i]) : static_cast<int64_t>(l4[i]);
                         hi[i] = v.val_type == VDF_VAL_I32 ? static_cast<int32_t>(h4[i]) : static_cast<int64_t>(h4[i]);
                     }
                 } else {
-                    d2h(lo.data(), out.aggs[c].min, g * 8);
-                    d2h(hi.data(), out.aggs[c].max, g * 8);
+                    d2h(lo.data(), a.min, g * 8);
+                    d2h(hi.data(), a.max, g * 8);
                 }
                 R.sum_i[c] = std::move(s);
                 R.mean[c] = std::move(m);
@@ -317,42 +435,6 @@ class Dataframe {
         return R;
     }
 
-    // transform_group_by (transform_group_by.hpp:11): append a per-row column
-    // holding the aggregate of the row's group, as CreateValueCol does
-    // (create_value_col.inl:36-61).  One key column; Mean = sum / count.
-    template <GroupFunction F>
-    void transform_group_by(const std::vector<unsigned int>& x, unsigned int n_col, const std::string& colname = "n")
-    {
-        if (x.size() != 1) throw Error(VDF_ERR_UNSUPPORTED, "transform_group_by: one key column");
-        const Column& k = col(x[0]);
-        std::vector<double> per_row(nrow_);
-        auto fill = [&](auto R) {
-            for (size_t g = 0; g < R.n_groups; ++g) {
-                double v = 0;
-                if constexpr (F == GroupFunction::Occurence) v = static_cast<double>(R.counts[g]);
-                else {
-                    const bool f = !R.sum_f[0].empty() || !R.min_f[0].empty();
-                    if (F == GroupFunction::Sum) v = f ? R.sum_f[0][g] : static_cast<double>(R.sum_i[0][g]);
-                    if (F == GroupFunction::Mean) v = R.mean[0][g];
-                    if (F == GroupFunction::Min) v = f ? R.min_f[0][g] : static_cast<double>(R.min_i[0][g]);
-                    if (F == GroupFunction::Max) v = f ? R.max_f[0][g] : static_cast<double>(R.max_i[0][g]);
-                }
-                for (uint32_t p = R.offsets[g]; p < R.offsets[g + 1]; ++p) per_row[R.row_view[p]] = v;
-            }
-        };
-        const std::vector<unsigned int> vals = (F == GroupFunction::Occurence) ? std::vector<unsigned int>{}
-                                                                               : std::vector<unsigned int>{n_col};
-        switch (k.key_type) {
-        case VDF_KEY_U32: fill(group_by<uint32_t>(x[0], vals)); break;
-        case VDF_KEY_I32: fill(group_by<int32_t>(x[0], vals)); break;
-        case VDF_KEY_U64: fill(group_by<uint64_t>(x[0], vals)); break;
-        case VDF_KEY_I64: fill(group_by<int64_t>(x[0], vals)); break;
-        default: fill(group_by<Char16>(x[0], vals)); break;
-        }
-        add_col(per_row, colname);
-    }
-
-  private:
     const Column& col(unsigned int n) const
     {
         if (n >= cols_.size()) throw Error(VDF_ERR_INVALID, "column index out of range");

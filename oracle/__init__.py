@@ -15,7 +15,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liboracle.so")
 REF_LIB_PATH = os.path.join(_HERE, "_ref", "libvref.so")
 
-KEY_U32, KEY_I32, KEY_U64, KEY_I64, KEY_CHAR16 = 0, 1, 2, 3, 4
+KEY_U32, KEY_I32, KEY_U64, KEY_I64, KEY_CHAR16, KEY_F32, KEY_F64 = 0, 1, 2, 3, 4, 5, 6
 VAL_F64, VAL_I64, VAL_I32, VAL_U32, VAL_U64 = 0, 1, 2, 3, 4
 
 _lib = None
@@ -77,7 +77,8 @@ def key_type_of(a: np.ndarray) -> int:
     if a.dtype == np.uint8 and a.ndim == 2 and a.shape[1] == 16:
         return KEY_CHAR16
     return {np.dtype(np.uint32): KEY_U32, np.dtype(np.int32): KEY_I32,
-            np.dtype(np.uint64): KEY_U64, np.dtype(np.int64): KEY_I64}[a.dtype]
+            np.dtype(np.uint64): KEY_U64, np.dtype(np.int64): KEY_I64,
+            np.dtype(np.float32): KEY_F32, np.dtype(np.float64): KEY_F64}[a.dtype]
 
 
 def val_type_of(a: np.ndarray) -> int:
@@ -154,3 +155,68 @@ def hash_groupby_f64(keys: np.ndarray, val: np.ndarray, threads: int = 1, refapi
                                        _p(gk), _p(cnt), _p(s), _p(m), _p(mn), _p(mx)))
     return {"n_groups": g, "keys": gk[:g], "counts": cnt[:g], "sum": s[:g], "mean": m[:g], "min": mn[:g],
             "max": mx[:g]}
+
+
+# ---------------------------------------------------------------------------
+# multi-column keys
+# ---------------------------------------------------------------------------
+def order_bits(a: np.ndarray) -> np.ndarray:
+    """Order-preserving unsigned form of one key column, as the sort maps keys:
+    unsigned as is, signed with the sign bit flipped (sort_integers.inl:46,53),
+    floats by the transform of sort_flt.inl:44-50, char[16] bytes ^ 0x80
+    (sort_char.inl:73; returned as a (n, 16) uint8 array)."""
+    if a.ndim == 2:
+        return a ^ np.uint8(0x80)
+    if a.dtype == np.float32:
+        b = a.view(np.uint32)
+        return b ^ np.where(b >> np.uint32(31), np.uint32(0xFFFFFFFF), np.uint32(0x80000000))
+    if a.dtype == np.float64:
+        b = a.view(np.uint64)
+        return b ^ np.where(b >> np.uint64(63), np.uint64(0xFFFFFFFFFFFFFFFF), np.uint64(1 << 63))
+    if a.dtype == np.int32:
+        return a.view(np.uint32) ^ np.uint32(0x80000000)
+    if a.dtype == np.int64:
+        return a.view(np.uint64) ^ np.uint64(1 << 63)
+    return a
+
+
+def argsort_multi(keys_list, mode: int = 0) -> np.ndarray:
+    """Stable argsort by several key columns, lexicographic with the first
+    column most significant; mode as argsort (1: stable descending over the
+    whole key, 2: ascending reversed)."""
+    cols = []  # numpy lexsort: the last entry is the primary key
+    for k in reversed(list(keys_list)):
+        ob = order_bits(np.ascontiguousarray(k))
+        if mode == 1:
+            ob = ~ob
+        if ob.ndim == 2:
+            cols.extend(ob[:, j] for j in range(15, -1, -1))
+        else:
+            cols.append(ob)
+    perm = np.lexsort(cols) if cols and len(cols[0]) else np.zeros(0, dtype=np.int64)
+    if mode == 2:
+        perm = perm[::-1]
+    return perm.astype(np.uint64)
+
+
+def groupby_multi(keys_list, vals=(), threads: int = 1) -> dict:
+    """Group-by on several key columns: rows are equal when every column's
+    bytes are equal (the reference's concatenated key bytes, key_build.inl:3-110);
+    groups in lexicographic column order.  Computed as the single-key group-by
+    of each row's dense rank among the distinct keys, so the aggregates, the
+    permutation and the offsets come from the pinned single-key oracle."""
+    keys_list = [np.ascontiguousarray(k) for k in keys_list]
+    n = keys_list[0].shape[0]
+    perm = argsort_multi(keys_list, 0).astype(np.int64)
+    diff = np.zeros(max(n - 1, 0), dtype=bool)
+    for k in keys_list:
+        b = k.reshape(n, -1).view(np.uint8) if k.ndim == 2 else k.view(np.uint8).reshape(n, -1)
+        s = b[perm]
+        diff |= np.any(s[1:] != s[:-1], axis=1)
+    rank_sorted = np.concatenate([[0], np.cumsum(diff)]).astype(np.uint32) if n else np.zeros(0, np.uint32)
+    rank = np.empty(n, dtype=np.uint32)
+    rank[perm] = rank_sorted
+    r = groupby(rank, list(vals), threads)
+    first = r["perm"][r["offsets"][:-1].astype(np.int64)].astype(np.int64)
+    r["keys"] = [k[first] for k in keys_list]
+    return r

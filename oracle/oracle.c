@@ -198,6 +198,11 @@ static void reverse_u64(uint64_t* a, size_t n)
     for (size_t i = 0, j = n ? n - 1 : 0; i < j; ++i, --j) { uint64_t t = a[i]; a[i] = a[j]; a[j] = t; }
 }
 
+/* Float keys (sort_flt.inl:44-50): mask = -(bits >> 31);
+ * tkeys = bits ^ (mask | signbit), then the unsigned radix. */
+static uint32_t flt_key32(uint32_t b) { return b ^ ((0u - (b >> 31)) | 0x80000000u); }
+static uint64_t flt_key64(uint64_t b) { return b ^ ((0ull - (b >> 63)) | 0x8000000000000000ull); }
+
 int orc_argsort(int key_type, const void* keys, size_t n, int mode, int threads,
                 uint64_t* perm_out)
 {
@@ -205,7 +210,23 @@ int orc_argsort(int key_type, const void* keys, size_t n, int mode, int threads,
     for (size_t i = 0; i < n; ++i) perm_out[i] = i; /* sort_by_mt.hpp:21-24 iota */
     if (n == 0) return 0;
     const int flip = (mode == 1);
-    if (key_type == 0 || key_type == 1) {
+    if (key_type == 5 || key_type == 6) {
+        if (key_type == 5) {
+            const uint32_t* k = (const uint32_t*)keys;
+            uint32_t* tk = (uint32_t*)malloc(n * sizeof(uint32_t));
+            for (size_t i = 0; i < n; ++i) tk[i] = flt_key32(k[i]) ^ (flip ? 0xFFFFFFFFu : 0u);
+            if (threads > 1) orc_radix_sort_u32_mt(tk, perm_out, n, threads);
+            else orc_radix_sort_u32(tk, perm_out, n);
+            free(tk);
+        } else {
+            const uint64_t* k = (const uint64_t*)keys;
+            uint64_t* tk = (uint64_t*)malloc(n * sizeof(uint64_t));
+            for (size_t i = 0; i < n; ++i) tk[i] = flt_key64(k[i]) ^ (flip ? ~0ull : 0ull);
+            if (threads > 1) orc_radix_sort_u64_mt(tk, perm_out, n, threads);
+            else orc_radix_sort_u64(tk, perm_out, n);
+            free(tk);
+        }
+    } else if (key_type == 0 || key_type == 1) {
         const uint32_t* k = (const uint32_t*)keys;
         uint32_t x = (key_type == 1 ? 0x80000000u : 0u) ^ (flip ? 0xFFFFFFFFu : 0u);
         uint32_t* tk = (uint32_t*)malloc(n * sizeof(uint32_t));
@@ -325,7 +346,7 @@ double orc_exact_sum(const double* x, size_t n)
 /* (first of equal values wins); integer sums wrap mod 2^64; integer means   */
 /* use the exact 128-bit sum; f64 sums are correctly rounded.                */
 /* ------------------------------------------------------------------------ */
-static size_t key_bytes(int kt) { return (kt == 0 || kt == 1) ? 4 : (kt == 4 ? 16 : 8); }
+static size_t key_bytes(int kt) { return (kt == 0 || kt == 1 || kt == 5) ? 4 : (kt == 4 ? 16 : 8); }
 static size_t val_bytes(int vt) { return (vt == 2 || vt == 3) ? 4 : 8; }
 
 static int64_t ival(int vt, const void* col, uint64_t r, int* is_unsigned)

@@ -190,6 +190,40 @@ int main()
             ok = s[i] == static_cast<double>(m[f.k32[i]].first) && c[i] == static_cast<double>(m[f.k32[i]].second);
         CHECK(ok);
     }
+    // two key columns: group_by_multi and a per-row transform column over (k64, k32)
+    std::printf("group_by_multi + transform_group_by on two keys\n");
+    {
+        Dataframe df = load(f);
+        const auto M = df.group_by_multi({1, 0}, {3});
+        std::map<std::pair<int64_t, uint32_t>, std::pair<int64_t, uint64_t>> m;
+        for (size_t i = 0; i < n; ++i) {
+            auto& e = m[{f.k64[i], f.k32[i]}];
+            e.first += f.vi[i];
+            e.second += 1;
+        }
+        CHECK(M.r.n_groups == m.size());
+        size_t g = 0;
+        bool ok = true;
+        for (const auto& [key, e] : m) {
+            if (g >= M.r.n_groups) break;
+            int64_t k64;
+            uint32_t k32;
+            std::memcpy(&k64, M.keys_of[0].data() + 8 * g, 8);
+            std::memcpy(&k32, M.keys_of[1].data() + 4 * g, 4);
+            ok = ok && k64 == key.first && k32 == key.second && M.r.counts[g] == e.second && M.r.sum_i[0][g] == e.first;
+            ++g;
+        }
+        CHECK(ok);
+        df.transform_group_by<GroupFunction::Mean>({1, 0}, 3, "mean_vi");
+        const auto mc = df.get_col<double>(5);
+        ok = true;
+        for (size_t i = 0; i < n && ok; ++i) {
+            const auto& e = m[{f.k64[i], f.k32[i]}];
+            const double exp = static_cast<double>(e.first) / static_cast<double>(e.second);
+            ok = std::fabs(mc[i] - exp) <= 1e-15 * std::fabs(exp);
+        }
+        CHECK(ok);
+    }
     // 16-byte string keys: group count and ascending byte order
     std::printf("group_by char16\n");
     {
@@ -207,13 +241,24 @@ int main()
         }
         CHECK(ok);
     }
+    // float64 key column: the order of sort_flt.inl:44-50 (no NaN or signed zero here: numeric order)
+    std::printf("sort_by<true, false> float64\n");
+    {
+        Dataframe df = load(f);
+        unsigned int c = 2;
+        df.sort_by<true, false>(c);
+        const auto idx = stable_order(f.vf);
+        const auto vi = df.get_col<int32_t>(3);
+        bool ok = true;
+        for (size_t i = 0; i < n && ok; ++i) ok = vi[i] == f.vi[idx[i]];
+        CHECK(ok);
+    }
     // errors surface as exceptions with the library's message
     {
         Dataframe df = load(f);
         bool thrown = false;
         try {
-            unsigned int c = 2;  // a float64 column has no radix key path here
-            df.sort_by<true, false>(c);
+            df.group_by<int64_t>(1, {4});  // a char[16] column is not an aggregatable value
         } catch (const vison_gpu::Error& e) {
             thrown = e.status == VDF_ERR_UNSUPPORTED;
         }

@@ -54,3 +54,21 @@ def test_scratch_sizes():
     # two key buffers + two row-id buffers dominate: 16 B/row for u32 keys
     assert 16 * 100_000_000 <= big < 18 * 100_000_000
     assert lib.vdf_argsort_scratch_bytes(N.KEY_CHAR16, 1000, 0, 0) > small
+
+
+def test_header_constants_match_binding():
+    """Every VDF_KEY_/VAL_/AGG_/SORT_/ERR_ constant of include/vdf.h has the
+    same value in the Python binding."""
+    src = open(os.path.join(ROOT, "include", "vdf.h")).read()
+    defs = dict(re.findall(r"#define\s+VDF_([A-Z0-9_]+)\s+([0-9xA-Fa-fu]+)", src))
+    checked = 0
+    for name, text in defs.items():
+        val = int(text.rstrip("uU"), 0)
+        prefix = name.split("_")[0]
+        if prefix in ("KEY", "VAL", "AGG", "SORT"):
+            assert getattr(N, name) == val, name
+            checked += 1
+        elif prefix == "ERR" or name == "OK":
+            assert getattr(N, "VDF_" + name) == val, name
+            checked += 1
+    assert checked >= 25

@@ -1,0 +1,137 @@
+"""GPU parity of multi-column keys (group-by and argsort through the packed
+composite key) and of the per-row broadcast of group results, against the
+oracle."""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from visondf_amd import _native as N
+from visondf_amd import datagen
+
+pytestmark = pytest.mark.gpu
+
+F64_RTOL = 1e-12
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _dev(a: np.ndarray):
+    import torch
+
+    a = np.ascontiguousarray(a)
+    if a.dtype == np.uint32:
+        a = a.view(np.int32)
+    elif a.dtype == np.uint64:
+        a = a.view(np.int64)
+    return torch.from_numpy(a).cuda()
+
+
+def _bytes(t):
+    return t.cpu().numpy().view(np.uint8).reshape(-1)
+
+
+def _key_cols(combo, n, rng):
+    out = []
+    for kind in combo:
+        if kind == "i32":
+            out.append(rng.integers(-30, 30, n).astype(np.int32))
+        elif kind == "u32":
+            out.append(rng.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32) % np.uint32(40))
+        elif kind == "i64":
+            out.append(rng.integers(-2**62, 2**62, n).astype(np.int64) % 17 - 8)
+        elif kind == "u64":
+            out.append(datagen.splitmix64(9, 0, n) % np.uint64(25) * np.uint64(0x9E3779B97F4A7C15))
+        elif kind == "f32":
+            f = rng.integers(-4, 4, n).astype(np.float32) / np.float32(2)
+            f[rng.integers(0, n, n // 50)] = np.float32(-0.0)
+            out.append(f)
+        elif kind == "f64":
+            f = rng.integers(-6, 6, n).astype(np.float64) * 0.25
+            f[rng.integers(0, n, n // 50)] = np.nan
+            out.append(f)
+        else:
+            out.append(datagen.char16_keys(11, 0, n, distinct=12))
+    return out
+
+
+COMBOS = [("i32", "i32"), ("u32",), ("i64",), ("u32", "i64"), ("i64", "f64"), ("char16",), ("f32", "i32", "u32"),
+          ("u64", "i32"), ("i32", "u32", "i32", "u32")]
+
+
+@pytest.mark.parametrize("combo", COMBOS, ids=lambda c: "+".join(c))
+def test_groupby_multi_vs_oracle(vdf, combo):
+    rng = np.random.default_rng(len(combo) * 7 + 1)
+    n = 300_007
+    keys = _key_cols(combo, n, rng)
+    vals = [rng.standard_normal(n), rng.integers(-2**40, 2**40, n).astype(np.int64)]
+    _say("groupby_multi", combo, n)
+    kts = [oracle.key_type_of(k) for k in keys]  # device tensors of unsigned data are signed views
+    r = vdf.groupby_multi([_dev(k) for k in keys], [_dev(v) for v in vals], key_types=kts)
+    o = oracle.groupby_multi(keys, vals, threads=8)
+    assert r.n_groups == o["n_groups"]
+    for c, k in enumerate(keys):
+        assert np.array_equal(_bytes(r.keys[c]), o["keys"][c].view(np.uint8).reshape(-1)), (combo, c)
+    assert np.array_equal(r.counts.cpu().numpy().view(np.uint32).astype(np.uint64), o["counts"])
+    assert np.array_equal(r.offsets.cpu().numpy().view(np.uint32).astype(np.uint64), o["offsets"])
+    assert np.array_equal(r.perm.cpu().numpy().view(np.uint32).astype(np.uint64), o["perm"])
+    a, oa = r.aggs[0], o["aggs"][0]
+    got = a["sum"].cpu().numpy()
+    assert np.all(np.abs(got - oa["sum"]) <= F64_RTOL * np.abs(oa["sum"]))
+    assert np.array_equal(_bytes(a["min"]), oa["min"].view(np.uint8))
+    assert np.array_equal(r.aggs[1]["sum"].cpu().numpy(), o["aggs"][1]["sum"].view(np.int64))
+
+
+@pytest.mark.parametrize("mode", ["asc", "desc", "reverse"])
+@pytest.mark.parametrize("combo", [("i32", "u32"), ("u64", "i32"), ("char16",), ("f64", "i32")],
+                         ids=lambda c: "+".join(c) if isinstance(c, tuple) else c)
+def test_argsort_multi_vs_oracle(vdf, combo, mode):
+    rng = np.random.default_rng(3)
+    keys = _key_cols(combo, 200_003, rng)
+    m = {"asc": 0, "desc": 1, "reverse": 2}[mode]
+    perm = vdf.argsort_multi([_dev(k) for k in keys], key_types=[oracle.key_type_of(k) for k in keys],
+                             descending=(m == 1), reverse_desc=(m == 2))
+    assert np.array_equal(perm.cpu().numpy().view(np.uint32).astype(np.uint64), oracle.argsort_multi(keys, m))
+
+
+def test_multi_key_too_wide_rejected(vdf):
+    import torch
+
+    k = torch.zeros(10, dtype=torch.int64, device="cuda")
+    c = torch.zeros((10, 16), dtype=torch.uint8, device="cuda")
+    with pytest.raises(N.VdfError) as e:
+        vdf.groupby_multi([k, c])
+    assert e.value.status == N.VDF_ERR_UNSUPPORTED
+
+
+@pytest.mark.parametrize("src", ["f64_sum", "i64_sum", "count", "i32_min"])
+def test_broadcast_per_row(vdf, src):
+    rng = np.random.default_rng(12)
+    n = 500_001
+    k = rng.integers(0, 7000, n).astype(np.int64)
+    vf = rng.standard_normal(n)
+    vi = rng.integers(-1000, 1000, n).astype(np.int32)
+    r = vdf.groupby(_dev(k), [_dev(vf), _dev(vi)], trim=False)
+    if src == "f64_sum":
+        gv, to64 = r.aggs[0]["sum"], False
+    elif src == "i64_sum":
+        gv, to64 = r.aggs[1]["sum"], True
+    elif src == "count":
+        gv, to64 = r.counts, True  # counts (int32 cells) as a float64 column
+    else:
+        gv, to64 = r.aggs[1]["min"], False
+    out = vdf.broadcast(gv, r, n, to_f64=to64).cpu().numpy()
+    g = int(r.n_groups.item())
+    gkeys = r.keys[:g].cpu().numpy()
+    gval = gv[:g].cpu().numpy()
+    row_group = np.searchsorted(gkeys, k)
+    exp = gval[row_group]
+    if to64:
+        exp = exp.astype(np.float64)
+    assert np.array_equal(out.view(np.uint8), exp.view(np.uint8)), src

@@ -114,6 +114,39 @@ def test_char16_vs_python_sorted():
     assert oracle.argsort(keys, 0).tolist() == ref
 
 
+# ------------------------------------------------------------- float keys
+# sort_flt.inl:44-50: negative values have all bits complemented, others the
+# sign bit set, so the radix order is -NaN < -inf < ... < -0 < +0 < ... < +inf < +NaN.
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_float_keys_numeric_order(dt, mode):
+    """Without NaN and signed zeros the order is numpy's stable numeric order."""
+    rng = np.random.default_rng(41)
+    keys = np.round(rng.standard_normal(20_000) * 50).astype(dt)  # many ties
+    keys[keys == 0] = 1
+    keys[:10] = np.array([np.inf, -np.inf, 1e-38, -1e-38, 3, 3, -3, 7, 1e30, -1e30], dtype=dt)
+    exp = np.argsort(-keys if mode == 1 else keys, kind="stable")  # negation is exact: no NaN, no zeros
+    if mode == 2:
+        exp = exp[::-1]
+    assert np.array_equal(oracle.argsort(keys, mode, 3), exp.astype(np.uint64))
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_float_keys_signed_zero_and_nan(dt):
+    """Known answers of the reference's float transform for the values numeric
+    order leaves open."""
+    nan, neg_nan = dt(np.nan), np.copysign(dt(np.nan), dt(-1))
+    keys = np.array([nan, -0.0, 1.0, neg_nan, 0.0, -np.inf, -0.0, np.inf, -1.0], dtype=dt)
+    # -nan(3) < -inf(5) < -1(8) < -0(1) = -0(6) < +0(4) < 1(2) < inf(7) < nan(0)
+    assert oracle.argsort(keys, 0).tolist() == [3, 5, 8, 1, 6, 4, 2, 7, 0]
+    assert oracle.argsort(keys, 1).tolist() == [0, 7, 2, 4, 1, 6, 8, 5, 3]
+    assert oracle.argsort(keys, 2).tolist() == [0, 7, 2, 4, 6, 1, 8, 5, 3]
+    r = oracle.groupby(keys, [np.arange(9, dtype=np.int64)])
+    assert r["n_groups"] == 8  # the two -0.0 rows form one group; -0.0 and +0.0 differ
+    assert r["counts"].tolist() == [1, 1, 1, 2, 1, 1, 1, 1]
+    assert r["aggs"][0]["sum"].tolist() == [3, 5, 8, 7, 4, 2, 7, 0]
+
+
 def test_radix_planted_mistake_is_caught():
     """A sort that ignores stability must disagree with the pinned answer."""
     keys = DOC_COL3
@@ -252,3 +285,62 @@ def test_hash_groupby_matches_sorted(threads, refapi):
     # first-occurrence order (ankerl iterates in insertion order)
     _, first = np.unique(key, return_index=True)
     assert np.array_equal(h["keys"], key[np.sort(first)])
+
+
+# ------------------------------------------------------ multi-column keys
+def test_groupby_multi_vs_pandas():
+    import pandas as pd
+
+    rng = np.random.default_rng(77)
+    n = 30_000
+    k1 = rng.integers(-20, 20, n).astype(np.int32)
+    k2 = rng.integers(0, 50, n).astype(np.uint64)
+    k3 = rng.integers(-3, 3, n).astype(np.int64)
+    v = rng.integers(-10**9, 10**9, n).astype(np.int64)
+    r = oracle.groupby_multi([k1, k2, k3], [v], threads=3)
+    df = pd.DataFrame({"a": k1, "b": k2, "c": k3, "v": v})
+    g = df.groupby(["a", "b", "c"], sort=True)["v"]
+    agg = g.agg(["count", "sum", "min", "max"])
+    assert r["n_groups"] == len(agg)
+    idx = agg.index.to_frame(index=False)
+    assert np.array_equal(r["keys"][0], idx["a"].to_numpy().astype(np.int32))
+    assert np.array_equal(r["keys"][1], idx["b"].to_numpy().astype(np.uint64))
+    assert np.array_equal(r["keys"][2], idx["c"].to_numpy().astype(np.int64))
+    assert np.array_equal(r["counts"], agg["count"].to_numpy().astype(np.uint64))
+    assert np.array_equal(r["aggs"][0]["sum"], agg["sum"].to_numpy())
+    assert np.array_equal(r["aggs"][0]["min"], agg["min"].to_numpy())
+    assert np.array_equal(r["aggs"][0]["max"], agg["max"].to_numpy())
+    # the permutation lists each group's rows in row order, groups in key order
+    order = sorted(range(n), key=lambda i: (int(k1[i]), int(k2[i]), int(k3[i]), i))
+    assert r["perm"].tolist() == order
+
+
+def test_groupby_multi_brute_force_bytes_and_floats():
+    """char[16] and float columns: groups are equal bytes; -0.0 and +0.0 differ."""
+    rng = np.random.default_rng(5)
+    n = 600
+    c = datagen.char16_keys(2, 0, n, distinct=4)
+    f = np.array([0.0, -0.0, 1.5, -2.0], dtype=np.float32)[rng.integers(0, 4, n)]
+    v = rng.integers(0, 100, n).astype(np.int64)
+    r = oracle.groupby_multi([c, f], [v])
+    d = {}
+    for i in range(n):
+        d.setdefault((bytes(c[i]), int(oracle.order_bits(f[i:i + 1])[0])), []).append(int(v[i]))
+    ks = sorted(d)
+    assert r["n_groups"] == len(ks)
+    assert [bytes(x) for x in r["keys"][0]] == [k[0] for k in ks]
+    assert r["aggs"][0]["sum"].tolist() == [sum(d[k]) for k in ks]
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_argsort_multi_vs_python_sorted(mode):
+    rng = np.random.default_rng(6)
+    n = 5000
+    k1 = rng.integers(-5, 5, n).astype(np.int64)
+    k2 = rng.standard_normal(n).round(1)
+    k2[k2 == 0] = 0.5  # no signed zeros: Python orders -0.0 == 0.0
+    key = [(int(k1[i]), float(k2[i])) for i in range(n)]
+    exp = sorted(range(n), key=lambda i: key[i], reverse=(mode == 1))  # stable under reverse too
+    if mode == 2:
+        exp = sorted(range(n), key=lambda i: key[i])[::-1]
+    assert oracle.argsort_multi([k1, k2], mode).tolist() == exp

@@ -75,6 +75,11 @@ def _key_sets():
     yield "i64_equal", np.full(77_777, -5, dtype=np.int64)
     yield "char16", datagen.char16_keys(4, 0, 60_001)
     yield "char16_few", datagen.char16_keys(4, 0, 60_001, distinct=9)
+    for dt in (np.float32, np.float64):
+        f = np.round(rng.standard_normal(100_003) * 20).astype(dt)  # ties, +0.0
+        sp = np.array([np.nan, np.copysign(np.nan, -1), np.inf, -np.inf, 0.0, -0.0, 1e-40, -1e-40], dtype=dt)
+        f[rng.integers(0, f.shape[0], 4000)] = sp[rng.integers(0, sp.shape[0], 4000)]
+        yield f"{np.dtype(dt).name}_specials", f
 
 
 @pytest.mark.parametrize("mode", ["asc", "desc", "reverse"])
@@ -87,8 +92,8 @@ def test_argsort_modes_vs_oracle(vdf, mode):
                                   reverse_desc=(mode == "reverse"), want_keys=True)
         got = _perm_host(perm)
         assert np.array_equal(got, exp.astype(np.uint32)), name
-        sk = skeys.cpu().numpy()
-        assert np.array_equal(sk.view(keys.dtype) if keys.ndim == 1 else sk, keys[exp.astype(np.int64)]), name
+        sk = skeys.cpu().numpy()  # bytes compared: NaN keys come back bit-exact
+        assert np.array_equal(sk.view(np.uint8), keys[exp.astype(np.int64)].view(np.uint8)), name
 
 
 def test_argsort_perm64(vdf):

@@ -22,7 +22,7 @@ VDF_ERR_BUDGET = 5
 VDF_ERR_TIMEOUT = 6
 
 # key types
-KEY_U32, KEY_I32, KEY_U64, KEY_I64, KEY_CHAR16 = 0, 1, 2, 3, 4
+KEY_U32, KEY_I32, KEY_U64, KEY_I64, KEY_CHAR16, KEY_F32, KEY_F64 = 0, 1, 2, 3, 4, 5, 6
 # value types
 VAL_F64, VAL_I64, VAL_I32, VAL_U32, VAL_U64 = 0, 1, 2, 3, 4
 # argsort flags
@@ -32,6 +32,7 @@ AGG_COUNT, AGG_SUM, AGG_MEAN, AGG_MIN, AGG_MAX, AGG_ALL = 0x1, 0x2, 0x4, 0x8, 0x
 
 MAX_ROWS = (1 << 30) - 1
 MAX_VALUE_COLS = 4
+MAX_KEY_COLS = 4
 
 # every symbol include/vdf.h declares
 EXPORTS = (
@@ -45,6 +46,7 @@ EXPORTS = (
     "vdf_server_wait", "vdf_server_release", "vdf_server_stats_get",
     "vdf_autotune_argsort", "vdf_autotune_groupby_batch",
     "vdf_plan_groupby_batch", "vdf_plan_launch", "vdf_plan_scratch_bytes", "vdf_plan_destroy",
+    "vdf_groupby_multi", "vdf_argsort_multi", "vdf_broadcast",
 )
 
 
@@ -82,6 +84,10 @@ class VdfServed(C.Structure):
 class VdfServerStats(C.Structure):
     _fields_ = [(k, C.c_uint64) for k in ("submitted", "completed", "failed", "launches", "retries", "rows",
                                           "queued", "arena_bytes", "arena_used", "arena_peak")]
+
+
+class VdfKeyCol(C.Structure):
+    _fields_ = [("type", C.c_int), ("data", C.c_void_p)]
 
 
 class VdfTuneReport(C.Structure):
@@ -175,6 +181,13 @@ def load() -> C.CDLL:
     lib.vdf_plan_scratch_bytes.argtypes = [vp, C.POINTER(u64)]
     lib.vdf_plan_destroy.restype = i32
     lib.vdf_plan_destroy.argtypes = [vp]
+    lib.vdf_groupby_multi.restype = i32
+    lib.vdf_groupby_multi.argtypes = [vp, C.POINTER(VdfKeyCol), i32, u64, C.POINTER(VdfValueCol), i32, u32,
+                                      C.POINTER(VdfGroupbyOut), C.POINTER(vp), vp]
+    lib.vdf_argsort_multi.restype = i32
+    lib.vdf_argsort_multi.argtypes = [vp, C.POINTER(VdfKeyCol), i32, u64, u32, vp, vp]
+    lib.vdf_broadcast.restype = i32
+    lib.vdf_broadcast.argtypes = [vp, i32, i32, vp, vp, vp, u64, vp, u64, vp, vp]
     lib.vdf_tune_set.restype = i32
     lib.vdf_tune_set.argtypes = [vp, C.c_char_p, C.c_int64]
     lib.vdf_tune_get.restype = i32

@@ -372,6 +372,120 @@ vdf_status vdf_groupby_batch(vdf_handle* h, int key_type, const vdf_table* table
     });
 }
 
+static std::vector<int> key_types_of(const vdf_key_col* keys, int nkeys, uint64_t n, const char* who)
+{
+    require(keys != nullptr, "null key column list");
+    require(nkeys >= 1 && nkeys <= VDF_MAX_KEY_COLS, "nkeys out of range (1 .. VDF_MAX_KEY_COLS)");
+    std::vector<int> types(static_cast<size_t>(nkeys));
+    for (int c = 0; c < nkeys; ++c) {
+        require(key_bytes(keys[c].type) != 0, "unknown key type");
+        require(n == 0 || keys[c].data != nullptr, "null key column data");
+        types[c] = keys[c].type;
+    }
+    if (composite_key_type(types.data(), nkeys) < 0)
+        throw Error{VDF_ERR_UNSUPPORTED, std::string(who) + ": key columns wider than 16 bytes together"};
+    return types;
+}
+
+vdf_status vdf_groupby_multi(vdf_handle* h, const vdf_key_col* keys, int nkeys, uint64_t n,
+                             const vdf_value_col* vals, int nvals, uint32_t agg_mask, vdf_groupby_out* out,
+                             void* const* group_keys, void* stream)
+{
+    return guarded([&] {
+        require(h != nullptr && out != nullptr, "vdf_groupby_multi: null handle or output");
+        const std::vector<int> types = key_types_of(keys, nkeys, n, "vdf_groupby_multi");
+        require(nvals >= 0 && nvals <= VDF_MAX_VALUE_COLS, "vdf_groupby_multi: nvals out of range");
+        require(nvals == 0 || vals != nullptr, "vdf_groupby_multi: null value columns");
+        require((agg_mask & ~VDF_AGG_ALL) == 0, "vdf_groupby_multi: unknown aggregate bits");
+        require(n <= VDF_MAX_ROWS, "vdf_groupby_multi: n exceeds VDF_MAX_ROWS (2^30 - 1)");
+        require(out->n_groups != nullptr && out->offsets != nullptr,
+                "vdf_groupby_multi: n_groups and offsets are required");
+        std::vector<int> vts(static_cast<size_t>(std::max(nvals, 1)), 0);
+        GbTable t{};
+        for (int c = 0; c < nvals; ++c) {
+            require(val_bytes(vals[c].type) != 0, "vdf_groupby_multi: unknown value type");
+            require(n == 0 || vals[c].data != nullptr, "vdf_groupby_multi: null value column data");
+            vts[c] = vals[c].type;
+            t.vals[c] = vals[c].data;
+        }
+        CallScope cs(h);
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        const int kt = composite_key_type(types.data(), nkeys);
+        const size_t cb = key_bytes(kt);
+        const uint64_t cap = (out->capacity == 0 || out->capacity > n) ? n : out->capacity;
+        const uint64_t hr = static_cast<uint64_t>(h->tune.hist_rows);
+        const size_t gb = groupby_batch_scratch(kt, &n, 1, hr);
+        void* scr = h->ensure_scratch(align_up(n * cb, 256) + align_up(std::max<uint64_t>(cap, 1) * cb, 256) + gb +
+                                      512);
+        Carve cv(scr);
+        void* comp = cv.take<char>(n * cb);
+        void* gkeys = cv.take<char>(std::max<uint64_t>(cap, 1) * cb);
+        void* gscr = cv.take<char>(gb);
+        std::vector<const void*> data(static_cast<size_t>(nkeys));
+        for (int c = 0; c < nkeys; ++c) data[c] = keys[c].data;
+        pack_keys_enqueue(h, types.data(), data.data(), nkeys, n, comp, s);
+        t.keys = comp;
+        t.n = n;
+        vdf_groupby_out o = *out;
+        o.group_keys = gkeys;
+        h->staging_begin(groupby_batch_staging(kt, &n, 1, hr));
+        {
+            StagingScope st(h, s);
+            groupby_batch_enqueue(h, kt, &t, 1, vts.data(), nvals, agg_mask, &o, gscr, s);
+        }
+        if (group_keys) unpack_keys_enqueue(h, types.data(), group_keys, nkeys, gkeys, out->n_groups, cap, s);
+    });
+}
+
+vdf_status vdf_argsort_multi(vdf_handle* h, const vdf_key_col* keys, int nkeys, uint64_t n, uint32_t flags,
+                             void* d_perm_out, void* stream)
+{
+    return guarded([&] {
+        require(h != nullptr, "vdf_argsort_multi: null handle");
+        const std::vector<int> types = key_types_of(keys, nkeys, n, "vdf_argsort_multi");
+        require((flags & ~(VDF_SORT_DESC | VDF_SORT_DESC_REVERSE | VDF_SORT_PERM64)) == 0,
+                "vdf_argsort_multi: unknown flag bits");
+        require(!((flags & VDF_SORT_DESC) && (flags & VDF_SORT_DESC_REVERSE)),
+                "vdf_argsort_multi: VDF_SORT_DESC and VDF_SORT_DESC_REVERSE are exclusive");
+        require(n <= VDF_MAX_ROWS, "vdf_argsort_multi: n exceeds VDF_MAX_ROWS (2^30 - 1)");
+        if (n == 0) return;
+        require(d_perm_out != nullptr, "vdf_argsort_multi: null permutation output");
+        CallScope cs(h);
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        const int kt = composite_key_type(types.data(), nkeys);
+        const size_t cb = key_bytes(kt);
+        const uint64_t hr = static_cast<uint64_t>(h->tune.hist_rows);
+        const size_t as = argsort_batch_scratch(kt, &n, 1, hr);
+        void* scr = h->ensure_scratch(align_up(n * cb, 256) + as + 512);
+        Carve cv(scr);
+        void* comp = cv.take<char>(n * cb);
+        void* ascr = cv.take<char>(as);
+        std::vector<const void*> data(static_cast<size_t>(nkeys));
+        for (int c = 0; c < nkeys; ++c) data[c] = keys[c].data;
+        pack_keys_enqueue(h, types.data(), data.data(), nkeys, n, comp, s);
+        h->staging_begin(argsort_batch_staging(kt, &n, 1, hr));
+        StagingScope st(h, s);
+        argsort_enqueue(h, kt, comp, n, flags, d_perm_out, nullptr, ascr, s);
+    });
+}
+
+vdf_status vdf_broadcast(vdf_handle* h, int src_type, int dst_type, const void* d_group_vals,
+                         const uint32_t* d_offsets, const uint64_t* d_n_groups, uint64_t capacity,
+                         const uint32_t* d_perm, uint64_t n, void* d_out, void* stream)
+{
+    return guarded([&] {
+        require(h != nullptr, "vdf_broadcast: null handle");
+        require(val_bytes(src_type) != 0, "vdf_broadcast: unknown source type");
+        require(dst_type == src_type || dst_type == VDF_VAL_F64, "vdf_broadcast: dst_type must be src_type or F64");
+        require(n <= VDF_MAX_ROWS, "vdf_broadcast: n exceeds VDF_MAX_ROWS (2^30 - 1)");
+        if (n == 0) return;
+        require(d_group_vals && d_offsets && d_n_groups && d_perm && d_out, "vdf_broadcast: null device pointer");
+        CallScope cs(h);
+        broadcast_enqueue(h, src_type, dst_type, d_group_vals, d_offsets, d_n_groups, capacity, d_perm, n, d_out,
+                          static_cast<hipStream_t>(stream));
+    });
+}
+
 static int* tune_slot(Tuning& t, const std::string& n)
 {
     if (n == "sort_items_32") return &t.sort_items_32;

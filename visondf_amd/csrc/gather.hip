@@ -7,6 +7,7 @@
 #include "vdf_internal.h"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace vdf {
 
@@ -57,6 +58,78 @@ static void launch_gather(const Tuning& t, const void* src, const void* perm, ui
     if (t.gather_rows == 4) return launch_rows<V, P, 4, false>(src, perm, n, dst, s);
     if (t.gather_rows == 16) return launch_rows<V, P, 16, false>(src, perm, n, dst, s);
     return launch_rows<V, P, 8, false>(src, perm, n, dst, s);
+}
+
+// ---------------------------------------------------------------------------
+// Per-row broadcast of per-group values: out[perm[p]] = val[g] for every
+// sorted position p of group g (offsets[g] <= p < offsets[g+1]), converted to
+// the output type.  This is CreateValueCol of the reference
+// (create_value_col.inl:36-61), driven by the group-by's outputs instead of a
+// per-row hash lookup.  Each thread takes 16 consecutive sorted positions:
+// one binary search over the offsets for the first, then a forward walk.
+// Nothing is written when the group count exceeds the arrays' capacity.
+// ---------------------------------------------------------------------------
+template <typename S, typename D>
+__global__ __launch_bounds__(256) void k_broadcast(const S* __restrict__ gval, const uint32_t* __restrict__ offsets,
+                                                   const uint64_t* __restrict__ n_groups, uint64_t cap,
+                                                   const uint32_t* __restrict__ perm, uint64_t n, D* __restrict__ out)
+{
+    constexpr int R = 16;
+    const uint64_t p0 = (static_cast<uint64_t>(blockIdx.x) * 256 + threadIdx.x) * R;
+    if (p0 >= n) return;
+    const uint64_t G = *n_groups;
+    if (G == 0 || G > cap) return;
+    // last group whose first row is <= p0 (offsets[0] = 0, offsets[G] = n > p0)
+    uint64_t lo = 0, hi = G;
+    while (hi - lo > 1) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (offsets[mid] <= p0) lo = mid;
+        else hi = mid;
+    }
+    uint64_t g = lo;
+    uint64_t next = offsets[g + 1];
+    S v = gval[g];
+    for (int j = 0; j < R; ++j) {
+        const uint64_t p = p0 + j;
+        if (p >= n) break;
+        while (p >= next) {  // groups are non-empty: one step per group boundary
+            ++g;
+            next = offsets[g + 1];
+            v = gval[g];
+        }
+        out[perm[p]] = static_cast<D>(v);
+    }
+}
+
+template <typename S>
+static void launch_broadcast_from(int dst_type, const void* gval, const uint32_t* offsets, const uint64_t* ng,
+                                  uint64_t cap, const uint32_t* perm, uint64_t n, void* out, hipStream_t s)
+{
+    const unsigned grid = static_cast<unsigned>((n + 256 * 16 - 1) / (256 * 16));
+    if (dst_type == VDF_VAL_F64 && !std::is_same<S, double>::value)
+        hipLaunchKernelGGL((k_broadcast<S, double>), dim3(grid), dim3(256), 0, s, static_cast<const S*>(gval),
+                           offsets, ng, cap, perm, n, static_cast<double*>(out));
+    else
+        hipLaunchKernelGGL((k_broadcast<S, S>), dim3(grid), dim3(256), 0, s, static_cast<const S*>(gval), offsets,
+                           ng, cap, perm, n, static_cast<S*>(out));
+    check_launch("k_broadcast");
+}
+
+void broadcast_enqueue(Handle* h, int src_type, int dst_type, const void* gval, const uint32_t* offsets,
+                       const uint64_t* n_groups, uint64_t cap, const uint32_t* perm, uint64_t n, void* out,
+                       hipStream_t s)
+{
+    if (n == 0) return;
+    const uint64_t ob = dst_type == VDF_VAL_F64 ? 8 : static_cast<uint64_t>(val_bytes(src_type));
+    LaunchTimer t(h, s, "broadcast", n * (4 + ob));
+    switch (src_type) {
+    case VDF_VAL_F64: launch_broadcast_from<double>(dst_type, gval, offsets, n_groups, cap, perm, n, out, s); break;
+    case VDF_VAL_I64: launch_broadcast_from<int64_t>(dst_type, gval, offsets, n_groups, cap, perm, n, out, s); break;
+    case VDF_VAL_U64: launch_broadcast_from<uint64_t>(dst_type, gval, offsets, n_groups, cap, perm, n, out, s); break;
+    case VDF_VAL_I32: launch_broadcast_from<int32_t>(dst_type, gval, offsets, n_groups, cap, perm, n, out, s); break;
+    case VDF_VAL_U32: launch_broadcast_from<uint32_t>(dst_type, gval, offsets, n_groups, cap, perm, n, out, s); break;
+    default: throw Error{VDF_ERR_INVALID, "vdf_broadcast: unknown source type"};
+    }
 }
 
 void gather_enqueue(Handle* h, uint32_t elem_bytes, const void* d_src, const void* d_perm, int perm64,

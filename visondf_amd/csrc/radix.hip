@@ -20,9 +20,10 @@
 //                    persistent, see k_pass.
 //
 // Keys are mapped to an unsigned order-preserving form on first load
-// (u32/u64: identity, i32/i64: sign bit flipped, char[16]: bytes ^ 0x80 read
-// big-endian; descending: all bits complemented) and mapped back when the
-// last pass writes the sorted keys.
+// (u32/u64: identity, i32/i64: sign bit flipped, f32/f64: sign bit set for
+// positive values and all bits complemented for negative ones, char[16]:
+// bytes ^ 0x80 read big-endian; descending: all bits complemented) and mapped
+// back when the last pass writes the sorted keys.
 #include "vdf_internal.h"
 
 #include <algorithm>
@@ -36,16 +37,24 @@ struct K128 {
 template <typename K>
 struct KeyOps;
 
+// 4- and 8-byte keys: `mlo` is the xor mask (sign flip, descending
+// complement); `mhi` != 0 marks IEEE floats, whose order-preserving form is
+// the bits with the sign bit set for positive values and all bits
+// complemented for negative ones (sort_flt.inl:44-50).
 template <>
 struct KeyOps<uint32_t> {
     static constexpr int kDigits = 4;
-    __device__ static uint32_t load(const void* p, uint64_t i, uint64_t mlo, uint64_t)
+    __device__ static uint32_t load(const void* p, uint64_t i, uint64_t mlo, uint64_t mhi)
     {
-        return static_cast<const uint32_t*>(p)[i] ^ static_cast<uint32_t>(mlo);
+        uint32_t x = static_cast<const uint32_t*>(p)[i];
+        if (mhi) x ^= (x >> 31) ? 0xFFFFFFFFu : 0x80000000u;
+        return x ^ static_cast<uint32_t>(mlo);
     }
-    __device__ static void store(void* p, uint64_t i, uint32_t k, uint64_t mlo, uint64_t)
+    __device__ static void store(void* p, uint64_t i, uint32_t k, uint64_t mlo, uint64_t mhi)
     {
-        static_cast<uint32_t*>(p)[i] = k ^ static_cast<uint32_t>(mlo);
+        uint32_t x = k ^ static_cast<uint32_t>(mlo);
+        if (mhi) x = (x >> 31) ? (x ^ 0x80000000u) : ~x;
+        static_cast<uint32_t*>(p)[i] = x;
     }
     __device__ static uint32_t digit(uint32_t k, int d) { return (k >> (8 * d)) & 0xFFu; }
     __device__ static uint32_t zero() { return 0u; }
@@ -54,13 +63,17 @@ struct KeyOps<uint32_t> {
 template <>
 struct KeyOps<uint64_t> {
     static constexpr int kDigits = 8;
-    __device__ static uint64_t load(const void* p, uint64_t i, uint64_t mlo, uint64_t)
+    __device__ static uint64_t load(const void* p, uint64_t i, uint64_t mlo, uint64_t mhi)
     {
-        return static_cast<const uint64_t*>(p)[i] ^ mlo;
+        uint64_t x = static_cast<const uint64_t*>(p)[i];
+        if (mhi) x ^= (x >> 63) ? ~0ull : 0x8000000000000000ull;
+        return x ^ mlo;
     }
-    __device__ static void store(void* p, uint64_t i, uint64_t k, uint64_t mlo, uint64_t)
+    __device__ static void store(void* p, uint64_t i, uint64_t k, uint64_t mlo, uint64_t mhi)
     {
-        static_cast<uint64_t*>(p)[i] = k ^ mlo;
+        uint64_t x = k ^ mlo;
+        if (mhi) x = (x >> 63) ? (x ^ 0x8000000000000000ull) : ~x;
+        static_cast<uint64_t*>(p)[i] = x;
     }
     __device__ static uint32_t digit(uint64_t k, int d)
     {
@@ -591,9 +604,11 @@ size_t key_bytes(int key_type)
 {
     switch (key_type) {
     case VDF_KEY_U32:
-    case VDF_KEY_I32: return 4;
+    case VDF_KEY_I32:
+    case VDF_KEY_F32: return 4;
     case VDF_KEY_U64:
-    case VDF_KEY_I64: return 8;
+    case VDF_KEY_I64:
+    case VDF_KEY_F64: return 8;
     case VDF_KEY_CHAR16: return 16;
     default: return 0;
     }
@@ -861,16 +876,28 @@ static void launch_sort(Handle* h, const SortSeg* segs, int nsegs, uint64_t mlo,
     const uint64_t kout = segs[0].keys_out ? kb : 0;
     const uint64_t pass_bytes[4] = {n * (kb + kout + pb_out), n * (kb + kb + 4), n * (2 * kb + 8),
                                     n * (kb + 4 + kout + pb_out)};
+    bool coop = xcd;
     for (int slot = 0; slot < D; ++slot) {
         LaunchTimer t(h, s, "radix_pass", passes_host ? 0 : pass_bytes[2], passes_host ? slot : -1, passes_host,
                       pass_bytes);
         uint32_t sl = static_cast<uint32_t>(slot);
         void* args[] = {&a, &sl};
-        if (xcd)
-            VDF_HIP_CHECK(
-                hipLaunchCooperativeKernel(reinterpret_cast<const void*>(pass), dim3(grid), dim3(kBlock), args, 0, s));
-        else
-            VDF_HIP_CHECK(hipLaunchKernel(reinterpret_cast<const void*>(pass), dim3(grid), dim3(kBlock), args, 0, s));
+        if (coop) {
+            const hipError_t e =
+                hipLaunchCooperativeKernel(reinterpret_cast<const void*>(pass), dim3(grid), dim3(kBlock), args, 0, s);
+            if (e == hipSuccess) continue;
+            // The runtime can refuse cooperative launches (for instance while another
+            // process on the device holds the cooperative queue); nothing was
+            // enqueued.  Each pass is a complete stable scatter whatever order
+            // its tiles are claimed in, so this and the remaining passes run in
+            // global tile order, which needs no co-residency.
+            (void)e;
+            (void)hipGetLastError();
+            coop = false;
+            a.groups = 1;  // global tile order: no co-residency needed
+            a.chunk = 1;
+        }
+        VDF_HIP_CHECK(hipLaunchKernel(reinterpret_cast<const void*>(pass), dim3(grid), dim3(kBlock), args, 0, s));
     }
 }
 
@@ -887,6 +914,14 @@ static void masks_for(int key_type, uint32_t flags, uint64_t& mlo, uint64_t& mhi
     case VDF_KEY_I64:
         mlo = (key_type == VDF_KEY_I64 ? 0x8000000000000000ull : 0ull) ^ (desc ? ~0ull : 0ull);
         mhi = 0;
+        break;
+    case VDF_KEY_F32:
+        mlo = desc ? 0xFFFFFFFFull : 0ull;
+        mhi = 1;  // float order transform
+        break;
+    case VDF_KEY_F64:
+        mlo = desc ? ~0ull : 0ull;
+        mhi = 1;
         break;
     case VDF_KEY_CHAR16:
         mlo = mhi = 0x8080808080808080ull ^ (desc ? ~0ull : 0ull);

@@ -241,6 +241,10 @@ void argsort_batch_enqueue(Handle* h, int key_type, const SortSeg* segs, int nse
 
 void gather_enqueue(Handle* h, uint32_t elem_bytes, const void* d_src, const void* d_perm,
                     int perm64, uint64_t n, void* d_dst, hipStream_t s);
+// out[perm[p]] = group value of sorted position p (per-row broadcast)
+void broadcast_enqueue(Handle* h, int src_type, int dst_type, const void* gval, const uint32_t* offsets,
+                       const uint64_t* n_groups, uint64_t cap, const uint32_t* perm, uint64_t n, void* out,
+                       hipStream_t s);
 
 // One table of a group-by batch.
 struct GbTable {
@@ -257,6 +261,14 @@ void groupby_enqueue(Handle* h, int key_type, const void* d_keys, uint64_t n,
                      const vdf_groupby_out* out, void* scr, hipStream_t s);
 void groupby_batch_enqueue(Handle* h, int key_type, const GbTable* tables, int ntables, const int* val_types,
                            int nvals, uint32_t agg_mask, const vdf_groupby_out* outs, void* scr, hipStream_t s);
+
+// multi-column keys: composite key type (-1 when wider than 16 bytes), pack
+// the columns into composites, unpack composite group keys into the columns
+int composite_key_type(const int* types, int ncols);
+void pack_keys_enqueue(Handle* h, const int* types, const void* const* data, int ncols, uint64_t n, void* comp,
+                       hipStream_t s);
+void unpack_keys_enqueue(Handle* h, const int* types, void* const* outs, int ncols, const void* comp,
+                         const uint64_t* n_groups, uint64_t cap, hipStream_t s);
 
 // fill kernel (graph-safe replacement for hipMemsetAsync)
 void fill_u32(uint32_t* p, size_t count, uint32_t value, hipStream_t s);

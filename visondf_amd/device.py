@@ -18,6 +18,8 @@ _KEY_DTYPES = {
     N.KEY_U64: (torch.uint64, 8),
     N.KEY_I64: (torch.int64, 8),
     N.KEY_CHAR16: (torch.uint8, 16),
+    N.KEY_F32: (torch.float32, 4),
+    N.KEY_F64: (torch.float64, 8),
 }
 _VAL_DTYPES = {
     N.VAL_F64: torch.float64,
@@ -227,6 +229,69 @@ class Vdf:
         N.check(self.lib.vdf_argsort_batch(self.h, kt, segs, len(keys_list), flags,
                                            C.c_void_p(_stream_ptr(stream))))
         return res
+
+    def groupby_multi(self, keys, vals=(), aggs: int = N.AGG_ALL, want_perm: bool = True, stream=None,
+                      trim: bool = True, key_types=None, val_types=None, capacity: int = 0):
+        """Group by several key columns (widths summing to <= 16 bytes); the
+        result's keys is a list with one tensor of group keys per column."""
+        keys = list(keys)
+        n = keys[0].shape[0]
+        kts = [key_type_of(k) if key_types is None else key_types[c] for c, k in enumerate(keys)]
+        vts = [val_type_of(v) if val_types is None else val_types[c] for c, v in enumerate(vals)]
+        out, t = self._alloc_out(keys[0], vts, [v.dtype for v in vals], aggs, want_perm, capacity)
+        cap = t["cap"]
+        gk = [torch.empty((max(cap, 1),) + tuple(k.shape[1:]), dtype=k.dtype, device=k.device) for k in keys]
+        kc = (N.VdfKeyCol * len(keys))()
+        for c, k in enumerate(keys):
+            kc[c].type = kts[c]
+            kc[c].data = _ptr(k)
+        cols = (N.VdfValueCol * max(len(vals), 1))()
+        for c, v in enumerate(vals):
+            cols[c].type = vts[c]
+            cols[c].data = v.data_ptr()
+        gptrs = (C.c_void_p * len(keys))(*[g.data_ptr() for g in gk])
+        N.check(self.lib.vdf_groupby_multi(self.h, kc, len(keys), n, cols, len(vals), aggs, C.byref(out), gptrs,
+                                           C.c_void_p(_stream_ptr(stream))))
+        t["keys"] = gk
+        if not trim:
+            return GroupbyResult(t["ng"], gk, t["offsets"], t["counts"], t["perm"], t["aggs"])
+        g = int(t["ng"].item())
+        if g > cap:
+            raise N.VdfError(N.VDF_ERR_INVALID, f"{g} groups exceed the output capacity {cap}")
+        return GroupbyResult(g, [x[:g] for x in gk], t["offsets"][:g + 1], t["counts"][:g],
+                             t["perm"][:n] if t["perm"] is not None else None,
+                             [{k: x[:g] for k, x in d.items()} for d in t["aggs"]])
+
+    def argsort_multi(self, keys, *, key_types=None, descending: bool = False, reverse_desc: bool = False,
+                      perm64: bool = False, stream=None) -> torch.Tensor:
+        keys = list(keys)
+        n = keys[0].shape[0]
+        kts = [key_type_of(k) if key_types is None else key_types[c] for c, k in enumerate(keys)]
+        flags = (N.SORT_DESC if descending else 0) | (N.SORT_DESC_REVERSE if reverse_desc else 0) \
+            | (N.SORT_PERM64 if perm64 else 0)
+        perm = torch.empty(n, dtype=torch.int64 if perm64 else torch.int32, device=keys[0].device)
+        kc = (N.VdfKeyCol * len(keys))()
+        for c, k in enumerate(keys):
+            kc[c].type = kts[c]
+            kc[c].data = _ptr(k)
+        N.check(self.lib.vdf_argsort_multi(self.h, kc, len(keys), n, flags, _ptr(perm),
+                                           C.c_void_p(_stream_ptr(stream))))
+        return perm
+
+    def broadcast(self, group_vals: torch.Tensor, result: "GroupbyResult", n: int, to_f64: bool = False,
+                  stream=None) -> torch.Tensor:
+        """Per-row column: row r gets the value of its group (the group-by's
+        offsets / n_groups / perm from an untrimmed result)."""
+        st = val_type_of(group_vals)
+        dt = N.VAL_F64 if to_f64 else st
+        out = torch.empty(n, dtype=torch.float64 if to_f64 else group_vals.dtype, device=group_vals.device)
+        ng = result.n_groups
+        if not isinstance(ng, torch.Tensor):
+            raise ValueError("broadcast needs an untrimmed group-by result (device group count)")
+        N.check(self.lib.vdf_broadcast(self.h, st, dt, group_vals.data_ptr(), result.offsets.data_ptr(),
+                                       ng.data_ptr(), result.offsets.shape[0] - 1, result.perm.data_ptr(), n,
+                                       _ptr(out), C.c_void_p(_stream_ptr(stream))))
+        return out
 
     def prepare_groupby_batch(self, tables, aggs: int = N.AGG_ALL, want_perm: bool = True, key_type=None,
                               val_types=None, capacities=None) -> "PreparedBatch":
