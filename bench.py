@@ -52,13 +52,15 @@ def _threads(args) -> int:
     return args.cpu_threads or min(16, os.cpu_count() or 1)
 
 
-def _load_traffic(name: str):
+def _load_traffic(workload: str, name: str):
+    """HBM bytes per launch of a kernel in a workload, from the committed
+    rocprofv3 counter summaries (profiles/traffic.json), or None."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         d = json.load(f)
-    return d.get(name)
+    return d.get(workload, {}).get(name)
 
 
 def _kernel_table(v) -> dict:
@@ -72,7 +74,7 @@ def _kernel_table(v) -> dict:
     return kernels
 
 
-def _roofline(kernels: dict, name: str | None = None) -> dict:
+def _roofline(workload: str, kernels: dict, name: str | None = None) -> dict:
     """HBM roofline of one kernel (default: the one with the most time that
     moves algorithmic bytes)."""
     if name is None:
@@ -80,7 +82,7 @@ def _roofline(kernels: dict, name: str | None = None) -> dict:
         name = max(cands)[1]
     k = kernels[name]
     return {"bound": "hbm", "kernel": name, "achieved": k["gbps"], "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-            "frac": k["gbps"] / HBM_PEAK_GBPS, "traffic": _load_traffic(name)}
+            "frac": k["gbps"] / HBM_PEAK_GBPS, "traffic": _load_traffic(workload, name)}
 
 
 def _time_steps(step, steps: int, warmup: int, v=None):
@@ -155,12 +157,6 @@ def run_argsort_u32(args) -> dict:
     perm = torch.empty(n, dtype=torch.int32, device="cuda")
     out = torch.empty(n, dtype=torch.float64, device="cuda")
     v = Vdf(0)
-    if args.calibrate:
-        a = torch.empty(1 << 27, dtype=torch.float64, device="cuda")
-        b = torch.empty_like(a)
-        b.copy_(a)
-        torch.cuda.synchronize()
-        del a, b
 
     def step():
         v.argsort(d_keys, key_type=N.KEY_U32, perm_out=perm)
@@ -182,7 +178,7 @@ def run_argsort_u32(args) -> dict:
                         "parallelism": "single GPU"})
     rec["hbm_gbps_step"] = step_bytes / (ms_step * 1e-3) / 1e9
     rec["hbm_pct_peak_step"] = 100.0 * rec["hbm_gbps_step"] / HBM_PEAK_GBPS
-    rec["roofline"] = _roofline(kernels, "radix_pass")
+    rec["roofline"] = _roofline("argsort_u32", kernels, "radix_pass")
     rec["kernels"] = kernels
     if not args.no_cpu:
         rec["cpu_baseline"] = cpu_baseline_argsort(keys, val, _threads(args))
@@ -241,7 +237,7 @@ def run_zipf_groupby(args) -> dict:
                        {"workload": "zipf_groupby", "rows": n, "key": "int32 Zipf(1.1) over 2^24 values",
                         "values": ["float64 N(0,1)", "int64 uniform [0,2^40)"],
                         "aggregates": "count,sum,mean,min,max", "groups": g, "parallelism": "single GPU"})
-    rec["roofline"] = _roofline(kernels)
+    rec["roofline"] = _roofline("zipf_groupby", kernels)
     rec["kernels"] = kernels
     if not args.no_cpu:
         m = min(n, 20_000_000)
@@ -297,6 +293,13 @@ def run_batched(args) -> dict:
     cnt = sum(int(t["counts"][:g].to(torch.int64).sum().item()) for t, g in zip(pb.keep[:64], ng[:64]))
     assert cnt == int(sizes[:64].sum()), "group counts do not add up to the rows"
 
+    if args.skip_compare:  # profiling runs: the batched launches only
+        rec = _base_record(args, rows_per_s, "rows/s", ms_step, "i32+f64",
+                           {"workload": "batched_groupby", "tables": ntab, "rows": total})
+        rec["kernels"] = kernels
+        v.close()
+        return rec
+
     # the same batch as a captured HIP-graph plan
     pb.capture()
     ms_graph, _ = _time_steps(pb.replay, args.steps, 1)
@@ -329,7 +332,7 @@ def run_batched(args) -> dict:
     rec["graph_replay"] = {"ms_per_step": ms_graph / args.steps,
                            "rows_per_s": total * args.steps / (ms_graph * 1e-3)}
     rec["autotune"] = tune
-    rec["roofline"] = _roofline(kernels)
+    rec["roofline"] = _roofline("batched", kernels)
     rec["kernels"] = kernels
     if not args.no_cpu:
         sample = [(keys[offs[i]:offs[i + 1]], [vals[offs[i]:offs[i + 1]]]) for i in range(min(ntab, 96))]
@@ -494,6 +497,8 @@ def main():
     ap.add_argument("--arena-gb", type=int, default=48, help="served: server arena (admission budget), GiB")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
     ap.add_argument("--no-tune", action="store_true", help="batched: skip the launch-shape autotuner")
+    ap.add_argument("--skip-compare", action="store_true",
+                    help="batched: time the batched launches only (no per-table calls, no graph replay)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--calibrate", action="store_true",
                     help="first run one plain 1 GiB device copy (the known-size kernel that scales "
@@ -505,6 +510,14 @@ def main():
         sys.exit(2)
 
     if args.workload:
+        if args.calibrate:
+            import torch
+
+            a = torch.empty(1 << 27, dtype=torch.float64, device="cuda")
+            b = torch.empty_like(a)
+            b.copy_(a)
+            torch.cuda.synchronize()
+            del a, b
         print(json.dumps(RUNNERS[args.workload](args)), flush=True)
         return
 

@@ -135,3 +135,28 @@ def test_broadcast_per_row(vdf, src):
     if to64:
         exp = exp.astype(np.float64)
     assert np.array_equal(out.view(np.uint8), exp.view(np.uint8)), src
+
+
+@pytest.mark.parametrize("kind", ["i32", "u64", "char16", "f64"])
+def test_cached_grouping_equals_groupby(vdf, kind):
+    """Aggregating against a kept grouping equals a fresh group-by, for
+    several value columns in turn (no re-sort between them)."""
+    rng = np.random.default_rng(17)
+    n = 400_003
+    keys = _key_cols((kind,), n, rng)[0]
+    kt = oracle.key_type_of(keys)
+    dk = _dev(keys)
+    g = vdf.grouping(dk, key_type=kt)
+    for c, vals in enumerate([[rng.standard_normal(n)], [rng.integers(-2**40, 2**40, n).astype(np.int64)],
+                              [rng.integers(0, 1000, n).astype(np.int32), rng.standard_normal(n)], []]):
+        dv = [_dev(v) for v in vals]
+        got = g.aggregate(dv)
+        exp = vdf.groupby(dk, dv, key_type=kt)
+        assert got.n_groups == exp.n_groups, (kind, c)
+        for x, y in ((got.keys, exp.keys), (got.offsets, exp.offsets), (got.counts, exp.counts),
+                     (got.perm, exp.perm)):
+            assert np.array_equal(_bytes(x), _bytes(y)), (kind, c)
+        for da, db in zip(got.aggs, exp.aggs):
+            for k in da:
+                assert np.array_equal(_bytes(da[k]), _bytes(db[k])), (kind, c, k)
+    g.close()

@@ -162,6 +162,13 @@ def test_profile_counts_active_passes(vdf):
     vdf.argsort(d_k24, key_type=N.KEY_U32)  # top digit constant: 3 passes
     _, launches, _ = vdf.profile_read("radix_pass")
     assert launches == 3
+    # a batch: each pass launch is charged the tables whose plan has that pass
+    small = (keys[:300_000] & np.uint32(0xFFFF))  # 2 passes
+    vdf.profile_reset()
+    vdf.argsort_batch([_dev(small), d_keys], key_type=N.KEY_U32)
+    _, launches, nbytes = vdf.profile_read("radix_pass")
+    assert launches == 4
+    assert nbytes == 300_000 * (12 + 12) + n * (12 + 16 + 16 + 12)
     vdf.profile(False)
     vdf.profile_reset()
 

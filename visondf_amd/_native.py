@@ -47,6 +47,7 @@ EXPORTS = (
     "vdf_autotune_argsort", "vdf_autotune_groupby_batch",
     "vdf_plan_groupby_batch", "vdf_plan_launch", "vdf_plan_scratch_bytes", "vdf_plan_destroy",
     "vdf_groupby_multi", "vdf_argsort_multi", "vdf_broadcast",
+    "vdf_grouping_create", "vdf_grouping_aggregate", "vdf_grouping_destroy",
 )
 
 
@@ -188,6 +189,12 @@ def load() -> C.CDLL:
     lib.vdf_argsort_multi.argtypes = [vp, C.POINTER(VdfKeyCol), i32, u64, u32, vp, vp]
     lib.vdf_broadcast.restype = i32
     lib.vdf_broadcast.argtypes = [vp, i32, i32, vp, vp, vp, u64, vp, u64, vp, vp]
+    lib.vdf_grouping_create.restype = i32
+    lib.vdf_grouping_create.argtypes = [vp, i32, vp, u64, vp, C.POINTER(vp)]
+    lib.vdf_grouping_aggregate.restype = i32
+    lib.vdf_grouping_aggregate.argtypes = [vp, C.POINTER(VdfValueCol), i32, u32, C.POINTER(VdfGroupbyOut), vp]
+    lib.vdf_grouping_destroy.restype = i32
+    lib.vdf_grouping_destroy.argtypes = [vp]
     lib.vdf_tune_set.restype = i32
     lib.vdf_tune_set.argtypes = [vp, C.c_char_p, C.c_int64]
     lib.vdf_tune_get.restype = i32

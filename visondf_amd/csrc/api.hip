@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cstring>
 #include <exception>
+#include <memory>
 
 namespace vdf {
 
@@ -137,11 +138,19 @@ void Handle::prof_collect()
         VDF_HIP_CHECK(hipEventElapsedTime(&ms, r.beg, r.end));
         uint64_t bytes = r.bytes;
         bool active = true;
-        if (r.plan_slot >= 0 && r.passes_host) {
-            const uint32_t D = *r.passes_host;
+        if (r.plan_slot >= 0 && r.acct) {
+            // a pass slot did work for the tables whose plan has more passes
+            const PassAcct& pa = *r.acct;
             const uint32_t s = static_cast<uint32_t>(r.plan_slot);
-            active = s < D;
-            bytes = D == 1 ? r.pass_bytes[0] : s == 0 ? r.pass_bytes[1] : s == D - 1 ? r.pass_bytes[3] : r.pass_bytes[2];
+            bytes = 0;
+            active = false;
+            for (size_t i = 0; i < pa.rows.size(); ++i) {
+                const uint32_t D = pa.passes[i];
+                if (s >= D) continue;
+                active = true;
+                const int pos = D == 1 ? 0 : s == 0 ? 1 : s == D - 1 ? 3 : 2;
+                bytes += pa.rows[i] * pa.per_row[pos];
+            }
         }
         if (active) {
             auto it = std::find_if(totals.begin(), totals.end(), [&](const ProfTotal& t) { return t.name == r.name; });
@@ -483,6 +492,102 @@ vdf_status vdf_broadcast(vdf_handle* h, int src_type, int dst_type, const void* 
         CallScope cs(h);
         broadcast_enqueue(h, src_type, dst_type, d_group_vals, d_offsets, d_n_groups, capacity, d_perm, n, d_out,
                           static_cast<hipStream_t>(stream));
+    });
+}
+
+}  // extern "C"
+
+// A cached grouping: the sorted keys and the group-contiguous permutation of
+// one key column, kept on the device for further aggregations.
+struct vdf_grouping {
+    vdf_handle* h = nullptr;
+    int key_type = 0;
+    uint64_t n = 0;
+    void* mem = nullptr;
+    void* skeys = nullptr;
+    uint32_t* perm = nullptr;
+};
+
+extern "C" {
+
+vdf_status vdf_grouping_create(vdf_handle* h, int key_type, const void* d_keys, uint64_t n, void* stream,
+                               vdf_grouping** out)
+{
+    return guarded([&] {
+        require(h != nullptr && out != nullptr, "vdf_grouping_create: null argument");
+        *out = nullptr;
+        require(key_bytes(key_type) != 0, "vdf_grouping_create: unknown key type");
+        require(n <= VDF_MAX_ROWS, "vdf_grouping_create: n exceeds VDF_MAX_ROWS (2^30 - 1)");
+        require(n == 0 || d_keys != nullptr, "vdf_grouping_create: null keys");
+        CallScope cs(h);
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        const size_t kb = key_bytes(key_type);
+        const size_t kbytes = align_up(std::max<uint64_t>(n, 1) * kb, 256);
+        auto g = std::make_unique<vdf_grouping>();
+        g->h = h;
+        g->key_type = key_type;
+        g->n = n;
+        if (hipMalloc(&g->mem, kbytes + align_up(std::max<uint64_t>(n, 1) * 4, 256)) != hipSuccess) {
+            (void)hipGetLastError();
+            throw Error{VDF_ERR_OOM, "hipMalloc of a grouping failed"};
+        }
+        g->skeys = g->mem;
+        g->perm = reinterpret_cast<uint32_t*>(static_cast<char*>(g->mem) + kbytes);
+        try {
+            if (n) {
+                const uint64_t hr = static_cast<uint64_t>(h->tune.hist_rows);
+                void* scr = h->ensure_scratch(argsort_batch_scratch(key_type, &n, 1, hr));
+                h->staging_begin(argsort_batch_staging(key_type, &n, 1, hr));
+                StagingScope st(h, s);
+                argsort_enqueue(h, key_type, d_keys, n, 0, g->perm, g->skeys, scr, s);
+            }
+        } catch (...) {
+            (void)hipFree(g->mem);
+            throw;
+        }
+        *out = g.release();
+    });
+}
+
+vdf_status vdf_grouping_aggregate(vdf_grouping* g, const vdf_value_col* vals, int nvals, uint32_t agg_mask,
+                                  vdf_groupby_out* out, void* stream)
+{
+    return guarded([&] {
+        require(g != nullptr && out != nullptr, "vdf_grouping_aggregate: null grouping or output");
+        require(nvals >= 0 && nvals <= VDF_MAX_VALUE_COLS, "vdf_grouping_aggregate: nvals out of range");
+        require(nvals == 0 || vals != nullptr, "vdf_grouping_aggregate: null value columns");
+        require((agg_mask & ~VDF_AGG_ALL) == 0, "vdf_grouping_aggregate: unknown aggregate bits");
+        require(out->n_groups != nullptr && out->offsets != nullptr,
+                "vdf_grouping_aggregate: n_groups and offsets are required");
+        const uint64_t n = g->n;
+        std::vector<int> vts(static_cast<size_t>(std::max(nvals, 1)), 0);
+        GbTable t{};
+        t.n = n;
+        for (int c = 0; c < nvals; ++c) {
+            require(val_bytes(vals[c].type) != 0, "vdf_grouping_aggregate: unknown value type");
+            require(n == 0 || vals[c].data != nullptr, "vdf_grouping_aggregate: null value column data");
+            vts[c] = vals[c].type;
+            t.vals[c] = vals[c].data;
+        }
+        vdf_handle* h = g->h;
+        CallScope cs(h);
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        void* scr = h->ensure_scratch(groupby_agg_scratch(&n, 1));
+        if (out->perm && n)
+            VDF_HIP_CHECK(hipMemcpyAsync(out->perm, g->perm, n * 4, hipMemcpyDeviceToDevice, s));
+        h->staging_begin(groupby_agg_staging(&n, 1));
+        StagingScope st(h, s);
+        const uint32_t* perms[1] = {g->perm};
+        groupby_aggregate_enqueue(h, g->key_type, g->skeys, perms, &t, 1, vts.data(), nvals, agg_mask, out, scr, s);
+    });
+}
+
+vdf_status vdf_grouping_destroy(vdf_grouping* g)
+{
+    return guarded([&] {
+        if (!g) return;
+        if (g->mem) (void)hipFree(g->mem);
+        delete g;
     });
 }
 

@@ -598,8 +598,21 @@ int kind_of(int type)
 
 struct GbLayout {
     uint64_t rows = 0, tiles = 0;
-    size_t sort_bytes = 0, bytes = 0;
+    size_t sort_bytes = 0, agg_bytes = 0, bytes = 0;
 };
+
+// scratch of the aggregate stage alone (descriptors, tile map, look-back,
+// per-tile partials)
+size_t agg_bytes_for(uint64_t tiles, int ntables)
+{
+    size_t b = 0;
+    b += align_up(static_cast<size_t>(ntables) * sizeof(GbSeg), 256);
+    b += align_up(std::max<uint64_t>(tiles, 1) * 4, 256);      // tile -> table
+    b += align_up((tiles + 1) * 4, 256);                       // status + ticket
+    b += 2 * align_up(std::max<uint64_t>(tiles, 1) * 4, 256);  // tile_gbase, tile_heads
+    b += 2 * align_up(VDF_MAX_VALUE_COLS * std::max<uint64_t>(tiles, 1) * sizeof(Agg), 256);
+    return b + 256;
+}
 
 GbLayout gb_layout(int key_type, const uint64_t* n, int ntables, uint64_t hist_rows)
 {
@@ -609,14 +622,11 @@ GbLayout gb_layout(int key_type, const uint64_t* n, int ntables, uint64_t hist_r
         L.tiles += (n[i] + kGbTile - 1) / kGbTile;
     }
     L.sort_bytes = align_up(argsort_batch_scratch(key_type, n, ntables, hist_rows), 256);
+    L.agg_bytes = agg_bytes_for(L.tiles, ntables);
     size_t b = L.sort_bytes;
-    b += align_up(L.rows * key_bytes(key_type), 256);            // sorted keys
-    b += align_up(L.rows * 4, 256);                              // perms the caller does not take
-    b += align_up(static_cast<size_t>(ntables) * sizeof(GbSeg), 256);
-    b += align_up(std::max<uint64_t>(L.tiles, 1) * 4, 256);      // tile -> table
-    b += align_up((L.tiles + 1) * 4, 256);                       // status + ticket
-    b += 2 * align_up(std::max<uint64_t>(L.tiles, 1) * 4, 256);  // tile_gbase, tile_heads
-    b += 2 * align_up(VDF_MAX_VALUE_COLS * std::max<uint64_t>(L.tiles, 1) * sizeof(Agg), 256);
+    b += align_up(L.rows * key_bytes(key_type), 256);  // sorted keys
+    b += align_up(L.rows * 4, 256);                    // perms the caller does not take
+    b += L.agg_bytes;
     L.bytes = b + 256;
     return L;
 }
@@ -630,53 +640,54 @@ size_t groupby_batch_scratch(int key_type, const uint64_t* n, int ntables, uint6
     return gb_layout(key_type, n, ntables, hist_rows).bytes;
 }
 
-size_t groupby_batch_staging(int key_type, const uint64_t* n, int ntables, uint64_t hist_rows)
+size_t groupby_agg_staging(const uint64_t* n, int ntables)
 {
-    const GbLayout L = gb_layout(key_type, n, ntables, hist_rows);
-    return argsort_batch_staging(key_type, n, ntables, hist_rows) + 512 +
-           align_up(static_cast<size_t>(ntables) * sizeof(GbSeg), 256) + std::max<uint64_t>(L.tiles, 1) * 4;
+    uint64_t tiles = 0;
+    for (int i = 0; i < ntables; ++i) tiles += (n[i] + kGbTile - 1) / kGbTile;
+    return 512 + align_up(static_cast<size_t>(ntables) * sizeof(GbSeg), 256) + std::max<uint64_t>(tiles, 1) * 4;
 }
 
-void groupby_batch_enqueue(Handle* h, int key_type, const GbTable* tables, int ntables, const int* val_types,
-                           int nvals, uint32_t agg_mask, const vdf_groupby_out* outs, void* scr, hipStream_t s)
+size_t groupby_agg_scratch(const uint64_t* n, int ntables)
 {
-    std::vector<uint64_t> ns(ntables);
-    for (int i = 0; i < ntables; ++i) ns[i] = tables[i].n;
-    const uint64_t hist_rows = static_cast<uint64_t>(h->tune.hist_rows);
-    const GbLayout L = gb_layout(key_type, ns.data(), ntables, hist_rows);
+    uint64_t tiles = 0;
+    for (int i = 0; i < ntables; ++i) tiles += (n[i] + kGbTile - 1) / kGbTile;
+    return agg_bytes_for(tiles, ntables);
+}
+
+size_t groupby_batch_staging(int key_type, const uint64_t* n, int ntables, uint64_t hist_rows)
+{
+    return argsort_batch_staging(key_type, n, ntables, hist_rows) + groupby_agg_staging(n, ntables);
+}
+
+// The aggregate stage over tables already sorted: sorted keys of every table
+// at `skeys` (table starts 16-row aligned), each table's permutation in
+// perms[i]; scratch at `scr` (groupby_agg_scratch bytes).
+void groupby_aggregate_enqueue(Handle* h, int key_type, const void* skeys, const uint32_t* const* perms,
+                               const GbTable* tables, int ntables, const int* val_types, int nvals, uint32_t agg_mask,
+                               const vdf_groupby_out* outs, void* scr, hipStream_t s)
+{
     const size_t kb = key_bytes(key_type);
-    Carve cv(scr);
-    void* sort_scr = cv.take<char>(L.sort_bytes);
-    char* skeys = cv.take<char>(L.rows * kb);
-    uint32_t* perm_scr = cv.take<uint32_t>(L.rows);
-    GbSeg* dsegs = cv.take<GbSeg>(ntables);
-    uint32_t* dtile = cv.take<uint32_t>(std::max<uint64_t>(L.tiles, 1));
-    uint32_t* status = cv.take<uint32_t>(L.tiles + 1);
-    GbArgs a{};
-    a.tile_gbase = cv.take<uint32_t>(std::max<uint64_t>(L.tiles, 1));
-    a.tile_heads = cv.take<uint32_t>(std::max<uint64_t>(L.tiles, 1));
-    a.prefix = cv.take<Agg>(VDF_MAX_VALUE_COLS * std::max<uint64_t>(L.tiles, 1));
-    a.suffix = cv.take<Agg>(VDF_MAX_VALUE_COLS * std::max<uint64_t>(L.tiles, 1));
-
-    // 1. sort every table: sorted keys into scratch, perms to the caller or scratch
-    std::vector<SortSeg> ss(ntables);
-    std::vector<const uint32_t*> perms(ntables);
-    uint64_t row = 0;
+    uint64_t tiles_total = 0, rows_total = 0;
     for (int i = 0; i < ntables; ++i) {
-        uint32_t* p = outs[i].perm ? outs[i].perm : perm_scr + row;
-        perms[i] = p;
-        ss[i] = SortSeg{tables[i].keys, tables[i].n, p, skeys + row * kb};
-        row += align_up(tables[i].n, kGbRows);
+        tiles_total += (tables[i].n + kGbTile - 1) / kGbTile;
+        rows_total += align_up(tables[i].n, kGbRows);
     }
-    argsort_batch_enqueue(h, key_type, ss.data(), ntables, 0, sort_scr, s);
+    Carve cv(scr);
+    GbSeg* dsegs = cv.take<GbSeg>(ntables);
+    uint32_t* dtile = cv.take<uint32_t>(std::max<uint64_t>(tiles_total, 1));
+    uint32_t* status = cv.take<uint32_t>(tiles_total + 1);
+    GbArgs a{};
+    a.tile_gbase = cv.take<uint32_t>(std::max<uint64_t>(tiles_total, 1));
+    a.tile_heads = cv.take<uint32_t>(std::max<uint64_t>(tiles_total, 1));
+    a.prefix = cv.take<Agg>(VDF_MAX_VALUE_COLS * std::max<uint64_t>(tiles_total, 1));
+    a.suffix = cv.take<Agg>(VDF_MAX_VALUE_COLS * std::max<uint64_t>(tiles_total, 1));
 
-    // 2. table descriptions and the tile map (pinned staging -> device)
+    // table descriptions and the tile map (pinned staging -> device)
     const size_t seg_bytes = align_up(static_cast<size_t>(ntables) * sizeof(GbSeg), 256);
-    char* stg = static_cast<char*>(h->staging_take(seg_bytes + std::max<uint64_t>(L.tiles, 1) * 4));
+    char* stg = static_cast<char*>(h->staging_take(seg_bytes + std::max<uint64_t>(tiles_total, 1) * 4));
     GbSeg* hs = reinterpret_cast<GbSeg*>(stg);
     uint32_t* ht = reinterpret_cast<uint32_t*>(stg + seg_bytes);
-    row = 0;
-    uint64_t tiles = 0;
+    uint64_t row = 0, tiles = 0;
     for (int i = 0; i < ntables; ++i) {
         const uint64_t n = tables[i].n;
         const uint64_t nt = (n + kGbTile - 1) / kGbTile;
@@ -732,7 +743,7 @@ void groupby_batch_enqueue(Handle* h, int key_type, const GbTable* tables, int n
     }
     {
         // algorithmic bytes: sorted keys (+ the previous key) + per column row id and value
-        LaunchTimer t(h, s, "groupby_tile", L.rows * (kb + val_bytes));
+        LaunchTimer t(h, s, "groupby_tile", rows_total * (kb + val_bytes));
         switch (kb) {
         case 4: hipLaunchKernelGGL(k_gb_tile<uint32_t>, dim3(a.ntiles), dim3(kBlock), 0, s, a); break;
         case 8: hipLaunchKernelGGL(k_gb_tile<uint64_t>, dim3(a.ntiles), dim3(kBlock), 0, s, a); break;
@@ -745,6 +756,37 @@ void groupby_batch_enqueue(Handle* h, int key_type, const GbTable* tables, int n
         hipLaunchKernelGGL(k_gb_fixup, dim3(a.ntiles), dim3(kBlock), 0, s, a);
         check_launch("k_gb_fixup");
     }
+}
+
+void groupby_batch_enqueue(Handle* h, int key_type, const GbTable* tables, int ntables, const int* val_types,
+                           int nvals, uint32_t agg_mask, const vdf_groupby_out* outs, void* scr, hipStream_t s)
+{
+    std::vector<uint64_t> ns(ntables);
+    for (int i = 0; i < ntables; ++i) ns[i] = tables[i].n;
+    const uint64_t hist_rows = static_cast<uint64_t>(h->tune.hist_rows);
+    const GbLayout L = gb_layout(key_type, ns.data(), ntables, hist_rows);
+    const size_t kb = key_bytes(key_type);
+    Carve cv(scr);
+    void* sort_scr = cv.take<char>(L.sort_bytes);
+    char* skeys = cv.take<char>(L.rows * kb);
+    uint32_t* perm_scr = cv.take<uint32_t>(L.rows);
+    void* agg_scr = cv.take<char>(L.agg_bytes);
+
+    // 1. sort every table: sorted keys into scratch, perms to the caller or scratch
+    std::vector<SortSeg> ss(ntables);
+    std::vector<const uint32_t*> perms(ntables);
+    uint64_t row = 0;
+    for (int i = 0; i < ntables; ++i) {
+        uint32_t* p = outs[i].perm ? outs[i].perm : perm_scr + row;
+        perms[i] = p;
+        ss[i] = SortSeg{tables[i].keys, tables[i].n, p, skeys + row * kb};
+        row += align_up(tables[i].n, kGbRows);
+    }
+    argsort_batch_enqueue(h, key_type, ss.data(), ntables, 0, sort_scr, s);
+
+    // 2. boundaries and aggregates
+    groupby_aggregate_enqueue(h, key_type, skeys, perms.data(), tables, ntables, val_types, nvals, agg_mask, outs,
+                              agg_scr, s);
 }
 
 void groupby_enqueue(Handle* h, int key_type, const void* d_keys, uint64_t n, const vdf_value_col* vals,

@@ -860,26 +860,32 @@ static void launch_sort(Handle* h, const SortSeg* segs, int nsegs, uint64_t mlo,
         hipLaunchKernelGGL(k_plan<D>, dim3(static_cast<unsigned>(nsegs)), dim3(kBlock), 0, s, a);
         check_launch("k_plan");
     }
-    // Active passes per segment are decided on the device; for timing, one
-    // segment's pass count is copied back so inactive launches can be told
-    // apart (bytes of batched launches are counted as if every segment ran
-    // the slot).
-    uint32_t* passes_host = nullptr;
-    if (h->prof && nsegs == 1) {
-        passes_host = h->passes_ring + (h->ring_pos++ % Handle::kRing);
-        VDF_HIP_CHECK(hipMemcpyAsync(passes_host, a.plan_hdr, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    }
-    // algorithmic bytes of a slot when it is the only / first / middle / last pass:
-    // keys read + keys written + row ids read + row ids written
-    const uint64_t n = L.rows;
+    // Active passes per table are decided on the device; for timing, the
+    // tables' pass counts are copied back (into the handle's pinned ring) so
+    // each pass launch is charged the bytes of the tables it worked on.
+    std::shared_ptr<PassAcct> acct;
     const uint64_t pb_out = a.perm64 ? 8 : 4;
     const uint64_t kout = segs[0].keys_out ? kb : 0;
-    const uint64_t pass_bytes[4] = {n * (kb + kout + pb_out), n * (kb + kb + 4), n * (2 * kb + 8),
-                                    n * (kb + 4 + kout + pb_out)};
+    if (h->prof && h->passes_ring && static_cast<size_t>(nsegs) <= Handle::kRing) {
+        if (h->ring_pos + nsegs > Handle::kRing) h->ring_pos = 0;
+        uint32_t* dst = h->passes_ring + h->ring_pos;
+        h->ring_pos += nsegs;
+        VDF_HIP_CHECK(hipMemcpy2DAsync(dst, sizeof(uint32_t), a.plan_hdr, kPlanHdr * sizeof(uint32_t),
+                                       sizeof(uint32_t), static_cast<size_t>(nsegs), hipMemcpyDeviceToHost, s));
+        acct = std::make_shared<PassAcct>();
+        acct->passes = dst;
+        acct->rows = ns;
+        // keys read + keys written + row ids read + row ids written, per row, when
+        // the pass is the table's only / first / middle / last one
+        acct->per_row[0] = kb + kout + pb_out;
+        acct->per_row[1] = kb + kb + 4;
+        acct->per_row[2] = 2 * kb + 8;
+        acct->per_row[3] = kb + 4 + kout + pb_out;
+    }
+    const uint64_t middle_bytes = L.rows * (2 * kb + 8);
     bool coop = xcd;
     for (int slot = 0; slot < D; ++slot) {
-        LaunchTimer t(h, s, "radix_pass", passes_host ? 0 : pass_bytes[2], passes_host ? slot : -1, passes_host,
-                      pass_bytes);
+        LaunchTimer t(h, s, "radix_pass", middle_bytes, slot, acct);
         uint32_t sl = static_cast<uint32_t>(slot);
         void* args[] = {&a, &sl};
         if (coop) {

@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <exception>
+#include <memory>
 #include <mutex>
 #include <new>
 #include <string>
@@ -94,13 +95,21 @@ struct SortSeg {
 // ---------------------------------------------------------------------------
 // handle: scratch arena + timing
 // ---------------------------------------------------------------------------
+// Byte accounting of the radix pass launches of one sort: each table's pass
+// count (pinned copy of the device plan) and rows, and the bytes per row of a
+// pass that is the table's only / first / middle / last one.
+struct PassAcct {
+    const uint32_t* passes = nullptr;  // nsegs words, valid once the launch's stream reached the copy
+    std::vector<uint64_t> rows;
+    uint64_t per_row[4] = {0, 0, 0, 0};
+};
+
 struct ProfRec {
     std::string name;
     hipEvent_t beg, end;
-    uint64_t bytes;          // algorithmic bytes of the launch
-    int plan_slot;           // >= 0: radix pass slot, active iff slot < *passes_host
-    uint32_t* passes_host;   // pinned copy of Plan::passes for that sort (or null)
-    uint64_t pass_bytes[4];  // slot bytes when it is the only / first / middle / last pass
+    uint64_t bytes;                        // algorithmic bytes of the launch (no acct)
+    int plan_slot;                         // >= 0: radix pass slot
+    std::shared_ptr<const PassAcct> acct;  // pass slots: per-table activity and bytes
 };
 
 struct ProfTotal {
@@ -138,9 +147,9 @@ struct Handle {
     std::vector<ProfRec> recs;
     std::vector<hipEvent_t> free_events;
     std::vector<ProfTotal> totals;
-    uint32_t* passes_ring = nullptr;   // pinned host ring for Plan::passes copies
+    uint32_t* passes_ring = nullptr;   // pinned host ring for per-table pass counts
     size_t ring_pos = 0;
-    static constexpr size_t kRing = 4096;
+    static constexpr size_t kRing = size_t(1) << 20;
 
     // pinned host staging for batch descriptions: a ring of buffers, one per
     // call; a buffer is reused once the upload of its previous call finished
@@ -175,12 +184,11 @@ struct LaunchTimer {
     const char* name;
     uint64_t bytes;
     int slot;
-    uint32_t* passes_host;
-    const uint64_t* pass_bytes;
+    std::shared_ptr<const PassAcct> acct;
     hipEvent_t b = nullptr;
     LaunchTimer(Handle* h_, hipStream_t s_, const char* n_, uint64_t bytes_, int slot_ = -1,
-                uint32_t* ph = nullptr, const uint64_t* pb = nullptr)
-        : h(h_), s(s_), name(n_), bytes(bytes_), slot(slot_), passes_host(ph), pass_bytes(pb)
+                std::shared_ptr<const PassAcct> acct_ = nullptr)
+        : h(h_), s(s_), name(n_), bytes(bytes_), slot(slot_), acct(std::move(acct_))
     {
         if (h->prof) {
             b = h->get_event();
@@ -192,10 +200,7 @@ struct LaunchTimer {
         if (h->prof) {
             hipEvent_t e = h->get_event();
             VDF_HIP_CHECK(hipEventRecord(e, s));
-            ProfRec r{name, b, e, bytes, slot, passes_host, {0, 0, 0, 0}};
-            if (pass_bytes)
-                for (int i = 0; i < 4; ++i) r.pass_bytes[i] = pass_bytes[i];
-            h->recs.push_back(r);
+            h->recs.push_back(ProfRec{name, b, e, bytes, slot, acct});
         }
     }
 };
@@ -261,6 +266,13 @@ void groupby_enqueue(Handle* h, int key_type, const void* d_keys, uint64_t n,
                      const vdf_groupby_out* out, void* scr, hipStream_t s);
 void groupby_batch_enqueue(Handle* h, int key_type, const GbTable* tables, int ntables, const int* val_types,
                            int nvals, uint32_t agg_mask, const vdf_groupby_out* outs, void* scr, hipStream_t s);
+// the aggregate stage alone, over already sorted tables (keys at skeys with
+// 16-row aligned table starts, permutations perms[i])
+size_t groupby_agg_scratch(const uint64_t* n, int ntables);
+size_t groupby_agg_staging(const uint64_t* n, int ntables);
+void groupby_aggregate_enqueue(Handle* h, int key_type, const void* skeys, const uint32_t* const* perms,
+                               const GbTable* tables, int ntables, const int* val_types, int nvals, uint32_t agg_mask,
+                               const vdf_groupby_out* outs, void* scr, hipStream_t s);
 
 // multi-column keys: composite key type (-1 when wider than 16 bytes), pack
 // the columns into composites, unpack composite group keys into the columns

@@ -293,6 +293,10 @@ class Vdf:
                                        _ptr(out), C.c_void_p(_stream_ptr(stream))))
         return out
 
+    def grouping(self, keys: torch.Tensor, *, key_type: int | None = None, stream=None) -> "Grouping":
+        """Sort a key column once; aggregate value columns against it later."""
+        return Grouping(self, keys, key_type_of(keys) if key_type is None else key_type, stream)
+
     def prepare_groupby_batch(self, tables, aggs: int = N.AGG_ALL, want_perm: bool = True, key_type=None,
                               val_types=None, capacities=None) -> "PreparedBatch":
         """Descriptors and outputs of one batch, built once for repeated runs."""
@@ -412,3 +416,40 @@ class PreparedBatch:
 
     def result(self, i: int, trim: bool = True) -> GroupbyResult:
         return Vdf._result(self.keep[i], self.sizes[i], trim)
+
+
+class Grouping:
+    """A cached grouping of one key column (vdf_grouping_*)."""
+
+    def __init__(self, vdf: Vdf, keys: torch.Tensor, key_type: int, stream=None):
+        self.v = vdf
+        self.keys = keys
+        self.n = keys.shape[0]
+        g = C.c_void_p()
+        N.check(vdf.lib.vdf_grouping_create(vdf.h, key_type, _ptr(keys), self.n, C.c_void_p(_stream_ptr(stream)),
+                                            C.byref(g)))
+        self.g = g
+
+    def aggregate(self, vals=(), aggs: int = N.AGG_ALL, want_perm: bool = True, stream=None, trim: bool = True,
+                  val_types=None) -> GroupbyResult:
+        vals = list(vals)
+        vts = [val_type_of(x) if val_types is None else val_types[c] for c, x in enumerate(vals)]
+        out, t = Vdf._alloc_out(self.keys, vts, [x.dtype for x in vals], aggs, want_perm)
+        cols = (N.VdfValueCol * max(len(vals), 1))()
+        for c, x in enumerate(vals):
+            cols[c].type = vts[c]
+            cols[c].data = x.data_ptr()
+        N.check(self.v.lib.vdf_grouping_aggregate(self.g, cols, len(vals), aggs, C.byref(out),
+                                                  C.c_void_p(_stream_ptr(stream))))
+        return Vdf._result(t, self.n, trim)
+
+    def close(self):
+        if self.g:
+            N.check(self.v.lib.vdf_grouping_destroy(self.g))
+            self.g = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
