@@ -86,14 +86,13 @@ def _roofline(workload: str, kernels: dict, name: str | None = None) -> dict:
 
 
 def _time_steps(step, steps: int, warmup: int, v=None):
+    """Times `steps` steps between torch.cuda.synchronize() calls (no per-launch
+    events in the timed region); with a handle, then runs the same number of
+    steps again with per-launch HIP events for the kernel table."""
     import torch
 
     for _ in range(warmup):
         step()
-    torch.cuda.synchronize()
-    if v is not None:
-        v.profile(True)
-        v.profile_reset()
     torch.cuda.synchronize()
     t0 = torch.cuda.Event(enable_timing=True)
     t1 = torch.cuda.Event(enable_timing=True)
@@ -104,6 +103,11 @@ def _time_steps(step, steps: int, warmup: int, v=None):
     torch.cuda.synchronize()
     kernels = None
     if v is not None:
+        v.profile(True)
+        v.profile_reset()
+        for _ in range(steps):
+            step()
+        torch.cuda.synchronize()
         kernels = _kernel_table(v)
         v.profile(False)
     return t0.elapsed_time(t1), kernels
@@ -157,6 +161,11 @@ def run_argsort_u32(args) -> dict:
     perm = torch.empty(n, dtype=torch.int32, device="cuda")
     out = torch.empty(n, dtype=torch.float64, device="cuda")
     v = Vdf(0)
+    tune = None
+    if not args.no_tune:
+        _log("argsort_u32: autotuning the launch shape")
+        tune = v.autotune_argsort(d_keys, perm, key_type=N.KEY_U32, reps=3)
+        _log(f"argsort_u32: autotune {tune}")
 
     def step():
         v.argsort(d_keys, key_type=N.KEY_U32, perm_out=perm)
@@ -180,6 +189,7 @@ def run_argsort_u32(args) -> dict:
     rec["hbm_pct_peak_step"] = 100.0 * rec["hbm_gbps_step"] / HBM_PEAK_GBPS
     rec["roofline"] = _roofline("argsort_u32", kernels, "radix_pass")
     rec["kernels"] = kernels
+    rec["autotune"] = tune
     if not args.no_cpu:
         rec["cpu_baseline"] = cpu_baseline_argsort(keys, val, _threads(args))
         rec["speedup_vs_cpu"] = rows_per_s / rec["cpu_baseline"]["value"]
@@ -496,7 +506,7 @@ def main():
     ap.add_argument("--workers", type=int, default=3, help="served: launch sequences in flight")
     ap.add_argument("--arena-gb", type=int, default=48, help="served: server arena (admission budget), GiB")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
-    ap.add_argument("--no-tune", action="store_true", help="batched: skip the launch-shape autotuner")
+    ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner (argsort_u32, batched)")
     ap.add_argument("--skip-compare", action="store_true",
                     help="batched: time the batched launches only (no per-table calls, no graph replay)")
     ap.add_argument("--cpu-threads", type=int, default=0)

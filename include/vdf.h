@@ -183,6 +183,23 @@ vdf_status vdf_groupby_multi(vdf_handle* h, const vdf_key_col* keys, int nkeys, 
 vdf_status vdf_argsort_multi(vdf_handle* h, const vdf_key_col* keys, int nkeys, uint64_t n, uint32_t flags,
                              void* d_perm_out, void* stream);
 
+/* ---- cached grouping ----
+ * Sort one key column once and aggregate value columns against it any number
+ * of times (boundaries, counts and aggregates are recomputed from the kept
+ * sorted keys and permutation; no further sort).  Replaces the
+ * already-grouped path of the reference (transform_group_by_alrd_mt.hpp:10,
+ * functions_alrd.inl:13-40), which keeps its grouping to aggregate more
+ * columns.  The grouping holds n * (key bytes + 4) bytes of device memory and
+ * uses its handle (which must outlive it) for scratch. */
+typedef struct vdf_grouping vdf_grouping;
+vdf_status vdf_grouping_create(vdf_handle* h, int key_type, const void* d_keys, uint64_t n, void* stream,
+                               vdf_grouping** out);
+/* As vdf_groupby on the grouping's key column (out->perm, when set, receives
+ * a copy of the kept permutation). */
+vdf_status vdf_grouping_aggregate(vdf_grouping* g, const vdf_value_col* vals, int nvals, uint32_t agg_mask,
+                                  vdf_groupby_out* out, void* stream);
+vdf_status vdf_grouping_destroy(vdf_grouping* g);
+
 /* ---- per-row broadcast of group results ----
  * d_out[perm[p]] = value of the group that sorted position p belongs to, for
  * the outputs of one group-by (values d_group_vals of src_type, offsets,

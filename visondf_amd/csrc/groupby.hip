@@ -46,6 +46,25 @@ enum : int { KIND_F64 = 0, KIND_SIGNED = 1, KIND_UNSIGNED = 2, KIND_COUNT = 3 };
 struct K16 {
     uint64_t lo, hi;
 };
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// key loads / stores through global-address-space pointers
+__device__ inline uint32_t get_key(const uint32_t* p, uint64_t i) { return gmem(p)[i]; }
+__device__ inline uint64_t get_key(const uint64_t* p, uint64_t i) { return gmem(p)[i]; }
+__device__ inline K16 get_key(const K16* p, uint64_t i)
+{
+    const auto* w = gmem(reinterpret_cast<const uint64_t*>(p)) + 2 * i;
+    return K16{w[0], w[1]};
+}
+__device__ inline void put_key(uint32_t* p, uint64_t i, uint32_t k) { gmem(p)[i] = k; }
+__device__ inline void put_key(uint64_t* p, uint64_t i, uint64_t k) { gmem(p)[i] = k; }
+__device__ inline void put_key(K16* p, uint64_t i, K16 k)
+{
+    auto* w = gmem(reinterpret_cast<uint64_t*>(p)) + 2 * i;
+    w[0] = k.lo;
+    w[1] = k.hi;
+}
+
 __device__ inline bool keq(uint32_t a, uint32_t b) { return a == b; }
 __device__ inline bool keq(uint64_t a, uint64_t b) { return a == b; }
 __device__ inline bool keq(const K16& a, const K16& b) { return a.lo == b.lo && a.hi == b.hi; }
@@ -194,45 +213,71 @@ struct GbArgs {
     Agg* suffix;               // [ncols][ntiles] rows from the tile's last head to its end
 };
 
-__device__ inline uint64_t load_val(int type, const void* data, uint64_t row)
+// Output pointers of one value column, read from the table descriptor once
+// (in registers, so the per-group stores cannot force reloads of the
+// descriptor, which the compiler could not otherwise tell apart from the
+// outputs).
+struct ColOut {
+    uint32_t* counts;  // column 0 only, when counts are asked for
+    void* sum;
+    double* mean;
+    void* min;
+    void* max;
+    uint64_t cap;
+};
+
+// a wave-uniform 64-bit value, held in scalar registers
+template <typename T>
+__device__ inline T uniform64(T x)
 {
-    switch (type) {
-    case VDF_VAL_I32: return static_cast<uint64_t>(static_cast<int64_t>(static_cast<const int32_t*>(data)[row]));
-    case VDF_VAL_U32: return static_cast<uint64_t>(static_cast<const uint32_t*>(data)[row]);
-    default: return static_cast<const uint64_t*>(data)[row];
-    }
+    uint64_t u;
+    __builtin_memcpy(&u, &x, 8);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(u));
+    const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(u >> 32));
+    u = (static_cast<uint64_t>(hi) << 32) | lo;
+    __builtin_memcpy(&x, &u, 8);
+    return x;
 }
 
-__device__ inline void write_final(const GbArgs& a, const GbSeg& sg, int c, uint32_t gid, const Agg& s)
+// (sg is the same table for the whole wave)
+__device__ inline ColOut col_out(const GbArgs& a, const GbSeg& sg, int c)
 {
-    if (gid >= sg.cap) return;
+    ColOut o;
+    const vdf_agg_out& ag = sg.out.aggs[c];
+    o.counts = uniform64((c == 0 && (a.agg_mask & VDF_AGG_COUNT)) ? sg.out.counts : nullptr);
+    o.sum = uniform64((a.agg_mask & VDF_AGG_SUM) ? ag.sum : nullptr);
+    o.mean = uniform64((a.agg_mask & VDF_AGG_MEAN) ? ag.mean : nullptr);
+    o.min = uniform64((a.agg_mask & VDF_AGG_MIN) ? ag.min : nullptr);
+    o.max = uniform64((a.agg_mask & VDF_AGG_MAX) ? ag.max : nullptr);
+    o.cap = uniform64(sg.cap);
+    return o;
+}
+
+__device__ inline void write_final(const GbArgs& a, const ColOut& o, int c, uint32_t gid, const Agg& s)
+{
+    if (gid >= o.cap) return;
     const GbColType col = a.cols[c];
-    if (c == 0 && (a.agg_mask & VDF_AGG_COUNT) && sg.out.counts) sg.out.counts[gid] = static_cast<uint32_t>(s.cnt);
+    if (o.counts) gmem(o.counts)[gid] = static_cast<uint32_t>(s.cnt);
     if (col.kind == KIND_COUNT) return;
-    const vdf_agg_out& o = sg.out.aggs[c];
-    void* osum = (a.agg_mask & VDF_AGG_SUM) ? o.sum : nullptr;
-    double* omean = (a.agg_mask & VDF_AGG_MEAN) ? o.mean : nullptr;
-    void* omin = (a.agg_mask & VDF_AGG_MIN) ? o.min : nullptr;
-    void* omax = (a.agg_mask & VDF_AGG_MAX) ? o.max : nullptr;
     const bool w4 = (col.type == VDF_VAL_I32 || col.type == VDF_VAL_U32);
     if (col.kind == KIND_F64) {
         const double sum = as_d(s.a) + as_d(s.b);
-        if (osum) static_cast<double*>(osum)[gid] = sum;
-        if (omean) omean[gid] = sum / static_cast<double>(s.cnt);
+        if (o.sum) gmem(static_cast<double*>(o.sum))[gid] = sum;
+        if (o.mean) gmem(o.mean)[gid] = sum / static_cast<double>(s.cnt);
     } else {
-        if (osum) static_cast<uint64_t*>(osum)[gid] = s.a;
-        if (omean) {
+        if (o.sum) gmem(static_cast<uint64_t*>(o.sum))[gid] = s.a;
+        if (o.mean) {
             const double tot = (col.kind == KIND_SIGNED) ? i128_to_double(s.a, s.b) : u128_to_double(s.a, s.b);
-            omean[gid] = tot / static_cast<double>(s.cnt);
+            gmem(o.mean)[gid] = tot / static_cast<double>(s.cnt);
         }
     }
-    if (omin) {
-        if (w4) static_cast<uint32_t*>(omin)[gid] = static_cast<uint32_t>(s.mn);
-        else static_cast<uint64_t*>(omin)[gid] = s.mn;
+    if (o.min) {
+        if (w4) gmem(static_cast<uint32_t*>(o.min))[gid] = static_cast<uint32_t>(s.mn);
+        else gmem(static_cast<uint64_t*>(o.min))[gid] = s.mn;
     }
-    if (omax) {
-        if (w4) static_cast<uint32_t*>(omax)[gid] = static_cast<uint32_t>(s.mx);
-        else static_cast<uint64_t*>(omax)[gid] = s.mx;
+    if (o.max) {
+        if (w4) gmem(static_cast<uint32_t*>(o.max))[gid] = static_cast<uint32_t>(s.mx);
+        else gmem(static_cast<uint64_t*>(o.max))[gid] = s.mx;
     }
 }
 
@@ -281,10 +326,10 @@ __device__ inline void seg_combine(Agg& x, uint32_t& f, const Agg& y, uint32_t f
 __device__ inline void load_rows_u32(const uint32_t* p, uint32_t cnt, uint32_t (&o)[kGbRows])
 {
     if (cnt == kGbRows && (reinterpret_cast<uintptr_t>(p) & 15u) == 0) {
-        const uint4* q = reinterpret_cast<const uint4*>(p);
+        const auto* q = gmem(reinterpret_cast<const u32x4*>(p));
 #pragma unroll
         for (int i = 0; i < kGbRows / 4; ++i) {
-            const uint4 t = q[i];
+            const u32x4 t = q[i];
             o[4 * i + 0] = t.x;
             o[4 * i + 1] = t.y;
             o[4 * i + 2] = t.z;
@@ -292,7 +337,7 @@ __device__ inline void load_rows_u32(const uint32_t* p, uint32_t cnt, uint32_t (
         }
     } else {
 #pragma unroll
-        for (int j = 0; j < kGbRows; ++j) o[j] = (static_cast<uint32_t>(j) < cnt) ? p[j] : 0u;
+        for (int j = 0; j < kGbRows; ++j) o[j] = (static_cast<uint32_t>(j) < cnt) ? gmem(p)[j] : 0u;
     }
 }
 
@@ -303,14 +348,14 @@ template <typename K>
 __device__ inline uint32_t head_bits(const K* keys, uint64_t r0, uint32_t cnt)
 {
     if (cnt == 0) return 0u;
-    K prev = keys[r0 == 0 ? 0 : r0 - 1];
+    K prev = get_key(keys, r0 == 0 ? 0 : r0 - 1);
     uint32_t hb = 0;
     if (cnt == kGbRows) {
         constexpr int V = static_cast<int>(4 * sizeof(K) / 16);  // uint4 per 4 rows
-        const uint4* q = reinterpret_cast<const uint4*>(keys + r0);
+        const auto* q = gmem(reinterpret_cast<const u32x4*>(keys + r0));
 #pragma unroll
         for (int c = 0; c < kGbRows / 4; ++c) {
-            uint4 t[V];
+            u32x4 t[V];
 #pragma unroll
             for (int i = 0; i < V; ++i) t[i] = q[c * V + i];
             K k[4];
@@ -324,7 +369,7 @@ __device__ inline uint32_t head_bits(const K* keys, uint64_t r0, uint32_t cnt)
         }
     } else {
         for (uint32_t j = 0; j < cnt; ++j) {
-            const K k = keys[r0 + j];
+            const K k = get_key(keys, r0 + j);
             if (!keq(k, prev) || (r0 + j == 0)) hb |= 1u << j;
             prev = k;
         }
@@ -340,14 +385,34 @@ __device__ __attribute__((always_inline)) inline void gb_column(const GbArgs& a,
 {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint64_t r0 = tbase + static_cast<uint64_t>(tid) * kGbRows;
+    const ColOut co = col_out(a, sg, c);
     uint64_t v[kGbRows];
     if (KIND != KIND_COUNT) {
         uint32_t p[kGbRows];
         load_rows_u32(sg.perm + r0, cnt, p);
+        // All 16 gathers are issued before any conversion waits on one.
+        // Rows past the tile's end hold row id 0 (in bounds: the table is
+        // not empty) and are never aggregated.
         const void* vd = sg.vals[c];
         const int type = a.cols[c].type;
+        if (type == VDF_VAL_I32 || type == VDF_VAL_U32) {
+            const auto* q = gmem(static_cast<const uint32_t*>(vd));
+            uint32_t w[kGbRows];
 #pragma unroll
-        for (int j = 0; j < kGbRows; ++j) v[j] = (static_cast<uint32_t>(j) < cnt) ? load_val(type, vd, p[j]) : 0ull;
+            for (int j = 0; j < kGbRows; ++j) w[j] = q[p[j]];
+            if (type == VDF_VAL_I32) {
+#pragma unroll
+                for (int j = 0; j < kGbRows; ++j)
+                    v[j] = static_cast<uint64_t>(static_cast<int64_t>(static_cast<int32_t>(w[j])));
+            } else {
+#pragma unroll
+                for (int j = 0; j < kGbRows; ++j) v[j] = w[j];
+            }
+        } else {
+            const auto* q = gmem(static_cast<const uint64_t*>(vd));
+#pragma unroll
+            for (int j = 0; j < kGbRows; ++j) v[j] = q[p[j]];
+        }
     }
     Agg cur = agg_empty(), first = agg_empty();
     bool seen = false;
@@ -360,7 +425,7 @@ __device__ __attribute__((always_inline)) inline void gb_column(const GbArgs& a,
                     first = cur;
                     seen = true;
                 } else {
-                    write_final(a, sg, c, g - 1, cur);
+                    write_final(a, co, c, g - 1, cur);
                 }
                 cur = agg_empty();
                 ++g;
@@ -411,7 +476,7 @@ __device__ __attribute__((always_inline)) inline void gb_column(const GbArgs& a,
     Agg* suf = a.suffix + static_cast<uint64_t>(c) * a.ntiles;
     if (seen) {
         const Agg seg = agg_join(carry, first, KIND);
-        if (cf) write_final(a, sg, c, gfirst - 1, seg);
+        if (cf) write_final(a, co, c, gfirst - 1, seg);
         else pre[tile] = seg;  // the tile's first head: rows before it
     }
     if (tid == kBlock - 1) {
@@ -491,8 +556,8 @@ __global__ __launch_bounds__(kBlock) void k_gb_tile(GbArgs a)
         a.tile_gbase[tile] = excl;
         a.tile_heads[tile] = H;
         if (lt == sg.ntiles - 1) {
-            *sg.out.n_groups = static_cast<uint64_t>(excl) + H;
-            if (excl + H <= sg.cap) sg.out.offsets[excl + H] = static_cast<uint32_t>(n);
+            *gmem(sg.out.n_groups) = static_cast<uint64_t>(excl) + H;
+            if (excl + H <= sg.cap) gmem(sg.out.offsets)[excl + H] = static_cast<uint32_t>(n);
         }
     }
     __syncthreads();
@@ -507,8 +572,8 @@ __global__ __launch_bounds__(kBlock) void k_gb_tile(GbArgs a)
             bits &= bits - 1;
             const uint64_t r = r0 + k;
             if (g < sg.cap) {
-                sg.out.offsets[g] = static_cast<uint32_t>(r);
-                if (sg.out.group_keys) static_cast<K*>(sg.out.group_keys)[g] = keys[r];
+                gmem(sg.out.offsets)[g] = static_cast<uint32_t>(r);
+                if (sg.out.group_keys) put_key(static_cast<K*>(sg.out.group_keys), g, get_key(keys, r));
             }
             ++g;
         }
@@ -550,7 +615,7 @@ __global__ __launch_bounds__(kBlock) void k_gb_fixup(GbArgs a)
         if (short_chain) {
             if (tid == 0) {
                 if (b + 1 < tend) acc = agg_join(acc, pre[b + 1], kind);
-                write_final(a, sg, c, gid, acc);
+                write_final(a, col_out(a, sg, c), c, gid, acc);
             }
             continue;
         }
@@ -572,7 +637,7 @@ __global__ __launch_bounds__(kBlock) void k_gb_fixup(GbArgs a)
             __syncthreads();
             if (first < static_cast<uint32_t>(kBlock)) break;
         }
-        if (tid == 0) write_final(a, sg, c, gid, acc);
+        if (tid == 0) write_final(a, col_out(a, sg, c), c, gid, acc);
     }
 }
 
@@ -581,8 +646,8 @@ __global__ void k_gb_empty(const GbSeg* segs, uint32_t nsegs)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < nsegs && segs[i].n == 0) {
-        *segs[i].out.n_groups = 0;
-        segs[i].out.offsets[0] = 0;
+        *gmem(segs[i].out.n_groups) = 0;
+        gmem(segs[i].out.offsets)[0] = 0;
     }
 }
 

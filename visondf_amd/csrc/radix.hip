@@ -44,42 +44,56 @@ struct KeyOps;
 template <>
 struct KeyOps<uint32_t> {
     static constexpr int kDigits = 4;
+    // fetch() reads the caller's key; xform() maps it (kept apart so that a
+    // tile's loads are all in flight before any transform waits on one)
+    __device__ static uint32_t fetch(const void* p, uint64_t i) { return gmem(static_cast<const uint32_t*>(p))[i]; }
+    __device__ static uint32_t xform(uint32_t x, uint64_t mlo, uint64_t mhi)
+    {
+        const uint32_t fm = (x >> 31) ? 0xFFFFFFFFu : 0x80000000u;
+        return x ^ (mhi ? fm : 0u) ^ static_cast<uint32_t>(mlo);
+    }
     __device__ static uint32_t load(const void* p, uint64_t i, uint64_t mlo, uint64_t mhi)
     {
-        uint32_t x = static_cast<const uint32_t*>(p)[i];
-        if (mhi) x ^= (x >> 31) ? 0xFFFFFFFFu : 0x80000000u;
-        return x ^ static_cast<uint32_t>(mlo);
+        return xform(fetch(p, i), mlo, mhi);
     }
     __device__ static void store(void* p, uint64_t i, uint32_t k, uint64_t mlo, uint64_t mhi)
     {
         uint32_t x = k ^ static_cast<uint32_t>(mlo);
         if (mhi) x = (x >> 31) ? (x ^ 0x80000000u) : ~x;
-        static_cast<uint32_t*>(p)[i] = x;
+        gmem(static_cast<uint32_t*>(p))[i] = x;
     }
     __device__ static uint32_t digit(uint32_t k, int d) { return (k >> (8 * d)) & 0xFFu; }
     __device__ static uint32_t zero() { return 0u; }
+    __device__ static uint32_t raw(const uint32_t* p, uint64_t i) { return gmem(p)[i]; }
+    __device__ static void put(uint32_t* p, uint64_t i, uint32_t k) { gmem(p)[i] = k; }
 };
 
 template <>
 struct KeyOps<uint64_t> {
     static constexpr int kDigits = 8;
+    __device__ static uint64_t fetch(const void* p, uint64_t i) { return gmem(static_cast<const uint64_t*>(p))[i]; }
+    __device__ static uint64_t xform(uint64_t x, uint64_t mlo, uint64_t mhi)
+    {
+        const uint64_t fm = (x >> 63) ? ~0ull : 0x8000000000000000ull;
+        return x ^ (mhi ? fm : 0ull) ^ mlo;
+    }
     __device__ static uint64_t load(const void* p, uint64_t i, uint64_t mlo, uint64_t mhi)
     {
-        uint64_t x = static_cast<const uint64_t*>(p)[i];
-        if (mhi) x ^= (x >> 63) ? ~0ull : 0x8000000000000000ull;
-        return x ^ mlo;
+        return xform(fetch(p, i), mlo, mhi);
     }
     __device__ static void store(void* p, uint64_t i, uint64_t k, uint64_t mlo, uint64_t mhi)
     {
         uint64_t x = k ^ mlo;
         if (mhi) x = (x >> 63) ? (x ^ 0x8000000000000000ull) : ~x;
-        static_cast<uint64_t*>(p)[i] = x;
+        gmem(static_cast<uint64_t*>(p))[i] = x;
     }
     __device__ static uint32_t digit(uint64_t k, int d)
     {
         return static_cast<uint32_t>(k >> (8 * d)) & 0xFFu;
     }
     __device__ static uint64_t zero() { return 0ull; }
+    __device__ static uint64_t raw(const uint64_t* p, uint64_t i) { return gmem(p)[i]; }
+    __device__ static void put(uint64_t* p, uint64_t i, uint64_t k) { gmem(p)[i] = k; }
 };
 
 // char[16]: byte 0 is the most significant digit (radix_sort_charbuf_flat.inl:94
@@ -88,17 +102,25 @@ struct KeyOps<uint64_t> {
 template <>
 struct KeyOps<K128> {
     static constexpr int kDigits = 16;
+    __device__ static K128 fetch(const void* p, uint64_t i)
+    {
+        const auto* w = gmem(static_cast<const uint64_t*>(p)) + 2 * i;
+        return K128{w[1], w[0]};  // bytes 8..15, bytes 0..7 (unswapped)
+    }
+    __device__ static K128 xform(K128 r, uint64_t mlo, uint64_t mhi)
+    {
+        K128 k;
+        k.hi = __builtin_bswap64(r.hi) ^ mhi;
+        k.lo = __builtin_bswap64(r.lo) ^ mlo;
+        return k;
+    }
     __device__ static K128 load(const void* p, uint64_t i, uint64_t mlo, uint64_t mhi)
     {
-        const uint64_t* w = static_cast<const uint64_t*>(p) + 2 * i;
-        K128 k;
-        k.hi = __builtin_bswap64(w[0]) ^ mhi;
-        k.lo = __builtin_bswap64(w[1]) ^ mlo;
-        return k;
+        return xform(fetch(p, i), mlo, mhi);
     }
     __device__ static void store(void* p, uint64_t i, K128 k, uint64_t mlo, uint64_t mhi)
     {
-        uint64_t* w = static_cast<uint64_t*>(p) + 2 * i;
+        auto* w = gmem(static_cast<uint64_t*>(p)) + 2 * i;
         w[0] = __builtin_bswap64(k.hi ^ mhi);
         w[1] = __builtin_bswap64(k.lo ^ mlo);
     }
@@ -108,15 +130,26 @@ struct KeyOps<K128> {
                      : static_cast<uint32_t>(k.hi >> (8 * (d - 8))) & 0xFFu;
     }
     __device__ static K128 zero() { return K128{0ull, 0ull}; }
+    __device__ static K128 raw(const K128* p, uint64_t i)
+    {
+        const auto* w = gmem(reinterpret_cast<const uint64_t*>(p)) + 2 * i;
+        return K128{w[0], w[1]};
+    }
+    __device__ static void put(K128* p, uint64_t i, K128 k)
+    {
+        auto* w = gmem(reinterpret_cast<uint64_t*>(p)) + 2 * i;
+        w[0] = k.lo;
+        w[1] = k.hi;
+    }
 };
 
 __device__ inline void store_relaxed(uint32_t* p, uint32_t v)
 {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(gmem(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ inline uint32_t load_relaxed(uint32_t* p)
 {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_load(gmem(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Exclusive scan of one value per thread over a 256-thread block.
@@ -173,6 +206,10 @@ struct HistItem {
 };
 
 constexpr int kPlanHdr = 1 + kMaxDigits;  // passes + digit_of[]
+// Tile groups number at most 8 (one per XCD), so the last ticket word of slot
+// 0 is free: k_plan keeps the largest pass count of the batch there.
+constexpr uint32_t kMaxPassWord = kTicketStride - 1;
+static_assert(kTicketStride > 8, "ticket words of slot 0 hold 8 tile groups and the pass count");
 
 struct SortArgs {
     const SegDesc* segs;
@@ -249,12 +286,13 @@ __global__ __launch_bounds__(kBlock) void k_histogram(SortArgs a)
     }
     // waves step through the item in strides of the block; full rounds first
     uint64_t i0 = wave0;
-    for (; i0 + 63 + 3 * kBlock < it.end; i0 += 4 * kBlock) {
-        K k[4];
+    constexpr int U = 8;  // loads in flight per lane
+    for (; i0 + 63 + (U - 1) * kBlock < it.end; i0 += U * kBlock) {
+        K k[U];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) k[u] = KO::load(keys, i0 + lane + u * kBlock, a.mlo, a.mhi);
+        for (int u = 0; u < U; ++u) k[u] = KO::fetch(keys, i0 + lane + u * kBlock);
 #pragma unroll
-        for (int u = 0; u < 4; ++u) hist_add<K>(sh, k[u], true, shared);
+        for (int u = 0; u < U; ++u) hist_add<K>(sh, KO::xform(k[u], a.mlo, a.mhi), true, shared);
     }
     for (; i0 < it.end; i0 += kBlock) {
         const uint64_t i = i0 + lane;
@@ -302,6 +340,7 @@ __global__ __launch_bounds__(kBlock) void k_plan(SortArgs a)
             P = 1;
         }
         hdr[0] = P;
+        atomicMax(a.tickets + kMaxPassWord, P);
     }
 }
 
@@ -372,8 +411,18 @@ struct TileCtx {
 // counter and clears the mask; every peer reads the counter back.  LDS
 // operations of one wave complete in program order, so each read sees exactly
 // the items before j, and the result is the stable rank.
+// Blocks of a pass kernel that fit one CU's 160 KiB of LDS; the kernel is
+// compiled to keep that many waves per SIMD (one wave of each block per SIMD),
+// so registers never cut occupancy below what LDS allows.
 template <typename K, int ITEMS>
-__global__ __launch_bounds__(kBlock) void k_pass(SortArgs a, uint32_t slot)
+struct PassShape {
+    static constexpr int kLds = kBlock * ITEMS * static_cast<int>(sizeof(K) + 4) + kWaves * kRadix * 12 + kRadix * 4 + 64;
+    static constexpr int kBlocksPerCu = (160 * 1024 / kLds) < 3 ? (160 * 1024 / kLds) : 3;
+};
+
+template <typename K, int ITEMS>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PassShape<K, ITEMS>::kBlocksPerCu)))
+void k_pass(SortArgs a, uint32_t slot)
 {
     using KO = KeyOps<K>;
     constexpr int TILE = kBlock * ITEMS;
@@ -446,22 +495,41 @@ __global__ __launch_bounds__(kBlock) void k_pass(SortArgs a, uint32_t slot)
         }
         lbn[static_cast<uint64_t>(c.t) * kRadix + tid] = 0u;  // the next pass's row of this tile
         const uint64_t wl = c.lrow0 + static_cast<uint64_t>(wave) * 64 * ITEMS + lane;  // local row of item 0
+        // Every load is unconditional, so all are in flight at once: a full
+        // tile reads at constant offsets; in a segment's last, partial tile
+        // rows past the end read the segment's last row and are masked below.
+        const bool full = c.lrow0 + TILE <= c.n;
+        const uint64_t last = c.n - 1;
         if (c.first) {
+            if (full) {
+#pragma unroll
+                for (int j = 0; j < ITEMS; ++j) key[j] = KO::fetch(c.keys, wl + static_cast<uint64_t>(j) * 64);
+            } else {
+#pragma unroll
+                for (int j = 0; j < ITEMS; ++j)
+                    key[j] = KO::fetch(c.keys, std::min(wl + static_cast<uint64_t>(j) * 64, last));
+            }
 #pragma unroll
             for (int j = 0; j < ITEMS; ++j) {
-                const uint64_t i = wl + static_cast<uint64_t>(j) * 64;
-                key[j] = (i < c.n) ? KO::load(c.keys, i, a.mlo, a.mhi) : KO::zero();
-                val[j] = static_cast<uint32_t>(i);
+                key[j] = KO::xform(key[j], a.mlo, a.mhi);
+                val[j] = static_cast<uint32_t>(wl + static_cast<uint64_t>(j) * 64);
             }
         } else {
             const K* kp = kin + c.row_off;
-            const uint32_t* vp = vin + c.row_off;
+            const auto* vp = gmem(vin + c.row_off);
+            if (full) {
 #pragma unroll
-            for (int j = 0; j < ITEMS; ++j) {
-                const uint64_t i = wl + static_cast<uint64_t>(j) * 64;
-                const bool v = i < c.n;
-                key[j] = v ? kp[i] : KO::zero();
-                val[j] = v ? vp[i] : 0u;
+                for (int j = 0; j < ITEMS; ++j) {
+                    key[j] = KO::raw(kp, wl + static_cast<uint64_t>(j) * 64);
+                    val[j] = vp[wl + static_cast<uint64_t>(j) * 64];
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < ITEMS; ++j) {
+                    const uint64_t i = std::min(wl + static_cast<uint64_t>(j) * 64, last);
+                    key[j] = KO::raw(kp, i);
+                    val[j] = vp[i];
+                }
             }
         }
         lds_barrier();
@@ -556,11 +624,11 @@ __global__ __launch_bounds__(kBlock) void k_pass(SortArgs a, uint32_t slot)
         const int dig = static_cast<int>(c.dig);
         if (!c.last) {
             K* kout = static_cast<K*>(a.kbuf[slot & 1]) + c.row_off;
-            uint32_t* vout = a.vbuf[slot & 1] + c.row_off;
+            auto* vout = gmem(a.vbuf[slot & 1] + c.row_off);
             for (uint32_t i = tid; i < tn; i += kBlock) {
                 const K kk = s_keys[i];
                 const uint32_t p = s_gbase[KO::digit(kk, dig)] + i;
-                kout[p] = kk;
+                KO::put(kout, p, kk);
                 vout[p] = s_vals[i];
             }
         } else {
@@ -571,14 +639,16 @@ __global__ __launch_bounds__(kBlock) void k_pass(SortArgs a, uint32_t slot)
                 if (a.reverse) p = nm1 - p;
                 if (c.keys_out) KO::store(c.keys_out, p, kk, a.mlo, a.mhi);
                 if (a.perm64)
-                    static_cast<uint64_t*>(c.perm_out)[p] = s_vals[i];
+                    gmem(static_cast<uint64_t*>(c.perm_out))[p] = s_vals[i];
                 else
-                    static_cast<uint32_t*>(c.perm_out)[p] = s_vals[i];
+                    gmem(static_cast<uint32_t*>(c.perm_out))[p] = s_vals[i];
             }
         }
         lds_barrier();
     };
 
+    // a slot beyond every segment's pass count has no tile to move
+    if (slot >= load_relaxed(a.tickets + kMaxPassWord)) return;
     TileCtx c0, c1;
     if (!next_tile(c0)) return;
     uint32_t tot0, ls0;

@@ -213,6 +213,22 @@ inline void check_launch(const char* what)
 
 inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
+// Device pointers read from descriptors in memory are generic: accesses
+// through them compile to flat instructions, which count against lgkmcnt as
+// well as vmcnt, so every LDS wait (barriers, shuffles) would also wait for
+// outstanding global traffic.  These casts put them in the global address
+// space (all of them point into device memory).
+template <typename T>
+__device__ inline __attribute__((address_space(1))) T* gmem(T* p)
+{
+    return (__attribute__((address_space(1))) T*)p;
+}
+template <typename T>
+__device__ inline const __attribute__((address_space(1))) T* gmem(const T* p)
+{
+    return (const __attribute__((address_space(1))) T*)p;
+}
+
 // Bump allocator over the handle's scratch arena.
 struct Carve {
     char* base;
