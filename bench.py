@@ -164,31 +164,49 @@ def run_argsort_u32(args) -> dict:
     tune = None
     if not args.no_tune:
         _log("argsort_u32: autotuning the launch shape")
-        tune = v.autotune_argsort(d_keys, perm, key_type=N.KEY_U32, reps=3)
+        tune = v.autotune_sort_by(d_keys, d_val, perm, out, key_type=N.KEY_U32, reps=3)
         _log(f"argsort_u32: autotune {tune}")
 
+    # One step: the stable argsort of the keys with the float64 column carried
+    # through the radix passes (vdf_sort_by: permutation + column in key order).
     def step():
-        v.argsort(d_keys, key_type=N.KEY_U32, perm_out=perm)
-        v.gather(d_val, perm, out=out)
+        v.sort_by(d_keys, d_val, key_type=N.KEY_U32, perm_out=perm, vals_out=out)
 
     ms_total, kernels = _time_steps(step, args.steps, args.warmup, v)
     ms_step = ms_total / args.steps
     rows_per_s = n * args.steps / (ms_total * 1e-3)
     step_bytes = sum(k["bytes_per_launch"] * k["launches"] for k in kernels.values()) / args.steps
-    # spot check of the last step: the permutation sorts the keys and the gather follows it
+    # spot check of the last step: the permutation sorts the keys (stably) and
+    # the column follows it
     ks = d_keys.to(torch.int64) & 0xFFFFFFFF
-    sk = ks[perm.to(torch.int64)]
+    p64 = perm.to(torch.int64)
+    sk = ks[p64]
     assert bool((sk[1:] >= sk[:-1]).all()), "permutation does not sort the keys"
-    assert torch.equal(out, d_val[perm.to(torch.int64)]), "gather does not follow the permutation"
+    assert bool((p64[1:][sk[1:] == sk[:-1]] > p64[:-1][sk[1:] == sk[:-1]]).all()), "sort is not stable"
+    assert torch.equal(out, d_val[p64]), "value column does not follow the permutation"
+
+    # the same result through two calls (argsort, then a random-access gather)
+    perm2 = torch.empty_like(perm)
+    out2 = torch.empty_like(out)
+
+    def step_two_calls():
+        v.argsort(d_keys, key_type=N.KEY_U32, perm_out=perm2)
+        v.gather(d_val, perm2, out=out2)
+
+    ms2, kernels2 = _time_steps(step_two_calls, args.steps, args.warmup, v)
+    assert torch.equal(perm2, perm) and torch.equal(out2, out), "argsort + gather differs from sort_by"
 
     rec = _base_record(args, rows_per_s, "rows/s", ms_step, "u32",
                        {"workload": "argsort_u32_gather_f64", "rows": n, "key": "uint32 uniform over 2^32",
                         "value_column": "float64", "digit_bits": 8, "radix_passes": 4,
+                        "method": "vdf_sort_by (column carried through the passes)",
                         "parallelism": "single GPU"})
     rec["hbm_gbps_step"] = step_bytes / (ms_step * 1e-3) / 1e9
     rec["hbm_pct_peak_step"] = 100.0 * rec["hbm_gbps_step"] / HBM_PEAK_GBPS
     rec["roofline"] = _roofline("argsort_u32", kernels, "radix_pass")
     rec["kernels"] = kernels
+    rec["two_calls"] = {"method": "vdf_argsort + vdf_gather", "ms_per_step": ms2 / args.steps,
+                        "rows_per_s": n * args.steps / (ms2 * 1e-3), "kernels": kernels2}
     rec["autotune"] = tune
     if not args.no_cpu:
         rec["cpu_baseline"] = cpu_baseline_argsort(keys, val, _threads(args))

@@ -121,6 +121,18 @@ vdf_status vdf_gather(vdf_handle* h, uint32_t elem_bytes, const void* d_src,
                       const void* d_perm, int perm64, uint64_t n, void* d_dst,
                       void* stream);
 
+/* ---- sort with one value column carried along ----
+ * The hard sort_by of the reference sorts the row ids and then permutes every
+ * column through them (sort_by_mt.hpp:117-158, permute_block_mt.inl:39-41).
+ * Here one value column of val_bytes (4 or 8) bytes per row moves with the
+ * keys through the radix passes, so d_vals_out[r] = d_vals[perm[r]] is
+ * produced by the sort itself, with no random-access gather.  Results are
+ * identical to vdf_argsort followed by vdf_gather.  d_perm_out and
+ * d_sorted_keys_out are nullable; flags as vdf_argsort. */
+vdf_status vdf_sort_by(vdf_handle* h, int key_type, const void* d_keys, uint64_t n, uint32_t flags,
+                       void* d_perm_out, void* d_sorted_keys_out, uint32_t val_bytes, const void* d_vals,
+                       void* d_vals_out, void* stream);
+
 /* ---- sort-based group-by ---- */
 typedef struct vdf_value_col {
     int type;          /* VDF_VAL_* */
@@ -282,6 +294,10 @@ typedef struct vdf_tune_report {
 
 vdf_status vdf_autotune_argsort(vdf_handle* h, int key_type, const void* d_keys, uint64_t n, uint32_t flags,
                                 void* d_perm_out, int reps, void* stream, vdf_tune_report* report);
+/* As vdf_autotune_argsort for vdf_sort_by (a value column carried along). */
+vdf_status vdf_autotune_sort_by(vdf_handle* h, int key_type, const void* d_keys, uint64_t n, uint32_t flags,
+                                void* d_perm_out, uint32_t val_bytes, const void* d_vals, void* d_vals_out, int reps,
+                                void* stream, vdf_tune_report* report);
 vdf_status vdf_autotune_groupby_batch(vdf_handle* h, int key_type, const vdf_table* tables, int ntables,
                                       const int* val_types, int nvals, uint32_t agg_mask,
                                       vdf_groupby_out* outs, int reps, void* stream,

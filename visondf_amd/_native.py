@@ -38,13 +38,13 @@ MAX_KEY_COLS = 4
 EXPORTS = (
     "vdf_last_error", "vdf_version", "vdf_create", "vdf_destroy",
     "vdf_argsort_scratch_bytes", "vdf_groupby_scratch_bytes",
-    "vdf_argsort", "vdf_gather", "vdf_groupby",
+    "vdf_argsort", "vdf_sort_by", "vdf_gather", "vdf_groupby",
     "vdf_profile_enable", "vdf_profile_reset", "vdf_profile_read",
     "vdf_tune_set", "vdf_tune_get",
     "vdf_argsort_batch", "vdf_groupby_batch",
     "vdf_server_create", "vdf_server_destroy", "vdf_server_add_table", "vdf_server_submit",
     "vdf_server_wait", "vdf_server_release", "vdf_server_stats_get",
-    "vdf_autotune_argsort", "vdf_autotune_groupby_batch",
+    "vdf_autotune_argsort", "vdf_autotune_sort_by", "vdf_autotune_groupby_batch",
     "vdf_plan_groupby_batch", "vdf_plan_launch", "vdf_plan_scratch_bytes", "vdf_plan_destroy",
     "vdf_groupby_multi", "vdf_argsort_multi", "vdf_broadcast",
     "vdf_grouping_create", "vdf_grouping_aggregate", "vdf_grouping_destroy",
@@ -137,6 +137,8 @@ def load() -> C.CDLL:
     lib.vdf_groupby_scratch_bytes.argtypes = [i32, u64]
     lib.vdf_argsort.restype = i32
     lib.vdf_argsort.argtypes = [vp, i32, vp, u64, u32, vp, vp, vp]
+    lib.vdf_sort_by.restype = i32
+    lib.vdf_sort_by.argtypes = [vp, i32, vp, u64, u32, vp, vp, u32, vp, vp, vp]
     lib.vdf_gather.restype = i32
     lib.vdf_gather.argtypes = [vp, u32, vp, vp, i32, u64, vp, vp]
     lib.vdf_groupby.restype = i32
@@ -170,6 +172,8 @@ def load() -> C.CDLL:
     lib.vdf_server_stats_get.argtypes = [vp, C.POINTER(VdfServerStats)]
     lib.vdf_autotune_argsort.restype = i32
     lib.vdf_autotune_argsort.argtypes = [vp, i32, vp, u64, u32, vp, i32, vp, C.POINTER(VdfTuneReport)]
+    lib.vdf_autotune_sort_by.restype = i32
+    lib.vdf_autotune_sort_by.argtypes = [vp, i32, vp, u64, u32, vp, u32, vp, vp, i32, vp, C.POINTER(VdfTuneReport)]
     lib.vdf_autotune_groupby_batch.restype = i32
     lib.vdf_autotune_groupby_batch.argtypes = [vp, i32, C.POINTER(VdfTable), i32, C.POINTER(C.c_int), i32, u32,
                                                C.POINTER(VdfGroupbyOut), i32, vp, C.POINTER(VdfTuneReport)]

@@ -272,6 +272,35 @@ vdf_status vdf_argsort(vdf_handle* h, int key_type, const void* d_keys, uint64_t
     });
 }
 
+vdf_status vdf_sort_by(vdf_handle* h, int key_type, const void* d_keys, uint64_t n, uint32_t flags,
+                       void* d_perm_out, void* d_sorted_keys_out, uint32_t val_bytes, const void* d_vals,
+                       void* d_vals_out, void* stream)
+{
+    return guarded([&] {
+        require(h != nullptr, "vdf_sort_by: null handle");
+        require(key_bytes(key_type) != 0, "vdf_sort_by: unknown key type");
+        require((flags & ~(VDF_SORT_DESC | VDF_SORT_DESC_REVERSE | VDF_SORT_PERM64)) == 0,
+                "vdf_sort_by: unknown flag bits");
+        require(!((flags & VDF_SORT_DESC) && (flags & VDF_SORT_DESC_REVERSE)),
+                "vdf_sort_by: VDF_SORT_DESC and VDF_SORT_DESC_REVERSE are exclusive");
+        require(val_bytes == 4 || val_bytes == 8, "vdf_sort_by: val_bytes must be 4 or 8");
+        require(n <= VDF_MAX_ROWS, "vdf_sort_by: n exceeds VDF_MAX_ROWS (2^30 - 1)");
+        if (n == 0) return;
+        require(d_keys != nullptr && d_vals != nullptr && d_vals_out != nullptr, "vdf_sort_by: null device pointer");
+        CallScope cs(h);
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        const uint64_t hr = static_cast<uint64_t>(h->tune.hist_rows);
+        const int pb = static_cast<int>(val_bytes);
+        void* scr = h->ensure_scratch(argsort_batch_scratch(key_type, &n, 1, hr, pb));
+        h->staging_begin(argsort_batch_staging(key_type, &n, 1, hr));
+        StagingScope ss(h, s);
+        SortSeg seg{d_keys, n, d_perm_out, d_sorted_keys_out};
+        seg.pay_in = d_vals;
+        seg.pay_out = d_vals_out;
+        argsort_batch_enqueue(h, key_type, &seg, 1, flags, scr, s, pb);
+    });
+}
+
 vdf_status vdf_gather(vdf_handle* h, uint32_t elem_bytes, const void* d_src, const void* d_perm, int perm64,
                       uint64_t n, void* d_dst, void* stream)
 {

@@ -27,6 +27,7 @@
 #include "vdf_internal.h"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace vdf {
 
@@ -191,8 +192,10 @@ void fill_u32(uint32_t* p, size_t count, uint32_t value, hipStream_t s)
 // ---------------------------------------------------------------------------
 struct SegDesc {
     const void* keys;   // the segment's keys in the caller's form
-    void* perm_out;     // the segment's permutation (local row ids), u32 or u64
+    void* perm_out;     // the segment's permutation (local row ids), u32 or u64 (nullable)
     void* keys_out;     // the segment's sorted keys (nullable)
+    const void* pay_in; // payload column (batches with a payload)
+    void* pay_out;      // payload in sorted order
     uint64_t n;         // rows
     uint64_t row_off;   // first row in the concatenated scratch buffers
     uint32_t tile_off;  // first pass tile of the segment
@@ -220,6 +223,7 @@ struct SortArgs {
     uint32_t* plan_base;       // [S][D][256] segment-local bucket starts
     void* kbuf[2];             // transformed keys between passes (concatenated)
     uint32_t* vbuf[2];         // local row ids between passes (concatenated)
+    void* pbuf[2];             // payload between passes (concatenated)
     uint32_t* lb[2];           // look-back words [tiles][256], alternating by slot parity
     uint32_t* tickets;         // [kMaxDigits][kTicketStride] tile-claim counters
     uint64_t mlo, mhi;         // key transform masks
@@ -378,6 +382,8 @@ struct TileCtx {
     const void* keys;   // segment's caller keys (first pass)
     void* perm_out;
     void* keys_out;
+    const void* pay_in;
+    void* pay_out;
     const uint32_t* base;  // segment's bucket starts for this digit
 };
 
@@ -405,32 +411,35 @@ struct TileCtx {
 // out, so look-back waits overlap work.  Look-back never crosses the start of
 // the tile's segment.
 //
-// Ranking of item j in a wave: every lane ORs its lane bit into the wave's
-// 64-bit mask of its digit; reading the mask back gives the lanes sharing the
-// digit (peers).  The lowest peer adds the group size to the wave's bucket
-// counter and clears the mask; every peer reads the counter back.  LDS
-// operations of one wave complete in program order, so each read sees exactly
-// the items before j, and the result is the stable rank.
+// Ranking of item j in a wave: eight ballots, one per digit bit, give each
+// lane the lanes sharing its digit (peers).  Every lane reads the wave's
+// counter of its digit; the lowest peer writes it back advanced by the peer
+// count.  LDS operations of one wave complete in program order, so item j's
+// read sees exactly the items before j, and old + (peers below the lane) is
+// the stable rank.
 // Blocks of a pass kernel that fit one CU's 160 KiB of LDS; the kernel is
 // compiled to keep that many waves per SIMD (one wave of each block per SIMD),
 // so registers never cut occupancy below what LDS allows.
-template <typename K, int ITEMS>
+template <typename K, int ITEMS, int PB>
 struct PassShape {
-    static constexpr int kLds = kBlock * ITEMS * static_cast<int>(sizeof(K) + 4) + kWaves * kRadix * 12 + kRadix * 4 + 64;
-    static constexpr int kBlocksPerCu = (160 * 1024 / kLds) < 3 ? (160 * 1024 / kLds) : 3;
+    static constexpr int kLds = kBlock * ITEMS * static_cast<int>(sizeof(K) + 4 + PB) + kWaves * kRadix * 4 + kRadix * 4 + 64;
+    static constexpr int kBlocksPerCu = (160 * 1024 / kLds) < 4 ? (160 * 1024 / kLds) : 4;
 };
 
-template <typename K, int ITEMS>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PassShape<K, ITEMS>::kBlocksPerCu)))
+// PB: payload bytes per row (0, 4 or 8) moved with the keys.
+template <typename K, int ITEMS, int PB>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PassShape<K, ITEMS, PB>::kBlocksPerCu)))
 void k_pass(SortArgs a, uint32_t slot)
 {
     using KO = KeyOps<K>;
+    using P = typename std::conditional<PB == 8, uint64_t, uint32_t>::type;
+    constexpr int PI = PB ? ITEMS : 1;  // payload registers
     constexpr int TILE = kBlock * ITEMS;
     constexpr int D = KO::kDigits;
 
     __shared__ K s_keys[TILE];
     __shared__ uint32_t s_vals[TILE];
-    __shared__ uint64_t s_mask[kWaves][kRadix];
+    __shared__ P s_pay[PB ? TILE : 1];
     __shared__ uint32_t s_wc[kWaves][kRadix];
     __shared__ uint32_t s_gbase[kRadix];
     __shared__ uint32_t s_wsum[kWaves];
@@ -443,6 +452,7 @@ void k_pass(SortArgs a, uint32_t slot)
     const uint32_t* vin = a.vbuf[(slot - 1) & 1];
     K key[ITEMS];
     uint32_t val[ITEMS];
+    P pay[PI];
     uint32_t rnk[ITEMS];
 
     auto claim = [&]() -> uint32_t {
@@ -479,6 +489,8 @@ void k_pass(SortArgs a, uint32_t slot)
             c.keys = sd.keys;
             c.perm_out = sd.perm_out;
             c.keys_out = sd.keys_out;
+            c.pay_in = sd.pay_in;
+            c.pay_out = sd.pay_out;
             c.base = a.plan_base + (static_cast<uint64_t>(sg) * D + c.dig) * kRadix;
             return true;
         }
@@ -491,7 +503,6 @@ void k_pass(SortArgs a, uint32_t slot)
 #pragma unroll
         for (int w = 0; w < kWaves; ++w) {
             s_wc[w][tid] = 0;
-            s_mask[w][tid] = 0;
         }
         lbn[static_cast<uint64_t>(c.t) * kRadix + tid] = 0u;  // the next pass's row of this tile
         const uint64_t wl = c.lrow0 + static_cast<uint64_t>(wave) * 64 * ITEMS + lane;  // local row of item 0
@@ -500,6 +511,17 @@ void k_pass(SortArgs a, uint32_t slot)
         // rows past the end read the segment's last row and are masked below.
         const bool full = c.lrow0 + TILE <= c.n;
         const uint64_t last = c.n - 1;
+        if constexpr (PB != 0) {
+            const auto* pp = c.first ? gmem(static_cast<const P*>(c.pay_in))
+                                     : gmem(static_cast<const P*>(a.pbuf[(slot - 1) & 1]) + c.row_off);
+            if (full) {
+#pragma unroll
+                for (int j = 0; j < PI; ++j) pay[j] = pp[wl + static_cast<uint64_t>(j) * 64];
+            } else {
+#pragma unroll
+                for (int j = 0; j < PI; ++j) pay[j] = pp[std::min(wl + static_cast<uint64_t>(j) * 64, last)];
+            }
+        }
         if (c.first) {
             if (full) {
 #pragma unroll
@@ -534,26 +556,24 @@ void k_pass(SortArgs a, uint32_t slot)
         }
         lds_barrier();
         uint32_t* wc = s_wc[wave];
-        unsigned long long* mk = reinterpret_cast<unsigned long long*>(s_mask[wave]);
-        const unsigned long long lanebit = 1ull << lane;
         const int dig = static_cast<int>(c.dig);
 #pragma unroll
         for (int j = 0; j < ITEMS; ++j) {
             const bool v = (wl + static_cast<uint64_t>(j) * 64) < c.n;
             const uint32_t d = KO::digit(key[j], dig);
-            if (v) atomicOr(&mk[d], lanebit);
-            unsigned long long peers = 0;
-            if (v) peers = mk[d];
+            uint64_t peers = __ballot(v);
+#pragma unroll
+            for (int b = 0; b < 8; ++b) {
+                const bool bit = (d >> b) & 1u;
+                const uint64_t m = __ballot(bit);
+                peers &= bit ? m : ~m;
+            }
             const uint32_t pl = static_cast<uint32_t>(peers), ph = static_cast<uint32_t>(peers >> 32);
             const uint32_t lt = __builtin_amdgcn_mbcnt_hi(ph, __builtin_amdgcn_mbcnt_lo(pl, 0u));
             const uint32_t cnt = static_cast<uint32_t>(__popc(pl) + __popc(ph));
-            if (v && lt == 0u) {
-                atomicAdd(&wc[d], cnt);
-                mk[d] = 0ull;
-            }
-            uint32_t after = 0;
-            if (v) after = wc[d];
-            rnk[j] = after - cnt + lt;
+            const uint32_t old = wc[d];
+            if (v && lt == 0u) wc[d] = old + cnt;
+            rnk[j] = old + lt;
         }
         lds_barrier();
         tot = 0;
@@ -579,6 +599,7 @@ void k_pass(SortArgs a, uint32_t slot)
                 const uint32_t p = s_wc[wave][KO::digit(key[j], dig)] + rnk[j];
                 s_keys[p] = key[j];
                 s_vals[p] = val[j];
+                if constexpr (PB != 0) s_pay[p] = pay[j];
             }
         }
         lds_barrier();
@@ -625,11 +646,13 @@ void k_pass(SortArgs a, uint32_t slot)
         if (!c.last) {
             K* kout = static_cast<K*>(a.kbuf[slot & 1]) + c.row_off;
             auto* vout = gmem(a.vbuf[slot & 1] + c.row_off);
+            auto* pout = gmem(static_cast<P*>(a.pbuf[slot & 1]) + c.row_off);
             for (uint32_t i = tid; i < tn; i += kBlock) {
                 const K kk = s_keys[i];
                 const uint32_t p = s_gbase[KO::digit(kk, dig)] + i;
                 KO::put(kout, p, kk);
                 vout[p] = s_vals[i];
+                if constexpr (PB != 0) pout[p] = s_pay[i];
             }
         } else {
             const uint32_t nm1 = static_cast<uint32_t>(c.n - 1);
@@ -638,10 +661,13 @@ void k_pass(SortArgs a, uint32_t slot)
                 uint32_t p = s_gbase[KO::digit(kk, dig)] + i;
                 if (a.reverse) p = nm1 - p;
                 if (c.keys_out) KO::store(c.keys_out, p, kk, a.mlo, a.mhi);
-                if (a.perm64)
-                    gmem(static_cast<uint64_t*>(c.perm_out))[p] = s_vals[i];
-                else
-                    gmem(static_cast<uint32_t*>(c.perm_out))[p] = s_vals[i];
+                if (c.perm_out) {
+                    if (a.perm64)
+                        gmem(static_cast<uint64_t*>(c.perm_out))[p] = s_vals[i];
+                    else
+                        gmem(static_cast<uint32_t*>(c.perm_out))[p] = s_vals[i];
+                }
+                if constexpr (PB != 0) gmem(static_cast<P*>(c.pay_out))[p] = s_pay[i];
             }
         }
         lds_barrier();
@@ -701,7 +727,7 @@ static uint64_t hist_items(uint64_t n, uint64_t rows_per_item)
     return n == 0 ? 0 : (n + rows_per_item - 1) / rows_per_item;
 }
 
-static SortLayout sort_layout(int key_type, const uint64_t* n, int nsegs, uint64_t hist_rows)
+static SortLayout sort_layout(int key_type, const uint64_t* n, int nsegs, uint64_t hist_rows, int pay_bytes = 0)
 {
     SortLayout L;
     for (int s = 0; s < nsegs; ++s) {
@@ -722,6 +748,7 @@ static SortLayout sort_layout(int key_type, const uint64_t* n, int nsegs, uint64
     b += 2 * align_up(std::max<uint64_t>(L.tiles, 1) * kRadix * 4, 256);
     b += 2 * align_up(L.rows * kb, 256);
     b += 2 * align_up(L.rows * 4, 256);
+    b += 2 * align_up(L.rows * static_cast<uint64_t>(pay_bytes), 256);
     L.bytes = b + 256;
     return L;
 }
@@ -739,9 +766,9 @@ size_t argsort_scratch(int key_type, uint64_t n, uint32_t)
     return sort_layout(key_type, &n, 1, kDefaultHistRows).bytes;
 }
 
-size_t argsort_batch_scratch(int key_type, const uint64_t* n, int nsegs, uint64_t hist_rows)
+size_t argsort_batch_scratch(int key_type, const uint64_t* n, int nsegs, uint64_t hist_rows, int pay_bytes)
 {
-    return sort_layout(key_type, n, nsegs, hist_rows).bytes;
+    return sort_layout(key_type, n, nsegs, hist_rows, pay_bytes).bytes;
 }
 
 size_t argsort_batch_staging(int key_type, const uint64_t* n, int nsegs, uint64_t hist_rows)
@@ -751,32 +778,63 @@ size_t argsort_batch_staging(int key_type, const uint64_t* n, int nsegs, uint64_
 
 using PassFn = void (*)(SortArgs, uint32_t);
 
-// Pass kernel instantiations the tuning table can select.
+// Pass kernel instantiations the tuning table can select (tile rows per
+// thread), without and with a 4- or 8-byte payload.
 template <typename K>
-static PassFn pick_pass(int items);
+static PassFn pick_pass(int items, int pay_bytes = 0);
 
 template <>
-PassFn pick_pass<uint32_t>(int items)
+PassFn pick_pass<uint32_t>(int items, int pay_bytes)
 {
-    if (items == 12) return k_pass<uint32_t, 12>;
-    if (items == 16) return k_pass<uint32_t, 16>;
-    if (items == 24) return k_pass<uint32_t, 24>;
-    return nullptr;
+    switch (pay_bytes) {
+    case 0:
+        if (items == 12) return k_pass<uint32_t, 12, 0>;
+        if (items == 16) return k_pass<uint32_t, 16, 0>;
+        if (items == 24) return k_pass<uint32_t, 24, 0>;
+        return nullptr;
+    case 4:
+        if (items == 8) return k_pass<uint32_t, 8, 4>;
+        if (items == 12) return k_pass<uint32_t, 12, 4>;
+        return nullptr;
+    case 8:
+        if (items == 12) return k_pass<uint32_t, 12, 8>;
+        if (items == 16) return k_pass<uint32_t, 16, 8>;
+        return nullptr;
+    default: return nullptr;
+    }
 }
 
 template <>
-PassFn pick_pass<uint64_t>(int items)
+PassFn pick_pass<uint64_t>(int items, int pay_bytes)
 {
-    if (items == 8) return k_pass<uint64_t, 8>;
-    if (items == 12) return k_pass<uint64_t, 12>;
-    if (items == 16) return k_pass<uint64_t, 16>;
-    return nullptr;
+    switch (pay_bytes) {
+    case 0:
+        if (items == 8) return k_pass<uint64_t, 8, 0>;
+        if (items == 12) return k_pass<uint64_t, 12, 0>;
+        if (items == 16) return k_pass<uint64_t, 16, 0>;
+        return nullptr;
+    case 4:
+        if (items == 8) return k_pass<uint64_t, 8, 4>;
+        if (items == 12) return k_pass<uint64_t, 12, 4>;
+        return nullptr;
+    case 8:
+        if (items == 8) return k_pass<uint64_t, 8, 8>;
+        if (items == 12) return k_pass<uint64_t, 12, 8>;
+        return nullptr;
+    default: return nullptr;
+    }
 }
 
 template <>
-PassFn pick_pass<K128>(int items)
+PassFn pick_pass<K128>(int items, int pay_bytes)
 {
-    return items == 8 ? k_pass<K128, 8> : nullptr;
+    if (items != 8) return nullptr;
+    switch (pay_bytes) {
+    case 0: return k_pass<K128, 8, 0>;
+    case 4: return k_pass<K128, 8, 4>;
+    case 8: return k_pass<K128, 8, 8>;
+    default: return nullptr;
+    }
 }
 
 bool tune_supported(const char* name, int64_t v)
@@ -830,21 +888,22 @@ static unsigned resident_blocks(PassFn fn, int device)
 
 template <typename K>
 static void launch_sort(Handle* h, const SortSeg* segs, int nsegs, uint64_t mlo, uint64_t mhi, uint32_t flags,
-                        void* scr, hipStream_t s)
+                        void* scr, hipStream_t s, int pay_bytes)
 {
     constexpr int D = KeyOps<K>::kDigits;
     int items = sizeof(K) == 4 ? h->tune.sort_items_32 : (sizeof(K) == 8 ? h->tune.sort_items_64 : 8);
-    PassFn pass = pick_pass<K>(items);
-    if (!pass) {  // tuning value without an instantiation for this key width
-        items = sizeof(K) == 4 ? 16 : 8;
-        pass = pick_pass<K>(items);
+    PassFn pass = pick_pass<K>(items, pay_bytes);
+    if (!pass) {  // tuning value without an instantiation for this key width / payload
+        items = sizeof(K) == 4 ? (pay_bytes == 4 ? 12 : 16) : 8;
+        pass = pick_pass<K>(items, pay_bytes);
     }
+    require(pass != nullptr, "sort payload must be 0, 4 or 8 bytes per row");
     const uint64_t tile_rows = static_cast<uint64_t>(kBlock) * items;
     const uint64_t hist_rows = static_cast<uint64_t>(h->tune.hist_rows);
     std::vector<uint64_t> ns(nsegs);
     for (int i = 0; i < nsegs; ++i) ns[i] = segs[i].n;
     const SortLayout L = sort_layout(sizeof(K) == 4 ? VDF_KEY_U32 : (sizeof(K) == 8 ? VDF_KEY_U64 : VDF_KEY_CHAR16),
-                                     ns.data(), nsegs, hist_rows);
+                                     ns.data(), nsegs, hist_rows, pay_bytes);
 
     // ---- host-built batch description, uploaded through pinned staging ----
     char* stg = static_cast<char*>(h->staging_take(staging_bytes(L, nsegs)));
@@ -856,7 +915,8 @@ static void launch_sort(Handle* h, const SortSeg* segs, int nsegs, uint64_t mlo,
     for (int i = 0; i < nsegs; ++i) {
         const uint64_t n = segs[i].n;
         const uint64_t nt = (n + tile_rows - 1) / tile_rows;
-        hs[i] = SegDesc{segs[i].keys, segs[i].perm_out, segs[i].keys_out, n, row, static_cast<uint32_t>(tiles), 0};
+        hs[i] = SegDesc{segs[i].keys, segs[i].perm_out, segs[i].keys_out, segs[i].pay_in, segs[i].pay_out,
+                        n, row, static_cast<uint32_t>(tiles), 0};
         for (uint64_t t = 0; t < nt; ++t) ht[tiles + t] = static_cast<uint32_t>(i);
         for (uint64_t b = 0; b < n; b += hist_rows) hi[nitems++] = HistItem{static_cast<uint32_t>(i), 0, b, std::min(n, b + hist_rows)};
         row += n;
@@ -881,6 +941,8 @@ static void launch_sort(Handle* h, const SortSeg* segs, int nsegs, uint64_t mlo,
     a.kbuf[1] = c.take<char>(L.rows * sizeof(K));
     a.vbuf[0] = c.take<uint32_t>(L.rows);
     a.vbuf[1] = c.take<uint32_t>(L.rows);
+    a.pbuf[0] = c.take<char>(L.rows * static_cast<uint64_t>(pay_bytes));
+    a.pbuf[1] = c.take<char>(L.rows * static_cast<uint64_t>(pay_bytes));
     const size_t up = reinterpret_cast<char*>(ht + ntiles) - stg;
     VDF_HIP_CHECK(hipMemcpyAsync(dsegs, stg, up, hipMemcpyHostToDevice, s));  // descs, items, tile map
     // the host carve mirrors the device carve (same order, same alignment)
@@ -947,12 +1009,14 @@ static void launch_sort(Handle* h, const SortSeg* segs, int nsegs, uint64_t mlo,
         acct->rows = ns;
         // keys read + keys written + row ids read + row ids written, per row, when
         // the pass is the table's only / first / middle / last one
-        acct->per_row[0] = kb + kout + pb_out;
-        acct->per_row[1] = kb + kb + 4;
-        acct->per_row[2] = 2 * kb + 8;
-        acct->per_row[3] = kb + 4 + kout + pb_out;
+        // (plus the payload read and written in every pass)
+        const uint64_t pay2 = 2 * static_cast<uint64_t>(pay_bytes);
+        acct->per_row[0] = kb + kout + pb_out + pay2;
+        acct->per_row[1] = kb + kb + 4 + pay2;
+        acct->per_row[2] = 2 * kb + 8 + pay2;
+        acct->per_row[3] = kb + 4 + kout + pb_out + pay2;
     }
-    const uint64_t middle_bytes = L.rows * (2 * kb + 8);
+    const uint64_t middle_bytes = L.rows * (2 * kb + 8 + 2 * static_cast<uint64_t>(pay_bytes));
     bool coop = xcd;
     for (int slot = 0; slot < D; ++slot) {
         LaunchTimer t(h, s, "radix_pass", middle_bytes, slot, acct);
@@ -1007,14 +1071,14 @@ static void masks_for(int key_type, uint32_t flags, uint64_t& mlo, uint64_t& mhi
 }
 
 void argsort_batch_enqueue(Handle* h, int key_type, const SortSeg* segs, int nsegs, uint32_t flags, void* scr,
-                           hipStream_t s)
+                           hipStream_t s, int pay_bytes)
 {
     uint64_t mlo = 0, mhi = 0;
     masks_for(key_type, flags, mlo, mhi);
     switch (key_bytes(key_type)) {
-    case 4: launch_sort<uint32_t>(h, segs, nsegs, mlo, mhi, flags, scr, s); break;
-    case 8: launch_sort<uint64_t>(h, segs, nsegs, mlo, mhi, flags, scr, s); break;
-    default: launch_sort<K128>(h, segs, nsegs, mlo, mhi, flags, scr, s); break;
+    case 4: launch_sort<uint32_t>(h, segs, nsegs, mlo, mhi, flags, scr, s, pay_bytes); break;
+    case 8: launch_sort<uint64_t>(h, segs, nsegs, mlo, mhi, flags, scr, s, pay_bytes); break;
+    default: launch_sort<K128>(h, segs, nsegs, mlo, mhi, flags, scr, s, pay_bytes); break;
     }
 }
 

@@ -46,13 +46,18 @@ void run_groupby(Handle* h, int kt, const std::vector<GbTable>& ts, const int* v
     h->staging_end(s);
 }
 
-void run_argsort(Handle* h, int kt, const void* keys, uint64_t n, uint32_t flags, void* perm, hipStream_t s)
+// one sort; with pay_bytes > 0 a value column rides along (pay_in -> pay_out)
+void run_argsort(Handle* h, int kt, const void* keys, uint64_t n, uint32_t flags, void* perm, hipStream_t s,
+                 int pay_bytes = 0, const void* pay_in = nullptr, void* pay_out = nullptr)
 {
     const uint64_t hr = static_cast<uint64_t>(h->tune.hist_rows);
-    void* scr = h->ensure_scratch(argsort_batch_scratch(kt, &n, 1, hr));
+    void* scr = h->ensure_scratch(argsort_batch_scratch(kt, &n, 1, hr, pay_bytes));
     h->staging_begin(argsort_batch_staging(kt, &n, 1, hr));
     try {
-        argsort_enqueue(h, kt, keys, n, flags, perm, nullptr, scr, s);
+        SortSeg seg{keys, n, perm, nullptr};
+        seg.pay_in = pay_in;
+        seg.pay_out = pay_out;
+        argsort_batch_enqueue(h, kt, &seg, 1, flags, scr, s, pay_bytes);
     } catch (...) {
         h->staging_end(s);
         throw;
@@ -60,8 +65,11 @@ void run_argsort(Handle* h, int kt, const void* keys, uint64_t n, uint32_t flags
     h->staging_end(s);
 }
 
-// Coordinate descent over the knobs; returns trials run.
-int descend(Handle* h, int kb, int reps, hipStream_t s, const std::function<void()>& run, vdf_tune_report* rep)
+// Coordinate descent over the knobs; returns trials run.  kb: key bytes;
+// pay_bytes: payload bytes moved with the keys (selects the tile sizes that
+// have pass kernels with that payload).
+int descend(Handle* h, int kb, int reps, hipStream_t s, const std::function<void()>& run, vdf_tune_report* rep,
+            int pay_bytes = 0)
 {
     hipEvent_t e0, e1;
     VDF_HIP_CHECK(hipEventCreate(&e0));
@@ -84,8 +92,12 @@ int descend(Handle* h, int kb, int reps, hipStream_t s, const std::function<void
         return best;
     };
     std::vector<Knob> knobs;
-    if (kb == 4) knobs.push_back({&Tuning::sort_items_32, {12, 16, 24}});
-    if (kb == 8) knobs.push_back({&Tuning::sort_items_64, {8, 12, 16}});
+    if (kb == 4)
+        knobs.push_back({&Tuning::sort_items_32,
+                         pay_bytes == 0 ? std::vector<int>{12, 16, 24}
+                                        : (pay_bytes == 4 ? std::vector<int>{8, 12} : std::vector<int>{12, 16})});
+    if (kb == 8)
+        knobs.push_back({&Tuning::sort_items_64, pay_bytes == 0 ? std::vector<int>{8, 12, 16} : std::vector<int>{8, 12}});
     knobs.push_back({&Tuning::sort_bpc, {0, 1, 2, 3}});
     knobs.push_back({&Tuning::sort_xcd, {1, 0}});
     knobs.push_back({&Tuning::sort_chunk, {4, 16, 64, 256}});
@@ -194,6 +206,29 @@ vdf_status vdf_autotune_argsort(vdf_handle* h, int key_type, const void* d_keys,
         const hipStream_t s = static_cast<hipStream_t>(stream);
         descend(hh, static_cast<int>(key_bytes(key_type)), reps, s,
                 [&] { run_argsort(hh, key_type, d_keys, n, flags, d_perm_out, s); }, report);
+    });
+}
+
+vdf_status vdf_autotune_sort_by(vdf_handle* h, int key_type, const void* d_keys, uint64_t n, uint32_t flags,
+                                void* d_perm_out, uint32_t val_bytes, const void* d_vals, void* d_vals_out, int reps,
+                                void* stream, vdf_tune_report* report)
+{
+    const vdf_status st =
+        vdf_sort_by(h, key_type, d_keys, n, flags, d_perm_out, nullptr, val_bytes, d_vals, d_vals_out, stream);
+    if (st != VDF_OK) return st;
+    return guarded([&] {
+        require(report != nullptr, "vdf_autotune_sort_by: null report");
+        require(reps >= 1 && reps <= 100, "vdf_autotune_sort_by: reps must be 1 .. 100");
+        *report = vdf_tune_report{};
+        if (n == 0) return;
+        Handle* hh = h;
+        TuneScope scope(hh);
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        const int pb = static_cast<int>(val_bytes);
+        descend(
+            hh, static_cast<int>(key_bytes(key_type)), reps, s,
+            [&] { run_argsort(hh, key_type, d_keys, n, flags, d_perm_out, s, pb, d_vals, d_vals_out); }, report,
+            pb);
     });
 }
 

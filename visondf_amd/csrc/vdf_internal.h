@@ -88,8 +88,10 @@ constexpr uint64_t kDefaultHistRows = 65536;  // rows per histogram work item
 struct SortSeg {
     const void* keys;
     uint64_t n;
-    void* perm_out;   // local row ids, u32 or (VDF_SORT_PERM64) u64
+    void* perm_out;   // local row ids, u32 or (VDF_SORT_PERM64) u64; nullable
     void* keys_out;   // nullable
+    const void* pay_in = nullptr;  // payload column carried through the passes (batch pay_bytes per row)
+    void* pay_out = nullptr;       // the payload in sorted order
 };
 
 // ---------------------------------------------------------------------------
@@ -250,15 +252,17 @@ struct Carve {
 size_t key_bytes(int key_type);
 size_t argsort_scratch(int key_type, uint64_t n, uint32_t flags);
 
-size_t argsort_batch_scratch(int key_type, const uint64_t* n, int nsegs, uint64_t hist_rows);
+size_t argsort_batch_scratch(int key_type, const uint64_t* n, int nsegs, uint64_t hist_rows, int pay_bytes = 0);
 size_t argsort_batch_staging(int key_type, const uint64_t* n, int nsegs, uint64_t hist_rows);
 
 // Sorts into d_perm_out / d_keys_out using scratch at `scr` (argsort_scratch bytes).
 void argsort_enqueue(Handle* h, int key_type, const void* d_keys, uint64_t n, uint32_t flags,
                      void* d_perm_out, void* d_keys_out, void* scr, hipStream_t s);
 // Sorts every segment of a batch (argsort_batch_scratch bytes of scratch).
+// pay_bytes (0, 4 or 8): every segment's payload column (pay_in -> pay_out)
+// moves with its keys through the passes.
 void argsort_batch_enqueue(Handle* h, int key_type, const SortSeg* segs, int nsegs, uint32_t flags, void* scr,
-                           hipStream_t s);
+                           hipStream_t s, int pay_bytes = 0);
 
 void gather_enqueue(Handle* h, uint32_t elem_bytes, const void* d_src, const void* d_perm,
                     int perm64, uint64_t n, void* d_dst, hipStream_t s);

@@ -105,6 +105,26 @@ class Vdf:
                                      _ptr(keys_out) if want_keys else None, C.c_void_p(_stream_ptr(stream))))
         return (perm_out, keys_out) if want_keys else perm_out
 
+    def sort_by(self, keys: torch.Tensor, vals: torch.Tensor, *, key_type: int | None = None,
+                descending: bool = False, reverse_desc: bool = False, perm64: bool = False,
+                want_perm: bool = True, want_keys: bool = False, perm_out=None, vals_out=None, stream=None):
+        """Sorts `keys` and carries one 4- or 8-byte value column through the
+        passes: returns (perm or None, vals in sorted order[, sorted keys])."""
+        kt = key_type_of(keys) if key_type is None else key_type
+        n = keys.shape[0]
+        flags = (N.SORT_DESC if descending else 0) | (N.SORT_DESC_REVERSE if reverse_desc else 0) \
+            | (N.SORT_PERM64 if perm64 else 0)
+        if want_perm and perm_out is None:
+            perm_out = torch.empty(n, dtype=torch.int64 if perm64 else torch.int32, device=keys.device)
+        if vals_out is None:
+            vals_out = torch.empty_like(vals)
+        keys_out = torch.empty_like(keys) if want_keys else None
+        N.check(self.lib.vdf_sort_by(self.h, kt, _ptr(keys), n, flags, _ptr(perm_out) if want_perm else None,
+                                     _ptr(keys_out), vals.element_size(), _ptr(vals), _ptr(vals_out),
+                                     C.c_void_p(_stream_ptr(stream))))
+        out = (perm_out if want_perm else None, vals_out)
+        return out + (keys_out,) if want_keys else out
+
     # ----------------------------------------------------------------- gather
     def gather(self, src: torch.Tensor, perm: torch.Tensor, out=None, stream=None):
         n = perm.shape[0]
@@ -307,6 +327,16 @@ class Vdf:
         kt = key_type_of(keys) if key_type is None else key_type
         rep = N.VdfTuneReport()
         N.check(self.lib.vdf_autotune_argsort(self.h, kt, _ptr(keys), keys.shape[0], flags, _ptr(perm_out), reps,
+                                              C.c_void_p(_stream_ptr(stream)), C.byref(rep)))
+        return {k: getattr(rep, k) for k, _ in N.VdfTuneReport._fields_ if k != "reserved"}
+
+    def autotune_sort_by(self, keys: torch.Tensor, vals: torch.Tensor, perm_out: torch.Tensor | None,
+                         vals_out: torch.Tensor, *, key_type: int | None = None, flags: int = 0, reps: int = 3,
+                         stream=None) -> dict:
+        kt = key_type_of(keys) if key_type is None else key_type
+        rep = N.VdfTuneReport()
+        N.check(self.lib.vdf_autotune_sort_by(self.h, kt, _ptr(keys), keys.shape[0], flags, _ptr(perm_out),
+                                              vals.element_size(), _ptr(vals), _ptr(vals_out), reps,
                                               C.c_void_p(_stream_ptr(stream)), C.byref(rep)))
         return {k: getattr(rep, k) for k, _ in N.VdfTuneReport._fields_ if k != "reserved"}
 
