@@ -561,14 +561,18 @@ void k_pass(SortArgs a, uint32_t slot)
         for (int j = 0; j < ITEMS; ++j) {
             const bool v = (wl + static_cast<uint64_t>(j) * 64) < c.n;
             const uint32_t d = KO::digit(key[j], dig);
-            uint64_t peers = __ballot(v);
+            const uint64_t vm = __ballot(v);
+            uint32_t pl = static_cast<uint32_t>(vm), ph = static_cast<uint32_t>(vm >> 32);
 #pragma unroll
             for (int b = 0; b < 8; ++b) {
-                const bool bit = (d >> b) & 1u;
-                const uint64_t m = __ballot(bit);
-                peers &= bit ? m : ~m;
+                // bm: bit b of the digit, sign-extended (all ones or zero); m: lanes with
+                // the bit set; peers &= ~(m ^ bm) keeps the lanes that agree on bit b
+                // (v_bitop3 truth table 0x90 = src0 & ~(src1 ^ src2))
+                const uint32_t bm = static_cast<uint32_t>(static_cast<int32_t>(d << (31 - b)) >> 31);
+                const uint64_t m = __ballot(bm != 0u);
+                pl = __builtin_amdgcn_bitop3_b32(pl, static_cast<uint32_t>(m), bm, 0x90);
+                ph = __builtin_amdgcn_bitop3_b32(ph, static_cast<uint32_t>(m >> 32), bm, 0x90);
             }
-            const uint32_t pl = static_cast<uint32_t>(peers), ph = static_cast<uint32_t>(peers >> 32);
             const uint32_t lt = __builtin_amdgcn_mbcnt_hi(ph, __builtin_amdgcn_mbcnt_lo(pl, 0u));
             const uint32_t cnt = static_cast<uint32_t>(__popc(pl) + __popc(ph));
             const uint32_t old = wc[d];
