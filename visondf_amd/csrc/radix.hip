@@ -423,7 +423,8 @@ struct TileCtx {
 template <typename K, int ITEMS, int PB>
 struct PassShape {
     static constexpr int kLds = kBlock * ITEMS * static_cast<int>(sizeof(K) + 4 + PB) + kWaves * kRadix * 4 + kRadix * 4 + 64;
-    static constexpr int kBlocksPerCu = (160 * 1024 / kLds) < 4 ? (160 * 1024 / kLds) : 4;
+    static constexpr int kCap = ITEMS >= 24 ? 2 : 3;  // register room for the larger tiles
+    static constexpr int kBlocksPerCu = (160 * 1024 / kLds) < kCap ? (160 * 1024 / kLds) : kCap;
 };
 
 // PB: payload bytes per row (0, 4 or 8) moved with the keys.
@@ -597,10 +598,14 @@ void k_pass(SortArgs a, uint32_t slot)
     auto stage = [&](const TileCtx& c) {
         const uint64_t wl = c.lrow0 + static_cast<uint64_t>(wave) * 64 * ITEMS + lane;
         const int dig = static_cast<int>(c.dig);
+        // every item's slot is read first (one LDS wait for all), then the
+        // valid items are written
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) rnk[j] += s_wc[wave][KO::digit(key[j], dig)];
 #pragma unroll
         for (int j = 0; j < ITEMS; ++j) {
             if (wl + static_cast<uint64_t>(j) * 64 < c.n) {
-                const uint32_t p = s_wc[wave][KO::digit(key[j], dig)] + rnk[j];
+                const uint32_t p = rnk[j];
                 s_keys[p] = key[j];
                 s_vals[p] = val[j];
                 if constexpr (PB != 0) s_pay[p] = pay[j];
@@ -647,31 +652,66 @@ void k_pass(SortArgs a, uint32_t slot)
         lds_barrier();
         const uint32_t tn = static_cast<uint32_t>(std::min<uint64_t>(TILE, c.n - c.lrow0));
         const int dig = static_cast<int>(c.dig);
+        // Rows go out U per thread at a time: the LDS reads of U rows are
+        // issued before their stores, so one LDS wait covers U rows.  Tiles
+        // are a multiple of U * kBlock rows, so every LDS index is inside the
+        // staging arrays; rows past a partial tile's end are not stored.
+        constexpr int U = 4;
+        static_assert(ITEMS % U == 0, "tile rows per thread must be a multiple of U");
         if (!c.last) {
             K* kout = static_cast<K*>(a.kbuf[slot & 1]) + c.row_off;
             auto* vout = gmem(a.vbuf[slot & 1] + c.row_off);
             auto* pout = gmem(static_cast<P*>(a.pbuf[slot & 1]) + c.row_off);
-            for (uint32_t i = tid; i < tn; i += kBlock) {
-                const K kk = s_keys[i];
-                const uint32_t p = s_gbase[KO::digit(kk, dig)] + i;
-                KO::put(kout, p, kk);
-                vout[p] = s_vals[i];
-                if constexpr (PB != 0) pout[p] = s_pay[i];
+            for (uint32_t i0 = tid; i0 < tn; i0 += U * kBlock) {
+                K kk[U];
+                uint32_t pos[U], vv[U];
+                P pv[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) kk[u] = s_keys[i0 + u * kBlock];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    pos[u] = s_gbase[KO::digit(kk[u], dig)] + i0 + u * kBlock;
+                    vv[u] = s_vals[i0 + u * kBlock];
+                    if constexpr (PB != 0) pv[u] = s_pay[i0 + u * kBlock];
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    if (i0 + u * kBlock < tn) {
+                        KO::put(kout, pos[u], kk[u]);
+                        vout[pos[u]] = vv[u];
+                        if constexpr (PB != 0) pout[pos[u]] = pv[u];
+                    }
+                }
             }
         } else {
             const uint32_t nm1 = static_cast<uint32_t>(c.n - 1);
-            for (uint32_t i = tid; i < tn; i += kBlock) {
-                const K kk = s_keys[i];
-                uint32_t p = s_gbase[KO::digit(kk, dig)] + i;
-                if (a.reverse) p = nm1 - p;
-                if (c.keys_out) KO::store(c.keys_out, p, kk, a.mlo, a.mhi);
-                if (c.perm_out) {
-                    if (a.perm64)
-                        gmem(static_cast<uint64_t*>(c.perm_out))[p] = s_vals[i];
-                    else
-                        gmem(static_cast<uint32_t*>(c.perm_out))[p] = s_vals[i];
+            for (uint32_t i0 = tid; i0 < tn; i0 += U * kBlock) {
+                K kk[U];
+                uint32_t pos[U], vv[U];
+                P pv[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) kk[u] = s_keys[i0 + u * kBlock];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const uint32_t p = s_gbase[KO::digit(kk[u], dig)] + i0 + u * kBlock;
+                    pos[u] = a.reverse ? nm1 - p : p;
+                    vv[u] = s_vals[i0 + u * kBlock];
+                    if constexpr (PB != 0) pv[u] = s_pay[i0 + u * kBlock];
                 }
-                if constexpr (PB != 0) gmem(static_cast<P*>(c.pay_out))[p] = s_pay[i];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    if (i0 + u * kBlock < tn) {
+                        const uint32_t p = pos[u];
+                        if (c.keys_out) KO::store(c.keys_out, p, kk[u], a.mlo, a.mhi);
+                        if (c.perm_out) {
+                            if (a.perm64)
+                                gmem(static_cast<uint64_t*>(c.perm_out))[p] = vv[u];
+                            else
+                                gmem(static_cast<uint32_t*>(c.perm_out))[p] = vv[u];
+                        }
+                        if constexpr (PB != 0) gmem(static_cast<P*>(c.pay_out))[p] = pv[u];
+                    }
+                }
             }
         }
         lds_barrier();
