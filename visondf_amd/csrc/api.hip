@@ -333,7 +333,10 @@ vdf_status vdf_groupby(vdf_handle* h, int key_type, const void* d_keys, uint64_t
         require(n == 0 || d_keys != nullptr, "vdf_groupby: null keys");
         CallScope cs(h);
         const uint64_t hr = static_cast<uint64_t>(h->tune.hist_rows);
-        void* scr = h->ensure_scratch(groupby_batch_scratch(key_type, &n, 1, hr));
+        int vt[VDF_MAX_VALUE_COLS] = {0, 0, 0, 0};
+        for (int c = 0; c < nvals; ++c) vt[c] = vals[c].type;
+        const int pay = groupby_carry_bytes(h->tune, vt, nvals);
+        void* scr = h->ensure_scratch(groupby_batch_scratch(key_type, &n, 1, hr, pay));
         h->staging_begin(groupby_batch_staging(key_type, &n, 1, hr));
         StagingScope ss(h, static_cast<hipStream_t>(stream));
         groupby_enqueue(h, key_type, d_keys, n, vals, nvals, agg_mask, out, scr, static_cast<hipStream_t>(stream));
@@ -402,7 +405,8 @@ vdf_status vdf_groupby_batch(vdf_handle* h, int key_type, const vdf_table* table
         }
         CallScope cs(h);
         const uint64_t hr = static_cast<uint64_t>(h->tune.hist_rows);
-        void* scr = h->ensure_scratch(groupby_batch_scratch(key_type, ns.data(), ntables, hr));
+        void* scr = h->ensure_scratch(groupby_batch_scratch(key_type, ns.data(), ntables, hr,
+                                                            groupby_carry_bytes(h->tune, val_types, nvals)));
         h->staging_begin(groupby_batch_staging(key_type, ns.data(), ntables, hr));
         StagingScope st(h, static_cast<hipStream_t>(stream));
         groupby_batch_enqueue(h, key_type, ts.data(), ntables, val_types, nvals, agg_mask, outs, scr,
@@ -452,7 +456,7 @@ vdf_status vdf_groupby_multi(vdf_handle* h, const vdf_key_col* keys, int nkeys, 
         const size_t cb = key_bytes(kt);
         const uint64_t cap = (out->capacity == 0 || out->capacity > n) ? n : out->capacity;
         const uint64_t hr = static_cast<uint64_t>(h->tune.hist_rows);
-        const size_t gb = groupby_batch_scratch(kt, &n, 1, hr);
+        const size_t gb = groupby_batch_scratch(kt, &n, 1, hr, groupby_carry_bytes(h->tune, vts.data(), nvals));
         void* scr = h->ensure_scratch(align_up(n * cb, 256) + align_up(std::max<uint64_t>(cap, 1) * cb, 256) + gb +
                                       512);
         Carve cv(scr);
@@ -630,6 +634,7 @@ static int* tune_slot(Tuning& t, const std::string& n)
     if (n == "hist_rows") return &t.hist_rows;
     if (n == "gather_rows") return &t.gather_rows;
     if (n == "gather_nt") return &t.gather_nt;
+    if (n == "groupby_carry") return &t.gb_carry;
     return nullptr;
 }
 

@@ -47,6 +47,7 @@ struct K16 {
     uint64_t lo, hi;
 };
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
 
 // key loads / stores through global-address-space pointers
 __device__ inline uint32_t get_key(const uint32_t* p, uint64_t i) { return gmem(p)[i]; }
@@ -193,7 +194,8 @@ struct GbSeg {
     uint32_t gtile_off;      // first group-by tile
     uint32_t ntiles;         // group-by tiles of the table
     uint64_t cap;            // groups the outputs hold: ids >= cap are not written
-    const void* vals[VDF_MAX_VALUE_COLS];
+    const void* vals[VDF_MAX_VALUE_COLS];   // caller's columns (row order), gathered through perm
+    const void* svals[VDF_MAX_VALUE_COLS];  // columns already in key order (null: gather)
     vdf_groupby_out out;
 };
 
@@ -341,6 +343,23 @@ __device__ inline void load_rows_u32(const uint32_t* p, uint32_t cnt, uint32_t (
     }
 }
 
+// 16 consecutive 8-byte values of a lane (rows past cnt read as 0)
+__device__ inline void load_rows_u64(const uint64_t* p, uint32_t cnt, uint64_t (&o)[kGbRows])
+{
+    if (cnt == kGbRows && (reinterpret_cast<uintptr_t>(p) & 15u) == 0) {
+        const auto* q = gmem(reinterpret_cast<const u64x2*>(p));
+#pragma unroll
+        for (int i = 0; i < kGbRows / 2; ++i) {
+            const u64x2 t = q[i];
+            o[2 * i + 0] = t.x;
+            o[2 * i + 1] = t.y;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < kGbRows; ++j) o[j] = (static_cast<uint32_t>(j) < cnt) ? gmem(p)[j] : 0ull;
+    }
+}
+
 // Head bits of this lane's rows (bit j: row r0 + j starts a group).  Rows are
 // read in 4-row chunks; a chunk is one or more 16-byte vector loads when the
 // lane has all 16 rows (table starts are 16-row aligned in scratch).
@@ -395,7 +414,25 @@ __device__ __attribute__((always_inline)) inline void gb_column(const GbArgs& a,
         // not empty) and are never aggregated.
         const void* vd = sg.vals[c];
         const int type = a.cols[c].type;
-        if (type == VDF_VAL_I32 || type == VDF_VAL_U32) {
+        const bool w4 = type == VDF_VAL_I32 || type == VDF_VAL_U32;
+        if (sg.svals[c]) {
+            // the column is in key order: the lane's 16 rows are consecutive
+            // (16-byte vector loads; table starts are 16-row aligned)
+            if (w4) {
+                uint32_t w[kGbRows];
+                load_rows_u32(static_cast<const uint32_t*>(sg.svals[c]) + r0, cnt, w);
+                if (type == VDF_VAL_I32) {
+#pragma unroll
+                    for (int j = 0; j < kGbRows; ++j)
+                        v[j] = static_cast<uint64_t>(static_cast<int64_t>(static_cast<int32_t>(w[j])));
+                } else {
+#pragma unroll
+                    for (int j = 0; j < kGbRows; ++j) v[j] = w[j];
+                }
+            } else {
+                load_rows_u64(static_cast<const uint64_t*>(sg.svals[c]) + r0, cnt, v);
+            }
+        } else if (w4) {
             const auto* q = gmem(static_cast<const uint32_t*>(vd));
             uint32_t w[kGbRows];
 #pragma unroll
@@ -679,18 +716,19 @@ size_t agg_bytes_for(uint64_t tiles, int ntables)
     return b + 256;
 }
 
-GbLayout gb_layout(int key_type, const uint64_t* n, int ntables, uint64_t hist_rows)
+GbLayout gb_layout(int key_type, const uint64_t* n, int ntables, uint64_t hist_rows, int pay_bytes)
 {
     GbLayout L;
     for (int i = 0; i < ntables; ++i) {
         L.rows += align_up(n[i], kGbRows);  // table starts 16-row aligned (vector loads)
         L.tiles += (n[i] + kGbTile - 1) / kGbTile;
     }
-    L.sort_bytes = align_up(argsort_batch_scratch(key_type, n, ntables, hist_rows), 256);
+    L.sort_bytes = align_up(argsort_batch_scratch(key_type, n, ntables, hist_rows, pay_bytes), 256);
     L.agg_bytes = agg_bytes_for(L.tiles, ntables);
     size_t b = L.sort_bytes;
-    b += align_up(L.rows * key_bytes(key_type), 256);  // sorted keys
-    b += align_up(L.rows * 4, 256);                    // perms the caller does not take
+    b += align_up(L.rows * key_bytes(key_type), 256);                   // sorted keys
+    b += align_up(L.rows * 4, 256);                                     // perms the caller does not take
+    b += align_up(L.rows * static_cast<uint64_t>(pay_bytes), 256);      // carried columns in key order
     b += L.agg_bytes;
     L.bytes = b + 256;
     return L;
@@ -698,11 +736,25 @@ GbLayout gb_layout(int key_type, const uint64_t* n, int ntables, uint64_t hist_r
 
 }  // namespace
 
-size_t groupby_scratch(int key_type, uint64_t n) { return gb_layout(key_type, &n, 1, kDefaultHistRows).bytes; }
-
-size_t groupby_batch_scratch(int key_type, const uint64_t* n, int ntables, uint64_t hist_rows)
+// Value columns a group-by carries through its sort into key order (the
+// aggregate kernel then reads them sequentially instead of gathering them
+// through the permutation, a random 8-byte read per row and column that costs
+// a whole memory line): column 0 when it is 4 or 8 bytes, columns 0 and 1 when
+// both are 8 bytes.  Returns the payload bytes per row (0: gather everything).
+int groupby_carry_bytes(const Tuning& t, const int* vt, int nv)
 {
-    return gb_layout(key_type, n, ntables, hist_rows).bytes;
+    if (!t.gb_carry || nv <= 0) return 0;
+    const int w0 = val_bytes(vt[0]);
+    if (nv >= 2 && w0 == 8 && val_bytes(vt[1]) == 8) return 16;
+    return w0;
+}
+
+// (the budget query assumes the largest payload)
+size_t groupby_scratch(int key_type, uint64_t n) { return gb_layout(key_type, &n, 1, kDefaultHistRows, 16).bytes; }
+
+size_t groupby_batch_scratch(int key_type, const uint64_t* n, int ntables, uint64_t hist_rows, int pay_bytes)
+{
+    return gb_layout(key_type, n, ntables, hist_rows, pay_bytes).bytes;
 }
 
 size_t groupby_agg_staging(const uint64_t* n, int ntables)
@@ -729,7 +781,8 @@ size_t groupby_batch_staging(int key_type, const uint64_t* n, int ntables, uint6
 // perms[i]; scratch at `scr` (groupby_agg_scratch bytes).
 void groupby_aggregate_enqueue(Handle* h, int key_type, const void* skeys, const uint32_t* const* perms,
                                const GbTable* tables, int ntables, const int* val_types, int nvals, uint32_t agg_mask,
-                               const vdf_groupby_out* outs, void* scr, hipStream_t s)
+                               const vdf_groupby_out* outs, void* scr, hipStream_t s, const void* spay,
+                               int pay_bytes)
 {
     const size_t kb = key_bytes(key_type);
     uint64_t tiles_total = 0, rows_total = 0;
@@ -764,6 +817,14 @@ void groupby_aggregate_enqueue(Handle* h, int key_type, const void* skeys, const
         g.ntiles = static_cast<uint32_t>(nt);
         g.cap = (outs[i].capacity == 0 || outs[i].capacity > n) ? n : outs[i].capacity;
         for (int c = 0; c < nvals; ++c) g.vals[c] = tables[i].vals[c];
+        if (spay != nullptr && pay_bytes != 0) {
+            // carried columns: 0 (4 or 8 bytes per row), and 1 in the upper
+            // half of a 16-byte payload
+            const char* base = static_cast<const char*>(spay);
+            const uint64_t w0 = pay_bytes == 16 ? 8 : static_cast<uint64_t>(pay_bytes);
+            g.svals[0] = base + row * w0;
+            if (pay_bytes == 16) g.svals[1] = base + rows_total * 8 + row * 8;
+        }
         g.out = outs[i];
         hs[i] = g;
         for (uint64_t t = 0; t < nt; ++t) ht[tiles + t] = static_cast<uint32_t>(i);
@@ -797,9 +858,11 @@ void groupby_aggregate_enqueue(Handle* h, int key_type, const void* skeys, const
         a.cols[0] = GbColType{VDF_VAL_U32, KIND_COUNT};
     } else {
         a.ncols = nvals;
+        const int ncarried = pay_bytes == 16 ? 2 : (pay_bytes != 0 ? 1 : 0);
         for (int c = 0; c < nvals; ++c) {
             a.cols[c] = GbColType{val_types[c], kind_of(val_types[c])};
-            val_bytes += 4 + ((val_types[c] == VDF_VAL_I32 || val_types[c] == VDF_VAL_U32) ? 4 : 8);
+            // a carried column is read in order; a gathered one costs its row id too
+            val_bytes += (c < ncarried ? 0 : 4) + ((val_types[c] == VDF_VAL_I32 || val_types[c] == VDF_VAL_U32) ? 4 : 8);
         }
     }
     {
@@ -807,7 +870,7 @@ void groupby_aggregate_enqueue(Handle* h, int key_type, const void* skeys, const
         fill_u32(status, tiles + 1, 0u, s);
     }
     {
-        // algorithmic bytes: sorted keys (+ the previous key) + per column row id and value
+        // algorithmic bytes: sorted keys + per column its value (and row id when gathered)
         LaunchTimer t(h, s, "groupby_tile", rows_total * (kb + val_bytes));
         switch (kb) {
         case 4: hipLaunchKernelGGL(k_gb_tile<uint32_t>, dim3(a.ntiles), dim3(kBlock), 0, s, a); break;
@@ -829,29 +892,42 @@ void groupby_batch_enqueue(Handle* h, int key_type, const GbTable* tables, int n
     std::vector<uint64_t> ns(ntables);
     for (int i = 0; i < ntables; ++i) ns[i] = tables[i].n;
     const uint64_t hist_rows = static_cast<uint64_t>(h->tune.hist_rows);
-    const GbLayout L = gb_layout(key_type, ns.data(), ntables, hist_rows);
+    const int pay = groupby_carry_bytes(h->tune, val_types, nvals);
+    const GbLayout L = gb_layout(key_type, ns.data(), ntables, hist_rows, pay);
     const size_t kb = key_bytes(key_type);
     Carve cv(scr);
     void* sort_scr = cv.take<char>(L.sort_bytes);
     char* skeys = cv.take<char>(L.rows * kb);
     uint32_t* perm_scr = cv.take<uint32_t>(L.rows);
+    char* spay = cv.take<char>(L.rows * static_cast<uint64_t>(pay));
     void* agg_scr = cv.take<char>(L.agg_bytes);
 
-    // 1. sort every table: sorted keys into scratch, perms to the caller or scratch
+    // 1. sort every table: sorted keys into scratch, perms to the caller or
+    //    scratch, carried value columns into key order (16-byte payload: the
+    //    second column in the upper half of spay)
     std::vector<SortSeg> ss(ntables);
     std::vector<const uint32_t*> perms(ntables);
+    const uint64_t w0 = pay == 16 ? 8 : static_cast<uint64_t>(pay);
     uint64_t row = 0;
     for (int i = 0; i < ntables; ++i) {
         uint32_t* p = outs[i].perm ? outs[i].perm : perm_scr + row;
         perms[i] = p;
         ss[i] = SortSeg{tables[i].keys, tables[i].n, p, skeys + row * kb};
+        if (pay != 0) {
+            ss[i].pay_in = tables[i].vals[0];
+            ss[i].pay_out = spay + row * w0;
+        }
+        if (pay == 16) {
+            ss[i].pay_in2 = tables[i].vals[1];
+            ss[i].pay_out2 = spay + L.rows * 8 + row * 8;
+        }
         row += align_up(tables[i].n, kGbRows);
     }
-    argsort_batch_enqueue(h, key_type, ss.data(), ntables, 0, sort_scr, s);
+    argsort_batch_enqueue(h, key_type, ss.data(), ntables, 0, sort_scr, s, pay);
 
     // 2. boundaries and aggregates
     groupby_aggregate_enqueue(h, key_type, skeys, perms.data(), tables, ntables, val_types, nvals, agg_mask, outs,
-                              agg_scr, s);
+                              agg_scr, s, pay != 0 ? spay : nullptr, pay);
 }
 
 void groupby_enqueue(Handle* h, int key_type, const void* d_keys, uint64_t n, const vdf_value_col* vals,

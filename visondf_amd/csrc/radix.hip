@@ -196,6 +196,8 @@ struct SegDesc {
     void* keys_out;     // the segment's sorted keys (nullable)
     const void* pay_in; // payload column (batches with a payload)
     void* pay_out;      // payload in sorted order
+    const void* pay_in2;  // second 8-byte payload column (16-byte payloads)
+    void* pay_out2;
     uint64_t n;         // rows
     uint64_t row_off;   // first row in the concatenated scratch buffers
     uint32_t tile_off;  // first pass tile of the segment
@@ -224,6 +226,7 @@ struct SortArgs {
     void* kbuf[2];             // transformed keys between passes (concatenated)
     uint32_t* vbuf[2];         // local row ids between passes (concatenated)
     void* pbuf[2];             // payload between passes (concatenated)
+    void* pbuf2[2];            // second payload column between passes (16-byte payloads)
     uint32_t* lb[2];           // look-back words [tiles][256], alternating by slot parity
     uint32_t* tickets;         // [kMaxDigits][kTicketStride] tile-claim counters
     uint64_t mlo, mhi;         // key transform masks
@@ -384,6 +387,8 @@ struct TileCtx {
     void* keys_out;
     const void* pay_in;
     void* pay_out;
+    const void* pay_in2;
+    void* pay_out2;
     const uint32_t* base;  // segment's bucket starts for this digit
 };
 
@@ -423,24 +428,28 @@ struct TileCtx {
 template <typename K, int ITEMS, int PB>
 struct PassShape {
     static constexpr int kLds = kBlock * ITEMS * static_cast<int>(sizeof(K) + 4 + PB) + kWaves * kRadix * 4 + kRadix * 4 + 64;
-    static constexpr int kCap = ITEMS >= 24 ? 2 : 3;  // register room for the larger tiles
+    static constexpr int kCap = (ITEMS >= 24 || PB == 16) ? 2 : 3;  // register room for the larger tiles
     static constexpr int kBlocksPerCu = (160 * 1024 / kLds) < kCap ? (160 * 1024 / kLds) : kCap;
 };
 
-// PB: payload bytes per row (0, 4 or 8) moved with the keys.
+// PB: payload bytes per row moved with the keys: 0, 4, 8, or 16 (two 8-byte
+// columns).
 template <typename K, int ITEMS, int PB>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PassShape<K, ITEMS, PB>::kBlocksPerCu)))
 void k_pass(SortArgs a, uint32_t slot)
 {
     using KO = KeyOps<K>;
-    using P = typename std::conditional<PB == 8, uint64_t, uint32_t>::type;
-    constexpr int PI = PB ? ITEMS : 1;  // payload registers
+    using P = typename std::conditional<PB >= 8, uint64_t, uint32_t>::type;
+    constexpr bool TWO = PB == 16;        // a second 8-byte payload column
+    constexpr int PI = PB ? ITEMS : 1;    // payload registers
+    constexpr int PI2 = TWO ? ITEMS : 1;
     constexpr int TILE = kBlock * ITEMS;
     constexpr int D = KO::kDigits;
 
     __shared__ K s_keys[TILE];
     __shared__ uint32_t s_vals[TILE];
     __shared__ P s_pay[PB ? TILE : 1];
+    __shared__ uint64_t s_pay2[TWO ? TILE : 1];
     __shared__ uint32_t s_wc[kWaves][kRadix];
     __shared__ uint32_t s_gbase[kRadix];
     __shared__ uint32_t s_wsum[kWaves];
@@ -454,6 +463,7 @@ void k_pass(SortArgs a, uint32_t slot)
     K key[ITEMS];
     uint32_t val[ITEMS];
     P pay[PI];
+    uint64_t pay2[PI2];
     uint32_t rnk[ITEMS];
 
     auto claim = [&]() -> uint32_t {
@@ -492,6 +502,8 @@ void k_pass(SortArgs a, uint32_t slot)
             c.keys_out = sd.keys_out;
             c.pay_in = sd.pay_in;
             c.pay_out = sd.pay_out;
+            c.pay_in2 = sd.pay_in2;
+            c.pay_out2 = sd.pay_out2;
             c.base = a.plan_base + (static_cast<uint64_t>(sg) * D + c.dig) * kRadix;
             return true;
         }
@@ -521,6 +533,17 @@ void k_pass(SortArgs a, uint32_t slot)
             } else {
 #pragma unroll
                 for (int j = 0; j < PI; ++j) pay[j] = pp[std::min(wl + static_cast<uint64_t>(j) * 64, last)];
+            }
+        }
+        if constexpr (TWO) {
+            const auto* pp = c.first ? gmem(static_cast<const uint64_t*>(c.pay_in2))
+                                     : gmem(static_cast<const uint64_t*>(a.pbuf2[(slot - 1) & 1]) + c.row_off);
+            if (full) {
+#pragma unroll
+                for (int j = 0; j < PI2; ++j) pay2[j] = pp[wl + static_cast<uint64_t>(j) * 64];
+            } else {
+#pragma unroll
+                for (int j = 0; j < PI2; ++j) pay2[j] = pp[std::min(wl + static_cast<uint64_t>(j) * 64, last)];
             }
         }
         if (c.first) {
@@ -609,6 +632,7 @@ void k_pass(SortArgs a, uint32_t slot)
                 s_keys[p] = key[j];
                 s_vals[p] = val[j];
                 if constexpr (PB != 0) s_pay[p] = pay[j];
+                if constexpr (TWO) s_pay2[p] = pay2[j];
             }
         }
         lds_barrier();
@@ -662,10 +686,12 @@ void k_pass(SortArgs a, uint32_t slot)
             K* kout = static_cast<K*>(a.kbuf[slot & 1]) + c.row_off;
             auto* vout = gmem(a.vbuf[slot & 1] + c.row_off);
             auto* pout = gmem(static_cast<P*>(a.pbuf[slot & 1]) + c.row_off);
+            auto* pout2 = gmem(static_cast<uint64_t*>(a.pbuf2[slot & 1]) + c.row_off);
             for (uint32_t i0 = tid; i0 < tn; i0 += U * kBlock) {
                 K kk[U];
                 uint32_t pos[U], vv[U];
                 P pv[U];
+                uint64_t pv2[U];
 #pragma unroll
                 for (int u = 0; u < U; ++u) kk[u] = s_keys[i0 + u * kBlock];
 #pragma unroll
@@ -673,6 +699,7 @@ void k_pass(SortArgs a, uint32_t slot)
                     pos[u] = s_gbase[KO::digit(kk[u], dig)] + i0 + u * kBlock;
                     vv[u] = s_vals[i0 + u * kBlock];
                     if constexpr (PB != 0) pv[u] = s_pay[i0 + u * kBlock];
+                    if constexpr (TWO) pv2[u] = s_pay2[i0 + u * kBlock];
                 }
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
@@ -680,6 +707,7 @@ void k_pass(SortArgs a, uint32_t slot)
                         KO::put(kout, pos[u], kk[u]);
                         vout[pos[u]] = vv[u];
                         if constexpr (PB != 0) pout[pos[u]] = pv[u];
+                        if constexpr (TWO) pout2[pos[u]] = pv2[u];
                     }
                 }
             }
@@ -689,6 +717,7 @@ void k_pass(SortArgs a, uint32_t slot)
                 K kk[U];
                 uint32_t pos[U], vv[U];
                 P pv[U];
+                uint64_t pv2[U];
 #pragma unroll
                 for (int u = 0; u < U; ++u) kk[u] = s_keys[i0 + u * kBlock];
 #pragma unroll
@@ -697,6 +726,7 @@ void k_pass(SortArgs a, uint32_t slot)
                     pos[u] = a.reverse ? nm1 - p : p;
                     vv[u] = s_vals[i0 + u * kBlock];
                     if constexpr (PB != 0) pv[u] = s_pay[i0 + u * kBlock];
+                    if constexpr (TWO) pv2[u] = s_pay2[i0 + u * kBlock];
                 }
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
@@ -710,6 +740,7 @@ void k_pass(SortArgs a, uint32_t slot)
                                 gmem(static_cast<uint32_t*>(c.perm_out))[p] = vv[u];
                         }
                         if constexpr (PB != 0) gmem(static_cast<P*>(c.pay_out))[p] = pv[u];
+                        if constexpr (TWO) gmem(static_cast<uint64_t*>(c.pay_out2))[p] = pv2[u];
                     }
                 }
             }
@@ -844,6 +875,10 @@ PassFn pick_pass<uint32_t>(int items, int pay_bytes)
         if (items == 12) return k_pass<uint32_t, 12, 8>;
         if (items == 16) return k_pass<uint32_t, 16, 8>;
         return nullptr;
+    case 16:
+        if (items == 8) return k_pass<uint32_t, 8, 16>;
+        if (items == 12) return k_pass<uint32_t, 12, 16>;
+        return nullptr;
     default: return nullptr;
     }
 }
@@ -865,6 +900,9 @@ PassFn pick_pass<uint64_t>(int items, int pay_bytes)
         if (items == 8) return k_pass<uint64_t, 8, 8>;
         if (items == 12) return k_pass<uint64_t, 12, 8>;
         return nullptr;
+    case 16:
+        if (items == 8) return k_pass<uint64_t, 8, 16>;
+        return nullptr;
     default: return nullptr;
     }
 }
@@ -877,6 +915,7 @@ PassFn pick_pass<K128>(int items, int pay_bytes)
     case 0: return k_pass<K128, 8, 0>;
     case 4: return k_pass<K128, 8, 4>;
     case 8: return k_pass<K128, 8, 8>;
+    case 16: return k_pass<K128, 8, 16>;
     default: return nullptr;
     }
 }
@@ -892,6 +931,7 @@ bool tune_supported(const char* name, int64_t v)
     if (n == "hist_rows") return v >= 4096 && v <= (1 << 24);
     if (n == "gather_rows") return v == 4 || v == 8 || v == 16;
     if (n == "gather_nt") return v == 0 || v == 1;
+    if (n == "groupby_carry") return v == 0 || v == 1;
     return false;
 }
 
@@ -938,10 +978,10 @@ static void launch_sort(Handle* h, const SortSeg* segs, int nsegs, uint64_t mlo,
     int items = sizeof(K) == 4 ? h->tune.sort_items_32 : (sizeof(K) == 8 ? h->tune.sort_items_64 : 8);
     PassFn pass = pick_pass<K>(items, pay_bytes);
     if (!pass) {  // tuning value without an instantiation for this key width / payload
-        items = sizeof(K) == 4 ? (pay_bytes == 4 ? 12 : 16) : 8;
+        items = sizeof(K) == 4 ? (pay_bytes == 4 || pay_bytes == 16 ? 12 : 16) : 8;
         pass = pick_pass<K>(items, pay_bytes);
     }
-    require(pass != nullptr, "sort payload must be 0, 4 or 8 bytes per row");
+    require(pass != nullptr, "sort payload must be 0, 4, 8 or 16 bytes per row");
     const uint64_t tile_rows = static_cast<uint64_t>(kBlock) * items;
     const uint64_t hist_rows = static_cast<uint64_t>(h->tune.hist_rows);
     std::vector<uint64_t> ns(nsegs);
@@ -959,8 +999,9 @@ static void launch_sort(Handle* h, const SortSeg* segs, int nsegs, uint64_t mlo,
     for (int i = 0; i < nsegs; ++i) {
         const uint64_t n = segs[i].n;
         const uint64_t nt = (n + tile_rows - 1) / tile_rows;
-        hs[i] = SegDesc{segs[i].keys, segs[i].perm_out, segs[i].keys_out, segs[i].pay_in, segs[i].pay_out,
-                        n, row, static_cast<uint32_t>(tiles), 0};
+        hs[i] = SegDesc{segs[i].keys,    segs[i].perm_out, segs[i].keys_out, segs[i].pay_in, segs[i].pay_out,
+                        segs[i].pay_in2, segs[i].pay_out2, n,             row,            static_cast<uint32_t>(tiles),
+                        0};
         for (uint64_t t = 0; t < nt; ++t) ht[tiles + t] = static_cast<uint32_t>(i);
         for (uint64_t b = 0; b < n; b += hist_rows) hi[nitems++] = HistItem{static_cast<uint32_t>(i), 0, b, std::min(n, b + hist_rows)};
         row += n;
@@ -987,6 +1028,9 @@ static void launch_sort(Handle* h, const SortSeg* segs, int nsegs, uint64_t mlo,
     a.vbuf[1] = c.take<uint32_t>(L.rows);
     a.pbuf[0] = c.take<char>(L.rows * static_cast<uint64_t>(pay_bytes));
     a.pbuf[1] = c.take<char>(L.rows * static_cast<uint64_t>(pay_bytes));
+    // a 16-byte payload is two 8-byte columns: the second in the upper half
+    a.pbuf2[0] = static_cast<char*>(a.pbuf[0]) + (pay_bytes == 16 ? L.rows * 8 : 0);
+    a.pbuf2[1] = static_cast<char*>(a.pbuf[1]) + (pay_bytes == 16 ? L.rows * 8 : 0);
     const size_t up = reinterpret_cast<char*>(ht + ntiles) - stg;
     VDF_HIP_CHECK(hipMemcpyAsync(dsegs, stg, up, hipMemcpyHostToDevice, s));  // descs, items, tile map
     // the host carve mirrors the device carve (same order, same alignment)

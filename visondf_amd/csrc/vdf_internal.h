@@ -92,6 +92,8 @@ struct SortSeg {
     void* keys_out;   // nullable
     const void* pay_in = nullptr;  // payload column carried through the passes (batch pay_bytes per row)
     void* pay_out = nullptr;       // the payload in sorted order
+    const void* pay_in2 = nullptr; // second 8-byte payload column (pay_bytes 16)
+    void* pay_out2 = nullptr;
 };
 
 // ---------------------------------------------------------------------------
@@ -132,6 +134,7 @@ struct Tuning {
     int hist_rows = 65536;   // rows per histogram work item
     int gather_rows = 8;     // rows per thread of the gather kernel
     int gather_nt = 0;       // non-temporal loads of the gathered column
+    int gb_carry = 1;        // group-by: value columns carried through the sort (1) or gathered (0)
 };
 
 bool tune_supported(const char* name, int64_t value);
@@ -259,8 +262,9 @@ size_t argsort_batch_staging(int key_type, const uint64_t* n, int nsegs, uint64_
 void argsort_enqueue(Handle* h, int key_type, const void* d_keys, uint64_t n, uint32_t flags,
                      void* d_perm_out, void* d_keys_out, void* scr, hipStream_t s);
 // Sorts every segment of a batch (argsort_batch_scratch bytes of scratch).
-// pay_bytes (0, 4 or 8): every segment's payload column (pay_in -> pay_out)
-// moves with its keys through the passes.
+// pay_bytes (0, 4, 8 or 16): every segment's payload column (pay_in ->
+// pay_out; with 16, two 8-byte columns, the second pay_in2 -> pay_out2) moves
+// with its keys through the passes.
 void argsort_batch_enqueue(Handle* h, int key_type, const SortSeg* segs, int nsegs, uint32_t flags, void* scr,
                            hipStream_t s, int pay_bytes = 0);
 
@@ -279,7 +283,10 @@ struct GbTable {
 };
 
 size_t groupby_scratch(int key_type, uint64_t n);
-size_t groupby_batch_scratch(int key_type, const uint64_t* n, int ntables, uint64_t hist_rows);
+// Payload bytes per row of the value columns a group-by carries through its
+// sort (see groupby.hip); 0 when every column is gathered.
+int groupby_carry_bytes(const Tuning& t, const int* val_types, int nvals);
+size_t groupby_batch_scratch(int key_type, const uint64_t* n, int ntables, uint64_t hist_rows, int pay_bytes);
 size_t groupby_batch_staging(int key_type, const uint64_t* n, int ntables, uint64_t hist_rows);
 void groupby_enqueue(Handle* h, int key_type, const void* d_keys, uint64_t n,
                      const vdf_value_col* vals, int nvals, uint32_t agg_mask,
@@ -290,9 +297,12 @@ void groupby_batch_enqueue(Handle* h, int key_type, const GbTable* tables, int n
 // 16-row aligned table starts, permutations perms[i])
 size_t groupby_agg_scratch(const uint64_t* n, int ntables);
 size_t groupby_agg_staging(const uint64_t* n, int ntables);
+// spay/pay_bytes: value columns 0 (and 1 with 16 bytes) already in key order
+// (as the sort that carried them wrote them), else null/0.
 void groupby_aggregate_enqueue(Handle* h, int key_type, const void* skeys, const uint32_t* const* perms,
                                const GbTable* tables, int ntables, const int* val_types, int nvals, uint32_t agg_mask,
-                               const vdf_groupby_out* outs, void* scr, hipStream_t s);
+                               const vdf_groupby_out* outs, void* scr, hipStream_t s, const void* spay = nullptr,
+                               int pay_bytes = 0);
 
 // multi-column keys: composite key type (-1 when wider than 16 bytes), pack
 // the columns into composites, unpack composite group keys into the columns
