@@ -186,15 +186,19 @@ def run_argsort_u32(args) -> dict:
     assert torch.equal(out, d_val[p64]), "value column does not follow the permutation"
 
     # the same result through two calls (argsort, then a random-access gather)
-    perm2 = torch.empty_like(perm)
-    out2 = torch.empty_like(out)
+    two = None
+    if not args.skip_compare:
+        perm2 = torch.empty_like(perm)
+        out2 = torch.empty_like(out)
 
-    def step_two_calls():
-        v.argsort(d_keys, key_type=N.KEY_U32, perm_out=perm2)
-        v.gather(d_val, perm2, out=out2)
+        def step_two_calls():
+            v.argsort(d_keys, key_type=N.KEY_U32, perm_out=perm2)
+            v.gather(d_val, perm2, out=out2)
 
-    ms2, kernels2 = _time_steps(step_two_calls, args.steps, args.warmup, v)
-    assert torch.equal(perm2, perm) and torch.equal(out2, out), "argsort + gather differs from sort_by"
+        ms2, kernels2 = _time_steps(step_two_calls, args.steps, args.warmup, v)
+        assert torch.equal(perm2, perm) and torch.equal(out2, out), "argsort + gather differs from sort_by"
+        two = {"method": "vdf_argsort + vdf_gather", "ms_per_step": ms2 / args.steps,
+               "rows_per_s": n * args.steps / (ms2 * 1e-3), "kernels": kernels2}
 
     rec = _base_record(args, rows_per_s, "rows/s", ms_step, "u32",
                        {"workload": "argsort_u32_gather_f64", "rows": n, "key": "uint32 uniform over 2^32",
@@ -205,8 +209,7 @@ def run_argsort_u32(args) -> dict:
     rec["hbm_pct_peak_step"] = 100.0 * rec["hbm_gbps_step"] / HBM_PEAK_GBPS
     rec["roofline"] = _roofline("argsort_u32", kernels, "radix_pass")
     rec["kernels"] = kernels
-    rec["two_calls"] = {"method": "vdf_argsort + vdf_gather", "ms_per_step": ms2 / args.steps,
-                        "rows_per_s": n * args.steps / (ms2 * 1e-3), "kernels": kernels2}
+    rec["two_calls"] = two
     rec["autotune"] = tune
     if not args.no_cpu:
         rec["cpu_baseline"] = cpu_baseline_argsort(keys, val, _threads(args))
@@ -526,7 +529,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner (argsort_u32, batched)")
     ap.add_argument("--skip-compare", action="store_true",
-                    help="batched: time the batched launches only (no per-table calls, no graph replay)")
+                    help="time the workload's own step only: argsort_u32 without the two-call (argsort + "
+                         "gather) comparison, batched without per-table calls and graph replay")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--calibrate", action="store_true",
                     help="first run one plain 1 GiB device copy (the known-size kernel that scales "
