@@ -182,7 +182,7 @@ def test_groupby_deterministic_repeat(vdf):
 def _outputs(r, vals):
     """Every output of a group-by result as bytes (for bit-identity checks)."""
     out = [r.keys.cpu().numpy().tobytes(), r.offsets.cpu().numpy().tobytes(), r.counts.cpu().numpy().tobytes(),
-           r.perm.cpu().numpy().tobytes()]
+           r.perm.cpu().numpy().tobytes() if r.perm is not None else b""]
     for c in range(len(vals)):
         for k in ("sum", "mean", "min", "max"):
             out.append(r.aggs[c][k].cpu().numpy().tobytes())
@@ -215,3 +215,29 @@ def test_groupby_carried_columns_bit_identical(vdf, combo):
     finally:
         vdf.tune("groupby_carry", 1)
     assert all(x == y for x, y in zip(a, b)), combo
+
+
+@pytest.mark.parametrize("combo", ["", "f64", "i32", "f64+i64", "f64+u64+i32"])
+def test_groupby_without_permutation_identical(vdf, combo):
+    """A group-by whose caller takes no permutation (and whose value columns are
+    all carried, or which counts only) sorts without row ids; every other output
+    is bit-identical to the call that takes the permutation."""
+    rng = np.random.default_rng(43)
+    n = 200_003
+    keys = datagen.scramble24(datagen.zipf_ranks(6, 0, n, 1.1, 1 << 14)).astype(np.int32)
+    make = {"f64": lambda: rng.standard_normal(n),
+            "i32": lambda: rng.integers(-(2**31), 2**31, n, dtype=np.int64).astype(np.int32),
+            "i64": lambda: rng.integers(-(2**62), 2**62, n, dtype=np.int64),
+            "u64": lambda: rng.integers(0, 2**63, n, dtype=np.uint64)}
+    vals = [make[t]() for t in combo.split("+") if t]
+    kt = oracle.key_type_of(keys)
+    vts = [oracle.val_type_of(v) for v in vals]
+    d_keys, d_vals = _dev(keys), [_dev(v) for v in vals]
+    with_p = vdf.groupby(d_keys, d_vals, key_type=kt, val_types=vts, want_perm=True)
+    without = vdf.groupby(d_keys, d_vals, key_type=kt, val_types=vts, want_perm=False)
+    assert with_p.n_groups == without.n_groups
+    a, b = _outputs(with_p, vals), _outputs(without, vals)
+    del a[3], b[3]  # the permutation itself
+    assert all(x == y for x, y in zip(a, b)), combo
+    o = oracle.groupby(keys, vals, threads=8)
+    assert np.array_equal(_host(without.counts, np.uint32).astype(np.uint64), o["counts"])

@@ -97,6 +97,22 @@ def test_plan_replays_equal_eager(vdf):
     pb.release_plan()
 
 
+def test_plan_without_permutation_equals_eager(vdf):
+    """A plan whose outputs take no permutation replays equal to the eager call
+    without one (the eager call may skip row ids; the plan always moves them)."""
+    import torch
+
+    tabs = [(_dev(k), [_dev(x) for x in vs]) for k, vs in _batch(4, SIZES)]
+    eager = vdf.groupby_batch(tabs, key_type=N.KEY_I32, want_perm=False)
+    pb = vdf.prepare_groupby_batch(tabs, key_type=N.KEY_I32, want_perm=False).capture()
+    for rep in range(2):
+        pb.replay()
+        torch.cuda.synchronize()
+        for i, e in enumerate(eager):
+            _same(pb.result(i), e, f"no-perm replay {rep} table {i}")
+    pb.release_plan()
+
+
 def test_plan_rejects_other_shape(vdf):
     import ctypes as C
 

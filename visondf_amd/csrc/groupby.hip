@@ -407,11 +407,9 @@ __device__ __attribute__((always_inline)) inline void gb_column(const GbArgs& a,
     const ColOut co = col_out(a, sg, c);
     uint64_t v[kGbRows];
     if (KIND != KIND_COUNT) {
-        uint32_t p[kGbRows];
-        load_rows_u32(sg.perm + r0, cnt, p);
-        // All 16 gathers are issued before any conversion waits on one.
-        // Rows past the tile's end hold row id 0 (in bounds: the table is
-        // not empty) and are never aggregated.
+        // A gathered column: all 16 gathers are issued before any conversion
+        // waits on one.  Rows past the tile's end hold row id 0 (in bounds:
+        // the table is not empty) and are never aggregated.
         const void* vd = sg.vals[c];
         const int type = a.cols[c].type;
         const bool w4 = type == VDF_VAL_I32 || type == VDF_VAL_U32;
@@ -433,6 +431,8 @@ __device__ __attribute__((always_inline)) inline void gb_column(const GbArgs& a,
                 load_rows_u64(static_cast<const uint64_t*>(sg.svals[c]) + r0, cnt, v);
             }
         } else if (w4) {
+            uint32_t p[kGbRows];
+            load_rows_u32(sg.perm + r0, cnt, p);
             const auto* q = gmem(static_cast<const uint32_t*>(vd));
             uint32_t w[kGbRows];
 #pragma unroll
@@ -446,6 +446,8 @@ __device__ __attribute__((always_inline)) inline void gb_column(const GbArgs& a,
                 for (int j = 0; j < kGbRows; ++j) v[j] = w[j];
             }
         } else {
+            uint32_t p[kGbRows];
+            load_rows_u32(sg.perm + r0, cnt, p);
             const auto* q = gmem(static_cast<const uint64_t*>(vd));
 #pragma unroll
             for (int j = 0; j < kGbRows; ++j) v[j] = q[p[j]];
@@ -570,31 +572,51 @@ __global__ __launch_bounds__(kBlock) void k_gb_tile(GbArgs a)
     const uint32_t H = s_wsum[0] + s_wsum[1] + s_wsum[2] + s_wsum[3];
 
     // ---- decoupled look-back over the table's tiles' head counts -> first group id ----
-    if (tid == 0) {
+    // Wave 0 reads the 64 previous tiles' status words at once (one round
+    // trip; they are contiguous), takes the nearest one with an inclusive
+    // prefix (lane L), and adds the aggregates of lanes 0..L.  A window with
+    // an unpublished word before L is read again; a window without an
+    // inclusive prefix is added whole and the next 64 tiles are read.  The
+    // table's first tile publishes its inclusive prefix at once, so the walk
+    // ends there at the latest.
+    if (wave == 0) {
         uint32_t* my = a.status + tile;
-        store_relaxed(my, (lt == 0 ? kFlagP : kFlagA) | H);
+        if (lane == 0) store_relaxed(my, (lt == 0 ? kFlagP : kFlagA) | H);
         uint32_t excl = 0;
         if (lt > 0) {
-            uint32_t k = tile - 1;
+            const int64_t first = static_cast<int64_t>(sg.gtile_off);
+            int64_t k = static_cast<int64_t>(tile) - 1;
             for (;;) {
-                const uint32_t w = load_relaxed(a.status + k);
+                const int64_t idx = k - lane;
+                // lanes before the table's first tile: a zero prefix (never reached
+                // past the first tile's own inclusive prefix)
+                const uint32_t w = idx >= first ? load_relaxed(a.status + idx) : kFlagP;
                 const uint32_t f = w & ~kValMask;
-                if (f == 0u) {
+                const uint64_t pm = __ballot(f == kFlagP);
+                const uint64_t zm = __ballot(f == 0u);
+                const int L = pm ? __ffsll(static_cast<long long>(pm)) - 1 : 64;
+                const uint64_t upto = L >= 63 ? ~0ull : ((2ull << L) - 1);  // lanes 0..L
+                if (zm & upto) {
                     __builtin_amdgcn_s_sleep(1);
                     continue;
                 }
-                excl += w & kValMask;
-                if (f == kFlagP) break;
-                --k;  // the table's first tile holds P
+                uint32_t part = lane <= L ? (w & kValMask) : 0u;
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
+                excl += part;
+                if (L < 64) break;
+                k -= 64;
             }
-            store_relaxed(my, kFlagP | (excl + H));
+            if (lane == 0) store_relaxed(my, kFlagP | (excl + H));
         }
-        s_gbase = excl;
-        a.tile_gbase[tile] = excl;
-        a.tile_heads[tile] = H;
-        if (lt == sg.ntiles - 1) {
-            *gmem(sg.out.n_groups) = static_cast<uint64_t>(excl) + H;
-            if (excl + H <= sg.cap) gmem(sg.out.offsets)[excl + H] = static_cast<uint32_t>(n);
+        if (lane == 0) {
+            s_gbase = excl;
+            a.tile_gbase[tile] = excl;
+            a.tile_heads[tile] = H;
+            if (lt == sg.ntiles - 1) {
+                *gmem(sg.out.n_groups) = static_cast<uint64_t>(excl) + H;
+                if (excl + H <= sg.cap) gmem(sg.out.offsets)[excl + H] = static_cast<uint32_t>(n);
+            }
         }
     }
     __syncthreads();
@@ -908,9 +930,14 @@ void groupby_batch_enqueue(Handle* h, int key_type, const GbTable* tables, int n
     std::vector<SortSeg> ss(ntables);
     std::vector<const uint32_t*> perms(ntables);
     const uint64_t w0 = pay == 16 ? 8 : static_cast<uint64_t>(pay);
+    // The permutation is needed for gathered columns and for callers that
+    // take it; otherwise the passes do not move row ids at all.
+    const int ncarried = pay == 16 ? 2 : (pay != 0 ? 1 : 0);
+    bool rowids = h->keep_rowids || nvals > ncarried;
+    for (int i = 0; i < ntables; ++i) rowids = rowids || outs[i].perm != nullptr;
     uint64_t row = 0;
     for (int i = 0; i < ntables; ++i) {
-        uint32_t* p = outs[i].perm ? outs[i].perm : perm_scr + row;
+        uint32_t* p = outs[i].perm ? outs[i].perm : (rowids ? perm_scr + row : nullptr);
         perms[i] = p;
         ss[i] = SortSeg{tables[i].keys, tables[i].n, p, skeys + row * kb};
         if (pay != 0) {
@@ -923,7 +950,7 @@ void groupby_batch_enqueue(Handle* h, int key_type, const GbTable* tables, int n
         }
         row += align_up(tables[i].n, kGbRows);
     }
-    argsort_batch_enqueue(h, key_type, ss.data(), ntables, 0, sort_scr, s, pay);
+    argsort_batch_enqueue(h, key_type, ss.data(), ntables, 0, sort_scr, s, pay, rowids);
 
     // 2. boundaries and aggregates
     groupby_aggregate_enqueue(h, key_type, skeys, perms.data(), tables, ntables, val_types, nvals, agg_mask, outs,

@@ -143,6 +143,7 @@ vdf_status vdf_plan_groupby_batch(vdf_handle* h, int key_type, const vdf_table* 
         // Radix passes in global tile order with plain launches: a replay may run
         // beside any other work, and this order needs no co-residency.
         p->ph.tune.sort_xcd = 0;
+        p->ph.keep_rowids = true;
         VDF_HIP_CHECK(hipSetDevice(p->ph.device));
         p->key_type = key_type;
         p->nvals = nvals;

@@ -237,6 +237,7 @@ struct SortArgs {
     uint32_t chunk;            // consecutive tiles a group claims at a time
     uint32_t perm64;
     uint32_t reverse;
+    uint32_t norows;           // 1: no segment needs its permutation; row ids are not moved
 };
 
 // Adds one key per lane to the block histogram.  Digits flagged in `shared`
@@ -565,17 +566,22 @@ void k_pass(SortArgs a, uint32_t slot)
             const auto* vp = gmem(vin + c.row_off);
             if (full) {
 #pragma unroll
-                for (int j = 0; j < ITEMS; ++j) {
-                    key[j] = KO::raw(kp, wl + static_cast<uint64_t>(j) * 64);
-                    val[j] = vp[wl + static_cast<uint64_t>(j) * 64];
+                for (int j = 0; j < ITEMS; ++j) key[j] = KO::raw(kp, wl + static_cast<uint64_t>(j) * 64);
+                if (!a.norows) {
+#pragma unroll
+                    for (int j = 0; j < ITEMS; ++j) val[j] = vp[wl + static_cast<uint64_t>(j) * 64];
                 }
             } else {
 #pragma unroll
-                for (int j = 0; j < ITEMS; ++j) {
-                    const uint64_t i = std::min(wl + static_cast<uint64_t>(j) * 64, last);
-                    key[j] = KO::raw(kp, i);
-                    val[j] = vp[i];
+                for (int j = 0; j < ITEMS; ++j) key[j] = KO::raw(kp, std::min(wl + static_cast<uint64_t>(j) * 64, last));
+                if (!a.norows) {
+#pragma unroll
+                    for (int j = 0; j < ITEMS; ++j) val[j] = vp[std::min(wl + static_cast<uint64_t>(j) * 64, last)];
                 }
+            }
+            if (a.norows) {
+#pragma unroll
+                for (int j = 0; j < ITEMS; ++j) val[j] = 0u;
             }
         }
         lds_barrier();
@@ -705,7 +711,7 @@ void k_pass(SortArgs a, uint32_t slot)
                 for (int u = 0; u < U; ++u) {
                     if (i0 + u * kBlock < tn) {
                         KO::put(kout, pos[u], kk[u]);
-                        vout[pos[u]] = vv[u];
+                        if (!a.norows) vout[pos[u]] = vv[u];
                         if constexpr (PB != 0) pout[pos[u]] = pv[u];
                         if constexpr (TWO) pout2[pos[u]] = pv2[u];
                     }
@@ -972,7 +978,7 @@ static unsigned resident_blocks(PassFn fn, int device)
 
 template <typename K>
 static void launch_sort(Handle* h, const SortSeg* segs, int nsegs, uint64_t mlo, uint64_t mhi, uint32_t flags,
-                        void* scr, hipStream_t s, int pay_bytes)
+                        void* scr, hipStream_t s, int pay_bytes, bool rowids)
 {
     constexpr int D = KeyOps<K>::kDigits;
     int items = sizeof(K) == 4 ? h->tune.sort_items_32 : (sizeof(K) == 8 ? h->tune.sort_items_64 : 8);
@@ -1044,6 +1050,7 @@ static void launch_sort(Handle* h, const SortSeg* segs, int nsegs, uint64_t mlo,
     a.ntiles = static_cast<uint32_t>(ntiles);
     a.perm64 = (flags & VDF_SORT_PERM64) ? 1u : 0u;
     a.reverse = (flags & VDF_SORT_DESC_REVERSE) ? 1u : 0u;
+    a.norows = rowids ? 0u : 1u;
     // persistent grid: every resident block (or the tuned blocks per CU), a
     // multiple of the group count
     uint64_t cap = resident_blocks(pass, h->device);
@@ -1084,8 +1091,9 @@ static void launch_sort(Handle* h, const SortSeg* segs, int nsegs, uint64_t mlo,
     // tables' pass counts are copied back (into the handle's pinned ring) so
     // each pass launch is charged the bytes of the tables it worked on.
     std::shared_ptr<PassAcct> acct;
-    const uint64_t pb_out = a.perm64 ? 8 : 4;
+    const uint64_t pb_out = segs[0].perm_out ? (a.perm64 ? 8 : 4) : 0;
     const uint64_t kout = segs[0].keys_out ? kb : 0;
+    const uint64_t rid = rowids ? 4 : 0;  // row ids read / written by a pass
     if (h->prof && h->passes_ring && static_cast<size_t>(nsegs) <= Handle::kRing) {
         if (h->ring_pos + nsegs > Handle::kRing) h->ring_pos = 0;
         uint32_t* dst = h->passes_ring + h->ring_pos;
@@ -1100,11 +1108,11 @@ static void launch_sort(Handle* h, const SortSeg* segs, int nsegs, uint64_t mlo,
         // (plus the payload read and written in every pass)
         const uint64_t pay2 = 2 * static_cast<uint64_t>(pay_bytes);
         acct->per_row[0] = kb + kout + pb_out + pay2;
-        acct->per_row[1] = kb + kb + 4 + pay2;
-        acct->per_row[2] = 2 * kb + 8 + pay2;
-        acct->per_row[3] = kb + 4 + kout + pb_out + pay2;
+        acct->per_row[1] = kb + kb + rid + pay2;
+        acct->per_row[2] = 2 * kb + 2 * rid + pay2;
+        acct->per_row[3] = kb + rid + kout + pb_out + pay2;
     }
-    const uint64_t middle_bytes = L.rows * (2 * kb + 8 + 2 * static_cast<uint64_t>(pay_bytes));
+    const uint64_t middle_bytes = L.rows * (2 * kb + 2 * rid + 2 * static_cast<uint64_t>(pay_bytes));
     bool coop = xcd;
     for (int slot = 0; slot < D; ++slot) {
         LaunchTimer t(h, s, "radix_pass", middle_bytes, slot, acct);
@@ -1159,14 +1167,17 @@ static void masks_for(int key_type, uint32_t flags, uint64_t& mlo, uint64_t& mhi
 }
 
 void argsort_batch_enqueue(Handle* h, int key_type, const SortSeg* segs, int nsegs, uint32_t flags, void* scr,
-                           hipStream_t s, int pay_bytes)
+                           hipStream_t s, int pay_bytes, bool rowids)
 {
     uint64_t mlo = 0, mhi = 0;
     masks_for(key_type, flags, mlo, mhi);
+    if (!rowids)
+        for (int i = 0; i < nsegs; ++i)
+            require(segs[i].perm_out == nullptr, "a sort without row ids cannot write a permutation");
     switch (key_bytes(key_type)) {
-    case 4: launch_sort<uint32_t>(h, segs, nsegs, mlo, mhi, flags, scr, s, pay_bytes); break;
-    case 8: launch_sort<uint64_t>(h, segs, nsegs, mlo, mhi, flags, scr, s, pay_bytes); break;
-    default: launch_sort<K128>(h, segs, nsegs, mlo, mhi, flags, scr, s, pay_bytes); break;
+    case 4: launch_sort<uint32_t>(h, segs, nsegs, mlo, mhi, flags, scr, s, pay_bytes, rowids); break;
+    case 8: launch_sort<uint64_t>(h, segs, nsegs, mlo, mhi, flags, scr, s, pay_bytes, rowids); break;
+    default: launch_sort<K128>(h, segs, nsegs, mlo, mhi, flags, scr, s, pay_bytes, rowids); break;
     }
 }
 

@@ -172,6 +172,9 @@ struct Handle {
     // dry run: enqueue functions write the host staging of a batch and return
     // before uploading or launching anything (refreshes a captured plan)
     bool dry = false;
+    // Group-by sorts always move row ids (graph plans: the outputs of a replay,
+    // with or without a permutation, are only known after capture).
+    bool keep_rowids = false;
 
     void* ensure_scratch(size_t bytes);
     void staging_begin(size_t bytes);      // start a call's staging region
@@ -265,8 +268,10 @@ void argsort_enqueue(Handle* h, int key_type, const void* d_keys, uint64_t n, ui
 // pay_bytes (0, 4, 8 or 16): every segment's payload column (pay_in ->
 // pay_out; with 16, two 8-byte columns, the second pay_in2 -> pay_out2) moves
 // with its keys through the passes.
+// rowids false: no segment takes a permutation (perm_out null), so the
+// passes do not move row ids.
 void argsort_batch_enqueue(Handle* h, int key_type, const SortSeg* segs, int nsegs, uint32_t flags, void* scr,
-                           hipStream_t s, int pay_bytes = 0);
+                           hipStream_t s, int pay_bytes = 0, bool rowids = true);
 
 void gather_enqueue(Handle* h, uint32_t elem_bytes, const void* d_src, const void* d_perm,
                     int perm64, uint64_t n, void* d_dst, hipStream_t s);
