@@ -429,7 +429,10 @@ struct TileCtx {
 template <typename K, int ITEMS, int PB>
 struct PassShape {
     static constexpr int kLds = kBlock * ITEMS * static_cast<int>(sizeof(K) + 4 + PB) + kWaves * kRadix * 4 + kRadix * 4 + 64;
-    static constexpr int kCap = (ITEMS >= 24 || PB == 16) ? 2 : 3;  // register room for the larger tiles
+    // register room (bytes of rows a thread holds): the larger tiles and
+    // payloads get fewer blocks, so that no instantiation spills
+    static constexpr int kRowBytes = ITEMS * static_cast<int>(sizeof(K) + 4 + PB);
+    static constexpr int kCap = (ITEMS >= 24 || PB == 16 || kRowBytes > 160 || (PB != 0 && ITEMS >= 12)) ? 2 : 3;
     static constexpr int kBlocksPerCu = (160 * 1024 / kLds) < kCap ? (160 * 1024 / kLds) : kCap;
 };
 
