@@ -314,6 +314,35 @@ __device__ inline Agg shfl_up_agg(const Agg& x, int off)
     return y;
 }
 
+// Lane moves by DPP (data-parallel primitives: VALU operand routing, no LDS
+// round trip).  CTRL: row_shr:n = 0x110 + n (within 16-lane rows),
+// row_bcast:15 = 0x142, row_bcast:31 = 0x143; RM: rows written.  Lanes a move
+// does not reach are not used by the caller.
+template <int CTRL, int RM>
+__device__ inline uint32_t dpp32(uint32_t x)
+{
+    return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(x), CTRL, RM, 0xF, false));
+}
+template <int CTRL, int RM>
+__device__ inline uint64_t dpp64(uint64_t x)
+{
+    return (static_cast<uint64_t>(dpp32<CTRL, RM>(static_cast<uint32_t>(x >> 32))) << 32) |
+           dpp32<CTRL, RM>(static_cast<uint32_t>(x));
+}
+template <int KIND, int CTRL, int RM>
+__device__ inline Agg dpp_agg(const Agg& x)
+{
+    Agg y = agg_empty();
+    y.cnt = dpp64<CTRL, RM>(x.cnt);
+    if (KIND != KIND_COUNT) {
+        y.a = dpp64<CTRL, RM>(x.a);
+        y.b = dpp64<CTRL, RM>(x.b);
+        y.mn = dpp64<CTRL, RM>(x.mn);
+        y.mx = dpp64<CTRL, RM>(x.mx);
+    }
+    return y;
+}
+
 // The segmented-scan operator: (f1, x1) then (f2, x2) -> (f1 | f2, f2 ? x2 : x1 + x2).
 template <int KIND>
 __device__ inline void seg_combine(Agg& x, uint32_t& f, const Agg& y, uint32_t fy)
@@ -473,14 +502,33 @@ __device__ __attribute__((always_inline)) inline void gb_column(const GbArgs& a,
         }
     }
 
-    // inclusive segmented scan over the wave's lanes
+    // inclusive segmented scan over the wave's lanes, by DPP: a Hillis-Steele
+    // scan inside each 16-lane row (shifts 1, 2, 4, 8), then row 0's total into
+    // row 1 and row 2's into row 3 (row_bcast:15), then lane 31's prefix into
+    // rows 2 and 3 (row_bcast:31).  The operator is associative, so every lane
+    // ends with the combination of lanes 0 .. itself, in a fixed order.
     Agg x = cur;
     uint32_t f = seen ? 1u : 0u;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const Agg y = shfl_up_agg<KIND>(x, off);
-        const uint32_t fy = __shfl_up(f, off, 64);
-        if (lane >= off) seg_combine<KIND>(x, f, y, fy);
+    const int rl = lane & 15, row = lane >> 4;
+    {
+        Agg y = dpp_agg<KIND, 0x111, 0xF>(x);
+        uint32_t fy = dpp32<0x111, 0xF>(f);
+        if (rl >= 1) seg_combine<KIND>(x, f, y, fy);
+        y = dpp_agg<KIND, 0x112, 0xF>(x);
+        fy = dpp32<0x112, 0xF>(f);
+        if (rl >= 2) seg_combine<KIND>(x, f, y, fy);
+        y = dpp_agg<KIND, 0x114, 0xF>(x);
+        fy = dpp32<0x114, 0xF>(f);
+        if (rl >= 4) seg_combine<KIND>(x, f, y, fy);
+        y = dpp_agg<KIND, 0x118, 0xF>(x);
+        fy = dpp32<0x118, 0xF>(f);
+        if (rl >= 8) seg_combine<KIND>(x, f, y, fy);
+        y = dpp_agg<KIND, 0x142, 0xA>(x);
+        fy = dpp32<0x142, 0xA>(f);
+        if (row & 1) seg_combine<KIND>(x, f, y, fy);
+        y = dpp_agg<KIND, 0x143, 0xC>(x);
+        fy = dpp32<0x143, 0xC>(f);
+        if (row >= 2) seg_combine<KIND>(x, f, y, fy);
     }
     if (lane == 63) {
         s_wagg[wave] = x;
