@@ -579,6 +579,26 @@ __device__ __attribute__((always_inline)) inline void gb_column(const GbArgs& a,
     __syncthreads();  // s_wagg is reused by the next column
 }
 
+// What a block keeps of one claimed tile.
+struct GbTileState {
+    uint32_t tile;   // global tile id
+    uint32_t sgi;    // table
+    uint32_t lt;     // tile index inside the table
+    uint32_t H;      // heads in the tile
+    uint32_t gbase;  // table-local id of the tile's first head's group (after resolve)
+    uint64_t tbase;  // first table row of the tile
+    uint64_t r0;     // this lane's first row
+    uint32_t cnt;    // this lane's rows inside the table
+    uint32_t hb;     // this lane's head bits
+    uint32_t hx;     // heads before this lane in the tile
+};
+
+// Persistent over tiles and pipelined: a block claims tile k+1 and publishes
+// its head count before it aggregates tile k, and resolves tile k+1's group
+// ids only afterwards, so that the look-back finds its predecessors' prefixes
+// published instead of waiting for them.  Tiles are claimed in increasing
+// order by running blocks and a block waits only on smaller tiles, so the
+// smallest unresolved tile always completes, at any residency.
 template <typename K>
 __global__ __launch_bounds__(kBlock) void k_gb_tile(GbArgs a)
 {
@@ -588,38 +608,44 @@ __global__ __launch_bounds__(kBlock) void k_gb_tile(GbArgs a)
     __shared__ uint32_t s_tile, s_gbase;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    if (tid == 0) s_tile = atomicAdd(a.ticket, 1u);
-    __syncthreads();
-    const uint32_t tile = __builtin_amdgcn_readfirstlane(s_tile);
-    const uint32_t sgi = a.tile_seg ? __builtin_amdgcn_readfirstlane(a.tile_seg[tile]) : 0u;
-    const GbSeg& sg = a.segs[sgi];
-    const uint64_t n = sg.n;
-    const uint32_t lt = tile - sg.gtile_off;  // tile index inside the table
-    const uint64_t tbase = static_cast<uint64_t>(lt) * kGbTile;
-    const K* keys = static_cast<const K*>(a.skeys) + sg.row_off;
 
-    // ---- this lane's rows: tbase + tid*16 .. +15 (cnt of them inside the table) ----
-    const uint32_t tn = static_cast<uint32_t>(std::min<uint64_t>(kGbTile, n - tbase));
-    const uint32_t lr0 = static_cast<uint32_t>(tid) * kGbRows;
-    const uint32_t cnt = lr0 < tn ? std::min<uint32_t>(kGbRows, tn - lr0) : 0u;
-    const uint64_t r0 = tbase + lr0;
-    const uint32_t hb = head_bits<K>(keys, r0, cnt);
-    const uint32_t hcount = __popc(hb);
-    uint32_t inc = hcount;  // inclusive scan of head counts over the wave
+    // claim a tile, find its heads, publish its head count; false at the end
+    auto begin = [&](GbTileState& t) -> bool {
+        if (tid == 0) s_tile = atomicAdd(a.ticket, 1u);
+        __syncthreads();
+        t.tile = __builtin_amdgcn_readfirstlane(s_tile);
+        if (t.tile >= a.ntiles) return false;
+        t.sgi = a.tile_seg ? __builtin_amdgcn_readfirstlane(a.tile_seg[t.tile]) : 0u;
+        const GbSeg& sg = a.segs[t.sgi];
+        const uint64_t n = sg.n;
+        t.lt = t.tile - sg.gtile_off;
+        t.tbase = static_cast<uint64_t>(t.lt) * kGbTile;
+        const K* keys = static_cast<const K*>(a.skeys) + sg.row_off;
+        // this lane's rows: tbase + tid*16 .. +15 (cnt of them inside the table)
+        const uint32_t tn = static_cast<uint32_t>(std::min<uint64_t>(kGbTile, n - t.tbase));
+        const uint32_t lr0 = static_cast<uint32_t>(tid) * kGbRows;
+        t.cnt = lr0 < tn ? std::min<uint32_t>(kGbRows, tn - lr0) : 0u;
+        t.r0 = t.tbase + lr0;
+        t.hb = head_bits<K>(keys, t.r0, t.cnt);
+        const uint32_t hcount = __popc(t.hb);
+        uint32_t inc = hcount;  // inclusive scan of head counts over the wave
 #pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t y = __shfl_up(inc, off, 64);
-        if (lane >= off) inc += y;
-    }
-    if (lane == 63) s_wsum[wave] = inc;
-    __syncthreads();
-    uint32_t pre_w = 0;
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(inc, off, 64);
+            if (lane >= off) inc += y;
+        }
+        if (lane == 63) s_wsum[wave] = inc;
+        __syncthreads();
+        uint32_t pre_w = 0;
 #pragma unroll
-    for (int i = 0; i < kWaves; ++i) pre_w += (i < wave) ? s_wsum[i] : 0u;
-    const uint32_t hx = pre_w + inc - hcount;  // heads before this lane in the tile
-    const uint32_t H = s_wsum[0] + s_wsum[1] + s_wsum[2] + s_wsum[3];
+        for (int i = 0; i < kWaves; ++i) pre_w += (i < wave) ? s_wsum[i] : 0u;
+        t.hx = pre_w + inc - hcount;
+        t.H = __builtin_amdgcn_readfirstlane(s_wsum[0] + s_wsum[1] + s_wsum[2] + s_wsum[3]);
+        if (tid == 0) store_relaxed(a.status + t.tile, (t.lt == 0 ? kFlagP : kFlagA) | t.H);
+        return true;
+    };
 
-    // ---- decoupled look-back over the table's tiles' head counts -> first group id ----
+    // decoupled look-back over the table's tiles' head counts -> first group id.
     // Wave 0 reads the 64 previous tiles' status words at once (one round
     // trip; they are contiguous), takes the nearest one with an inclusive
     // prefix (lane L), and adds the aggregates of lanes 0..L.  A window with
@@ -627,75 +653,96 @@ __global__ __launch_bounds__(kBlock) void k_gb_tile(GbArgs a)
     // inclusive prefix is added whole and the next 64 tiles are read.  The
     // table's first tile publishes its inclusive prefix at once, so the walk
     // ends there at the latest.
-    if (wave == 0) {
-        uint32_t* my = a.status + tile;
-        if (lane == 0) store_relaxed(my, (lt == 0 ? kFlagP : kFlagA) | H);
-        uint32_t excl = 0;
-        if (lt > 0) {
-            const int64_t first = static_cast<int64_t>(sg.gtile_off);
-            int64_t k = static_cast<int64_t>(tile) - 1;
-            for (;;) {
-                const int64_t idx = k - lane;
-                // lanes before the table's first tile: a zero prefix (never reached
-                // past the first tile's own inclusive prefix)
-                const uint32_t w = idx >= first ? load_relaxed(a.status + idx) : kFlagP;
-                const uint32_t f = w & ~kValMask;
-                const uint64_t pm = __ballot(f == kFlagP);
-                const uint64_t zm = __ballot(f == 0u);
-                const int L = pm ? __ffsll(static_cast<long long>(pm)) - 1 : 64;
-                const uint64_t upto = L >= 63 ? ~0ull : ((2ull << L) - 1);  // lanes 0..L
-                if (zm & upto) {
-                    __builtin_amdgcn_s_sleep(1);
-                    continue;
-                }
-                uint32_t part = lane <= L ? (w & kValMask) : 0u;
+    auto resolve = [&](GbTileState& t) {
+        const GbSeg& sg = a.segs[t.sgi];
+        if (wave == 0) {
+            uint32_t excl = 0;
+            if (t.lt > 0) {
+                const int64_t first = static_cast<int64_t>(sg.gtile_off);
+                int64_t k = static_cast<int64_t>(t.tile) - 1;
+                for (;;) {
+                    const int64_t idx = k - lane;
+                    // lanes before the table's first tile: a zero prefix (never
+                    // reached past the first tile's own inclusive prefix)
+                    const uint32_t w = idx >= first ? load_relaxed(a.status + idx) : kFlagP;
+                    const uint32_t f = w & ~kValMask;
+                    const uint64_t pm = __ballot(f == kFlagP);
+                    const uint64_t zm = __ballot(f == 0u);
+                    const int L = pm ? __ffsll(static_cast<long long>(pm)) - 1 : 64;
+                    const uint64_t upto = L >= 63 ? ~0ull : ((2ull << L) - 1);  // lanes 0..L
+                    if (zm & upto) {
+                        __builtin_amdgcn_s_sleep(1);
+                        continue;
+                    }
+                    uint32_t part = lane <= L ? (w & kValMask) : 0u;
 #pragma unroll
-                for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
-                excl += part;
-                if (L < 64) break;
-                k -= 64;
+                    for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
+                    excl += part;
+                    if (L < 64) break;
+                    k -= 64;
+                }
+                if (lane == 0) store_relaxed(a.status + t.tile, kFlagP | (excl + t.H));
             }
-            if (lane == 0) store_relaxed(my, kFlagP | (excl + H));
-        }
-        if (lane == 0) {
-            s_gbase = excl;
-            a.tile_gbase[tile] = excl;
-            a.tile_heads[tile] = H;
-            if (lt == sg.ntiles - 1) {
-                *gmem(sg.out.n_groups) = static_cast<uint64_t>(excl) + H;
-                if (excl + H <= sg.cap) gmem(sg.out.offsets)[excl + H] = static_cast<uint32_t>(n);
+            if (lane == 0) {
+                s_gbase = excl;
+                a.tile_gbase[t.tile] = excl;
+                a.tile_heads[t.tile] = t.H;
+                if (t.lt == sg.ntiles - 1) {
+                    *gmem(sg.out.n_groups) = static_cast<uint64_t>(excl) + t.H;
+                    if (excl + t.H <= sg.cap) gmem(sg.out.offsets)[excl + t.H] = static_cast<uint32_t>(sg.n);
+                }
             }
         }
-    }
-    __syncthreads();
-    const uint32_t gfirst = s_gbase + hx;
+        __syncthreads();
+        t.gbase = __builtin_amdgcn_readfirstlane(s_gbase);
+    };
 
-    // ---- group keys and offsets at the heads ----
-    {
-        uint32_t bits = hb;
-        uint32_t g = gfirst;
-        while (bits) {
-            const int k = __ffs(bits) - 1;
-            bits &= bits - 1;
-            const uint64_t r = r0 + k;
-            if (g < sg.cap) {
-                gmem(sg.out.offsets)[g] = static_cast<uint32_t>(r);
-                if (sg.out.group_keys) put_key(static_cast<K*>(sg.out.group_keys), g, get_key(keys, r));
+    // group keys, offsets and every column's aggregates of a resolved tile
+    auto finish = [&](const GbTileState& t) {
+        const GbSeg& sg = a.segs[t.sgi];
+        const K* keys = static_cast<const K*>(a.skeys) + sg.row_off;
+        const uint32_t gfirst = t.gbase + t.hx;
+        {
+            uint32_t bits = t.hb;
+            uint32_t g = gfirst;
+            while (bits) {
+                const int k = __ffs(bits) - 1;
+                bits &= bits - 1;
+                const uint64_t r = t.r0 + k;
+                if (g < sg.cap) {
+                    gmem(sg.out.offsets)[g] = static_cast<uint32_t>(r);
+                    if (sg.out.group_keys) put_key(static_cast<K*>(sg.out.group_keys), g, get_key(keys, r));
+                }
+                ++g;
             }
-            ++g;
         }
-    }
+        for (int c = 0; c < a.ncols; ++c) {
+            switch (a.cols[c].kind) {
+            case KIND_F64:
+                gb_column<KIND_F64>(a, sg, c, t.tile, t.tbase, t.cnt, t.hb, gfirst, s_wagg, s_wflag);
+                break;
+            case KIND_SIGNED:
+                gb_column<KIND_SIGNED>(a, sg, c, t.tile, t.tbase, t.cnt, t.hb, gfirst, s_wagg, s_wflag);
+                break;
+            case KIND_UNSIGNED:
+                gb_column<KIND_UNSIGNED>(a, sg, c, t.tile, t.tbase, t.cnt, t.hb, gfirst, s_wagg, s_wflag);
+                break;
+            default:
+                gb_column<KIND_COUNT>(a, sg, c, t.tile, t.tbase, t.cnt, t.hb, gfirst, s_wagg, s_wflag);
+                break;
+            }
+        }
+    };
 
-    // ---- per value column ----
-    for (int c = 0; c < a.ncols; ++c) {
-        switch (a.cols[c].kind) {
-        case KIND_F64: gb_column<KIND_F64>(a, sg, c, tile, tbase, cnt, hb, gfirst, s_wagg, s_wflag); break;
-        case KIND_SIGNED: gb_column<KIND_SIGNED>(a, sg, c, tile, tbase, cnt, hb, gfirst, s_wagg, s_wflag); break;
-        case KIND_UNSIGNED:
-            gb_column<KIND_UNSIGNED>(a, sg, c, tile, tbase, cnt, hb, gfirst, s_wagg, s_wflag);
-            break;
-        default: gb_column<KIND_COUNT>(a, sg, c, tile, tbase, cnt, hb, gfirst, s_wagg, s_wflag); break;
-        }
+    GbTileState c0, c1;
+    if (!begin(c0)) return;
+    resolve(c0);
+    for (;;) {
+        const bool has1 = begin(c1);
+        finish(c0);
+        if (!has1) break;
+        resolve(c1);
+        c0 = c1;
     }
 }
 
@@ -756,6 +803,27 @@ __global__ void k_gb_empty(const GbSeg* segs, uint32_t nsegs)
         *gmem(segs[i].out.n_groups) = 0;
         gmem(segs[i].out.offsets)[0] = 0;
     }
+}
+
+// Resident blocks of a tile kernel on a device (cached per kernel and device).
+unsigned gb_resident_blocks(const void* fn, int device)
+{
+    struct Entry {
+        const void* fn;
+        int device;
+        unsigned blocks;
+    };
+    static std::mutex mu;
+    static std::vector<Entry> cache;
+    std::lock_guard<std::mutex> lk(mu);
+    for (const Entry& e : cache)
+        if (e.fn == fn && e.device == device) return e.blocks;
+    int per_cu = 0, cus = 0;
+    VDF_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kBlock, 0));
+    VDF_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    const unsigned blocks = static_cast<unsigned>(std::max(1, per_cu) * std::max(1, cus));
+    cache.push_back(Entry{fn, device, blocks});
+    return blocks;
 }
 
 int kind_of(int type)
@@ -942,10 +1010,15 @@ void groupby_aggregate_enqueue(Handle* h, int key_type, const void* skeys, const
     {
         // algorithmic bytes: sorted keys + per column its value (and row id when gathered)
         LaunchTimer t(h, s, "groupby_tile", rows_total * (kb + val_bytes));
+        // persistent grid: the resident blocks (tiles are claimed dynamically)
+        const void* fn = kb == 4 ? reinterpret_cast<const void*>(k_gb_tile<uint32_t>)
+                                 : (kb == 8 ? reinterpret_cast<const void*>(k_gb_tile<uint64_t>)
+                                            : reinterpret_cast<const void*>(k_gb_tile<K16>));
+        const unsigned grid = static_cast<unsigned>(std::min<uint64_t>(a.ntiles, gb_resident_blocks(fn, h->device)));
         switch (kb) {
-        case 4: hipLaunchKernelGGL(k_gb_tile<uint32_t>, dim3(a.ntiles), dim3(kBlock), 0, s, a); break;
-        case 8: hipLaunchKernelGGL(k_gb_tile<uint64_t>, dim3(a.ntiles), dim3(kBlock), 0, s, a); break;
-        default: hipLaunchKernelGGL(k_gb_tile<K16>, dim3(a.ntiles), dim3(kBlock), 0, s, a); break;
+        case 4: hipLaunchKernelGGL(k_gb_tile<uint32_t>, dim3(grid), dim3(kBlock), 0, s, a); break;
+        case 8: hipLaunchKernelGGL(k_gb_tile<uint64_t>, dim3(grid), dim3(kBlock), 0, s, a); break;
+        default: hipLaunchKernelGGL(k_gb_tile<K16>, dim3(grid), dim3(kBlock), 0, s, a); break;
         }
         check_launch("k_gb_tile");
     }
