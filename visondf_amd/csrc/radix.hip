@@ -223,8 +223,9 @@ struct SortArgs {
     uint32_t* hist;            // [S][D][256]
     uint32_t* plan_hdr;        // [S][kPlanHdr]
     uint32_t* plan_base;       // [S][D][256] segment-local bucket starts
-    void* kbuf[2];             // transformed keys between passes (concatenated)
-    uint32_t* vbuf[2];         // local row ids between passes (concatenated)
+    void* kbuf[2];             // transformed keys between passes (concatenated); with
+                               // 4-byte keys and row ids, (key, row id) records
+    uint32_t* vbuf[2];         // local row ids between passes (8-byte and 16-byte keys)
     void* pbuf[2];             // payload between passes (concatenated)
     void* pbuf2[2];            // second payload column between passes (16-byte payloads)
     uint32_t* lb[2];           // look-back words [tiles][256], alternating by slot parity
@@ -449,9 +450,14 @@ void k_pass(SortArgs a, uint32_t slot)
     constexpr int PI2 = TWO ? ITEMS : 1;
     constexpr int TILE = kBlock * ITEMS;
     constexpr int D = KO::kDigits;
+    // 4-byte keys travel between passes packed with their row ids: one 8-byte
+    // record (key low, row id high) per row, so a bucket run is written as one
+    // stream of full 8-byte words instead of two half-size streams
+    constexpr bool KV = std::is_same<K, uint32_t>::value;
 
-    __shared__ K s_keys[TILE];
-    __shared__ uint32_t s_vals[TILE];
+    __shared__ K s_keys[KV ? 1 : TILE];
+    __shared__ uint32_t s_vals[KV ? 1 : TILE];
+    __shared__ uint64_t s_kv[KV ? TILE : 1];
     __shared__ P s_pay[PB ? TILE : 1];
     __shared__ uint64_t s_pay2[TWO ? TILE : 1];
     __shared__ uint32_t s_wc[kWaves][kRadix];
@@ -564,6 +570,23 @@ void k_pass(SortArgs a, uint32_t slot)
                 key[j] = KO::xform(key[j], a.mlo, a.mhi);
                 val[j] = static_cast<uint32_t>(wl + static_cast<uint64_t>(j) * 64);
             }
+        } else if (KV && !a.norows) {
+            if constexpr (KV) {
+                const auto* rp = gmem(static_cast<const uint64_t*>(a.kbuf[(slot - 1) & 1]) + c.row_off);
+                uint64_t r[ITEMS];
+                if (full) {
+#pragma unroll
+                    for (int j = 0; j < ITEMS; ++j) r[j] = rp[wl + static_cast<uint64_t>(j) * 64];
+                } else {
+#pragma unroll
+                    for (int j = 0; j < ITEMS; ++j) r[j] = rp[std::min(wl + static_cast<uint64_t>(j) * 64, last)];
+                }
+#pragma unroll
+                for (int j = 0; j < ITEMS; ++j) {
+                    key[j] = static_cast<K>(r[j]);
+                    val[j] = static_cast<uint32_t>(r[j] >> 32);
+                }
+            }
         } else {
             const K* kp = kin + c.row_off;
             const auto* vp = gmem(vin + c.row_off);
@@ -638,8 +661,12 @@ void k_pass(SortArgs a, uint32_t slot)
         for (int j = 0; j < ITEMS; ++j) {
             if (wl + static_cast<uint64_t>(j) * 64 < c.n) {
                 const uint32_t p = rnk[j];
-                s_keys[p] = key[j];
-                s_vals[p] = val[j];
+                if constexpr (KV) {
+                    s_kv[p] = key[j] | (static_cast<uint64_t>(val[j]) << 32);
+                } else {
+                    s_keys[p] = key[j];
+                    s_vals[p] = val[j];
+                }
                 if constexpr (PB != 0) s_pay[p] = pay[j];
                 if constexpr (TWO) s_pay2[p] = pay2[j];
             }
@@ -691,8 +718,20 @@ void k_pass(SortArgs a, uint32_t slot)
         // staging arrays; rows past a partial tile's end are not stored.
         constexpr int U = 4;
         static_assert(ITEMS % U == 0, "tile rows per thread must be a multiple of U");
+        // staged row i: key and row id
+        auto staged = [&](uint32_t i, K& kk, uint32_t& vv) {
+            if constexpr (KV) {
+                const uint64_t r = s_kv[i];
+                kk = static_cast<K>(r);
+                vv = static_cast<uint32_t>(r >> 32);
+            } else {
+                kk = s_keys[i];
+                vv = s_vals[i];
+            }
+        };
         if (!c.last) {
             K* kout = static_cast<K*>(a.kbuf[slot & 1]) + c.row_off;
+            auto* rout = gmem(static_cast<uint64_t*>(a.kbuf[slot & 1]) + c.row_off);
             auto* vout = gmem(a.vbuf[slot & 1] + c.row_off);
             auto* pout = gmem(static_cast<P*>(a.pbuf[slot & 1]) + c.row_off);
             auto* pout2 = gmem(static_cast<uint64_t*>(a.pbuf2[slot & 1]) + c.row_off);
@@ -702,19 +741,22 @@ void k_pass(SortArgs a, uint32_t slot)
                 P pv[U];
                 uint64_t pv2[U];
 #pragma unroll
-                for (int u = 0; u < U; ++u) kk[u] = s_keys[i0 + u * kBlock];
+                for (int u = 0; u < U; ++u) staged(i0 + u * kBlock, kk[u], vv[u]);
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
                     pos[u] = s_gbase[KO::digit(kk[u], dig)] + i0 + u * kBlock;
-                    vv[u] = s_vals[i0 + u * kBlock];
                     if constexpr (PB != 0) pv[u] = s_pay[i0 + u * kBlock];
                     if constexpr (TWO) pv2[u] = s_pay2[i0 + u * kBlock];
                 }
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
                     if (i0 + u * kBlock < tn) {
-                        KO::put(kout, pos[u], kk[u]);
-                        if (!a.norows) vout[pos[u]] = vv[u];
+                        if (KV && !a.norows) {
+                            if constexpr (KV) rout[pos[u]] = kk[u] | (static_cast<uint64_t>(vv[u]) << 32);
+                        } else {
+                            KO::put(kout, pos[u], kk[u]);
+                            if (!a.norows) vout[pos[u]] = vv[u];
+                        }
                         if constexpr (PB != 0) pout[pos[u]] = pv[u];
                         if constexpr (TWO) pout2[pos[u]] = pv2[u];
                     }
@@ -728,12 +770,11 @@ void k_pass(SortArgs a, uint32_t slot)
                 P pv[U];
                 uint64_t pv2[U];
 #pragma unroll
-                for (int u = 0; u < U; ++u) kk[u] = s_keys[i0 + u * kBlock];
+                for (int u = 0; u < U; ++u) staged(i0 + u * kBlock, kk[u], vv[u]);
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
                     const uint32_t p = s_gbase[KO::digit(kk[u], dig)] + i0 + u * kBlock;
                     pos[u] = a.reverse ? nm1 - p : p;
-                    vv[u] = s_vals[i0 + u * kBlock];
                     if constexpr (PB != 0) pv[u] = s_pay[i0 + u * kBlock];
                     if constexpr (TWO) pv2[u] = s_pay2[i0 + u * kBlock];
                 }
@@ -1031,10 +1072,16 @@ static void launch_sort(Handle* h, const SortSeg* segs, int nsegs, uint64_t mlo,
     uint32_t* dtiles = c.take<uint32_t>(std::max<uint64_t>(L.tiles, 1));
     a.lb[0] = c.take<uint32_t>(std::max<uint64_t>(L.tiles, 1) * kRadix);
     a.lb[1] = c.take<uint32_t>(std::max<uint64_t>(L.tiles, 1) * kRadix);
+    // kbuf[i] directly followed by vbuf[i]: with 4-byte keys the pair is one
+    // buffer of 8-byte (key, row id) records
     a.kbuf[0] = c.take<char>(L.rows * sizeof(K));
-    a.kbuf[1] = c.take<char>(L.rows * sizeof(K));
     a.vbuf[0] = c.take<uint32_t>(L.rows);
+    a.kbuf[1] = c.take<char>(L.rows * sizeof(K));
     a.vbuf[1] = c.take<uint32_t>(L.rows);
+    if (sizeof(K) == 4)
+        for (int i = 0; i < 2; ++i)
+            require(static_cast<char*>(a.kbuf[i]) + L.rows * 8 <= reinterpret_cast<char*>(a.vbuf[i] + L.rows),
+                    "record buffer layout");
     a.pbuf[0] = c.take<char>(L.rows * static_cast<uint64_t>(pay_bytes));
     a.pbuf[1] = c.take<char>(L.rows * static_cast<uint64_t>(pay_bytes));
     // a 16-byte payload is two 8-byte columns: the second in the upper half
