@@ -134,6 +134,19 @@ def test_groupby_single_group_spans_tiles(vdf):
     _check(vdf, np.zeros(n, dtype=np.int64), [vals, np.arange(n, dtype=np.int64)], label="one-group")
 
 
+def test_groupby_group_spans_fixup_windows(vdf):
+    """One group of 8.5M rows (about 2,080 tiles, more than one fix-up window of
+    2,048 tiles) between groups of a few rows, and a second long group that
+    ends exactly at the table's end."""
+    rng = np.random.default_rng(21)
+    keys = np.concatenate([rng.integers(0, 1000, 5000), np.full(8_500_000, 5000), rng.integers(6000, 7000, 7001),
+                           np.full(3 * 4096 + 17, 9000)]).astype(np.int32)
+    perm = rng.permutation(keys.size)
+    keys = keys[perm]
+    n = keys.size
+    _check(vdf, keys, [datagen.normal01(21, 1, n), rng.integers(-(2**40), 2**40, n)], label="windows")
+
+
 def test_groupby_every_row_distinct(vdf):
     n = 262_147
     keys = np.random.default_rng(1).permutation(n).astype(np.uint32)

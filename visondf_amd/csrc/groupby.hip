@@ -25,9 +25,11 @@
 //        (cross-wave carry through LDS) joins the lanes.  Groups that start
 //        and end inside the tile are final here; each tile leaves two
 //        partials (before its first head, after its last head).
-//   3. k_gb_fixup: one block per tile owning the group that starts at the
-//        tile's last head; joins the following tiles' leading partials in
-//        tile order with a fixed reduction tree.
+//   3. k_gb_fixup: one thread per tile owning the group that starts at the
+//        tile's last head; a group ending in the next tile is joined there,
+//        longer ones are queued for k_gb_fixup_long, which joins the
+//        following tiles' leading partials in tile order with a fixed
+//        reduction tree (a block per queued tile).
 // The reduction order is a function of the row count and tile size only, so a
 // table's results do not depend on what else runs with it.
 #include "vdf_internal.h"
@@ -213,6 +215,8 @@ struct GbArgs {
     uint32_t* tile_heads;      // [ntiles] heads in the tile
     Agg* prefix;               // [ncols][ntiles] rows before the tile's first head (or the whole tile)
     Agg* suffix;               // [ncols][ntiles] rows from the tile's last head to its end
+    uint32_t* long_count;      // tiles queued in long_list (zeroed before the tile kernel)
+    uint32_t* long_list;       // [ntiles] tiles whose last group spans following tiles
 };
 
 // Output pointers of one value column, read from the table descriptor once
@@ -242,16 +246,27 @@ __device__ inline T uniform64(T x)
 }
 
 // (sg is the same table for the whole wave)
+// Output pointers of column c of table sg.  UNIFORM: every lane of the wave
+// works on the same table, so the pointers are moved to scalar registers.
+template <bool UNIFORM = true>
 __device__ inline ColOut col_out(const GbArgs& a, const GbSeg& sg, int c)
 {
     ColOut o;
     const vdf_agg_out& ag = sg.out.aggs[c];
-    o.counts = uniform64((c == 0 && (a.agg_mask & VDF_AGG_COUNT)) ? sg.out.counts : nullptr);
-    o.sum = uniform64((a.agg_mask & VDF_AGG_SUM) ? ag.sum : nullptr);
-    o.mean = uniform64((a.agg_mask & VDF_AGG_MEAN) ? ag.mean : nullptr);
-    o.min = uniform64((a.agg_mask & VDF_AGG_MIN) ? ag.min : nullptr);
-    o.max = uniform64((a.agg_mask & VDF_AGG_MAX) ? ag.max : nullptr);
-    o.cap = uniform64(sg.cap);
+    o.counts = (c == 0 && (a.agg_mask & VDF_AGG_COUNT)) ? sg.out.counts : nullptr;
+    o.sum = (a.agg_mask & VDF_AGG_SUM) ? ag.sum : nullptr;
+    o.mean = (a.agg_mask & VDF_AGG_MEAN) ? ag.mean : nullptr;
+    o.min = (a.agg_mask & VDF_AGG_MIN) ? ag.min : nullptr;
+    o.max = (a.agg_mask & VDF_AGG_MAX) ? ag.max : nullptr;
+    o.cap = sg.cap;
+    if (UNIFORM) {
+        o.counts = uniform64(o.counts);
+        o.sum = uniform64(o.sum);
+        o.mean = uniform64(o.mean);
+        o.min = uniform64(o.min);
+        o.max = uniform64(o.max);
+        o.cap = uniform64(o.cap);
+    }
     return o;
 }
 
@@ -292,11 +307,17 @@ __device__ inline uint32_t load_relaxed(uint32_t* p)
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// 64-bit wave shuffle as two 32-bit ones
+// 64-bit wave shuffles as two 32-bit ones
 __device__ inline uint64_t shfl_up64(uint64_t x, int off)
 {
     const uint32_t lo = __shfl_up(static_cast<uint32_t>(x), off, 64);
     const uint32_t hi = __shfl_up(static_cast<uint32_t>(x >> 32), off, 64);
+    return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+__device__ inline uint64_t shfl_down64(uint64_t x, int off)
+{
+    const uint32_t lo = __shfl_down(static_cast<uint32_t>(x), off, 64);
+    const uint32_t hi = __shfl_down(static_cast<uint32_t>(x >> 32), off, 64);
     return (static_cast<uint64_t>(hi) << 32) | lo;
 }
 
@@ -703,6 +724,24 @@ __global__ __launch_bounds__(kBlock) void k_gb_tile(GbArgs a)
         const K* keys = static_cast<const K*>(a.skeys) + sg.row_off;
         const uint32_t gfirst = t.gbase + t.hx;
         {
+            // 4-byte keys: the lane's 16 keys come back with four 16-byte loads
+            // (the lane's rows are 16-row aligned and inside the table's padded
+            // rows), and each head picks its key from registers; wider keys
+            // are read per head
+            uint32_t kw[kGbRows];
+            if constexpr (sizeof(K) == 4) {
+                if (t.hb) {
+                    const auto* q = gmem(reinterpret_cast<const u32x4*>(keys + t.r0));
+#pragma unroll
+                    for (int i = 0; i < kGbRows / 4; ++i) {
+                        const u32x4 v = q[i];
+                        kw[4 * i + 0] = v.x;
+                        kw[4 * i + 1] = v.y;
+                        kw[4 * i + 2] = v.z;
+                        kw[4 * i + 3] = v.w;
+                    }
+                }
+            }
             uint32_t bits = t.hb;
             uint32_t g = gfirst;
             while (bits) {
@@ -711,7 +750,16 @@ __global__ __launch_bounds__(kBlock) void k_gb_tile(GbArgs a)
                 const uint64_t r = t.r0 + k;
                 if (g < sg.cap) {
                     gmem(sg.out.offsets)[g] = static_cast<uint32_t>(r);
-                    if (sg.out.group_keys) put_key(static_cast<K*>(sg.out.group_keys), g, get_key(keys, r));
+                    if (sg.out.group_keys) {
+                        if constexpr (sizeof(K) == 4) {
+                            uint32_t kk = kw[0];
+#pragma unroll
+                            for (int j = 1; j < kGbRows; ++j) kk = (k == j) ? kw[j] : kk;
+                            put_key(static_cast<K*>(sg.out.group_keys), g, kk);
+                        } else {
+                            put_key(static_cast<K*>(sg.out.group_keys), g, get_key(keys, r));
+                        }
+                    }
                 }
                 ++g;
             }
@@ -748,50 +796,123 @@ __global__ __launch_bounds__(kBlock) void k_gb_tile(GbArgs a)
 
 // Joins the group that starts at tile b's last head with the leading
 // partials of the table's following tiles up to (and including) the next tile
-// that has a head.
+// that has a head.  One thread per tile: a group that ends in the next tile
+// (or at the table's end) is finished here; a tile whose group spans tiles
+// without heads is queued for k_gb_fixup_long.
 __global__ __launch_bounds__(kBlock) void k_gb_fixup(GbArgs a)
 {
-    __shared__ Agg s_red[kBlock];
-    __shared__ uint32_t s_first;
-    const uint32_t b = blockIdx.x;
-    const int tid = threadIdx.x;
+    const uint32_t b = blockIdx.x * kBlock + threadIdx.x;
+    if (b >= a.ntiles) return;
     const uint32_t heads = a.tile_heads[b];
     if (heads == 0) return;
     const uint32_t sgi = a.tile_seg ? a.tile_seg[b] : 0u;
     const GbSeg& sg = a.segs[sgi];
     const uint32_t tend = sg.gtile_off + sg.ntiles;  // one past the table's last tile
+    const bool has_next = b + 1 < tend;
+    if (has_next && a.tile_heads[b + 1] == 0) {
+        gmem(a.long_list)[atomicAdd(a.long_count, 1u)] = b;
+        return;
+    }
     const uint32_t gid = a.tile_gbase[b] + heads - 1;
-    const bool short_chain = (b + 1 >= tend) || a.tile_heads[b + 1] > 0;
     for (int c = 0; c < a.ncols; ++c) {
         const int kind = a.cols[c].kind;
-        const Agg* pre = a.prefix + static_cast<uint64_t>(c) * a.ntiles;
         Agg acc = a.suffix[static_cast<uint64_t>(c) * a.ntiles + b];
-        if (short_chain) {
-            if (tid == 0) {
-                if (b + 1 < tend) acc = agg_join(acc, pre[b + 1], kind);
-                write_final(a, col_out(a, sg, c), c, gid, acc);
+        if (has_next) acc = agg_join(acc, a.prefix[static_cast<uint64_t>(c) * a.ntiles + b + 1], kind);
+        write_final(a, col_out<false>(a, sg, c), c, gid, acc);  // lanes differ in table
+    }
+}
+
+// Groups that span whole tiles: a block per queued tile b walks the following
+// tiles in windows of kBlock * kFixRows; thread i joins its kFixRows
+// consecutive tiles in order, then the block joins the threads' partials by a
+// fixed tree (lane pairs, then the four waves in order).
+constexpr int kFixRows = 8;
+
+template <int KIND>
+__device__ inline Agg wave_join_tree(Agg x)
+{
+    // lane l joins lane l + off when l is a multiple of 2 * off: after the
+    // steps lane 0 holds lanes 0..63 joined in lane order
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        Agg y;
+        y.cnt = shfl_down64(x.cnt, off);
+        y.a = shfl_down64(x.a, off);
+        y.b = shfl_down64(x.b, off);
+        y.mn = shfl_down64(x.mn, off);
+        y.mx = shfl_down64(x.mx, off);
+        if ((lane & (2 * off - 1)) == 0) x = agg_join(x, y, KIND);
+    }
+    return x;
+}
+
+template <int KIND>
+__device__ inline void fix_window(const GbArgs& a, int c, uint32_t w0, uint32_t last, Agg* s_part, Agg& acc)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const Agg* pre = a.prefix + static_cast<uint64_t>(c) * a.ntiles;
+    Agg x = agg_empty();
+    const uint32_t t0 = w0 + static_cast<uint32_t>(tid) * kFixRows;
+    Agg p[kFixRows];  // every load issued before the joins (tiles past `last` read tile `last`)
+#pragma unroll
+    for (int k = 0; k < kFixRows; ++k) p[k] = pre[std::min(t0 + k, last)];
+#pragma unroll
+    for (int k = 0; k < kFixRows; ++k)
+        if (t0 + k <= last) x = agg_join(x, p[k], KIND);
+    x = wave_join_tree<KIND>(x);
+    if (lane == 0) s_part[wave] = x;
+    __syncthreads();
+    if (tid == 0)
+        for (int w = 0; w < kWaves; ++w) acc = agg_join(acc, s_part[w], KIND);
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(kBlock) void k_gb_fixup_long(GbArgs a)
+{
+    __shared__ Agg s_part[kWaves];
+    __shared__ Agg s_acc[VDF_MAX_VALUE_COLS];  // the group's running joins (thread 0)
+    __shared__ uint32_t s_stop;
+    const int tid = threadIdx.x;
+    const uint32_t nlong = *a.long_count;
+    for (uint32_t e = blockIdx.x; e < nlong; e += gridDim.x) {
+        const uint32_t b = a.long_list[e];
+        const uint32_t sgi = a.tile_seg ? a.tile_seg[b] : 0u;
+        const GbSeg& sg = a.segs[sgi];
+        const uint32_t tend = sg.gtile_off + sg.ntiles;
+        const uint32_t gid = a.tile_gbase[b] + a.tile_heads[b] - 1;
+        if (tid == 0)
+            for (int c = 0; c < a.ncols; ++c) s_acc[c] = a.suffix[static_cast<uint64_t>(c) * a.ntiles + b];
+        for (uint32_t w0 = b + 1;; w0 += kBlock * kFixRows) {
+            // the chain ends at the first tile with a head, or at the table's end
+            if (tid == 0) s_stop = 0xFFFFFFFFu;
+            __syncthreads();
+            const uint32_t t0 = w0 + static_cast<uint32_t>(tid) * kFixRows;
+            uint32_t hd[kFixRows];
+#pragma unroll
+            for (int k = 0; k < kFixRows; ++k) hd[k] = t0 + k < tend ? a.tile_heads[t0 + k] : 1u;
+            uint32_t mine = 0xFFFFFFFFu;  // this thread's first stop
+#pragma unroll
+            for (int k = kFixRows - 1; k >= 0; --k)
+                if (hd[k] > 0) mine = t0 + k;
+            if (mine != 0xFFFFFFFFu) atomicMin(&s_stop, mine);
+            __syncthreads();
+            const uint32_t stop = s_stop;
+            // tiles w0 .. last join the group (the stop tile with its leading rows)
+            const uint32_t last = std::min(stop, tend - 1);
+            for (int c = 0; c < a.ncols; ++c) {
+                switch (a.cols[c].kind) {
+                case KIND_F64: fix_window<KIND_F64>(a, c, w0, last, s_part, s_acc[c]); break;
+                case KIND_SIGNED: fix_window<KIND_SIGNED>(a, c, w0, last, s_part, s_acc[c]); break;
+                case KIND_UNSIGNED: fix_window<KIND_UNSIGNED>(a, c, w0, last, s_part, s_acc[c]); break;
+                default: fix_window<KIND_COUNT>(a, c, w0, last, s_part, s_acc[c]); break;
+                }
             }
-            continue;
+            if (stop != 0xFFFFFFFFu) break;
         }
-        for (uint32_t t0 = b + 1; t0 < tend; t0 += kBlock) {
-            const uint32_t t = t0 + tid;
-            if (tid == 0) s_first = kBlock;
-            __syncthreads();
-            const bool stop = (t >= tend) || a.tile_heads[t] > 0;
-            if (stop) atomicMin(&s_first, static_cast<uint32_t>(tid));
-            __syncthreads();
-            const uint32_t first = s_first;
-            s_red[tid] = (t < tend && static_cast<uint32_t>(tid) <= first) ? pre[t] : agg_empty();
-            __syncthreads();
-            for (int s = 1; s < kBlock; s <<= 1) {
-                if ((tid & (2 * s - 1)) == 0) s_red[tid] = agg_join(s_red[tid], s_red[tid + s], kind);
-                __syncthreads();
-            }
-            acc = agg_join(acc, s_red[0], kind);
-            __syncthreads();
-            if (first < static_cast<uint32_t>(kBlock)) break;
-        }
-        if (tid == 0) write_final(a, col_out(a, sg, c), c, gid, acc);
+        if (tid == 0)
+            for (int c = 0; c < a.ncols; ++c) write_final(a, col_out(a, sg, c), c, gid, s_acc[c]);
+        __syncthreads();  // s_acc is set again for the next queued tile
     }
 }
 
@@ -848,8 +969,8 @@ size_t agg_bytes_for(uint64_t tiles, int ntables)
     size_t b = 0;
     b += align_up(static_cast<size_t>(ntables) * sizeof(GbSeg), 256);
     b += align_up(std::max<uint64_t>(tiles, 1) * 4, 256);      // tile -> table
-    b += align_up((tiles + 1) * 4, 256);                       // status + ticket
-    b += 2 * align_up(std::max<uint64_t>(tiles, 1) * 4, 256);  // tile_gbase, tile_heads
+    b += align_up((tiles + 2) * 4, 256);                       // status + ticket + long_count
+    b += 3 * align_up(std::max<uint64_t>(tiles, 1) * 4, 256);  // tile_gbase, tile_heads, long_list
     b += 2 * align_up(VDF_MAX_VALUE_COLS * std::max<uint64_t>(tiles, 1) * sizeof(Agg), 256);
     return b + 256;
 }
@@ -931,10 +1052,11 @@ void groupby_aggregate_enqueue(Handle* h, int key_type, const void* skeys, const
     Carve cv(scr);
     GbSeg* dsegs = cv.take<GbSeg>(ntables);
     uint32_t* dtile = cv.take<uint32_t>(std::max<uint64_t>(tiles_total, 1));
-    uint32_t* status = cv.take<uint32_t>(tiles_total + 1);
+    uint32_t* status = cv.take<uint32_t>(tiles_total + 2);
     GbArgs a{};
     a.tile_gbase = cv.take<uint32_t>(std::max<uint64_t>(tiles_total, 1));
     a.tile_heads = cv.take<uint32_t>(std::max<uint64_t>(tiles_total, 1));
+    a.long_list = cv.take<uint32_t>(std::max<uint64_t>(tiles_total, 1));
     a.prefix = cv.take<Agg>(VDF_MAX_VALUE_COLS * std::max<uint64_t>(tiles_total, 1));
     a.suffix = cv.take<Agg>(VDF_MAX_VALUE_COLS * std::max<uint64_t>(tiles_total, 1));
 
@@ -990,6 +1112,7 @@ void groupby_aggregate_enqueue(Handle* h, int key_type, const void* skeys, const
     a.agg_mask = agg_mask;
     a.status = status;
     a.ticket = status + tiles;
+    a.long_count = status + tiles + 1;
     uint64_t val_bytes = 0;
     if (nvals == 0) {
         a.ncols = 1;
@@ -1005,7 +1128,7 @@ void groupby_aggregate_enqueue(Handle* h, int key_type, const void* skeys, const
     }
     {
         LaunchTimer t(h, s, "fill", 0);
-        fill_u32(status, tiles + 1, 0u, s);
+        fill_u32(status, tiles + 2, 0u, s);
     }
     {
         // algorithmic bytes: sorted keys + per column its value (and row id when gathered)
@@ -1024,8 +1147,12 @@ void groupby_aggregate_enqueue(Handle* h, int key_type, const void* skeys, const
     }
     {
         LaunchTimer t(h, s, "groupby_fixup", 0);
-        hipLaunchKernelGGL(k_gb_fixup, dim3(a.ntiles), dim3(kBlock), 0, s, a);
+        hipLaunchKernelGGL(k_gb_fixup, dim3((a.ntiles + kBlock - 1) / kBlock), dim3(kBlock), 0, s, a);
         check_launch("k_gb_fixup");
+        // queued long chains (their count is on the device): blocks take
+        // queue entries in turn
+        hipLaunchKernelGGL(k_gb_fixup_long, dim3(std::min<uint32_t>(a.ntiles, 1024)), dim3(kBlock), 0, s, a);
+        check_launch("k_gb_fixup_long");
     }
 }
 
