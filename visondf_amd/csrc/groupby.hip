@@ -270,31 +270,79 @@ __device__ inline ColOut col_out(const GbArgs& a, const GbSeg& sg, int c)
     return o;
 }
 
-__device__ inline void write_final(const GbArgs& a, const ColOut& o, int c, uint32_t gid, const Agg& s)
+// The output words of one finished group: sum (f64 bits or the low 64 bits
+// of the integer sum), mean (f64 bits), min / max (raw), count.
+struct Final {
+    uint64_t sum, mean, mn, mx;
+    uint32_t cnt;
+};
+
+__device__ inline Final final_of(const GbArgs& a, int c, const Agg& s)
+{
+    Final f{0, 0, s.mn, s.mx, static_cast<uint32_t>(s.cnt)};
+    const GbColType col = a.cols[c];
+    if (col.kind == KIND_COUNT) return f;
+    if (col.kind == KIND_F64) {
+        const double sum = as_d(s.a) + as_d(s.b);
+        f.sum = as_u(sum);
+        if (a.agg_mask & VDF_AGG_MEAN) f.mean = as_u(sum / static_cast<double>(s.cnt));
+    } else {
+        f.sum = s.a;
+        if (a.agg_mask & VDF_AGG_MEAN) {
+            const double tot = (col.kind == KIND_SIGNED) ? i128_to_double(s.a, s.b) : u128_to_double(s.a, s.b);
+            f.mean = as_u(tot / static_cast<double>(s.cnt));
+        }
+    }
+    return f;
+}
+
+__device__ inline void emit_final(const GbArgs& a, const ColOut& o, int c, uint32_t gid, const Final& f)
 {
     if (gid >= o.cap) return;
     const GbColType col = a.cols[c];
-    if (o.counts) gmem(o.counts)[gid] = static_cast<uint32_t>(s.cnt);
+    if (o.counts) gmem(o.counts)[gid] = f.cnt;
     if (col.kind == KIND_COUNT) return;
     const bool w4 = (col.type == VDF_VAL_I32 || col.type == VDF_VAL_U32);
-    if (col.kind == KIND_F64) {
-        const double sum = as_d(s.a) + as_d(s.b);
-        if (o.sum) gmem(static_cast<double*>(o.sum))[gid] = sum;
-        if (o.mean) gmem(o.mean)[gid] = sum / static_cast<double>(s.cnt);
-    } else {
-        if (o.sum) gmem(static_cast<uint64_t*>(o.sum))[gid] = s.a;
-        if (o.mean) {
-            const double tot = (col.kind == KIND_SIGNED) ? i128_to_double(s.a, s.b) : u128_to_double(s.a, s.b);
-            gmem(o.mean)[gid] = tot / static_cast<double>(s.cnt);
-        }
-    }
+    if (o.sum) gmem(static_cast<uint64_t*>(o.sum))[gid] = f.sum;
+    if (o.mean) gmem(reinterpret_cast<uint64_t*>(o.mean))[gid] = f.mean;
     if (o.min) {
-        if (w4) gmem(static_cast<uint32_t*>(o.min))[gid] = static_cast<uint32_t>(s.mn);
-        else gmem(static_cast<uint64_t*>(o.min))[gid] = s.mn;
+        if (w4) gmem(static_cast<uint32_t*>(o.min))[gid] = static_cast<uint32_t>(f.mn);
+        else gmem(static_cast<uint64_t*>(o.min))[gid] = f.mn;
     }
     if (o.max) {
-        if (w4) gmem(static_cast<uint32_t*>(o.max))[gid] = static_cast<uint32_t>(s.mx);
-        else gmem(static_cast<uint64_t*>(o.max))[gid] = s.mx;
+        if (w4) gmem(static_cast<uint32_t*>(o.max))[gid] = static_cast<uint32_t>(f.mx);
+        else gmem(static_cast<uint64_t*>(o.max))[gid] = f.mx;
+    }
+}
+
+__device__ inline void write_final(const GbArgs& a, const ColOut& o, int c, uint32_t gid, const Agg& s)
+{
+    emit_final(a, o, c, gid, final_of(a, c, s));
+}
+
+// Finished groups of a tile are staged in LDS by tile-local group index and
+// written out afterwards by consecutive lanes, so each output array takes one
+// store instruction per 256 groups instead of one per row position that holds
+// a head somewhere in the wave.  Groups past kGbStage are written directly.
+constexpr int kGbStage = 1024;
+
+struct GbStage {
+    uint64_t a[kGbStage], b[kGbStage], mn[kGbStage], mx[kGbStage];  // the group's aggregate
+    uint32_t cnt[kGbStage];
+};
+
+__device__ inline void stage_final(GbStage* st, const GbArgs& a, const ColOut& o, int c, uint32_t gbase, uint32_t gid,
+                                   const Agg& s)
+{
+    const uint32_t h = gid - gbase;
+    if (h < static_cast<uint32_t>(kGbStage)) {
+        st->cnt[h] = static_cast<uint32_t>(s.cnt);  // a group inside one tile: at most 4096 rows
+        st->a[h] = s.a;
+        st->b[h] = s.b;
+        st->mn[h] = s.mn;
+        st->mx[h] = s.mx;
+    } else {
+        write_final(a, o, c, gid, s);
     }
 }
 
@@ -450,7 +498,8 @@ __device__ inline uint32_t head_bits(const K* keys, uint64_t r0, uint32_t cnt)
 // lane's 16 rows, wave segmented scan by shuffles, cross-wave carry through LDS.
 template <int KIND>
 __device__ __attribute__((always_inline)) inline void gb_column(const GbArgs& a, const GbSeg& sg, int c, uint32_t tile, uint64_t tbase, uint32_t cnt,
-                          uint32_t hb, uint32_t gfirst, Agg* s_wagg, uint32_t* s_wflag)
+                          uint32_t hb, uint32_t gfirst, uint32_t gbase, uint32_t H, Agg* s_wagg, uint32_t* s_wflag,
+                          GbStage* st)
 {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint64_t r0 = tbase + static_cast<uint64_t>(tid) * kGbRows;
@@ -514,7 +563,7 @@ __device__ __attribute__((always_inline)) inline void gb_column(const GbArgs& a,
                     first = cur;
                     seen = true;
                 } else {
-                    write_final(a, co, c, g - 1, cur);
+                    stage_final(st, a, co, c, gbase, g - 1, cur);
                 }
                 cur = agg_empty();
                 ++g;
@@ -584,7 +633,7 @@ __device__ __attribute__((always_inline)) inline void gb_column(const GbArgs& a,
     Agg* suf = a.suffix + static_cast<uint64_t>(c) * a.ntiles;
     if (seen) {
         const Agg seg = agg_join(carry, first, KIND);
-        if (cf) write_final(a, co, c, gfirst - 1, seg);
+        if (cf) stage_final(st, a, co, c, gbase, gfirst - 1, seg);
         else pre[tile] = seg;  // the tile's first head: rows before it
     }
     if (tid == kBlock - 1) {
@@ -597,7 +646,14 @@ __device__ __attribute__((always_inline)) inline void gb_column(const GbArgs& a,
         if (bf) suf[tile] = bx;
         else pre[tile] = bx;
     }
-    __syncthreads();  // s_wagg is reused by the next column
+    __syncthreads();  // every staged group is in LDS
+    // groups 0 .. H-2 finish in this tile (group H-1 runs into the next tiles)
+    const uint32_t nfin = H ? std::min<uint32_t>(H - 1, kGbStage) : 0u;
+    for (uint32_t h = tid; h < nfin; h += kBlock) {
+        const Agg g{st->cnt[h], st->a[h], st->b[h], st->mn[h], st->mx[h]};
+        write_final(a, co, c, gbase + h, g);
+    }
+    __syncthreads();  // s_wagg and the stage are reused by the next column
 }
 
 // What a block keeps of one claimed tile.
@@ -621,12 +677,13 @@ struct GbTileState {
 // order by running blocks and a block waits only on smaller tiles, so the
 // smallest unresolved tile always completes, at any residency.
 template <typename K>
-__global__ __launch_bounds__(kBlock) void k_gb_tile(GbArgs a)
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_gb_tile(GbArgs a)
 {
     __shared__ Agg s_wagg[kWaves];
     __shared__ uint32_t s_wflag[kWaves];
     __shared__ uint32_t s_wsum[kWaves];
     __shared__ uint32_t s_tile, s_gbase;
+    __shared__ GbStage s_stage;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 
@@ -767,16 +824,16 @@ __global__ __launch_bounds__(kBlock) void k_gb_tile(GbArgs a)
         for (int c = 0; c < a.ncols; ++c) {
             switch (a.cols[c].kind) {
             case KIND_F64:
-                gb_column<KIND_F64>(a, sg, c, t.tile, t.tbase, t.cnt, t.hb, gfirst, s_wagg, s_wflag);
+                gb_column<KIND_F64>(a, sg, c, t.tile, t.tbase, t.cnt, t.hb, gfirst, t.gbase, t.H, s_wagg, s_wflag, &s_stage);
                 break;
             case KIND_SIGNED:
-                gb_column<KIND_SIGNED>(a, sg, c, t.tile, t.tbase, t.cnt, t.hb, gfirst, s_wagg, s_wflag);
+                gb_column<KIND_SIGNED>(a, sg, c, t.tile, t.tbase, t.cnt, t.hb, gfirst, t.gbase, t.H, s_wagg, s_wflag, &s_stage);
                 break;
             case KIND_UNSIGNED:
-                gb_column<KIND_UNSIGNED>(a, sg, c, t.tile, t.tbase, t.cnt, t.hb, gfirst, s_wagg, s_wflag);
+                gb_column<KIND_UNSIGNED>(a, sg, c, t.tile, t.tbase, t.cnt, t.hb, gfirst, t.gbase, t.H, s_wagg, s_wflag, &s_stage);
                 break;
             default:
-                gb_column<KIND_COUNT>(a, sg, c, t.tile, t.tbase, t.cnt, t.hb, gfirst, s_wagg, s_wflag);
+                gb_column<KIND_COUNT>(a, sg, c, t.tile, t.tbase, t.cnt, t.hb, gfirst, t.gbase, t.H, s_wagg, s_wflag, &s_stage);
                 break;
             }
         }
