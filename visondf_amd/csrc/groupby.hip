@@ -331,6 +331,21 @@ struct GbStage {
     uint32_t cnt[kGbStage];
 };
 
+// a group key in the stage (16-byte keys use two words)
+__device__ inline void stage_key(GbStage* st, uint32_t h, uint32_t k) { st->a[h] = k; }
+__device__ inline void stage_key(GbStage* st, uint32_t h, uint64_t k) { st->a[h] = k; }
+__device__ inline void stage_key(GbStage* st, uint32_t h, const K16& k)
+{
+    st->a[h] = k.lo;
+    st->b[h] = k.hi;
+}
+template <typename K>
+__device__ inline K staged_key(const GbStage* st, uint32_t h)
+{
+    if constexpr (sizeof(K) == 16) return K{st->a[h], st->b[h]};
+    else return static_cast<K>(st->a[h]);
+}
+
 __device__ inline void stage_final(GbStage* st, const GbArgs& a, const ColOut& o, int c, uint32_t gbase, uint32_t gid,
                                    const Agg& s)
 {
@@ -799,27 +814,45 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                     }
                 }
             }
+            // offsets and keys of the tile's groups go through the LDS stage
+            // (tile-local index h = g - gbase) and out by consecutive lanes
+            K* gkeys = static_cast<K*>(sg.out.group_keys);
             uint32_t bits = t.hb;
             uint32_t g = gfirst;
             while (bits) {
                 const int k = __ffs(bits) - 1;
                 bits &= bits - 1;
                 const uint64_t r = t.r0 + k;
-                if (g < sg.cap) {
-                    gmem(sg.out.offsets)[g] = static_cast<uint32_t>(r);
-                    if (sg.out.group_keys) {
-                        if constexpr (sizeof(K) == 4) {
-                            uint32_t kk = kw[0];
+                K kk;
+                if (gkeys) {
+                    if constexpr (sizeof(K) == 4) {
+                        kk = kw[0];
 #pragma unroll
-                            for (int j = 1; j < kGbRows; ++j) kk = (k == j) ? kw[j] : kk;
-                            put_key(static_cast<K*>(sg.out.group_keys), g, kk);
-                        } else {
-                            put_key(static_cast<K*>(sg.out.group_keys), g, get_key(keys, r));
-                        }
+                        for (int j = 1; j < kGbRows; ++j) kk = (k == j) ? kw[j] : kk;
+                    } else {
+                        kk = get_key(keys, r);
                     }
+                }
+                const uint32_t h = g - t.gbase;
+                if (h < static_cast<uint32_t>(kGbStage)) {
+                    s_stage.cnt[h] = static_cast<uint32_t>(r);
+                    if (gkeys) stage_key(&s_stage, h, kk);
+                } else if (g < sg.cap) {
+                    gmem(sg.out.offsets)[g] = static_cast<uint32_t>(r);
+                    if (gkeys) put_key(gkeys, g, kk);
                 }
                 ++g;
             }
+            __syncthreads();
+            const uint32_t nst = std::min<uint32_t>(t.H, kGbStage);
+            for (uint32_t h = tid; h < nst; h += kBlock) {
+                const uint32_t gg = t.gbase + h;
+                if (gg < sg.cap) {
+                    gmem(sg.out.offsets)[gg] = s_stage.cnt[h];
+                    if (gkeys) put_key(gkeys, gg, staged_key<K>(&s_stage, h));
+                }
+            }
+            __syncthreads();  // the stage is reused by the columns
         }
         for (int c = 0; c < a.ncols; ++c) {
             switch (a.cols[c].kind) {
