@@ -254,8 +254,12 @@ vdf_status vdf_groupby_batch(vdf_handle* h, int key_type, const vdf_table* table
 
 /* ---- tuning table (per handle) ----
  * Knobs (unknown names or unsupported values -> VDF_ERR_INVALID):
- *   "sort_items_32"  rows per thread of a radix pass tile, 4-byte keys: 12, 16, 24
- *   "sort_items_64"  same for 8-byte keys: 8, 12, 16   (16-byte keys use 8)
+ *   "sort_items_32"  rows per thread of a radix pass tile, 4-byte keys: 0 (default: by
+ *                    payload width -- 32 without a payload, 24 with an 8-byte one, 12
+ *                    with a 4- or 16-byte one), 8, 12, 16, 24, 32; a size without a
+ *                    kernel for a sort's payload width uses that width's default
+ *   "sort_items_64"  same for 8-byte keys: 0 (default: 12; 8 with a 16-byte payload),
+ *                    8, 12, 16   (16-byte keys use 8)
  *   "sort_chunk"     consecutive tiles one tile group (one XCD) claims at a time: 1 .. 1024
  *   "sort_xcd"       1 (default): XCD-grouped tile order, passes launched cooperatively;
  *                    0: global tile order, plain launches (for handles whose sorts run
@@ -285,7 +289,7 @@ vdf_status vdf_tune_get(vdf_handle* h, const char* name, int64_t* value);
  * No reference counterpart (radix_sort_uint32_mt.inl:3-5 fixes the shape). */
 typedef struct vdf_tune_report {
     int trials;
-    int sort_items;          /* rows per thread of a pass tile for this key width */
+    int sort_items;          /* rows per thread of a pass tile for this key width (0: default) */
     int sort_blocks_per_cu;  /* 0 = every resident block */
     int sort_xcd;
     int sort_chunk;

@@ -147,6 +147,21 @@ def test_groupby_group_spans_fixup_windows(vdf):
     _check(vdf, keys, [datagen.normal01(21, 1, n), rng.integers(-(2**40), 2**40, n)], label="windows")
 
 
+@pytest.mark.parametrize("items", [8, 12, 16])
+def test_groupby_carried_pass_tiles(vdf, items):
+    """The group-by's passes carry a 16-byte payload (two 8-byte columns) under
+    every pass tile size, including the split-staging ones: results equal the
+    oracle's."""
+    key, vf, vi = datagen.zipf_groupby(3_000_017, seed=31)
+    old = vdf.tune_get("sort_items_32")
+    try:
+        vdf.tune("sort_items_32", items)
+        _check(vdf, key, [vf, vi], label=f"carried items {items}")
+        _check(vdf, key, [vf], label=f"carried f64 items {items}")
+    finally:
+        vdf.tune("sort_items_32", old)
+
+
 def test_groupby_every_row_distinct(vdf):
     n = 262_147
     keys = np.random.default_rng(1).permutation(n).astype(np.uint32)

@@ -143,12 +143,12 @@ def test_autotune_argsort_and_identity(vdf):
     after = vdf.argsort(dk, key_type=N.KEY_U32).cpu().numpy()
     assert np.array_equal(before, after)
     assert np.array_equal(after.view(np.uint32).astype(np.int64), oracle.argsort(keys, 0, 8).astype(np.int64))
-    for name, val in (("sort_items_32", 16), ("sort_blocks_per_cu", 0), ("sort_xcd", 1), ("sort_chunk", 64),
+    for name, val in (("sort_items_32", 0), ("sort_blocks_per_cu", 0), ("sort_xcd", 1), ("sort_chunk", 64),
                       ("hist_rows", 65536)):
         vdf.tune(name, val)
 
 
-@pytest.mark.parametrize("knob,values", [("sort_items_32", [12, 16, 24]), ("sort_blocks_per_cu", [1, 2, 0]),
+@pytest.mark.parametrize("knob,values", [("sort_items_32", [12, 16, 24, 32, 0]), ("sort_blocks_per_cu", [1, 2, 0]),
                                          ("sort_xcd", [0, 1]), ("hist_rows", [4096, 262144, 65536])])
 def test_results_independent_of_launch_shape(vdf, knob, values):
     tabs = [(_dev(k), [_dev(x) for x in vs]) for k, vs in _batch(4, [100_000, 1_000_003, 5000])]
@@ -177,6 +177,6 @@ def test_autotune_groupby_batch(vdf):
     torch.cuda.synchronize()
     for i, e in enumerate(eager):
         _same(pb.result(i), e, f"tuned table {i}")
-    for name, val in (("sort_items_32", 16), ("sort_blocks_per_cu", 0), ("sort_xcd", 1), ("sort_chunk", 64),
+    for name, val in (("sort_items_32", 0), ("sort_blocks_per_cu", 0), ("sort_xcd", 1), ("sort_chunk", 64),
                       ("hist_rows", 65536)):
         vdf.tune(name, val)

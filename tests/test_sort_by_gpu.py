@@ -103,6 +103,28 @@ def test_sort_by_independent_of_launch_shape(vdf, items, xcd):
     assert torch.equal(out.view(torch.int64), ref_out.view(torch.int64))
 
 
+@pytest.mark.parametrize("width", [4, 8])
+@pytest.mark.parametrize("items", [24])
+@pytest.mark.parametrize("mode", ["asc", "reverse"])
+def test_sort_by_split_staging(vdf, width, items, mode):
+    """Large pass tiles stage a tile one 8-byte column at a time (4-byte keys
+    with a payload); bit-exact against the oracle on every key set, the last
+    pass writing the permutation, sorted keys and column (reversed too)."""
+    old = vdf.tune_get("sort_items_32")
+    try:
+        vdf.tune("sort_items_32", items)
+        for i, (name, keys) in enumerate(_key_sets()):
+            if keys.dtype.itemsize != 4 or keys.ndim != 1:
+                continue
+            _say("sort_by split", items, width, mode, name, keys.shape)
+            _check(vdf, keys, _vals(keys.shape[0], width, 300 + i), mode)
+        for n in (1, 6143, 6144, 6145, 8191, 8193, 1_000_003):
+            keys = (datagen.splitmix64(n, 5, n) >> np.uint64(40)).astype(np.uint32)
+            _check(vdf, keys, _vals(n, width, n), mode)
+    finally:
+        vdf.tune("sort_items_32", old)
+
+
 def test_sort_by_rejects_bad_width(vdf):
     import torch
 

@@ -427,20 +427,28 @@ struct TileCtx {
 // Blocks of a pass kernel that fit one CU's 160 KiB of LDS; the kernel is
 // compiled to keep that many waves per SIMD (one wave of each block per SIMD),
 // so registers never cut occupancy below what LDS allows.
-template <typename K, int ITEMS, int PB>
+// SPLIT kernels stage a tile one 8-byte column at a time (LDS: 9 bytes per
+// row, the ninth the row's digit), so that tiles twice as large fit two blocks
+// per CU.
+template <typename K, int ITEMS, int PB, bool SPLIT>
 struct PassShape {
-    static constexpr int kLds = kBlock * ITEMS * static_cast<int>(sizeof(K) + 4 + PB) + kWaves * kRadix * 4 + kRadix * 4 + 64;
+    static constexpr int kLds =
+        (SPLIT ? kBlock * ITEMS * 9 : kBlock * ITEMS * static_cast<int>(sizeof(K) + 4 + PB)) + kWaves * kRadix * 4 +
+        kRadix * 4 + 64;
     // register room (bytes of rows a thread holds): the larger tiles and
     // payloads get fewer blocks, so that no instantiation spills
     static constexpr int kRowBytes = ITEMS * static_cast<int>(sizeof(K) + 4 + PB);
-    static constexpr int kCap = (ITEMS >= 24 || PB == 16 || kRowBytes > 160 || (PB != 0 && ITEMS >= 12)) ? 2 : 3;
+    static constexpr int kCap = (SPLIT || ITEMS >= 24 || PB == 16 || kRowBytes > 160 || (PB != 0 && ITEMS >= 12)) ? 2 : 3;
     static constexpr int kBlocksPerCu = (160 * 1024 / kLds) < kCap ? (160 * 1024 / kLds) : kCap;
 };
 
 // PB: payload bytes per row moved with the keys: 0, 4, 8, or 16 (two 8-byte
-// columns).
-template <typename K, int ITEMS, int PB>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PassShape<K, ITEMS, PB>::kBlocksPerCu)))
+// columns).  SPLIT (4-byte keys with a payload): the staged tile is written out
+// in phases -- (key, row id) records, then each payload column staged at the
+// same slots -- and tiles are processed one at a time (the payload registers
+// stay live until their phase).
+template <typename K, int ITEMS, int PB, bool SPLIT = false>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PassShape<K, ITEMS, PB, SPLIT>::kBlocksPerCu)))
 void k_pass(SortArgs a, uint32_t slot)
 {
     using KO = KeyOps<K>;
@@ -454,12 +462,14 @@ void k_pass(SortArgs a, uint32_t slot)
     // record (key low, row id high) per row, so a bucket run is written as one
     // stream of full 8-byte words instead of two half-size streams
     constexpr bool KV = std::is_same<K, uint32_t>::value;
+    static_assert(!SPLIT || (KV && PB != 0), "split staging is for 4-byte keys with a payload");
 
     __shared__ K s_keys[KV ? 1 : TILE];
     __shared__ uint32_t s_vals[KV ? 1 : TILE];
-    __shared__ uint64_t s_kv[KV ? TILE : 1];
-    __shared__ P s_pay[PB ? TILE : 1];
-    __shared__ uint64_t s_pay2[TWO ? TILE : 1];
+    __shared__ uint64_t s_kv[KV ? TILE : 1];  // SPLIT: every phase's column
+    __shared__ P s_pay[(PB && !SPLIT) ? TILE : 1];
+    __shared__ uint64_t s_pay2[(TWO && !SPLIT) ? TILE : 1];
+    __shared__ uint8_t s_dig[SPLIT ? TILE : 1];  // SPLIT: digit of each staged slot
     __shared__ uint32_t s_wc[kWaves][kRadix];
     __shared__ uint32_t s_gbase[kRadix];
     __shared__ uint32_t s_wsum[kWaves];
@@ -667,8 +677,8 @@ void k_pass(SortArgs a, uint32_t slot)
                     s_keys[p] = key[j];
                     s_vals[p] = val[j];
                 }
-                if constexpr (PB != 0) s_pay[p] = pay[j];
-                if constexpr (TWO) s_pay2[p] = pay2[j];
+                if constexpr (PB != 0 && !SPLIT) s_pay[p] = pay[j];
+                if constexpr (TWO && !SPLIT) s_pay2[p] = pay2[j];
             }
         }
         lds_barrier();
@@ -729,6 +739,77 @@ void k_pass(SortArgs a, uint32_t slot)
                 vv = s_vals[i];
             }
         };
+        if constexpr (SPLIT) {
+            const uint32_t nm1 = static_cast<uint32_t>(c.n - 1);
+            auto dest = [&](uint32_t p) { return (c.last && a.reverse) ? nm1 - p : p; };
+            // phase 1: (key, row id) records; each slot's digit is kept for
+            // the payload phases
+            K* kout = static_cast<K*>(a.kbuf[slot & 1]) + c.row_off;
+            auto* rout = gmem(static_cast<uint64_t*>(a.kbuf[slot & 1]) + c.row_off);
+            for (uint32_t i0 = tid; i0 < tn; i0 += U * kBlock) {
+                K kk[U];
+                uint32_t pos[U], vv[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) staged(i0 + u * kBlock, kk[u], vv[u]);
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const uint32_t d = KO::digit(kk[u], dig);
+                    s_dig[i0 + u * kBlock] = static_cast<uint8_t>(d);
+                    pos[u] = dest(s_gbase[d] + i0 + u * kBlock);
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    if (i0 + u * kBlock < tn) {
+                        const uint32_t p = pos[u];
+                        if (!c.last) {
+                            if (!a.norows) {
+                                if constexpr (KV) rout[p] = kk[u] | (static_cast<uint64_t>(vv[u]) << 32);
+                            } else {
+                                KO::put(kout, p, kk[u]);
+                            }
+                        } else {
+                            if (c.keys_out) KO::store(c.keys_out, p, kk[u], a.mlo, a.mhi);
+                            if (c.perm_out) {
+                                if (a.perm64)
+                                    gmem(static_cast<uint64_t*>(c.perm_out))[p] = vv[u];
+                                else
+                                    gmem(static_cast<uint32_t*>(c.perm_out))[p] = vv[u];
+                            }
+                        }
+                    }
+                }
+            }
+            // payload phases: the column is staged at its rows' slots and each
+            // slot goes to the position phase 1 gave it
+            const uint64_t wl = c.lrow0 + static_cast<uint64_t>(wave) * 64 * ITEMS + lane;
+            auto phase = [&](const auto(&col)[ITEMS], auto* mid, auto* fin) {
+                using T = std::remove_cv_t<std::remove_reference_t<decltype(col[0])>>;
+                lds_barrier();  // the previous phase's LDS reads are done
+#pragma unroll
+                for (int j = 0; j < ITEMS; ++j)
+                    if (wl + static_cast<uint64_t>(j) * 64 < c.n) s_kv[rnk[j]] = static_cast<uint64_t>(col[j]);
+                lds_barrier();
+                auto* out = c.last ? fin : mid;
+                for (uint32_t i0 = tid; i0 < tn; i0 += U * kBlock) {
+                    T v[U];
+                    uint32_t pos[U];
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        v[u] = static_cast<T>(s_kv[i0 + u * kBlock]);
+                        pos[u] = dest(s_gbase[s_dig[i0 + u * kBlock]] + i0 + u * kBlock);
+                    }
+#pragma unroll
+                    for (int u = 0; u < U; ++u)
+                        if (i0 + u * kBlock < tn) out[pos[u]] = v[u];
+                }
+            };
+            phase(pay, gmem(static_cast<P*>(a.pbuf[slot & 1]) + c.row_off), gmem(static_cast<P*>(c.pay_out)));
+            if constexpr (TWO)
+                phase(pay2, gmem(static_cast<uint64_t*>(a.pbuf2[slot & 1]) + c.row_off),
+                      gmem(static_cast<uint64_t*>(c.pay_out2)));
+            lds_barrier();
+            return;
+        }
         if (!c.last) {
             K* kout = static_cast<K*>(a.kbuf[slot & 1]) + c.row_off;
             auto* rout = gmem(static_cast<uint64_t*>(a.kbuf[slot & 1]) + c.row_off);
@@ -800,6 +881,16 @@ void k_pass(SortArgs a, uint32_t slot)
 
     // a slot beyond every segment's pass count has no tile to move
     if (slot >= load_relaxed(a.tickets + kMaxPassWord)) return;
+    if constexpr (SPLIT) {
+        TileCtx c;
+        while (next_tile(c)) {
+            uint32_t tot, ls;
+            load_rank(c, tot, ls);
+            stage(c);
+            resolve_write(c, tot, ls);
+        }
+        return;
+    }
     TileCtx c0, c1;
     if (!next_tile(c0)) return;
     uint32_t tot0, ls0;
@@ -916,18 +1007,22 @@ PassFn pick_pass<uint32_t>(int items, int pay_bytes)
         if (items == 12) return k_pass<uint32_t, 12, 0>;
         if (items == 16) return k_pass<uint32_t, 16, 0>;
         if (items == 24) return k_pass<uint32_t, 24, 0>;
+        if (items == 32) return k_pass<uint32_t, 32, 0>;
         return nullptr;
     case 4:
         if (items == 8) return k_pass<uint32_t, 8, 4>;
         if (items == 12) return k_pass<uint32_t, 12, 4>;
+        if (items == 24) return k_pass<uint32_t, 24, 4, true>;
         return nullptr;
     case 8:
         if (items == 12) return k_pass<uint32_t, 12, 8>;
         if (items == 16) return k_pass<uint32_t, 16, 8>;
+        if (items == 24) return k_pass<uint32_t, 24, 8, true>;
         return nullptr;
     case 16:
         if (items == 8) return k_pass<uint32_t, 8, 16>;
         if (items == 12) return k_pass<uint32_t, 12, 16>;
+        if (items == 16) return k_pass<uint32_t, 16, 16, true>;
         return nullptr;
     default: return nullptr;
     }
@@ -973,8 +1068,15 @@ PassFn pick_pass<K128>(int items, int pay_bytes)
 bool tune_supported(const char* name, int64_t v)
 {
     const std::string n(name);
-    if (n == "sort_items_32") return pick_pass<uint32_t>(static_cast<int>(v)) != nullptr;
-    if (n == "sort_items_64") return pick_pass<uint64_t>(static_cast<int>(v)) != nullptr;
+    // a tile size is accepted when some payload width has a kernel for it (a
+    // sort whose payload has none uses that payload's default size)
+    auto any_pass = [&](auto pick) {
+        for (int pb : {0, 4, 8, 16})
+            if (pick(static_cast<int>(v), pb) != nullptr) return true;
+        return false;
+    };
+    if (n == "sort_items_32") return v == 0 || any_pass(pick_pass<uint32_t>);
+    if (n == "sort_items_64") return v == 0 || any_pass(pick_pass<uint64_t>);
     if (n == "sort_chunk") return v >= 1 && v <= 1024;
     if (n == "sort_xcd") return v == 0 || v == 1;
     if (n == "sort_blocks_per_cu") return v >= 0 && v <= 32;
@@ -1020,15 +1122,31 @@ static unsigned resident_blocks(PassFn fn, int device)
     return blocks;
 }
 
+// Default rows per thread of a pass tile by key and payload width (measured
+// on MI355X, 100M-row sorts): the largest tile whose kernel keeps two blocks
+// per CU without spilling.
+template <typename K>
+static int default_items(int pay_bytes)
+{
+    if (sizeof(K) == 4) {
+        switch (pay_bytes) {
+        case 0: return 32;   // 8-byte (key, row id) records: 8192-row tiles in 64 KiB
+        case 8: return 24;   // split staging
+        default: return 12;  // 4- and 16-byte payloads
+        }
+    }
+    return sizeof(K) == 8 ? (pay_bytes == 16 ? 8 : 12) : 8;
+}
+
 template <typename K>
 static void launch_sort(Handle* h, const SortSeg* segs, int nsegs, uint64_t mlo, uint64_t mhi, uint32_t flags,
                         void* scr, hipStream_t s, int pay_bytes, bool rowids)
 {
     constexpr int D = KeyOps<K>::kDigits;
     int items = sizeof(K) == 4 ? h->tune.sort_items_32 : (sizeof(K) == 8 ? h->tune.sort_items_64 : 8);
-    PassFn pass = pick_pass<K>(items, pay_bytes);
-    if (!pass) {  // tuning value without an instantiation for this key width / payload
-        items = sizeof(K) == 4 ? (pay_bytes == 4 || pay_bytes == 16 ? 12 : 16) : 8;
+    PassFn pass = items ? pick_pass<K>(items, pay_bytes) : nullptr;
+    if (!pass) {  // the default, or a tuning value without a kernel for this key width / payload
+        items = default_items<K>(pay_bytes);
         pass = pick_pass<K>(items, pay_bytes);
     }
     require(pass != nullptr, "sort payload must be 0, 4, 8 or 16 bytes per row");

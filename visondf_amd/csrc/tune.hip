@@ -94,8 +94,10 @@ int descend(Handle* h, int kb, int reps, hipStream_t s, const std::function<void
     std::vector<Knob> knobs;
     if (kb == 4)
         knobs.push_back({&Tuning::sort_items_32,
-                         pay_bytes == 0 ? std::vector<int>{12, 16, 24}
-                                        : (pay_bytes == 4 ? std::vector<int>{8, 12} : std::vector<int>{12, 16})});
+                         pay_bytes == 0   ? std::vector<int>{16, 24, 32}
+                         : pay_bytes == 4 ? std::vector<int>{8, 12, 24}
+                         : pay_bytes == 8 ? std::vector<int>{12, 16, 24}
+                                          : std::vector<int>{8, 12, 16}});
     if (kb == 8)
         knobs.push_back({&Tuning::sort_items_64, pay_bytes == 0 ? std::vector<int>{8, 12, 16} : std::vector<int>{8, 12}});
     knobs.push_back({&Tuning::sort_bpc, {0, 1, 2, 3}});
