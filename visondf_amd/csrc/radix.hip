@@ -245,7 +245,8 @@ struct SortArgs {
 // (wave-uniform) were constant across a sample of the wave's keys; for those
 // the wave first checks whether every lane has the same digit and then adds
 // the count once, instead of 64 atomics on one LDS word.
-template <typename K>
+// DS: words between two digits' counters in `sh` (the lane's histogram copy).
+template <typename K, int DS>
 __device__ inline void hist_add(uint32_t* sh, const K& k, bool valid, uint32_t shared)
 {
     using KO = KeyOps<K>;
@@ -257,11 +258,11 @@ __device__ inline void hist_add(uint32_t* sh, const K& k, bool valid, uint32_t s
             const uint32_t b0 = __builtin_amdgcn_readfirstlane(b);
             if (__ballot(valid && b == b0) == vm) {
                 if (__lane_id() == static_cast<uint32_t>(__ffsll(static_cast<long long>(vm)) - 1))
-                    atomicAdd(&sh[d * kRadix + b0], static_cast<uint32_t>(__popcll(vm)));
+                    atomicAdd(&sh[d * DS + b0], static_cast<uint32_t>(__popcll(vm)));
                 continue;
             }
         }
-        if (valid) atomicAdd(&sh[d * kRadix + b], 1u);
+        if (valid) atomicAdd(&sh[d * DS + b], 1u);
     }
 }
 
@@ -273,12 +274,19 @@ __global__ __launch_bounds__(kBlock) void k_histogram(SortArgs a)
 {
     using KO = KeyOps<K>;
     constexpr int D = KO::kDigits;
-    __shared__ uint32_t sh[D * kRadix];
-    for (int j = threadIdx.x; j < D * kRadix; j += kBlock) sh[j] = 0;
+    // Each quarter of a wave counts into its own copy of the histogram, offset
+    // by 16 banks, so lanes with equal digits (narrow or skewed digits) spread
+    // over four addresses and banks.  16-byte keys keep one copy (LDS).
+    constexpr int C = D <= 8 ? 4 : 1;
+    constexpr int kRow = kRadix + 16;
+    constexpr int DS = C * kRow;
+    __shared__ uint32_t sh[D * DS];
+    for (int j = threadIdx.x; j < D * DS; j += kBlock) sh[j] = 0;
     __syncthreads();
     const HistItem it = a.items[blockIdx.x];
     const void* keys = a.segs[it.seg].keys;
     const uint32_t lane = threadIdx.x & 63;
+    uint32_t* mine = sh + ((lane >> 4) % C) * kRow;
     const uint64_t wave0 = it.begin + (threadIdx.x & ~63u);  // first row of this wave
     uint32_t shared = 0;  // digits constant over the wave's first 64 keys
     {
@@ -301,17 +309,20 @@ __global__ __launch_bounds__(kBlock) void k_histogram(SortArgs a)
 #pragma unroll
         for (int u = 0; u < U; ++u) k[u] = KO::fetch(keys, i0 + lane + u * kBlock);
 #pragma unroll
-        for (int u = 0; u < U; ++u) hist_add<K>(sh, KO::xform(k[u], a.mlo, a.mhi), true, shared);
+        for (int u = 0; u < U; ++u) hist_add<K, DS>(mine, KO::xform(k[u], a.mlo, a.mhi), true, shared);
     }
     for (; i0 < it.end; i0 += kBlock) {
         const uint64_t i = i0 + lane;
         const bool v = i < it.end;
-        hist_add<K>(sh, v ? KO::load(keys, i, a.mlo, a.mhi) : KO::zero(), v, shared);
+        hist_add<K, DS>(mine, v ? KO::load(keys, i, a.mlo, a.mhi) : KO::zero(), v, shared);
     }
     __syncthreads();
     uint32_t* h = a.hist + static_cast<uint64_t>(it.seg) * D * kRadix;
     for (int j = threadIdx.x; j < D * kRadix; j += kBlock) {
-        const uint32_t c = sh[j];
+        const int d = j / kRadix, b = j % kRadix;
+        uint32_t c = 0;
+#pragma unroll
+        for (int q = 0; q < C; ++q) c += sh[d * DS + q * kRow + b];
         if (c) atomicAdd(&h[j], c);
     }
     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
