@@ -259,7 +259,8 @@ vdf_status vdf_groupby_batch(vdf_handle* h, int key_type, const vdf_table* table
  *                    with a 4- or 16-byte one), 8, 12, 16, 24, 32; a size without a
  *                    kernel for a sort's payload width uses that width's default
  *   "sort_items_64"  same for 8-byte keys: 0 (default: 12; 8 with a 16-byte payload),
- *                    8, 12, 16   (16-byte keys use 8)
+ *                    8, 12, 16
+ *   "sort_items_128" same for 16-byte keys: 0 (default: 8), 8, 16
  *   "sort_chunk"     consecutive tiles one tile group (one XCD) claims at a time: 1 .. 1024
  *   "sort_xcd"       1 (default): XCD-grouped tile order, passes launched cooperatively;
  *                    0: global tile order, plain launches (for handles whose sorts run

@@ -162,6 +162,21 @@ def test_groupby_carried_pass_tiles(vdf, items):
         vdf.tune("sort_items_32", old)
 
 
+def test_groupby_wide_keys_split_tiles(vdf):
+    """u64 and char[16] keys with an f64 column carried through split-staged
+    16-row-per-thread passes: equal to the oracle."""
+    old = (vdf.tune_get("sort_items_64"), vdf.tune_get("sort_items_128"))
+    try:
+        vdf.tune("sort_items_64", 16)
+        vdf.tune("sort_items_128", 16)
+        for kind in ("u64", "char16"):
+            key, val = datagen.served_table(300_007, 3, kind)
+            _check(vdf, key, [val], label=f"split {kind}")
+    finally:
+        vdf.tune("sort_items_64", old[0])
+        vdf.tune("sort_items_128", old[1])
+
+
 def test_groupby_every_row_distinct(vdf):
     n = 262_147
     keys = np.random.default_rng(1).permutation(n).astype(np.uint32)

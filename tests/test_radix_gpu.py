@@ -96,6 +96,30 @@ def test_argsort_modes_vs_oracle(vdf, mode):
         assert np.array_equal(sk.view(np.uint8), keys[exp.astype(np.int64)].view(np.uint8)), name
 
 
+@pytest.mark.parametrize("mode", ["asc", "desc", "reverse"])
+def test_argsort_wide_keys_split_tiles(vdf, mode):
+    """8- and 16-byte keys with 16 rows per thread: 16-byte keys use split
+    staging (a key word per phase, then the row ids); bit-exact permutation and
+    sorted keys against the oracle."""
+    m = {"asc": 0, "desc": 1, "reverse": 2}[mode]
+    old = (vdf.tune_get("sort_items_64"), vdf.tune_get("sort_items_128"))
+    try:
+        vdf.tune("sort_items_64", 16)
+        vdf.tune("sort_items_128", 16)
+        for name, keys in _key_sets():
+            if keys.dtype.itemsize == 4 and keys.ndim == 1:
+                continue
+            _say("wide split", mode, name, keys.shape)
+            exp = oracle.argsort(keys, m, 4)
+            perm, skeys = vdf.argsort(_dev(keys), key_type=_key_type(keys), descending=(mode == "desc"),
+                                      reverse_desc=(mode == "reverse"), want_keys=True)
+            assert np.array_equal(_perm_host(perm), exp.astype(np.uint32)), name
+            assert np.array_equal(skeys.cpu().numpy().view(np.uint8), keys[exp.astype(np.int64)].view(np.uint8)), name
+    finally:
+        vdf.tune("sort_items_64", old[0])
+        vdf.tune("sort_items_128", old[1])
+
+
 def test_argsort_perm64(vdf):
     keys = datagen.groupby_small(50_001, seed=5)[0]
     perm = vdf.argsort(_dev(keys), perm64=True)

@@ -125,6 +125,24 @@ def test_sort_by_split_staging(vdf, width, items, mode):
         vdf.tune("sort_items_32", old)
 
 
+@pytest.mark.parametrize("mode", ["asc", "reverse"])
+def test_sort_by_wide_keys_split_tiles(vdf, mode):
+    """8- and 16-byte keys carrying an 8-byte column with 16 rows per thread
+    (split staging: key, row ids and the column in separate phases)."""
+    old = (vdf.tune_get("sort_items_64"), vdf.tune_get("sort_items_128"))
+    try:
+        vdf.tune("sort_items_64", 16)
+        vdf.tune("sort_items_128", 16)
+        for i, (name, keys) in enumerate(_key_sets()):
+            if keys.dtype.itemsize == 4 and keys.ndim == 1:
+                continue
+            _say("sort_by wide split", mode, name, keys.shape)
+            _check(vdf, keys, _vals(keys.shape[0], 8, 500 + i), mode)
+    finally:
+        vdf.tune("sort_items_64", old[0])
+        vdf.tune("sort_items_128", old[1])
+
+
 def test_sort_by_rejects_bad_width(vdf):
     import torch
 

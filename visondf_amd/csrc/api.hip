@@ -628,6 +628,7 @@ static int* tune_slot(Tuning& t, const std::string& n)
 {
     if (n == "sort_items_32") return &t.sort_items_32;
     if (n == "sort_items_64") return &t.sort_items_64;
+    if (n == "sort_items_128") return &t.sort_items_128;
     if (n == "sort_chunk") return &t.sort_chunk;
     if (n == "sort_xcd") return &t.sort_xcd;
     if (n == "sort_blocks_per_cu") return &t.sort_bpc;

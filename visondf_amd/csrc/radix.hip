@@ -454,10 +454,11 @@ struct PassShape {
 };
 
 // PB: payload bytes per row moved with the keys: 0, 4, 8, or 16 (two 8-byte
-// columns).  SPLIT (4-byte keys with a payload): the staged tile is written out
-// in phases -- (key, row id) records, then each payload column staged at the
-// same slots -- and tiles are processed one at a time (the payload registers
-// stay live until their phase).
+// columns).  SPLIT: the staged tile is written out in phases, one 8-byte
+// column at a time staged at the rows' slots -- (key, row id) records for
+// 4-byte keys; the key (8-byte keys) or its two words (16-byte keys) and the
+// row ids otherwise; then each payload column -- and tiles are processed one
+// at a time (the registers of a column stay live until its phase).
 template <typename K, int ITEMS, int PB, bool SPLIT = false>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PassShape<K, ITEMS, PB, SPLIT>::kBlocksPerCu)))
 void k_pass(SortArgs a, uint32_t slot)
@@ -473,11 +474,9 @@ void k_pass(SortArgs a, uint32_t slot)
     // record (key low, row id high) per row, so a bucket run is written as one
     // stream of full 8-byte words instead of two half-size streams
     constexpr bool KV = std::is_same<K, uint32_t>::value;
-    static_assert(!SPLIT || (KV && PB != 0), "split staging is for 4-byte keys with a payload");
-
-    __shared__ K s_keys[KV ? 1 : TILE];
-    __shared__ uint32_t s_vals[KV ? 1 : TILE];
-    __shared__ uint64_t s_kv[KV ? TILE : 1];  // SPLIT: every phase's column
+    __shared__ K s_keys[(KV || SPLIT) ? 1 : TILE];
+    __shared__ uint32_t s_vals[(KV || SPLIT) ? 1 : TILE];
+    __shared__ uint64_t s_kv[(KV || SPLIT) ? TILE : 1];  // SPLIT: every phase's column
     __shared__ P s_pay[(PB && !SPLIT) ? TILE : 1];
     __shared__ uint64_t s_pay2[(TWO && !SPLIT) ? TILE : 1];
     __shared__ uint8_t s_dig[SPLIT ? TILE : 1];  // SPLIT: digit of each staged slot
@@ -678,6 +677,14 @@ void k_pass(SortArgs a, uint32_t slot)
         // valid items are written
 #pragma unroll
         for (int j = 0; j < ITEMS; ++j) rnk[j] += s_wc[wave][KO::digit(key[j], dig)];
+        if constexpr (SPLIT) {
+            // the columns are staged by phases at write-out; here each slot's digit
+#pragma unroll
+            for (int j = 0; j < ITEMS; ++j)
+                if (wl + static_cast<uint64_t>(j) * 64 < c.n) s_dig[rnk[j]] = static_cast<uint8_t>(KO::digit(key[j], dig));
+            lds_barrier();
+            return;
+        }
 #pragma unroll
         for (int j = 0; j < ITEMS; ++j) {
             if (wl + static_cast<uint64_t>(j) * 64 < c.n) {
@@ -751,73 +758,96 @@ void k_pass(SortArgs a, uint32_t slot)
             }
         };
         if constexpr (SPLIT) {
+            // Phases: one 8-byte column of the tile at a time is staged at its
+            // rows' slots and each slot goes to the position its digit gives.
             const uint32_t nm1 = static_cast<uint32_t>(c.n - 1);
-            auto dest = [&](uint32_t p) { return (c.last && a.reverse) ? nm1 - p : p; };
-            // phase 1: (key, row id) records; each slot's digit is kept for
-            // the payload phases
-            K* kout = static_cast<K*>(a.kbuf[slot & 1]) + c.row_off;
-            auto* rout = gmem(static_cast<uint64_t*>(a.kbuf[slot & 1]) + c.row_off);
-            for (uint32_t i0 = tid; i0 < tn; i0 += U * kBlock) {
-                K kk[U];
-                uint32_t pos[U], vv[U];
-#pragma unroll
-                for (int u = 0; u < U; ++u) staged(i0 + u * kBlock, kk[u], vv[u]);
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const uint32_t d = KO::digit(kk[u], dig);
-                    s_dig[i0 + u * kBlock] = static_cast<uint8_t>(d);
-                    pos[u] = dest(s_gbase[d] + i0 + u * kBlock);
-                }
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    if (i0 + u * kBlock < tn) {
-                        const uint32_t p = pos[u];
-                        if (!c.last) {
-                            if (!a.norows) {
-                                if constexpr (KV) rout[p] = kk[u] | (static_cast<uint64_t>(vv[u]) << 32);
-                            } else {
-                                KO::put(kout, p, kk[u]);
-                            }
-                        } else {
-                            if (c.keys_out) KO::store(c.keys_out, p, kk[u], a.mlo, a.mhi);
-                            if (c.perm_out) {
-                                if (a.perm64)
-                                    gmem(static_cast<uint64_t*>(c.perm_out))[p] = vv[u];
-                                else
-                                    gmem(static_cast<uint32_t*>(c.perm_out))[p] = vv[u];
-                            }
-                        }
-                    }
-                }
-            }
-            // payload phases: the column is staged at its rows' slots and each
-            // slot goes to the position phase 1 gave it
             const uint64_t wl = c.lrow0 + static_cast<uint64_t>(wave) * 64 * ITEMS + lane;
-            auto phase = [&](const auto(&col)[ITEMS], auto* mid, auto* fin) {
-                using T = std::remove_cv_t<std::remove_reference_t<decltype(col[0])>>;
+            auto phase = [&](auto get, auto put) {
                 lds_barrier();  // the previous phase's LDS reads are done
 #pragma unroll
                 for (int j = 0; j < ITEMS; ++j)
-                    if (wl + static_cast<uint64_t>(j) * 64 < c.n) s_kv[rnk[j]] = static_cast<uint64_t>(col[j]);
+                    if (wl + static_cast<uint64_t>(j) * 64 < c.n) s_kv[rnk[j]] = get(j);
                 lds_barrier();
-                auto* out = c.last ? fin : mid;
                 for (uint32_t i0 = tid; i0 < tn; i0 += U * kBlock) {
-                    T v[U];
+                    uint64_t v[U];
                     uint32_t pos[U];
 #pragma unroll
                     for (int u = 0; u < U; ++u) {
-                        v[u] = static_cast<T>(s_kv[i0 + u * kBlock]);
-                        pos[u] = dest(s_gbase[s_dig[i0 + u * kBlock]] + i0 + u * kBlock);
+                        v[u] = s_kv[i0 + u * kBlock];
+                        const uint32_t p = s_gbase[s_dig[i0 + u * kBlock]] + i0 + u * kBlock;
+                        pos[u] = (c.last && a.reverse) ? nm1 - p : p;
                     }
 #pragma unroll
                     for (int u = 0; u < U; ++u)
-                        if (i0 + u * kBlock < tn) out[pos[u]] = v[u];
+                        if (i0 + u * kBlock < tn) put(pos[u], v[u]);
                 }
             };
-            phase(pay, gmem(static_cast<P*>(a.pbuf[slot & 1]) + c.row_off), gmem(static_cast<P*>(c.pay_out)));
-            if constexpr (TWO)
-                phase(pay2, gmem(static_cast<uint64_t*>(a.pbuf2[slot & 1]) + c.row_off),
-                      gmem(static_cast<uint64_t*>(c.pay_out2)));
+            auto row_ids = [&]() {  // the row ids (permutation in the last pass)
+                if (a.norows) return;
+                auto* vout = gmem(a.vbuf[slot & 1] + c.row_off);
+                phase([&](int j) { return static_cast<uint64_t>(val[j]); },
+                      [&](uint32_t p, uint64_t v) {
+                          if (!c.last) vout[p] = static_cast<uint32_t>(v);
+                          else if (c.perm_out) {
+                              if (a.perm64) gmem(static_cast<uint64_t*>(c.perm_out))[p] = v;
+                              else gmem(static_cast<uint32_t*>(c.perm_out))[p] = static_cast<uint32_t>(v);
+                          }
+                      });
+            };
+            if constexpr (KV) {
+                // (key, row id) records
+                K* kout = static_cast<K*>(a.kbuf[slot & 1]) + c.row_off;
+                auto* rout = gmem(static_cast<uint64_t*>(a.kbuf[slot & 1]) + c.row_off);
+                phase([&](int j) { return key[j] | (static_cast<uint64_t>(val[j]) << 32); },
+                      [&](uint32_t p, uint64_t r) {
+                          const uint32_t k = static_cast<uint32_t>(r), v = static_cast<uint32_t>(r >> 32);
+                          if (!c.last) {
+                              if (!a.norows) rout[p] = r;
+                              else KO::put(kout, p, k);
+                          } else {
+                              if (c.keys_out) KO::store(c.keys_out, p, k, a.mlo, a.mhi);
+                              if (c.perm_out) {
+                                  if (a.perm64) gmem(static_cast<uint64_t*>(c.perm_out))[p] = v;
+                                  else gmem(static_cast<uint32_t*>(c.perm_out))[p] = v;
+                              }
+                          }
+                      });
+            } else if constexpr (sizeof(K) == 8) {
+                K* kout = static_cast<K*>(a.kbuf[slot & 1]) + c.row_off;
+                phase([&](int j) { return static_cast<uint64_t>(key[j]); },
+                      [&](uint32_t p, uint64_t k) {
+                          if (!c.last) KO::put(kout, p, k);
+                          else if (c.keys_out) KO::store(c.keys_out, p, k, a.mlo, a.mhi);
+                      });
+                row_ids();
+            } else {
+                // 16-byte keys, a word per phase: scratch holds (lo, hi); the
+                // caller's form is bytes 0..7 = bswap(hi), 8..15 = bswap(lo)
+                auto* kw = gmem(static_cast<uint64_t*>(a.kbuf[slot & 1])) + 2 * c.row_off;
+                auto* ow = gmem(static_cast<uint64_t*>(c.keys_out));
+                phase([&](int j) { return key[j].lo; },
+                      [&](uint32_t p, uint64_t x) {
+                          if (!c.last) kw[2 * static_cast<uint64_t>(p)] = x;
+                          else if (ow) ow[2 * static_cast<uint64_t>(p) + 1] = __builtin_bswap64(x ^ a.mlo);
+                      });
+                phase([&](int j) { return key[j].hi; },
+                      [&](uint32_t p, uint64_t x) {
+                          if (!c.last) kw[2 * static_cast<uint64_t>(p) + 1] = x;
+                          else if (ow) ow[2 * static_cast<uint64_t>(p)] = __builtin_bswap64(x ^ a.mhi);
+                      });
+                row_ids();
+            }
+            if constexpr (PB != 0) {
+                auto* mid = gmem(static_cast<P*>(a.pbuf[slot & 1]) + c.row_off);
+                auto* fin = gmem(static_cast<P*>(c.pay_out));
+                phase([&](int j) { return static_cast<uint64_t>(pay[j]); },
+                      [&](uint32_t p, uint64_t v) { (c.last ? fin : mid)[p] = static_cast<P>(v); });
+            }
+            if constexpr (TWO) {
+                auto* mid = gmem(static_cast<uint64_t*>(a.pbuf2[slot & 1]) + c.row_off);
+                auto* fin = gmem(static_cast<uint64_t*>(c.pay_out2));
+                phase([&](int j) { return pay2[j]; }, [&](uint32_t p, uint64_t v) { (c.last ? fin : mid)[p] = v; });
+            }
             lds_barrier();
             return;
         }
@@ -1055,9 +1085,11 @@ PassFn pick_pass<uint64_t>(int items, int pay_bytes)
     case 8:
         if (items == 8) return k_pass<uint64_t, 8, 8>;
         if (items == 12) return k_pass<uint64_t, 12, 8>;
+        if (items == 16) return k_pass<uint64_t, 16, 8, true>;
         return nullptr;
     case 16:
         if (items == 8) return k_pass<uint64_t, 8, 16>;
+        if (items == 16) return k_pass<uint64_t, 16, 16, true>;
         return nullptr;
     default: return nullptr;
     }
@@ -1066,14 +1098,24 @@ PassFn pick_pass<uint64_t>(int items, int pay_bytes)
 template <>
 PassFn pick_pass<K128>(int items, int pay_bytes)
 {
-    if (items != 8) return nullptr;
-    switch (pay_bytes) {
-    case 0: return k_pass<K128, 8, 0>;
-    case 4: return k_pass<K128, 8, 4>;
-    case 8: return k_pass<K128, 8, 8>;
-    case 16: return k_pass<K128, 8, 16>;
-    default: return nullptr;
+    if (items == 8) {
+        switch (pay_bytes) {
+        case 0: return k_pass<K128, 8, 0>;
+        case 4: return k_pass<K128, 8, 4>;
+        case 8: return k_pass<K128, 8, 8>;
+        case 16: return k_pass<K128, 8, 16>;
+        default: return nullptr;
+        }
     }
+    if (items == 16) {
+        switch (pay_bytes) {
+        case 0: return k_pass<K128, 16, 0, true>;
+        case 4: return k_pass<K128, 16, 4, true>;
+        case 8: return k_pass<K128, 16, 8, true>;
+        default: return nullptr;
+        }
+    }
+    return nullptr;
 }
 
 bool tune_supported(const char* name, int64_t v)
@@ -1088,6 +1130,7 @@ bool tune_supported(const char* name, int64_t v)
     };
     if (n == "sort_items_32") return v == 0 || any_pass(pick_pass<uint32_t>);
     if (n == "sort_items_64") return v == 0 || any_pass(pick_pass<uint64_t>);
+    if (n == "sort_items_128") return v == 0 || any_pass(pick_pass<K128>);
     if (n == "sort_chunk") return v >= 1 && v <= 1024;
     if (n == "sort_xcd") return v == 0 || v == 1;
     if (n == "sort_blocks_per_cu") return v >= 0 && v <= 32;
@@ -1154,7 +1197,7 @@ static void launch_sort(Handle* h, const SortSeg* segs, int nsegs, uint64_t mlo,
                         void* scr, hipStream_t s, int pay_bytes, bool rowids)
 {
     constexpr int D = KeyOps<K>::kDigits;
-    int items = sizeof(K) == 4 ? h->tune.sort_items_32 : (sizeof(K) == 8 ? h->tune.sort_items_64 : 8);
+    int items = sizeof(K) == 4 ? h->tune.sort_items_32 : (sizeof(K) == 8 ? h->tune.sort_items_64 : h->tune.sort_items_128);
     PassFn pass = items ? pick_pass<K>(items, pay_bytes) : nullptr;
     if (!pass) {  // the default, or a tuning value without a kernel for this key width / payload
         items = default_items<K>(pay_bytes);

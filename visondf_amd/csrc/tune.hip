@@ -99,7 +99,10 @@ int descend(Handle* h, int kb, int reps, hipStream_t s, const std::function<void
                          : pay_bytes == 8 ? std::vector<int>{12, 16, 24}
                                           : std::vector<int>{8, 12, 16}});
     if (kb == 8)
-        knobs.push_back({&Tuning::sort_items_64, pay_bytes == 0 ? std::vector<int>{8, 12, 16} : std::vector<int>{8, 12}});
+        knobs.push_back({&Tuning::sort_items_64, pay_bytes == 0   ? std::vector<int>{8, 12, 16}
+                                                 : pay_bytes == 4 ? std::vector<int>{8, 12}
+                                                                  : std::vector<int>{8, 12, 16}});
+    if (kb == 16) knobs.push_back({&Tuning::sort_items_128, {8, 16}});
     knobs.push_back({&Tuning::sort_bpc, {0, 1, 2, 3}});
     knobs.push_back({&Tuning::sort_xcd, {1, 0}});
     knobs.push_back({&Tuning::sort_chunk, {4, 16, 64, 256}});
@@ -127,7 +130,7 @@ int descend(Handle* h, int kb, int reps, hipStream_t s, const std::function<void
         }
         h->tune = best;
         rep->best_ms = best_ms;
-        rep->sort_items = kb == 4 ? best.sort_items_32 : (kb == 8 ? best.sort_items_64 : 8);
+        rep->sort_items = kb == 4 ? best.sort_items_32 : (kb == 8 ? best.sort_items_64 : best.sort_items_128);
         rep->sort_blocks_per_cu = best.sort_bpc;
         rep->sort_xcd = best.sort_xcd;
         rep->sort_chunk = best.sort_chunk;
