@@ -406,7 +406,7 @@ struct TileCtx {
 };
 
 // One radix scatter pass over the tiles of every segment, persistent and
-// software-pipelined.
+// software-pipelined (SPLIT kernels, below, take one tile at a time).
 //
 // Tile claiming.  Block b belongs to group g = b % groups; dispatch puts
 // consecutive blocks on consecutive XCDs, so a group is one XCD.  The k-th
@@ -439,8 +439,7 @@ struct TileCtx {
 // compiled to keep that many waves per SIMD (one wave of each block per SIMD),
 // so registers never cut occupancy below what LDS allows.
 // SPLIT kernels stage a tile one 8-byte column at a time (LDS: 9 bytes per
-// row, the ninth the row's digit), so that tiles twice as large fit two blocks
-// per CU.
+// row, the ninth the row's digit), so that larger tiles fit two blocks per CU.
 template <typename K, int ITEMS, int PB, bool SPLIT>
 struct PassShape {
     static constexpr int kLds =
