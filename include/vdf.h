@@ -92,6 +92,7 @@ typedef int vdf_status;
 #define VDF_MAX_ROWS        ((1ull << 30) - 1ull)
 #define VDF_MAX_VALUE_COLS  4
 #define VDF_MAX_KEY_COLS    4
+#define VDF_MAX_STRING_BYTES 4096
 
 /* ---- library / handle ---- */
 const char* vdf_last_error(void);
@@ -115,7 +116,9 @@ vdf_status vdf_argsort(vdf_handle* h, int key_type, const void* d_keys, uint64_t
                        void* stream);
 
 /* ---- permutation gather: d_dst[r] = d_src[d_perm[r]] ----
- * elem_bytes in {4, 8, 16}; perm is uint32_t[n] or (perm64) uint64_t[n].
+ * elem_bytes 1 .. VDF_MAX_STRING_BYTES (4, 8 and 16 have vectorised kernels;
+ * other widths, e.g. padded strings, are copied row by row); perm is
+ * uint32_t[n] or (perm64) uint64_t[n].
  * Replaces permute_block_mt.inl:39-41 for one column. */
 vdf_status vdf_gather(vdf_handle* h, uint32_t elem_bytes, const void* d_src,
                       const void* d_perm, int perm64, uint64_t n, void* d_dst,
@@ -194,6 +197,16 @@ vdf_status vdf_groupby_multi(vdf_handle* h, const vdf_key_col* keys, int nkeys, 
  * applies to the whole key). */
 vdf_status vdf_argsort_multi(vdf_handle* h, const vdf_key_col* keys, int nkeys, uint64_t n, uint32_t flags,
                              void* d_perm_out, void* stream);
+
+/* Stable argsort of n fixed-width byte strings: `width` bytes per row
+ * (1 .. VDF_MAX_STRING_BYTES), rows contiguous, shorter strings padded by the
+ * caller (the reference pads with PaddingChar = 0x00).  Order: bytewise as
+ * signed chars (each byte ^ 0x80), first byte most significant -- the order of
+ * sort_string (sort_string.inl:10-85, with radix_sort_charbuf_flat.inl).
+ * Flags as vdf_argsort.  d_sorted_out (device, n * width bytes; nullable)
+ * receives the rows in sorted order. */
+vdf_status vdf_argsort_bytes(vdf_handle* h, const void* d_keys, uint32_t width, uint64_t n, uint32_t flags,
+                             void* d_perm_out, void* d_sorted_out, void* stream);
 
 /* ---- cached grouping ----
  * Sort one key column once and aggregate value columns against it any number

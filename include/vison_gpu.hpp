@@ -127,13 +127,16 @@ template <> struct TypeCode<double> { static constexpr int key = VDF_KEY_F64, va
 template <> struct TypeCode<float> { static constexpr int key = VDF_KEY_F32, val = -1; static constexpr char letter = 'd'; };
 template <> struct TypeCode<Char16> { static constexpr int key = VDF_KEY_CHAR16, val = -1; static constexpr char letter = 'c'; };
 
-// A device column: element size, key / value type codes, name.
+// A device column: element size, key / value type codes, name.  A string
+// column holds its strings padded with 0x00 to the longest (elem bytes each),
+// as the reference's flat buffer for sorting strings (sort_string.inl:30-85).
 struct Column {
     std::string name;
     size_t elem = 0;
     int key_type = -1;
     int val_type = -1;
     char letter = '?';
+    bool str = false;
     DeviceBuffer data;
 };
 
@@ -178,6 +181,45 @@ class Dataframe {
         cols_.push_back(std::move(c));
     }
 
+    // std::string column: stored padded (PaddingChar 0x00); sort_by orders it
+    // bytewise as signed chars, as sort_string does.
+    void add_col(const std::vector<std::string>& host, const std::string& name)
+    {
+        if (!cols_.empty() && host.size() != nrow_) throw Error(VDF_ERR_INVALID, "add_col: row count mismatch");
+        size_t w = 1;
+        for (const std::string& x : host) w = std::max(w, x.size());
+        if (w > VDF_MAX_STRING_BYTES) throw Error(VDF_ERR_INVALID, "add_col: string longer than VDF_MAX_STRING_BYTES");
+        std::vector<char> flat(host.size() * w, 0);
+        for (size_t i = 0; i < host.size(); ++i) std::memcpy(flat.data() + i * w, host[i].data(), host[i].size());
+        Column c;
+        c.name = name;
+        c.elem = w;
+        c.letter = 's';
+        c.str = true;
+        c.data = DeviceBuffer(flat.size());
+        if (!flat.empty())
+            hip_check(hipMemcpy(c.data.get(), flat.data(), flat.size(), hipMemcpyHostToDevice), "hipMemcpy H2D");
+        nrow_ = static_cast<unsigned int>(host.size());
+        cols_.push_back(std::move(c));
+    }
+
+    // the strings of a string column (trailing padding removed)
+    std::vector<std::string> get_str_col(unsigned int n) const
+    {
+        const Column& c = col(n);
+        if (!c.str) throw Error(VDF_ERR_INVALID, "get_str_col: not a string column");
+        std::vector<char> flat(static_cast<size_t>(nrow_) * c.elem);
+        if (!flat.empty())
+            hip_check(hipMemcpy(flat.data(), c.data.get(), flat.size(), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+        std::vector<std::string> out(nrow_);
+        for (size_t i = 0; i < nrow_; ++i) {
+            size_t len = c.elem;
+            while (len > 0 && flat[i * c.elem + len - 1] == 0) --len;
+            out[i].assign(flat.data() + i * c.elem, len);
+        }
+        return out;
+    }
+
     template <typename T>
     std::vector<T> get_col(unsigned int n) const
     {
@@ -199,11 +241,15 @@ class Dataframe {
     {
         if (!Soft && in_view_) throw Error(VDF_ERR_INVALID, "sort_by: materialize() the view first");
         const Column& k = col(n);
-        if (k.key_type < 0) throw Error(VDF_ERR_UNSUPPORTED, "sort_by: column type has no radix key path");
+        if (k.key_type < 0 && !k.str) throw Error(VDF_ERR_UNSUPPORTED, "sort_by: column type has no radix key path");
         if (nrow_ == 0) return;
         DeviceBuffer perm(static_cast<size_t>(nrow_) * 4);
-        check(vdf_argsort(h_.get(), k.key_type, k.data.get(), nrow_, ASC ? 0u : VDF_SORT_DESC_REVERSE, perm.get(),
-                          nullptr, nullptr));
+        if (k.str)  // sort_string (sort_by_mt.hpp:72-115 dispatch)
+            check(vdf_argsort_bytes(h_.get(), k.data.get(), static_cast<uint32_t>(k.elem), nrow_,
+                                    ASC ? 0u : VDF_SORT_DESC_REVERSE, perm.get(), nullptr, nullptr));
+        else
+            check(vdf_argsort(h_.get(), k.key_type, k.data.get(), nrow_, ASC ? 0u : VDF_SORT_DESC_REVERSE, perm.get(),
+                              nullptr, nullptr));
         if constexpr (Soft) {
             std::vector<uint32_t> p(nrow_);
             hip_check(hipMemcpy(p.data(), perm.get(), nrow_ * 4ull, hipMemcpyDeviceToHost), "hipMemcpy D2H");

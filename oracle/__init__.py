@@ -96,6 +96,22 @@ def argsort(keys: np.ndarray, mode: int = 0, threads: int = 1) -> np.ndarray:
     return perm
 
 
+def argsort_bytes(keys: np.ndarray, mode: int = 0) -> np.ndarray:
+    """Stable argsort of fixed-width byte strings ((n, width) uint8, padded):
+    each byte ^ 0x80 (signed char order), then the byte-wise LSD from the last
+    byte to the first, as sort_string.inl:30-85 + radix_sort_charbuf_flat.inl:
+    4-107 (Simd=false); mode as argsort."""
+    keys = np.ascontiguousarray(keys, dtype=np.uint8)
+    n, width = keys.shape
+    flat = keys ^ np.uint8(0x80 ^ (0xFF if mode == 1 else 0))
+    perm = np.arange(n, dtype=np.uint64)
+    if n and width:
+        lib().orc_radix_sort_charbuf(_p(flat), n, width, _p(perm))
+    if mode == 2:
+        perm = perm[::-1].copy()
+    return perm
+
+
 def gather(src: np.ndarray, perm: np.ndarray, threads: int = 1) -> np.ndarray:
     src = np.ascontiguousarray(src)
     perm = np.ascontiguousarray(perm, dtype=np.uint64)

@@ -253,6 +253,45 @@ int main()
         for (size_t i = 0; i < n && ok; ++i) ok = vi[i] == f.vi[idx[i]];
         CHECK(ok);
     }
+    // std::string key column: padded with 0x00 to the longest, ordered bytewise
+    // as signed chars (sort_string); ascending is stable, descending reverses it
+    std::printf("sort_by std::string\n");
+    {
+        std::vector<std::string> s = {"pear", "apple", "", "apple pie", "Zebra", "apple", "\xc3\xa9" "clair", "pea",
+                                      "pear", "a"};
+        std::vector<int32_t> tag(s.size());
+        for (size_t i = 0; i < s.size(); ++i) tag[i] = static_cast<int32_t>(i);
+        auto key_of = [](const std::string& x) {
+            std::vector<int> k;
+            for (unsigned char ch : x) k.push_back(static_cast<signed char>(ch));
+            return k;  // shorter prefix first: padding 0x00 is above the negative bytes only
+        };
+        std::vector<size_t> idx(s.size());
+        for (size_t i = 0; i < idx.size(); ++i) idx[i] = i;
+        const size_t w = 9;  // the longest string
+        auto padded = [&](size_t i) {
+            std::vector<int> k = key_of(s[i]);
+            k.resize(w, 0);
+            return k;
+        };
+        std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return padded(a) < padded(b); });
+        for (int mode = 0; mode < 2; ++mode) {
+            Dataframe df;
+            df.add_col(s, "s");
+            df.add_col(tag, "tag");
+            unsigned int c = 0;
+            if (mode == 0) df.sort_by<true, false>(c);
+            else df.sort_by<false, false>(c);
+            const auto got = df.get_str_col(0);
+            const auto t = df.get_col<int32_t>(1);
+            bool ok = true;
+            for (size_t i = 0; i < s.size() && ok; ++i) {
+                const size_t e = mode == 0 ? idx[i] : idx[s.size() - 1 - i];
+                ok = t[i] == static_cast<int32_t>(e) && got[i] == s[e];
+            }
+            CHECK(ok);
+        }
+    }
     // errors surface as exceptions with the library's message
     {
         Dataframe df = load(f);

@@ -160,3 +160,52 @@ def test_cached_grouping_equals_groupby(vdf, kind):
             for k in da:
                 assert np.array_equal(_bytes(da[k]), _bytes(db[k])), (kind, c, k)
     g.close()
+
+
+# ------------------------------------------------------- fixed-width strings
+def _strings(n: int, width: int, seed: int) -> np.ndarray:
+    """n padded byte strings of up to `width` bytes: lengths vary (padding
+    0x00, as the reference's PaddingChar), bytes from a small alphabet with
+    signed-char extremes, so ties and shared prefixes are common."""
+    rng = np.random.default_rng(seed)
+    alphabet = np.array([0x00, 0x01, 0x41, 0x42, 0x7F, 0x80, 0xC3, 0xFF], dtype=np.uint8)
+    keys = alphabet[rng.integers(0, alphabet.size, (n, width))]
+    lens = rng.integers(0, width + 1, n)
+    keys[np.arange(width)[None, :] >= lens[:, None]] = 0
+    return keys
+
+
+@pytest.mark.parametrize("width", [1, 3, 16, 17, 32, 33, 70, 255])
+@pytest.mark.parametrize("mode", ["asc", "desc", "reverse"])
+def test_argsort_bytes_vs_oracle(vdf, width, mode):
+    import torch
+
+    m = {"asc": 0, "desc": 1, "reverse": 2}[mode]
+    for n in (1, 2, 4097, 100_003):
+        keys = _strings(n, width, width * 7 + n)
+        _say("argsort_bytes", width, mode, n)
+        exp = oracle.argsort_bytes(keys, m).astype(np.int64)
+        perm, rows = vdf.argsort_bytes(torch.from_numpy(keys).cuda(), descending=(mode == "desc"),
+                                       reverse_desc=(mode == "reverse"), want_sorted=True)
+        assert np.array_equal(perm.cpu().numpy().view(np.uint32).astype(np.int64), exp), (width, mode, n)
+        assert np.array_equal(rows.cpu().numpy(), keys[exp]), (width, mode, n)
+
+
+def test_argsort_bytes_width16_equals_char16_and_perm64(vdf):
+    import torch
+
+    keys = datagen.char16_keys(5, 0, 200_001, distinct=5000)
+    d = torch.from_numpy(keys).cuda()
+    a = vdf.argsort_bytes(d, perm64=True)
+    b = vdf.argsort(d, key_type=N.KEY_CHAR16, perm64=True)
+    assert torch.equal(a, b)
+    assert np.array_equal(a.cpu().numpy().astype(np.uint64), oracle.argsort(keys, 0))
+
+
+def test_argsort_bytes_rejects_bad_width_and_empty_is_fine(vdf):
+    import torch
+
+    with pytest.raises(N.VdfError) as e:
+        vdf.argsort_bytes(torch.zeros((4, 4097), dtype=torch.uint8, device="cuda"))
+    assert e.value.status == N.VDF_ERR_INVALID
+    assert vdf.argsort_bytes(torch.zeros((0, 8), dtype=torch.uint8, device="cuda")).shape[0] == 0

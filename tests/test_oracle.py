@@ -114,6 +114,44 @@ def test_char16_vs_python_sorted():
     assert oracle.argsort(keys, 0).tolist() == ref
 
 
+def _pad_strings(strs, pad=0):
+    """The reference's flat string buffer: every string padded to the longest
+    with PaddingChar (sort_string.inl:30-85)."""
+    w = max(len(x) for x in strs)
+    out = np.full((len(strs), w), pad, dtype=np.uint8)
+    for i, x in enumerate(strs):
+        out[i, :len(x)] = np.frombuffer(x, dtype=np.uint8)
+    return out
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_argsort_bytes_vs_python_sorted(mode):
+    """Byte strings of many widths (1, 5, 16, 17, 33, 70 bytes): the oracle's
+    charbuf LSD over the padded buffer equals Python's stable sort of the
+    signed-char byte tuples (bytes >= 0x80 sort before 0x00, as signed chars),
+    including strings that differ only in length (padding 0x00)."""
+    rng = np.random.default_rng(40 + mode)
+    for maxw in (1, 5, 16, 17, 33, 70):
+        n = 600
+        lens = rng.integers(0, maxw + 1, n)
+        alphabet = np.array([0x00, 0x01, 0x41, 0x42, 0x7F, 0x80, 0xC3, 0xFF], dtype=np.uint8)
+        strs = [bytes(alphabet[rng.integers(0, alphabet.size, int(L))]) for L in lens]
+        strs += [b"", b"AB", b"AB\x00", b"AB\x01", b"AB\xff"]
+        keys = _pad_strings(strs) if max(len(x) for x in strs) else np.zeros((len(strs), 1), np.uint8)
+        def skey(i):
+            return tuple(((b ^ 0x80) ^ (0xFF if mode == 1 else 0)) for b in keys[i])
+        ref = sorted(range(len(strs)), key=skey)
+        if mode == 2:
+            ref = ref[::-1]
+        assert oracle.argsort_bytes(keys, mode).tolist() == ref, maxw
+
+
+def test_argsort_bytes_width16_is_char16():
+    keys = datagen.char16_keys(12, 0, 5000, distinct=300)
+    for mode in (0, 1, 2):
+        assert np.array_equal(oracle.argsort_bytes(keys, mode), oracle.argsort(keys, mode)), mode
+
+
 # ------------------------------------------------------------- float keys
 # sort_flt.inl:44-50: negative values have all bits complemented, others the
 # sign bit set, so the radix order is -NaN < -inf < ... < -0 < +0 < ... < +inf < +NaN.

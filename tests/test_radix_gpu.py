@@ -157,6 +157,27 @@ def test_gather_vs_oracle(vdf):
     assert np.array_equal(vdf.gather(_dev(src), p64).cpu().numpy(), src[perm.astype(np.int64)])
 
 
+@pytest.mark.parametrize("width", [1, 3, 12, 20, 33, 256])
+def test_gather_any_width_vs_oracle(vdf, width):
+    """Cells of widths without a vectorised kernel (padded strings): row copies
+    in words or bytes, 32- and 64-bit row ids, and an unaligned source view."""
+    import torch
+
+    rng = np.random.default_rng(width)
+    n = 20_011
+    src = rng.integers(0, 256, (n, width), dtype=np.uint8)
+    perm = rng.permutation(n)
+    exp = oracle.gather(src, perm.astype(np.uint64))
+    d = torch.from_numpy(src).cuda()
+    for p in (torch.from_numpy(perm.astype(np.int32)).cuda(), torch.from_numpy(perm.astype(np.int64)).cuda()):
+        assert np.array_equal(vdf.gather(d, p).cpu().numpy(), exp), width
+    # the same rows starting one byte into a larger buffer
+    big = torch.zeros(n * width + 1, dtype=torch.uint8, device="cuda")
+    big[1:] = d.reshape(-1)
+    view = big[1:].view(n, width)
+    assert np.array_equal(vdf.gather(view, torch.from_numpy(perm.astype(np.int32)).cuda()).cpu().numpy(), exp)
+
+
 def test_reuse_handle_and_side_stream(vdf):
     import torch
 

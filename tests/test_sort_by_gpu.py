@@ -77,6 +77,10 @@ def test_sort_by_without_perm_and_perm64(vdf):
     keys = datagen.groupby_small(200_003, seed=3)[0]
     _check(vdf, keys, _vals(keys.shape[0], 8, 3), want_perm=False)
     _check(vdf, keys, _vals(keys.shape[0], 4, 4), perm64=True)
+    _check(vdf, keys, _vals(keys.shape[0], 8, 5), perm64=True)  # split staging, 64-bit row ids
+    u32 = (datagen.splitmix64(6, 0, 300_007) >> np.uint64(32)).astype(np.uint32)
+    _check(vdf, u32, _vals(u32.shape[0], 8, 6), perm64=True)
+    _check(vdf, u32, _vals(u32.shape[0], 8, 7), want_perm=False)
 
 
 @pytest.mark.parametrize("items,xcd", [(12, 1), (16, 1), (16, 0), (24, 1)])

@@ -306,7 +306,8 @@ vdf_status vdf_gather(vdf_handle* h, uint32_t elem_bytes, const void* d_src, con
 {
     return guarded([&] {
         require(h != nullptr, "vdf_gather: null handle");
-        require(elem_bytes == 4 || elem_bytes == 8 || elem_bytes == 16, "vdf_gather: elem_bytes must be 4, 8 or 16");
+        require(elem_bytes >= 1 && elem_bytes <= VDF_MAX_STRING_BYTES,
+                "vdf_gather: elem_bytes must be 1 .. VDF_MAX_STRING_BYTES");
         if (n == 0) return;
         require(d_src && d_perm && d_dst, "vdf_gather: null device pointer");
         CallScope cs(h);
@@ -508,6 +509,29 @@ vdf_status vdf_argsort_multi(vdf_handle* h, const vdf_key_col* keys, int nkeys, 
         h->staging_begin(argsort_batch_staging(kt, &n, 1, hr));
         StagingScope st(h, s);
         argsort_enqueue(h, kt, comp, n, flags, d_perm_out, nullptr, ascr, s);
+    });
+}
+
+vdf_status vdf_argsort_bytes(vdf_handle* h, const void* d_keys, uint32_t width, uint64_t n, uint32_t flags,
+                             void* d_perm_out, void* d_sorted_out, void* stream)
+{
+    return guarded([&] {
+        require(h != nullptr, "vdf_argsort_bytes: null handle");
+        require(width >= 1 && width <= VDF_MAX_STRING_BYTES, "vdf_argsort_bytes: width must be 1 .. VDF_MAX_STRING_BYTES");
+        require((flags & ~(VDF_SORT_DESC | VDF_SORT_DESC_REVERSE | VDF_SORT_PERM64)) == 0,
+                "vdf_argsort_bytes: unknown flag bits");
+        require(!((flags & VDF_SORT_DESC) && (flags & VDF_SORT_DESC_REVERSE)),
+                "vdf_argsort_bytes: VDF_SORT_DESC and VDF_SORT_DESC_REVERSE are exclusive");
+        require(n <= VDF_MAX_ROWS, "vdf_argsort_bytes: n exceeds VDF_MAX_ROWS (2^30 - 1)");
+        if (n == 0) return;
+        require(d_keys != nullptr && d_perm_out != nullptr, "vdf_argsort_bytes: null device pointer");
+        CallScope cs(h);
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        const uint64_t hr = static_cast<uint64_t>(h->tune.hist_rows);
+        void* scr = h->ensure_scratch(bytes_argsort_scratch(n, hr));
+        h->staging_begin(bytes_argsort_staging(width, n, hr));
+        StagingScope st(h, s);
+        bytes_argsort_enqueue(h, d_keys, width, n, flags, d_perm_out, d_sorted_out, scr, s);
     });
 }
 

@@ -132,6 +132,26 @@ void broadcast_enqueue(Handle* h, int src_type, int dst_type, const void* gval, 
     }
 }
 
+// Cells of any other width (padded strings): one row per thread, copied in
+// 4-byte words when the width and both buffers allow (words), else bytes.
+template <typename IDX>
+__global__ void k_gather_rows(const uint8_t* src, uint32_t elem, const IDX* perm, uint64_t n, uint8_t* dst,
+                              uint32_t words)
+{
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t r = gmem(perm)[i];
+    if (words) {
+        const auto* a = gmem(reinterpret_cast<const uint32_t*>(src + r * elem));
+        auto* b = gmem(reinterpret_cast<uint32_t*>(dst + i * elem));
+        for (uint32_t j = 0; j < elem / 4; ++j) b[j] = a[j];
+    } else {
+        const auto* a = gmem(src) + r * elem;
+        auto* b = gmem(dst) + i * elem;
+        for (uint32_t j = 0; j < elem; ++j) b[j] = a[j];
+    }
+}
+
 void gather_enqueue(Handle* h, uint32_t elem_bytes, const void* d_src, const void* d_perm, int perm64,
                     uint64_t n, void* d_dst, hipStream_t s)
 {
@@ -151,7 +171,21 @@ void gather_enqueue(Handle* h, uint32_t elem_bytes, const void* d_src, const voi
         perm64 ? launch_gather<V16, uint64_t>(h->tune, d_src, d_perm, n, d_dst, s)
                : launch_gather<V16, uint32_t>(h->tune, d_src, d_perm, n, d_dst, s);
         break;
-    default: throw Error{VDF_ERR_INVALID, "vdf_gather: elem_bytes must be 4, 8 or 16"};
+    default: {
+        const dim3 grid(static_cast<unsigned>((n + 255) / 256));
+        const auto* src = static_cast<const uint8_t*>(d_src);
+        auto* dst = static_cast<uint8_t*>(d_dst);
+        const uint32_t words =
+            ((elem_bytes | reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(dst)) & 3u) == 0 ? 1u : 0u;
+        if (perm64)
+            hipLaunchKernelGGL(k_gather_rows<uint64_t>, grid, dim3(256), 0, s, src, elem_bytes,
+                               static_cast<const uint64_t*>(d_perm), n, dst, words);
+        else
+            hipLaunchKernelGGL(k_gather_rows<uint32_t>, grid, dim3(256), 0, s, src, elem_bytes,
+                               static_cast<const uint32_t*>(d_perm), n, dst, words);
+        check_launch("k_gather_rows");
+        break;
+    }
     }
 }
 

@@ -313,6 +313,12 @@ void groupby_aggregate_enqueue(Handle* h, int key_type, const void* skeys, const
 // multi-column keys: composite key type (-1 when wider than 16 bytes), pack
 // the columns into composites, unpack composite group keys into the columns
 int composite_key_type(const int* types, int ncols);
+// fixed-width byte strings (strings.hip)
+size_t bytes_argsort_scratch(uint64_t n, uint64_t hist_rows);
+size_t bytes_argsort_staging(uint32_t width, uint64_t n, uint64_t hist_rows);
+void bytes_argsort_enqueue(Handle* h, const void* d_keys, uint32_t width, uint64_t n, uint32_t flags,
+                           void* d_perm_out, void* d_sorted_out, void* scr, hipStream_t s);
+
 void pack_keys_enqueue(Handle* h, const int* types, const void* const* data, int ncols, uint64_t n, void* comp,
                        hipStream_t s);
 void unpack_keys_enqueue(Handle* h, const int* types, void* const* outs, int ncols, const void* comp,

@@ -298,6 +298,24 @@ class Vdf:
                                            C.c_void_p(_stream_ptr(stream))))
         return perm
 
+    def argsort_bytes(self, keys: torch.Tensor, *, descending: bool = False, reverse_desc: bool = False,
+                      perm64: bool = False, want_sorted: bool = False, stream=None):
+        """Stable argsort of fixed-width byte strings: a (n, width) uint8
+        tensor, one padded string per row, ordered bytewise as signed chars.
+        Returns the permutation, or (permutation, rows in sorted order)."""
+        if keys.dtype != torch.uint8 or keys.dim() != 2:
+            raise N.VdfError(N.VDF_ERR_INVALID, "argsort_bytes: keys must be a (n, width) uint8 tensor")
+        keys = keys.contiguous()
+        n, width = keys.shape
+        flags = (N.SORT_DESC if descending else 0) | (N.SORT_DESC_REVERSE if reverse_desc else 0) \
+            | (N.SORT_PERM64 if perm64 else 0)
+        perm = torch.empty(n, dtype=torch.int64 if perm64 else torch.int32, device=keys.device)
+        out = torch.empty_like(keys) if want_sorted else None
+        N.check(self.lib.vdf_argsort_bytes(self.h, _ptr(keys), width, n, flags, _ptr(perm),
+                                           _ptr(out) if out is not None else None,
+                                           C.c_void_p(_stream_ptr(stream))))
+        return (perm, out) if want_sorted else perm
+
     def broadcast(self, group_vals: torch.Tensor, result: "GroupbyResult", n: int, to_f64: bool = False,
                   stream=None) -> torch.Tensor:
         """Per-row column: row r gets the value of its group (the group-by's
