@@ -207,6 +207,14 @@ vdf_status vdf_argsort_multi(vdf_handle* h, const vdf_key_col* keys, int nkeys, 
  * receives the rows in sorted order. */
 vdf_status vdf_argsort_bytes(vdf_handle* h, const void* d_keys, uint32_t width, uint64_t n, uint32_t flags,
                              void* d_perm_out, void* d_sorted_out, void* stream);
+/* Group by fixed-width byte strings (rows and order as vdf_argsort_bytes):
+ * the outputs of vdf_groupby, with out->group_keys (capacity * width bytes;
+ * nullable) receiving each group's string.  Replaces transform_group_by over a
+ * std::string key column (onecol_functions.inl:4-130, keys hashed as string_view
+ * of the string's bytes); groups come in ascending string order. */
+vdf_status vdf_groupby_bytes(vdf_handle* h, const void* d_keys, uint32_t width, uint64_t n,
+                             const vdf_value_col* vals, int nvals, uint32_t agg_mask,
+                             vdf_groupby_out* out, void* stream);
 
 /* ---- cached grouping ----
  * Sort one key column once and aggregate value columns against it any number

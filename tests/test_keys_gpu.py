@@ -209,3 +209,58 @@ def test_argsort_bytes_rejects_bad_width_and_empty_is_fine(vdf):
         vdf.argsort_bytes(torch.zeros((4, 4097), dtype=torch.uint8, device="cuda"))
     assert e.value.status == N.VDF_ERR_INVALID
     assert vdf.argsort_bytes(torch.zeros((0, 8), dtype=torch.uint8, device="cuda")).shape[0] == 0
+
+
+def _groupby_bytes_oracle(keys: np.ndarray, vals):
+    """The oracle's group-by over each string's rank among the distinct
+    strings (ascending signed-char byte order, i.e. bytes ^ 0x80 compared as
+    unsigned): the same groups, order, row view and aggregates as grouping by
+    the strings themselves."""
+    n, w = keys.shape
+    t = np.ascontiguousarray(keys ^ np.uint8(0x80)).view(np.dtype((np.void, w))).ravel()
+    _, ids = np.unique(t, return_inverse=True)
+    o = oracle.groupby(ids.reshape(-1).astype(np.uint32), vals)
+    g = o["n_groups"]
+    o["skeys"] = keys[o["perm"][o["offsets"][:g].astype(np.int64)].astype(np.int64)]
+    return o
+
+
+@pytest.mark.parametrize("width", [1, 7, 16, 40, 100])
+def test_groupby_bytes_vs_oracle(vdf, width):
+    import torch
+
+    rng = np.random.default_rng(width)
+    for n, distinct in ((1, 1), (4097, 50), (300_007, 20_000)):
+        pool = _strings(distinct, width, width + distinct)
+        keys = pool[rng.integers(0, distinct, n)]
+        vf = rng.standard_normal(n)
+        vi = rng.integers(-(2**40), 2**40, n)
+        _say("groupby_bytes", width, n, distinct)
+        r = vdf.groupby_bytes(torch.from_numpy(keys).cuda(), [torch.from_numpy(vf).cuda(), torch.from_numpy(vi).cuda()])
+        o = _groupby_bytes_oracle(keys, [vf, vi])
+        g = o["n_groups"]
+        assert r.n_groups == g
+        assert np.array_equal(r.keys.cpu().numpy(), o["skeys"]), (width, n)
+        assert np.array_equal(r.offsets.cpu().numpy().view(np.uint32).astype(np.uint64), o["offsets"])
+        assert np.array_equal(r.counts.cpu().numpy().view(np.uint32).astype(np.uint64), o["counts"])
+        assert np.array_equal(r.perm.cpu().numpy().view(np.uint32).astype(np.uint64), o["perm"])
+        fa, ia = r.aggs[0], r.aggs[1]
+        of, oi = o["aggs"][0], o["aggs"][1]
+        assert np.array_equal(ia["sum"].cpu().numpy(), oi["sum"].view(np.int64))
+        assert np.array_equal(ia["min"].cpu().numpy(), oi["min"]) and np.array_equal(ia["max"].cpu().numpy(), oi["max"])
+        assert np.array_equal(fa["min"].cpu().numpy().view(np.uint8), of["min"].view(np.uint8))
+        assert np.array_equal(fa["max"].cpu().numpy().view(np.uint8), of["max"].view(np.uint8))
+        for k in ("sum", "mean"):
+            got, exp = fa[k].cpu().numpy(), of[k]
+            assert np.all(np.abs(got - exp) <= F64_RTOL * np.abs(exp)), (width, n, k)
+
+
+def test_groupby_bytes_empty_and_one_group(vdf):
+    import torch
+
+    r = vdf.groupby_bytes(torch.zeros((0, 5), dtype=torch.uint8, device="cuda"), [])
+    assert r.n_groups == 0
+    keys = np.full((10_000, 33), 0x41, dtype=np.uint8)
+    r = vdf.groupby_bytes(torch.from_numpy(keys).cuda(), [], aggs=N.AGG_COUNT)
+    assert r.n_groups == 1 and int(r.counts[0]) == 10_000
+    assert np.array_equal(r.keys.cpu().numpy(), keys[:1])

@@ -46,7 +46,7 @@ EXPORTS = (
     "vdf_server_wait", "vdf_server_release", "vdf_server_stats_get",
     "vdf_autotune_argsort", "vdf_autotune_sort_by", "vdf_autotune_groupby_batch",
     "vdf_plan_groupby_batch", "vdf_plan_launch", "vdf_plan_scratch_bytes", "vdf_plan_destroy",
-    "vdf_groupby_multi", "vdf_argsort_multi", "vdf_argsort_bytes", "vdf_broadcast",
+    "vdf_groupby_multi", "vdf_argsort_multi", "vdf_argsort_bytes", "vdf_groupby_bytes", "vdf_broadcast",
     "vdf_grouping_create", "vdf_grouping_aggregate", "vdf_grouping_destroy",
 )
 
@@ -193,6 +193,8 @@ def load() -> C.CDLL:
     lib.vdf_argsort_multi.argtypes = [vp, C.POINTER(VdfKeyCol), i32, u64, u32, vp, vp]
     lib.vdf_argsort_bytes.restype = i32
     lib.vdf_argsort_bytes.argtypes = [vp, vp, u32, u64, u32, vp, vp, vp]
+    lib.vdf_groupby_bytes.restype = i32
+    lib.vdf_groupby_bytes.argtypes = [vp, vp, u32, u64, C.POINTER(VdfValueCol), i32, u32, C.POINTER(VdfGroupbyOut), vp]
     lib.vdf_broadcast.restype = i32
     lib.vdf_broadcast.argtypes = [vp, i32, i32, vp, vp, vp, u64, vp, u64, vp, vp]
     lib.vdf_grouping_create.restype = i32

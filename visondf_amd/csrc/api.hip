@@ -535,6 +535,48 @@ vdf_status vdf_argsort_bytes(vdf_handle* h, const void* d_keys, uint32_t width, 
     });
 }
 
+vdf_status vdf_groupby_bytes(vdf_handle* h, const void* d_keys, uint32_t width, uint64_t n,
+                             const vdf_value_col* vals, int nvals, uint32_t agg_mask, vdf_groupby_out* out,
+                             void* stream)
+{
+    return guarded([&] {
+        require(h != nullptr && out != nullptr, "vdf_groupby_bytes: null handle or output");
+        require(width >= 1 && width <= VDF_MAX_STRING_BYTES, "vdf_groupby_bytes: width must be 1 .. VDF_MAX_STRING_BYTES");
+        require(nvals >= 0 && nvals <= VDF_MAX_VALUE_COLS, "vdf_groupby_bytes: nvals out of range");
+        require(nvals == 0 || vals != nullptr, "vdf_groupby_bytes: null value columns");
+        require((agg_mask & ~VDF_AGG_ALL) == 0, "vdf_groupby_bytes: unknown aggregate bits");
+        require(out->n_groups != nullptr && out->offsets != nullptr, "vdf_groupby_bytes: n_groups and offsets are required");
+        require(n <= VDF_MAX_ROWS, "vdf_groupby_bytes: n exceeds VDF_MAX_ROWS (2^30 - 1)");
+        require(n == 0 || d_keys != nullptr, "vdf_groupby_bytes: null keys");
+        std::vector<int> vts(static_cast<size_t>(std::max(nvals, 1)), 0);
+        GbTable t{};
+        t.n = n;
+        for (int c = 0; c < nvals; ++c) {
+            require(val_bytes(vals[c].type) != 0, "vdf_groupby_bytes: unknown value type");
+            require(n == 0 || vals[c].data != nullptr, "vdf_groupby_bytes: null value column data");
+            vts[c] = vals[c].type;
+            t.vals[c] = vals[c].data;
+        }
+        CallScope cs(h);
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        if (n == 0) {  // zero groups (the aggregate stage writes them for empty tables)
+            void* scr = h->ensure_scratch(groupby_agg_scratch(&n, 1));
+            h->staging_begin(groupby_agg_staging(&n, 1));
+            StagingScope st(h, s);
+            const uint32_t* perms[1] = {nullptr};
+            vdf_groupby_out o = *out;
+            o.group_keys = nullptr;
+            groupby_aggregate_enqueue(h, VDF_KEY_U32, nullptr, perms, &t, 1, vts.data(), nvals, agg_mask, &o, scr, s);
+            return;
+        }
+        const uint64_t hr = static_cast<uint64_t>(h->tune.hist_rows);
+        void* scr = h->ensure_scratch(bytes_groupby_scratch(width, n, hr));
+        h->staging_begin(bytes_groupby_staging(width, n, hr));
+        StagingScope st(h, s);
+        bytes_groupby_enqueue(h, d_keys, width, n, t, vts.data(), nvals, agg_mask, out, scr, s);
+    });
+}
+
 vdf_status vdf_broadcast(vdf_handle* h, int src_type, int dst_type, const void* d_group_vals,
                          const uint32_t* d_offsets, const uint64_t* d_n_groups, uint64_t capacity,
                          const uint32_t* d_perm, uint64_t n, void* d_out, void* stream)

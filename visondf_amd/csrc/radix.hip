@@ -153,25 +153,6 @@ __device__ inline uint32_t load_relaxed(uint32_t* p)
     return __hip_atomic_load(gmem(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Exclusive scan of one value per thread over a 256-thread block.
-__device__ inline uint32_t block_excl_scan(uint32_t x, uint32_t* s_wsum)
-{
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    uint32_t inc = x;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t y = __shfl_up(inc, off, 64);
-        if (lane >= off) inc += y;
-    }
-    if (lane == 63) s_wsum[w] = inc;
-    __syncthreads();
-    uint32_t pre = 0;
-#pragma unroll
-    for (int i = 0; i < kWaves; ++i) pre += (i < w) ? s_wsum[i] : 0u;
-    __syncthreads();
-    return pre + inc - x;
-}
-
 __global__ void k_fill(uint32_t* p, uint64_t count, uint32_t value)
 {
     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;

@@ -313,11 +313,36 @@ void groupby_aggregate_enqueue(Handle* h, int key_type, const void* skeys, const
 // multi-column keys: composite key type (-1 when wider than 16 bytes), pack
 // the columns into composites, unpack composite group keys into the columns
 int composite_key_type(const int* types, int ncols);
+// Exclusive scan of one value per thread over a 256-thread block.
+__device__ inline uint32_t block_excl_scan(uint32_t x, uint32_t* s_wsum)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t inc = x;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(inc, off, 64);
+        if (lane >= off) inc += y;
+    }
+    if (lane == 63) s_wsum[w] = inc;
+    __syncthreads();
+    uint32_t pre = 0;
+#pragma unroll
+    for (int i = 0; i < kWaves; ++i) pre += (i < w) ? s_wsum[i] : 0u;
+    __syncthreads();
+    return pre + inc - x;
+}
+
 // fixed-width byte strings (strings.hip)
 size_t bytes_argsort_scratch(uint64_t n, uint64_t hist_rows);
 size_t bytes_argsort_staging(uint32_t width, uint64_t n, uint64_t hist_rows);
 void bytes_argsort_enqueue(Handle* h, const void* d_keys, uint32_t width, uint64_t n, uint32_t flags,
                            void* d_perm_out, void* d_sorted_out, void* scr, hipStream_t s);
+struct GbTable;
+size_t bytes_groupby_scratch(uint32_t width, uint64_t n, uint64_t hist_rows);
+size_t bytes_groupby_staging(uint32_t width, uint64_t n, uint64_t hist_rows);
+void bytes_groupby_enqueue(Handle* h, const void* d_keys, uint32_t width, uint64_t n, const GbTable& t,
+                           const int* val_types, int nvals, uint32_t agg_mask, const vdf_groupby_out* out, void* scr,
+                           hipStream_t s);
 
 void pack_keys_enqueue(Handle* h, const int* types, const void* const* data, int ncols, uint64_t n, void* comp,
                        hipStream_t s);

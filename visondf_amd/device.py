@@ -316,6 +316,24 @@ class Vdf:
                                            C.c_void_p(_stream_ptr(stream))))
         return (perm, out) if want_sorted else perm
 
+    def groupby_bytes(self, keys: torch.Tensor, vals=(), aggs: int = N.AGG_ALL, want_perm: bool = True,
+                      stream=None, trim: bool = True, val_types=None, capacity: int = 0):
+        """Group-by over fixed-width byte strings ((n, width) uint8, padded);
+        group keys come back as (groups, width) uint8 rows, ascending."""
+        if keys.dtype != torch.uint8 or keys.dim() != 2:
+            raise N.VdfError(N.VDF_ERR_INVALID, "groupby_bytes: keys must be a (n, width) uint8 tensor")
+        keys = keys.contiguous()
+        n, width = keys.shape
+        vts = [val_type_of(v) if val_types is None else val_types[c] for c, v in enumerate(vals)]
+        out, t = self._alloc_out(keys, vts, [v.dtype for v in vals], aggs, want_perm, capacity)
+        cols = (N.VdfValueCol * max(len(vals), 1))()
+        for c, v in enumerate(vals):
+            cols[c].type = vts[c]
+            cols[c].data = v.data_ptr()
+        N.check(self.lib.vdf_groupby_bytes(self.h, _ptr(keys) if n else None, width, n, cols, len(vals), aggs,
+                                           C.byref(out), C.c_void_p(_stream_ptr(stream))))
+        return self._result(t, n, trim)
+
     def broadcast(self, group_vals: torch.Tensor, result: "GroupbyResult", n: int, to_f64: bool = False,
                   stream=None) -> torch.Tensor:
         """Per-row column: row r gets the value of its group (the group-by's
