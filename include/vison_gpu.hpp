@@ -314,7 +314,8 @@ class Dataframe {
         std::vector<void*> kout;
         for (unsigned int k : key_cols) {
             const Column& c = col(k);
-            if (c.key_type < 0) throw Error(VDF_ERR_UNSUPPORTED, "group_by: column type has no key path");
+            if (c.key_type < 0 && !(c.str && key_cols.size() == 1))
+                throw Error(VDF_ERR_UNSUPPORTED, "group_by: column type has no key path (string keys: one key column)");
             kc.push_back(vdf_key_col{c.key_type, c.data.get()});
             D.keys.emplace_back(cap * c.elem);
             kout.push_back(D.keys.back().get());
@@ -336,7 +337,11 @@ class Dataframe {
             out.aggs[c].min = D.agg[4 * c + 2].get();
             out.aggs[c].max = D.agg[4 * c + 3].get();
         }
-        if (kc.size() == 1)
+        const Column& k0 = col(key_cols[0]);
+        if (k0.str)  // a std::string key column
+            check(vdf_groupby_bytes(h_.get(), k0.data.get(), static_cast<uint32_t>(k0.elem), n, vc.data(),
+                                    static_cast<int>(vc.size()), aggs, &out, nullptr));
+        else if (kc.size() == 1)
             check(vdf_groupby(h_.get(), kc[0].type, kc[0].data, n, vc.data(), static_cast<int>(vc.size()), aggs, &out,
                               nullptr));
         else
@@ -353,6 +358,39 @@ class Dataframe {
         if (k.key_type < 0 || k.elem != sizeof(K)) throw Error(VDF_ERR_INVALID, "group_by: bad key column");
         DeviceGroups D = group_by_device({key}, vals, aggs);
         return to_host<K>(D, vals);
+    }
+
+    // Group by a std::string key column: groups in ascending order of the
+    // strings' bytes as signed chars (keys without their padding).
+    GroupResult<std::string> group_by_str(unsigned int key, const std::vector<unsigned int>& vals,
+                                          uint32_t aggs = VDF_AGG_ALL)
+    {
+        const Column& k = col(key);
+        if (!k.str) throw Error(VDF_ERR_INVALID, "group_by_str: not a string column");
+        DeviceGroups D = group_by_device({key}, vals, aggs);
+        GroupResult<uint8_t> r = to_host<uint8_t>(D, vals, /*with_keys=*/false);
+        GroupResult<std::string> R;
+        R.n_groups = r.n_groups;
+        std::vector<char> flat(r.n_groups * k.elem);
+        if (!flat.empty())
+            hip_check(hipMemcpy(flat.data(), D.keys[0].get(), flat.size(), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+        R.keys.resize(r.n_groups);
+        for (size_t g = 0; g < r.n_groups; ++g) {
+            size_t len = k.elem;
+            while (len > 0 && flat[g * k.elem + len - 1] == 0) --len;
+            R.keys[g].assign(flat.data() + g * k.elem, len);
+        }
+        R.offsets = std::move(r.offsets);
+        R.counts = std::move(r.counts);
+        R.row_view = std::move(r.row_view);
+        R.sum_f = std::move(r.sum_f);
+        R.mean = std::move(r.mean);
+        R.min_f = std::move(r.min_f);
+        R.max_f = std::move(r.max_f);
+        R.sum_i = std::move(r.sum_i);
+        R.min_i = std::move(r.min_i);
+        R.max_i = std::move(r.max_i);
+        return R;
     }
 
     // Group by several key columns; group keys of column i in keys_of[i] (raw bytes).

@@ -292,6 +292,26 @@ int main()
             CHECK(ok);
         }
     }
+    // group-by over the std::string column: groups ascending, counts and
+    // sums of the tag column, and a transform_group_by count broadcast per row
+    std::printf("group_by_str\n");
+    {
+        std::vector<std::string> s = {"kiwi", "fig", "kiwi", "", "fig", "date palm", "kiwi", ""};
+        std::vector<int64_t> v = {1, 2, 3, 4, 5, 6, 7, 8};
+        Dataframe df;
+        df.add_col(s, "s");
+        df.add_col(v, "v");
+        const auto R = df.group_by_str(0, {1});
+        const std::vector<std::string> keys = {"", "date palm", "fig", "kiwi"};
+        const std::vector<uint32_t> counts = {2, 1, 2, 3};
+        const std::vector<int64_t> sums = {12, 6, 7, 11};
+        bool ok = R.n_groups == 4 && R.keys == keys && R.counts == counts && R.sum_i[0] == sums;
+        CHECK(ok);
+        df.transform_group_by<vison_gpu::GroupFunction::Occurence>({0}, 0, "n");
+        const auto nper = df.get_col<double>(2);
+        const std::vector<double> want = {3, 2, 3, 2, 2, 1, 3, 2};
+        CHECK(nper == want);
+    }
     // errors surface as exceptions with the library's message
     {
         Dataframe df = load(f);
