@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <cstring>
+#include <initializer_list>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -244,6 +245,28 @@ class Dataframe {
         if (k.key_type < 0 && !k.str) throw Error(VDF_ERR_UNSUPPORTED, "sort_by: column type has no radix key path");
         if (nrow_ == 0) return;
         DeviceBuffer perm(static_cast<size_t>(nrow_) * 4);
+        if constexpr (!Soft) {
+            // hard sort of a numeric key: the first other 4- or 8-byte column
+            // rides through the radix passes (vdf_sort_by) and the sorted keys
+            // come out of the last pass; only the remaining columns are gathered
+            int carry = -1;
+            for (size_t c = 0; c < cols_.size() && !k.str; ++c)
+                if (c != n && !cols_[c].str && (cols_[c].elem == 4 || cols_[c].elem == 8)) {
+                    carry = static_cast<int>(c);
+                    break;
+                }
+            if (carry >= 0) {
+                Column& cv = cols_[static_cast<size_t>(carry)];
+                DeviceBuffer skeys(k.data.bytes()), svals(cv.data.bytes());
+                check(vdf_sort_by(h_.get(), k.key_type, k.data.get(), nrow_, ASC ? 0u : VDF_SORT_DESC_REVERSE,
+                                  perm.get(), skeys.get(), static_cast<uint32_t>(cv.elem), cv.data.get(), svals.get(),
+                                  nullptr));
+                permute_all(perm.get(), {n, static_cast<unsigned int>(carry)});
+                cols_[n].data = std::move(skeys);
+                cv.data = std::move(svals);
+                return;
+            }
+        }
         if (k.str)  // sort_string (sort_by_mt.hpp:72-115 dispatch)
             check(vdf_argsort_bytes(h_.get(), k.data.get(), static_cast<uint32_t>(k.elem), nrow_,
                                     ASC ? 0u : VDF_SORT_DESC_REVERSE, perm.get(), nullptr, nullptr));
@@ -525,9 +548,12 @@ class Dataframe {
         return cols_[n];
     }
 
-    void permute_all(void* perm)
+    // gather every column through perm, except the columns in `skip`
+    void permute_all(void* perm, std::initializer_list<unsigned int> skip = {})
     {
-        for (Column& c : cols_) {
+        for (size_t i = 0; i < cols_.size(); ++i) {
+            if (std::find(skip.begin(), skip.end(), static_cast<unsigned int>(i)) != skip.end()) continue;
+            Column& c = cols_[i];
             DeviceBuffer dst(c.data.bytes());
             check(vdf_gather(h_.get(), static_cast<uint32_t>(c.elem), c.data.get(), perm, 0, nrow_, dst.get(),
                              nullptr));
