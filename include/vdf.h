@@ -279,7 +279,8 @@ vdf_status vdf_groupby_batch(vdf_handle* h, int key_type, const vdf_table* table
  *                    payload width -- 32 without a payload, 24 with an 8-byte one, 12
  *                    with a 4- or 16-byte one), 8, 12, 16, 24, 32; a size without a
  *                    kernel for a sort's payload width uses that width's default
- *   "sort_items_64"  same for 8-byte keys: 0 (default: 12; 8 with a 16-byte payload),
+ *   "sort_items_64"  same for 8-byte keys: 0 (default: 12; 16 with an 8-byte payload, 8 with
+ *                    a 16-byte one),
  *                    8, 12, 16
  *   "sort_items_128" same for 16-byte keys: 0 (default: 8), 8, 16
  *   "sort_chunk"     consecutive tiles one tile group (one XCD) claims at a time: 1 .. 1024

@@ -1169,7 +1169,8 @@ static int default_items(int pay_bytes)
         default: return 12;  // 4- and 16-byte payloads
         }
     }
-    return sizeof(K) == 8 ? (pay_bytes == 16 ? 8 : 12) : 8;
+    if (sizeof(K) == 8) return pay_bytes == 16 ? 8 : (pay_bytes == 8 ? 16 : 12);  // 16: split staging
+    return 8;
 }
 
 template <typename K>
