@@ -122,24 +122,43 @@ def _base_record(args, value, unit, ms_step, dtype, config, higher=True) -> dict
 # ---------------------------------------------------------------------------
 # headline: 100M uint32 argsort + f64 gather
 # ---------------------------------------------------------------------------
-def cpu_baseline_argsort(keys, val, threads: int, reps: int = 3) -> dict:
-    """The oracle's restatement of radix_sort_uint32_mt (16-bit digits) + gather,
-    timed on the host cores (median of reps after one warm-up)."""
+def cpu_baseline_argsort(keys, val, threads: int, reps: int = 3, port: bool = False) -> dict:
+    """The reference's own radix_sort_uint32_mt (compiled from its .inl files
+    into oracle/_ref, 16 threads) when that build is present, else the oracle's
+    restatement of it, plus the gather (permute_block_mt restatement), timed on
+    the host cores: median of reps after one warm-up."""
+    import ctypes as C
+
     import numpy as np
 
     import oracle
 
     n = keys.shape[0]
+    ref = None if port else oracle.ref_lib()
+    use_ref = ref is not None and threads >= 16
     times = []
     for r in range(reps + 1):
         t0 = time.perf_counter()
-        perm = oracle.argsort(keys, 0, threads)
+        if use_ref:
+            # sort_uintegers copies the key column and starts from iota (sort_by_mt.hpp:21-24)
+            perm = np.arange(n, dtype=np.uint64)
+            k = np.ascontiguousarray(keys, dtype=np.uint32).copy()
+            ref.vref_radix_sort_u32_mt16(C.c_void_p(k.ctypes.data), C.c_void_p(perm.ctypes.data), n)
+        else:
+            perm = oracle.argsort(keys, 0, threads)
         oracle.gather(val, perm, threads)
         dt = time.perf_counter() - t0
         if r:
             times.append(dt)
         _log(f"cpu baseline rep {r}: {dt:.3f} s")
     t = float(np.median(times))
+    if use_ref:
+        exp = oracle.argsort(keys, 0, threads)
+        assert np.array_equal(perm, exp), "reference radix and oracle disagree"
+        return {"value": n / t, "unit": "rows/s", "cores": 16, "kind": "reference",
+                "sample": f"full workload ({n} rows): the reference's radix_sort_uint32_mt<16, Simd=false> "
+                          f"(its .inl files compiled in place, oracle/_ref) + the f64 gather on {threads} threads, "
+                          f"median of {reps} after 1 warm-up"}
     return {"value": n / t, "unit": "rows/s", "cores": threads, "kind": "port",
             "sample": f"full workload ({n} rows) argsort + f64 gather, median of {reps} after 1 warm-up; "
                       f"oracle radix_sort_uint32_mt restatement, 16-bit digits, {threads} OpenMP threads"}
@@ -214,6 +233,9 @@ def run_argsort_u32(args) -> dict:
     if not args.no_cpu:
         rec["cpu_baseline"] = cpu_baseline_argsort(keys, val, _threads(args))
         rec["speedup_vs_cpu"] = rows_per_s / rec["cpu_baseline"]["value"]
+        if rec["cpu_baseline"]["kind"] == "reference":
+            # the oracle's restatement as well (faster than the reference's own loops)
+            rec["cpu_port"] = cpu_baseline_argsort(keys, val, _threads(args), port=True)
     else:
         rec["cpu_baseline"] = None
     v.close()

@@ -59,8 +59,8 @@ def ref_lib():
     if _ref is None and os.path.exists(REF_LIB_PATH):
         L = C.CDLL(REF_LIB_PATH)
         vp, sz = C.c_void_p, C.c_size_t
-        for name in ("vref_radix_sort_u32", "vref_radix_sort_u32_mt8", "vref_radix_sort_u64",
-                     "vref_radix_sort_u64_mt8"):
+        for name in ("vref_radix_sort_u32", "vref_radix_sort_u32_mt8", "vref_radix_sort_u32_mt16",
+                     "vref_radix_sort_u64", "vref_radix_sort_u64_mt8"):
             getattr(L, name).restype = None
             getattr(L, name).argtypes = [vp, vp, sz]
         L.vref_radix_sort_charbuf.restype = None

@@ -49,6 +49,15 @@ void vref_radix_sort_u32_mt8(uint32_t* keys, size_t* idx, size_t n)
     std::memcpy(keys, t.data(), n * sizeof(uint32_t));
 }
 
+// the reference's multithreaded path at 16 threads (CORES <= MAX_THREADS = 32):
+// the CPU baseline of the benchmark
+void vref_radix_sort_u32_mt16(uint32_t* keys, size_t* idx, size_t n)
+{
+    std::vector<uint32_t> t(keys, keys + n);
+    vref::radix_sort_uint32_mt<16, false>(t, idx, n);
+    std::memcpy(keys, t.data(), n * sizeof(uint32_t));
+}
+
 void vref_radix_sort_u64(uint64_t* keys, size_t* idx, size_t n)
 {
     std::vector<uint64_t> t(keys, keys + n);
