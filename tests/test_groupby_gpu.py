@@ -232,6 +232,32 @@ def _outputs(r, vals):
     return out
 
 
+@pytest.mark.parametrize("kind", ["u64", "char16"])
+@pytest.mark.parametrize("items", [0, 8, 12, 16])
+def test_groupby_wide_keys_without_permutation(vdf, kind, items):
+    """8- and 16-byte keys with carried columns and no permutation (the passes
+    move no row ids): every other output is bit-identical to the call with the
+    permutation, which matches the oracle."""
+    knob = "sort_items_64" if kind == "u64" else "sort_items_128"
+    key, val = datagen.served_table(400_009, 7, kind)
+    ival = np.random.default_rng(7).integers(-(2**62), 2**62, key.shape[0], dtype=np.int64)
+    old = vdf.tune_get(knob)
+    for vals in ([val], [val, ival]):  # 8- and 16-byte payloads
+        d_keys, d_vals = _dev(key), [_dev(v) for v in vals]
+        try:
+            if items:
+                vdf.tune(knob, items)
+            with_p = vdf.groupby(d_keys, d_vals, want_perm=True)
+            without = vdf.groupby(d_keys, d_vals, want_perm=False)
+        finally:
+            vdf.tune(knob, old)
+        assert with_p.n_groups == without.n_groups
+        a, b = _outputs(with_p, vals), _outputs(without, vals)
+        del a[3], b[3]
+        assert all(x == y for x, y in zip(a, b)), (kind, items, len(vals))
+        _check(vdf, key, vals, label=f"{kind} reference")
+
+
 @pytest.mark.parametrize("combo", ["f64", "i32", "u32", "i64", "f64+i64", "f64+u64+i32", "i32+f64", "u64+f64+i64+i32"])
 def test_groupby_carried_columns_bit_identical(vdf, combo):
     """Value columns carried through the sort (the default) give results

@@ -421,14 +421,15 @@ struct TileCtx {
 // so registers never cut occupancy below what LDS allows.
 // SPLIT kernels stage a tile one 8-byte column at a time (LDS: 9 bytes per
 // row, the ninth the row's digit), so that larger tiles fit two blocks per CU.
-template <typename K, int ITEMS, int PB, bool SPLIT>
+template <typename K, int ITEMS, int PB, bool SPLIT, bool ROWS>
 struct PassShape {
+    static constexpr int kRid = ROWS ? 4 : 0;  // staged row-id bytes
     static constexpr int kLds =
-        (SPLIT ? kBlock * ITEMS * 9 : kBlock * ITEMS * static_cast<int>(sizeof(K) + 4 + PB)) + kWaves * kRadix * 4 +
-        kRadix * 4 + 64;
+        (SPLIT ? kBlock * ITEMS * 9 : kBlock * ITEMS * static_cast<int>(sizeof(K) + kRid + PB)) +
+        kWaves * kRadix * 4 + kRadix * 4 + 64;
     // register room (bytes of rows a thread holds): the larger tiles and
     // payloads get fewer blocks, so that no instantiation spills
-    static constexpr int kRowBytes = ITEMS * static_cast<int>(sizeof(K) + 4 + PB);
+    static constexpr int kRowBytes = ITEMS * static_cast<int>(sizeof(K) + kRid + PB);
     static constexpr int kCap = (SPLIT || ITEMS >= 24 || PB == 16 || kRowBytes > 160 || (PB != 0 && ITEMS >= 12)) ? 2 : 3;
     static constexpr int kBlocksPerCu = (160 * 1024 / kLds) < kCap ? (160 * 1024 / kLds) : kCap;
 };
@@ -439,8 +440,11 @@ struct PassShape {
 // 4-byte keys; the key (8-byte keys) or its two words (16-byte keys) and the
 // row ids otherwise; then each payload column -- and tiles are processed one
 // at a time (the registers of a column stay live until its phase).
-template <typename K, int ITEMS, int PB, bool SPLIT = false>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PassShape<K, ITEMS, PB, SPLIT>::kBlocksPerCu)))
+// ROWS = false: a kernel for sorts that move no row ids, whose tiles stage no
+// row-id column (4-byte keys: no (key, row id) records either), so larger
+// tiles fit the LDS.
+template <typename K, int ITEMS, int PB, bool SPLIT = false, bool ROWS = true>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PassShape<K, ITEMS, PB, SPLIT, ROWS>::kBlocksPerCu)))
 void k_pass(SortArgs a, uint32_t slot)
 {
     using KO = KeyOps<K>;
@@ -453,9 +457,10 @@ void k_pass(SortArgs a, uint32_t slot)
     // 4-byte keys travel between passes packed with their row ids: one 8-byte
     // record (key low, row id high) per row, so a bucket run is written as one
     // stream of full 8-byte words instead of two half-size streams
-    constexpr bool KV = std::is_same<K, uint32_t>::value;
+    constexpr bool KV = std::is_same<K, uint32_t>::value && ROWS;
     __shared__ K s_keys[(KV || SPLIT) ? 1 : TILE];
-    __shared__ uint32_t s_vals[(KV || SPLIT) ? 1 : TILE];
+    __shared__ uint32_t s_vals[(KV || SPLIT || !ROWS) ? 1 : TILE];
+    static_assert(ROWS || !SPLIT, "split kernels move row ids when asked");
     __shared__ uint64_t s_kv[(KV || SPLIT) ? TILE : 1];  // SPLIT: every phase's column
     __shared__ P s_pay[(PB && !SPLIT) ? TILE : 1];
     __shared__ uint64_t s_pay2[(TWO && !SPLIT) ? TILE : 1];
@@ -593,19 +598,19 @@ void k_pass(SortArgs a, uint32_t slot)
             if (full) {
 #pragma unroll
                 for (int j = 0; j < ITEMS; ++j) key[j] = KO::raw(kp, wl + static_cast<uint64_t>(j) * 64);
-                if (!a.norows) {
+                if (ROWS && !a.norows) {
 #pragma unroll
                     for (int j = 0; j < ITEMS; ++j) val[j] = vp[wl + static_cast<uint64_t>(j) * 64];
                 }
             } else {
 #pragma unroll
                 for (int j = 0; j < ITEMS; ++j) key[j] = KO::raw(kp, std::min(wl + static_cast<uint64_t>(j) * 64, last));
-                if (!a.norows) {
+                if (ROWS && !a.norows) {
 #pragma unroll
                     for (int j = 0; j < ITEMS; ++j) val[j] = vp[std::min(wl + static_cast<uint64_t>(j) * 64, last)];
                 }
             }
-            if (a.norows) {
+            if (!ROWS || a.norows) {
 #pragma unroll
                 for (int j = 0; j < ITEMS; ++j) val[j] = 0u;
             }
@@ -673,7 +678,7 @@ void k_pass(SortArgs a, uint32_t slot)
                     s_kv[p] = key[j] | (static_cast<uint64_t>(val[j]) << 32);
                 } else {
                     s_keys[p] = key[j];
-                    s_vals[p] = val[j];
+                    if constexpr (ROWS) s_vals[p] = val[j];
                 }
                 if constexpr (PB != 0 && !SPLIT) s_pay[p] = pay[j];
                 if constexpr (TWO && !SPLIT) s_pay2[p] = pay2[j];
@@ -734,7 +739,7 @@ void k_pass(SortArgs a, uint32_t slot)
                 vv = static_cast<uint32_t>(r >> 32);
             } else {
                 kk = s_keys[i];
-                vv = s_vals[i];
+                vv = ROWS ? s_vals[i] : 0u;
             }
         };
         if constexpr (SPLIT) {
@@ -1020,6 +1025,25 @@ using PassFn = void (*)(SortArgs, uint32_t);
 template <typename K>
 static PassFn pick_pass(int items, int pay_bytes = 0);
 
+// kernels that move no row ids (sorts without a permutation), where they exist
+template <typename K>
+static PassFn pick_pass_norows(int, int) { return nullptr; }
+template <>
+PassFn pick_pass_norows<uint32_t>(int items, int pay_bytes)
+{
+    if (pay_bytes == 8 && items == 24) return k_pass<uint32_t, 24, 8, false, false>;
+    if (pay_bytes == 16 && items == 12) return k_pass<uint32_t, 12, 16, false, false>;
+    return nullptr;
+}
+
+template <typename K>
+static PassFn pick_pass_for(int items, int pay_bytes, bool rows)
+{
+    if (!rows)
+        if (PassFn f = pick_pass_norows<K>(items, pay_bytes)) return f;
+    return pick_pass<K>(items, pay_bytes);
+}
+
 template <>
 PassFn pick_pass<uint32_t>(int items, int pay_bytes)
 {
@@ -1160,15 +1184,16 @@ static unsigned resident_blocks(PassFn fn, int device)
 // on MI355X, 100M-row sorts): the largest tile whose kernel keeps two blocks
 // per CU without spilling.
 template <typename K>
-static int default_items(int pay_bytes)
+static int default_items(int pay_bytes, bool rows)
 {
     if (sizeof(K) == 4) {
         switch (pay_bytes) {
         case 0: return 32;   // 8-byte (key, row id) records: 8192-row tiles in 64 KiB
-        case 8: return 24;   // split staging
-        default: return 12;  // 4- and 16-byte payloads
+        case 8: return 24;   // split staging; without row ids, the row-id-free kernel
+        default: return 12;  // 4- and 16-byte payloads (16 without row ids: row-id-free)
         }
     }
+    (void)rows;
     if (sizeof(K) == 8) return pay_bytes == 16 ? 8 : (pay_bytes == 8 ? 16 : 12);  // 16: split staging
     return 8;
 }
@@ -1179,10 +1204,10 @@ static void launch_sort(Handle* h, const SortSeg* segs, int nsegs, uint64_t mlo,
 {
     constexpr int D = KeyOps<K>::kDigits;
     int items = sizeof(K) == 4 ? h->tune.sort_items_32 : (sizeof(K) == 8 ? h->tune.sort_items_64 : h->tune.sort_items_128);
-    PassFn pass = items ? pick_pass<K>(items, pay_bytes) : nullptr;
+    PassFn pass = items ? pick_pass_for<K>(items, pay_bytes, rowids) : nullptr;
     if (!pass) {  // the default, or a tuning value without a kernel for this key width / payload
-        items = default_items<K>(pay_bytes);
-        pass = pick_pass<K>(items, pay_bytes);
+        items = default_items<K>(pay_bytes, rowids);
+        pass = pick_pass_for<K>(items, pay_bytes, rowids);
     }
     require(pass != nullptr, "sort payload must be 0, 4, 8 or 16 bytes per row");
     const uint64_t tile_rows = static_cast<uint64_t>(kBlock) * items;
