@@ -1204,10 +1204,11 @@ static void launch_sort(Handle* h, const SortSeg* segs, int nsegs, uint64_t mlo,
 {
     constexpr int D = KeyOps<K>::kDigits;
     int items = sizeof(K) == 4 ? h->tune.sort_items_32 : (sizeof(K) == 8 ? h->tune.sort_items_64 : h->tune.sort_items_128);
-    PassFn pass = items ? pick_pass_for<K>(items, pay_bytes, rowids) : nullptr;
+    const bool rows_kernel = rowids || !h->rowfree_kernels;
+    PassFn pass = items ? pick_pass_for<K>(items, pay_bytes, rows_kernel) : nullptr;
     if (!pass) {  // the default, or a tuning value without a kernel for this key width / payload
-        items = default_items<K>(pay_bytes, rowids);
-        pass = pick_pass_for<K>(items, pay_bytes, rowids);
+        items = default_items<K>(pay_bytes, rows_kernel);
+        pass = pick_pass_for<K>(items, pay_bytes, rows_kernel);
     }
     require(pass != nullptr, "sort payload must be 0, 4, 8 or 16 bytes per row");
     const uint64_t tile_rows = static_cast<uint64_t>(kBlock) * items;

@@ -348,6 +348,7 @@ vdf_status vdf_server_create(vdf_server** out, const vdf_server_config* cfg)
                 W.h = std::make_unique<Handle>();
                 W.h->device = cfg->device;
                 W.h->tune.sort_xcd = 0;  // plain launches: passes of several workers run together
+                W.h->rowfree_kernels = false;  // the kernels the served workload was validated with
                 VDF_HIP_CHECK(hipStreamCreateWithFlags(&W.stream, hipStreamNonBlocking));
                 VDF_HIP_CHECK(hipMalloc(&W.d_ng, static_cast<size_t>(nt) * 8));
                 VDF_HIP_CHECK(hipHostMalloc(&W.h_ng, static_cast<size_t>(nt) * 8, hipHostMallocDefault));

@@ -176,6 +176,9 @@ struct Handle {
     // Group-by sorts always move row ids (graph plans: the outputs of a replay,
     // with or without a permutation, are only known after capture).
     bool keep_rowids = false;
+    // row-id-free pass kernels for sorts without row ids (off for server
+    // workers, whose sorts run concurrently on several streams)
+    bool rowfree_kernels = true;
 
     void* ensure_scratch(size_t bytes);
     void staging_begin(size_t bytes);      // start a call's staging region
