@@ -40,8 +40,8 @@ HBM_PEAK_GBPS = 8000.0
 METRIC = "group-by rows/sec + HBM GB/s (% peak) at 100M rows; served p50/p99 latency"
 KERNELS = ("radix_histogram", "radix_plan", "radix_pass", "gather", "fill", "groupby_empty", "groupby_tile",
            "groupby_fixup")
-WORKLOADS = ("argsort_u32", "zipf_groupby", "batched", "served")
-CHILD_TIMEOUT = {"argsort_u32": 600, "zipf_groupby": 400, "batched": 500, "served": 500}
+WORKLOADS = ("argsort_u32", "groupby_i32", "zipf_groupby", "batched", "served")
+CHILD_TIMEOUT = {"argsort_u32": 600, "groupby_i32": 400, "zipf_groupby": 400, "batched": 500, "served": 500}
 
 
 def _log(*a):
@@ -259,6 +259,48 @@ def _cpu_groupby(tables, threads: int, what: str) -> dict:
     return {"value": rows / dt, "unit": "rows/s", "cores": threads, "kind": "port",
             "sample": f"{what}; oracle restatement of the reference's radix sort + group-by loops, "
                       f"{threads} OpenMP threads"}
+
+
+def run_groupby_i32(args) -> dict:
+    """100M rows, int32 key uniform in [0, 1000), float64 and int32 values,
+    every aggregate (the CPU reference configuration's group-by at 100M rows)."""
+    import torch
+
+    from visondf_amd import _native as N
+    from visondf_amd import datagen
+    from visondf_amd.device import Vdf
+
+    n = args.rows or 100_000_000
+    _log(f"groupby_i32: generating {n} rows")
+    key, vf, vi = datagen.groupby_i32(n)
+    d_key = torch.from_numpy(key).cuda()
+    d_vf = torch.from_numpy(vf).cuda()
+    d_vi = torch.from_numpy(vi).cuda()
+    v = Vdf(0)
+    res = {}
+
+    def step():
+        res["r"] = v.groupby(d_key, [d_vf, d_vi], aggs=N.AGG_ALL, want_perm=False, trim=False)
+
+    ms_total, kernels = _time_steps(step, args.steps, args.warmup, v)
+    ms_step = ms_total / args.steps
+    rows_per_s = n * args.steps / (ms_total * 1e-3)
+    g = int(res["r"].n_groups.item())
+    cnt = res["r"].counts[:g].to(torch.int64).sum().item()
+    assert cnt == n, "group counts do not add up to the rows"
+    rec = _base_record(args, rows_per_s, "rows/s", ms_step, "f64+i32",
+                       {"workload": "groupby_i32", "rows": n, "key": "int32 uniform [0, 1000)",
+                        "values": ["float64 uniform [0,1)", "int32 uniform [-1e6,1e6]"],
+                        "aggregates": "count,sum,mean,min,max", "groups": g, "parallelism": "single GPU"})
+    rec["roofline"] = _roofline("groupby_i32", kernels)
+    rec["kernels"] = kernels
+    if not args.no_cpu:
+        m = min(n, 20_000_000)
+        rec["cpu_baseline"] = _cpu_groupby([(key[:m], [vf[:m], vi[:m]])], _threads(args),
+                                           f"first {m} rows of the workload")
+        rec["speedup_vs_cpu"] = rows_per_s / rec["cpu_baseline"]["value"]
+    v.close()
+    return rec
 
 
 def run_zipf_groupby(args) -> dict:
@@ -507,13 +549,13 @@ def run_served(args) -> dict:
     return rec
 
 
-RUNNERS = {"argsort_u32": run_argsort_u32, "zipf_groupby": run_zipf_groupby, "batched": run_batched,
+RUNNERS = {"argsort_u32": run_argsort_u32, "groupby_i32": run_groupby_i32, "zipf_groupby": run_zipf_groupby, "batched": run_batched,
            "served": run_served}
 
 
 def _child(name: str, args) -> dict:
     steps, warmup = args.steps, args.warmup
-    if name in ("zipf_groupby", "batched"):  # long steps: fewer of them
+    if name in ("groupby_i32", "zipf_groupby", "batched"):  # long steps: fewer of them
         steps, warmup = max(3, args.steps // 4), min(args.warmup, 2)
     cmd = [sys.executable, os.path.abspath(__file__), "--workload", name, "--steps", str(steps),
            "--warmup", str(warmup)]
@@ -542,7 +584,7 @@ def main():
     ap.add_argument("--workload", choices=WORKLOADS, default=None,
                     help="run one workload in this process (default: all, each in a child process)")
     ap.add_argument("--workloads", default=",".join(WORKLOADS), help="workloads of the default run")
-    ap.add_argument("--rows", type=int, default=0, help="rows of argsort_u32 / zipf_groupby (0 = the workload's)")
+    ap.add_argument("--rows", type=int, default=0, help="rows of argsort_u32 / groupby_i32 / zipf_groupby (0 = the workload's)")
     ap.add_argument("--tables", type=int, default=0, help="tables of batched / served (0 = the workload's)")
     ap.add_argument("--requests", type=int, default=1000, help="served: requests replayed")
     ap.add_argument("--rate", type=float, default=2000.0, help="served: Poisson arrival rate, requests/s")

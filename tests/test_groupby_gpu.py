@@ -211,6 +211,40 @@ def test_groupby_zipf_20m(vdf):
     _check(vdf, key, [vf, vi], label="zipf20m")
 
 
+def test_groupby_100m_i32(vdf):
+    """The 100M-row int32-key group-by (key uniform in [0, 1000), float64 and
+    int32 values): every output against the oracle at full size."""
+    key, vf, vi = datagen.groupby_i32(100_000_000)
+    _check(vdf, key, [vf, vi], label="i32 100m")
+
+
+@pytest.mark.parametrize("items", [0, 8, 12, 16])
+@pytest.mark.parametrize("n", [1, 4095, 4097, 12_289, 1_000_003])
+def test_groupby_carried_8_4_pair(vdf, items, n):
+    """4-byte keys carrying an 8-byte and a 4-byte column (12-byte payload)
+    under every pass tile size, with the permutation (row-id kernels) and
+    without it (row-id-free kernel): the two calls agree bit for bit outside
+    the permutation, and the first matches the oracle."""
+    rng = np.random.default_rng(n + items)
+    keys = rng.integers(-(2**31), 2**31, n, dtype=np.int64).astype(np.int32) // max(n // 50, 1)
+    vals = [rng.standard_normal(n), rng.integers(-(2**31), 2**31, n, dtype=np.int64).astype(np.int32)]
+    d_keys, d_vals = _dev(keys), [_dev(v) for v in vals]
+    old = vdf.tune_get("sort_items_32")
+    try:
+        vdf.tune("sort_items_32", items)
+        with_p = vdf.groupby(d_keys, d_vals, want_perm=True)
+        without = vdf.groupby(d_keys, d_vals, want_perm=False)
+    finally:
+        vdf.tune("sort_items_32", old)
+    ref = vdf.groupby(d_keys, d_vals, want_perm=True)  # default tiles
+    assert with_p.n_groups == without.n_groups == ref.n_groups
+    assert _outputs(with_p, vals) == _outputs(ref, vals), (n, items)
+    a, b = _outputs(with_p, vals), _outputs(without, vals)
+    del a[3], b[3]
+    assert all(x == y for x, y in zip(a, b)), (n, items)
+    _check(vdf, keys, vals, label=f"8+4 pair n={n} items={items}")
+
+
 def test_groupby_deterministic_repeat(vdf):
     """Same table twice (and after other work) gives bit-identical results."""
     key, vf, _ = datagen.zipf_groupby(2_000_000, seed=8)
@@ -258,7 +292,8 @@ def test_groupby_wide_keys_without_permutation(vdf, kind, items):
         _check(vdf, key, vals, label=f"{kind} reference")
 
 
-@pytest.mark.parametrize("combo", ["f64", "i32", "u32", "i64", "f64+i64", "f64+u64+i32", "i32+f64", "u64+f64+i64+i32"])
+@pytest.mark.parametrize("combo", ["f64", "i32", "u32", "i64", "f64+i64", "f64+u64+i32", "i32+f64", "u64+f64+i64+i32",
+                                   "f64+i32", "f64+u32", "i64+i32+f64"])
 def test_groupby_carried_columns_bit_identical(vdf, combo):
     """Value columns carried through the sort (the default) give results
     bit-identical to gathering every column through the permutation, for every

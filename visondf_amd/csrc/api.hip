@@ -336,7 +336,7 @@ vdf_status vdf_groupby(vdf_handle* h, int key_type, const void* d_keys, uint64_t
         const uint64_t hr = static_cast<uint64_t>(h->tune.hist_rows);
         int vt[VDF_MAX_VALUE_COLS] = {0, 0, 0, 0};
         for (int c = 0; c < nvals; ++c) vt[c] = vals[c].type;
-        const int pay = groupby_carry_bytes(h->tune, vt, nvals);
+        const int pay = groupby_carry_bytes(h->tune, key_type, vt, nvals);
         void* scr = h->ensure_scratch(groupby_batch_scratch(key_type, &n, 1, hr, pay));
         h->staging_begin(groupby_batch_staging(key_type, &n, 1, hr));
         StagingScope ss(h, static_cast<hipStream_t>(stream));
@@ -407,7 +407,7 @@ vdf_status vdf_groupby_batch(vdf_handle* h, int key_type, const vdf_table* table
         CallScope cs(h);
         const uint64_t hr = static_cast<uint64_t>(h->tune.hist_rows);
         void* scr = h->ensure_scratch(groupby_batch_scratch(key_type, ns.data(), ntables, hr,
-                                                            groupby_carry_bytes(h->tune, val_types, nvals)));
+                                                            groupby_carry_bytes(h->tune, key_type, val_types, nvals)));
         h->staging_begin(groupby_batch_staging(key_type, ns.data(), ntables, hr));
         StagingScope st(h, static_cast<hipStream_t>(stream));
         groupby_batch_enqueue(h, key_type, ts.data(), ntables, val_types, nvals, agg_mask, outs, scr,
@@ -457,7 +457,7 @@ vdf_status vdf_groupby_multi(vdf_handle* h, const vdf_key_col* keys, int nkeys, 
         const size_t cb = key_bytes(kt);
         const uint64_t cap = (out->capacity == 0 || out->capacity > n) ? n : out->capacity;
         const uint64_t hr = static_cast<uint64_t>(h->tune.hist_rows);
-        const size_t gb = groupby_batch_scratch(kt, &n, 1, hr, groupby_carry_bytes(h->tune, vts.data(), nvals));
+        const size_t gb = groupby_batch_scratch(kt, &n, 1, hr, groupby_carry_bytes(h->tune, kt, vts.data(), nvals));
         void* scr = h->ensure_scratch(align_up(n * cb, 256) + align_up(std::max<uint64_t>(cap, 1) * cb, 256) + gb +
                                       512);
         Carve cv(scr);

@@ -1089,12 +1089,14 @@ GbLayout gb_layout(int key_type, const uint64_t* n, int ntables, uint64_t hist_r
 // aggregate kernel then reads them sequentially instead of gathering them
 // through the permutation, a random 8-byte read per row and column that costs
 // a whole memory line): column 0 when it is 4 or 8 bytes, columns 0 and 1 when
-// both are 8 bytes.  Returns the payload bytes per row (0: gather everything).
-int groupby_carry_bytes(const Tuning& t, const int* vt, int nv)
+// both are 8 bytes, or (4-byte keys) when column 0 is 8 bytes and column 1 is
+// 4 bytes.  Returns the payload bytes per row (0: gather everything).
+int groupby_carry_bytes(const Tuning& t, int key_type, const int* vt, int nv)
 {
     if (!t.gb_carry || nv <= 0) return 0;
     const int w0 = val_bytes(vt[0]);
     if (nv >= 2 && w0 == 8 && val_bytes(vt[1]) == 8) return 16;
+    if (nv >= 2 && w0 == 8 && val_bytes(vt[1]) == 4 && key_bytes(key_type) == 4) return 12;
     return w0;
 }
 
@@ -1168,12 +1170,12 @@ void groupby_aggregate_enqueue(Handle* h, int key_type, const void* skeys, const
         g.cap = (outs[i].capacity == 0 || outs[i].capacity > n) ? n : outs[i].capacity;
         for (int c = 0; c < nvals; ++c) g.vals[c] = tables[i].vals[c];
         if (spay != nullptr && pay_bytes != 0) {
-            // carried columns: 0 (4 or 8 bytes per row), and 1 in the upper
-            // half of a 16-byte payload
+            // carried columns: 0 (4 or 8 bytes per row), and with a 12- or
+            // 16-byte payload 1 (4 or 8 bytes per row) after column 0's rows
             const char* base = static_cast<const char*>(spay);
-            const uint64_t w0 = pay_bytes == 16 ? 8 : static_cast<uint64_t>(pay_bytes);
+            const uint64_t w0 = pay_bytes >= 12 ? 8 : static_cast<uint64_t>(pay_bytes);
             g.svals[0] = base + row * w0;
-            if (pay_bytes == 16) g.svals[1] = base + rows_total * 8 + row * 8;
+            if (pay_bytes >= 12) g.svals[1] = base + rows_total * 8 + row * static_cast<uint64_t>(pay_bytes - 8);
         }
         g.out = outs[i];
         hs[i] = g;
@@ -1209,7 +1211,7 @@ void groupby_aggregate_enqueue(Handle* h, int key_type, const void* skeys, const
         a.cols[0] = GbColType{VDF_VAL_U32, KIND_COUNT};
     } else {
         a.ncols = nvals;
-        const int ncarried = pay_bytes == 16 ? 2 : (pay_bytes != 0 ? 1 : 0);
+        const int ncarried = pay_bytes >= 12 ? 2 : (pay_bytes != 0 ? 1 : 0);
         for (int c = 0; c < nvals; ++c) {
             a.cols[c] = GbColType{val_types[c], kind_of(val_types[c])};
             // a carried column is read in order; a gathered one costs its row id too
@@ -1252,7 +1254,7 @@ void groupby_batch_enqueue(Handle* h, int key_type, const GbTable* tables, int n
     std::vector<uint64_t> ns(ntables);
     for (int i = 0; i < ntables; ++i) ns[i] = tables[i].n;
     const uint64_t hist_rows = static_cast<uint64_t>(h->tune.hist_rows);
-    const int pay = groupby_carry_bytes(h->tune, val_types, nvals);
+    const int pay = groupby_carry_bytes(h->tune, key_type, val_types, nvals);
     const GbLayout L = gb_layout(key_type, ns.data(), ntables, hist_rows, pay);
     const size_t kb = key_bytes(key_type);
     Carve cv(scr);
@@ -1267,10 +1269,10 @@ void groupby_batch_enqueue(Handle* h, int key_type, const GbTable* tables, int n
     //    second column in the upper half of spay)
     std::vector<SortSeg> ss(ntables);
     std::vector<const uint32_t*> perms(ntables);
-    const uint64_t w0 = pay == 16 ? 8 : static_cast<uint64_t>(pay);
+    const uint64_t w0 = pay >= 12 ? 8 : static_cast<uint64_t>(pay);
     // The permutation is needed for gathered columns and for callers that
     // take it; otherwise the passes do not move row ids at all.
-    const int ncarried = pay == 16 ? 2 : (pay != 0 ? 1 : 0);
+    const int ncarried = pay >= 12 ? 2 : (pay != 0 ? 1 : 0);
     bool rowids = h->keep_rowids || nvals > ncarried;
     for (int i = 0; i < ntables; ++i) rowids = rowids || outs[i].perm != nullptr;
     uint64_t row = 0;
@@ -1282,9 +1284,9 @@ void groupby_batch_enqueue(Handle* h, int key_type, const GbTable* tables, int n
             ss[i].pay_in = tables[i].vals[0];
             ss[i].pay_out = spay + row * w0;
         }
-        if (pay == 16) {
+        if (pay >= 12) {
             ss[i].pay_in2 = tables[i].vals[1];
-            ss[i].pay_out2 = spay + L.rows * 8 + row * 8;
+            ss[i].pay_out2 = spay + L.rows * 8 + row * static_cast<uint64_t>(pay - 8);
         }
         row += align_up(tables[i].n, kGbRows);
     }

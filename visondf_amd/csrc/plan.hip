@@ -156,7 +156,7 @@ vdf_status vdf_plan_groupby_batch(vdf_handle* h, int key_type, const vdf_table* 
         }
         const uint64_t hr = static_cast<uint64_t>(p->ph.tune.hist_rows);
         p->scratch_bytes = groupby_batch_scratch(key_type, p->ns.data(), ntables, hr,
-                                                 groupby_carry_bytes(p->ph.tune, p->val_types, p->nvals));
+                                                 groupby_carry_bytes(p->ph.tune, key_type, p->val_types, p->nvals));
         p->staging_bytes = groupby_batch_staging(key_type, p->ns.data(), ntables, hr);
         try {
             if (hipMalloc(&p->scratch, p->scratch_bytes) != hipSuccess) {

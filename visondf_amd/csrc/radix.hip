@@ -434,8 +434,8 @@ struct PassShape {
     static constexpr int kBlocksPerCu = (160 * 1024 / kLds) < kCap ? (160 * 1024 / kLds) : kCap;
 };
 
-// PB: payload bytes per row moved with the keys: 0, 4, 8, or 16 (two 8-byte
-// columns).  SPLIT: the staged tile is written out in phases, one 8-byte
+// PB: payload bytes per row moved with the keys: 0, 4, 8, 12 (an 8-byte and a
+// 4-byte column) or 16 (two 8-byte columns).  SPLIT: the staged tile is written out in phases, one 8-byte
 // column at a time staged at the rows' slots -- (key, row id) records for
 // 4-byte keys; the key (8-byte keys) or its two words (16-byte keys) and the
 // row ids otherwise; then each payload column -- and tiles are processed one
@@ -449,7 +449,8 @@ void k_pass(SortArgs a, uint32_t slot)
 {
     using KO = KeyOps<K>;
     using P = typename std::conditional<PB >= 8, uint64_t, uint32_t>::type;
-    constexpr bool TWO = PB == 16;        // a second 8-byte payload column
+    using P2 = typename std::conditional<PB == 12, uint32_t, uint64_t>::type;
+    constexpr bool TWO = PB == 12 || PB == 16;  // a second payload column (4 or 8 bytes)
     constexpr int PI = PB ? ITEMS : 1;    // payload registers
     constexpr int PI2 = TWO ? ITEMS : 1;
     constexpr int TILE = kBlock * ITEMS;
@@ -463,7 +464,7 @@ void k_pass(SortArgs a, uint32_t slot)
     static_assert(ROWS || !SPLIT, "split kernels move row ids when asked");
     __shared__ uint64_t s_kv[(KV || SPLIT) ? TILE : 1];  // SPLIT: every phase's column
     __shared__ P s_pay[(PB && !SPLIT) ? TILE : 1];
-    __shared__ uint64_t s_pay2[(TWO && !SPLIT) ? TILE : 1];
+    __shared__ P2 s_pay2[(TWO && !SPLIT) ? TILE : 1];
     __shared__ uint8_t s_dig[SPLIT ? TILE : 1];  // SPLIT: digit of each staged slot
     __shared__ uint32_t s_wc[kWaves][kRadix];
     __shared__ uint32_t s_gbase[kRadix];
@@ -478,7 +479,7 @@ void k_pass(SortArgs a, uint32_t slot)
     K key[ITEMS];
     uint32_t val[ITEMS];
     P pay[PI];
-    uint64_t pay2[PI2];
+    P2 pay2[PI2];
     uint32_t rnk[ITEMS];
 
     auto claim = [&]() -> uint32_t {
@@ -551,8 +552,8 @@ void k_pass(SortArgs a, uint32_t slot)
             }
         }
         if constexpr (TWO) {
-            const auto* pp = c.first ? gmem(static_cast<const uint64_t*>(c.pay_in2))
-                                     : gmem(static_cast<const uint64_t*>(a.pbuf2[(slot - 1) & 1]) + c.row_off);
+            const auto* pp = c.first ? gmem(static_cast<const P2*>(c.pay_in2))
+                                     : gmem(static_cast<const P2*>(a.pbuf2[(slot - 1) & 1]) + c.row_off);
             if (full) {
 #pragma unroll
                 for (int j = 0; j < PI2; ++j) pay2[j] = pp[wl + static_cast<uint64_t>(j) * 64];
@@ -829,9 +830,10 @@ void k_pass(SortArgs a, uint32_t slot)
                       [&](uint32_t p, uint64_t v) { (c.last ? fin : mid)[p] = static_cast<P>(v); });
             }
             if constexpr (TWO) {
-                auto* mid = gmem(static_cast<uint64_t*>(a.pbuf2[slot & 1]) + c.row_off);
-                auto* fin = gmem(static_cast<uint64_t*>(c.pay_out2));
-                phase([&](int j) { return pay2[j]; }, [&](uint32_t p, uint64_t v) { (c.last ? fin : mid)[p] = v; });
+                auto* mid = gmem(static_cast<P2*>(a.pbuf2[slot & 1]) + c.row_off);
+                auto* fin = gmem(static_cast<P2*>(c.pay_out2));
+                phase([&](int j) { return static_cast<uint64_t>(pay2[j]); },
+                      [&](uint32_t p, uint64_t v) { (c.last ? fin : mid)[p] = static_cast<P2>(v); });
             }
             lds_barrier();
             return;
@@ -841,12 +843,12 @@ void k_pass(SortArgs a, uint32_t slot)
             auto* rout = gmem(static_cast<uint64_t*>(a.kbuf[slot & 1]) + c.row_off);
             auto* vout = gmem(a.vbuf[slot & 1] + c.row_off);
             auto* pout = gmem(static_cast<P*>(a.pbuf[slot & 1]) + c.row_off);
-            auto* pout2 = gmem(static_cast<uint64_t*>(a.pbuf2[slot & 1]) + c.row_off);
+            auto* pout2 = gmem(static_cast<P2*>(a.pbuf2[slot & 1]) + c.row_off);
             for (uint32_t i0 = tid; i0 < tn; i0 += U * kBlock) {
                 K kk[U];
                 uint32_t pos[U], vv[U];
                 P pv[U];
-                uint64_t pv2[U];
+                P2 pv2[U];
 #pragma unroll
                 for (int u = 0; u < U; ++u) staged(i0 + u * kBlock, kk[u], vv[u]);
 #pragma unroll
@@ -875,7 +877,7 @@ void k_pass(SortArgs a, uint32_t slot)
                 K kk[U];
                 uint32_t pos[U], vv[U];
                 P pv[U];
-                uint64_t pv2[U];
+                P2 pv2[U];
 #pragma unroll
                 for (int u = 0; u < U; ++u) staged(i0 + u * kBlock, kk[u], vv[u]);
 #pragma unroll
@@ -897,7 +899,7 @@ void k_pass(SortArgs a, uint32_t slot)
                                 gmem(static_cast<uint32_t*>(c.perm_out))[p] = vv[u];
                         }
                         if constexpr (PB != 0) gmem(static_cast<P*>(c.pay_out))[p] = pv[u];
-                        if constexpr (TWO) gmem(static_cast<uint64_t*>(c.pay_out2))[p] = pv2[u];
+                        if constexpr (TWO) gmem(static_cast<P2*>(c.pay_out2))[p] = pv2[u];
                     }
                 }
             }
@@ -1033,6 +1035,7 @@ PassFn pick_pass_norows<uint32_t>(int items, int pay_bytes)
 {
     if (pay_bytes == 8 && items == 24) return k_pass<uint32_t, 24, 8, false, false>;
     if (pay_bytes == 16 && items == 12) return k_pass<uint32_t, 12, 16, false, false>;
+    if (pay_bytes == 12 && items == 16) return k_pass<uint32_t, 16, 12, false, false>;
     return nullptr;
 }
 
@@ -1063,6 +1066,10 @@ PassFn pick_pass<uint32_t>(int items, int pay_bytes)
         if (items == 12) return k_pass<uint32_t, 12, 8>;
         if (items == 16) return k_pass<uint32_t, 16, 8>;
         if (items == 24) return k_pass<uint32_t, 24, 8, true>;
+        return nullptr;
+    case 12:  // an 8-byte and a 4-byte column (4-byte keys only)
+        if (items == 8) return k_pass<uint32_t, 8, 12>;
+        if (items == 12) return k_pass<uint32_t, 12, 12>;
         return nullptr;
     case 16:
         if (items == 8) return k_pass<uint32_t, 8, 16>;
@@ -1190,6 +1197,7 @@ static int default_items(int pay_bytes, bool rows)
         switch (pay_bytes) {
         case 0: return 32;   // 8-byte (key, row id) records: 8192-row tiles in 64 KiB
         case 8: return 24;   // split staging; without row ids, the row-id-free kernel
+        case 12: return rows ? 12 : 16;  // 8 + 4 bytes; without row ids 4096-row tiles
         default: return 12;  // 4- and 16-byte payloads (16 without row ids: row-id-free)
         }
     }
@@ -1264,8 +1272,8 @@ static void launch_sort(Handle* h, const SortSeg* segs, int nsegs, uint64_t mlo,
     a.pbuf[0] = c.take<char>(L.rows * static_cast<uint64_t>(pay_bytes));
     a.pbuf[1] = c.take<char>(L.rows * static_cast<uint64_t>(pay_bytes));
     // a 16-byte payload is two 8-byte columns: the second in the upper half
-    a.pbuf2[0] = static_cast<char*>(a.pbuf[0]) + (pay_bytes == 16 ? L.rows * 8 : 0);
-    a.pbuf2[1] = static_cast<char*>(a.pbuf[1]) + (pay_bytes == 16 ? L.rows * 8 : 0);
+    a.pbuf2[0] = static_cast<char*>(a.pbuf[0]) + (pay_bytes >= 12 ? L.rows * 8 : 0);
+    a.pbuf2[1] = static_cast<char*>(a.pbuf[1]) + (pay_bytes >= 12 ? L.rows * 8 : 0);
     const size_t up = reinterpret_cast<char*>(ht + ntiles) - stg;
     VDF_HIP_CHECK(hipMemcpyAsync(dsegs, stg, up, hipMemcpyHostToDevice, s));  // descs, items, tile map
     // the host carve mirrors the device carve (same order, same alignment)

@@ -174,7 +174,7 @@ void Server::run(int w)
                     plan = plan_batch(items, lim, arena, [&](const std::vector<uint64_t>& ns) {
                         return static_cast<uint64_t>(
                             groupby_batch_scratch(ht.key_type, ns.data(), static_cast<int>(ns.size()), hist_rows,
-                                                  groupby_carry_bytes(h.tune, ht.val_types, ht.nvals)));
+                                                  groupby_carry_bytes(h.tune, ht.key_type, ht.val_types, ht.nvals)));
                     });
                     if (plan.head_never_fits) {
                         Request* r = queue.front();

@@ -35,7 +35,7 @@ void run_groupby(Handle* h, int kt, const std::vector<GbTable>& ts, const int* v
     for (size_t i = 0; i < ts.size(); ++i) ns[i] = ts[i].n;
     const int nt = static_cast<int>(ts.size());
     const uint64_t hr = static_cast<uint64_t>(h->tune.hist_rows);
-    void* scr = h->ensure_scratch(groupby_batch_scratch(kt, ns.data(), nt, hr, groupby_carry_bytes(h->tune, vt, nv)));
+    void* scr = h->ensure_scratch(groupby_batch_scratch(kt, ns.data(), nt, hr, groupby_carry_bytes(h->tune, kt, vt, nv)));
     h->staging_begin(groupby_batch_staging(kt, ns.data(), nt, hr));
     try {
         groupby_batch_enqueue(h, kt, ts.data(), nt, vt, nv, mask, outs, scr, s);

@@ -294,7 +294,7 @@ struct GbTable {
 size_t groupby_scratch(int key_type, uint64_t n);
 // Payload bytes per row of the value columns a group-by carries through its
 // sort (see groupby.hip); 0 when every column is gathered.
-int groupby_carry_bytes(const Tuning& t, const int* val_types, int nvals);
+int groupby_carry_bytes(const Tuning& t, int key_type, const int* val_types, int nvals);
 size_t groupby_batch_scratch(int key_type, const uint64_t* n, int ntables, uint64_t hist_rows, int pay_bytes);
 size_t groupby_batch_staging(int key_type, const uint64_t* n, int ntables, uint64_t hist_rows);
 void groupby_enqueue(Handle* h, int key_type, const void* d_keys, uint64_t n,

@@ -135,6 +135,16 @@ def groupby_small(n: int = 1_000_000, seed: int = 1):
     return key, vf, vi
 
 
+def groupby_i32(n: int = 100_000_000, seed: int = 6):
+    """The CPU reference configuration's group-by (key uniform in [0, 1000),
+    float64 uniform [0, 1), int32 uniform [-1e6, 1e6]) at 100M rows with an
+    int32 key column."""
+    key = uniform_int(seed, 0, n, 0, 999).astype(np.int32)
+    vf = uniform01(seed, 1, n)
+    vi = uniform_int(seed, 2, n, -1_000_000, 1_000_000).astype(np.int32)
+    return key, vf, vi
+
+
 def argsort_u32(n: int = 100_000_000, seed: int = 2):
     key = (splitmix64(seed, 0, n) >> np.uint64(32)).astype(np.uint32)
     val = uniform01(seed, 1, n)
