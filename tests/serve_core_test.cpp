@@ -168,11 +168,62 @@ static void planner_keeps_room_for_skipped()
     CHECK(!p2.picked.empty() && p2.picked[0] == 0);  // the older request runs
 }
 
+// Requests passed for the row limit, and several passed requests at once,
+// keep their room too: after the batch, with its results still held, each
+// passed request still fits when it becomes the oldest.
+static void planner_keeps_room_for_row_limit_and_several()
+{
+    const uint64_t A = 256;
+    auto scr = [](const std::vector<uint64_t>& ns) { return uint64_t(ns.size() * 2 * 256); };
+    // row limit: the head (100 rows) and a 900-row request over max_rows 500,
+    // then small newer requests of the same signature
+    {
+        RangeAllocator ra(40 * A, A);
+        std::vector<PlanItem> q = {{100, 0, 6 * A}, {900, 0, 20 * A}, {50, 0, 6 * A}, {50, 0, 6 * A}, {50, 0, 6 * A}};
+        PlanLimits lim;
+        lim.max_rows = 500;
+        Plan p = plan_batch(q, lim, ra, scr);
+        CHECK(!p.picked.empty() && p.picked[0] == 0);
+        for (size_t k : p.picked) CHECK(k != 1);
+        CHECK(ra.largest_free() >= 22 * A);
+        ra.release(p.scr_off);  // finished; results held
+        Plan p2 = plan_batch({q[1]}, lim, ra, scr);
+        CHECK(p2.picked.size() == 1);
+    }
+    // two passed requests of another signature need room side by side
+    {
+        RangeAllocator ra(50 * A, A);
+        std::vector<PlanItem> q = {{1, 0, 4 * A}, {1, 1, 15 * A}, {1, 1, 15 * A}, {1, 0, 4 * A}, {1, 0, 4 * A},
+                                   {1, 0, 4 * A}, {1, 0, 4 * A}};
+        Plan p = plan_batch(q, PlanLimits{}, ra, scr);
+        CHECK(!p.picked.empty() && p.picked[0] == 0);
+        ra.release(p.scr_off);  // finished; results held
+        // both older sig-1 requests run (one batch of two, or one after the other
+        // with the first one's results held)
+        Plan p2 = plan_batch({q[1], q[2]}, PlanLimits{}, ra, scr);
+        CHECK(!p2.picked.empty());
+        if (p2.picked.size() == 1) {
+            ra.release(p2.scr_off);
+            Plan p3 = plan_batch({q[2]}, PlanLimits{}, ra, scr);
+            CHECK(p3.picked.size() == 1);
+        }
+    }
+    // the peak counts admitted memory only, not the holds of planning
+    {
+        RangeAllocator ra(40 * A, A);
+        std::vector<PlanItem> q = {{1, 0, 4 * A}, {1, 1, 20 * A}};
+        Plan p = plan_batch(q, PlanLimits{}, ra, scr);
+        CHECK(p.picked.size() == 1);
+        CHECK(ra.peak() == 6 * A);
+    }
+}
+
 int main()
 {
     for (uint64_t s = 1; s <= 4; ++s) allocator_random(s);
     planner_rules();
     planner_keeps_room_for_skipped();
+    planner_keeps_room_for_row_limit_and_several();
     if (g_fail) {
         std::printf("%d failures\n", g_fail);
         return 1;

@@ -11,6 +11,7 @@
 #include <cstdint>
 #include <iterator>
 #include <map>
+#include <set>
 #include <vector>
 
 namespace vdf {
@@ -29,15 +30,16 @@ public:
         size_ = size / align_ * align_;
         free_.clear();
         live_.clear();
-        used_ = peak_ = 0;
+        used_ = peak_ = held_ = 0;
+        holds_.clear();
         if (size_) free_[0] = size_;
     }
 
     uint64_t round(uint64_t bytes) const { return (std::max<uint64_t>(bytes, 1) + align_ - 1) / align_ * align_; }
 
     // Offset of a new range of at least `bytes`, or kNone when no free range is
-    // large enough.
-    uint64_t alloc(uint64_t bytes)
+    // large enough (hold: a transient reservation that the peak leaves out).
+    uint64_t alloc(uint64_t bytes, bool hold = false)
     {
         const uint64_t need = round(bytes);
         for (auto it = free_.begin(); it != free_.end(); ++it) {
@@ -48,7 +50,11 @@ public:
             if (rest) free_[off + need] = rest;
             live_[off] = need;
             used_ += need;
-            peak_ = std::max(peak_, used_);
+            if (hold) {
+                held_ += need;
+                holds_.insert(off);
+            }
+            peak_ = std::max(peak_, used_ - held_);
             return off;
         }
         return kNone;
@@ -62,6 +68,7 @@ public:
         uint64_t beg = off, len = it->second;
         live_.erase(it);
         used_ -= len;
+        if (holds_.erase(off)) held_ -= len;
         auto nx = free_.lower_bound(beg);
         if (nx != free_.end() && nx->first == beg + len) {
             len += nx->second;
@@ -94,7 +101,8 @@ public:
 private:
     uint64_t align_;
     uint64_t size_ = 0;
-    uint64_t used_ = 0, peak_ = 0;
+    uint64_t used_ = 0, peak_ = 0, held_ = 0;
+    std::set<uint64_t> holds_;  // live ranges that are planning holds
     std::map<uint64_t, uint64_t> free_;  // offset -> length
     std::map<uint64_t, uint64_t> live_;  // offset -> length
 };
@@ -127,11 +135,10 @@ struct Plan {
 // a large old request is never passed by newer ones for memory; when the
 // scratch block does not fit, the newest picks are dropped until it does.
 //
-// Requests of another signature are skipped (they wait for a batch of their
-// own), but memory is kept for them: a newer request is picked only if, with
-// its block (and finally the batch scratch) taken, one free range still holds
-// the result block plus single-request scratch of every older skipped
-// request.  Without this, newer results held by a caller that consumes
+// Requests of another signature, and requests over the row limit, are
+// skipped (they wait for a later batch), but memory is kept for each of them:
+// its result block plus single-request scratch is held while the batch is
+// planned.  Without this, newer results held by a caller that consumes
 // results in submission order could fill the arena while the older request
 // waits for memory that only the caller can free.
 //
@@ -151,47 +158,45 @@ Plan plan_batch(const std::vector<PlanItem>& queue, const PlanLimits& lim, Range
             return p;
         }
     }
+    // Every older request the batch passes (another signature, or over the
+    // row limit) gets its own need (result block + scratch alone) held in the
+    // arena while the batch is planned, so the batch only takes memory beside
+    // those needs.  Younger requests' results then never take the room an
+    // older request needs, whatever the caller holds: a caller that consumes
+    // results in submission order cannot wait on a request that no longer fits.
     uint64_t rows = 0;
-    uint64_t keep = 0;            // largest need of an older skipped request so far
     std::vector<uint64_t> ns;
-    std::vector<uint64_t> keep_at;  // `keep` when each pick was made
+    std::vector<uint64_t> held;
     for (size_t i = 0; i < queue.size() && p.picked.size() < lim.max_tables; ++i) {
         const PlanItem& q = queue[i];
-        if (q.sig != head.sig) {
+        if (q.sig != head.sig || (i > 0 && rows + q.rows > lim.max_rows)) {
             std::vector<uint64_t> one{q.rows};
-            keep = std::max(keep, arena.round(q.out_bytes) + arena.round(scratch_bytes(one)));
+            const uint64_t off = arena.alloc(arena.round(q.out_bytes) + arena.round(scratch_bytes(one)), true);
+            if (off == RangeAllocator::kNone) break;  // no room for it: take nothing younger
+            held.push_back(off);
             continue;
         }
-        if (i > 0 && rows + q.rows > lim.max_rows) continue;
         const uint64_t off = arena.alloc(q.out_bytes);
         if (off == RangeAllocator::kNone) break;
-        if (keep && arena.largest_free() < keep) {  // would starve an older request
-            arena.release(off);
-            break;
-        }
         p.picked.push_back(i);
         p.out_off.push_back(off);
         ns.push_back(q.rows);
-        keep_at.push_back(keep);
         rows += q.rows;
     }
     while (!p.picked.empty()) {
         const uint64_t sb = scratch_bytes(ns);
         const uint64_t off = arena.alloc(sb);
         if (off != RangeAllocator::kNone) {
-            if (keep_at.back() == 0 || arena.largest_free() >= keep_at.back()) {
-                p.scr_off = off;
-                p.scr_bytes = sb;
-                return p;
-            }
-            arena.release(off);
+            p.scr_off = off;
+            p.scr_bytes = sb;
+            break;
         }
         arena.release(p.out_off.back());
         p.out_off.pop_back();
         p.picked.pop_back();
         ns.pop_back();
-        keep_at.pop_back();
     }
+    for (uint64_t off : held) arena.release(off);
     return p;
 }
 
