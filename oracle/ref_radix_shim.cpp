@@ -16,6 +16,7 @@
 #include <cstring>
 #include <iostream>
 #include <numeric>
+#include <immintrin.h>
 #include <omp.h>
 #include <string>
 #include <utility>
@@ -25,6 +26,15 @@ namespace vref {
 #include "inlines/warning.inl"
 #include "inlines/radix_sorts/details/constants.inl"
 #include "inlines/radix_sorts/details/u16/get_local_u16.inl"
+// the SIMD histogram helpers the radix templates name (declared before them,
+// as vison.hpp does; the Simd = false paths used here do not call them)
+#include "inlines/radix_sorts/details/u8/get_local_u8.inl"
+#include "inlines/radix_sorts/details/u8/histogram_pass_u8_avx2.inl"
+#include "inlines/radix_sorts/details/u8/histogram_pass_u8_avx2_8x.inl"
+#include "inlines/radix_sorts/details/u32/histogram_pass_u32_avx2_8buckets.inl"
+#include "inlines/radix_sorts/details/u32/histogram_pass_u32_avx2.inl"
+#include "inlines/radix_sorts/details/u64/histogram_pass_u64_avx2.inl"
+#include "inlines/radix_sorts/details/u64/histogram_pass_u64_avx2_8x.inl"
 #include "inlines/radix_sorts/radix_sort_uint32.inl"
 #include "inlines/radix_sorts/radix_sort_uint32_mt.inl"
 #include "inlines/radix_sorts/radix_sort_uint64.inl"
