@@ -41,9 +41,7 @@ METRIC = "group-by rows/sec + HBM GB/s (% peak) at 100M rows; served p50/p99 lat
 KERNELS = ("radix_histogram", "radix_plan", "radix_pass", "gather", "fill", "groupby_empty", "groupby_tile",
            "groupby_fixup")
 WORKLOADS = ("argsort_u32", "groupby_i32", "zipf_groupby", "batched", "served")
-# The served replay stalled (no progress) in its continuous-batching phase on
-# two runs with its cause not found; it is switched off until that is fixed.
-DEFAULT_WORKLOADS = ("argsort_u32", "groupby_i32", "zipf_groupby", "batched")
+DEFAULT_WORKLOADS = ("argsort_u32", "groupby_i32", "zipf_groupby", "batched", "served")
 CHILD_TIMEOUT = {"argsort_u32": 600, "groupby_i32": 400, "zipf_groupby": 400, "batched": 500, "served": 500}
 
 
@@ -457,11 +455,14 @@ def _replay(srv, ids, rows, arrivals, picks):
     def collector():
         for i in range(R):
             ready[i].wait()
-            s = srv.wait(rids[i], timeout_s=300.0)
-            if s is None:
-                raise RuntimeError("served request did not finish in 300 s")
+            s = srv.wait(rids[i], timeout_s=60.0)
+            if s is None:  # a stall: report and stop collecting (the main thread raises)
+                _log(f"served: request {i} not finished after 60 s; stats {srv.stats()}")
+                return
             out[i] = s
             srv.release(rids[i])
+            if (i + 1) % 250 == 0:
+                _log(f"served: {i + 1} of {R} results collected")
 
     th = threading.Thread(target=collector)
     th.start()
@@ -480,13 +481,17 @@ def _replay(srv, ids, rows, arrivals, picks):
     bad = [s.status for s in out if s.status != 0]
     if bad:
         raise RuntimeError(f"{len(bad)} served requests failed (status {bad[0]})")
-    span = (max(s.done_ns for s in out) - min(s.submit_ns for s in out)) * 1e-9
-    return lat, qs, bt, span
+    t_first = min(s.submit_ns for s in out)
+    span = (max(s.done_ns for s in out) - t_first) * 1e-9
+    # queued requests (submitted, not yet started) over time, at 20 points of the makespan
+    sub = np.sort(np.array([s.submit_ns - t_first for s in out], dtype=np.int64))
+    sta = np.sort(np.array([s.start_ns - t_first for s in out], dtype=np.int64))
+    pts = np.linspace(0, span * 1e9, 21)[1:]
+    depth = (np.searchsorted(sub, pts, side="right") - np.searchsorted(sta, pts, side="right")).tolist()
+    return lat, qs, bt, span, depth
 
 
 def run_served(args) -> dict:
-    raise RuntimeError("the served workload is switched off: its continuous-batching replay stalled on two runs "
-                       "and the cause is not found yet")
     import numpy as np
     import torch
 
@@ -497,18 +502,14 @@ def run_served(args) -> dict:
     T = args.tables or 32
     R = args.requests
     rate = args.rate
-    sizes = datagen.served_table_sizes(T, 10_000, 50_000_000, seed=5)
+    sizes, kinds, arrivals, picks = datagen.served_trace(T, R, rate)
     _log(f"served: generating {T} tables, {int(sizes.sum())} rows")
     tables = []
-    for i, n in enumerate(sizes):
-        kind = "u64" if i % 2 == 0 else "char16"
+    for i, (n, kind) in enumerate(zip(sizes, kinds)):
         k, x = datagen.served_table(int(n), i, kind)
         dk = torch.from_numpy(k.view(np.int64) if kind == "u64" else k).cuda()
         tables.append((kind, int(n), dk, torch.from_numpy(x).cuda(), k, x))
     torch.cuda.synchronize()
-    rng = np.random.default_rng(6)
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, R))
-    picks = rng.integers(0, T, R)
     rows = np.array([t[1] for t in tables])
     req_rows = int(rows[picks].sum())
 
@@ -522,12 +523,13 @@ def run_served(args) -> dict:
                 if s is None or s.status != 0:
                     raise RuntimeError("warm-up request failed")
                 srv.release(r)
-            lat, qs, bt, span = _replay(srv, ids, rows, arrivals, picks)
+            lat, qs, bt, span, depth = _replay(srv, ids, rows, arrivals, picks)
             st = srv.stats()
         return {"workers": workers, "max_batch_tables": max_batch,
                 "p50_ms": 1e3 * float(np.percentile(lat, 50)), "p99_ms": 1e3 * float(np.percentile(lat, 99)),
                 "mean_ms": 1e3 * float(lat.mean()), "queue_p50_ms": 1e3 * float(np.percentile(qs, 50)),
                 "rows_per_s": req_rows / span, "requests_per_s": R / span, "makespan_s": span,
+                "queue_depth_over_makespan": depth, "max_queue_depth": int(max(depth)),
                 "mean_batch": float(bt.mean()), "max_batch": int(bt.max()), "launches": st["launches"],
                 "arena_peak_gb": st["arena_peak"] / 2**30}
 

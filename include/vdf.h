@@ -106,6 +106,11 @@ vdf_status vdf_destroy(vdf_handle* h);
 /* Scratch bytes vdf_argsort / vdf_groupby need for n rows (for budgeting). */
 uint64_t vdf_argsort_scratch_bytes(int key_type, uint64_t n, uint32_t flags, int want_keys);
 uint64_t vdf_groupby_scratch_bytes(int key_type, uint64_t n);
+/* Scratch bytes vdf_groupby_batch needs for these tables and value types on a
+ * handle with the default launch shape (what a request server reserves for a
+ * batch); 0 for an unknown key or value type.  Host arithmetic only. */
+uint64_t vdf_groupby_batch_scratch_bytes(int key_type, const uint64_t* n, int ntables, const int* val_types,
+                                         int nvals);
 
 /* ---- stable LSD radix argsort ----
  * d_perm_out[r] = row id of the r-th smallest key (stable: equal keys keep
@@ -388,7 +393,9 @@ typedef struct vdf_server_config {
     int max_batch_tables;     /* requests per launch sequence: 1 .. 65536               */
     int reserved;
     uint64_t arena_bytes;     /* HBM for scratch and results (the admission budget)     */
-    uint64_t max_batch_rows;  /* rows per launch sequence (0 = no limit)                */
+    uint64_t max_batch_rows;  /* rows per launch sequence (0 = arena_bytes / (192 *     */
+                              /* workers), at least 2^20; the oldest request is always  */
+                              /* taken whatever its rows)                               */
 } vdf_server_config;
 
 typedef struct vdf_served {

@@ -37,7 +37,7 @@ MAX_KEY_COLS = 4
 # every symbol include/vdf.h declares
 EXPORTS = (
     "vdf_last_error", "vdf_version", "vdf_create", "vdf_destroy",
-    "vdf_argsort_scratch_bytes", "vdf_groupby_scratch_bytes",
+    "vdf_argsort_scratch_bytes", "vdf_groupby_scratch_bytes", "vdf_groupby_batch_scratch_bytes",
     "vdf_argsort", "vdf_sort_by", "vdf_gather", "vdf_groupby",
     "vdf_profile_enable", "vdf_profile_reset", "vdf_profile_read",
     "vdf_tune_set", "vdf_tune_get",
@@ -135,6 +135,8 @@ def load() -> C.CDLL:
     lib.vdf_argsort_scratch_bytes.argtypes = [i32, u64, u32, i32]
     lib.vdf_groupby_scratch_bytes.restype = u64
     lib.vdf_groupby_scratch_bytes.argtypes = [i32, u64]
+    lib.vdf_groupby_batch_scratch_bytes.restype = u64
+    lib.vdf_groupby_batch_scratch_bytes.argtypes = [i32, C.POINTER(C.c_uint64), i32, C.POINTER(C.c_int), i32]
     lib.vdf_argsort.restype = i32
     lib.vdf_argsort.argtypes = [vp, i32, vp, u64, u32, vp, vp, vp]
     lib.vdf_sort_by.restype = i32

@@ -249,6 +249,18 @@ uint64_t vdf_groupby_scratch_bytes(int key_type, uint64_t n)
     return groupby_scratch(key_type, n);
 }
 
+uint64_t vdf_groupby_batch_scratch_bytes(int key_type, const uint64_t* n, int ntables, const int* val_types,
+                                         int nvals)
+{
+    if (key_bytes(key_type) == 0 || ntables < 0 || (ntables > 0 && n == nullptr)) return 0;
+    if (nvals < 0 || nvals > VDF_MAX_VALUE_COLS || (nvals > 0 && val_types == nullptr)) return 0;
+    for (int c = 0; c < nvals; ++c)
+        if (val_bytes(val_types[c]) == 0) return 0;
+    const Tuning t;
+    return groupby_batch_scratch(key_type, n, ntables, static_cast<uint64_t>(t.hist_rows),
+                                 groupby_carry_bytes(t, key_type, val_types, nvals));
+}
+
 vdf_status vdf_argsort(vdf_handle* h, int key_type, const void* d_keys, uint64_t n, uint32_t flags,
                        void* d_perm_out, void* d_sorted_keys_out, void* stream)
 {

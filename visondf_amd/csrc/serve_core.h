@@ -14,6 +14,8 @@
 #include <set>
 #include <vector>
 
+#include "../../include/vdf.h"
+
 namespace vdf {
 
 // First-fit allocator of byte ranges inside one arena of `size` bytes.  Every
@@ -109,9 +111,11 @@ private:
 
 // One queued request as the planner sees it.
 struct PlanItem {
-    uint64_t rows;       // rows of the request's table
-    uint64_t sig;        // batch signature: requests of one batch share it
-    uint64_t out_bytes;  // bytes of the request's result block
+    uint64_t rows;          // rows of the request's table
+    uint64_t sig;           // batch signature: requests of one batch share it
+    uint64_t out_bytes;     // bytes of the request's result block
+    uint64_t alone_scratch; // scratch of a batch that holds this request alone, from its
+                            // own key type, value types and carried columns
 };
 
 struct PlanLimits {
@@ -130,17 +134,20 @@ struct Plan {
 // Picks the next batch: the oldest queued request and, in queue order, later
 // requests with its signature while the batch stays within the limits; each
 // picked request gets its result block, then the batch gets one scratch block
-// of scratch_bytes(rows of the picked requests).  Memory is taken in queue
-// order and the scan stops at the first request whose block does not fit, so
-// a large old request is never passed by newer ones for memory; when the
-// scratch block does not fit, the newest picks are dropped until it does.
+// of scratch_bytes(rows of the picked requests), a function of the head's
+// signature (every pick shares it).  Memory is taken in queue order and the
+// scan stops at the first request whose block does not fit, so a large old
+// request is never passed by newer ones for memory; when the scratch block
+// does not fit, the newest picks are dropped until it does.
 //
 // Requests of another signature, and requests over the row limit, are
 // skipped (they wait for a later batch), but memory is kept for each of them:
-// its result block plus single-request scratch is held while the batch is
-// planned.  Without this, newer results held by a caller that consumes
-// results in submission order could fill the arena while the older request
-// waits for memory that only the caller can free.
+// its result block plus its own single-request scratch (alone_scratch, sized
+// by its own signature: a 16-byte-key request needs more per row than an
+// 8-byte-key head) is held while the batch is planned.  Without this, newer
+// results held by a caller that consumes results in submission order could
+// fill the arena while the older request waits for memory that only the
+// caller can free.
 //
 // An empty plan means nothing can start until memory is released (or, with
 // head_never_fits, that the oldest request can never run in this arena).
@@ -151,38 +158,60 @@ Plan plan_batch(const std::vector<PlanItem>& queue, const PlanLimits& lim, Range
     Plan p;
     if (queue.empty()) return p;
     const PlanItem& head = queue[0];
-    {
-        std::vector<uint64_t> one{head.rows};
-        if (arena.round(head.out_bytes) + arena.round(scratch_bytes(one)) > arena.size()) {
-            p.head_never_fits = true;
-            return p;
-        }
+    if (arena.round(head.out_bytes) + arena.round(head.alone_scratch) > arena.size()) {
+        p.head_never_fits = true;
+        return p;
     }
     // Every older request the batch passes (another signature, or over the
-    // row limit) gets its own need (result block + scratch alone) held in the
-    // arena while the batch is planned, so the batch only takes memory beside
-    // those needs.  Younger requests' results then never take the room an
-    // older request needs, whatever the caller holds: a caller that consumes
-    // results in submission order cannot wait on a request that no longer fits.
+    // row limit, and older than a pick) gets its own need (result block +
+    // scratch alone) held in the arena while the batch is planned, placed
+    // just before the first younger pick's result block, so the batch only
+    // takes memory beside those needs.  Younger requests' results then never
+    // take the room an older request needs, whatever the caller holds: a
+    // caller that consumes results in submission order cannot wait on a
+    // request that no longer fits.  Requests skipped after the last pick are
+    // not passed by anything and get no hold (holds for them would only take
+    // room from the batch, down to the head's own scratch).
     uint64_t rows = 0;
     std::vector<uint64_t> ns;
-    std::vector<uint64_t> held;
+    std::vector<uint64_t> held;        // arena offsets of the holds, in placement order
+    std::vector<size_t> holds_before;  // per pick: holds placed before it
+    std::vector<size_t> pending;       // skipped since the last pick
+    auto release_holds_from = [&](size_t k) {
+        for (size_t j = k; j < held.size(); ++j) arena.release(held[j]);
+        held.resize(k);
+    };
     for (size_t i = 0; i < queue.size() && p.picked.size() < lim.max_tables; ++i) {
         const PlanItem& q = queue[i];
         if (q.sig != head.sig || (i > 0 && rows + q.rows > lim.max_rows)) {
-            std::vector<uint64_t> one{q.rows};
-            const uint64_t off = arena.alloc(arena.round(q.out_bytes) + arena.round(scratch_bytes(one)), true);
-            if (off == RangeAllocator::kNone) break;  // no room for it: take nothing younger
-            held.push_back(off);
+            pending.push_back(i);
             continue;
         }
-        const uint64_t off = arena.alloc(q.out_bytes);
-        if (off == RangeAllocator::kNone) break;
+        const size_t mark = held.size();
+        bool room = true;
+        for (size_t j : pending) {
+            const PlanItem& o = queue[j];
+            const uint64_t off = arena.alloc(arena.round(o.out_bytes) + arena.round(o.alone_scratch), true);
+            if (off == RangeAllocator::kNone) {
+                room = false;  // no room for an older request: take nothing younger
+                break;
+            }
+            held.push_back(off);
+        }
+        const uint64_t off = room ? arena.alloc(q.out_bytes) : RangeAllocator::kNone;
+        if (off == RangeAllocator::kNone) {
+            release_holds_from(mark);
+            break;
+        }
+        pending.clear();
         p.picked.push_back(i);
         p.out_off.push_back(off);
+        holds_before.push_back(mark);
         ns.push_back(q.rows);
         rows += q.rows;
     }
+    // the batch's scratch, beside the holds of the requests it passes; a pick
+    // that is dropped takes the holds placed for it along
     while (!p.picked.empty()) {
         const uint64_t sb = scratch_bytes(ns);
         const uint64_t off = arena.alloc(sb);
@@ -192,12 +221,66 @@ Plan plan_batch(const std::vector<PlanItem>& queue, const PlanLimits& lim, Range
             break;
         }
         arena.release(p.out_off.back());
+        release_holds_from(holds_before.back());
         p.out_off.pop_back();
         p.picked.pop_back();
+        holds_before.pop_back();
         ns.pop_back();
     }
-    for (uint64_t off : held) arena.release(off);
+    release_holds_from(0);
     return p;
+}
+
+// Shape of a registered table as the server's accounting sees it.
+struct TableShape {
+    int key_type = 0;
+    uint32_t key_bytes = 0;  // bytes of one key cell
+    uint64_t n = 0;
+    int nvals = 0;
+    int val_types[VDF_MAX_VALUE_COLS] = {};
+    uint32_t val_bytes[VDF_MAX_VALUE_COLS] = {};
+};
+
+// Lays out a request's result block at `base` (base null: only measures it):
+// group keys, offsets, counts, the permutation when asked, and the per-column
+// aggregates of agg_mask, each array 256-byte aligned, for `cap` groups.
+// Returns the block's bytes.
+inline uint64_t result_layout(const TableShape& t, uint32_t agg_mask, bool want_perm, uint64_t cap, char* base,
+                              vdf_groupby_out* o)
+{
+    const uint64_t c = std::max<uint64_t>(cap, 1);
+    uint64_t off = 0;
+    auto take = [&](uint64_t bytes) -> void* {
+        off = (off + 255) / 256 * 256;
+        void* p = base ? base + off : nullptr;
+        off += bytes;
+        return p;
+    };
+    vdf_groupby_out r{};
+    r.group_keys = take(c * t.key_bytes);
+    r.offsets = static_cast<uint32_t*>(take((cap + 1) * 4));
+    if (agg_mask & VDF_AGG_COUNT) r.counts = static_cast<uint32_t*>(take(c * 4));
+    if (want_perm) r.perm = static_cast<uint32_t*>(take(std::max<uint64_t>(t.n, 1) * 4));
+    for (int i = 0; i < t.nvals; ++i) {
+        const uint64_t vb = t.val_bytes[i];
+        if (agg_mask & VDF_AGG_SUM) r.aggs[i].sum = take(c * 8);
+        if (agg_mask & VDF_AGG_MEAN) r.aggs[i].mean = static_cast<double*>(take(c * 8));
+        if (agg_mask & VDF_AGG_MIN) r.aggs[i].min = take(c * vb);
+        if (agg_mask & VDF_AGG_MAX) r.aggs[i].max = take(c * vb);
+    }
+    r.capacity = cap;
+    if (o) *o = r;
+    return (off + 255) / 256 * 256;
+}
+
+// Batch signature: requests of one batch share key type, value types and
+// aggregate mask.
+inline uint64_t signature(const TableShape& t, uint32_t agg_mask)
+{
+    uint64_t s = static_cast<uint64_t>(t.key_type) | (static_cast<uint64_t>(t.nvals) << 4) |
+                 (static_cast<uint64_t>(agg_mask) << 8);
+    for (int i = 0; i < t.nvals; ++i) s |= static_cast<uint64_t>(t.val_types[i] & 0xF) << (16 + 4 * i);
+    return s;
 }
 
 }  // namespace vdf

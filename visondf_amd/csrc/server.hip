@@ -27,12 +27,8 @@ uint64_t now_ns()
                                      .count());
 }
 
-struct Table {
-    int key_type;
+struct Table : TableShape {
     const void* keys;
-    uint64_t n;
-    int nvals;
-    int val_types[VDF_MAX_VALUE_COLS];
     const void* vals[VDF_MAX_VALUE_COLS];
     uint64_t max_groups;
 };
@@ -45,6 +41,7 @@ struct Request {
     uint64_t cap = 0;        // groups the result block holds
     uint64_t sig = 0;        // batch signature (key type, value types, aggregates)
     uint64_t out_bytes = 0;  // result block bytes
+    uint64_t alone_scratch = 0;  // scratch of a batch of this request alone
     uint64_t out_off = RangeAllocator::kNone;
     uint64_t submit_ns = 0, start_ns = 0, done_ns = 0;
     uint64_t n_groups = 0;
@@ -56,42 +53,8 @@ struct Request {
     vdf_groupby_out out{};
 };
 
-// Lays out a request's result block at `base` (base null: only measures it).
-uint64_t result_layout(const Table& t, uint32_t agg_mask, bool want_perm, uint64_t cap, char* base,
-                       vdf_groupby_out* o)
-{
-    const uint64_t c = std::max<uint64_t>(cap, 1);
-    uint64_t off = 0;
-    auto take = [&](uint64_t bytes) -> void* {
-        off = align_up(off, 256);
-        void* p = base ? base + off : nullptr;
-        off += bytes;
-        return p;
-    };
-    vdf_groupby_out r{};
-    r.group_keys = take(c * key_bytes(t.key_type));
-    r.offsets = static_cast<uint32_t*>(take((cap + 1) * 4));
-    if (agg_mask & VDF_AGG_COUNT) r.counts = static_cast<uint32_t*>(take(c * 4));
-    if (want_perm) r.perm = static_cast<uint32_t*>(take(std::max<uint64_t>(t.n, 1) * 4));
-    for (int i = 0; i < t.nvals; ++i) {
-        const uint64_t vb = static_cast<uint64_t>(val_bytes(t.val_types[i]));
-        if (agg_mask & VDF_AGG_SUM) r.aggs[i].sum = take(c * 8);
-        if (agg_mask & VDF_AGG_MEAN) r.aggs[i].mean = static_cast<double*>(take(c * 8));
-        if (agg_mask & VDF_AGG_MIN) r.aggs[i].min = take(c * vb);
-        if (agg_mask & VDF_AGG_MAX) r.aggs[i].max = take(c * vb);
-    }
-    r.capacity = cap;
-    if (o) *o = r;
-    return align_up(off, 256);
-}
-
-uint64_t signature(const Table& t, uint32_t agg_mask)
-{
-    uint64_t s = static_cast<uint64_t>(t.key_type) | (static_cast<uint64_t>(t.nvals) << 4) |
-                 (static_cast<uint64_t>(agg_mask) << 8);
-    for (int i = 0; i < t.nvals; ++i) s |= static_cast<uint64_t>(t.val_types[i] & 0xF) << (16 + 4 * i);
-    return s;
-}
+// bytes per row the default batch row limit assumes for one batch's scratch
+constexpr uint64_t kDefaultRowBytes = 96;
 
 struct Worker {
     std::unique_ptr<Handle> h;
@@ -121,6 +84,15 @@ struct Server {
     int inflight = 0;
     bool stop = false;
     vdf_server_stats st{};
+    Tuning tune;  // the workers' launch shape (all workers share it)
+
+    // scratch of a batch that holds `t` alone, as the workers size it
+    uint64_t alone_scratch(const Table& t) const
+    {
+        return static_cast<uint64_t>(groupby_batch_scratch(t.key_type, &t.n, 1, static_cast<uint64_t>(tune.hist_rows),
+                                                           groupby_carry_bytes(tune, t.key_type, t.val_types,
+                                                                               t.nvals)));
+    }
 
     void finish(Request* r, vdf_status s, const std::string& m, uint64_t t)
     {
@@ -169,7 +141,7 @@ void Server::run(int w)
                     items.reserve(m);
                     for (size_t i = 0; i < m; ++i) {
                         const Request* r = queue[i];
-                        items.push_back(PlanItem{tables[r->table].n, r->sig, r->out_bytes});
+                        items.push_back(PlanItem{tables[r->table].n, r->sig, r->out_bytes, r->alone_scratch});
                     }
                     plan = plan_batch(items, lim, arena, [&](const std::vector<uint64_t>& ns) {
                         return static_cast<uint64_t>(
@@ -329,7 +301,13 @@ vdf_status vdf_server_create(vdf_server** out, const vdf_server_config* cfg)
         auto S = std::make_unique<vdf_server>();
         S->device = cfg->device;
         S->lim.max_tables = static_cast<size_t>(cfg->max_batch_tables);
-        S->lim.max_rows = cfg->max_batch_rows ? cfg->max_batch_rows : ~0ull;
+        // default row limit: a batch's scratch (up to ~90 bytes per row for
+        // 16-byte keys with a carried f64 column) stays within half of the
+        // arena's share per worker, so several batches and held results fit
+        S->lim.max_rows = cfg->max_batch_rows
+                              ? cfg->max_batch_rows
+                              : std::max<uint64_t>(cfg->arena_bytes / (kDefaultRowBytes * 2 * cfg->workers), 1ull << 20);
+        S->tune.sort_xcd = 0;  // plain launches: passes of several workers run together
         try {
             void* p = nullptr;
             if (hipMalloc(&p, cfg->arena_bytes) != hipSuccess) {
@@ -347,7 +325,7 @@ vdf_status vdf_server_create(vdf_server** out, const vdf_server_config* cfg)
             for (Worker& W : S->workers) {
                 W.h = std::make_unique<Handle>();
                 W.h->device = cfg->device;
-                W.h->tune.sort_xcd = 0;  // plain launches: passes of several workers run together
+                W.h->tune = S->tune;
                 W.h->rowfree_kernels = false;  // the kernels the served workload was validated with
                 VDF_HIP_CHECK(hipStreamCreateWithFlags(&W.stream, hipStreamNonBlocking));
                 VDF_HIP_CHECK(hipMalloc(&W.d_ng, static_cast<size_t>(nt) * 8));
@@ -403,6 +381,7 @@ vdf_status vdf_server_add_table(vdf_server* s, int key_type, const void* d_keys,
         require(nvals == 0 || (val_types != nullptr && d_vals != nullptr), "vdf_server_add_table: null value list");
         Table t{};
         t.key_type = key_type;
+        t.key_bytes = static_cast<uint32_t>(key_bytes(key_type));
         t.keys = d_keys;
         t.n = n;
         t.nvals = nvals;
@@ -410,6 +389,7 @@ vdf_status vdf_server_add_table(vdf_server* s, int key_type, const void* d_keys,
             require(val_bytes(val_types[c]) != 0, "vdf_server_add_table: unknown value type");
             require(n == 0 || d_vals[c] != nullptr, "vdf_server_add_table: null value column");
             t.val_types[c] = val_types[c];
+            t.val_bytes[c] = static_cast<uint32_t>(val_bytes(val_types[c]));
             t.vals[c] = d_vals[c];
         }
         t.max_groups = max_groups;
@@ -440,6 +420,7 @@ vdf_status vdf_server_submit(vdf_server* s, uint32_t table_id, uint32_t agg_mask
             r->cap = (t.max_groups == 0 || t.max_groups > t.n) ? t.n : t.max_groups;
             r->sig = signature(t, agg_mask);
             r->out_bytes = result_layout(t, agg_mask, r->want_perm, r->cap, nullptr, nullptr);
+            r->alone_scratch = s->alone_scratch(t);
             *request_id = r->id;
             s->queue.push_back(r.get());
             s->reqs.emplace(r->id, std::move(r));

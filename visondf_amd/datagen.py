@@ -175,6 +175,19 @@ def served_table_sizes(ntables: int = 64, lo: int = 10_000, hi: int = 50_000_000
     return np.floor(np.exp(np.log(lo) + u * (np.log(hi) - np.log(lo)))).astype(np.int64)
 
 
+def served_trace(ntables: int = 32, requests: int = 1000, rate: float = 2000.0, seed: int = 6):
+    """The served workload's request trace: table row counts (log-uniform in
+    [10K, 50M]), key kinds (uint64 for even tables, 16-byte ASCII for odd
+    ones), Poisson arrival times (seconds, `rate` per second) and the table
+    each request groups."""
+    sizes = served_table_sizes(ntables, 10_000, 50_000_000, seed=5)
+    kinds = ["u64" if i % 2 == 0 else "char16" for i in range(ntables)]
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, requests))
+    picks = rng.integers(0, ntables, requests)
+    return sizes, kinds, arrivals, picks
+
+
 def served_table(rows: int, table_id: int, key_kind: str, seed: int = 5):
     """One served table: `rows` keys drawn from rows / 16 distinct values
     (uint64 spread over all 64 bits, or 16-byte printable ASCII) and a f64
