@@ -141,6 +141,25 @@ vdf_status vdf_sort_by(vdf_handle* h, int key_type, const void* d_keys, uint64_t
                        void* d_perm_out, void* d_sorted_keys_out, uint32_t val_bytes, const void* d_vals,
                        void* d_vals_out, void* stream);
 
+/* ---- vdf_sort_by without allocating ----
+ * No counterpart in the reference.  For callers that cannot allocate: code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc.  Same arguments and results as vdf_sort_by, but the call makes
+ * no device or pinned-host allocation: its scratch is one fixed device array
+ * of VDF_NOALLOC_SCRATCH_BYTES per device (held from the library's load until
+ * the process ends; its address is taken in vdf_create), and the sort's
+ * descriptors are written by a kernel instead of an upload.  A sort whose
+ * scratch exceeds the array fails with VDF_ERR_INVALID before any work: with
+ * an 8-byte column that is above about 2.0M rows for 4-byte keys, 1.6M for
+ * 8-byte keys and 1.1M for 16-byte keys.  Every call on one device shares
+ * the array: calls that overlap on one device (other streams, other handles
+ * or threads) race and must be serialised by the caller.  Profiling is off
+ * for these calls. */
+#define VDF_NOALLOC_SCRATCH_BYTES (64ull << 20)
+vdf_status vdf_sort_by_noalloc(vdf_handle* h, int key_type, const void* d_keys, uint64_t n, uint32_t flags,
+                               void* d_perm_out, void* d_sorted_keys_out, uint32_t val_bytes, const void* d_vals,
+                               void* d_vals_out, void* stream);
+
 /* ---- sort-based group-by ---- */
 typedef struct vdf_value_col {
     int type;          /* VDF_VAL_* */

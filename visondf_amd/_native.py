@@ -38,7 +38,7 @@ MAX_KEY_COLS = 4
 EXPORTS = (
     "vdf_last_error", "vdf_version", "vdf_create", "vdf_destroy",
     "vdf_argsort_scratch_bytes", "vdf_groupby_scratch_bytes", "vdf_groupby_batch_scratch_bytes",
-    "vdf_argsort", "vdf_sort_by", "vdf_gather", "vdf_groupby",
+    "vdf_argsort", "vdf_sort_by", "vdf_sort_by_noalloc", "vdf_gather", "vdf_groupby",
     "vdf_profile_enable", "vdf_profile_reset", "vdf_profile_read",
     "vdf_tune_set", "vdf_tune_get",
     "vdf_argsort_batch", "vdf_groupby_batch",
@@ -141,6 +141,8 @@ def load() -> C.CDLL:
     lib.vdf_argsort.argtypes = [vp, i32, vp, u64, u32, vp, vp, vp]
     lib.vdf_sort_by.restype = i32
     lib.vdf_sort_by.argtypes = [vp, i32, vp, u64, u32, vp, vp, u32, vp, vp, vp]
+    lib.vdf_sort_by_noalloc.restype = i32
+    lib.vdf_sort_by_noalloc.argtypes = [vp, i32, vp, u64, u32, vp, vp, u32, vp, vp, vp]
     lib.vdf_gather.restype = i32
     lib.vdf_gather.argtypes = [vp, u32, vp, vp, i32, u64, vp, vp]
     lib.vdf_groupby.restype = i32

@@ -6,6 +6,7 @@
 #include "vdf_internal.h"
 
 #include <algorithm>
+#include <cstdio>
 #include <cstring>
 #include <exception>
 #include <memory>
@@ -13,6 +14,48 @@
 namespace vdf {
 
 static thread_local std::string g_last_error;
+
+// Scratch of the allocation-free sort (vdf_sort_by_noalloc): one fixed array
+// per device, held from the library's load until the process ends.  Its name
+// on the host is only a stand-in for the symbol; each device's address comes
+// from hipGetSymbolAddress (taken in vdf_create with that device current) and
+// only that address is handed to kernels.
+static __device__ __align__(256) char scratch[VDF_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices] = {};
+static std::string g_scratch_err[kMaxDevices];
+
+// Looks up the device address of `scratch` on `dev`, which must be current,
+// once per device.  A failed lookup is kept as that device's error, reported
+// by the allocation-free entry point (other entry points do not need it).
+static void noalloc_scratch_setup(int dev)
+{
+    if (dev < 0 || dev >= kMaxDevices) return;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[dev]) return;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr) {
+        (void)hipGetLastError();
+        g_scratch_err[dev] = std::string("hipGetSymbolAddress of the fixed scratch failed: ") + hipGetErrorString(e);
+        return;
+    }
+    g_scratch_ptr[dev] = p;
+    g_scratch_err[dev].clear();
+}
+
+// The fixed scratch's address on `dev` (set up by vdf_create on that device).
+static void* noalloc_scratch(int dev)
+{
+    require(dev >= 0 && dev < kMaxDevices, "device index beyond the library's device table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev])
+        throw Error{VDF_ERR_HIP, g_scratch_err[dev].empty() ? "the fixed scratch of this device was not set up"
+                                                            : g_scratch_err[dev]};
+    return g_scratch_ptr[dev];
+}
 
 void set_error(const std::string& msg) { g_last_error = msg; }
 
@@ -216,6 +259,14 @@ vdf_status vdf_create(vdf_handle** out, int device, uint64_t hbm_budget_bytes)
         int ndev = 0;
         VDF_HIP_CHECK(hipGetDeviceCount(&ndev));
         require(device >= 0 && device < ndev, "vdf_create: no such device");
+        {
+            // the fixed scratch's address on this device, with the device current
+            int prev = 0;
+            VDF_HIP_CHECK(hipGetDevice(&prev));
+            if (prev != device) VDF_HIP_CHECK(hipSetDevice(device));
+            noalloc_scratch_setup(device);
+            if (prev != device) VDF_HIP_CHECK(hipSetDevice(prev));
+        }
         auto* h = new vdf_handle();
         h->device = device;
         h->budget = hbm_budget_bytes;
@@ -306,6 +357,56 @@ vdf_status vdf_sort_by(vdf_handle* h, int key_type, const void* d_keys, uint64_t
         void* scr = h->ensure_scratch(argsort_batch_scratch(key_type, &n, 1, hr, pb));
         h->staging_begin(argsort_batch_staging(key_type, &n, 1, hr));
         StagingScope ss(h, s);
+        SortSeg seg{d_keys, n, d_perm_out, d_sorted_keys_out};
+        seg.pay_in = d_vals;
+        seg.pay_out = d_vals_out;
+        argsort_batch_enqueue(h, key_type, &seg, 1, flags, scr, s, pb);
+    });
+}
+
+vdf_status vdf_sort_by_noalloc(vdf_handle* h, int key_type, const void* d_keys, uint64_t n, uint32_t flags,
+                               void* d_perm_out, void* d_sorted_keys_out, uint32_t val_bytes, const void* d_vals,
+                               void* d_vals_out, void* stream)
+{
+    return guarded([&] {
+        require(h != nullptr, "vdf_sort_by_noalloc: null handle");
+        require(key_bytes(key_type) != 0, "vdf_sort_by_noalloc: unknown key type");
+        require((flags & ~(VDF_SORT_DESC | VDF_SORT_DESC_REVERSE | VDF_SORT_PERM64)) == 0,
+                "vdf_sort_by_noalloc: unknown flag bits");
+        require(!((flags & VDF_SORT_DESC) && (flags & VDF_SORT_DESC_REVERSE)),
+                "vdf_sort_by_noalloc: VDF_SORT_DESC and VDF_SORT_DESC_REVERSE are exclusive");
+        require(val_bytes == 4 || val_bytes == 8, "vdf_sort_by_noalloc: val_bytes must be 4 or 8");
+        require(n <= VDF_MAX_ROWS, "vdf_sort_by_noalloc: n exceeds VDF_MAX_ROWS (2^30 - 1)");
+        if (n == 0) return;
+        require(d_keys != nullptr && d_vals != nullptr && d_vals_out != nullptr,
+                "vdf_sort_by_noalloc: null device pointer");
+        const int pb = static_cast<int>(val_bytes);
+        const uint64_t hr = static_cast<uint64_t>(h->tune.hist_rows);
+        const size_t need = argsort_batch_scratch(key_type, &n, 1, hr, pb);
+        if (need > VDF_NOALLOC_SCRATCH_BYTES) {
+            char msg[192];
+            std::snprintf(msg, sizeof msg,
+                          "vdf_sort_by_noalloc: %llu rows need %llu scratch bytes, above the fixed %llu "
+                          "(VDF_NOALLOC_SCRATCH_BYTES); use vdf_sort_by",
+                          static_cast<unsigned long long>(n), static_cast<unsigned long long>(need),
+                          static_cast<unsigned long long>(VDF_NOALLOC_SCRATCH_BYTES));
+            throw Error{VDF_ERR_INVALID, msg};
+        }
+        CallScope cs(h);
+        void* scr = noalloc_scratch(h->device);
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        // no pinned staging (descriptors written on the device), no profiling events
+        struct Restore {
+            Handle* h;
+            bool prof;
+            ~Restore()
+            {
+                h->device_desc = false;
+                h->prof = prof;
+            }
+        } restore{h, h->prof};
+        h->device_desc = true;
+        h->prof = false;
         SortSeg seg{d_keys, n, d_perm_out, d_sorted_keys_out};
         seg.pay_in = d_vals;
         seg.pay_out = d_vals_out;

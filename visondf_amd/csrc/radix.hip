@@ -250,6 +250,19 @@ __device__ inline void hist_add(uint32_t* sh, const K& k, bool valid, uint32_t s
 // One work item per block: rows [begin, end) of one segment, one read of the
 // keys, LDS histogram of every digit, flushed with one atomic per non-empty
 // bucket into the segment's histogram.  Also clears look-back buffer 0.
+// Descriptors of a one-segment sort written on the device (allocation-free
+// calls, which have no pinned host staging): the segment's SegDesc and its
+// histogram work items of `hist_rows` rows each.
+__global__ void k_setup_one(SegDesc seg, SegDesc* segs, HistItem* items, uint64_t nitems, uint64_t hist_rows)
+{
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i == 0) segs[0] = seg;
+    if (i < nitems) {
+        const uint64_t b = i * hist_rows;
+        items[i] = HistItem{0u, 0u, b, b + hist_rows < seg.n ? b + hist_rows : seg.n};
+    }
+}
+
 template <typename K>
 __global__ __launch_bounds__(kBlock) void k_histogram(SortArgs a)
 {
@@ -1221,28 +1234,48 @@ static void launch_sort(Handle* h, const SortSeg* segs, int nsegs, uint64_t mlo,
     require(pass != nullptr, "sort payload must be 0, 4, 8 or 16 bytes per row");
     const uint64_t tile_rows = static_cast<uint64_t>(kBlock) * items;
     const uint64_t hist_rows = static_cast<uint64_t>(h->tune.hist_rows);
-    std::vector<uint64_t> ns(nsegs);
-    for (int i = 0; i < nsegs; ++i) ns[i] = segs[i].n;
+    // one segment: its row count in place (no host allocation on that path)
+    uint64_t n_one = nsegs == 1 ? segs[0].n : 0;
+    std::vector<uint64_t> ns_many;
+    if (nsegs != 1) {
+        ns_many.resize(nsegs);
+        for (int i = 0; i < nsegs; ++i) ns_many[i] = segs[i].n;
+    }
+    const uint64_t* ns = nsegs == 1 ? &n_one : ns_many.data();
     const SortLayout L = sort_layout(sizeof(K) == 4 ? VDF_KEY_U32 : (sizeof(K) == 8 ? VDF_KEY_U64 : VDF_KEY_CHAR16),
-                                     ns.data(), nsegs, hist_rows, pay_bytes);
+                                     ns, nsegs, hist_rows, pay_bytes);
 
-    // ---- host-built batch description, uploaded through pinned staging ----
-    char* stg = static_cast<char*>(h->staging_take(staging_bytes(L, nsegs)));
-    Carve hc(stg);
-    SegDesc* hs = hc.take<SegDesc>(nsegs);
-    HistItem* hi = hc.take<HistItem>(std::max<uint64_t>(L.items, 1));
-    uint32_t* ht = hc.take<uint32_t>(std::max<uint64_t>(L.tiles, 1));
+    // ---- batch description: host-built and uploaded through pinned staging,
+    // or (device_desc, one segment) written on the device by k_setup_one ----
+    const bool devdesc = h->device_desc;
+    require(!devdesc || nsegs == 1, "device-written descriptors take one segment");
+    char* stg = nullptr;
+    uint32_t* ht = nullptr;
     uint64_t row = 0, tiles = 0, nitems = 0;
-    for (int i = 0; i < nsegs; ++i) {
-        const uint64_t n = segs[i].n;
-        const uint64_t nt = (n + tile_rows - 1) / tile_rows;
-        hs[i] = SegDesc{segs[i].keys,    segs[i].perm_out, segs[i].keys_out, segs[i].pay_in, segs[i].pay_out,
-                        segs[i].pay_in2, segs[i].pay_out2, n,             row,            static_cast<uint32_t>(tiles),
-                        0};
-        for (uint64_t t = 0; t < nt; ++t) ht[tiles + t] = static_cast<uint32_t>(i);
-        for (uint64_t b = 0; b < n; b += hist_rows) hi[nitems++] = HistItem{static_cast<uint32_t>(i), 0, b, std::min(n, b + hist_rows)};
-        row += n;
-        tiles += nt;
+    SegDesc one{};
+    if (devdesc) {
+        const uint64_t n = segs[0].n;
+        tiles = (n + tile_rows - 1) / tile_rows;
+        nitems = (n + hist_rows - 1) / hist_rows;
+        one = SegDesc{segs[0].keys, segs[0].perm_out, segs[0].keys_out, segs[0].pay_in, segs[0].pay_out,
+                      segs[0].pay_in2, segs[0].pay_out2, n, 0, 0, 0};
+    } else {
+        stg = static_cast<char*>(h->staging_take(staging_bytes(L, nsegs)));
+        Carve hc(stg);
+        SegDesc* hs = hc.take<SegDesc>(nsegs);
+        HistItem* hi = hc.take<HistItem>(std::max<uint64_t>(L.items, 1));
+        ht = hc.take<uint32_t>(std::max<uint64_t>(L.tiles, 1));
+        for (int i = 0; i < nsegs; ++i) {
+            const uint64_t n = segs[i].n;
+            const uint64_t nt = (n + tile_rows - 1) / tile_rows;
+            hs[i] = SegDesc{segs[i].keys,    segs[i].perm_out, segs[i].keys_out, segs[i].pay_in, segs[i].pay_out,
+                            segs[i].pay_in2, segs[i].pay_out2, n,             row,            static_cast<uint32_t>(tiles),
+                            0};
+            for (uint64_t t = 0; t < nt; ++t) ht[tiles + t] = static_cast<uint32_t>(i);
+            for (uint64_t b = 0; b < n; b += hist_rows) hi[nitems++] = HistItem{static_cast<uint32_t>(i), 0, b, std::min(n, b + hist_rows)};
+            row += n;
+            tiles += nt;
+        }
     }
     const uint64_t ntiles = tiles;
     if (ntiles == 0 || h->dry) return;  // every segment empty, or only the staging was wanted
@@ -1274,8 +1307,14 @@ static void launch_sort(Handle* h, const SortSeg* segs, int nsegs, uint64_t mlo,
     // a 16-byte payload is two 8-byte columns: the second in the upper half
     a.pbuf2[0] = static_cast<char*>(a.pbuf[0]) + (pay_bytes >= 12 ? L.rows * 8 : 0);
     a.pbuf2[1] = static_cast<char*>(a.pbuf[1]) + (pay_bytes >= 12 ? L.rows * 8 : 0);
-    const size_t up = reinterpret_cast<char*>(ht + ntiles) - stg;
-    VDF_HIP_CHECK(hipMemcpyAsync(dsegs, stg, up, hipMemcpyHostToDevice, s));  // descs, items, tile map
+    if (devdesc) {
+        const unsigned blocks = static_cast<unsigned>((nitems + kBlock - 1) / kBlock);
+        hipLaunchKernelGGL(k_setup_one, dim3(blocks), dim3(kBlock), 0, s, one, dsegs, ditems, nitems, hist_rows);
+        check_launch("k_setup_one");
+    } else {
+        const size_t up = reinterpret_cast<char*>(ht + ntiles) - stg;
+        VDF_HIP_CHECK(hipMemcpyAsync(dsegs, stg, up, hipMemcpyHostToDevice, s));  // descs, items, tile map
+    }
     // the host carve mirrors the device carve (same order, same alignment)
     a.segs = dsegs;
     a.items = ditems;
@@ -1339,7 +1378,7 @@ static void launch_sort(Handle* h, const SortSeg* segs, int nsegs, uint64_t mlo,
                                        sizeof(uint32_t), static_cast<size_t>(nsegs), hipMemcpyDeviceToHost, s));
         acct = std::make_shared<PassAcct>();
         acct->passes = dst;
-        acct->rows = ns;
+        acct->rows.assign(ns, ns + nsegs);
         // keys read + keys written + row ids read + row ids written, per row, when
         // the pass is the table's only / first / middle / last one
         // (plus the payload read and written in every pass)

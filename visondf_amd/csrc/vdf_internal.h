@@ -179,6 +179,9 @@ struct Handle {
     // row-id-free pass kernels for sorts without row ids (off for server
     // workers, whose sorts run concurrently on several streams)
     bool rowfree_kernels = true;
+    // one-table sorts write their descriptors on the device instead of
+    // uploading them from pinned staging (set only for allocation-free calls)
+    bool device_desc = false;
 
     void* ensure_scratch(size_t bytes);
     void staging_begin(size_t bytes);      // start a call's staging region
