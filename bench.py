@@ -40,17 +40,62 @@ HBM_PEAK_GBPS = 8000.0
 METRIC = "group-by rows/sec + HBM GB/s (% peak) at 100M rows; served p50/p99 latency"
 KERNELS = ("radix_histogram", "radix_plan", "radix_pass", "gather", "fill", "groupby_empty", "groupby_tile",
            "groupby_fixup")
-WORKLOADS = ("argsort_u32", "groupby_i32", "zipf_groupby", "batched", "served")
-DEFAULT_WORKLOADS = ("argsort_u32", "groupby_i32", "zipf_groupby", "batched", "served")
-CHILD_TIMEOUT = {"argsort_u32": 600, "groupby_i32": 400, "zipf_groupby": 400, "batched": 500, "served": 500}
+WORKLOADS = ("argsort_u32", "groupby_c1", "groupby_i32", "zipf_groupby", "batched", "served")
+DEFAULT_WORKLOADS = ("argsort_u32", "groupby_c1", "groupby_i32", "zipf_groupby", "batched", "served")
+CHILD_TIMEOUT = {"argsort_u32": 600, "groupby_c1": 300, "groupby_i32": 400, "zipf_groupby": 400, "batched": 500, "served": 500}
 
 
 def _log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
+def _host() -> dict:
+    """The host CPUs this process may use: model, logical CPUs in its affinity
+    mask, distinct physical cores among them, and the cgroup CPU quota."""
+    allowed = sorted(os.sched_getaffinity(0))
+    cores = set()
+    for c in allowed:
+        base = f"/sys/devices/system/cpu/cpu{c}/topology/"
+        try:
+            with open(base + "physical_package_id") as f:
+                pkg = f.read().strip()
+            with open(base + "core_id") as f:
+                core = f.read().strip()
+            cores.add((pkg, core))
+        except OSError:
+            cores.add((str(c), ""))
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, per = f.read().split()
+            if q != "max":
+                quota = int(q) / int(per)
+    except (OSError, ValueError):
+        pass
+    return {"model": model, "logical_cpus": len(allowed), "physical_cores": len(cores), "cgroup_cpus": quota,
+            "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS")}
+
+
 def _threads(args) -> int:
-    return args.cpu_threads or min(16, os.cpu_count() or 1)
+    """CPU-baseline threads: min(physical cores available, 32), the
+    reference's cap (MAX_THREADS, details/constants.inl:9), and within the
+    cgroup's CPU quota."""
+    if args.cpu_threads:
+        return args.cpu_threads
+    h = _host()
+    t = min(h["physical_cores"] or 1, 32)
+    if h["cgroup_cpus"]:
+        t = min(t, max(int(h["cgroup_cpus"]), 1))
+    return max(t, 1)
 
 
 def _load_traffic(workload: str, name: str):
@@ -136,7 +181,11 @@ def cpu_baseline_argsort(keys, val, threads: int, reps: int = 3, port: bool = Fa
 
     n = keys.shape[0]
     ref = None if port else oracle.ref_lib()
-    use_ref = ref is not None and threads >= 16
+    use_ref = ref is not None and threads >= 8
+    # radix_sort_uint32_mt<CORES>: CORES is a compile-time constant; the
+    # largest instantiation not above the threads available
+    cores = 32 if threads >= 32 else (16 if threads >= 16 else 8)
+    ref_fn = None if not use_ref else getattr(ref, f"vref_radix_sort_u32_mt{cores}")
     times = []
     for r in range(reps + 1):
         t0 = time.perf_counter()
@@ -144,7 +193,7 @@ def cpu_baseline_argsort(keys, val, threads: int, reps: int = 3, port: bool = Fa
             # sort_uintegers copies the key column and starts from iota (sort_by_mt.hpp:21-24)
             perm = np.arange(n, dtype=np.uint64)
             k = np.ascontiguousarray(keys, dtype=np.uint32).copy()
-            ref.vref_radix_sort_u32_mt16(C.c_void_p(k.ctypes.data), C.c_void_p(perm.ctypes.data), n)
+            ref_fn(C.c_void_p(k.ctypes.data), C.c_void_p(perm.ctypes.data), n)
         else:
             perm = oracle.argsort(keys, 0, threads)
         oracle.gather(val, perm, threads)
@@ -156,11 +205,11 @@ def cpu_baseline_argsort(keys, val, threads: int, reps: int = 3, port: bool = Fa
     if use_ref:
         exp = oracle.argsort(keys, 0, threads)
         assert np.array_equal(perm, exp), "reference radix and oracle disagree"
-        return {"value": n / t, "unit": "rows/s", "cores": 16, "kind": "reference",
-                "sample": f"full workload ({n} rows): the reference's radix_sort_uint32_mt<16, Simd=false> "
+        return {"value": n / t, "unit": "rows/s", "cores": cores, "kind": "reference", "host": _host(),
+                "sample": f"full workload ({n} rows): the reference's radix_sort_uint32_mt<{cores}, Simd=false> "
                           f"(its .inl files compiled in place, oracle/_ref) + the f64 gather on {threads} threads, "
                           f"median of {reps} after 1 warm-up"}
-    return {"value": n / t, "unit": "rows/s", "cores": threads, "kind": "port",
+    return {"value": n / t, "unit": "rows/s", "cores": threads, "kind": "port", "host": _host(),
             "sample": f"full workload ({n} rows) argsort + f64 gather, median of {reps} after 1 warm-up; "
                       f"oracle radix_sort_uint32_mt restatement, 16-bit digits, {threads} OpenMP threads"}
 
@@ -246,20 +295,50 @@ def run_argsort_u32(args) -> dict:
 # ---------------------------------------------------------------------------
 # 200M-row Zipf group-by
 # ---------------------------------------------------------------------------
-def _cpu_groupby(tables, threads: int, what: str) -> dict:
-    """The oracle's sort-based group-by over a list of (keys, [values]) tables."""
+def _cpu_ref_groupby(tables, threads: int, what: str, refapi: bool = False, reps: int = 1) -> dict:
+    """The reference's hash group-by (oracle/_ref: its ankerl map and
+    simd_hash under loops restated from onecol_functions.inl /
+    merge_functions.inl) over a list of (keys, f64 column, second column or
+    None) tables, one call per table, on `threads` OpenMP threads: median of
+    `reps` timed runs after a warm-up on the first table's first rows."""
+    import numpy as np
+
     import oracle
 
-    rows = sum(k.shape[0] for k, _ in tables)
-    oracle.groupby(tables[0][0][:1000], [x[:1000] for x in tables[0][1]], threads)  # warm-up
-    t0 = time.perf_counter()
-    for k, vals in tables:
-        oracle.groupby(k, vals, threads)
-    dt = time.perf_counter() - t0
-    _log(f"cpu baseline ({what}): {rows} rows in {dt:.3f} s")
-    return {"value": rows / dt, "unit": "rows/s", "cores": threads, "kind": "port",
-            "sample": f"{what}; oracle restatement of the reference's radix sort + group-by loops, "
-                      f"{threads} OpenMP threads"}
+    rows = sum(k.shape[0] for k, _, _ in tables)
+    k, a, b = tables[0]
+    m = min(k.shape[0], 100_000)
+    oracle.ref_hash_groupby(k[:m], a[:m], None if b is None else b[:m], threads=threads, refapi=refapi,
+                            outputs=False)
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        for k, a, b in tables:
+            oracle.ref_hash_groupby(k, a, b, threads=threads, refapi=refapi, outputs=False)
+        times.append(time.perf_counter() - t0)
+    dt = float(np.median(times))
+    form = ("reference API: one hash pass per aggregate per column, Gather + functor for min and max"
+            if refapi else "fused: one hash pass keeping every aggregate")
+    _log(f"cpu baseline ({what}, {'api' if refapi else 'fused'}): {rows} rows in {dt:.3f} s on {threads} threads")
+    return {"value": rows / dt, "unit": "rows/s", "cores": threads, "kind": "reference", "host": _host(),
+            "sample": f"{what}; the reference's hash group-by ({form}): its ankerl::unordered_dense map and "
+                      f"simd_hash compiled from its headers in oracle/_ref, loops restated from "
+                      f"onecol_functions.inl and merge_functions.inl; {threads} OpenMP threads; "
+                      f"median of {reps} after a warm-up"}
+
+
+def _cpu_groupby_legs(rec: dict, rows_per_s: float, full, sample_api, threads: int, what_full: str,
+                      what_api: str, reps: int = 1):
+    """cpu_baseline = the fused hash group-by on `full`; beside it the
+    reference API form on `sample_api`, and (when more cores are available
+    than the reference's cap allows) the fused form on every physical core."""
+    rec["cpu_baseline"] = _cpu_ref_groupby(full, threads, what_full, refapi=False, reps=reps)
+    rec["speedup_vs_cpu"] = rows_per_s / rec["cpu_baseline"]["value"]
+    rec["cpu_reference_api"] = _cpu_ref_groupby(sample_api, threads, what_api, refapi=True)
+    rec["speedup_vs_cpu_reference_api"] = rows_per_s / rec["cpu_reference_api"]["value"]
+    allc = _host()["physical_cores"]
+    if allc and allc > threads and (_host()["cgroup_cpus"] or allc) >= allc:
+        rec["cpu_all_cores"] = _cpu_ref_groupby(full, allc, what_full + ", all physical cores", refapi=False)
 
 
 def run_groupby_i32(args) -> dict:
@@ -297,9 +376,54 @@ def run_groupby_i32(args) -> dict:
     rec["kernels"] = kernels
     if not args.no_cpu:
         m = min(n, 20_000_000)
-        rec["cpu_baseline"] = _cpu_groupby([(key[:m], [vf[:m], vi[:m]])], _threads(args),
-                                           f"first {m} rows of the workload")
-        rec["speedup_vs_cpu"] = rows_per_s / rec["cpu_baseline"]["value"]
+        _cpu_groupby_legs(rec, rows_per_s, [(key, vf, vi)], [(key[:m], vf[:m], vi[:m])], _threads(args),
+                          f"full workload ({n} rows)", f"first {m} rows of the workload", reps=3)
+    v.close()
+    return rec
+
+
+def run_groupby_c1(args) -> dict:
+    """The CPU reference configuration: 1M rows, int64 key uniform in
+    [0, 1000), float64 uniform [0, 1) and int32 uniform [-1e6, 1e6] values,
+    every aggregate; the reference's hash group-by timed on the host (fused
+    and API forms, the reference's thread count and one thread) beside the
+    GPU group-by of the same rows."""
+    import torch
+
+    from visondf_amd import _native as N
+    from visondf_amd import datagen
+    from visondf_amd.device import Vdf
+
+    n = args.rows or 1_000_000
+    key, vf, vi = datagen.groupby_small(n, seed=1)
+    d_key = torch.from_numpy(key).cuda()
+    d_vf = torch.from_numpy(vf).cuda()
+    d_vi = torch.from_numpy(vi).cuda()
+    v = Vdf(0)
+    res = {}
+
+    def step():
+        res["r"] = v.groupby(d_key, [d_vf, d_vi], aggs=N.AGG_ALL, want_perm=False, trim=False)
+
+    ms_total, kernels = _time_steps(step, args.steps, args.warmup, v)
+    ms_step = ms_total / args.steps
+    rows_per_s = n * args.steps / (ms_total * 1e-3)
+    g = int(res["r"].n_groups.item())
+    assert int(res["r"].counts[:g].to(torch.int64).sum().item()) == n, "group counts do not add up to the rows"
+    rec = _base_record(args, rows_per_s, "rows/s", ms_step, "f64+i32",
+                       {"workload": "groupby_c1_cpu_reference_config", "rows": n, "key": "int64 uniform [0, 1000)",
+                        "values": ["float64 uniform [0,1)", "int32 uniform [-1e6,1e6]"],
+                        "aggregates": "count,sum,mean,min,max", "groups": g, "parallelism": "single GPU"})
+    rec["kernels"] = kernels
+    if not args.no_cpu:
+        T = _threads(args)
+        _cpu_groupby_legs(rec, rows_per_s, [(key, vf, vi)], [(key, vf, vi)], T, f"full workload ({n} rows)",
+                          f"full workload ({n} rows)", reps=5)
+        rec["cpu_one_thread"] = _cpu_ref_groupby([(key, vf, vi)], 1, f"full workload ({n} rows), one thread",
+                                                 reps=3)
+        rec["cpu_reference_api_one_thread"] = _cpu_ref_groupby([(key, vf, vi)], 1,
+                                                               f"full workload ({n} rows), one thread",
+                                                               refapi=True, reps=3)
     v.close()
     return rec
 
@@ -336,10 +460,9 @@ def run_zipf_groupby(args) -> dict:
     rec["roofline"] = _roofline("zipf_groupby", kernels)
     rec["kernels"] = kernels
     if not args.no_cpu:
-        m = min(n, 20_000_000)
-        rec["cpu_baseline"] = _cpu_groupby([(key[:m], [vf[:m], vi[:m]])], _threads(args),
-                                           f"first {m} rows of the workload")
-        rec["speedup_vs_cpu"] = rows_per_s / rec["cpu_baseline"]["value"]
+        m = min(n, 5_000_000)
+        _cpu_groupby_legs(rec, rows_per_s, [(key, vf, vi)], [(key[:m], vf[:m], vi[:m])], _threads(args),
+                          f"full workload ({n} rows)", f"first {m} rows of the workload")
     v.close()
     return rec
 
@@ -431,9 +554,13 @@ def run_batched(args) -> dict:
     rec["roofline"] = _roofline("batched", kernels)
     rec["kernels"] = kernels
     if not args.no_cpu:
-        sample = [(keys[offs[i]:offs[i + 1]], [vals[offs[i]:offs[i + 1]]]) for i in range(min(ntab, 96))]
-        rec["cpu_baseline"] = _cpu_groupby(sample, _threads(args), f"the first {len(sample)} tables, one call each")
-        rec["speedup_vs_cpu"] = rows_per_s / rec["cpu_baseline"]["value"]
+        # bounded samples: every 16th table (fused), every 256th (reference API)
+        sample = [(keys[offs[i]:offs[i + 1]], vals[offs[i]:offs[i + 1]], None) for i in range(0, ntab, 16)]
+        api = [(keys[offs[i]:offs[i + 1]], vals[offs[i]:offs[i + 1]], None) for i in range(0, ntab, 256)]
+        _cpu_groupby_legs(rec, rows_per_s, sample, api, _threads(args),
+                          f"every 16th table ({len(sample)} tables, {sum(t[0].shape[0] for t in sample)} rows), "
+                          f"one call each",
+                          f"every 256th table ({len(api)} tables, {sum(t[0].shape[0] for t in api)} rows)")
     v.close()
     return rec
 
@@ -550,13 +677,16 @@ def run_served(args) -> dict:
                      "p99_speedup": single["p99_ms"] / batched["p99_ms"],
                      "p50_speedup": single["p50_ms"] / batched["p50_ms"]}
     if not args.no_cpu:
-        sample = [(t[4], [t[5]]) for t in tables if t[1] <= 2_000_000][:8]
-        rec["cpu_baseline"] = _cpu_groupby(sample, _threads(args),
-                                           f"{len(sample)} served tables of <= 2M rows, one call each")
+        # every served table once (the requests' tables), one call each
+        full = [(t[4], t[5], None) for t in tables]
+        api = [(t[4], t[5], None) for t in tables if t[1] <= 2_000_000]
+        _cpu_groupby_legs(rec, batched["rows_per_s"], full, api, _threads(args),
+                          f"each of the {len(full)} served tables once ({sum(t[1] for t in tables)} rows)",
+                          f"the {len(api)} served tables of <= 2M rows")
     return rec
 
 
-RUNNERS = {"argsort_u32": run_argsort_u32, "groupby_i32": run_groupby_i32, "zipf_groupby": run_zipf_groupby, "batched": run_batched,
+RUNNERS = {"argsort_u32": run_argsort_u32, "groupby_c1": run_groupby_c1, "groupby_i32": run_groupby_i32, "zipf_groupby": run_zipf_groupby, "batched": run_batched,
            "served": run_served}
 
 

@@ -60,11 +60,15 @@ def ref_lib():
         L = C.CDLL(REF_LIB_PATH)
         vp, sz = C.c_void_p, C.c_size_t
         for name in ("vref_radix_sort_u32", "vref_radix_sort_u32_mt8", "vref_radix_sort_u32_mt16",
-                     "vref_radix_sort_u64", "vref_radix_sort_u64_mt8"):
+                     "vref_radix_sort_u32_mt32", "vref_radix_sort_u64", "vref_radix_sort_u64_mt8"):
             getattr(L, name).restype = None
             getattr(L, name).argtypes = [vp, vp, sz]
         L.vref_radix_sort_charbuf.restype = None
         L.vref_radix_sort_charbuf.argtypes = [vp, sz, sz, vp]
+        L.vref_hash_groupby.restype = C.c_uint64
+        L.vref_hash_groupby.argtypes = [vp, sz, sz, vp, C.c_int, vp, C.c_int, C.c_int] + [vp] * 8
+        L.vref_simd_hash.restype = C.c_uint64
+        L.vref_simd_hash.argtypes = [vp, sz]
         _ref = L
     return _ref
 
@@ -157,6 +161,52 @@ def groupby(keys: np.ndarray, vals=(), threads: int = 1) -> dict:
         "aggs": [{"sum": sums[c][:g], "mean": means[c][:g], "min": mins[c][:g], "max": maxs[c][:g]}
                  for c in range(nv)],
     }
+
+
+def ref_hash_groupby(keys: np.ndarray, v0: np.ndarray, v1: np.ndarray | None = None, threads: int = 1,
+                     refapi: bool = False, outputs: bool = True) -> dict:
+    """The reference's hash group-by (its ankerl map and simd_hash, compiled
+    in oracle/_ref, under loops restated from onecol_functions.inl and
+    merge_functions.inl) over a key column, a float64 column and an optional
+    int32/int64 column.  refapi: one hash pass per aggregate, as the
+    reference's API runs them; else one fused pass.  Groups come in the
+    reference's iteration order.  outputs=False only times the work."""
+    L = ref_lib()
+    if L is None:
+        raise RuntimeError("oracle/_ref/libvref.so is not built")
+    keys = np.ascontiguousarray(keys)
+    v0 = np.ascontiguousarray(v0, dtype=np.float64)
+    n = keys.shape[0]
+    kb = keys.itemsize * (keys.shape[1] if keys.ndim == 2 else 1)
+    k1 = -1 if v1 is None else {np.dtype(np.int64): 1, np.dtype(np.int32): 2}[v1.dtype]
+    cap = max(n, 1) if outputs else 0
+    gk = np.empty((cap,) + keys.shape[1:], dtype=keys.dtype) if outputs else None
+    cnt = np.empty(cap, dtype=np.uint64) if outputs else None
+    f = [np.empty(cap, dtype=np.float64) if outputs else None for _ in range(3)]
+    i = [np.empty(cap, dtype=np.int64) if outputs and v1 is not None else None for _ in range(3)]
+    g = int(L.vref_hash_groupby(_p(keys), kb, n, _p(v0), k1, _p(v1), threads, 1 if refapi else 0, _p(gk), _p(cnt),
+                                _p(f[0]), _p(f[1]), _p(f[2]), _p(i[0]), _p(i[1]), _p(i[2])))
+    if not outputs:
+        return {"n_groups": g}
+    r = {"n_groups": g, "keys": gk[:g], "counts": cnt[:g], "sum0": f[0][:g], "min0": f[1][:g], "max0": f[2][:g]}
+    if v1 is not None:
+        r.update({"sum1": i[0][:g], "min1": i[1][:g], "max1": i[2][:g]})
+    return r
+
+
+def ref_simd_hash(key: bytes) -> int:
+    """The reference's simd_hash of a key (oracle/_ref)."""
+    L = ref_lib()
+    if L is None:
+        raise RuntimeError("oracle/_ref/libvref.so is not built")
+    b = np.frombuffer(key, dtype=np.uint8).copy()
+    return int(L.vref_simd_hash(_p(b), len(key)))
+
+
+def hash_bytes16(key: bytes) -> int:
+    """The oracle's restatement of the reference's key hash (orc_hash_bytes16)."""
+    b = np.frombuffer(key, dtype=np.uint8).copy()
+    return int(lib().orc_hash_bytes16(_p(b), len(key)))
 
 
 def hash_groupby_f64(keys: np.ndarray, val: np.ndarray, threads: int = 1, refapi: bool = False) -> dict:

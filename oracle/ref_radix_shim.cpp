@@ -68,6 +68,14 @@ void vref_radix_sort_u32_mt16(uint32_t* keys, size_t* idx, size_t n)
     std::memcpy(keys, t.data(), n * sizeof(uint32_t));
 }
 
+// at the reference's thread cap (MAX_THREADS = 32, details/constants.inl:9)
+void vref_radix_sort_u32_mt32(uint32_t* keys, size_t* idx, size_t n)
+{
+    std::vector<uint32_t> t(keys, keys + n);
+    vref::radix_sort_uint32_mt<32, false>(t, idx, n);
+    std::memcpy(keys, t.data(), n * sizeof(uint32_t));
+}
+
 void vref_radix_sort_u64(uint64_t* keys, size_t* idx, size_t n)
 {
     std::vector<uint64_t> t(keys, keys + n);

@@ -325,6 +325,73 @@ def test_hash_groupby_matches_sorted(threads, refapi):
     assert np.array_equal(h["keys"], key[np.sort(first)])
 
 
+# ------------------------------------ the reference's hash group-by (oracle/_ref)
+def _need_ref():
+    if oracle.ref_lib() is None:
+        pytest.skip("oracle/_ref not built (the reference tree is absent)")
+
+
+def _key_order(keys: np.ndarray) -> np.ndarray:
+    if keys.ndim == 2:
+        return np.lexsort(keys.T[::-1])
+    return np.argsort(oracle.order_bits(keys), kind="stable")
+
+
+@pytest.mark.parametrize("threads,refapi", [(1, False), (5, False), (16, False), (1, True), (7, True)])
+@pytest.mark.parametrize("mix", ["f64", "f64+i32", "f64+i64"])
+@pytest.mark.parametrize("kind", ["i32", "u64", "char16"])
+def test_ref_hash_groupby_vs_oracle(kind, mix, threads, refapi):
+    """The timed CPU baseline (the reference's map and hash under restated
+    loops) gives the oracle's groups, counts, min, max and integer sums
+    exactly and its float64 sums within 1e-12."""
+    _need_ref()
+    rng = np.random.default_rng(len(kind) * 31 + threads)
+    n = 40_000
+    if kind == "i32":
+        keys = rng.integers(-500, 500, n).astype(np.int32)
+    elif kind == "u64":
+        keys = rng.integers(0, 2**63, n // 4, dtype=np.int64).astype(np.uint64)[rng.integers(0, n // 4, n)]
+    else:
+        keys = datagen.char16_keys(5, 0, n, distinct=3000)
+    v0 = rng.standard_normal(n)
+    v0[rng.integers(0, n, 50)] = -0.0
+    v1 = None
+    if mix == "f64+i32":
+        v1 = rng.integers(-10**6, 10**6, n).astype(np.int32)
+    elif mix == "f64+i64":
+        v1 = rng.integers(0, 2**62, n).astype(np.int64)
+    r = oracle.ref_hash_groupby(keys, v0, v1, threads=threads, refapi=refapi)
+    o = oracle.groupby(keys, [v0] + ([v1] if v1 is not None else []))
+    g = r["n_groups"]
+    assert g == o["n_groups"]
+    order = _key_order(r["keys"])
+    assert np.array_equal(r["keys"][order], o["keys"])
+    assert np.array_equal(r["counts"][order], o["counts"])
+    np.testing.assert_allclose(r["sum0"][order], o["aggs"][0]["sum"], rtol=1e-12, atol=1e-12)
+    assert np.array_equal(r["min0"][order].view(np.uint64), o["aggs"][0]["min"].view(np.uint64))
+    assert np.array_equal(r["max0"][order].view(np.uint64), o["aggs"][0]["max"].view(np.uint64))
+    if v1 is not None:
+        assert np.array_equal(r["sum1"][order], o["aggs"][1]["sum"])
+        assert np.array_equal(r["min1"][order], o["aggs"][1]["min"].astype(np.int64))
+        assert np.array_equal(r["max1"][order], o["aggs"][1]["max"].astype(np.int64))
+    if threads == 1:  # one map: first-occurrence order
+        _, first = np.unique(oracle.order_bits(keys) if keys.ndim == 1 else keys, axis=0 if keys.ndim == 2 else None,
+                             return_index=True)
+        assert np.array_equal(r["keys"], keys[np.sort(first)])
+
+
+def test_oracle_hash_is_reference_hash_up_to_8_bytes():
+    """orc_hash_bytes16 equals the reference's simd_hash (compiled from
+    custom_simd_hash/simd_hash.hpp) for keys of 1 to 8 bytes; 9 to 16 bytes
+    depart on purpose (the reference copies them into one 8-byte word)."""
+    _need_ref()
+    rng = np.random.default_rng(3)
+    for length in range(1, 9):
+        for _ in range(200):
+            key = rng.integers(0, 256, length).astype(np.uint8).tobytes()
+            assert oracle.hash_bytes16(key) == oracle.ref_simd_hash(key), (length, key)
+
+
 # ------------------------------------------------------ multi-column keys
 def test_groupby_multi_vs_pandas():
     import pandas as pd
