@@ -153,3 +153,73 @@ def test_groupby_batch_loguniform_tables(vdf):
     res2 = vdf.groupby_batch(dev_tables, key_type=N.KEY_I32)
     for i, (a, b) in enumerate(zip(res, res2)):
         _same(a, b, f"repeat[{i}]")
+
+
+def test_groupby_batch_full_config(vdf):
+    """The batched configuration at full size: 4096 tables, rows log-uniform
+    in [1K, 4M] (about 2.0B rows), int32 keys uniform in [0, rows/8), float64
+    values, every aggregate, group capacity rows/8, no permutation (as the
+    benchmark runs it).  Every table: group count within capacity, counts
+    adding up to its rows, keys strictly ascending inside [0, rows/8),
+    offsets the running sum of the counts, the mean between min and max.  A seeded
+    sample of 24 tables, the four largest among them, against the oracle."""
+    import torch
+
+    ntab = 4096
+    sizes = datagen.batched_table_sizes(ntab, 1_000, 4_000_000, seed=4)
+    total = int(sizes.sum())
+    _say("groupby_batch full config", ntab, total)
+    offs = np.zeros(ntab + 1, dtype=np.int64)
+    offs[1:] = np.cumsum(sizes)
+    keys = np.empty(total, dtype=np.int32)
+    vals = np.empty(total, dtype=np.float64)
+    for i, n in enumerate(sizes):
+        k, x = datagen.batched_table(int(n), i)
+        keys[offs[i]:offs[i + 1]] = k
+        vals[offs[i]:offs[i + 1]] = x
+    d_keys = torch.from_numpy(keys).cuda()
+    d_vals = torch.from_numpy(vals).cuda()
+    tables = [(d_keys[offs[i]:offs[i + 1]], [d_vals[offs[i]:offs[i + 1]]]) for i in range(ntab)]
+    caps = [max(int(n) // 8, 1) for n in sizes]
+    pb = vdf.prepare_groupby_batch(tables, aggs=N.AGG_ALL, want_perm=False, key_type=N.KEY_I32, capacities=caps)
+    pb.run()
+    torch.cuda.synchronize()
+    _say("groupby_batch full config: launched, checking every table")
+    ng = torch.cat([t["ng"] for t in pb.keep]).cpu().numpy().astype(np.int64)
+    assert np.all(ng <= np.array(caps)), "a table has more groups than its capacity"
+    rows_seen = 0
+    for i in range(ntab):
+        r = pb.result(i)
+        g = int(ng[i])
+        n = int(sizes[i])
+        cnt = r.counts.to(torch.int64)
+        k = r.keys
+        off = r.offsets.to(torch.int64)
+        assert int(cnt.sum().item()) == n, i
+        assert bool((cnt > 0).all().item()), i
+        assert bool((k[1:] > k[:-1]).all().item()) and int(k[0].item()) >= 0 and int(k[-1].item()) < max(n // 8, 1), i
+        assert int(off[0].item()) == 0 and bool((off[1:g] == torch.cumsum(cnt, 0)[:-1]).all().item()), i
+        a = r.aggs[0]
+        # values in [0, 1): the mean (rounded sum / count) within an ulp-scale margin of [min, max]
+        assert bool((a["mean"] - a["min"] >= -1e-12).all().item()), i
+        assert bool((a["max"] - a["mean"] >= -1e-12).all().item()), i
+        rows_seen += n
+    assert rows_seen == total
+    rng = np.random.default_rng(44)
+    sample = set(np.argsort(sizes)[-4:].tolist()) | set(rng.choice(ntab, 20, replace=False).tolist())
+    for i in sorted(sample):
+        k = keys[offs[i]:offs[i + 1]]
+        v = vals[offs[i]:offs[i + 1]]
+        o = oracle.groupby(k, [v], threads=16)
+        r = pb.result(i)
+        g = o["n_groups"]
+        assert r.n_groups == g, i
+        assert np.array_equal(_u(r.keys, np.int32), o["keys"]), i
+        assert np.array_equal(_u(r.offsets, np.uint32).astype(np.uint64)[:g], o["offsets"][:g]), i
+        assert np.array_equal(_u(r.counts, np.uint32).astype(np.uint64), o["counts"]), i
+        ra, oa = r.aggs[0], o["aggs"][0]
+        assert np.array_equal(_u(ra["min"], np.float64), oa["min"]), i
+        assert np.array_equal(_u(ra["max"], np.float64), oa["max"]), i
+        for kk in ("sum", "mean"):
+            got, exp = _u(ra[kk], np.float64), oa[kk]
+            assert np.all(np.abs(got - exp) <= F64_RTOL * np.abs(exp)), (i, kk)

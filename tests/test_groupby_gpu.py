@@ -41,12 +41,12 @@ def _host(t, dtype):
     return a.view(dtype) if a.ndim == 1 else a
 
 
-def _check(vdf, keys, vals, aggs=N.AGG_ALL, label=""):
+def _check(vdf, keys, vals, aggs=N.AGG_ALL, label="", threads=8):
     _say("groupby", label, keys.dtype, keys.shape, [v.dtype for v in vals])
     kt = oracle.key_type_of(keys)
     vts = [oracle.val_type_of(v) for v in vals]
     r = vdf.groupby(_dev(keys), [_dev(v) for v in vals], aggs=aggs, key_type=kt, val_types=vts)
-    o = oracle.groupby(keys, vals, threads=8)
+    o = oracle.groupby(keys, vals, threads=threads)
     g = o["n_groups"]
     assert r.n_groups == g, label
     if g == 0:
@@ -209,6 +209,41 @@ def test_groupby_zipf_20m(vdf):
     """Zipf(1.1) keys over 2^24 values (the skewed configuration at 20M rows)."""
     key, vf, vi = datagen.zipf_groupby(20_000_000, seed=3)
     _check(vdf, key, [vf, vi], label="zipf20m")
+
+
+def test_groupby_zipf_200m(vdf):
+    """The skewed configuration at full size: 200M rows, int32 keys Zipf(1.1)
+    over 2^24 values (about 8.6M groups, the top one about 23M rows), float64
+    N(0,1) and int64 uniform [0, 2^40) values.  Every group's key, offsets,
+    count, row ids, min, max, sums and means against the oracle; then the
+    1000 largest groups' int64 sums (the top group's wraps past 2^63) from
+    their rows by numpy's wrapping sum, and the 20 largest groups' float64
+    sums against math.fsum of their rows."""
+    import math
+
+    key, vf, vi = datagen.zipf_groupby(200_000_000, seed=3)
+    r, o = _check(vdf, key, [vf, vi], label="zipf200m", threads=16)
+    cnt = o["counts"].astype(np.int64)
+    off = o["offsets"].astype(np.int64)
+    perm = _host(r.perm, np.uint32).astype(np.int64)
+    gsum_f = _host(r.aggs[0]["sum"], np.float64)
+    gsum_i = _host(r.aggs[1]["sum"], np.int64)
+    top = np.argsort(cnt, kind="stable")[::-1][:1000]
+    assert cnt[top[0]] > 22_000_000
+    wrapped = 0
+    for j, g in enumerate(top):
+        rows = perm[off[g]:off[g + 1]]
+        s = int(vi[rows].sum())  # numpy int64 sum wraps modulo 2^64
+        exact = int(vi[rows].astype(object).sum()) if j == 0 else None
+        if exact is not None:
+            assert exact > 2**63 - 1  # the top group's true sum overflows int64
+            wrapped += 1
+            assert (exact - s) % 2**64 == 0
+        assert gsum_i[g] == s, (g, int(gsum_i[g]), s)
+        if j < 20:
+            e = math.fsum(vf[rows].tolist())
+            assert abs(gsum_f[g] - e) <= F64_RTOL * abs(e), (g, float(gsum_f[g]), e)
+    assert wrapped == 1
 
 
 def test_groupby_100m_i32(vdf):

@@ -110,37 +110,70 @@ def test_multi_key_too_wide_rejected(vdf):
     assert e.value.status == N.VDF_ERR_UNSUPPORTED
 
 
+def _agg_vs_oracle(r, o, vals, label):
+    """Per-group outputs of a GPU result against the oracle's."""
+    g = o["n_groups"]
+    assert r.n_groups == g, label
+    k = r.keys.cpu().numpy()
+    assert np.array_equal(k.view(np.uint8).reshape(-1), o["keys"].view(np.uint8).reshape(-1)), label
+    assert np.array_equal(r.offsets.cpu().numpy().view(np.uint32).astype(np.uint64), o["offsets"]), label
+    assert np.array_equal(r.counts.cpu().numpy().view(np.uint32).astype(np.uint64), o["counts"]), label
+    assert np.array_equal(r.perm.cpu().numpy().view(np.uint32).astype(np.uint64), o["perm"]), label
+    for c, v in enumerate(vals):
+        ra, oa = r.aggs[c], o["aggs"][c]
+        assert np.array_equal(_bytes(ra["min"]), oa["min"].view(np.uint8)), (label, c)
+        assert np.array_equal(_bytes(ra["max"]), oa["max"].view(np.uint8)), (label, c)
+        if v.dtype == np.float64:
+            for kk in ("sum", "mean"):
+                got = ra[kk].cpu().numpy()
+                assert np.all(np.abs(got - oa[kk]) <= F64_RTOL * np.abs(oa[kk])), (label, c, kk)
+        else:
+            assert np.array_equal(ra["sum"].cpu().numpy().view(np.int64), oa["sum"].view(np.int64)), (label, c)
+            assert np.array_equal(ra["mean"].cpu().numpy(), oa["mean"]), (label, c)
+
+
 @pytest.mark.parametrize("src", ["f64_sum", "i64_sum", "count", "i32_min"])
 def test_broadcast_per_row(vdf, src):
+    """vdf_broadcast writes each row its group's value: against the oracle's
+    group-by broadcast to rows by key (exact for counts, integer sums and
+    min; float64 sums within 1e-12 of the oracle's exact sums), and
+    bit-identical to the GPU group value it was given."""
     rng = np.random.default_rng(12)
     n = 500_001
     k = rng.integers(0, 7000, n).astype(np.int64)
     vf = rng.standard_normal(n)
     vi = rng.integers(-1000, 1000, n).astype(np.int32)
     r = vdf.groupby(_dev(k), [_dev(vf), _dev(vi)], trim=False)
+    o = oracle.groupby(k, [vf, vi], threads=8)
     if src == "f64_sum":
-        gv, to64 = r.aggs[0]["sum"], False
+        gv, to64, ov = r.aggs[0]["sum"], False, o["aggs"][0]["sum"]
     elif src == "i64_sum":
-        gv, to64 = r.aggs[1]["sum"], True
+        gv, to64, ov = r.aggs[1]["sum"], True, o["aggs"][1]["sum"].view(np.int64)
     elif src == "count":
-        gv, to64 = r.counts, True  # counts (int32 cells) as a float64 column
+        gv, to64, ov = r.counts, True, o["counts"]  # counts (int32 cells) as a float64 column
     else:
-        gv, to64 = r.aggs[1]["min"], False
+        gv, to64, ov = r.aggs[1]["min"], False, o["aggs"][1]["min"]
     out = vdf.broadcast(gv, r, n, to_f64=to64).cpu().numpy()
     g = int(r.n_groups.item())
-    gkeys = r.keys[:g].cpu().numpy()
-    gval = gv[:g].cpu().numpy()
-    row_group = np.searchsorted(gkeys, k)
-    exp = gval[row_group]
+    assert g == o["n_groups"]
+    row_group = np.searchsorted(o["keys"], k)  # the oracle's group of every row
+    exp = ov[row_group]
     if to64:
         exp = exp.astype(np.float64)
-    assert np.array_equal(out.view(np.uint8), exp.view(np.uint8)), src
+    if src == "f64_sum":
+        assert np.all(np.abs(out - exp) <= F64_RTOL * np.abs(exp)), src
+    else:
+        assert np.array_equal(out.view(np.uint8), exp.view(np.uint8)), src
+    gval = gv[:g].cpu().numpy()
+    same = gval[row_group].astype(np.float64) if to64 else gval[row_group]
+    assert np.array_equal(out.view(np.uint8), same.view(np.uint8)), src
 
 
 @pytest.mark.parametrize("kind", ["i32", "u64", "char16", "f64"])
-def test_cached_grouping_equals_groupby(vdf, kind):
-    """Aggregating against a kept grouping equals a fresh group-by, for
-    several value columns in turn (no re-sort between them)."""
+def test_cached_grouping_vs_oracle(vdf, kind):
+    """Aggregating against a kept grouping, for several value columns in
+    turn (no re-sort between them): every output against the oracle's
+    group-by of the same columns, and bit-identical to a fresh group-by."""
     rng = np.random.default_rng(17)
     n = 400_003
     keys = _key_cols((kind,), n, rng)[0]
@@ -151,6 +184,7 @@ def test_cached_grouping_equals_groupby(vdf, kind):
                               [rng.integers(0, 1000, n).astype(np.int32), rng.standard_normal(n)], []]):
         dv = [_dev(v) for v in vals]
         got = g.aggregate(dv)
+        _agg_vs_oracle(got, oracle.groupby(keys, vals, threads=8), vals, f"{kind}[{c}]")
         exp = vdf.groupby(dk, dv, key_type=kt)
         assert got.n_groups == exp.n_groups, (kind, c)
         for x, y in ((got.keys, exp.keys), (got.offsets, exp.offsets), (got.counts, exp.counts),

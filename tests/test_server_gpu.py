@@ -230,3 +230,52 @@ def test_groupby_capacity_truncation(vdf):
         assert np.array_equal(_bits(r.aggs[0][name][:cap]), _bits(full.aggs[0][name][:cap])), name
     with pytest.raises(N.VdfError):
         vdf.groupby(dk, [dx], key_type=N.KEY_U64, capacity=cap)
+
+
+def test_served_large_tables_vs_oracle():
+    """Served tables at the configuration's sizes: the benchmark trace's two
+    largest tables (32.7M rows of 16-byte keys, 29.1M rows of uint64 keys)
+    and a 50M-row uint64 table (the configuration's upper bound), submitted
+    between small requests to a 3-worker server; every result against the
+    oracle (keys, counts, min, max exact; sums and means within 1e-12)."""
+    import torch
+
+    from visondf_amd.server import Server
+
+    sizes, kinds, _, _ = datagen.served_trace()
+    order = np.argsort(sizes)[::-1]
+    big_c16 = int(next(i for i in order if kinds[i] == "char16"))
+    big_u64 = int(next(i for i in order if kinds[i] == "u64"))
+    specs = [("char16", int(sizes[big_c16]), big_c16), ("u64", int(sizes[big_u64]), big_u64),
+             ("u64", 50_000_000, 1000), ("char16", 20_000, 1001), ("u64", 70_001, 1002)]
+    tabs = []
+    for kind, n, tid in specs:
+        k, x = datagen.served_table(n, tid, kind)
+        tabs.append((kind, k, x, _dev(k), _dev(x)))
+    torch.cuda.synchronize()
+    with Server(0, arena_bytes=48 << 30, workers=3, max_batch_tables=64) as srv:
+        ids = [srv.add_table(dk, [dx], key_type=_kt(kind), max_groups=k.shape[0] // 16 + 1)
+               for kind, k, _, dk, dx in tabs]
+        order_sub = [3, 0, 4, 1, 3, 2, 4]
+        rids = [(t, srv.submit(ids[t])) for t in order_sub]
+        for t, rid in rids:
+            kind, k, x, _, _ = tabs[t]
+            _say("served large: table", t, kind, k.shape[0])
+            s = srv.wait(rid, timeout_s=120.0)
+            assert s is not None and s.status == N.VDF_OK, t
+            got = srv.tensors(ids[t], s)
+            o = oracle.groupby(k, [x], threads=16)
+            assert s.n_groups == o["n_groups"], t
+            gk = got["keys"].cpu().numpy()
+            assert np.array_equal(gk.view(np.uint64) if kind == "u64" else gk, o["keys"]), t
+            assert np.array_equal(got["offsets"].cpu().numpy().view(np.uint32).astype(np.uint64), o["offsets"]), t
+            assert np.array_equal(got["counts"].cpu().numpy().view(np.uint32).astype(np.uint64), o["counts"]), t
+            a, oa = got["aggs"][0], o["aggs"][0]
+            assert np.array_equal(a["min"].cpu().numpy(), oa["min"]), t
+            assert np.array_equal(a["max"].cpu().numpy(), oa["max"]), t
+            for name in ("sum", "mean"):
+                g = a[name].cpu().numpy()
+                assert np.all(np.abs(g - oa[name]) <= F64_RTOL * np.abs(oa[name])), (t, name)
+            srv.release(rid)
+        st = srv.stats()
+        assert st["completed"] == len(rids) and st["failed"] == 0 and st["arena_used"] == 0
