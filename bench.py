@@ -40,9 +40,9 @@ HBM_PEAK_GBPS = 8000.0
 METRIC = "group-by rows/sec + HBM GB/s (% peak) at 100M rows; served p50/p99 latency"
 KERNELS = ("radix_histogram", "radix_plan", "radix_pass", "gather", "fill", "groupby_empty", "groupby_tile",
            "groupby_fixup")
-WORKLOADS = ("argsort_u32", "groupby_c1", "groupby_i32", "zipf_groupby", "batched", "served")
-DEFAULT_WORKLOADS = ("argsort_u32", "groupby_c1", "groupby_i32", "zipf_groupby", "batched", "served")
-CHILD_TIMEOUT = {"argsort_u32": 600, "groupby_c1": 300, "groupby_i32": 400, "zipf_groupby": 400, "batched": 500, "served": 500}
+WORKLOADS = ("argsort_u32", "groupby_c1", "groupby_i32", "groupby_i32_full", "zipf_groupby", "batched", "served")
+DEFAULT_WORKLOADS = ("argsort_u32", "groupby_c1", "groupby_i32", "groupby_i32_full", "zipf_groupby", "batched", "served")
+CHILD_TIMEOUT = {"argsort_u32": 600, "groupby_c1": 300, "groupby_i32": 400, "groupby_i32_full": 400, "zipf_groupby": 400, "batched": 500, "served": 500}
 
 
 def _log(*a):
@@ -341,9 +341,10 @@ def _cpu_groupby_legs(rec: dict, rows_per_s: float, full, sample_api, threads: i
         rec["cpu_all_cores"] = _cpu_ref_groupby(full, allc, what_full + ", all physical cores", refapi=False)
 
 
-def run_groupby_i32(args) -> dict:
-    """100M rows, int32 key uniform in [0, 1000), float64 and int32 values,
-    every aggregate (the CPU reference configuration's group-by at 100M rows)."""
+def run_groupby_i32(args, full_range: bool = False) -> dict:
+    """100M rows, int32 key uniform in [0, 1000) (or, full_range, over all
+    2^32 values: about 98M groups), float64 and int32 values, every
+    aggregate (the CPU reference configuration's group-by at 100M rows)."""
     import torch
 
     from visondf_amd import _native as N
@@ -351,8 +352,9 @@ def run_groupby_i32(args) -> dict:
     from visondf_amd.device import Vdf
 
     n = args.rows or 100_000_000
-    _log(f"groupby_i32: generating {n} rows")
-    key, vf, vi = datagen.groupby_i32(n)
+    name = "groupby_i32_full" if full_range else "groupby_i32"
+    _log(f"{name}: generating {n} rows")
+    key, vf, vi = datagen.groupby_i32_full(n) if full_range else datagen.groupby_i32(n)
     d_key = torch.from_numpy(key).cuda()
     d_vf = torch.from_numpy(vf).cuda()
     d_vi = torch.from_numpy(vi).cuda()
@@ -369,17 +371,27 @@ def run_groupby_i32(args) -> dict:
     cnt = res["r"].counts[:g].to(torch.int64).sum().item()
     assert cnt == n, "group counts do not add up to the rows"
     rec = _base_record(args, rows_per_s, "rows/s", ms_step, "f64+i32",
-                       {"workload": "groupby_i32", "rows": n, "key": "int32 uniform [0, 1000)",
+                       {"workload": name, "rows": n,
+                        "key": "int32 uniform over 2^32" if full_range else "int32 uniform [0, 1000)",
                         "values": ["float64 uniform [0,1)", "int32 uniform [-1e6,1e6]"],
                         "aggregates": "count,sum,mean,min,max", "groups": g, "parallelism": "single GPU"})
-    rec["roofline"] = _roofline("groupby_i32", kernels)
+    rec["roofline"] = _roofline(name, kernels)
     rec["kernels"] = kernels
     if not args.no_cpu:
-        m = min(n, 20_000_000)
-        _cpu_groupby_legs(rec, rows_per_s, [(key, vf, vi)], [(key[:m], vf[:m], vi[:m])], _threads(args),
-                          f"full workload ({n} rows)", f"first {m} rows of the workload", reps=3)
+        if full_range:  # ~one group per row: bounded samples
+            m, ma = min(n, 20_000_000), min(n, 2_000_000)
+            _cpu_groupby_legs(rec, rows_per_s, [(key[:m], vf[:m], vi[:m])], [(key[:ma], vf[:ma], vi[:ma])],
+                              _threads(args), f"first {m} rows of the workload", f"first {ma} rows of the workload")
+        else:
+            m = min(n, 20_000_000)
+            _cpu_groupby_legs(rec, rows_per_s, [(key, vf, vi)], [(key[:m], vf[:m], vi[:m])], _threads(args),
+                              f"full workload ({n} rows)", f"first {m} rows of the workload", reps=3)
     v.close()
     return rec
+
+
+def run_groupby_i32_full(args) -> dict:
+    return run_groupby_i32(args, full_range=True)
 
 
 def run_groupby_c1(args) -> dict:
@@ -686,13 +698,14 @@ def run_served(args) -> dict:
     return rec
 
 
-RUNNERS = {"argsort_u32": run_argsort_u32, "groupby_c1": run_groupby_c1, "groupby_i32": run_groupby_i32, "zipf_groupby": run_zipf_groupby, "batched": run_batched,
+RUNNERS = {"argsort_u32": run_argsort_u32, "groupby_c1": run_groupby_c1, "groupby_i32": run_groupby_i32,
+           "groupby_i32_full": run_groupby_i32_full, "zipf_groupby": run_zipf_groupby, "batched": run_batched,
            "served": run_served}
 
 
 def _child(name: str, args) -> dict:
     steps, warmup = args.steps, args.warmup
-    if name in ("groupby_i32", "zipf_groupby", "batched"):  # long steps: fewer of them
+    if name in ("groupby_i32", "groupby_i32_full", "zipf_groupby", "batched"):  # long steps: fewer of them
         steps, warmup = max(3, args.steps // 4), min(args.warmup, 2)
     cmd = [sys.executable, os.path.abspath(__file__), "--workload", name, "--steps", str(steps),
            "--warmup", str(warmup)]

@@ -267,16 +267,33 @@ class Dataframe {
                 return;
             }
         }
+        // In view mode the key is sorted as the view presents it (gathered
+        // through the view), so the new view orders the viewed rows by the key,
+        // stably: new_view[j] = view[q[j]] for the sort q of the viewed keys.
+        // The reference instead scatters the stored-order permutation p into
+        // the view by cycle swaps (sort_by_mt.hpp:164-173), new_view[p[i]] =
+        // view[i], which does not order the rows by the key.
+        const void* keys = k.data.get();
+        DeviceBuffer viewed;
+        if (Soft && in_view_) {
+            std::vector<uint32_t> v(row_view_idx_.begin(), row_view_idx_.end());
+            DeviceBuffer vp(v.size() * 4);
+            hip_check(hipMemcpy(vp.get(), v.data(), v.size() * 4, hipMemcpyHostToDevice), "hipMemcpy H2D");
+            viewed = DeviceBuffer(k.data.bytes());
+            check(vdf_gather(h_.get(), static_cast<uint32_t>(k.elem), k.data.get(), vp.get(), 0, nrow_, viewed.get(),
+                             nullptr));
+            keys = viewed.get();
+        }
         if (k.str)  // sort_string (sort_by_mt.hpp:72-115 dispatch)
-            check(vdf_argsort_bytes(h_.get(), k.data.get(), static_cast<uint32_t>(k.elem), nrow_,
+            check(vdf_argsort_bytes(h_.get(), keys, static_cast<uint32_t>(k.elem), nrow_,
                                     ASC ? 0u : VDF_SORT_DESC_REVERSE, perm.get(), nullptr, nullptr));
         else
-            check(vdf_argsort(h_.get(), k.key_type, k.data.get(), nrow_, ASC ? 0u : VDF_SORT_DESC_REVERSE, perm.get(),
-                              nullptr, nullptr));
+            check(vdf_argsort(h_.get(), k.key_type, keys, nrow_, ASC ? 0u : VDF_SORT_DESC_REVERSE, perm.get(), nullptr,
+                              nullptr));
         if constexpr (Soft) {
             std::vector<uint32_t> p(nrow_);
             hip_check(hipMemcpy(p.data(), perm.get(), nrow_ * 4ull, hipMemcpyDeviceToHost), "hipMemcpy D2H");
-            if (in_view_) {  // compose with the existing view
+            if (in_view_) {  // compose with the existing view (gather form)
                 std::vector<size_t> r(nrow_);
                 for (size_t i = 0; i < nrow_; ++i) r[i] = row_view_idx_[p[i]];
                 row_view_idx_.swap(r);

@@ -122,6 +122,48 @@ int main()
         b.sort_by<true, false>(c);
         CHECK(a.get_col<double>(2) == b.get_col<double>(2));
     }
+    // two soft sorts in a row: the second orders the viewed rows by its key,
+    // stably, so the result is ordered by (int64 key, then the uint32 key's
+    // order, then row); checked against the host's stable sorts composed the
+    // same way and against a lexicographic stable sort
+    std::printf("soft sort twice + materialize\n");
+    {
+        Dataframe df = load(f);
+        unsigned int c0 = 0, c1 = 1;
+        df.sort_by<true, true>(c0);
+        df.sort_by<true, true>(c1);
+        CHECK(df.in_view());
+        const auto v1 = stable_order(f.k32);
+        std::vector<int64_t> seen(n);
+        for (size_t j = 0; j < n; ++j) seen[j] = f.k64[v1[j]];
+        const auto q = stable_order(seen);
+        std::vector<size_t> want(n);
+        for (size_t j = 0; j < n; ++j) want[j] = v1[q[j]];
+        const auto& view = df.get_row_view_idx();
+        CHECK(view.size() == n && std::equal(view.begin(), view.end(), want.begin()));
+        std::vector<size_t> lex(n);
+        std::iota(lex.begin(), lex.end(), 0);
+        std::stable_sort(lex.begin(), lex.end(), [&](size_t a, size_t b) {
+            return f.k64[a] != f.k64[b] ? f.k64[a] < f.k64[b] : f.k32[a] < f.k32[b];
+        });
+        CHECK(want == lex);
+        df.materialize();
+        const auto k64 = df.get_col<int64_t>(1);
+        const auto vi = df.get_col<int32_t>(3);
+        bool ok = true;
+        for (size_t i = 0; i < n && ok; ++i) ok = k64[i] == f.k64[want[i]] && vi[i] == f.vi[want[i]];
+        CHECK(ok);
+        // and a descending soft sort over a view: the reversed ascending order of the viewed keys
+        Dataframe d2 = load(f);
+        d2.sort_by<true, true>(c0);
+        d2.sort_by<false, true>(c1);
+        std::vector<size_t> rq = q;
+        std::reverse(rq.begin(), rq.end());
+        const auto& view2 = d2.get_row_view_idx();
+        bool ok2 = view2.size() == n;
+        for (size_t j = 0; j < n && ok2; ++j) ok2 = view2[j] == v1[rq[j]];
+        CHECK(ok2);
+    }
     // group_by on the int64 key: counts, integer sums, min / max exact; f64 within 1e-12
     std::printf("group_by int64\n");
     {

@@ -380,3 +380,12 @@ def test_groupby_without_permutation_identical(vdf, combo):
     assert all(x == y for x, y in zip(a, b)), combo
     o = oracle.groupby(keys, vals, threads=8)
     assert np.array_equal(_host(without.counts, np.uint32).astype(np.uint64), o["counts"])
+
+
+def test_groupby_100m_i32_full_range(vdf):
+    """100M rows of int32 keys uniform over the full 32-bit range (about 98M
+    groups, four radix passes, output-heavy), float64 and int32 values:
+    every output against the oracle at full size."""
+    key, vf, vi = datagen.groupby_i32_full(100_000_000)
+    r, o = _check(vdf, key, [vf, vi], label="i32 full 100m", threads=16)
+    assert o["n_groups"] > 95_000_000

@@ -142,6 +142,7 @@ void Handle::release_resources()
     for (ProfRec& r : recs) {
         (void)hipEventDestroy(r.beg);
         (void)hipEventDestroy(r.end);
+        if (r.groups_done) (void)hipEventDestroy(r.groups_done);
     }
     recs.clear();
     for (hipEvent_t e : free_events) (void)hipEventDestroy(e);
@@ -194,6 +195,11 @@ void Handle::prof_collect()
                 const int pos = D == 1 ? 0 : s == 0 ? 1 : s == D - 1 ? 3 : 2;
                 bytes += pa.rows[i] * pa.per_row[pos];
             }
+        }
+        if (r.groups) {
+            VDF_HIP_CHECK(hipEventSynchronize(r.groups_done));
+            bytes += *r.groups * r.group_bytes;
+            free_events.push_back(r.groups_done);
         }
         if (active) {
             auto it = std::find_if(totals.begin(), totals.end(), [&](const ProfTotal& t) { return t.name == r.name; });

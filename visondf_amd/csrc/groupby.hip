@@ -217,6 +217,7 @@ struct GbArgs {
     Agg* suffix;               // [ncols][ntiles] rows from the tile's last head to its end
     uint32_t* long_count;      // tiles queued in long_list (zeroed before the tile kernel)
     uint32_t* long_list;       // [ntiles] tiles whose last group spans following tiles
+    unsigned long long* group_total;  // groups written over all tables (profiling only, else null)
 };
 
 // Output pointers of one value column, read from the table descriptor once
@@ -783,6 +784,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                 if (t.lt == sg.ntiles - 1) {
                     *gmem(sg.out.n_groups) = static_cast<uint64_t>(excl) + t.H;
                     if (excl + t.H <= sg.cap) gmem(sg.out.offsets)[excl + t.H] = static_cast<uint32_t>(sg.n);
+                    if (a.group_total)
+                        atomicAdd(a.group_total, static_cast<unsigned long long>(
+                                                     std::min<uint64_t>(static_cast<uint64_t>(excl) + t.H, sg.cap)));
                 }
             }
         }
@@ -1062,6 +1066,7 @@ size_t agg_bytes_for(uint64_t tiles, int ntables)
     b += align_up((tiles + 2) * 4, 256);                       // status + ticket + long_count
     b += 3 * align_up(std::max<uint64_t>(tiles, 1) * 4, 256);  // tile_gbase, tile_heads, long_list
     b += 2 * align_up(VDF_MAX_VALUE_COLS * std::max<uint64_t>(tiles, 1) * sizeof(Agg), 256);
+    b += 256;  // group total (profiling)
     return b + 256;
 }
 
@@ -1151,6 +1156,7 @@ void groupby_aggregate_enqueue(Handle* h, int key_type, const void* skeys, const
     a.long_list = cv.take<uint32_t>(std::max<uint64_t>(tiles_total, 1));
     a.prefix = cv.take<Agg>(VDF_MAX_VALUE_COLS * std::max<uint64_t>(tiles_total, 1));
     a.suffix = cv.take<Agg>(VDF_MAX_VALUE_COLS * std::max<uint64_t>(tiles_total, 1));
+    unsigned long long* gtotal = cv.take<unsigned long long>(1);
 
     // table descriptions and the tile map (pinned staging -> device)
     const size_t seg_bytes = align_up(static_cast<size_t>(ntables) * sizeof(GbSeg), 256);
@@ -1218,12 +1224,27 @@ void groupby_aggregate_enqueue(Handle* h, int key_type, const void* skeys, const
             val_bytes += (c < ncarried ? 0 : 4) + ((val_types[c] == VDF_VAL_I32 || val_types[c] == VDF_VAL_U32) ? 4 : 8);
         }
     }
+    // group outputs per group (of table 0: every table of a batch asks for the same ones)
+    uint64_t group_bytes = 0;
+    {
+        const vdf_groupby_out& o = outs[0];
+        group_bytes += (o.group_keys ? kb : 0) + (o.offsets ? 4 : 0) + (o.counts ? 4 : 0);
+        for (int c = 0; c < nvals; ++c) {
+            const uint64_t vb = (val_types[c] == VDF_VAL_I32 || val_types[c] == VDF_VAL_U32) ? 4 : 8;
+            group_bytes += (o.aggs[c].sum ? 8 : 0) + (o.aggs[c].mean ? 8 : 0) + (o.aggs[c].min ? vb : 0) +
+                           (o.aggs[c].max ? vb : 0);
+        }
+    }
+    const bool count_groups = h->prof && h->passes_ring;
+    a.group_total = count_groups ? gtotal : nullptr;
     {
         LaunchTimer t(h, s, "fill", 0);
         fill_u32(status, tiles + 2, 0u, s);
+        if (count_groups) fill_u32(reinterpret_cast<uint32_t*>(gtotal), 2, 0u, s);
     }
     {
-        // algorithmic bytes: sorted keys + per column its value (and row id when gathered)
+        // algorithmic bytes: sorted keys + per column its value (and row id when
+        // gathered), plus (profiling: counted on the device) the group outputs
         LaunchTimer t(h, s, "groupby_tile", rows_total * (kb + val_bytes));
         // persistent grid: the resident blocks (tiles are claimed dynamically)
         const void* fn = kb == 4 ? reinterpret_cast<const void*>(k_gb_tile<uint32_t>)
@@ -1236,6 +1257,20 @@ void groupby_aggregate_enqueue(Handle* h, int key_type, const void* skeys, const
         default: hipLaunchKernelGGL(k_gb_tile<K16>, dim3(grid), dim3(kBlock), 0, s, a); break;
         }
         check_launch("k_gb_tile");
+    }
+    if (count_groups) {
+        // the group count comes back into the pinned ring; the tile launch is
+        // charged the group outputs of every table (the fix-up writes some of them)
+        if (h->ring_pos % 2) ++h->ring_pos;
+        if (h->ring_pos + 2 > Handle::kRing) h->ring_pos = 0;
+        uint64_t* dst = reinterpret_cast<uint64_t*>(h->passes_ring + h->ring_pos);
+        h->ring_pos += 2;
+        VDF_HIP_CHECK(hipMemcpyAsync(dst, gtotal, 8, hipMemcpyDeviceToHost, s));
+        hipEvent_t done = h->get_event();
+        VDF_HIP_CHECK(hipEventRecord(done, s));
+        h->recs.back().groups = dst;
+        h->recs.back().groups_done = done;
+        h->recs.back().group_bytes = group_bytes;
     }
     {
         LaunchTimer t(h, s, "groupby_fixup", 0);

@@ -114,6 +114,9 @@ struct ProfRec {
     uint64_t bytes;                        // algorithmic bytes of the launch (no acct)
     int plan_slot;                         // >= 0: radix pass slot
     std::shared_ptr<const PassAcct> acct;  // pass slots: per-table activity and bytes
+    const uint64_t* groups = nullptr;      // group-by tile launch: pinned copy of the groups written,
+    hipEvent_t groups_done = nullptr;      // valid once this event completed,
+    uint64_t group_bytes = 0;              // and the output bytes per group
 };
 
 struct ProfTotal {

@@ -145,6 +145,16 @@ def groupby_i32(n: int = 100_000_000, seed: int = 6):
     return key, vf, vi
 
 
+def groupby_i32_full(n: int = 100_000_000, seed: int = 8):
+    """100M rows, int32 key uniform over the full 32-bit range (about 98M
+    groups at 100M rows: four radix passes, output-heavy), float64 uniform
+    [0, 1) and int32 uniform [-1e6, 1e6] values."""
+    key = (splitmix64(seed, 0, n) >> np.uint64(32)).astype(np.uint32).view(np.int32)
+    vf = uniform01(seed, 1, n)
+    vi = uniform_int(seed, 2, n, -1_000_000, 1_000_000).astype(np.int32)
+    return key, vf, vi
+
+
 def argsort_u32(n: int = 100_000_000, seed: int = 2):
     key = (splitmix64(seed, 0, n) >> np.uint64(32)).astype(np.uint32)
     val = uniform01(seed, 1, n)

@@ -6,11 +6,39 @@ GPU or without the built library these calls raise.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
+import weakref
 
 import torch
 
 from . import _native as N
+
+# Library objects still open when the interpreter exits are closed by an
+# atexit hook (registered after torch's, so it runs before torch and the HIP
+# runtime tear down), never by __del__ during interpreter teardown.
+# Objects that use a handle (groupings, captured plans, servers) close
+# before the handles.
+_live_users = weakref.WeakSet()
+_live_handles = weakref.WeakSet()
+
+
+def _close_live():
+    for group in (_live_users, _live_handles):
+        for obj in list(group):
+            try:
+                obj.close()
+            except Exception:
+                pass
+
+
+atexit.register(_close_live)
+
+
+def track(obj, handle: bool = False):
+    """Registers a library object whose close() must run before exit."""
+    (_live_handles if handle else _live_users).add(obj)
+    return obj
 
 _KEY_DTYPES = {
     N.KEY_U32: (torch.uint32, 4),
@@ -76,6 +104,7 @@ class Vdf:
         h = C.c_void_p()
         N.check(self.lib.vdf_create(C.byref(h), device, hbm_budget_bytes))
         self.h = h
+        track(self, handle=True)
 
     def close(self):
         if self.h:
@@ -430,6 +459,7 @@ class PreparedBatch:
         self.set_tables(tables)
         self.vt_arr = (C.c_int * max(self.nv, 1))(*self.vts)
         self.plan = None
+        track(self)
 
     def set_tables(self, tables):
         """Points the batch at other tables with the same row counts."""
@@ -474,6 +504,9 @@ class PreparedBatch:
             N.check(self.v.lib.vdf_plan_destroy(self.plan))
             self.plan = None
 
+    def close(self):
+        self.release_plan()
+
     def __del__(self):
         try:
             self.release_plan()
@@ -495,6 +528,7 @@ class Grouping:
         N.check(vdf.lib.vdf_grouping_create(vdf.h, key_type, _ptr(keys), self.n, C.c_void_p(_stream_ptr(stream)),
                                             C.byref(g)))
         self.g = g
+        track(self)
 
     def aggregate(self, vals=(), aggs: int = N.AGG_ALL, want_perm: bool = True, stream=None, trim: bool = True,
                   val_types=None) -> GroupbyResult:

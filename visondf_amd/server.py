@@ -15,7 +15,7 @@ from dataclasses import dataclass
 import torch
 
 from . import _native as N
-from .device import key_type_of, val_type_of
+from .device import key_type_of, track, val_type_of
 
 _HIP_D2D = 3  # hipMemcpyDeviceToDevice
 _hip = None
@@ -74,6 +74,7 @@ class Server:
         h = C.c_void_p()
         N.check(self.lib.vdf_server_create(C.byref(h), C.byref(cfg)))
         self.h = h
+        track(self)
         self._tables = {}  # id -> (keys, vals, key_type, val dtypes) kept alive for the server's life
 
     def close(self):
