@@ -65,6 +65,9 @@ typedef int vdf_status;
 #define VDF_KEY_CHAR16  4 /* fixed-width 16-byte char[16] cells (CharT = char[16]) */
 #define VDF_KEY_F32     5 /* IEEE float, total order of sort_flt.inl:44-50:              */
 #define VDF_KEY_F64     6 /* -NaN < -inf < ... < -0 < +0 < ... < +inf < +NaN; groups by bits */
+#define VDF_KEY_BYTES   7 /* multi-column keys only: fixed-width byte strings of           */
+                          /* vdf_key_col.width bytes (padded std::string cells), ordered */
+                          /* as vdf_argsort_bytes orders them                             */
 
 /* value column types */
 #define VDF_VAL_F64 0
@@ -199,16 +202,20 @@ vdf_status vdf_groupby(vdf_handle* h, int key_type, const void* d_keys, uint64_t
                        vdf_groupby_out* out, void* stream);
 
 /* ---- multi-column keys ----
- * Rows are keyed by several columns (their byte widths summing to at most 16);
- * equal keys are equal in every column, as the reference's concatenated key
- * bytes (transform_group_by_sametype_mt.hpp:13, transform_group_by_difftype_mt.hpp:13,
+ * Rows are keyed by several columns of any key type, byte strings included
+ * (their byte widths summing to at most VDF_MAX_STRING_BYTES); equal keys are
+ * equal in every column, as the reference's concatenated key bytes
+ * (transform_group_by_sametype_mt.hpp:13, transform_group_by_difftype_mt.hpp:13,
  * key_build.inl:3-110).  Order: lexicographic over the columns, first column
- * most significant, each column in its own order (as vdf_argsort).  The
- * columns are packed into one composite key on the device, so the sort moves
- * 4, 8 or 16 key bytes per row. */
+ * most significant, each column in its own order (as vdf_argsort; byte
+ * strings as vdf_argsort_bytes).  The columns are packed into one composite
+ * key on the device in order-preserving big-endian form: up to 16 bytes the
+ * sort moves 4, 8 or 16 key bytes per row; wider composites are fixed-width
+ * byte rows sorted and grouped as vdf_argsort_bytes / vdf_groupby_bytes do. */
 typedef struct vdf_key_col {
     int type;           /* VDF_KEY_* */
     const void* data;   /* device, n cells */
+    uint32_t width;     /* VDF_KEY_BYTES: bytes per cell (1 .. VDF_MAX_STRING_BYTES); else ignored */
 } vdf_key_col;
 
 /* Group by several key columns.  out->group_keys is ignored; group_keys[c]

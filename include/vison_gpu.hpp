@@ -354,9 +354,10 @@ class Dataframe {
         std::vector<void*> kout;
         for (unsigned int k : key_cols) {
             const Column& c = col(k);
-            if (c.key_type < 0 && !(c.str && key_cols.size() == 1))
-                throw Error(VDF_ERR_UNSUPPORTED, "group_by: column type has no key path (string keys: one key column)");
-            kc.push_back(vdf_key_col{c.key_type, c.data.get()});
+            if (c.key_type < 0 && !c.str) throw Error(VDF_ERR_UNSUPPORTED, "group_by: column type has no key path");
+            // a std::string column (stored padded) is a byte-string key column
+            kc.push_back(c.str ? vdf_key_col{VDF_KEY_BYTES, c.data.get(), static_cast<uint32_t>(c.elem)}
+                               : vdf_key_col{c.key_type, c.data.get(), 0});
             D.keys.emplace_back(cap * c.elem);
             kout.push_back(D.keys.back().get());
         }
@@ -378,7 +379,7 @@ class Dataframe {
             out.aggs[c].max = D.agg[4 * c + 3].get();
         }
         const Column& k0 = col(key_cols[0]);
-        if (k0.str)  // a std::string key column
+        if (k0.str && kc.size() == 1)  // a std::string key column
             check(vdf_groupby_bytes(h_.get(), k0.data.get(), static_cast<uint32_t>(k0.elem), n, vc.data(),
                                     static_cast<int>(vc.size()), aggs, &out, nullptr));
         else if (kc.size() == 1)

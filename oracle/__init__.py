@@ -255,8 +255,8 @@ def argsort_multi(keys_list, mode: int = 0) -> np.ndarray:
         ob = order_bits(np.ascontiguousarray(k))
         if mode == 1:
             ob = ~ob
-        if ob.ndim == 2:
-            cols.extend(ob[:, j] for j in range(15, -1, -1))
+        if ob.ndim == 2:  # char[16] or a byte-string column: bytes ^ 0x80, first byte most significant
+            cols.extend(ob[:, j] for j in range(ob.shape[1] - 1, -1, -1))
         else:
             cols.append(ob)
     perm = np.lexsort(cols) if cols and len(cols[0]) else np.zeros(0, dtype=np.int64)

@@ -100,11 +100,67 @@ def test_argsort_multi_vs_oracle(vdf, combo, mode):
     assert np.array_equal(perm.cpu().numpy().view(np.uint32).astype(np.uint64), oracle.argsort_multi(keys, m))
 
 
+def _bytes_col(n: int, width: int, seed: int, maxlen: int = 3) -> np.ndarray:
+    """Padded byte strings of up to maxlen bytes from a small alphabet with
+    signed-char extremes (a byte-string key column of `width` bytes)."""
+    rng = np.random.default_rng(seed)
+    alphabet = np.array([0x00, 0x41, 0x7F, 0x80, 0xFF], dtype=np.uint8)
+    c = alphabet[rng.integers(0, alphabet.size, (n, width))]
+    c[:, maxlen:] = 0
+    return c
+
+
+WIDE = {
+    "char16+char16": lambda n, rng: [datagen.char16_keys(1, 0, n, distinct=6), datagen.char16_keys(2, 0, n, distinct=5)],
+    "bytes23+i32": lambda n, rng: [_bytes_col(n, 23, 3), rng.integers(-4, 4, n).astype(np.int32)],
+    "i64+char16+bytes5": lambda n, rng: [rng.integers(-2, 2, n).astype(np.int64),
+                                         datagen.char16_keys(4, 0, n, distinct=3), _bytes_col(n, 5, 5, 2)],
+    "bytes5+u32": lambda n, rng: [_bytes_col(n, 5, 6), rng.integers(0, 3, n).astype(np.uint32)],  # 9 bytes
+    "bytes300": lambda n, rng: [_bytes_col(n, 300, 7)],
+    "f64+bytes40+u64+char16": lambda n, rng: [np.where(rng.random(n) < 0.5, -0.0, 1.5),
+                                              _bytes_col(n, 40, 8, 2),
+                                              rng.integers(0, 2, n).astype(np.uint64) << np.uint64(40),
+                                              datagen.char16_keys(9, 0, n, distinct=2)],
+}
+
+
+@pytest.mark.parametrize("combo", list(WIDE))
+def test_groupby_multi_wide_keys_vs_oracle(vdf, combo):
+    """Multi-column keys of any width (byte-string columns and composites
+    over 16 bytes, sorted and grouped through the byte-string path): keys of
+    every column, offsets, counts, permutation and aggregates against the
+    oracle; the argsort in every order mode."""
+    rng = np.random.default_rng(13)
+    n = 150_007
+    keys = WIDE[combo](n, rng)
+    vals = [rng.standard_normal(n), rng.integers(-2**40, 2**40, n).astype(np.int64)]
+    _say("groupby_multi wide", combo, [k.shape for k in keys])
+    r = vdf.groupby_multi([_dev(k) for k in keys], [_dev(v) for v in vals])
+    o = oracle.groupby_multi(keys, vals, threads=8)
+    assert r.n_groups == o["n_groups"], combo
+    for c, k in enumerate(keys):
+        got = r.keys[c].cpu().numpy()
+        assert np.array_equal(got.view(np.uint8).reshape(got.shape[0], -1),
+                              o["keys"][c].view(np.uint8).reshape(got.shape[0], -1)), (combo, c)
+    assert np.array_equal(r.offsets.cpu().numpy().view(np.uint32).astype(np.uint64), o["offsets"]), combo
+    assert np.array_equal(r.counts.cpu().numpy().view(np.uint32).astype(np.uint64), o["counts"]), combo
+    assert np.array_equal(r.perm.cpu().numpy().view(np.uint32).astype(np.uint64), o["perm"]), combo
+    ra, oa = r.aggs[1], o["aggs"][1]
+    assert np.array_equal(ra["sum"].cpu().numpy().view(np.int64), oa["sum"].view(np.int64)), combo
+    assert np.array_equal(ra["min"].cpu().numpy(), oa["min"]), combo
+    got = r.aggs[0]["sum"].cpu().numpy()
+    assert np.all(np.abs(got - o["aggs"][0]["sum"]) <= F64_RTOL * np.abs(o["aggs"][0]["sum"])), combo
+    for m, kw in ((0, {}), (1, {"descending": True}), (2, {"reverse_desc": True})):
+        perm = vdf.argsort_multi([_dev(k) for k in keys], **kw)
+        assert np.array_equal(perm.cpu().numpy().view(np.uint32).astype(np.uint64),
+                              oracle.argsort_multi(keys, m)), (combo, m)
+
+
 def test_multi_key_too_wide_rejected(vdf):
     import torch
 
     k = torch.zeros(10, dtype=torch.int64, device="cuda")
-    c = torch.zeros((10, 16), dtype=torch.uint8, device="cuda")
+    c = torch.zeros((10, 4090), dtype=torch.uint8, device="cuda")
     with pytest.raises(N.VdfError) as e:
         vdf.groupby_multi([k, c])
     assert e.value.status == N.VDF_ERR_UNSUPPORTED

@@ -393,6 +393,42 @@ def test_oracle_hash_is_reference_hash_up_to_8_bytes():
 
 
 # ------------------------------------------------------ multi-column keys
+def _signed_bytes(row) -> tuple:
+    return tuple((int(b) ^ 0x80) for b in row)
+
+
+def test_groupby_multi_wide_keys_vs_brute_force():
+    """Key sets wider than 16 bytes (two char[16] columns, a 23-byte string
+    column, an int32): groups, counts and sums against a brute-force dict,
+    ordered by the columns' tuple (byte strings as signed chars), and the
+    argsort against Python's stable sort of those tuples."""
+    rng = np.random.default_rng(31)
+    n = 6000
+    a = datagen.char16_keys(1, 0, n, distinct=5)
+    b = datagen.char16_keys(2, 0, n, distinct=4)
+    alphabet = np.array([0x00, 0x41, 0x7F, 0x80, 0xFF], dtype=np.uint8)
+    c = alphabet[rng.integers(0, alphabet.size, (n, 23))]
+    c[:, 3:] = 0  # strings of up to 3 bytes, padded
+    d = rng.integers(-3, 3, n).astype(np.int32)
+    v = rng.integers(-10**9, 10**9, n).astype(np.int64)
+    keys = [a, b, c, d]
+    tup = [_signed_bytes(a[i]) + _signed_bytes(b[i]) + _signed_bytes(c[i]) + (int(d[i]),) for i in range(n)]
+    groups = {}
+    for i in range(n):
+        groups.setdefault(tup[i], []).append(int(v[i]))
+    order = sorted(groups)
+    r = oracle.groupby_multi(keys, [v], threads=3)
+    assert r["n_groups"] == len(order)
+    got = [_signed_bytes(r["keys"][0][g]) + _signed_bytes(r["keys"][1][g]) + _signed_bytes(r["keys"][2][g])
+           + (int(r["keys"][3][g]),) for g in range(r["n_groups"])]
+    assert got == order
+    assert r["counts"].tolist() == [len(groups[t]) for t in order]
+    assert r["aggs"][0]["sum"].view(np.int64).tolist() == [sum(groups[t]) for t in order]
+    perm = oracle.argsort_multi(keys, 0)
+    assert perm.tolist() == sorted(range(n), key=lambda i: tup[i])
+
+
+
 def test_groupby_multi_vs_pandas():
     import pandas as pd
 

@@ -23,6 +23,7 @@ VDF_ERR_TIMEOUT = 6
 
 # key types
 KEY_U32, KEY_I32, KEY_U64, KEY_I64, KEY_CHAR16, KEY_F32, KEY_F64 = 0, 1, 2, 3, 4, 5, 6
+KEY_BYTES = 7  # multi-column keys only: fixed-width byte strings
 # value types
 VAL_F64, VAL_I64, VAL_I32, VAL_U32, VAL_U64 = 0, 1, 2, 3, 4
 # argsort flags
@@ -88,7 +89,7 @@ class VdfServerStats(C.Structure):
 
 
 class VdfKeyCol(C.Structure):
-    _fields_ = [("type", C.c_int), ("data", C.c_void_p)]
+    _fields_ = [("type", C.c_int), ("data", C.c_void_p), ("width", C.c_uint32)]
 
 
 class VdfTuneReport(C.Structure):

@@ -534,19 +534,35 @@ vdf_status vdf_groupby_batch(vdf_handle* h, int key_type, const vdf_table* table
     });
 }
 
-static std::vector<int> key_types_of(const vdf_key_col* keys, int nkeys, uint64_t n, const char* who)
+// Key column types and cell widths of a multi-column key.
+struct KeyCols {
+    std::vector<int> types;
+    std::vector<uint32_t> widths;
+    std::vector<const void*> data;
+    uint32_t width = 0;  // composite bytes
+};
+
+static KeyCols key_cols_of(const vdf_key_col* keys, int nkeys, uint64_t n, const char* who)
 {
     require(keys != nullptr, "null key column list");
     require(nkeys >= 1 && nkeys <= VDF_MAX_KEY_COLS, "nkeys out of range (1 .. VDF_MAX_KEY_COLS)");
-    std::vector<int> types(static_cast<size_t>(nkeys));
+    KeyCols k;
     for (int c = 0; c < nkeys; ++c) {
-        require(key_bytes(keys[c].type) != 0, "unknown key type");
+        const int t = keys[c].type;
+        if (t == VDF_KEY_BYTES)
+            require(keys[c].width >= 1 && keys[c].width <= VDF_MAX_STRING_BYTES,
+                    "VDF_KEY_BYTES column width must be 1 .. VDF_MAX_STRING_BYTES");
+        else
+            require(key_bytes(t) != 0, "unknown key type");
         require(n == 0 || keys[c].data != nullptr, "null key column data");
-        types[c] = keys[c].type;
+        k.types.push_back(t);
+        k.widths.push_back(key_col_width(t, keys[c].width));
+        k.data.push_back(keys[c].data);
     }
-    if (composite_key_type(types.data(), nkeys) < 0)
-        throw Error{VDF_ERR_UNSUPPORTED, std::string(who) + ": key columns wider than 16 bytes together"};
-    return types;
+    k.width = composite_width(k.types.data(), k.widths.data(), nkeys);
+    if (k.width > VDF_MAX_STRING_BYTES)
+        throw Error{VDF_ERR_UNSUPPORTED, std::string(who) + ": key columns wider than VDF_MAX_STRING_BYTES together"};
+    return k;
 }
 
 vdf_status vdf_groupby_multi(vdf_handle* h, const vdf_key_col* keys, int nkeys, uint64_t n,
@@ -555,7 +571,7 @@ vdf_status vdf_groupby_multi(vdf_handle* h, const vdf_key_col* keys, int nkeys, 
 {
     return guarded([&] {
         require(h != nullptr && out != nullptr, "vdf_groupby_multi: null handle or output");
-        const std::vector<int> types = key_types_of(keys, nkeys, n, "vdf_groupby_multi");
+        const KeyCols kc = key_cols_of(keys, nkeys, n, "vdf_groupby_multi");
         require(nvals >= 0 && nvals <= VDF_MAX_VALUE_COLS, "vdf_groupby_multi: nvals out of range");
         require(nvals == 0 || vals != nullptr, "vdf_groupby_multi: null value columns");
         require((agg_mask & ~VDF_AGG_ALL) == 0, "vdf_groupby_multi: unknown aggregate bits");
@@ -572,30 +588,45 @@ vdf_status vdf_groupby_multi(vdf_handle* h, const vdf_key_col* keys, int nkeys, 
         }
         CallScope cs(h);
         const hipStream_t s = static_cast<hipStream_t>(stream);
-        const int kt = composite_key_type(types.data(), nkeys);
-        const size_t cb = key_bytes(kt);
+        const int kt = composite_key_type(kc.width);
+        const size_t cb = kt >= 0 ? key_bytes(kt) : kc.width;  // composite bytes per row
         const uint64_t cap = (out->capacity == 0 || out->capacity > n) ? n : out->capacity;
         const uint64_t hr = static_cast<uint64_t>(h->tune.hist_rows);
-        const size_t gb = groupby_batch_scratch(kt, &n, 1, hr, groupby_carry_bytes(h->tune, kt, vts.data(), nvals));
+        if (kt < 0 && n == 0) {  // zero groups through the aggregate stage
+            void* scr = h->ensure_scratch(groupby_agg_scratch(&n, 1));
+            h->staging_begin(groupby_agg_staging(&n, 1));
+            StagingScope st(h, s);
+            const uint32_t* perms[1] = {nullptr};
+            vdf_groupby_out o = *out;
+            o.group_keys = nullptr;
+            groupby_aggregate_enqueue(h, VDF_KEY_U32, nullptr, perms, &t, 1, vts.data(), nvals, agg_mask, &o, scr, s);
+            return;
+        }
+        const size_t gb = kt >= 0 ? groupby_batch_scratch(kt, &n, 1, hr,
+                                                          groupby_carry_bytes(h->tune, kt, vts.data(), nvals))
+                                  : bytes_groupby_scratch(kc.width, n, hr);
         void* scr = h->ensure_scratch(align_up(n * cb, 256) + align_up(std::max<uint64_t>(cap, 1) * cb, 256) + gb +
                                       512);
         Carve cv(scr);
         void* comp = cv.take<char>(n * cb);
         void* gkeys = cv.take<char>(std::max<uint64_t>(cap, 1) * cb);
         void* gscr = cv.take<char>(gb);
-        std::vector<const void*> data(static_cast<size_t>(nkeys));
-        for (int c = 0; c < nkeys; ++c) data[c] = keys[c].data;
-        pack_keys_enqueue(h, types.data(), data.data(), nkeys, n, comp, s);
+        pack_keys_enqueue(h, kc.types.data(), kc.widths.data(), kc.data.data(), nkeys, n, comp, s);
         t.keys = comp;
         t.n = n;
         vdf_groupby_out o = *out;
         o.group_keys = gkeys;
-        h->staging_begin(groupby_batch_staging(kt, &n, 1, hr));
-        {
+        if (kt >= 0) {
+            h->staging_begin(groupby_batch_staging(kt, &n, 1, hr));
             StagingScope st(h, s);
             groupby_batch_enqueue(h, kt, &t, 1, vts.data(), nvals, agg_mask, &o, gscr, s);
+        } else {  // a wide composite: the byte-string group-by over its rows
+            h->staging_begin(bytes_groupby_staging(kc.width, n, hr));
+            StagingScope st(h, s);
+            bytes_groupby_enqueue(h, comp, kc.width, n, t, vts.data(), nvals, agg_mask, &o, gscr, s);
         }
-        if (group_keys) unpack_keys_enqueue(h, types.data(), group_keys, nkeys, gkeys, out->n_groups, cap, s);
+        if (group_keys)
+            unpack_keys_enqueue(h, kc.types.data(), kc.widths.data(), group_keys, nkeys, gkeys, out->n_groups, cap, s);
     });
 }
 
@@ -604,7 +635,7 @@ vdf_status vdf_argsort_multi(vdf_handle* h, const vdf_key_col* keys, int nkeys, 
 {
     return guarded([&] {
         require(h != nullptr, "vdf_argsort_multi: null handle");
-        const std::vector<int> types = key_types_of(keys, nkeys, n, "vdf_argsort_multi");
+        const KeyCols kc = key_cols_of(keys, nkeys, n, "vdf_argsort_multi");
         require((flags & ~(VDF_SORT_DESC | VDF_SORT_DESC_REVERSE | VDF_SORT_PERM64)) == 0,
                 "vdf_argsort_multi: unknown flag bits");
         require(!((flags & VDF_SORT_DESC) && (flags & VDF_SORT_DESC_REVERSE)),
@@ -614,20 +645,24 @@ vdf_status vdf_argsort_multi(vdf_handle* h, const vdf_key_col* keys, int nkeys, 
         require(d_perm_out != nullptr, "vdf_argsort_multi: null permutation output");
         CallScope cs(h);
         const hipStream_t s = static_cast<hipStream_t>(stream);
-        const int kt = composite_key_type(types.data(), nkeys);
-        const size_t cb = key_bytes(kt);
+        const int kt = composite_key_type(kc.width);
+        const size_t cb = kt >= 0 ? key_bytes(kt) : kc.width;
         const uint64_t hr = static_cast<uint64_t>(h->tune.hist_rows);
-        const size_t as = argsort_batch_scratch(kt, &n, 1, hr);
+        const size_t as = kt >= 0 ? argsort_batch_scratch(kt, &n, 1, hr) : bytes_argsort_scratch(n, hr);
         void* scr = h->ensure_scratch(align_up(n * cb, 256) + as + 512);
         Carve cv(scr);
         void* comp = cv.take<char>(n * cb);
         void* ascr = cv.take<char>(as);
-        std::vector<const void*> data(static_cast<size_t>(nkeys));
-        for (int c = 0; c < nkeys; ++c) data[c] = keys[c].data;
-        pack_keys_enqueue(h, types.data(), data.data(), nkeys, n, comp, s);
-        h->staging_begin(argsort_batch_staging(kt, &n, 1, hr));
-        StagingScope st(h, s);
-        argsort_enqueue(h, kt, comp, n, flags, d_perm_out, nullptr, ascr, s);
+        pack_keys_enqueue(h, kc.types.data(), kc.widths.data(), kc.data.data(), nkeys, n, comp, s);
+        if (kt >= 0) {
+            h->staging_begin(argsort_batch_staging(kt, &n, 1, hr));
+            StagingScope st(h, s);
+            argsort_enqueue(h, kt, comp, n, flags, d_perm_out, nullptr, ascr, s);
+        } else {
+            h->staging_begin(bytes_argsort_staging(kc.width, n, hr));
+            StagingScope st(h, s);
+            bytes_argsort_enqueue(h, comp, kc.width, n, flags, d_perm_out, nullptr, ascr, s);
+        }
     });
 }
 

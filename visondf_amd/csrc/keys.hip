@@ -15,7 +15,12 @@
 //   total width W <= 4: a u32 composite, W <= 8: a u64 composite (numeric
 //   order = byte order), W <= 16: a 16-byte composite on the char[16] path,
 //   whose load maps each byte b to b ^ 0x80, so the composite stores the
-//   encoded bytes ^ 0x80 (padding 0x80, which the sort's plan skips).
+//   encoded bytes ^ 0x80 (padding 0x80, which the sort's plan skips);
+//   W > 16: a W-byte row for the byte-string sort and group-by, which order
+//   rows by each byte ^ 0x80 as well, so the row stores the encoded bytes
+//   ^ 0x80 too.
+// Byte-string columns (VDF_KEY_BYTES, padded std::string cells) encode as
+// char[16] does, byte by byte with ^ 0x80 (sort_string.inl:10-85 order).
 // Group keys come back as composites and are unpacked per column.
 #include "vdf_internal.h"
 
@@ -35,8 +40,10 @@ struct ColDesc {
 struct PackArgs {
     ColDesc cols[VDF_MAX_KEY_COLS];
     int ncols;
-    int comp;  // composite bytes: 4, 8 or 16
+    int comp;  // composite bytes: 4, 8 or 16, or the row width W (> 16) of a wide composite
 };
+
+__device__ inline bool bytewise(int type) { return type == VDF_KEY_CHAR16 || type == VDF_KEY_BYTES; }
 
 // order-preserving unsigned form of one cell, right-aligned in a u64
 // (char[16] columns are handled byte-wise)
@@ -80,9 +87,9 @@ __device__ inline void encode_row(const PackArgs& a, uint64_t r, uint8_t* b)
     int pos = 0;
     for (int c = 0; c < a.ncols; ++c) {
         const ColDesc& cd = a.cols[c];
-        if (cd.type == VDF_KEY_CHAR16) {
-            const uint8_t* s = static_cast<const uint8_t*>(cd.data) + 16 * r;
-            for (int i = 0; i < 16; ++i) b[pos + i] = s[i] ^ 0x80u;
+        if (bytewise(cd.type)) {
+            const uint8_t* s = static_cast<const uint8_t*>(cd.data) + static_cast<uint64_t>(cd.width) * r;
+            for (int i = 0; i < cd.width; ++i) b[pos + i] = s[i] ^ 0x80u;
         } else {
             const uint64_t e = enc_scalar(cd.type, cd.data, r);
             for (int i = 0; i < cd.width; ++i) b[pos + i] = static_cast<uint8_t>(e >> (8 * (cd.width - 1 - i)));
@@ -137,9 +144,9 @@ __global__ __launch_bounds__(256) void k_unpack(PackArgs a, const void* comp, co
     for (int c = 0; c < a.ncols; ++c) {
         const ColDesc& cd = a.cols[c];
         if (cd.out) {
-            if (cd.type == VDF_KEY_CHAR16) {
-                uint8_t* d = static_cast<uint8_t*>(cd.out) + 16 * g;
-                for (int i = 0; i < 16; ++i) d[i] = b[pos + i] ^ 0x80u;
+            if (bytewise(cd.type)) {
+                uint8_t* d = static_cast<uint8_t*>(cd.out) + static_cast<uint64_t>(cd.width) * g;
+                for (int i = 0; i < cd.width; ++i) d[i] = b[pos + i] ^ 0x80u;
             } else {
                 uint64_t e = 0;
                 for (int i = 0; i < cd.width; ++i) e = (e << 8) | b[pos + i];
@@ -150,44 +157,113 @@ __global__ __launch_bounds__(256) void k_unpack(PackArgs a, const void* comp, co
     }
 }
 
+// Wide composite (W > 16 bytes): row r's encoded bytes ^ 0x80, one thread per
+// row, written byte by byte (the byte-string sort maps each back with ^ 0x80).
+__global__ __launch_bounds__(256) void k_pack_wide(PackArgs a, uint64_t n, uint8_t* comp)
+{
+    const uint64_t r = static_cast<uint64_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (r >= n) return;
+    uint8_t* o = comp + static_cast<uint64_t>(a.comp) * r;
+    int pos = 0;
+    for (int c = 0; c < a.ncols; ++c) {
+        const ColDesc& cd = a.cols[c];
+        if (bytewise(cd.type)) {
+            const uint8_t* s = static_cast<const uint8_t*>(cd.data) + static_cast<uint64_t>(cd.width) * r;
+            for (int i = 0; i < cd.width; ++i) o[pos + i] = s[i];  // (s ^ 0x80) ^ 0x80
+        } else {
+            const uint64_t e = enc_scalar(cd.type, cd.data, r);
+            for (int i = 0; i < cd.width; ++i)
+                o[pos + i] = static_cast<uint8_t>(e >> (8 * (cd.width - 1 - i))) ^ 0x80u;
+        }
+        pos += cd.width;
+    }
+}
+
+// Group g's wide composite row -> each column's group key.
+__global__ __launch_bounds__(256) void k_unpack_wide(PackArgs a, const uint8_t* comp, const uint64_t* n_groups,
+                                                     uint64_t cap)
+{
+    const uint64_t g = static_cast<uint64_t>(blockIdx.x) * 256 + threadIdx.x;
+    const uint64_t G = std::min<uint64_t>(*n_groups, cap);
+    if (g >= G) return;
+    const uint8_t* row = comp + static_cast<uint64_t>(a.comp) * g;
+    int pos = 0;
+    for (int c = 0; c < a.ncols; ++c) {
+        const ColDesc& cd = a.cols[c];
+        if (cd.out) {
+            if (bytewise(cd.type)) {
+                uint8_t* d = static_cast<uint8_t*>(cd.out) + static_cast<uint64_t>(cd.width) * g;
+                for (int i = 0; i < cd.width; ++i) d[i] = row[pos + i];
+            } else {
+                uint64_t e = 0;
+                for (int i = 0; i < cd.width; ++i) e = (e << 8) | static_cast<uint8_t>(row[pos + i] ^ 0x80u);
+                dec_scalar(cd.type, e, cd.out, g);
+            }
+        }
+        pos += cd.width;
+    }
+}
+
+PackArgs pack_args(const int* types, const uint32_t* widths, int ncols)
+{
+    PackArgs a{};
+    a.ncols = ncols;
+    const uint32_t w = composite_width(types, widths, ncols);
+    const int kt = composite_key_type(w);
+    a.comp = kt >= 0 ? static_cast<int>(key_bytes(kt)) : static_cast<int>(w);
+    for (int c = 0; c < ncols; ++c)
+        a.cols[c] = ColDesc{types[c], static_cast<int>(key_col_width(types[c], widths ? widths[c] : 0)), nullptr,
+                            nullptr};
+    return a;
+}
+
 }  // namespace
 
-int composite_key_type(const int* types, int ncols)
+uint32_t key_col_width(int type, uint32_t width)
 {
-    size_t w = 0;
-    for (int c = 0; c < ncols; ++c) w += key_bytes(types[c]);
+    return type == VDF_KEY_BYTES ? width : static_cast<uint32_t>(key_bytes(type));
+}
+
+uint32_t composite_width(const int* types, const uint32_t* widths, int ncols)
+{
+    uint32_t w = 0;
+    for (int c = 0; c < ncols; ++c) w += key_col_width(types[c], widths ? widths[c] : 0);
+    return w;
+}
+
+int composite_key_type(uint32_t w)
+{
     return w <= 4 ? VDF_KEY_U32 : (w <= 8 ? VDF_KEY_U64 : (w <= 16 ? VDF_KEY_CHAR16 : -1));
 }
 
-void pack_keys_enqueue(Handle* h, const int* types, const void* const* data, int ncols, uint64_t n, void* comp,
-                       hipStream_t s)
+void pack_keys_enqueue(Handle* h, const int* types, const uint32_t* widths, const void* const* data, int ncols,
+                       uint64_t n, void* comp, hipStream_t s)
 {
-    PackArgs a{};
-    a.ncols = ncols;
-    a.comp = static_cast<int>(key_bytes(composite_key_type(types, ncols)));
+    PackArgs a = pack_args(types, widths, ncols);
     uint64_t in_bytes = 0;
     for (int c = 0; c < ncols; ++c) {
-        a.cols[c] = ColDesc{types[c], static_cast<int>(key_bytes(types[c])), data[c], nullptr};
-        in_bytes += key_bytes(types[c]);
+        a.cols[c].data = data[c];
+        in_bytes += static_cast<uint64_t>(a.cols[c].width);
     }
     if (n == 0) return;
-    LaunchTimer t(h, s, "pack_keys", n * (in_bytes + a.comp));
-    hipLaunchKernelGGL(k_pack, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, s, a, n, comp);
+    LaunchTimer t(h, s, "pack_keys", n * (in_bytes + static_cast<uint64_t>(a.comp)));
+    const dim3 grid(static_cast<unsigned>((n + 255) / 256));
+    if (a.comp > 16) hipLaunchKernelGGL(k_pack_wide, grid, dim3(256), 0, s, a, n, static_cast<uint8_t*>(comp));
+    else hipLaunchKernelGGL(k_pack, grid, dim3(256), 0, s, a, n, comp);
     check_launch("k_pack");
 }
 
-void unpack_keys_enqueue(Handle* h, const int* types, void* const* outs, int ncols, const void* comp,
-                         const uint64_t* n_groups, uint64_t cap, hipStream_t s)
+void unpack_keys_enqueue(Handle* h, const int* types, const uint32_t* widths, void* const* outs, int ncols,
+                         const void* comp, const uint64_t* n_groups, uint64_t cap, hipStream_t s)
 {
-    PackArgs a{};
-    a.ncols = ncols;
-    a.comp = static_cast<int>(key_bytes(composite_key_type(types, ncols)));
-    for (int c = 0; c < ncols; ++c)
-        a.cols[c] = ColDesc{types[c], static_cast<int>(key_bytes(types[c])), nullptr, outs ? outs[c] : nullptr};
+    PackArgs a = pack_args(types, widths, ncols);
+    for (int c = 0; c < ncols; ++c) a.cols[c].out = outs ? outs[c] : nullptr;
     if (cap == 0) return;
     LaunchTimer t(h, s, "unpack_keys", 0);
-    hipLaunchKernelGGL(k_unpack, dim3(static_cast<unsigned>((cap + 255) / 256)), dim3(256), 0, s, a, comp, n_groups,
-                       cap);
+    const dim3 grid(static_cast<unsigned>((cap + 255) / 256));
+    if (a.comp > 16)
+        hipLaunchKernelGGL(k_unpack_wide, grid, dim3(256), 0, s, a, static_cast<const uint8_t*>(comp), n_groups, cap);
+    else hipLaunchKernelGGL(k_unpack, grid, dim3(256), 0, s, a, comp, n_groups, cap);
     check_launch("k_unpack");
 }
 

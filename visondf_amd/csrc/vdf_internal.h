@@ -319,9 +319,13 @@ void groupby_aggregate_enqueue(Handle* h, int key_type, const void* skeys, const
                                const vdf_groupby_out* outs, void* scr, hipStream_t s, const void* spay = nullptr,
                                int pay_bytes = 0);
 
-// multi-column keys: composite key type (-1 when wider than 16 bytes), pack
-// the columns into composites, unpack composite group keys into the columns
-int composite_key_type(const int* types, int ncols);
+// multi-column keys: a column's bytes per cell (VDF_KEY_BYTES: its width),
+// the composite's width, the composite key type of a width (-1 when wider
+// than 16 bytes: a byte-string row of that width), pack the columns into
+// composites, unpack composite group keys into the columns
+uint32_t key_col_width(int type, uint32_t width);
+uint32_t composite_width(const int* types, const uint32_t* widths, int ncols);
+int composite_key_type(uint32_t width);
 // Exclusive scan of one value per thread over a 256-thread block.
 __device__ inline uint32_t block_excl_scan(uint32_t x, uint32_t* s_wsum)
 {
@@ -353,10 +357,10 @@ void bytes_groupby_enqueue(Handle* h, const void* d_keys, uint32_t width, uint64
                            const int* val_types, int nvals, uint32_t agg_mask, const vdf_groupby_out* out, void* scr,
                            hipStream_t s);
 
-void pack_keys_enqueue(Handle* h, const int* types, const void* const* data, int ncols, uint64_t n, void* comp,
-                       hipStream_t s);
-void unpack_keys_enqueue(Handle* h, const int* types, void* const* outs, int ncols, const void* comp,
-                         const uint64_t* n_groups, uint64_t cap, hipStream_t s);
+void pack_keys_enqueue(Handle* h, const int* types, const uint32_t* widths, const void* const* data, int ncols,
+                       uint64_t n, void* comp, hipStream_t s);
+void unpack_keys_enqueue(Handle* h, const int* types, const uint32_t* widths, void* const* outs, int ncols,
+                         const void* comp, const uint64_t* n_groups, uint64_t cap, hipStream_t s);
 
 // fill kernel (graph-safe replacement for hipMemsetAsync)
 void fill_u32(uint32_t* p, size_t count, uint32_t value, hipStream_t s);

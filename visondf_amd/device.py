@@ -67,6 +67,15 @@ def key_type_of(t: torch.Tensor) -> int:
     raise TypeError(f"unsupported key dtype {t.dtype}")
 
 
+def key_col_of(t: torch.Tensor) -> tuple[int, int]:
+    """(key type, cell width) of one column of a multi-column key: a 2-D
+    uint8 tensor of width other than 16 is a fixed-width byte-string column."""
+    if t.dtype == torch.uint8 and t.dim() == 2 and t.shape[1] != 16:
+        return N.KEY_BYTES, int(t.shape[1])
+    kt = key_type_of(t)
+    return kt, _KEY_DTYPES[kt][1]
+
+
 def val_type_of(t: torch.Tensor) -> int:
     for k, dt in _VAL_DTYPES.items():
         if t.dtype == dt:
@@ -281,11 +290,14 @@ class Vdf:
 
     def groupby_multi(self, keys, vals=(), aggs: int = N.AGG_ALL, want_perm: bool = True, stream=None,
                       trim: bool = True, key_types=None, val_types=None, capacity: int = 0):
-        """Group by several key columns (widths summing to <= 16 bytes); the
-        result's keys is a list with one tensor of group keys per column."""
+        """Group by several key columns (any key type; a (n, w) uint8 column
+        with w != 16 is a byte-string column); the result's keys is a list
+        with one tensor of group keys per column."""
         keys = list(keys)
         n = keys[0].shape[0]
-        kts = [key_type_of(k) if key_types is None else key_types[c] for c, k in enumerate(keys)]
+        kcs = [key_col_of(k) if key_types is None else (key_types[c], int(k.shape[1]) if k.dim() == 2 else 0)
+               for c, k in enumerate(keys)]
+        kts = [t for t, _ in kcs]
         vts = [val_type_of(v) if val_types is None else val_types[c] for c, v in enumerate(vals)]
         out, t = self._alloc_out(keys[0], vts, [v.dtype for v in vals], aggs, want_perm, capacity)
         cap = t["cap"]
@@ -294,6 +306,7 @@ class Vdf:
         for c, k in enumerate(keys):
             kc[c].type = kts[c]
             kc[c].data = _ptr(k)
+            kc[c].width = kcs[c][1] if kts[c] == N.KEY_BYTES else 0
         cols = (N.VdfValueCol * max(len(vals), 1))()
         for c, v in enumerate(vals):
             cols[c].type = vts[c]
@@ -315,14 +328,16 @@ class Vdf:
                       perm64: bool = False, stream=None) -> torch.Tensor:
         keys = list(keys)
         n = keys[0].shape[0]
-        kts = [key_type_of(k) if key_types is None else key_types[c] for c, k in enumerate(keys)]
+        kcs = [key_col_of(k) if key_types is None else (key_types[c], int(k.shape[1]) if k.dim() == 2 else 0)
+               for c, k in enumerate(keys)]
         flags = (N.SORT_DESC if descending else 0) | (N.SORT_DESC_REVERSE if reverse_desc else 0) \
             | (N.SORT_PERM64 if perm64 else 0)
         perm = torch.empty(n, dtype=torch.int64 if perm64 else torch.int32, device=keys[0].device)
         kc = (N.VdfKeyCol * len(keys))()
         for c, k in enumerate(keys):
-            kc[c].type = kts[c]
+            kc[c].type = kcs[c][0]
             kc[c].data = _ptr(k)
+            kc[c].width = kcs[c][1] if kcs[c][0] == N.KEY_BYTES else 0
         N.check(self.lib.vdf_argsort_multi(self.h, kc, len(keys), n, flags, _ptr(perm),
                                            C.c_void_p(_stream_ptr(stream))))
         return perm
