@@ -324,7 +324,8 @@ __device__ inline void write_final(const GbArgs& a, const ColOut& o, int c, uint
 // Finished groups of a tile are staged in LDS by tile-local group index and
 // written out afterwards by consecutive lanes, so each output array takes one
 // store instruction per 256 groups instead of one per row position that holds
-// a head somewhere in the wave.  Groups past kGbStage are written directly.
+// a head somewhere in the wave.  Tiles with more than kGbStage groups stage
+// them in windows of kGbStage ids, one after the other.
 constexpr int kGbStage = 1024;
 
 struct GbStage {
@@ -347,18 +348,17 @@ __device__ inline K staged_key(const GbStage* st, uint32_t h)
     else return static_cast<K>(st->a[h]);
 }
 
-__device__ inline void stage_final(GbStage* st, const GbArgs& a, const ColOut& o, int c, uint32_t gbase, uint32_t gid,
-                                   const Agg& s)
+// Stages group gid when it falls in the window of kGbStage ids from wbase
+// (groups outside the window are staged when their window's turn comes).
+__device__ inline void stage_final(GbStage* st, uint32_t wbase, uint32_t gid, const Agg& s)
 {
-    const uint32_t h = gid - gbase;
+    const uint32_t h = gid - wbase;  // groups before the window wrap to large values
     if (h < static_cast<uint32_t>(kGbStage)) {
         st->cnt[h] = static_cast<uint32_t>(s.cnt);  // a group inside one tile: at most 4096 rows
         st->a[h] = s.a;
         st->b[h] = s.b;
         st->mn[h] = s.mn;
         st->mx[h] = s.mx;
-    } else {
-        write_final(a, o, c, gid, s);
     }
 }
 
@@ -521,6 +521,7 @@ __device__ __attribute__((always_inline)) inline void gb_column(const GbArgs& a,
     const uint64_t r0 = tbase + static_cast<uint64_t>(tid) * kGbRows;
     const ColOut co = col_out(a, sg, c);
     uint64_t v[kGbRows];
+    auto load_vals = [&]() {
     if (KIND != KIND_COUNT) {
         // A gathered column: all 16 gathers are issued before any conversion
         // waits on one.  Rows past the tile's end hold row id 0 (in bounds:
@@ -568,25 +569,38 @@ __device__ __attribute__((always_inline)) inline void gb_column(const GbArgs& a,
             for (int j = 0; j < kGbRows; ++j) v[j] = q[p[j]];
         }
     }
+    };
+    load_vals();
+    // Groups finishing in the tile (0 .. H-2) are staged in windows of
+    // kGbStage tile-local ids: window 0 in the first run of the row loop, each
+    // further window (tiles with more groups, e.g. keys nearly all distinct)
+    // by running the row loop again over the same values, staging only that
+    // window's groups, so every group is written by consecutive lanes.
     Agg cur = agg_empty(), first = agg_empty();
     bool seen = false;
-    uint32_t g = gfirst;  // id of the next head's group
+    auto row_loop = [&](uint32_t lo) {
+        cur = agg_empty();
+        first = agg_empty();
+        seen = false;
+        uint32_t g = gfirst;  // id of the next head's group
 #pragma unroll
-    for (int j = 0; j < kGbRows; ++j) {
-        if (static_cast<uint32_t>(j) < cnt) {
-            if ((hb >> j) & 1u) {
-                if (!seen) {
-                    first = cur;
-                    seen = true;
-                } else {
-                    stage_final(st, a, co, c, gbase, g - 1, cur);
+        for (int j = 0; j < kGbRows; ++j) {
+            if (static_cast<uint32_t>(j) < cnt) {
+                if ((hb >> j) & 1u) {
+                    if (!seen) {
+                        first = cur;
+                        seen = true;
+                    } else {
+                        stage_final(st, gbase + lo, g - 1, cur);
+                    }
+                    cur = agg_empty();
+                    ++g;
                 }
-                cur = agg_empty();
-                ++g;
+                agg_add(cur, KIND == KIND_COUNT ? 0ull : v[j], KIND);
             }
-            agg_add(cur, KIND == KIND_COUNT ? 0ull : v[j], KIND);
         }
-    }
+    };
+    row_loop(0);
 
     // inclusive segmented scan over the wave's lanes, by DPP: a Hillis-Steele
     // scan inside each 16-lane row (shifts 1, 2, 4, 8), then row 0's total into
@@ -647,9 +661,12 @@ __device__ __attribute__((always_inline)) inline void gb_column(const GbArgs& a,
     }
     Agg* pre = a.prefix + static_cast<uint64_t>(c) * a.ntiles;
     Agg* suf = a.suffix + static_cast<uint64_t>(c) * a.ntiles;
+    // the group ending at this lane's first head (carry from earlier lanes)
+    Agg seg = agg_empty();
+    const bool seg_fin = seen && cf;
     if (seen) {
-        const Agg seg = agg_join(carry, first, KIND);
-        if (cf) stage_final(st, a, co, c, gbase, gfirst - 1, seg);
+        seg = agg_join(carry, first, KIND);
+        if (cf) stage_final(st, gbase, gfirst - 1, seg);
         else pre[tile] = seg;  // the tile's first head: rows before it
     }
     if (tid == kBlock - 1) {
@@ -662,14 +679,23 @@ __device__ __attribute__((always_inline)) inline void gb_column(const GbArgs& a,
         if (bf) suf[tile] = bx;
         else pre[tile] = bx;
     }
-    __syncthreads();  // every staged group is in LDS
+    __syncthreads();  // every staged group of window 0 is in LDS
     // groups 0 .. H-2 finish in this tile (group H-1 runs into the next tiles)
-    const uint32_t nfin = H ? std::min<uint32_t>(H - 1, kGbStage) : 0u;
-    for (uint32_t h = tid; h < nfin; h += kBlock) {
-        const Agg g{st->cnt[h], st->a[h], st->b[h], st->mn[h], st->mx[h]};
-        write_final(a, co, c, gbase + h, g);
+    const uint32_t nfin = H ? H - 1 : 0u;
+    for (uint32_t lo = 0;;) {
+        const uint32_t nw = std::min<uint32_t>(nfin - lo, kGbStage);
+        for (uint32_t h = tid; h < nw; h += kBlock) {
+            const Agg g{st->cnt[h], st->a[h], st->b[h], st->mn[h], st->mx[h]};
+            write_final(a, co, c, gbase + lo + h, g);
+        }
+        __syncthreads();  // s_wagg and the stage are reused (next window, next column)
+        lo += kGbStage;
+        if (lo >= nfin) break;  // block-uniform: H is the tile's
+        load_vals();
+        row_loop(lo);
+        if (seg_fin) stage_final(st, gbase + lo, gfirst - 1, seg);
+        __syncthreads();
     }
-    __syncthreads();  // s_wagg and the stage are reused by the next column
 }
 
 // What a block keeps of one claimed tile.
@@ -819,44 +845,47 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                 }
             }
             // offsets and keys of the tile's groups go through the LDS stage
-            // (tile-local index h = g - gbase) and out by consecutive lanes
+            // (tile-local index h = g - gbase, in windows of kGbStage ids when
+            // the tile has more groups) and out by consecutive lanes
             K* gkeys = static_cast<K*>(sg.out.group_keys);
-            uint32_t bits = t.hb;
-            uint32_t g = gfirst;
-            while (bits) {
-                const int k = __ffs(bits) - 1;
-                bits &= bits - 1;
-                const uint64_t r = t.r0 + k;
-                K kk;
-                if (gkeys) {
-                    if constexpr (sizeof(K) == 4) {
-                        kk = kw[0];
+            const uint32_t nheads = static_cast<uint32_t>(__popc(t.hb));
+            for (uint32_t lo = 0; lo < t.H; lo += kGbStage) {  // block-uniform
+                const uint32_t h0 = gfirst - t.gbase;  // this lane's first head's tile-local id
+                if (nheads && h0 + nheads > lo && h0 < lo + kGbStage) {
+                    uint32_t bits = t.hb;
+                    uint32_t g = gfirst;
+                    while (bits) {
+                        const int k = __ffs(bits) - 1;
+                        bits &= bits - 1;
+                        const uint32_t h = g - t.gbase - lo;
+                        ++g;
+                        if (h >= static_cast<uint32_t>(kGbStage)) continue;  // another window's
+                        const uint64_t r = t.r0 + k;
+                        s_stage.cnt[h] = static_cast<uint32_t>(r);
+                        if (gkeys) {
+                            K kk;
+                            if constexpr (sizeof(K) == 4) {
+                                kk = kw[0];
 #pragma unroll
-                        for (int j = 1; j < kGbRows; ++j) kk = (k == j) ? kw[j] : kk;
-                    } else {
-                        kk = get_key(keys, r);
+                                for (int j = 1; j < kGbRows; ++j) kk = (k == j) ? kw[j] : kk;
+                            } else {
+                                kk = get_key(keys, r);
+                            }
+                            stage_key(&s_stage, h, kk);
+                        }
                     }
                 }
-                const uint32_t h = g - t.gbase;
-                if (h < static_cast<uint32_t>(kGbStage)) {
-                    s_stage.cnt[h] = static_cast<uint32_t>(r);
-                    if (gkeys) stage_key(&s_stage, h, kk);
-                } else if (g < sg.cap) {
-                    gmem(sg.out.offsets)[g] = static_cast<uint32_t>(r);
-                    if (gkeys) put_key(gkeys, g, kk);
+                __syncthreads();
+                const uint32_t nst = std::min<uint32_t>(t.H - lo, kGbStage);
+                for (uint32_t h = tid; h < nst; h += kBlock) {
+                    const uint32_t gg = t.gbase + lo + h;
+                    if (gg < sg.cap) {
+                        gmem(sg.out.offsets)[gg] = s_stage.cnt[h];
+                        if (gkeys) put_key(gkeys, gg, staged_key<K>(&s_stage, h));
+                    }
                 }
-                ++g;
+                __syncthreads();  // the stage is reused (next window, the columns)
             }
-            __syncthreads();
-            const uint32_t nst = std::min<uint32_t>(t.H, kGbStage);
-            for (uint32_t h = tid; h < nst; h += kBlock) {
-                const uint32_t gg = t.gbase + h;
-                if (gg < sg.cap) {
-                    gmem(sg.out.offsets)[gg] = s_stage.cnt[h];
-                    if (gkeys) put_key(gkeys, gg, staged_key<K>(&s_stage, h));
-                }
-            }
-            __syncthreads();  // the stage is reused by the columns
         }
         for (int c = 0; c < a.ncols; ++c) {
             switch (a.cols[c].kind) {
