@@ -131,6 +131,17 @@ def _roofline(workload: str, kernels: dict, name: str | None = None) -> dict:
             "frac": k["gbps"] / HBM_PEAK_GBPS, "traffic": _load_traffic(workload, name)}
 
 
+def _handle(args):
+    """A library handle on device 0 with the --tune settings applied."""
+    from visondf_amd.device import Vdf
+
+    v = Vdf(0)
+    for item in args.tune:
+        name, _, val = item.partition("=")
+        v.tune(name, int(val))
+    return v
+
+
 def _time_steps(step, steps: int, warmup: int, v=None):
     """Times `steps` steps between torch.cuda.synchronize() calls (no per-launch
     events in the timed region); with a handle, then runs the same number of
@@ -229,7 +240,7 @@ def run_argsort_u32(args) -> dict:
     d_val = torch.from_numpy(val).cuda()
     perm = torch.empty(n, dtype=torch.int32, device="cuda")
     out = torch.empty(n, dtype=torch.float64, device="cuda")
-    v = Vdf(0)
+    v = _handle(args)
     tune = None
     if not args.no_tune:
         _log("argsort_u32: autotuning the launch shape")
@@ -358,7 +369,7 @@ def run_groupby_i32(args, full_range: bool = False) -> dict:
     d_key = torch.from_numpy(key).cuda()
     d_vf = torch.from_numpy(vf).cuda()
     d_vi = torch.from_numpy(vi).cuda()
-    v = Vdf(0)
+    v = _handle(args)
     res = {}
 
     def step():
@@ -411,7 +422,7 @@ def run_groupby_c1(args) -> dict:
     d_key = torch.from_numpy(key).cuda()
     d_vf = torch.from_numpy(vf).cuda()
     d_vi = torch.from_numpy(vi).cuda()
-    v = Vdf(0)
+    v = _handle(args)
     res = {}
 
     def step():
@@ -453,7 +464,7 @@ def run_zipf_groupby(args) -> dict:
     d_key = torch.from_numpy(key).cuda()
     d_vf = torch.from_numpy(vf).cuda()
     d_vi = torch.from_numpy(vi).cuda()
-    v = Vdf(0)
+    v = _handle(args)
     res = {}
 
     def step():
@@ -506,7 +517,7 @@ def run_batched(args) -> dict:
         vals[offs[i]:offs[i + 1]] = x
     d_keys = torch.from_numpy(keys).cuda()
     d_vals = torch.from_numpy(vals).cuda()
-    v = Vdf(0)
+    v = _handle(args)
     caps = [max(int(n) // 8, 1) for n in sizes]  # keys are drawn from [0, rows/8)
     tables = [(d_keys[offs[i]:offs[i + 1]], [d_vals[offs[i]:offs[i + 1]]]) for i in range(ntab)]
     # descriptors and outputs built once: the timed region is the library call
@@ -746,6 +757,9 @@ def main():
                     help="time the workload's own step only: argsort_u32 without the two-call (argsort + "
                          "gather) comparison, batched without per-table calls and graph replay")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--tune", action="append", default=[], metavar="KNOB=VALUE",
+                    help="set a launch-shape knob of the handle (vdf_tune_set), e.g. sort_items_32=12; "
+                         "with --no-tune the setting is what is timed")
     ap.add_argument("--calibrate", action="store_true",
                     help="first run one plain 1 GiB device copy (the known-size kernel that scales "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")

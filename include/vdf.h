@@ -97,6 +97,54 @@ typedef int vdf_status;
 #define VDF_MAX_KEY_COLS    4
 #define VDF_MAX_STRING_BYTES 4096
 
+/* ---- CSV ingest ----
+ * Replaces read_csv (read/read_csv/read_csv.hpp:9-216, with the column type
+ * inference of inlines/classify_column.inl:13-130 and the conversions of
+ * std::from_chars / fast_float::from_chars).  The text (d_text, bytes, on the
+ * device, 16-byte aligned) is split into rows at '\n' (a '\r' before it is
+ * dropped; an unterminated last line is a row), rows into fields at `delim`
+ * outside pairs of `quote`; with `header` the first line names the columns.
+ * Column types come from the first 10 data rows: numbers
+ * ([-]digits[.digits][e[+-]digits]) give DOUBLE when one has a '.' or an
+ * exponent, INT when one starts with '-', BOOL when all are 0 or 1, else
+ * UINT; other columns are CHAR (fixed char_width bytes, padded with
+ * `trailing`) when their first 10 fields fit char_width bytes, else STRING
+ * (padded with 0x00 to the column's longest field).  Every field converts
+ * from its longest valid numeric prefix (0 when there is none or an integer
+ * is out of range); doubles are correctly rounded.  A row with another field
+ * count fails with VDF_ERR_INVALID.  The columns live in the table's device
+ * memory until vdf_csv_free.  Synchronous (the null stream). */
+#define VDF_CSV_STRING 0 /* type letters of type_classification.inl: s */
+#define VDF_CSV_CHAR   1 /* c */
+#define VDF_CSV_BOOL   2 /* b: uint32 cells, 0 or 1 */
+#define VDF_CSV_INT    3 /* i: int32 or int64 cells (int_bytes) */
+#define VDF_CSV_UINT   4 /* u: uint32 or uint64 cells (int_bytes) */
+#define VDF_CSV_DOUBLE 5 /* d: float64 cells */
+
+typedef struct vdf_csv_options {
+    char delim;      /* ',' */
+    char quote;      /* str_context ('\'' in the reference) */
+    char trailing;   /* TrailingChar: pads CHAR cells ('0' in the reference) */
+    char reserved0;
+    int header;      /* header_name: the first line holds column names */
+    int int_bytes;   /* 4 or 8: cells of INT / UINT columns (IntT / UIntT) */
+    int char_width;  /* df_charbuf_size: CHAR cell bytes (0: no CHAR columns) */
+} vdf_csv_options;
+
+typedef struct vdf_csv_column {
+    int kind;          /* VDF_CSV_* */
+    uint32_t width;    /* bytes per cell */
+    void* data;        /* device, nrow cells (owned by the table) */
+    const char* name;  /* header name ("NA" for an empty one; "" without a header) */
+} vdf_csv_column;
+
+typedef struct vdf_csv_table vdf_csv_table;
+vdf_status vdf_read_csv(vdf_handle* h, const void* d_text, uint64_t bytes, const vdf_csv_options* opt,
+                        vdf_csv_table** out);
+vdf_status vdf_csv_info(vdf_csv_table* t, uint64_t* nrow, int* ncol);
+vdf_status vdf_csv_column_get(vdf_csv_table* t, int c, vdf_csv_column* out);
+vdf_status vdf_csv_free(vdf_csv_table* t);
+
 /* ---- library / handle ---- */
 const char* vdf_last_error(void);
 const char* vdf_version(void);

@@ -9,6 +9,8 @@
 //                                         transform_group_by.hpp:11-14
 //   group_by(key_col, value_cols)         per-group results (keys ascending)
 //   materialize()                         materialize/materialize_mt.hpp:10
+//   read_csv(file_name, delim, header_name, str_context)
+//                                         read/read_csv/read_csv.hpp:9-216
 // Columns live in HBM; every operation runs in the HIP library.  Errors from
 // the library are thrown as vison_gpu::Error (the reference prints to
 // std::cerr and returns; a thrown error keeps the frame unchanged instead).
@@ -19,6 +21,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <cstring>
+#include <fstream>
 #include <initializer_list>
 #include <stdexcept>
 #include <string>
@@ -499,7 +502,78 @@ class Dataframe {
         cols_.push_back(std::move(c));
     }
 
+    // Load a CSV file into this (empty) frame: the file's bytes go to HBM once
+    // and vdf_read_csv parses them there (read_csv.hpp:9-216; types as its
+    // classification of the first 10 rows).  IntT / UIntT columns are
+    // int64_t / uint64_t (int_bytes 8), CharT cells Char16 (char_width 16,
+    // padded with TrailingChar), bool columns uint32_t 0 / 1, other text
+    // columns std::string columns.
+    template <char TrailingChar = '0'>
+    void read_csv(const std::string& file_name, char delim = ',', bool header_name = true, char str_context = '\'')
+    {
+        if (!cols_.empty()) throw Error(VDF_ERR_INVALID, "read_csv: the frame already has columns");
+        std::ifstream in(file_name, std::ios::binary | std::ios::ate);
+        if (!in) throw Error(VDF_ERR_INVALID, "read_csv: cannot open " + file_name);
+        const std::streamsize size = in.tellg();
+        in.seekg(0);
+        std::vector<char> host(static_cast<size_t>(std::max<std::streamsize>(size, 0)));
+        if (size > 0 && !in.read(host.data(), size)) throw Error(VDF_ERR_INVALID, "read_csv: cannot read " + file_name);
+        DeviceBuffer text(host.size());
+        if (!host.empty())
+            hip_check(hipMemcpy(text.get(), host.data(), host.size(), hipMemcpyHostToDevice), "hipMemcpy H2D");
+        vdf_csv_options opt{};
+        opt.delim = delim;
+        opt.quote = str_context;
+        opt.trailing = TrailingChar;
+        opt.header = header_name ? 1 : 0;
+        opt.int_bytes = 8;
+        opt.char_width = 16;
+        vdf_csv_table* t = nullptr;
+        check(vdf_read_csv(h_.get(), text.get(), host.size(), &opt, &t));
+        struct Free {
+            vdf_csv_table* t;
+            ~Free() { (void)vdf_csv_free(t); }
+        } guard{t};
+        uint64_t nrow = 0;
+        int ncol = 0;
+        check(vdf_csv_info(t, &nrow, &ncol));
+        for (int k = 0; k < ncol; ++k) {
+            vdf_csv_column vc{};
+            check(vdf_csv_column_get(t, k, &vc));
+            Column c;
+            c.name = vc.name ? vc.name : "";
+            c.elem = vc.width;
+            switch (vc.kind) {
+            case VDF_CSV_DOUBLE: set_types<double>(c); break;
+            case VDF_CSV_INT: set_types<int64_t>(c); break;
+            case VDF_CSV_UINT: set_types<uint64_t>(c); break;
+            case VDF_CSV_BOOL: set_types<uint32_t>(c); break;
+            case VDF_CSV_CHAR: set_types<Char16>(c); break;
+            default:
+                c.letter = 's';
+                c.str = true;
+                break;
+            }
+            c.data = DeviceBuffer(static_cast<size_t>(nrow) * c.elem);
+            if (nrow)
+                hip_check(hipMemcpy(c.data.get(), vc.data, static_cast<size_t>(nrow) * c.elem, hipMemcpyDeviceToDevice),
+                          "hipMemcpy D2D");
+            cols_.push_back(std::move(c));
+        }
+        nrow_ = static_cast<unsigned int>(nrow);
+    }
+
+    const std::string& get_colname(unsigned int n) const { return col(n).name; }
+
   private:
+    template <typename T>
+    static void set_types(Column& c)
+    {
+        c.key_type = TypeCode<T>::key;
+        c.val_type = TypeCode<T>::val;
+        c.letter = TypeCode<T>::letter;
+    }
+
     template <typename K>
     GroupResult<K> to_host(const DeviceGroups& D, const std::vector<unsigned int>& vals, bool with_keys = true) const
     {

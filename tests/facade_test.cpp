@@ -11,6 +11,7 @@
 #include <map>
 #include <numeric>
 #include <random>
+#include <unistd.h>
 
 using vison_gpu::Char16;
 using vison_gpu::Dataframe;
@@ -364,6 +365,67 @@ int main()
             thrown = e.status == VDF_ERR_UNSUPPORTED;
         }
         CHECK(thrown);
+    }
+    // read_csv: a file of uint64 keys, int64 values with negatives, float64
+    // values, short and long text; every column as written, then a group-by
+    // over the file's key column
+    std::printf("read_csv\n");
+    {
+        char path[] = "/tmp/vdf_facade_XXXXXX";
+        const int fd = mkstemp(path);
+        CHECK(fd >= 0);
+        if (fd >= 0) {
+            const size_t m = 50007;
+            std::mt19937_64 rng(5);
+            std::vector<uint64_t> key(m);
+            std::vector<int64_t> iv(m);
+            std::vector<double> dv(m);
+            std::vector<std::string> name(m), note(m);
+            std::string text = "key,iv,dv,name,note\n";
+            char buf[64];
+            for (size_t i = 0; i < m; ++i) {
+                key[i] = (rng() % 300) * 0x9E3779B97F4A7C15ull;
+                iv[i] = i == 0 ? -7 : static_cast<int64_t>(rng() % 2000001) - 1000000;  // a '-' in the first rows
+                dv[i] = std::ldexp(static_cast<double>(rng() >> 11), -53) * ((rng() & 1) ? 1.0 : -1.0);
+                name[i] = std::string(1 + rng() % 10, static_cast<char>('a' + rng() % 26));
+                note[i] = std::string(17 + rng() % 30, static_cast<char>('A' + rng() % 26));
+                std::snprintf(buf, sizeof buf, "%.17g", dv[i]);
+                text += std::to_string(key[i]) + "," + std::to_string(iv[i]) + "," + buf + "," + name[i] + "," +
+                        note[i] + "\n";
+            }
+            CHECK(write(fd, text.data(), text.size()) == static_cast<ssize_t>(text.size()));
+            close(fd);
+            Dataframe df;
+            df.read_csv(path);
+            std::remove(path);
+            CHECK(df.get_nrow() == m && df.get_ncol() == 5);
+            CHECK(df.get_colname(0) == "key" && df.get_colname(4) == "note");
+            CHECK(df.get_col<uint64_t>(0) == key);
+            CHECK(df.get_col<int64_t>(1) == iv);
+            const auto d2 = df.get_col<double>(2);
+            bool ok = d2.size() == m;
+            for (size_t i = 0; i < m && ok; ++i) ok = same_bits(d2[i], dv[i]);
+            CHECK(ok);
+            const auto nm = df.get_col<Char16>(3);
+            ok = nm.size() == m;
+            for (size_t i = 0; i < m && ok; ++i) {
+                std::string want = name[i] + std::string(16 - name[i].size(), '0');
+                ok = std::memcmp(nm[i].c, want.data(), 16) == 0;
+            }
+            CHECK(ok);
+            CHECK(df.get_str_col(4) == note);
+            const auto R = df.group_by<uint64_t>(0, {1});
+            std::map<uint64_t, std::pair<uint64_t, int64_t>> acc;
+            for (size_t i = 0; i < m; ++i) {
+                acc[key[i]].first++;
+                acc[key[i]].second += iv[i];
+            }
+            ok = R.n_groups == acc.size();
+            size_t g = 0;
+            for (auto it = acc.begin(); ok && it != acc.end(); ++it, ++g)
+                ok = R.keys[g] == it->first && R.counts[g] == it->second.first && R.sum_i[0][g] == it->second.second;
+            CHECK(ok);
+        }
     }
     if (g_fail) {
         std::printf("%d failures\n", g_fail);

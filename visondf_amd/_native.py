@@ -49,7 +49,19 @@ EXPORTS = (
     "vdf_plan_groupby_batch", "vdf_plan_launch", "vdf_plan_scratch_bytes", "vdf_plan_destroy",
     "vdf_groupby_multi", "vdf_argsort_multi", "vdf_argsort_bytes", "vdf_groupby_bytes", "vdf_broadcast",
     "vdf_grouping_create", "vdf_grouping_aggregate", "vdf_grouping_destroy",
+    "vdf_read_csv", "vdf_csv_info", "vdf_csv_column_get", "vdf_csv_free",
 )
+
+CSV_STRING, CSV_CHAR, CSV_BOOL, CSV_INT, CSV_UINT, CSV_DOUBLE = 0, 1, 2, 3, 4, 5
+
+
+class VdfCsvOptions(C.Structure):
+    _fields_ = [("delim", C.c_char), ("quote", C.c_char), ("trailing", C.c_char), ("reserved0", C.c_char),
+                ("header", C.c_int), ("int_bytes", C.c_int), ("char_width", C.c_int)]
+
+
+class VdfCsvColumn(C.Structure):
+    _fields_ = [("kind", C.c_int), ("width", C.c_uint32), ("data", C.c_void_p), ("name", C.c_char_p)]
 
 
 class VdfValueCol(C.Structure):
@@ -134,6 +146,14 @@ def load() -> C.CDLL:
     lib.vdf_destroy.argtypes = [vp]
     lib.vdf_argsort_scratch_bytes.restype = u64
     lib.vdf_argsort_scratch_bytes.argtypes = [i32, u64, u32, i32]
+    lib.vdf_read_csv.restype = i32
+    lib.vdf_read_csv.argtypes = [vp, vp, u64, C.POINTER(VdfCsvOptions), C.POINTER(vp)]
+    lib.vdf_csv_info.restype = i32
+    lib.vdf_csv_info.argtypes = [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_int)]
+    lib.vdf_csv_column_get.restype = i32
+    lib.vdf_csv_column_get.argtypes = [vp, i32, C.POINTER(VdfCsvColumn)]
+    lib.vdf_csv_free.restype = i32
+    lib.vdf_csv_free.argtypes = [vp]
     lib.vdf_groupby_scratch_bytes.restype = u64
     lib.vdf_groupby_scratch_bytes.argtypes = [i32, u64]
     lib.vdf_groupby_batch_scratch_bytes.restype = u64

@@ -420,6 +420,52 @@ class Vdf:
                                               C.c_void_p(_stream_ptr(stream)), C.byref(rep)))
         return {k: getattr(rep, k) for k, _ in N.VdfTuneReport._fields_ if k != "reserved"}
 
+    # ------------------------------------------------------------- CSV ingest
+    def read_csv(self, text, delim: str = ",", quote: str = "'", header: bool = True, int_bytes: int = 8,
+                 char_width: int = 16, trailing: str = "0") -> list:
+        """Parses CSV text (bytes, or a uint8 device tensor) on the device
+        (vdf_read_csv).  Returns [(name, kind, tensor)] per column: DOUBLE
+        float64, INT int32/int64, UINT uint32/uint64 (as int tensors of the
+        same bits), BOOL int32 0/1, CHAR and STRING (n, width) uint8."""
+        if isinstance(text, (bytes, bytearray)):
+            buf = torch.frombuffer(bytearray(text), dtype=torch.uint8) if len(text) else torch.empty(0, dtype=torch.uint8)
+            dt = buf.cuda()
+        else:
+            dt = text
+        torch.cuda.synchronize()
+        opt = N.VdfCsvOptions(delim=delim.encode(), quote=quote.encode(), trailing=trailing.encode(), reserved0=b"\0",
+                              header=1 if header else 0, int_bytes=int_bytes, char_width=char_width)
+        t = C.c_void_p()
+        N.check(self.lib.vdf_read_csv(self.h, C.c_void_p(dt.data_ptr()) if dt.numel() else None, dt.numel(),
+                                      C.byref(opt), C.byref(t)))
+        try:
+            nrow, ncol = C.c_uint64(), C.c_int()
+            N.check(self.lib.vdf_csv_info(t, C.byref(nrow), C.byref(ncol)))
+            out = []
+            for c in range(ncol.value):
+                col = N.VdfCsvColumn()
+                N.check(self.lib.vdf_csv_column_get(t, c, C.byref(col)))
+                n = nrow.value
+                if col.kind == N.CSV_DOUBLE:
+                    x = torch.empty(n, dtype=torch.float64, device=dt.device)
+                elif col.kind in (N.CSV_INT, N.CSV_UINT):
+                    x = torch.empty(n, dtype=torch.int64 if col.width == 8 else torch.int32, device=dt.device)
+                elif col.kind == N.CSV_BOOL:
+                    x = torch.empty(n, dtype=torch.int32, device=dt.device)
+                else:
+                    x = torch.empty((n, col.width), dtype=torch.uint8, device=dt.device)
+                nbytes = x.numel() * x.element_size()
+                if nbytes:
+                    from .server import _hip_lib
+
+                    rc = _hip_lib().hipMemcpy(x.data_ptr(), col.data, nbytes, 3)
+                    if rc != 0:
+                        raise N.VdfError(N.VDF_ERR_HIP, f"hipMemcpy of a CSV column failed ({rc})")
+                out.append(((col.name or b"").decode(), col.kind, x))
+            return out
+        finally:
+            N.check(self.lib.vdf_csv_free(t))
+
     # ----------------------------------------------------------------- tuning
     def tune(self, name: str, value: int):
         N.check(self.lib.vdf_tune_set(self.h, name.encode(), int(value)))
