@@ -98,15 +98,34 @@ def _threads(args) -> int:
     return max(t, 1)
 
 
+def _sources_hash() -> str:
+    """sha256 prefix over the kernel sources (visondf_amd/csrc/*.hip, *.h),
+    the stamp profiles/traffic.json records for the build it measured."""
+    import glob
+    import hashlib
+
+    h = hashlib.sha256()
+    csrc = os.path.join(ROOT, "visondf_amd", "csrc")
+    for p in sorted(glob.glob(os.path.join(csrc, "*.hip")) + glob.glob(os.path.join(csrc, "*.h"))):
+        h.update(os.path.basename(p).encode())
+        with open(p, "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
+
+
 def _load_traffic(workload: str, name: str):
-    """HBM bytes per launch of a kernel in a workload, from the committed
-    rocprofv3 counter summaries (profiles/traffic.json), or None."""
+    """(HBM bytes per launch of a kernel in a workload, whether the kernel
+    sources are the ones that were measured) from the committed rocprofv3
+    counter summaries (profiles/traffic.json); (None, None) when absent."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
-        return None
+        return None, None
     with open(path) as f:
         d = json.load(f)
-    return d.get(workload, {}).get(name)
+    t = d.get(workload, {}).get(name)
+    if t is None:
+        return None, None
+    return t, d.get("_sources") == _sources_hash()
 
 
 def _kernel_table(v) -> dict:
@@ -127,8 +146,9 @@ def _roofline(workload: str, kernels: dict, name: str | None = None) -> dict:
         cands = [(k["total_ms"], n) for n, k in kernels.items() if k["gbps"]]
         name = max(cands)[1]
     k = kernels[name]
+    traffic, current = _load_traffic(workload, name)
     return {"bound": "hbm", "kernel": name, "achieved": k["gbps"], "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-            "frac": k["gbps"] / HBM_PEAK_GBPS, "traffic": _load_traffic(workload, name)}
+            "frac": k["gbps"] / HBM_PEAK_GBPS, "traffic": traffic, "traffic_sources_current": current}
 
 
 def _handle(args):
