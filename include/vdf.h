@@ -362,10 +362,10 @@ vdf_status vdf_groupby_batch(vdf_handle* h, int key_type, const vdf_table* table
  *                    a 16-byte one),
  *                    8, 12, 16
  *   "sort_items_128" same for 16-byte keys: 0 (default: 8), 8, 16
- *   "sort_chunk"     consecutive tiles one tile group (one XCD) claims at a time: 1 .. 1024
- *   "sort_xcd"       1 (default): XCD-grouped tile order, passes launched cooperatively;
- *                    0: global tile order, plain launches (for handles whose sorts run
- *                    concurrently with other persistent work on the device)
+ *   "sort_chunk"     consecutive tiles one XCD claims at a time: 1 .. 1024
+ *   "sort_xcd"       1 (default): XCD-grouped tile order (each XCD works through chunks of
+ *                    consecutive tiles); 0: global tile order.  Both are plain launches
+ *                    that need no co-residency, so sorts may run beside other work
  *   "sort_blocks_per_cu"  radix pass blocks per CU: 0 (default: every resident block) .. 32
  *   "hist_rows"      rows per digit-histogram work item: 4096 .. 2^24
  *   "gather_rows"    rows per thread of the gather kernel: 4, 8, 16
@@ -421,7 +421,7 @@ vdf_status vdf_autotune_groupby_batch(vdf_handle* h, int key_type, const vdf_tab
  * shape: the host rewrites the batch description the graph uploads first,
  * then launches the graph on `stream`.  Replays of one plan share its scratch:
  * launch them on one stream or order them.  The tuning table is the handle's
- * at capture, with radix passes in global tile order (plain launches).
+ * at capture.
  * No reference counterpart. */
 typedef struct vdf_plan vdf_plan;
 vdf_status vdf_plan_groupby_batch(vdf_handle* h, int key_type, const vdf_table* tables, int ntables,

@@ -99,7 +99,7 @@ def test_plan_replays_equal_eager(vdf):
 
 def test_plan_without_permutation_equals_eager(vdf):
     """A plan whose outputs take no permutation replays equal to the eager call
-    without one (the eager call may skip row ids; the plan always moves them)."""
+    without one (both run the sort without row ids)."""
     import torch
 
     tabs = [(_dev(k), [_dev(x) for x in vs]) for k, vs in _batch(4, SIZES)]
@@ -110,6 +110,35 @@ def test_plan_without_permutation_equals_eager(vdf):
         torch.cuda.synchronize()
         for i, e in enumerate(eager):
             _same(pb.result(i), e, f"no-perm replay {rep} table {i}")
+    pb.release_plan()
+
+
+def test_plan_switches_permutation_between_replays(vdf):
+    """One plan replayed with permutation outputs, then without (the variant
+    captured without row ids), then with them again: every replay equals the
+    eager call with the same outputs."""
+    import torch
+
+    tabs = [(_dev(k), [_dev(x) for x in vs]) for k, vs in _batch(5, SIZES)]
+    eager = vdf.groupby_batch(tabs, key_type=N.KEY_I32, want_perm=True)
+    pb = vdf.prepare_groupby_batch(tabs, key_type=N.KEY_I32, want_perm=True).capture()
+    perms = [pb.outs[i].perm for i in range(pb.ntab)]
+    for rep, with_perm in enumerate((True, False, True, False)):
+        _say("plan replay", rep, "with permutation" if with_perm else "without permutation")
+        for i in range(pb.ntab):
+            pb.outs[i].perm = perms[i] if with_perm else None
+            if not with_perm and pb.keep[i]["perm"] is not None:
+                pb.keep[i]["perm"].fill_(-1)
+        pb.replay()
+        torch.cuda.synchronize()
+        for i, e in enumerate(eager):
+            r = pb.result(i)
+            if not with_perm:
+                # nothing writes the permutation buffers on this replay
+                if r.perm is not None:
+                    assert bool((r.perm == -1).all()), f"replay {rep} table {i} wrote a permutation"
+                r.perm = None
+            _same(r, e, f"replay {rep} table {i}")
     pb.release_plan()
 
 

@@ -8,8 +8,8 @@
 // graph: a replay only rewrites the pinned buffer on the host (a "dry"
 // enqueue that builds the description and launches nothing) and launches the
 // graph.  The plan owns its scratch, so a plan replays independently of the
-// handle's other work; its radix passes claim tiles in global order with
-// plain launches, which needs no co-residency beside other work.  Replays of
+// handle's other work; its radix passes are plain launches whose tile claiming
+// needs no co-residency beside other work.  Replays of
 // one plan share its scratch: launch them on one stream (or order them).  Two
 // pinned buffers, each captured into its own graph,
 // let the host prepare the next replay while the previous one runs.
@@ -32,10 +32,13 @@ struct Plan {
     size_t staging_bytes = 0;
     void* scratch = nullptr;
     size_t scratch_bytes = 0;
+    // graph[v]: v = 1 moves row ids through the sort (a replay that asks for a
+    // permutation), v = 0 does not (the row-id-free kernels; null when the
+    // value columns need row ids anyway)
     struct Buf {
         void* stg = nullptr;
-        hipGraph_t graph = nullptr;
-        hipGraphExec_t exec = nullptr;
+        hipGraph_t graph[2] = {nullptr, nullptr};
+        hipGraphExec_t exec[2] = {nullptr, nullptr};
         hipEvent_t done = nullptr;
         bool pending = false;
     } buf[2];
@@ -45,8 +48,10 @@ struct Plan {
     {
         for (Buf& b : buf) {
             if (b.pending && b.done) (void)hipEventSynchronize(b.done);
-            if (b.exec) (void)hipGraphExecDestroy(b.exec);
-            if (b.graph) (void)hipGraphDestroy(b.graph);
+            for (int v = 0; v < 2; ++v) {
+                if (b.exec[v]) (void)hipGraphExecDestroy(b.exec[v]);
+                if (b.graph[v]) (void)hipGraphDestroy(b.graph[v]);
+            }
             if (b.done) (void)hipEventDestroy(b.done);
             if (b.stg) (void)hipHostFree(b.stg);
             b = Buf{};
@@ -70,11 +75,12 @@ std::vector<GbTable> to_tables(const vdf_table* t, int n, int nvals)
     return ts;
 }
 
-// Captures buffer k's graph: the enqueue runs on a capturing stream with the
-// plan's scratch and buffer k as staging.
-void capture(Plan& p, int k, const std::vector<GbTable>& ts, const std::vector<vdf_groupby_out>& outs)
+// Captures buffer k's graph of variant v (row ids moved or not): the enqueue
+// runs on a capturing stream with the plan's scratch and buffer k as staging.
+void capture(Plan& p, int k, int v, const std::vector<GbTable>& ts, const std::vector<vdf_groupby_out>& outs)
 {
     Plan::Buf& b = p.buf[k];
+    p.ph.keep_rowids = v == 1;
     p.ph.staging_fixed = b.stg;
     p.ph.staging_fixed_cap = p.staging_bytes;
     hipStream_t cs = nullptr;
@@ -101,8 +107,8 @@ void capture(Plan& p, int k, const std::vector<GbTable>& ts, const std::vector<v
         (void)hipGetLastError();
         throw Error{VDF_ERR_HIP, err.empty() ? std::string("hipStreamEndCapture: ") + hipGetErrorString(e) : err};
     }
-    b.graph = g;
-    e = hipGraphInstantiate(&b.exec, g, nullptr, nullptr, 0);
+    b.graph[v] = g;
+    e = hipGraphInstantiate(&b.exec[v], g, nullptr, nullptr, 0);
     if (e != hipSuccess) {
         (void)hipGetLastError();
         throw Error{VDF_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e)};
@@ -140,10 +146,6 @@ vdf_status vdf_plan_groupby_batch(vdf_handle* h, int key_type, const vdf_table* 
             p->ph.device = h->device;
             p->ph.tune = h->tune;
         }
-        // Radix passes in global tile order with plain launches: a replay may run
-        // beside any other work, and this order needs no co-residency.
-        p->ph.tune.sort_xcd = 0;
-        p->ph.keep_rowids = true;
         VDF_HIP_CHECK(hipSetDevice(p->ph.device));
         p->key_type = key_type;
         p->nvals = nvals;
@@ -172,7 +174,14 @@ vdf_status vdf_plan_groupby_batch(vdf_handle* h, int key_type, const vdf_table* 
                 VDF_HIP_CHECK(hipHostMalloc(&p->buf[k].stg, p->staging_bytes, hipHostMallocDefault));
                 VDF_HIP_CHECK(hipEventCreateWithFlags(&p->buf[k].done, hipEventDisableTiming));
             }
-            for (int k = 0; k < 2; ++k) capture(*p, k, ts, outs);
+            // value columns beyond the carried ones are gathered through row ids,
+            // so such a plan moves them in either case and needs one variant
+            const int pay = groupby_carry_bytes(p->ph.tune, key_type, p->val_types, nvals);
+            const bool rows_always = nvals > (pay >= 12 ? 2 : (pay != 0 ? 1 : 0));
+            for (int k = 0; k < 2; ++k) {
+                capture(*p, k, 1, ts, outs);
+                if (!rows_always) capture(*p, k, 0, ts, outs);
+            }
         } catch (...) {
             p->release();
             (void)hipSetDevice(prev);
@@ -203,6 +212,9 @@ vdf_status vdf_plan_launch(vdf_plan* p, const vdf_table* tables, int ntables, vd
         VDF_HIP_CHECK(hipSetDevice(p->ph.device));
         Plan::Buf& b = p->buf[p->next];
         p->next ^= 1;
+        bool want_perm = false;
+        for (int i = 0; i < ntables; ++i) want_perm = want_perm || outs[i].perm != nullptr;
+        const int v = (want_perm || !b.exec[0]) ? 1 : 0;
         try {
             if (b.pending) {
                 VDF_HIP_CHECK(hipEventSynchronize(b.done));  // this buffer's previous upload is done
@@ -210,6 +222,7 @@ vdf_status vdf_plan_launch(vdf_plan* p, const vdf_table* tables, int ntables, vd
             }
             // rebuild the batch description in the buffer the graph uploads from
             const std::vector<GbTable> ts = to_tables(tables, ntables, p->nvals);
+            p->ph.keep_rowids = v == 1;
             p->ph.staging_fixed = b.stg;
             p->ph.staging_fixed_cap = p->staging_bytes;
             p->ph.dry = true;
@@ -224,7 +237,7 @@ vdf_status vdf_plan_launch(vdf_plan* p, const vdf_table* tables, int ntables, vd
             }
             p->ph.dry = false;
             const hipStream_t s = static_cast<hipStream_t>(stream);
-            VDF_HIP_CHECK(hipGraphLaunch(b.exec, s));
+            VDF_HIP_CHECK(hipGraphLaunch(b.exec[v], s));
             VDF_HIP_CHECK(hipEventRecord(b.done, s));
             b.pending = true;
         } catch (...) {

@@ -192,10 +192,13 @@ struct HistItem {
 };
 
 constexpr int kPlanHdr = 1 + kMaxDigits;  // passes + digit_of[]
-// Tile groups number at most 8 (one per XCD), so the last ticket word of slot
-// 0 is free: k_plan keeps the largest pass count of the batch there.
+// Ticket words of a pass slot: one claim counter per XCD (kXcds), then the
+// slot's chunk counter.  The last word of slot 0 is free: k_plan keeps the
+// largest pass count of the batch there.
+constexpr uint32_t kXcds = 8;
+constexpr uint32_t kChunkWord = kXcds;
 constexpr uint32_t kMaxPassWord = kTicketStride - 1;
-static_assert(kTicketStride > 8, "ticket words of slot 0 hold 8 tile groups and the pass count");
+static_assert(kTicketStride > kXcds + 1, "ticket words of slot 0 hold the XCD counters, the chunk counter and the pass count");
 
 struct SortArgs {
     const SegDesc* segs;
@@ -211,12 +214,14 @@ struct SortArgs {
     void* pbuf2[2];            // second payload column between passes (16-byte payloads)
     uint32_t* lb[2];           // look-back words [tiles][256], alternating by slot parity
     uint32_t* tickets;         // [kMaxDigits][kTicketStride] tile-claim counters
+    uint32_t* xmap;            // [D][kXcds][xstride]: chunk id + 1 of each XCD's j-th chunk (0: not yet)
+    uint32_t xstride;          // chunks of one pass (entries per XCD)
     uint64_t mlo, mhi;         // key transform masks
     uint64_t lb_words;         // words of one look-back buffer
     uint32_t nsegs;
     uint32_t ntiles;
-    uint32_t groups;           // tile groups (one per XCD when the grid allows)
-    uint32_t chunk;            // consecutive tiles a group claims at a time
+    uint32_t groups;           // kXcds: XCD-grouped claiming; 1: global tile order
+    uint32_t chunk;            // consecutive tiles an XCD claims at a time
     uint32_t perm64;
     uint32_t reverse;
     uint32_t norows;           // 1: no segment needs its permutation; row ids are not moved
@@ -402,17 +407,22 @@ struct TileCtx {
 // One radix scatter pass over the tiles of every segment, persistent and
 // software-pipelined (SPLIT kernels, below, take one tile at a time).
 //
-// Tile claiming.  Block b belongs to group g = b % groups; dispatch puts
-// consecutive blocks on consecutive XCDs, so a group is one XCD.  The k-th
-// claim of group g gets tile (k / chunk * groups + g) * chunk + k % chunk:
-// runs of `chunk` consecutive tiles are written by CUs sharing one L2, which
-// merges the partial lines of adjacent bucket runs.  With several groups the
-// kernel is launched cooperatively, so every block is resident; each group
-// claims its tiles in increasing order and every claimed tile publishes its
-// aggregate before its block waits, so the smallest unfinished tile always
-// completes: no deadlock.  With one group and chunk 1 the claimed tiles are
-// always a prefix of the tile order held by running blocks, which is
-// deadlock-free at any residency (plain launch).
+// Tile claiming (XCD-grouped).  The tiles are cut into chunks of `chunk`
+// consecutive tiles, and each XCD (read from the hardware's XCC_ID register,
+// not assumed from the block index) works through chunks of its own, so a
+// chunk's bucket runs are written by CUs sharing one L2, which merges the
+// partial lines of adjacent runs.  The k-th claim of XCD x is tile r = k %
+// chunk of x's j-th chunk, j = k / chunk.  The claim with r = 0 waits until
+// x's chunk j - 1 is known, takes the next chunk id from the slot's chunk
+// counter and publishes it in xmap; the other claims of chunk j wait for that
+// entry.  Both waits are on a block that already holds a claim and is running
+// the claim code, so they end.  Each XCD thus takes chunks in increasing order
+// and claims increasing tiles, every tile of an XCD's earlier chunks is held
+// by a running block, and every claimed tile publishes its aggregate before
+// its block waits on a predecessor.  So the smallest unfinished tile is held
+// by a running block whose predecessors are done, at any residency: the
+// kernel needs no cooperative launch and runs beside other work.  With one
+// group the tiles are claimed in global order from one counter.
 //
 // Per tile: load (coalesced, striped rows), stable rank inside each wave,
 // publish the tile's per-bucket counts (flag A; a segment's first tile
@@ -497,9 +507,34 @@ void k_pass(SortArgs a, uint32_t slot)
 
     auto claim = [&]() -> uint32_t {
         if (tid == 0) {
-            const uint32_t g = blockIdx.x % a.groups;
-            const uint32_t k = atomicAdd(a.tickets + slot * kTicketStride + g, 1u);
-            const uint64_t t = (static_cast<uint64_t>(k / a.chunk) * a.groups + g) * a.chunk + (k % a.chunk);
+            uint32_t* tk = a.tickets + slot * kTicketStride;
+            uint64_t t;
+            if (a.groups == 1) {
+                t = atomicAdd(tk, 1u);
+            } else {
+                // XCC_ID (hardware register 20), bits 3:0
+                const uint32_t x = static_cast<uint32_t>(__builtin_amdgcn_s_getreg((3 << 11) | 20)) % kXcds;
+                const uint32_t k = atomicAdd(tk + x, 1u);
+                const uint32_t j = k / a.chunk, r = k % a.chunk;
+                if (j >= a.xstride) {
+                    // x's first xstride chunks took xstride ids: none is left
+                    t = a.ntiles;
+                } else {
+                    uint32_t* e = a.xmap + (static_cast<uint64_t>(slot) * kXcds + x) * a.xstride + j;
+                    uint32_t c;
+                    if (r == 0) {
+                        if (j > 0)
+                            while (load_relaxed(e - 1) == 0u) __builtin_amdgcn_s_sleep(1);
+                        c = atomicAdd(tk + kChunkWord, 1u);
+                        store_relaxed(e, c + 1u);
+                    } else {
+                        uint32_t v;
+                        while ((v = load_relaxed(e)) == 0u) __builtin_amdgcn_s_sleep(1);
+                        c = v - 1u;
+                    }
+                    t = static_cast<uint64_t>(c) * a.chunk + r;
+                }
+            }
             s_tile = static_cast<uint32_t>(std::min<uint64_t>(t, 0xFFFFFFFFull));
         }
         lds_barrier();
@@ -975,7 +1010,8 @@ constexpr uint64_t kMinTileRows = static_cast<uint64_t>(kBlock) * 8;  // smalles
 // pass tile, so they hold every tuning choice.
 struct SortLayout {
     uint64_t rows = 0, tiles = 0, items = 0;
-    size_t zero_words = 0;
+    size_t zero_words = 0;  // tickets and histograms (zeroed by every sort)
+    size_t xmap_words = 0;  // XCD chunk maps (the used part is zeroed by every sort)
     size_t bytes = 0;
 };
 
@@ -995,8 +1031,9 @@ static SortLayout sort_layout(int key_type, const uint64_t* n, int nsegs, uint64
     const int D = key_digits(key_type);
     const size_t kb = key_bytes(key_type);
     L.zero_words = kMaxDigits * kTicketStride + static_cast<size_t>(nsegs) * D * kRadix;
+    L.xmap_words = static_cast<size_t>(D) * kXcds * std::max<uint64_t>(L.tiles, 1);
     size_t b = 0;
-    b += align_up(L.zero_words * 4, 256);
+    b += align_up((L.zero_words + L.xmap_words) * 4, 256);
     b += align_up(static_cast<size_t>(nsegs) * kPlanHdr * 4, 256);
     b += align_up(static_cast<size_t>(nsegs) * D * kRadix * 4, 256);
     b += align_up(static_cast<size_t>(nsegs) * sizeof(SegDesc), 256);
@@ -1281,9 +1318,10 @@ static void launch_sort(Handle* h, const SortSeg* segs, int nsegs, uint64_t mlo,
     if (ntiles == 0 || h->dry) return;  // every segment empty, or only the staging was wanted
 
     Carve c(scr);
-    uint32_t* zero = c.take<uint32_t>(L.zero_words);
+    uint32_t* zero = c.take<uint32_t>(L.zero_words + L.xmap_words);
     SortArgs a{};
     a.tickets = zero;
+    a.xmap = zero + L.zero_words;
     a.hist = zero + kMaxDigits * kTicketStride;
     a.plan_hdr = c.take<uint32_t>(static_cast<size_t>(nsegs) * kPlanHdr);
     a.plan_base = c.take<uint32_t>(static_cast<size_t>(nsegs) * D * kRadix);
@@ -1327,31 +1365,30 @@ static void launch_sort(Handle* h, const SortSeg* segs, int nsegs, uint64_t mlo,
     a.perm64 = (flags & VDF_SORT_PERM64) ? 1u : 0u;
     a.reverse = (flags & VDF_SORT_DESC_REVERSE) ? 1u : 0u;
     a.norows = rowids ? 0u : 1u;
-    // persistent grid: every resident block (or the tuned blocks per CU), a
-    // multiple of the group count
+    // persistent grid: every resident block (or the tuned blocks per CU)
     uint64_t cap = resident_blocks(pass, h->device);
     if (h->tune.sort_bpc > 0) cap = std::min<uint64_t>(cap, static_cast<uint64_t>(h->tune.sort_bpc) * device_cus(h->device));
-    unsigned grid = static_cast<unsigned>(std::min<uint64_t>(cap, ntiles));
-    const bool xcd = h->tune.sort_xcd != 0;
-    if (xcd) {
-        a.groups = std::min(8u, grid);
-        grid = grid / a.groups * a.groups;
-        // enough chunks that every group gets several
+    const unsigned grid = static_cast<unsigned>(std::min<uint64_t>(cap, ntiles));
+    size_t zero_n = L.zero_words;
+    if (h->tune.sort_xcd != 0) {
+        a.groups = kXcds;
+        // enough chunks that every XCD gets several
         a.chunk = static_cast<uint32_t>(std::max<uint64_t>(
-            1, std::min<uint64_t>(static_cast<uint64_t>(h->tune.sort_chunk),
-                                  ntiles / (static_cast<uint64_t>(a.groups) * 8))));
+            1, std::min<uint64_t>(static_cast<uint64_t>(h->tune.sort_chunk), ntiles / (kXcds * 8ull))));
+        a.xstride = static_cast<uint32_t>((ntiles + a.chunk - 1) / a.chunk);
+        require(static_cast<size_t>(D) * kXcds * a.xstride <= L.xmap_words, "XCD chunk map size");
+        zero_n += static_cast<size_t>(D) * kXcds * a.xstride;  // the maps follow the tickets and histograms
     } else {
-        // one group, chunk 1: tiles are claimed in global order, so the claimed
-        // tiles are always a prefix held by running blocks and no co-residency
-        // is needed
+        // tiles claimed in global order from one counter
         a.groups = 1;
         a.chunk = 1;
+        a.xstride = 0;
     }
 
     const size_t kb = sizeof(K);
     {
         LaunchTimer t(h, s, "fill", 0);
-        fill_u32(zero, L.zero_words, 0u, s);
+        fill_u32(zero, zero_n, 0u, s);
     }
     {
         LaunchTimer t(h, s, "radix_histogram", L.rows * kb);
@@ -1389,26 +1426,10 @@ static void launch_sort(Handle* h, const SortSeg* segs, int nsegs, uint64_t mlo,
         acct->per_row[3] = kb + rid + kout + pb_out + pay2;
     }
     const uint64_t middle_bytes = L.rows * (2 * kb + 2 * rid + 2 * static_cast<uint64_t>(pay_bytes));
-    bool coop = xcd;
     for (int slot = 0; slot < D; ++slot) {
         LaunchTimer t(h, s, "radix_pass", middle_bytes, slot, acct);
         uint32_t sl = static_cast<uint32_t>(slot);
         void* args[] = {&a, &sl};
-        if (coop) {
-            const hipError_t e =
-                hipLaunchCooperativeKernel(reinterpret_cast<const void*>(pass), dim3(grid), dim3(kBlock), args, 0, s);
-            if (e == hipSuccess) continue;
-            // The runtime can refuse cooperative launches (for instance while another
-            // process on the device holds the cooperative queue); nothing was
-            // enqueued.  Each pass is a complete stable scatter whatever order
-            // its tiles are claimed in, so this and the remaining passes run in
-            // global tile order, which needs no co-residency.
-            (void)e;
-            (void)hipGetLastError();
-            coop = false;
-            a.groups = 1;  // global tile order: no co-residency needed
-            a.chunk = 1;
-        }
         VDF_HIP_CHECK(hipLaunchKernel(reinterpret_cast<const void*>(pass), dim3(grid), dim3(kBlock), args, 0, s));
     }
 }

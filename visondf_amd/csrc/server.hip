@@ -307,7 +307,11 @@ vdf_status vdf_server_create(vdf_server** out, const vdf_server_config* cfg)
         S->lim.max_rows = cfg->max_batch_rows
                               ? cfg->max_batch_rows
                               : std::max<uint64_t>(cfg->arena_bytes / (kDefaultRowBytes * 2 * cfg->workers), 1ull << 20);
-        S->tune.sort_xcd = 0;  // plain launches: passes of several workers run together
+        // Global tile order: batches of several workers run at once, so a pass
+        // kernel may hold only a few blocks on an XCD, and a chunk of
+        // consecutive tiles claimed there would hold up every later tile's
+        // look-back while those few blocks work through it.
+        S->tune.sort_xcd = 0;
         try {
             void* p = nullptr;
             if (hipMalloc(&p, cfg->arena_bytes) != hipSuccess) {

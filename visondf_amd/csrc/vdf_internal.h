@@ -131,9 +131,9 @@ struct Tuning {
     int sort_items_32 = 0;   // rows per thread of a radix pass tile, 4-byte keys (0: by payload width)
     int sort_items_64 = 0;   // same, 8-byte keys
     int sort_items_128 = 0;  // same, 16-byte keys
-    int sort_chunk = 64;     // consecutive tiles a tile group (one XCD) claims at a time
-    int sort_xcd = 1;        // 1: XCD-grouped tile order, cooperative launch; 0: global tile
-                             //    order, plain launch (safe beside other persistent work)
+    int sort_chunk = 64;     // consecutive tiles one XCD claims at a time
+    int sort_xcd = 1;        // 1: XCD-grouped tile order; 0: global tile order (both plain
+                             //    launches, safe beside other persistent work)
     int sort_bpc = 0;        // radix pass blocks per CU (persistent grid); 0 = every resident block
     int hist_rows = 65536;   // rows per histogram work item
     int gather_rows = 8;     // rows per thread of the gather kernel
