@@ -682,6 +682,9 @@ __device__ __attribute__((always_inline)) inline void gb_column(const GbArgs& a,
     __syncthreads();  // every staged group of window 0 is in LDS
     // groups 0 .. H-2 finish in this tile (group H-1 runs into the next tiles)
     const uint32_t nfin = H ? H - 1 : 0u;
+    // the row loop stages this lane's groups of tile-local ids h0 .. h0 + nheads - 2
+    const uint32_t nheads = static_cast<uint32_t>(__popc(hb));
+    const uint32_t h0 = gfirst - gbase;
     for (uint32_t lo = 0;;) {
         const uint32_t nw = std::min<uint32_t>(nfin - lo, kGbStage);
         for (uint32_t h = tid; h < nw; h += kBlock) {
@@ -691,8 +694,11 @@ __device__ __attribute__((always_inline)) inline void gb_column(const GbArgs& a,
         __syncthreads();  // s_wagg and the stage are reused (next window, next column)
         lo += kGbStage;
         if (lo >= nfin) break;  // block-uniform: H is the tile's
-        load_vals();
-        row_loop(lo);
+        // only lanes with a group in this window read their values again
+        if (nheads > 1 && h0 + nheads - 1 > lo && h0 < lo + kGbStage) {
+            load_vals();
+            row_loop(lo);
+        }
         if (seg_fin) stage_final(st, gbase + lo, gfirst - 1, seg);
         __syncthreads();
     }

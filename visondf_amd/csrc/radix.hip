@@ -192,13 +192,14 @@ struct HistItem {
 };
 
 constexpr int kPlanHdr = 1 + kMaxDigits;  // passes + digit_of[]
-// Ticket words of a pass slot: one claim counter per XCD (kXcds), then the
-// slot's chunk counter.  The last word of slot 0 is free: k_plan keeps the
-// largest pass count of the batch there.
+// Ticket words of a pass slot: one 64-bit claim word per XCD (kXcds), then
+// the slot's chunk counter (also the tile counter of global order).  The last
+// word of slot 0 is free: k_plan keeps the largest pass count of the batch
+// there.
 constexpr uint32_t kXcds = 8;
-constexpr uint32_t kChunkWord = kXcds;
+constexpr uint32_t kChunkWord = 2 * kXcds;
 constexpr uint32_t kMaxPassWord = kTicketStride - 1;
-static_assert(kTicketStride > kXcds + 1, "ticket words of slot 0 hold the XCD counters, the chunk counter and the pass count");
+static_assert(kTicketStride > kChunkWord + 1, "ticket words of slot 0 hold the XCD words, the chunk counter and the pass count");
 
 struct SortArgs {
     const SegDesc* segs;
@@ -214,8 +215,6 @@ struct SortArgs {
     void* pbuf2[2];            // second payload column between passes (16-byte payloads)
     uint32_t* lb[2];           // look-back words [tiles][256], alternating by slot parity
     uint32_t* tickets;         // [kMaxDigits][kTicketStride] tile-claim counters
-    uint32_t* xmap;            // [D][kXcds][xstride]: chunk id + 1 of each XCD's j-th chunk (0: not yet)
-    uint32_t xstride;          // chunks of one pass (entries per XCD)
     uint64_t mlo, mhi;         // key transform masks
     uint64_t lb_words;         // words of one look-back buffer
     uint32_t nsegs;
@@ -411,18 +410,21 @@ struct TileCtx {
 // consecutive tiles, and each XCD (read from the hardware's XCC_ID register,
 // not assumed from the block index) works through chunks of its own, so a
 // chunk's bucket runs are written by CUs sharing one L2, which merges the
-// partial lines of adjacent runs.  The k-th claim of XCD x is tile r = k %
-// chunk of x's j-th chunk, j = k / chunk.  The claim with r = 0 waits until
-// x's chunk j - 1 is known, takes the next chunk id from the slot's chunk
-// counter and publishes it in xmap; the other claims of chunk j wait for that
-// entry.  Both waits are on a block that already holds a claim and is running
-// the claim code, so they end.  Each XCD thus takes chunks in increasing order
-// and claims increasing tiles, every tile of an XCD's earlier chunks is held
-// by a running block, and every claimed tile publishes its aggregate before
-// its block waits on a predecessor.  So the smallest unfinished tile is held
-// by a running block whose predecessors are done, at any residency: the
-// kernel needs no cooperative launch and runs beside other work.  With one
-// group the tiles are claimed in global order from one counter.
+// partial lines of adjacent runs.  XCD x's claim word holds its current
+// chunk id (low half) and R = r - chunk mod 2^32 (high half), r the claims
+// taken from that chunk; a claim is one 64-bit atomicAdd of 2^32:
+//  - r < chunk: the claim is tile id * chunk + r;
+//  - r = chunk (exactly one claim per chunk, and the very first claim of the
+//    pass, since the word starts at 0): this block takes the next chunk id
+//    from the slot's chunk counter and installs it with r = 1, taking tile 0;
+//  - r > chunk: the install is in progress (by a running block); claim again.
+// Each XCD thus takes chunks in increasing order and claims increasing tiles,
+// every tile of an XCD's earlier chunks is held by a running block, and
+// every claimed tile publishes its aggregate before its block waits on a
+// predecessor.  So the smallest unfinished tile is held by a running block
+// whose predecessors are done, at any residency: the kernel needs no
+// cooperative launch and runs beside other work.  With one group the tiles
+// are claimed in global order from the chunk counter.
 //
 // Per tile: load (coalesced, striped rows), stable rank inside each wave,
 // publish the tile's per-bucket counts (flag A; a segment's first tile
@@ -510,29 +512,25 @@ void k_pass(SortArgs a, uint32_t slot)
             uint32_t* tk = a.tickets + slot * kTicketStride;
             uint64_t t;
             if (a.groups == 1) {
-                t = atomicAdd(tk, 1u);
+                t = atomicAdd(tk + kChunkWord, 1u);
             } else {
                 // XCC_ID (hardware register 20), bits 3:0
                 const uint32_t x = static_cast<uint32_t>(__builtin_amdgcn_s_getreg((3 << 11) | 20)) % kXcds;
-                const uint32_t k = atomicAdd(tk + x, 1u);
-                const uint32_t j = k / a.chunk, r = k % a.chunk;
-                if (j >= a.xstride) {
-                    // x's first xstride chunks took xstride ids: none is left
-                    t = a.ntiles;
-                } else {
-                    uint32_t* e = a.xmap + (static_cast<uint64_t>(slot) * kXcds + x) * a.xstride + j;
-                    uint32_t c;
-                    if (r == 0) {
-                        if (j > 0)
-                            while (load_relaxed(e - 1) == 0u) __builtin_amdgcn_s_sleep(1);
-                        c = atomicAdd(tk + kChunkWord, 1u);
-                        store_relaxed(e, c + 1u);
-                    } else {
-                        uint32_t v;
-                        while ((v = load_relaxed(e)) == 0u) __builtin_amdgcn_s_sleep(1);
-                        c = v - 1u;
+                auto* w = reinterpret_cast<unsigned long long*>(tk) + x;
+                for (;;) {
+                    const unsigned long long v = atomicAdd(w, 1ull << 32);
+                    const uint32_t r = static_cast<uint32_t>(v >> 32) + a.chunk;
+                    if (r < a.chunk) {
+                        t = static_cast<uint64_t>(static_cast<uint32_t>(v)) * a.chunk + r;
+                        break;
                     }
-                    t = static_cast<uint64_t>(c) * a.chunk + r;
+                    if (r == a.chunk) {
+                        const uint32_t c = atomicAdd(tk + kChunkWord, 1u);
+                        atomicExch(w, (static_cast<unsigned long long>(1u - a.chunk) << 32) | c);
+                        t = static_cast<uint64_t>(c) * a.chunk;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
                 }
             }
             s_tile = static_cast<uint32_t>(std::min<uint64_t>(t, 0xFFFFFFFFull));
@@ -1011,7 +1009,6 @@ constexpr uint64_t kMinTileRows = static_cast<uint64_t>(kBlock) * 8;  // smalles
 struct SortLayout {
     uint64_t rows = 0, tiles = 0, items = 0;
     size_t zero_words = 0;  // tickets and histograms (zeroed by every sort)
-    size_t xmap_words = 0;  // XCD chunk maps (the used part is zeroed by every sort)
     size_t bytes = 0;
 };
 
@@ -1031,9 +1028,8 @@ static SortLayout sort_layout(int key_type, const uint64_t* n, int nsegs, uint64
     const int D = key_digits(key_type);
     const size_t kb = key_bytes(key_type);
     L.zero_words = kMaxDigits * kTicketStride + static_cast<size_t>(nsegs) * D * kRadix;
-    L.xmap_words = static_cast<size_t>(D) * kXcds * std::max<uint64_t>(L.tiles, 1);
     size_t b = 0;
-    b += align_up((L.zero_words + L.xmap_words) * 4, 256);
+    b += align_up(L.zero_words * 4, 256);
     b += align_up(static_cast<size_t>(nsegs) * kPlanHdr * 4, 256);
     b += align_up(static_cast<size_t>(nsegs) * D * kRadix * 4, 256);
     b += align_up(static_cast<size_t>(nsegs) * sizeof(SegDesc), 256);
@@ -1318,10 +1314,9 @@ static void launch_sort(Handle* h, const SortSeg* segs, int nsegs, uint64_t mlo,
     if (ntiles == 0 || h->dry) return;  // every segment empty, or only the staging was wanted
 
     Carve c(scr);
-    uint32_t* zero = c.take<uint32_t>(L.zero_words + L.xmap_words);
+    uint32_t* zero = c.take<uint32_t>(L.zero_words);
     SortArgs a{};
     a.tickets = zero;
-    a.xmap = zero + L.zero_words;
     a.hist = zero + kMaxDigits * kTicketStride;
     a.plan_hdr = c.take<uint32_t>(static_cast<size_t>(nsegs) * kPlanHdr);
     a.plan_base = c.take<uint32_t>(static_cast<size_t>(nsegs) * D * kRadix);
@@ -1369,26 +1364,21 @@ static void launch_sort(Handle* h, const SortSeg* segs, int nsegs, uint64_t mlo,
     uint64_t cap = resident_blocks(pass, h->device);
     if (h->tune.sort_bpc > 0) cap = std::min<uint64_t>(cap, static_cast<uint64_t>(h->tune.sort_bpc) * device_cus(h->device));
     const unsigned grid = static_cast<unsigned>(std::min<uint64_t>(cap, ntiles));
-    size_t zero_n = L.zero_words;
     if (h->tune.sort_xcd != 0) {
         a.groups = kXcds;
         // enough chunks that every XCD gets several
         a.chunk = static_cast<uint32_t>(std::max<uint64_t>(
             1, std::min<uint64_t>(static_cast<uint64_t>(h->tune.sort_chunk), ntiles / (kXcds * 8ull))));
-        a.xstride = static_cast<uint32_t>((ntiles + a.chunk - 1) / a.chunk);
-        require(static_cast<size_t>(D) * kXcds * a.xstride <= L.xmap_words, "XCD chunk map size");
-        zero_n += static_cast<size_t>(D) * kXcds * a.xstride;  // the maps follow the tickets and histograms
     } else {
         // tiles claimed in global order from one counter
         a.groups = 1;
         a.chunk = 1;
-        a.xstride = 0;
     }
 
     const size_t kb = sizeof(K);
     {
         LaunchTimer t(h, s, "fill", 0);
-        fill_u32(zero, zero_n, 0u, s);
+        fill_u32(zero, L.zero_words, 0u, s);
     }
     {
         LaunchTimer t(h, s, "radix_histogram", L.rows * kb);

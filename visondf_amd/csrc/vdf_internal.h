@@ -80,7 +80,7 @@ constexpr int kMaxDigits = 16;         // 128-bit keys
 constexpr uint32_t kFlagA = 1u << 30;  // look-back word: tile aggregate available
 constexpr uint32_t kFlagP = 2u << 30;  // look-back word: inclusive prefix available
 constexpr uint32_t kValMask = (1u << 30) - 1u;
-constexpr uint32_t kTicketStride = 16;   // words of tile-claim counters per pass slot (one per tile group)
+constexpr uint32_t kTicketStride = 32;   // 32-bit words of tile-claim counters per pass slot
 
 constexpr uint64_t kDefaultHistRows = 65536;  // rows per histogram work item
 
