@@ -40,13 +40,15 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBPS = 8000.0
 METRIC = "group-by rows/sec + HBM GB/s (% peak) at 100M rows; served p50/p99 latency"
 KERNELS = ("radix_histogram", "radix_plan", "radix_pass", "gather", "fill", "groupby_empty", "groupby_tile",
-           "groupby_fixup")
-WORKLOADS = ("argsort_u32", "groupby_c1", "groupby_i32", "groupby_i32_full", "zipf_groupby", "batched", "served")
+           "groupby_fixup", "csv_newlines", "csv_lines", "csv_measure", "csv_convert")
+WORKLOADS = ("argsort_u32", "groupby_c1", "groupby_i32", "groupby_i32_full", "zipf_groupby", "batched", "csv_ingest",
+             "served")
 # served is left out of the default run and refused by --workload: its
 # continuous-batching replay (three workers) stopped making progress with
 # batches in flight and the arena far from full (cause not found)
-DEFAULT_WORKLOADS = ("argsort_u32", "groupby_c1", "groupby_i32", "groupby_i32_full", "zipf_groupby", "batched")
-CHILD_TIMEOUT = {"argsort_u32": 600, "groupby_c1": 300, "groupby_i32": 400, "groupby_i32_full": 400, "zipf_groupby": 400, "batched": 500, "served": 500}
+DEFAULT_WORKLOADS = ("argsort_u32", "groupby_c1", "groupby_i32", "groupby_i32_full", "zipf_groupby", "batched",
+                     "csv_ingest")
+CHILD_TIMEOUT = {"argsort_u32": 600, "groupby_c1": 300, "groupby_i32": 400, "groupby_i32_full": 400, "zipf_groupby": 400, "batched": 500, "csv_ingest": 300, "served": 500}
 
 
 def _log(*a):
@@ -102,17 +104,19 @@ def _threads(args) -> int:
     return max(t, 1)
 
 
+TRAFFIC_SOURCES = ("radix.hip", "groupby.hip", "gather.hip")
+
+
 def _sources_hash() -> str:
-    """sha256 prefix over the kernel sources (visondf_amd/csrc/*.hip, *.h),
-    the stamp profiles/traffic.json records for the build it measured."""
-    import glob
+    """sha256 prefix over the sources of the kernels whose traffic
+    profiles/traffic.json records (TRAFFIC_SOURCES in visondf_amd/csrc), the
+    stamp that file keeps for the build it measured."""
     import hashlib
 
     h = hashlib.sha256()
-    csrc = os.path.join(ROOT, "visondf_amd", "csrc")
-    for p in sorted(glob.glob(os.path.join(csrc, "*.hip")) + glob.glob(os.path.join(csrc, "*.h"))):
-        h.update(os.path.basename(p).encode())
-        with open(p, "rb") as f:
+    for name in TRAFFIC_SOURCES:
+        h.update(name.encode())
+        with open(os.path.join(ROOT, "visondf_amd", "csrc", name), "rb") as f:
             h.update(f.read())
     return h.hexdigest()[:16]
 
@@ -429,6 +433,85 @@ def run_groupby_i32_full(args) -> dict:
     return run_groupby_i32(args, full_range=True)
 
 
+def run_csv_ingest(args) -> dict:
+    """CSV text resident in HBM (airline delay-cause shape: 21 columns,
+    quoted names holding the delimiter, two-decimal numbers; 5M rows, about
+    1 GB) parsed into typed device columns by vdf_read_csv; MB/s as the
+    reference's CSV benchmark reports it (benchmarks/readf.cpp:36-45)."""
+    import ctypes as C
+
+    import numpy as np
+    import torch
+
+    from visondf_amd import _native as N
+    from visondf_amd import datagen
+
+    nrows = args.rows or 5_000_000
+    _log(f"csv_ingest: generating {nrows} rows")
+    text = datagen.csv_airline(nrows)
+    nbytes = int(text.size)
+    pad = torch.zeros(nbytes + 16, dtype=torch.uint8)
+    pad[:nbytes] = torch.from_numpy(text)
+    d_text = pad.cuda()[:nbytes]
+    v = _handle(args)
+    opt = N.VdfCsvOptions(delim=b",", quote=b'"', trailing=b"0", reserved0=b"\0", header=1, int_bytes=8,
+                          char_width=0)
+    tables = []
+
+    def step():
+        t = C.c_void_p()
+        N.check(v.lib.vdf_read_csv(v.h, C.c_void_p(d_text.data_ptr()), nbytes, C.byref(opt), C.byref(t)))
+        tables.append(t)
+
+    ms_total, kernels = _time_steps(step, args.steps, args.warmup, v)
+    ms_step = ms_total / args.steps
+    mbps = nbytes * args.steps / (ms_total * 1e-3) / 2**20
+    # spot check of the last table against the oracle on the first 20000 rows
+    nrow, ncol = C.c_uint64(), C.c_int()
+    N.check(v.lib.vdf_csv_info(tables[-1], C.byref(nrow), C.byref(ncol)))
+    assert nrow.value == nrows and ncol.value == 21, (nrow.value, ncol.value)
+    import oracle
+
+    cut = int(np.nonzero(text == 10)[0][20_000]) + 1
+    o = oracle.read_csv(text[:cut], quote='"', threads=4)
+    for c in (0, 1, 6, 13, 20):  # numeric columns (text widths depend on all rows)
+        col = N.VdfCsvColumn()
+        N.check(v.lib.vdf_csv_column_get(tables[-1], c, C.byref(col)))
+        exp = o["cols"][c]
+        got = np.empty(exp.shape[0] * col.width, dtype=np.uint8)
+        from visondf_amd.server import _hip_lib
+
+        assert _hip_lib().hipMemcpy(got.ctypes.data, col.data, got.size, 2) == 0
+        assert np.array_equal(got, exp.view(np.uint8).reshape(-1) if exp.ndim == 1 else exp.reshape(-1)), c
+    for t in tables:
+        N.check(v.lib.vdf_csv_free(t))
+    rec = _base_record(args, mbps, "MB/s", ms_step, "u8",
+                       {"workload": "csv_ingest", "rows": nrows, "bytes": nbytes, "columns": 21,
+                        "text": "airline delay-cause shape: year, month, 2-char carrier, carrier name, 3-char "
+                                "airport, quoted airport name with a comma, 15 two-decimal numbers",
+                        "types": "2 uint64, 4 string, 15 float64 (inferred from the first 10 rows)",
+                        "parallelism": "single GPU"})
+    rec["kernels"] = kernels
+    rec["roofline"] = _roofline("csv_ingest", kernels)
+    if not args.no_cpu:
+        threads = _threads(args)
+        times = []
+        for r in range(3):
+            t0 = time.perf_counter()
+            oracle.read_csv(text, quote='"', threads=threads)
+            times.append(time.perf_counter() - t0)
+            _log(f"csv_ingest: cpu rep {r}: {times[-1]:.3f} s")
+        tc = float(np.median(times))
+        rec["cpu_baseline"] = {"value": nbytes / tc / 2**20, "unit": "MB/s", "cores": threads, "kind": "port",
+                               "host": _host(),
+                               "sample": f"full workload ({nbytes} bytes): the oracle's CSV ingest (newline "
+                                         f"index, field split, from_chars-prefix conversions through strtod), "
+                                         f"{threads} OpenMP threads, median of 3"}
+        rec["speedup_vs_cpu"] = mbps / rec["cpu_baseline"]["value"]
+    v.close()
+    return rec
+
+
 def run_groupby_c1(args) -> dict:
     """The CPU reference configuration: 1M rows, int64 key uniform in
     [0, 1000), float64 uniform [0, 1) and int32 uniform [-1e6, 1e6] values,
@@ -738,7 +821,7 @@ def run_served(args) -> dict:
 
 RUNNERS = {"argsort_u32": run_argsort_u32, "groupby_c1": run_groupby_c1, "groupby_i32": run_groupby_i32,
            "groupby_i32_full": run_groupby_i32_full, "zipf_groupby": run_zipf_groupby, "batched": run_batched,
-           "served": run_served}
+           "csv_ingest": run_csv_ingest, "served": run_served}
 
 
 def _child(name: str, args) -> dict:

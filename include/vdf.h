@@ -370,6 +370,8 @@ vdf_status vdf_groupby_batch(vdf_handle* h, int key_type, const vdf_table* table
  *   "hist_rows"      rows per digit-histogram work item: 4096 .. 2^24
  *   "gather_rows"    rows per thread of the gather kernel: 4, 8, 16
  *   "gather_nt"      1 = non-temporal loads of the gathered column
+ *   "csv_stage"      1 (default): vdf_read_csv's row kernels parse each wave's rows from a
+ *                    copy in LDS; 0: from the text in memory
  *   "groupby_carry"  1 (default): the group-by's sort carries value column 0 (4 or 8
  *                    bytes; columns 0 and 1 when both are 8 bytes) into key order, so
  *                    the aggregate kernel reads them sequentially; 0: every column is

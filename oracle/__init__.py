@@ -49,6 +49,15 @@ def lib() -> C.CDLL:
             getattr(L, name).argtypes = [vp, vp, sz, i32]
         L.orc_radix_sort_charbuf.restype = None
         L.orc_radix_sort_charbuf.argtypes = [vp, sz, sz, vp]
+        ch = C.c_char
+        L.orc_csv_infer.restype = i32
+        L.orc_csv_infer.argtypes = [vp, sz, ch, ch, i32, i32, vp, i32]
+        L.orc_csv_lines.restype = C.c_uint64
+        L.orc_csv_lines.argtypes = [vp, sz, vp, i32]
+        L.orc_csv_measure.restype = C.c_int64
+        L.orc_csv_measure.argtypes = [vp, sz, vp, C.c_uint64, ch, ch, i32, vp, vp, i32]
+        L.orc_csv_convert.restype = None
+        L.orc_csv_convert.argtypes = [vp, sz, vp, C.c_uint64, ch, ch, ch, i32, vp, vp, vp, i32]
         _lib = L
     return _lib
 
@@ -286,3 +295,73 @@ def groupby_multi(keys_list, vals=(), threads: int = 1) -> dict:
     first = r["perm"][r["offsets"][:-1].astype(np.int64)].astype(np.int64)
     r["keys"] = [k[first] for k in keys_list]
     return r
+
+
+CSV_STRING, CSV_CHAR, CSV_BOOL, CSV_INT, CSV_UINT, CSV_DOUBLE = range(6)
+
+
+def read_csv(text, delim: str = ",", quote: str = "'", header: bool = True, int_bytes: int = 8,
+             char_width: int = 0, trailing: str = "0", threads: int = 1) -> dict:
+    """The oracle's CSV ingest (orc_csv_*): returns {"names", "kinds",
+    "widths", "cols"}; cols[c] is a numpy array (float64 / int / uint32 for
+    bool) or, for string and char columns, an (nrow, width) uint8 array."""
+    t = np.frombuffer(bytes(text), dtype=np.uint8) if not isinstance(text, np.ndarray) else text
+    t = np.ascontiguousarray(t, dtype=np.uint8)
+    L = lib()
+    d, q, tr = delim.encode(), quote.encode(), trailing.encode()
+    kinds = np.zeros(65536, dtype=np.int32)
+    ncol = L.orc_csv_infer(_p(t), t.size, d, q, int(header), char_width, _p(kinds), kinds.size)
+    if ncol < 0:
+        raise ValueError("more than 65535 columns")
+    kinds = kinds[:ncol].copy()
+    line = np.zeros(int((t == 10).sum()) + 2, dtype=np.uint64)
+    nl = int(L.orc_csv_lines(_p(t), t.size, _p(line), threads))
+    first = 1 if header else 0
+    nrow = max(nl - first, 0)
+    lv = line[first:]
+    names = []
+    if ncol:
+        e = int(line[1]) - 1 if nl else t.size
+        hl = bytes(t[: min(e, t.size)]).rstrip(b"\r").decode("latin-1")
+        raw, cur, inq = [], "", False
+        for ch_ in hl:
+            if ch_ == quote:
+                inq = not inq
+                cur += ch_
+            elif ch_ == delim and not inq:
+                raw.append(cur)
+                cur = ""
+            else:
+                cur += ch_
+        raw.append(cur)
+        names = [(x if x else "NA") if header else "" for x in raw]
+    maxlen = np.zeros(max(ncol, 1), dtype=np.uint32)
+    bad = int(L.orc_csv_measure(_p(t), t.size, _p(lv), nrow, d, q, ncol, _p(kinds), _p(maxlen), threads)) \
+        if nrow else -1
+    if bad >= 0:
+        raise ValueError(f"data row {bad} does not have {ncol} fields")
+    widths = np.zeros(max(ncol, 1), dtype=np.uint32)
+    cols = []
+    for c in range(ncol):
+        k = int(kinds[c])
+        if k == CSV_DOUBLE:
+            w, a = 8, np.zeros(nrow, dtype=np.float64)
+        elif k == CSV_INT:
+            w, a = int_bytes, np.zeros(nrow, dtype=np.int64 if int_bytes == 8 else np.int32)
+        elif k == CSV_UINT:
+            w, a = int_bytes, np.zeros(nrow, dtype=np.uint64 if int_bytes == 8 else np.uint32)
+        elif k == CSV_BOOL:
+            w, a = 4, np.zeros(nrow, dtype=np.uint32)
+        elif k == CSV_CHAR:
+            w = char_width
+            a = np.zeros((nrow, w), dtype=np.uint8)
+        else:
+            w = max(int(maxlen[c]), 1)
+            a = np.zeros((nrow, w), dtype=np.uint8)
+        widths[c] = w
+        cols.append(a)
+    ptrs = (C.c_void_p * max(ncol, 1))(*[a.ctypes.data for a in cols])
+    if nrow:
+        L.orc_csv_convert(_p(t), t.size, _p(lv), nrow, d, q, tr, ncol, _p(kinds), _p(widths), ptrs, threads)
+    return {"names": names, "kinds": [int(k) for k in kinds], "widths": [int(w) for w in widths[:ncol]],
+            "cols": cols, "nrow": nrow}

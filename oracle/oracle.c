@@ -678,3 +678,329 @@ uint64_t orc_hash_groupby_f64(int key_type, const void* keys, size_t n,
     free(cnt);
     return ng;
 }
+
+/* ---- CSV ingest (read/read_csv/read_csv.hpp:9-216) ---- */
+
+/* calls f(ctx, column, begin, length) for each field of text[b, e) */
+typedef void (*csv_field_fn)(void* ctx, int c, const char* s, size_t len);
+
+static int csv_fields(const char* text, uint64_t b, uint64_t e, char delim, char quote, csv_field_fn f, void* ctx)
+{
+    if (e > b && text[e - 1] == '\r') --e;
+    int c = 0, inq = 0;
+    uint64_t fs = b;
+    for (uint64_t i = b; i < e; ++i) {
+        const char ch = text[i];
+        if (ch == quote) inq = !inq;
+        else if (ch == delim && !inq) {
+            f(ctx, c++, text + fs, i - fs);
+            fs = i + 1;
+        }
+    }
+    f(ctx, c, text + fs, e - fs);
+    return c + 1;
+}
+
+static int csv_digit(char c) { return c >= '0' && c <= '9'; }
+static char csv_lower(char c) { return (c >= 'A' && c <= 'Z') ? (char)(c - 'A' + 'a') : c; }
+
+/* [-]digits[.digits][(e|E)[+-]digits], digits on at least one side of the
+ * dot, the whole field (classify_column.inl's test, with '-' and exponents) */
+static int csv_is_number(const char* s, size_t n, int* dot_or_exp, int* neg)
+{
+    size_t i = 0, digits = 0;
+    *dot_or_exp = 0;
+    *neg = 0;
+    if (i < n && s[i] == '-') {
+        *neg = 1;
+        ++i;
+    }
+    while (i < n && csv_digit(s[i])) ++i, ++digits;
+    if (i < n && s[i] == '.') {
+        *dot_or_exp = 1;
+        ++i;
+        while (i < n && csv_digit(s[i])) ++i, ++digits;
+    }
+    if (digits == 0) return 0;
+    if (i < n && (s[i] == 'e' || s[i] == 'E')) {
+        size_t ed = 0;
+        *dot_or_exp = 1;
+        ++i;
+        if (i < n && (s[i] == '+' || s[i] == '-')) ++i;
+        while (i < n && csv_digit(s[i])) ++i, ++ed;
+        if (ed == 0) return 0;
+    }
+    return i == n;
+}
+
+typedef struct {
+    int want;
+    const char* s;
+    size_t len;
+    int found;
+} csv_pick;
+
+static void csv_pick_fn(void* ctx, int c, const char* s, size_t len)
+{
+    csv_pick* p = (csv_pick*)ctx;
+    if (c == p->want) {
+        p->s = s;
+        p->len = len;
+        p->found = 1;
+    }
+}
+
+static void csv_count_fn(void* ctx, int c, const char* s, size_t len)
+{
+    (void)ctx;
+    (void)c;
+    (void)s;
+    (void)len;
+}
+
+int orc_csv_infer(const char* text, size_t bytes, char delim, char quote, int header, int char_width, int* kinds,
+                  int max_cols)
+{
+    /* the first 12 lines (header + 10 data rows + one) */
+    uint64_t ls[13];
+    int nl = 0;
+    uint64_t fs = 0;
+    for (uint64_t i = 0; i < bytes && nl < 12; ++i)
+        if (text[i] == '\n') {
+            ls[nl++] = fs;
+            fs = i + 1;
+        }
+    uint64_t le[13];
+    for (int k = 0; k < nl; ++k) le[k] = (k + 1 < nl) ? ls[k + 1] - 1 : fs - 1;
+    if (nl < 12 && fs < bytes) {
+        ls[nl] = fs;
+        le[nl] = bytes;
+        ++nl;
+    }
+    if (nl == 0) return 0;
+    const int ncol = csv_fields(text, ls[0], le[0], delim, quote, csv_count_fn, NULL);
+    if (ncol > max_cols) return -1;
+    const int first = header ? 1 : 0;
+    for (int c = 0; c < ncol; ++c) {
+        int numeric = 1, any_dot = 0, any_neg = 0, boolean = 1, fits = 1, seen = 0;
+        for (int l = first; l < nl && seen < 10; ++l, ++seen) {
+            csv_pick p = {c, NULL, 0, 0};
+            csv_fields(text, ls[l], le[l], delim, quote, csv_pick_fn, &p);
+            const char* v = p.found ? p.s : "";
+            const size_t n = p.found ? p.len : 0;
+            int dot = 0, neg = 0;
+            if (!csv_is_number(v, n, &dot, &neg)) numeric = 0;
+            any_dot |= dot;
+            any_neg |= neg;
+            if (!(n == 1 && (v[0] == '0' || v[0] == '1'))) boolean = 0;
+            if (n > (size_t)char_width) fits = 0;
+        }
+        if (!numeric) kinds[c] = (fits && char_width > 0) ? 1 : 0;
+        else if (any_dot) kinds[c] = 5;
+        else if (any_neg) kinds[c] = 3;
+        else if (boolean && seen > 0) kinds[c] = 2;
+        else kinds[c] = 4;
+    }
+    return ncol;
+}
+
+uint64_t orc_csv_lines(const char* text, size_t bytes, uint64_t* line, int threads)
+{
+    if (threads < 1) threads = 1;
+    const size_t chunk = 1 << 20;
+    const size_t nb = (bytes + chunk - 1) / chunk;
+    uint64_t* cnt = (uint64_t*)calloc(nb + 1, sizeof(uint64_t));
+    #pragma omp parallel for num_threads(threads) schedule(static)
+    for (size_t b = 0; b < nb; ++b) {
+        uint64_t c = 0;
+        const size_t e = (b + 1) * chunk < bytes ? (b + 1) * chunk : bytes;
+        for (size_t i = b * chunk; i < e; ++i) c += text[i] == '\n';
+        cnt[b + 1] = c;
+    }
+    for (size_t b = 0; b < nb; ++b) cnt[b + 1] += cnt[b];
+    line[0] = 0;
+    #pragma omp parallel for num_threads(threads) schedule(static)
+    for (size_t b = 0; b < nb; ++b) {
+        uint64_t k = cnt[b];
+        const size_t e = (b + 1) * chunk < bytes ? (b + 1) * chunk : bytes;
+        for (size_t i = b * chunk; i < e; ++i)
+            if (text[i] == '\n') line[1 + k++] = i + 1;
+    }
+    uint64_t nl = cnt[nb];
+    free(cnt);
+    if ((nl ? line[nl] : 0) < bytes) line[++nl] = bytes + 1;  /* an unterminated last line */
+    return nl;
+}
+
+typedef struct {
+    int ncol;
+    const int* kinds;
+    uint32_t* maxlen;
+} csv_measure_ctx;
+
+static void csv_measure_fn(void* ctx, int c, const char* s, size_t len)
+{
+    (void)s;
+    csv_measure_ctx* m = (csv_measure_ctx*)ctx;
+    if (c < m->ncol && m->kinds[c] == 0 && len > m->maxlen[c]) m->maxlen[c] = (uint32_t)len;
+}
+
+int64_t orc_csv_measure(const char* text, size_t bytes, const uint64_t* line, uint64_t nrow, char delim,
+                        char quote, int ncol, const int* kinds, uint32_t* maxlen, int threads)
+{
+    if (threads < 1) threads = 1;
+    int64_t bad = -1;
+    for (int c = 0; c < ncol; ++c) maxlen[c] = 0;
+    #pragma omp parallel num_threads(threads)
+    {
+        uint32_t* mine = (uint32_t*)calloc((size_t)(ncol > 0 ? ncol : 1), sizeof(uint32_t));
+        csv_measure_ctx m = {ncol, kinds, mine};
+        int64_t bad_t = -1;
+        #pragma omp for schedule(static)
+        for (uint64_t r = 0; r < nrow; ++r) {
+            uint64_t e = line[r + 1] - 1;
+            if (e > bytes) e = bytes;
+            const int nf = csv_fields(text, line[r], e, delim, quote, csv_measure_fn, &m);
+            if (nf != ncol && (bad_t < 0 || (int64_t)r < bad_t)) bad_t = (int64_t)r;
+        }
+        #pragma omp critical
+        {
+            for (int c = 0; c < ncol; ++c)
+                if (mine[c] > maxlen[c]) maxlen[c] = mine[c];
+            if (bad_t >= 0 && (bad < 0 || bad_t < bad)) bad = bad_t;
+        }
+        free(mine);
+    }
+    return bad;
+}
+
+/* std::from_chars, base 10 */
+static int64_t csv_int64(const char* p, size_t n)
+{
+    size_t i = 0;
+    int neg = 0;
+    if (i < n && p[i] == '-') {
+        neg = 1;
+        ++i;
+    }
+    if (i >= n || !csv_digit(p[i])) return 0;
+    const uint64_t lim = neg ? 9223372036854775808ull : 9223372036854775807ull;
+    uint64_t v = 0;
+    for (; i < n && csv_digit(p[i]); ++i) {
+        const uint64_t d = (uint64_t)(p[i] - '0');
+        if (v > (lim - d) / 10) return 0;
+        v = v * 10 + d;
+    }
+    return neg ? (int64_t)(0 - v) : (int64_t)v;
+}
+
+static uint64_t csv_uint64(const char* p, size_t n)
+{
+    if (n == 0 || !csv_digit(p[0])) return 0;
+    uint64_t v = 0;
+    for (size_t i = 0; i < n && csv_digit(p[i]); ++i) {
+        const uint64_t d = (uint64_t)(p[i] - '0');
+        if (v > (18446744073709551615ull - d) / 10) return 0;
+        v = v * 10 + d;
+    }
+    return v;
+}
+
+/* fast_float::from_chars: the longest prefix [-](digits[.digits] | .digits)
+ * [(e|E)[+-]digits], or inf / infinity / nan in any case; strtod of that
+ * prefix (correctly rounded by glibc) */
+static double csv_double(const char* p, size_t n)
+{
+    size_t i = 0;
+    int neg = 0;
+    if (i < n && p[i] == '-') {
+        neg = 1;
+        ++i;
+    }
+    if (i < n && (csv_lower(p[i]) == 'i' || csv_lower(p[i]) == 'n')) {
+        if (i + 3 <= n && csv_lower(p[i]) == 'n' && csv_lower(p[i + 1]) == 'a' && csv_lower(p[i + 2]) == 'n')
+            return neg ? -NAN : NAN;
+        if (i + 3 <= n && csv_lower(p[i]) == 'i' && csv_lower(p[i + 1]) == 'n' && csv_lower(p[i + 2]) == 'f')
+            return neg ? -INFINITY : INFINITY;
+        return 0.0;
+    }
+    size_t digits = 0;
+    int dot = 0;
+    for (; i < n; ++i) {
+        if (p[i] == '.') {
+            if (dot) break;
+            dot = 1;
+            continue;
+        }
+        if (!csv_digit(p[i])) break;
+        ++digits;
+    }
+    if (digits == 0) return 0.0;
+    size_t end = i;
+    if (i < n && (p[i] == 'e' || p[i] == 'E')) {
+        size_t j = i + 1;
+        if (j < n && (p[j] == '-' || p[j] == '+')) ++j;
+        if (j < n && csv_digit(p[j])) {
+            while (j < n && csv_digit(p[j])) ++j;
+            end = j;
+        }
+    }
+    char small[128];
+    char* buf = end < sizeof small ? small : (char*)malloc(end + 1);
+    memcpy(buf, p, end);
+    buf[end] = '\0';
+    const double x = strtod(buf, NULL);
+    if (buf != small) free(buf);
+    return x;
+}
+
+typedef struct {
+    uint64_t r;
+    int ncol;
+    const int* kinds;
+    const uint32_t* widths;
+    void* const* outs;
+    char trailing;
+} csv_convert_ctx;
+
+static void csv_convert_fn(void* ctx, int c, const char* s, size_t len)
+{
+    const csv_convert_ctx* x = (const csv_convert_ctx*)ctx;
+    if (c >= x->ncol) return;
+    const uint32_t w = x->widths[c];
+    void* out = x->outs[c];
+    switch (x->kinds[c]) {
+    case 5: ((double*)out)[x->r] = csv_double(s, len); break;
+    case 3:
+        if (w == 8) ((int64_t*)out)[x->r] = csv_int64(s, len);
+        else ((uint32_t*)out)[x->r] = (uint32_t)csv_int64(s, len);
+        break;
+    case 4:
+        if (w == 8) ((uint64_t*)out)[x->r] = csv_uint64(s, len);
+        else ((uint32_t*)out)[x->r] = (uint32_t)csv_uint64(s, len);
+        break;
+    case 2: ((uint32_t*)out)[x->r] = csv_int64(s, len) != 0 ? 1u : 0u; break;
+    default: {
+        char* d = (char*)out + x->r * w;
+        const char pad = x->kinds[c] == 1 ? x->trailing : '\0';
+        const size_t m = len < w ? len : w;
+        memcpy(d, s, m);
+        memset(d + m, pad, w - m);
+        break;
+    }
+    }
+}
+
+void orc_csv_convert(const char* text, size_t bytes, const uint64_t* line, uint64_t nrow, char delim,
+                     char quote, char trailing, int ncol, const int* kinds, const uint32_t* widths,
+                     void* const* outs, int threads)
+{
+    if (threads < 1) threads = 1;
+    #pragma omp parallel for num_threads(threads) schedule(static)
+    for (uint64_t r = 0; r < nrow; ++r) {
+        csv_convert_ctx x = {r, ncol, kinds, widths, outs, trailing};
+        uint64_t e = line[r + 1] - 1;
+        if (e > bytes) e = bytes;
+        csv_fields(text, line[r], e, delim, quote, csv_convert_fn, &x);
+    }
+}

@@ -72,6 +72,36 @@ uint64_t orc_hash_groupby_f64(int key_type, const void* keys, size_t n,
 /* the reference's scalar key hash for keys of <= 16 bytes */
 uint64_t orc_hash_bytes16(const void* p, size_t len);
 
+/* ---- CSV ingest restated from read_csv (read/read_csv/read_csv.hpp:9-216) ----
+ * Rows are the text between newlines (a '\r' before a newline is dropped; a
+ * last line without a newline is a row); fields split at delim outside quote
+ * pairs and keep their quote characters.  Kinds (0 string, 1 char, 2 bool,
+ * 3 int, 4 uint, 5 double) come from the first 10 data rows
+ * (inlines/classify_column.inl:13-130, with signed and exponent numbers
+ * counted as numbers).  Conversions: the longest valid prefix in the grammar
+ * of std::from_chars / fast_float::from_chars, 0 when there is none or an
+ * integer is out of range; doubles through strtod of that prefix (correctly
+ * rounded).  OpenMP over rows.
+ *   orc_csv_infer   column kinds from the header and first rows; returns ncol
+ *                   (at most max_cols; -1 when more)
+ *   orc_csv_lines   line starts: line[0] = 0, line[k + 1] one past the k-th
+ *                   newline, and a sentinel bytes + 1 after an unterminated
+ *                   last line; returns the line count (line needs count + 2)
+ *   orc_csv_measure longest field of each string column (maxlen[c]) over the
+ *                   rows line[first ..]; returns the first row whose field
+ *                   count differs from ncol, or -1
+ *   orc_csv_convert every field into its column (outs[c]: nrow cells of
+ *                   widths[c] bytes; char cells padded with `trailing`,
+ *                   string cells with 0x00) */
+int orc_csv_infer(const char* text, size_t bytes, char delim, char quote, int header, int char_width,
+                  int* kinds, int max_cols);
+uint64_t orc_csv_lines(const char* text, size_t bytes, uint64_t* line, int threads);
+int64_t orc_csv_measure(const char* text, size_t bytes, const uint64_t* line, uint64_t nrow, char delim,
+                        char quote, int ncol, const int* kinds, uint32_t* maxlen, int threads);
+void orc_csv_convert(const char* text, size_t bytes, const uint64_t* line, uint64_t nrow, char delim,
+                     char quote, char trailing, int ncol, const int* kinds, const uint32_t* widths,
+                     void* const* outs, int threads);
+
 #ifdef __cplusplus
 }
 #endif

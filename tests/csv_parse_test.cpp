@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <random>
 #include <string>
 
@@ -24,12 +25,29 @@ static double conv(const std::string& s, int& status)
     return x;
 }
 
+static long g_fast = 0;
+
 static void check_double(const std::string& s)
 {
     int st = 0;
     const double got = conv(s, st);
     const double want = std::strtod(s.c_str(), nullptr);
     ++g_checked;
+    // the big-integer-free conversion: where it answers, the same bits
+    int fst = 0;
+    const double fast = parse_double_fast(s.data(), static_cast<int>(s.size()), fst);
+    if (fst == kParsed) {
+        ++g_fast;
+        if (bits_of(fast) != bits_of(got) && !(std::isnan(fast) && std::isnan(got))) {
+            if (g_fail < 20) std::printf("FAIL fast double '%s': got %.17g want %.17g\n", s.c_str(), fast, got);
+            ++g_fail;
+        }
+    } else if (fst != kNeedsWider || st == kNoNumber) {
+        if (fst != st) {
+            if (g_fail < 20) std::printf("FAIL fast status '%s': %d vs %d\n", s.c_str(), fst, st);
+            ++g_fail;
+        }
+    }
     if (st != kParsed || bits_of(got) != bits_of(want)) {
         if (!(std::isnan(got) && std::isnan(want))) {
             if (g_fail < 20)
@@ -141,6 +159,12 @@ int main()
                {"-", 0.0, kNoNumber}, {".", 0.0, kNoNumber}, {"", 0.0, kNoNumber}, {"1.2.3", 1.2, kParsed},
                {"inf", INFINITY, kParsed}, {"-Infinity", -INFINITY, kParsed}};
     for (const auto& c : pre) {
+        int fst = 0;
+        const double fx = parse_double_fast(c.s, static_cast<int>(std::strlen(c.s)), fst);
+        if (fst != c.st || bits_of(fx) != bits_of(c.v)) {
+            std::printf("FAIL fast prefix '%s': got %g status %d\n", c.s, fx, fst);
+            ++g_fail;
+        }
         int st = 0;
         const double x = conv(c.s, st);
         if (st != c.st || bits_of(x) != bits_of(c.v)) {
@@ -188,6 +212,10 @@ int main()
         std::printf("%d failures of %ld doubles checked\n", g_fail, g_checked);
         return 1;
     }
-    std::printf("csv_parse ok (%ld doubles)\n", g_checked);
+    if (g_fast < 50000) {
+        std::printf("FAIL only %ld doubles took the fast conversion\n", g_fast);
+        return 1;
+    }
+    std::printf("csv_parse ok (%ld doubles, %ld by the fast conversion)\n", g_checked, g_fast);
     return 0;
 }

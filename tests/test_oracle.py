@@ -485,3 +485,68 @@ def test_argsort_multi_vs_python_sorted(mode):
     if mode == 2:
         exp = sorted(range(n), key=lambda i: key[i])[::-1]
     assert oracle.argsort_multi([k1, k2], mode).tolist() == exp
+
+
+def _py_fields(line: str, delim: str, quote: str):
+    out, cur, inq = [], "", False
+    for ch in line:
+        if ch == quote:
+            inq = not inq
+            cur += ch
+        elif ch == delim and not inq:
+            out.append(cur)
+            cur = ""
+        else:
+            cur += ch
+    out.append(cur)
+    return out
+
+
+def test_oracle_csv_vs_python():
+    """The oracle's CSV ingest on airline-shaped text (quoted fields holding
+    the delimiter, two-decimal numbers) against a Python parse: str.split
+    into lines, a quote-aware field split, float() (correctly rounded) and
+    int(); plus from_chars prefix cases and integer range limits."""
+    from visondf_amd import datagen
+
+    t = datagen.csv_airline(3000, seed=4)
+    r = oracle.read_csv(t, quote='"', threads=3)
+    lines = bytes(t).decode().split("\n")
+    assert lines[-1] == ""
+    head, rows = lines[0], lines[1:-1]
+    assert r["nrow"] == len(rows) == 3000
+    assert r["names"] == _py_fields(head, ",", '"')
+    assert r["kinds"] == [oracle.CSV_UINT] * 2 + [oracle.CSV_STRING] * 4 + [oracle.CSV_DOUBLE] * 15
+    f = [_py_fields(x, ",", '"') for x in rows]
+    assert all(len(x) == 21 for x in f)
+    for c in range(21):
+        col = r["cols"][c]
+        if c < 2:
+            assert col.tolist() == [int(x[c]) for x in f]
+        elif c < 6:
+            w = max(len(x[c]) for x in f)
+            assert r["widths"][c] == w
+            exp = np.zeros((len(f), w), dtype=np.uint8)
+            for k, x in enumerate(f):
+                exp[k, :len(x[c])] = list(x[c].encode())
+            assert np.array_equal(col, exp), c
+        else:
+            assert np.array_equal(col.view(np.uint64), np.array([float(x[c]) for x in f]).view(np.uint64)), c
+    # prefixes (the first 10 rows make the column numeric)
+    vals = ["1.5"] * 10 + ["12abc", "1.5e", "1.5e+", "abc", "-", ".", "1.2.3", "inf", "-Infinity", ".5", "5.",
+                           "-.25e1", "1e400", "1e-400", "0x10", "+1"]
+    exp = [1.5] * 10 + [12.0, 1.5, 1.5, 0.0, 0.0, 0.0, 1.2, np.inf, -np.inf, 0.5, 5.0, -2.5, np.inf, 0.0, 0.0, 0.0]
+    r = oracle.read_csv(("x\n" + "\n".join(vals) + "\n").encode())
+    assert r["kinds"] == [oracle.CSV_DOUBLE]
+    assert np.array_equal(r["cols"][0].view(np.uint64), np.array(exp).view(np.uint64))
+    head = [str(v) for v in range(-5, 5)]
+    ints = ["7", "-3", "9223372036854775807", "-9223372036854775808", "9223372036854775808", "12x", "x", "-"]
+    r = oracle.read_csv(("k\n" + "\n".join(head + ints) + "\n").encode())
+    assert r["kinds"] == [oracle.CSV_INT]
+    assert r["cols"][0].tolist() == list(range(-5, 5)) + [7, -3, 2**63 - 1, -2**63, 0, 12, 0, 0]
+    # CRLF, an unterminated last line, bool / char kinds, a bad row
+    r = oracle.read_csv(b"a,b,c\r\n1,x,0\r\n0,yy,1\r\n1,z,1", char_width=2)
+    assert r["nrow"] == 3 and r["kinds"] == [oracle.CSV_BOOL, oracle.CSV_CHAR, oracle.CSV_BOOL]
+    assert r["cols"][0].tolist() == [1, 0, 1] and bytes(r["cols"][1][0]) == b"x0"
+    with pytest.raises(ValueError):
+        oracle.read_csv(b"a,b\n1,2\n3\n")

@@ -855,6 +855,7 @@ static int* tune_slot(Tuning& t, const std::string& n)
     if (n == "hist_rows") return &t.hist_rows;
     if (n == "gather_rows") return &t.gather_rows;
     if (n == "gather_nt") return &t.gather_nt;
+    if (n == "csv_stage") return &t.csv_stage;
     if (n == "groupby_carry") return &t.gb_carry;
     return nullptr;
 }

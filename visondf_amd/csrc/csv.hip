@@ -60,9 +60,10 @@ struct CsvArgs {
     const CsvCol* cols;
     uint32_t* maxlen;              // [ncol] longest field of each column
     unsigned long long* bad_row;   // first row whose field count differs (else ~0)
-    uint32_t* slow_count;          // queued wide conversions
+    uint32_t* slow_count;          // queued exact conversions
+    unsigned long long* slow_need; // fields the measure pass found needing the exact conversion
     uint64_t* slow;                // [slow_cap] (row << 16 | column)
-    uint32_t slow_cap;
+    uint64_t slow_cap;
     uint32_t* failed;              // conversions beyond the wide big integers
 };
 
@@ -88,52 +89,175 @@ __global__ __launch_bounds__(256) void k_nl_count(const char* text, uint64_t byt
     if (threadIdx.x == 0) counts[blockIdx.x] = s_sum[0] + s_sum[1] + s_sum[2] + s_sum[3];
 }
 
-// line[1 + k] = (position of the k-th newline) + 1
+// line[1 + k] = (position of the k-th newline) + 1.  A thread's 256 bytes
+// are read as 16 16-byte vectors (one read; every newline found in
+// registers), as k_nl_count reads them; the text's last partial 256 bytes
+// byte by byte.  The block scan of the counts is outside both branches: a
+// wave can hold threads of each, and the scan's shuffles and barriers need
+// every thread of the block.
 __global__ __launch_bounds__(256) void k_nl_write(const char* text, uint64_t bytes, const uint64_t* block_off,
                                                   uint64_t* line)
 {
     __shared__ uint32_t s_wsum[4];
     const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kChunk + static_cast<uint64_t>(threadIdx.x) * 256;
+    const bool full = base + 256 <= bytes;
+    // bit j of m[i]: byte 16 i + j of the thread's 256 bytes is a newline
+    uint32_t m[16];
     uint32_t c = 0;
-    for (uint64_t i = base; i < base + 256 && i < bytes; ++i) c += text[i] == '\n';
+    if (full) {
+        const auto* q = reinterpret_cast<const uint4*>(text + base);
+        uint4 v[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) v[i] = q[i];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const uint32_t w[4] = {v[i].x, v[i].y, v[i].z, v[i].w};
+            uint32_t bits = 0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) bits |= (((w[k] >> (8 * b)) & 0xFFu) == '\n' ? 1u : 0u) << (4 * k + b);
+            m[i] = bits;
+            c += static_cast<uint32_t>(__popc(bits));
+        }
+    } else {
+        for (uint64_t i = base; i < base + 256 && i < bytes; ++i) c += text[i] == '\n';
+    }
     const uint32_t pre = block_excl_scan(c, s_wsum);
     uint64_t k = block_off[blockIdx.x] + pre;
-    for (uint64_t i = base; i < base + 256 && i < bytes; ++i)
-        if (text[i] == '\n') line[1 + k++] = i + 1;
+    if (full) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            uint32_t bits = m[i];
+            while (bits) {
+                const int j = __ffs(bits) - 1;
+                bits &= bits - 1;
+                line[1 + k++] = base + 16 * i + j + 1;
+            }
+        }
+    } else {
+        for (uint64_t i = base; i < base + 256 && i < bytes; ++i)
+            if (text[i] == '\n') line[1 + k++] = i + 1;
+    }
 }
 
-// Walks row r's fields: f(column, begin, length) for each; returns the field count.
+// Walks row r's fields column by column: f(column, begin, length) for
+// columns 0 .. ncol-1 that the row has; returns the row's field count (ncol
+// + 1 when it has more).  The loop over columns is the same for every lane
+// of a wave (one row per lane), so each column's conversion runs once for
+// all lanes together; a lane only scans ahead to its next delimiter.  Text
+// byte i is src[i - off] (the text itself, or a copy in LDS).  A delimiter
+// inside a pair of quote characters does not split; a field never ends
+// inside quotes, so the quote state restarts at each field.
 template <typename F>
-__device__ inline int for_fields(const CsvArgs& a, uint64_t r, F f)
+__device__ inline int for_fields(const CsvArgs& a, const char* src, uint64_t off, uint64_t r, F f)
 {
     uint64_t b = a.line[r], e = a.line[r + 1] - 1;  // e: the newline (or one past the text)
     if (e > a.bytes) e = a.bytes;
-    if (e > b && a.text[e - 1] == '\r') --e;
-    int c = 0;
-    bool inq = false;
-    uint64_t fs = b;
-    for (uint64_t i = b; i < e; ++i) {
-        const char ch = a.text[i];
-        if (ch == a.quote) inq = !inq;
-        else if (ch == a.delim && !inq) {
-            f(c, fs, i - fs);
-            ++c;
-            fs = i + 1;
+    if (e > b && src[e - 1 - off] == '\r') --e;
+    uint64_t pos = b;
+    int nf = 0;
+    bool more = true;  // the row has a field starting at pos
+    for (int c = 0; c < a.ncol; ++c) {
+        if (more) {
+            uint64_t i = pos;
+            bool inq = false;
+            for (; i < e; ++i) {
+                const char ch = src[i - off];
+                if (ch == a.quote) inq = !inq;
+                else if (ch == a.delim && !inq) break;
+            }
+            f(c, pos, i - pos);
+            ++nf;
+            more = i < e;  // ended at a delimiter: another field follows
+            pos = i + 1;
         }
     }
-    f(c, fs, e - fs);
-    return c + 1;
+    return more ? nf + 1 : nf;
 }
 
+// Rows of one wave (lane = row) are parsed from a copy of their text in
+// LDS: the wave copies the span of its 64 rows with coalesced 4-byte loads,
+// so the per-row walks read LDS instead of issuing one scattered byte load
+// per character.  A span longer than kWaveText bytes is read from memory.
+constexpr uint32_t kWaveText = 16384;
+
+// Copies the text of rows [r0, r1) into buf; returns the text index of
+// buf[0], or ~0 when the span does not fit (every lane gets the same answer).
+__device__ inline uint64_t stage_rows(const CsvArgs& a, uint64_t r0, uint64_t r1, char* buf)
+{
+    if (r0 >= r1) return ~0ull;
+    const int lane = threadIdx.x & 63;
+    const uint64_t b = a.line[r0];
+    uint64_t e = a.line[r1];  // one past the last row's newline (a sentinel past an unterminated one)
+    if (e > a.bytes) e = a.bytes;
+    const uint64_t a0 = b & ~3ull;
+    const uint64_t nw = (e - a0 + 3) / 4;
+    if (nw * 4 > kWaveText) return ~0ull;
+    auto* bw = reinterpret_cast<uint32_t*>(buf);
+    const auto* tw = reinterpret_cast<const uint32_t*>(a.text + a0);  // the text is 16-byte aligned
+    for (uint64_t w = static_cast<uint64_t>(lane); w < nw; w += 64) {
+        const uint64_t gi = a0 + 4 * w;
+        if (gi + 4 <= a.bytes) {
+            bw[w] = tw[w];
+        } else {  // the text's last bytes: no read past its end
+            uint32_t x = 0;
+            for (uint64_t k = 0; k < 4 && gi + k < a.bytes; ++k)
+                x |= static_cast<uint32_t>(static_cast<uint8_t>(a.text[gi + k])) << (8 * k);
+            bw[w] = x;
+        }
+    }
+    return a0;
+}
+
+// Longest field of each text column: maxima kept in LDS per block (tables
+// of up to kMeasureCols columns), then one global atomic per column and
+// block; wider tables take a global atomic per field.
+constexpr int kMeasureCols = 1024;
+
+// Also counts the double fields that need the exact conversion (the queue
+// of k_csv_slow is sized from it).
+template <bool STAGE>
 __global__ __launch_bounds__(256) void k_csv_measure(CsvArgs a)
 {
+    __shared__ uint32_t s_max[kMeasureCols];
+    __shared__ uint32_t s_slow;
+    __shared__ __align__(16) char s_txt[STAGE ? 4 : 1][STAGE ? kWaveText : 16];
+    const bool local = a.ncol <= kMeasureCols;
+    if (local)
+        for (int c = threadIdx.x; c < a.ncol; c += 256) s_max[c] = 0;
+    if (threadIdx.x == 0) s_slow = 0;
     const uint64_t r = static_cast<uint64_t>(blockIdx.x) * 256 + threadIdx.x;
-    if (r >= a.nrow) return;
-    const int nf = for_fields(a, r, [&](int c, uint64_t, uint64_t len) {
-        if (c < a.ncol && a.cols[c].kind == VDF_CSV_STRING)
-            atomicMax(a.maxlen + c, static_cast<uint32_t>(std::min<uint64_t>(len, 0xFFFFFFFFull)));
-    });
-    if (nf != a.ncol) atomicMin(a.bad_row, static_cast<unsigned long long>(r));
+    const int wave = threadIdx.x >> 6;
+    const uint64_t w0 = static_cast<uint64_t>(blockIdx.x) * 256 + static_cast<uint64_t>(wave) * 64;
+    uint64_t off = ~0ull;
+    if constexpr (STAGE) off = stage_rows(a, w0, std::min<uint64_t>(w0 + 64, a.nrow), s_txt[wave]);
+    __syncthreads();
+    const char* src = off == ~0ull ? a.text : s_txt[STAGE ? wave : 0];
+    const uint64_t base = off == ~0ull ? 0 : off;
+    uint32_t slow = 0;
+    if (r < a.nrow) {
+        const int nf = for_fields(a, src, base, r, [&](int c, uint64_t fb, uint64_t len) {
+            if (c >= a.ncol) return;
+            const int kind = a.cols[c].kind;
+            if (kind == VDF_CSV_STRING) {
+                const uint32_t l = static_cast<uint32_t>(std::min<uint64_t>(len, 0xFFFFFFFFull));
+                if (local) atomicMax(s_max + c, l);
+                else atomicMax(a.maxlen + c, l);
+            } else if (kind == VDF_CSV_DOUBLE) {
+                int st = 0;
+                (void)csv::parse_double_fast(src + (fb - base), static_cast<int>(std::min<uint64_t>(len, 0x7FFFFFFFull)), st);
+                slow += st == csv::kNeedsWider ? 1u : 0u;
+            }
+        });
+        if (nf != a.ncol) atomicMin(a.bad_row, static_cast<unsigned long long>(r));
+    }
+    if (slow) atomicAdd(&s_slow, slow);
+    __syncthreads();
+    if (local)
+        for (int c = threadIdx.x; c < a.ncol; c += 256)
+            if (s_max[c]) atomicMax(a.maxlen + c, s_max[c]);
+    if (threadIdx.x == 0 && s_slow) atomicAdd(a.slow_need, static_cast<unsigned long long>(s_slow));
 }
 
 __device__ inline void put_int(void* out, uint64_t r, uint32_t width, uint64_t v)
@@ -142,21 +266,34 @@ __device__ inline void put_int(void* out, uint64_t r, uint32_t width, uint64_t v
     else static_cast<uint32_t*>(out)[r] = static_cast<uint32_t>(v);
 }
 
+// Every field into its column; doubles outside the fast conversion are
+// queued for k_csv_slow (no big integers here, so no scratch memory).
+template <bool STAGE>
 __global__ __launch_bounds__(256) void k_csv_convert(CsvArgs a)
 {
+    __shared__ __align__(16) char s_txt[STAGE ? 4 : 1][STAGE ? kWaveText : 16];
     const uint64_t r = static_cast<uint64_t>(blockIdx.x) * 256 + threadIdx.x;
+    const int wave = threadIdx.x >> 6;
+    const uint64_t w0 = static_cast<uint64_t>(blockIdx.x) * 256 + static_cast<uint64_t>(wave) * 64;
+    uint64_t off = ~0ull;
+    if constexpr (STAGE) {
+        off = stage_rows(a, w0, std::min<uint64_t>(w0 + 64, a.nrow), s_txt[wave]);
+        __syncthreads();
+    }
     if (r >= a.nrow) return;
-    for_fields(a, r, [&](int c, uint64_t fb, uint64_t len) {
+    const char* src = off == ~0ull ? a.text : s_txt[STAGE ? wave : 0];
+    const uint64_t base = off == ~0ull ? 0 : off;
+    for_fields(a, src, base, r, [&](int c, uint64_t fb, uint64_t len) {
         if (c >= a.ncol) return;
         const CsvCol col = a.cols[c];
-        const char* s = a.text + fb;
+        const char* s = src + (fb - base);
         const int n = static_cast<int>(std::min<uint64_t>(len, 0x7FFFFFFFull));
         switch (col.kind) {
         case VDF_CSV_DOUBLE: {
             int st = 0;
-            double x = csv::parse_double<kSmallWords>(s, n, st);
+            double x = csv::parse_double_fast(s, n, st);
             if (st == csv::kNeedsWider) {
-                const uint32_t k = atomicAdd(a.slow_count, 1u);
+                const uint64_t k = atomicAdd(reinterpret_cast<unsigned long long*>(a.slow_count), 1ull);
                 if (k < a.slow_cap) a.slow[k] = (r << 16) | static_cast<uint64_t>(c);
                 x = 0.0;
             } else if (st == csv::kNoNumber) {
@@ -180,17 +317,31 @@ __global__ __launch_bounds__(256) void k_csv_convert(CsvArgs a)
     });
 }
 
-__global__ __launch_bounds__(64) void k_csv_slow(CsvArgs a, uint32_t count)
+// The queued doubles: the exact conversion in big integers of W words.  The
+// first pass (kSmallWords) marks the entries it cannot finish (bit 63) and
+// counts them in *wide; the second (kWideWords, launched only when some are
+// marked) converts the marked ones and counts failures in *failed.
+template <int W, bool SECOND>
+__global__ __launch_bounds__(64) void k_csv_slow(CsvArgs a, uint64_t count, uint32_t* wide)
 {
-    const uint32_t k = blockIdx.x * 64 + threadIdx.x;
+    const uint64_t k = static_cast<uint64_t>(blockIdx.x) * 64 + threadIdx.x;
     if (k >= count) return;
-    const uint64_t r = a.slow[k] >> 16;
-    const int want = static_cast<int>(a.slow[k] & 0xFFFFu);
-    for_fields(a, r, [&](int c, uint64_t fb, uint64_t len) {
+    const uint64_t q = a.slow[k];
+    if (SECOND && !(q >> 63)) return;
+    const uint64_t r = (q & ~(1ull << 63)) >> 16;
+    const int want = static_cast<int>(q & 0xFFFFu);
+    for_fields(a, a.text, 0, r, [&](int c, uint64_t fb, uint64_t len) {
         if (c != want) return;
         int st = 0;
-        const double x = csv::parse_double<kWideWords>(a.text + fb, static_cast<int>(std::min<uint64_t>(len, 0x7FFFFFFFull)), st);
-        if (st == csv::kNeedsWider) atomicAdd(a.failed, 1u);
+        const double x = csv::parse_double<W>(a.text + fb, static_cast<int>(std::min<uint64_t>(len, 0x7FFFFFFFull)), st);
+        if (st == csv::kNeedsWider) {
+            if (SECOND) {
+                atomicAdd(a.failed, 1u);
+            } else {
+                a.slow[k] = q | (1ull << 63);
+                atomicAdd(wide, 1u);
+            }
+        }
         static_cast<double*>(a.cols[c].out)[r] = st == csv::kParsed ? x : 0.0;
     });
 }
@@ -354,9 +505,12 @@ vdf_status vdf_read_csv(vdf_handle* h, const void* d_text, uint64_t bytes, const
         VDF_HIP_CHECK(hipMalloc(&d_boff, nblk * 8));
         T->bufs.push_back(d_boff);
         if (bytes) {
-            hipLaunchKernelGGL(k_nl_count, dim3(static_cast<unsigned>(nblk)), dim3(256), 0, nullptr, text, bytes,
-                               static_cast<uint32_t*>(d_counts));
-            check_launch("k_nl_count");
+            {
+                LaunchTimer lt(h, nullptr, "csv_newlines", bytes);
+                hipLaunchKernelGGL(k_nl_count, dim3(static_cast<unsigned>(nblk)), dim3(256), 0, nullptr, text,
+                                   bytes, static_cast<uint32_t*>(d_counts));
+                check_launch("k_nl_count");
+            }
             VDF_HIP_CHECK(hipMemcpy(counts.data(), d_counts, nblk * 4, hipMemcpyDeviceToHost));
         }
         uint64_t nl = 0;
@@ -374,9 +528,12 @@ vdf_status vdf_read_csv(vdf_handle* h, const void* d_text, uint64_t bytes, const
         VDF_HIP_CHECK(hipMemcpy(d_line, &zero, 8, hipMemcpyHostToDevice));
         if (nl) {
             VDF_HIP_CHECK(hipMemcpy(d_boff, boff.data(), nblk * 8, hipMemcpyHostToDevice));
-            hipLaunchKernelGGL(k_nl_write, dim3(static_cast<unsigned>(nblk)), dim3(256), 0, nullptr, text, bytes,
-                               static_cast<const uint64_t*>(d_boff), static_cast<uint64_t*>(d_line));
-            check_launch("k_nl_write");
+            {
+                LaunchTimer lt(h, nullptr, "csv_lines", bytes + nl * 8);
+                hipLaunchKernelGGL(k_nl_write, dim3(static_cast<unsigned>(nblk)), dim3(256), 0, nullptr, text,
+                                   bytes, static_cast<const uint64_t*>(d_boff), static_cast<uint64_t*>(d_line));
+                check_launch("k_nl_write");
+            }
             VDF_HIP_CHECK(hipMemcpy(&last_nl_end, static_cast<uint64_t*>(d_line) + nl, 8, hipMemcpyDeviceToHost));
         }
         uint64_t nlines = nl;
@@ -421,16 +578,20 @@ vdf_status vdf_read_csv(vdf_handle* h, const void* d_text, uint64_t bytes, const
         a.cols = static_cast<const CsvCol*>(d_cols);
         a.maxlen = static_cast<uint32_t*>(d_maxlen);
         a.bad_row = static_cast<unsigned long long*>(d_misc);
-        a.slow_count = reinterpret_cast<uint32_t*>(static_cast<char*>(d_misc) + 8);
-        a.failed = reinterpret_cast<uint32_t*>(static_cast<char*>(d_misc) + 12);
+        a.slow_count = reinterpret_cast<uint32_t*>(static_cast<char*>(d_misc) + 8);  // 64-bit counter
+        a.failed = reinterpret_cast<uint32_t*>(static_cast<char*>(d_misc) + 16);
+        a.slow_need = reinterpret_cast<unsigned long long*>(static_cast<char*>(d_misc) + 24);
         const unsigned long long none = ~0ull;
         VDF_HIP_CHECK(hipMemcpy(a.bad_row, &none, 8, hipMemcpyHostToDevice));
-        VDF_HIP_CHECK(hipMemset(a.slow_count, 0, 8));
+        VDF_HIP_CHECK(hipMemset(static_cast<char*>(d_misc) + 8, 0, 24));
         VDF_HIP_CHECK(hipMemset(d_maxlen, 0, std::max(ncol, 1) * 4));
         VDF_HIP_CHECK(hipMemcpy(d_cols, cols.data(), ncol * sizeof(CsvCol), hipMemcpyHostToDevice));
         const unsigned rgrid = static_cast<unsigned>((nrow + 255) / 256);
+        const bool stage = h->tune.csv_stage != 0;
         if (nrow) {
-            hipLaunchKernelGGL(k_csv_measure, dim3(rgrid), dim3(256), 0, nullptr, a);
+            LaunchTimer lt(h, nullptr, "csv_measure", bytes + (nrow + 1) * 8);
+            if (stage) hipLaunchKernelGGL(k_csv_measure<true>, dim3(rgrid), dim3(256), 0, nullptr, a);
+            else hipLaunchKernelGGL(k_csv_measure<false>, dim3(rgrid), dim3(256), 0, nullptr, a);
             check_launch("k_csv_measure");
         }
         unsigned long long bad = none;
@@ -451,20 +612,35 @@ vdf_status vdf_read_csv(vdf_handle* h, const void* d_text, uint64_t bytes, const
             cols[c].out = p;
         }
         VDF_HIP_CHECK(hipMemcpy(d_cols, cols.data(), ncol * sizeof(CsvCol), hipMemcpyHostToDevice));
-        a.slow_cap = static_cast<uint32_t>(std::min<uint64_t>(std::max<uint64_t>(nrow * ncol, 1), 1u << 20));
+        unsigned long long need = 0;
+        VDF_HIP_CHECK(hipMemcpy(&need, a.slow_need, 8, hipMemcpyDeviceToHost));
+        a.slow_cap = std::max<uint64_t>(need, 1);
         void* d_slow = nullptr;
         VDF_HIP_CHECK(hipMalloc(&d_slow, a.slow_cap * 8ull));
         T->bufs.push_back(d_slow);
         a.slow = static_cast<uint64_t*>(d_slow);
         if (nrow) {
-            hipLaunchKernelGGL(k_csv_convert, dim3(rgrid), dim3(256), 0, nullptr, a);
+            uint64_t out_bytes = 0;
+            for (int c = 0; c < ncol; ++c) out_bytes += nrow * cols[c].width;
+            LaunchTimer lt(h, nullptr, "csv_convert", bytes + (nrow + 1) * 8 + out_bytes);
+            if (stage) hipLaunchKernelGGL(k_csv_convert<true>, dim3(rgrid), dim3(256), 0, nullptr, a);
+            else hipLaunchKernelGGL(k_csv_convert<false>, dim3(rgrid), dim3(256), 0, nullptr, a);
             check_launch("k_csv_convert");
         }
-        uint32_t nslow = 0;
-        VDF_HIP_CHECK(hipMemcpy(&nslow, a.slow_count, 4, hipMemcpyDeviceToHost));
-        require(nslow <= a.slow_cap, "vdf_read_csv: too many numbers needing the wide conversion");
+        uint64_t nslow = 0;
+        VDF_HIP_CHECK(hipMemcpy(&nslow, a.slow_count, 8, hipMemcpyDeviceToHost));
+        require(nslow == need, "vdf_read_csv: the queued exact conversions differ from the count");
         if (nslow) {
-            hipLaunchKernelGGL(k_csv_slow, dim3((nslow + 63) / 64), dim3(64), 0, nullptr, a, nslow);
+            uint32_t* wide = reinterpret_cast<uint32_t*>(static_cast<char*>(d_misc) + 20);
+            const dim3 sg(static_cast<unsigned>((nslow + 63) / 64));
+            {
+                LaunchTimer lt(h, nullptr, "csv_slow", nslow * 16);
+                hipLaunchKernelGGL((k_csv_slow<kSmallWords, false>), sg, dim3(64), 0, nullptr, a, nslow, wide);
+                check_launch("k_csv_slow");
+            }
+            uint32_t nwide = 0;
+            VDF_HIP_CHECK(hipMemcpy(&nwide, wide, 4, hipMemcpyDeviceToHost));
+            if (nwide) hipLaunchKernelGGL((k_csv_slow<kWideWords, true>), sg, dim3(64), 0, nullptr, a, nslow, wide);
             check_launch("k_csv_slow");
             uint32_t failed = 0;
             VDF_HIP_CHECK(hipMemcpy(&failed, a.failed, 4, hipMemcpyDeviceToHost));

@@ -331,6 +331,85 @@ VDF_HD inline double parse_double(const char* p, int n, int& status)
     return neg ? -x : x;
 }
 
+// The same conversion when it needs no big integers: fields without a
+// number, zero, inf and nan, and decimals of at most 15 significant digits
+// with |exponent| <= 22 (one exact operation).  Any other decimal reports
+// kNeedsWider (the caller converts it with parse_double).
+VDF_HD inline double parse_double_fast(const char* p, int n, int& status)
+{
+    status = kParsed;
+    int i = 0;
+    bool neg = false;
+    if (i < n && p[i] == '-') {
+        neg = true;
+        ++i;
+    }
+    if (i < n && (lower(p[i]) == 'i' || lower(p[i]) == 'n')) {
+        if (i + 3 <= n && lower(p[i]) == 'n' && lower(p[i + 1]) == 'a' && lower(p[i + 2]) == 'n')
+            return make_double(neg ? 0xFFF8000000000000ull : 0x7FF8000000000000ull);
+        if (i + 3 <= n && lower(p[i]) == 'i' && lower(p[i + 1]) == 'n' && lower(p[i + 2]) == 'f')
+            return make_double(neg ? 0xFFF0000000000000ull : 0x7FF0000000000000ull);
+        status = kNoNumber;
+        return 0.0;
+    }
+    uint64_t d = 0;      // the significant digits read, trailing zeros included (16 at most)
+    int sig = 0, nz = 0, exp_adj = 0;
+    bool any = false, dot = false, started = false;
+    for (; i < n; ++i) {
+        const char c = p[i];
+        if (c == '.') {
+            if (dot) break;
+            dot = true;
+            continue;
+        }
+        if (!is_digit(c)) break;
+        any = true;
+        if (!started && c == '0') {
+            if (dot) --exp_adj;
+            continue;
+        }
+        started = true;
+        ++sig;
+        nz = c == '0' ? nz + 1 : 0;
+        if (sig <= 16) d = d * 10 + static_cast<uint64_t>(c - '0');
+        if (dot) --exp_adj;
+    }
+    if (!any) {
+        status = kNoNumber;
+        return 0.0;
+    }
+    if (i < n && (p[i] == 'e' || p[i] == 'E')) {
+        int j = i + 1;
+        bool eneg = false;
+        if (j < n && (p[j] == '-' || p[j] == '+')) {
+            eneg = p[j] == '-';
+            ++j;
+        }
+        if (j < n && is_digit(p[j])) {
+            int64_t ev = 0;
+            while (j < n && is_digit(p[j])) {
+                if (ev < 100000000) ev = ev * 10 + (p[j] - '0');
+                ++j;
+            }
+            exp_adj += static_cast<int>(eneg ? -ev : ev);
+        }
+    }
+    if (sig == 0) return neg ? -0.0 : 0.0;
+    const int nd = sig - nz;
+    const int e10 = exp_adj + nz;
+    if (sig > 16 || nd > 15 || e10 < -22 || e10 > 22) {
+        status = kNeedsWider;
+        return 0.0;
+    }
+    for (int k = 0; k < nz; ++k) d /= 10;  // drop the trailing zeros (exact)
+    double p10 = 1.0;
+    const int aq = e10 < 0 ? -e10 : e10;
+    for (int k = 0; k < aq; ++k) p10 *= 10.0;  // 10^k is exact up to 1e22
+    const double lead = static_cast<double>(d);  // exact: at most 15 digits
+    const double x = e10 < 0 ? lead / p10 : lead * p10;
+    return neg ? -x : x;
+}
+
 // std::from_chars of a signed / unsigned 64-bit integer (base 10): optional
 // '-' (signed only), digits; 0 when there are no digits or the value is out
 // of range.

@@ -1194,6 +1194,7 @@ bool tune_supported(const char* name, int64_t v)
     if (n == "hist_rows") return v >= 4096 && v <= (1 << 24);
     if (n == "gather_rows") return v == 4 || v == 8 || v == 16;
     if (n == "gather_nt") return v == 0 || v == 1;
+    if (n == "csv_stage") return v == 0 || v == 1;
     if (n == "groupby_carry") return v == 0 || v == 1;
     return false;
 }

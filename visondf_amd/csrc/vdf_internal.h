@@ -138,6 +138,7 @@ struct Tuning {
     int hist_rows = 65536;   // rows per histogram work item
     int gather_rows = 8;     // rows per thread of the gather kernel
     int gather_nt = 0;       // non-temporal loads of the gathered column
+    int csv_stage = 1;       // CSV row kernels parse each wave's rows from a copy in LDS
     int gb_carry = 1;        // group-by: value columns carried through the sort (1) or gathered (0)
 };
 

@@ -38,6 +38,8 @@ def _gen():
         L.vdg_uniform01.argtypes = [u64, u64, u64, vp, i32]
         L.vdg_normal01.argtypes = [u64, u64, u64, vp, i32]
         L.vdg_zipf_ranks.argtypes = [u64, u64, u64, dbl, u64, vp, i32]
+        L.vdg_csv_airline.argtypes = [u64, u64, vp, u64, i32]
+        L.vdg_csv_airline.restype = u64
         _lib = L
     return _lib
 
@@ -232,3 +234,19 @@ def char16_keys(seed: int, col: int, n: int, distinct: int | None = None) -> np.
         raw[:, j] = ((a >> np.uint64(8 * j)) & np.uint64(0xFF)).astype(np.uint8)
         raw[:, 8 + j] = ((b >> np.uint64(8 * j)) & np.uint64(0xFF)).astype(np.uint8)
     return (raw % np.uint8(95) + np.uint8(32)).astype(np.uint8)
+
+
+def csv_airline(nrows: int, seed: int = 9) -> np.ndarray:
+    """CSV text (uint8 array) shaped like the US airline delay-cause table:
+    a header and `nrows` rows of 21 columns -- year, month, a 2-character
+    carrier code, carrier name, 3-letter airport code, a quoted airport name
+    holding a comma, and 15 two-decimal numbers.  Needs libvdf_datagen.so."""
+    L = _gen()
+    if L is None:
+        raise RuntimeError("csv_airline needs libvdf_datagen.so (make -C visondf_amd/csrc)")
+    cap = 1024 + nrows * 400
+    buf = np.empty(cap, dtype=np.uint8)
+    n = L.vdg_csv_airline(seed, nrows, buf.ctypes.data, cap, _threads())
+    if n == 0:
+        raise RuntimeError("csv_airline: buffer too small")
+    return buf[:n]
