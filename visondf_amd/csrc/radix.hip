@@ -1017,11 +1017,23 @@ static uint64_t hist_items(uint64_t n, uint64_t rows_per_item)
     return n == 0 ? 0 : (n + rows_per_item - 1) / rows_per_item;
 }
 
+// Rows per histogram work item for a sort of `rows` rows in all: the tuned
+// grain, or a smaller one (down to 4096 rows) when the grain would give
+// fewer than kMinHistItems items, so small sorts still spread over the GPU.
+constexpr uint64_t kMinHistItems = 512;
+static uint64_t hist_grain(uint64_t hist_rows, uint64_t rows)
+{
+    if (rows / hist_rows >= kMinHistItems) return hist_rows;
+    const uint64_t g = (rows / kMinHistItems + 255) / 256 * 256;
+    return std::min(hist_rows, std::max<uint64_t>(g, 4096));
+}
+
 static SortLayout sort_layout(int key_type, const uint64_t* n, int nsegs, uint64_t hist_rows, int pay_bytes = 0)
 {
     SortLayout L;
+    for (int s = 0; s < nsegs; ++s) L.rows += n[s];
+    hist_rows = hist_grain(hist_rows, L.rows);
     for (int s = 0; s < nsegs; ++s) {
-        L.rows += n[s];
         L.tiles += (n[s] + kMinTileRows - 1) / kMinTileRows;
         L.items += hist_items(n[s], hist_rows);
     }
@@ -1267,17 +1279,22 @@ static void launch_sort(Handle* h, const SortSeg* segs, int nsegs, uint64_t mlo,
     }
     require(pass != nullptr, "sort payload must be 0, 4, 8 or 16 bytes per row");
     const uint64_t tile_rows = static_cast<uint64_t>(kBlock) * items;
-    const uint64_t hist_rows = static_cast<uint64_t>(h->tune.hist_rows);
     // one segment: its row count in place (no host allocation on that path)
     uint64_t n_one = nsegs == 1 ? segs[0].n : 0;
     std::vector<uint64_t> ns_many;
+    uint64_t all_rows = n_one;
     if (nsegs != 1) {
         ns_many.resize(nsegs);
-        for (int i = 0; i < nsegs; ++i) ns_many[i] = segs[i].n;
+        for (int i = 0; i < nsegs; ++i) {
+            ns_many[i] = segs[i].n;
+            all_rows += segs[i].n;
+        }
     }
     const uint64_t* ns = nsegs == 1 ? &n_one : ns_many.data();
     const SortLayout L = sort_layout(sizeof(K) == 4 ? VDF_KEY_U32 : (sizeof(K) == 8 ? VDF_KEY_U64 : VDF_KEY_CHAR16),
-                                     ns, nsegs, hist_rows, pay_bytes);
+                                     ns, nsegs, static_cast<uint64_t>(h->tune.hist_rows), pay_bytes);
+    // the grain sort_layout counted the work items with
+    const uint64_t hist_rows = hist_grain(static_cast<uint64_t>(h->tune.hist_rows), all_rows);
 
     // ---- batch description: host-built and uploaded through pinned staging,
     // or (device_desc, one segment) written on the device by k_setup_one ----
