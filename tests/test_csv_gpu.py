@@ -171,3 +171,21 @@ def test_read_csv_airline_vs_oracle(vdf):
     _, rows = _mixed(5_003, 5)
     text = ("u,i,d,b,c,s\n" + "\n".join(rows) + "\n").encode()
     _vs_oracle(vdf.read_csv(text, char_width=16), oracle.read_csv(text, char_width=16, threads=4))
+
+
+def test_read_csv_field_edges_vs_oracle(vdf):
+    """Quoted fields holding the delimiter at either end, empty and quoted
+    empty fields, and rows of different lengths inside one wave's rows:
+    equal to the oracle; a row with an extra field and one with a missing
+    field are rejected."""
+    rows = []
+    for k in range(300):
+        rows.append(f'"{k},",{"" if k % 7 == 0 else k},"x{k % 3}"')
+        if k % 5 == 0:
+            rows.append(f'",{k}",{k * 1000},""')
+    text = ("a,b,c\n" + "\n".join(rows) + "\n").encode()
+    _vs_oracle(vdf.read_csv(text, quote='"', char_width=0), oracle.read_csv(text, quote='"', threads=2))
+    for bad in (b"a,b\n1,2\n3,4,5\n", b"a,b\n1,2\n" + b"3,4\n" * 100 + b"5\n"):
+        with pytest.raises(N.VdfError) as e:
+            vdf.read_csv(bad)
+        assert e.value.status == N.VDF_ERR_INVALID

@@ -550,3 +550,11 @@ def test_oracle_csv_vs_python():
     assert r["cols"][0].tolist() == [1, 0, 1] and bytes(r["cols"][1][0]) == b"x0"
     with pytest.raises(ValueError):
         oracle.read_csv(b"a,b\n1,2\n3\n")
+    with pytest.raises(ValueError):  # a row with an extra field
+        oracle.read_csv(b"a,b\n1,2\n3,4,5\n")
+    # quoted fields holding the delimiter at either end, an empty field, a quoted empty field
+    r = oracle.read_csv(b"s,t,u\n\",x\",\"y,\",\n\"\",z,1\n", quote='"')
+    assert r["kinds"] == [oracle.CSV_STRING, oracle.CSV_STRING, oracle.CSV_STRING]
+    assert [bytes(x).rstrip(b"\0") for x in r["cols"][0]] == [b'",x"', b'""']
+    assert [bytes(x).rstrip(b"\0") for x in r["cols"][1]] == [b'"y,"', b"z"]
+    assert [bytes(x).rstrip(b"\0") for x in r["cols"][2]] == [b"", b"1"]
