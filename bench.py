@@ -545,6 +545,17 @@ def run_groupby_c1(args) -> dict:
                         "values": ["float64 uniform [0,1)", "int32 uniform [-1e6,1e6]"],
                         "aggregates": "count,sum,mean,min,max", "groups": g, "parallelism": "single GPU"})
     rec["kernels"] = kernels
+    # the same group-by as a captured HIP-graph plan (a batch of this one
+    # table): a replay launches the whole sequence at once, which at this
+    # size is mostly launch latency
+    pb = v.prepare_groupby_batch([(d_key, [d_vf, d_vi])], aggs=N.AGG_ALL, want_perm=False).capture()
+    ms_graph, _ = _time_steps(pb.replay, args.steps, args.warmup)
+    rg = pb.result(0)
+    assert rg.n_groups == g and torch.equal(rg.counts, res["r"].counts[:g]), "graph replay differs"
+    for c in range(2):
+        assert torch.equal(rg.aggs[c]["sum"], res["r"].aggs[c]["sum"][:g]), "graph replay differs"
+    pb.close()
+    rec["graph_replay"] = {"ms_per_step": ms_graph / args.steps, "rows_per_s": n * args.steps / (ms_graph * 1e-3)}
     if not args.no_cpu:
         T = _threads(args)
         _cpu_groupby_legs(rec, rows_per_s, [(key, vf, vi)], [(key, vf, vi)], T, f"full workload ({n} rows)",
