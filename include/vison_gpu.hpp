@@ -507,10 +507,16 @@ class Dataframe {
     // classification of the first 10 rows).  IntT / UIntT columns are
     // int64_t / uint64_t (int_bytes 8), CharT cells Char16 (char_width 16,
     // padded with TrailingChar), bool columns uint32_t 0 / 1, other text
-    // columns std::string columns.
-    template <char TrailingChar = '0'>
+    // columns std::string columns.  The template parameters are the
+    // reference's (read_csv.hpp:3-8): the frame keeps data rows strt_row ..
+    // end_row - 1 (end_row 0: to the last row), as standard_parser.inl:18-58
+    // slices them; CORES, WARMING and MEM_CLEAN choose among its CPU parsers
+    // and have no effect here.
+    template <unsigned int strt_row = 0, unsigned int end_row = 0, unsigned int CORES = 1, bool WARMING = false,
+              bool MEM_CLEAN = false, char TrailingChar = '0'>
     void read_csv(const std::string& file_name, char delim = ',', bool header_name = true, char str_context = '\'')
     {
+        static_assert(end_row == 0 || strt_row < end_row, "read_csv: strt_row must be lower than end_row");
         if (!cols_.empty()) throw Error(VDF_ERR_INVALID, "read_csv: the frame already has columns");
         std::ifstream in(file_name, std::ios::binary | std::ios::ate);
         if (!in) throw Error(VDF_ERR_INVALID, "read_csv: cannot open " + file_name);
@@ -534,9 +540,12 @@ class Dataframe {
             vdf_csv_table* t;
             ~Free() { (void)vdf_csv_free(t); }
         } guard{t};
-        uint64_t nrow = 0;
+        uint64_t all = 0;
         int ncol = 0;
-        check(vdf_csv_info(t, &nrow, &ncol));
+        check(vdf_csv_info(t, &all, &ncol));
+        const uint64_t first = std::min<uint64_t>(strt_row, all);
+        const uint64_t last = end_row == 0 ? all : std::min<uint64_t>(end_row, all);
+        const uint64_t nrow = last > first ? last - first : 0;
         for (int k = 0; k < ncol; ++k) {
             vdf_csv_column vc{};
             check(vdf_csv_column_get(t, k, &vc));
@@ -556,7 +565,8 @@ class Dataframe {
             }
             c.data = DeviceBuffer(static_cast<size_t>(nrow) * c.elem);
             if (nrow)
-                hip_check(hipMemcpy(c.data.get(), vc.data, static_cast<size_t>(nrow) * c.elem, hipMemcpyDeviceToDevice),
+                hip_check(hipMemcpy(c.data.get(), static_cast<const char*>(vc.data) + first * c.elem,
+                                    static_cast<size_t>(nrow) * c.elem, hipMemcpyDeviceToDevice),
                           "hipMemcpy D2D");
             cols_.push_back(std::move(c));
         }

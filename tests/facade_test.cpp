@@ -397,6 +397,16 @@ int main()
             close(fd);
             Dataframe df;
             df.read_csv(path);
+            // the reference's template parameters: a row range, and the CPU
+            // parser options (no effect here)
+            Dataframe part;
+            part.read_csv<100, 350, 4, false, false, '0'>(path, ',', true, '\'');
+            CHECK(part.get_nrow() == 250 && part.get_ncol() == 5);
+            CHECK(part.get_col<uint64_t>(0) == std::vector<uint64_t>(key.begin() + 100, key.begin() + 350));
+            Dataframe tail;
+            tail.read_csv<50000>(path);
+            CHECK(tail.get_nrow() == m - 50000);
+            CHECK(tail.get_col<int64_t>(1) == std::vector<int64_t>(iv.begin() + 50000, iv.end()));
             std::remove(path);
             CHECK(df.get_nrow() == m && df.get_ncol() == 5);
             CHECK(df.get_colname(0) == "key" && df.get_colname(4) == "note");
