@@ -39,7 +39,8 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0
 METRIC = "group-by rows/sec + HBM GB/s (% peak) at 100M rows; served p50/p99 latency"
-KERNELS = ("radix_histogram", "radix_plan", "radix_pass", "gather", "fill", "groupby_empty", "groupby_tile",
+KERNELS = ("radix_histogram", "radix_plan", "radix_pass", "radix_local_plan", "radix_local", "gather", "fill",
+           "groupby_empty", "groupby_tile",
            "groupby_fixup", "csv_newlines", "csv_lines", "csv_measure", "csv_convert")
 WORKLOADS = ("argsort_u32", "groupby_c1", "groupby_i32", "groupby_i32_full", "zipf_groupby", "batched", "csv_ingest",
              "served")

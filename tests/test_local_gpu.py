@@ -1,0 +1,196 @@
+"""GPU parity of the local finish of 4-byte-key sorts: two global radix passes
+over the top two non-constant digits, then each bucket of equal top digits
+sorted in LDS.  Every case is checked bit-exact against the oracle's stable
+argsort (the reference's radix_sort_uint32 order, radix_sort_uint32.inl) and
+against the same call with the local finish switched off; the profile shows
+which path ran (the local finish, or its fallback to the plain passes)."""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from tests.test_radix_gpu import _dev, _key_type, _perm_host
+from visondf_amd import _native as N
+from visondf_amd import datagen
+
+pytestmark = pytest.mark.gpu
+
+MODES = {"asc": 0, "desc": 1, "reverse": 2}
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _u32(seed: int, n: int, bits: int = 32) -> np.ndarray:
+    return (datagen.splitmix64(seed, 0, n) >> np.uint64(64 - bits)).astype(np.uint32)
+
+
+def _vals(n: int, width: int, seed: int) -> np.ndarray:
+    rng = np.random.default_rng(seed)
+    if width == 8:
+        return rng.standard_normal(n)
+    return rng.integers(-(2**31), 2**31, n, dtype=np.int64).astype(np.int32)
+
+
+def _run(vdf, local: int, fn):
+    old = vdf.tune_get("sort_local")
+    vdf.profile(True)
+    vdf.profile_reset()
+    try:
+        vdf.tune("sort_local", local)
+        out = fn()
+        _, launches, _ = vdf.profile_read("radix_local")
+        _, passes, _ = vdf.profile_read("radix_pass")
+    finally:
+        vdf.tune("sort_local", old)
+        vdf.profile(False)
+        vdf.profile_reset()
+    return out, launches, passes
+
+
+def _sort_by(vdf, keys, vals, mode, perm64=False, want_perm=True):
+    return vdf.sort_by(_dev(keys), _dev(vals), key_type=_key_type(keys), descending=(mode == "desc"),
+                       reverse_desc=(mode == "reverse"), perm64=perm64, want_perm=want_perm, want_keys=True)
+
+
+def _check_sort_by(vdf, keys, width, mode="asc", perm64=False, want_perm=True, expect_local=True):
+    vals = _vals(keys.shape[0], width, keys.shape[0] + width)
+    exp = oracle.argsort(keys, MODES[mode], 4).astype(np.int64)
+    (perm, out, skeys), launches, passes = _run(vdf, 1, lambda: _sort_by(vdf, keys, vals, mode, perm64, want_perm))
+    assert launches == (1 if expect_local else 0), f"local finish ran {launches} times"
+    if expect_local:
+        assert passes == 2
+    if want_perm:
+        assert np.array_equal(_perm_host(perm).astype(np.int64), exp)
+    assert np.array_equal(out.cpu().numpy().view(np.uint8), vals[exp].view(np.uint8))
+    assert np.array_equal(skeys.cpu().numpy().view(np.uint8), keys[exp].view(np.uint8))
+    # the same call through the plain passes
+    (perm0, out0, skeys0), launches0, _ = _run(vdf, 0, lambda: _sort_by(vdf, keys, vals, mode, perm64, want_perm))
+    assert launches0 == 0
+    import torch
+
+    if want_perm:
+        assert torch.equal(perm0, perm)
+    assert torch.equal(out0.view(torch.uint8), out.view(torch.uint8))
+    assert torch.equal(skeys0.view(torch.uint8), skeys.view(torch.uint8))
+
+
+@pytest.mark.parametrize("mode", ["asc", "desc", "reverse"])
+@pytest.mark.parametrize("width", [4, 8])
+def test_local_sort_by_uniform(vdf, mode, width):
+    for n in (70_001, 1_000_003, 3_000_017):
+        _say("local sort_by uniform", mode, width, n)
+        _check_sort_by(vdf, _u32(n, n), width, mode)
+
+
+def test_local_sort_by_key_types(vdf):
+    n = 1_500_007
+    rng = np.random.default_rng(5)
+    i32 = rng.integers(-(2**31), 2**31, n, dtype=np.int64).astype(np.int32)
+    _check_sort_by(vdf, i32, 8)
+    f32 = (rng.standard_normal(n) * 1e4).astype(np.float32)
+    sp = np.array([np.nan, np.copysign(np.nan, -1), np.inf, -np.inf, 0.0, -0.0, 1e-40, -1e-40], dtype=np.float32)
+    f32[rng.integers(0, n, 5000)] = sp[rng.integers(0, sp.shape[0], 5000)]
+    _check_sort_by(vdf, f32, 8)
+    _check_sort_by(vdf, f32, 4, "reverse")
+
+
+def test_local_sort_by_perm64_and_without_perm(vdf):
+    keys = _u32(31, 2_000_003)
+    _check_sort_by(vdf, keys, 8, perm64=True)
+    _check_sort_by(vdf, keys, 4, perm64=True)
+    _check_sort_by(vdf, keys, 8, want_perm=False)
+
+
+@pytest.mark.parametrize("bits", [17, 20, 24, 28])
+def test_local_three_digit_keys(vdf, bits):
+    """Keys with three non-constant digits: one digit sorted in LDS (plus the
+    top digits that vary inside a chunk)."""
+    n = 1_200_011
+    _say("local bits", bits)
+    _check_sort_by(vdf, _u32(bits, n, bits), 8)
+
+
+def test_local_argsort(vdf):
+    import torch
+
+    for n in (65_536, 999_999, 2_500_001):
+        keys = _u32(n + 1, n)
+        exp = oracle.argsort(keys, 0, 4).astype(np.uint32)
+        (perm, skeys), launches, _ = _run(vdf, 1, lambda: vdf.argsort(_dev(keys), key_type=N.KEY_U32, want_keys=True))
+        assert launches == 1
+        assert np.array_equal(_perm_host(perm), exp)
+        assert np.array_equal(skeys.cpu().numpy().view(np.uint32), keys[exp.astype(np.int64)])
+        (perm0, _), _, _ = _run(vdf, 0, lambda: vdf.argsort(_dev(keys), key_type=N.KEY_U32, want_keys=True))
+        assert torch.equal(perm0, perm)
+
+
+def _equal_buckets(n_per: int, seed: int) -> np.ndarray:
+    """256 buckets of exactly n_per rows: digits 2 and 3 equal (so each digit
+    alone looks uniform), the low 16 bits random."""
+    n = 256 * n_per
+    top = (np.arange(n, dtype=np.uint64) % 256).astype(np.uint32)
+    rng = np.random.default_rng(seed)
+    rng.shuffle(top)
+    low = _u32(seed, n, 16)
+    return (top << np.uint32(24)) | (top << np.uint32(16)) | low
+
+
+def test_local_bucket_size_limit(vdf):
+    """Buckets of exactly 3840 rows (4096 - 256, the largest a chunk takes)
+    finish locally; 3841-row buckets fall back to the plain passes, which then
+    read the caller's columns again and must give the same result."""
+    _say("local limit 3840")
+    _check_sort_by(vdf, _equal_buckets(3840, 1), 8)
+    _say("local fallback 3841")
+    _check_sort_by(vdf, _equal_buckets(3841, 2), 8, expect_local=False)
+    _check_sort_by(vdf, _equal_buckets(3841, 3), 4, "reverse", expect_local=False)
+
+
+def test_local_skewed_keys_stay_plain(vdf):
+    """A heavy key makes the digit histograms rule the local finish out
+    before any pass runs (no wasted passes)."""
+    n = 1_000_003
+    keys = _u32(77, n)
+    keys[::3] = 123456789
+    (perm, out, sk), launches, passes = _run(vdf, 1, lambda: _sort_by(vdf, keys, _vals(n, 8, 1), "asc"))
+    assert launches == 0 and passes == 4
+    exp = oracle.argsort(keys, 0, 4).astype(np.int64)
+    assert np.array_equal(_perm_host(perm).astype(np.int64), exp)
+
+
+@pytest.mark.parametrize("combo", ["f64+i32", "f64+f64", "i32", "f64"])
+def test_local_groupby_full_range(vdf, combo):
+    """Group-bys over full-range int32 keys sort with the local finish (the
+    carried columns ride through it); results equal the plain path's and the
+    oracle's."""
+    import torch
+
+    n = 1_000_003
+    keys = _u32(9, n).view(np.int32)
+    rng = np.random.default_rng(4)
+    cols = []
+    for c in combo.split("+"):
+        cols.append(rng.standard_normal(n) if c == "f64" else rng.integers(-10**6, 10**6, n).astype(np.int32))
+    d_cols = [_dev(c) for c in cols]
+    for want_perm in (False, True):
+        r1, launches, _ = _run(vdf, 1, lambda: vdf.groupby(_dev(keys), d_cols, want_perm=want_perm))
+        assert launches == 1
+        r0, _, _ = _run(vdf, 0, lambda: vdf.groupby(_dev(keys), d_cols, want_perm=want_perm))
+        assert r1.n_groups == r0.n_groups
+        assert torch.equal(r1.keys, r0.keys) and torch.equal(r1.counts, r0.counts)
+        if want_perm:
+            assert torch.equal(r1.perm, r0.perm)
+        for a1, a0 in zip(r1.aggs, r0.aggs):
+            for k in a1:
+                if a1[k] is not None:
+                    assert torch.equal(a1[k].view(torch.uint8), a0[k].view(torch.uint8)), k
+    o = oracle.groupby(keys, cols)
+    assert r1.n_groups == o["n_groups"]
+    assert np.array_equal(r1.keys.cpu().numpy(), o["keys"])
+    assert np.array_equal(r1.counts.cpu().numpy().astype(np.uint64), o["counts"])

@@ -196,6 +196,8 @@ def test_profile_counts_active_passes(vdf):
     keys = (datagen.splitmix64(9, 0, 1_000_000) >> np.uint64(32)).astype(np.uint32)
     k24 = keys >> np.uint32(8)
     d_keys, d_k24 = _dev(keys), _dev(k24)
+    old_local = vdf.tune_get("sort_local")
+    vdf.tune("sort_local", 0)  # the plain passes (the local finish: tests/test_local_gpu.py)
     vdf.profile(True)
     vdf.profile_reset()
     vdf.argsort(d_keys, key_type=N.KEY_U32)
@@ -214,6 +216,16 @@ def test_profile_counts_active_passes(vdf):
     _, launches, nbytes = vdf.profile_read("radix_pass")
     assert launches == 4
     assert nbytes == 300_000 * (12 + 12) + n * (12 + 16 + 16 + 12)
+    # local finish: two passes, then the LDS finish reads (key, row id) records
+    # and writes the permutation
+    vdf.tune("sort_local", 1)
+    vdf.profile_reset()
+    vdf.argsort(d_keys, key_type=N.KEY_U32)
+    _, launches, nbytes = vdf.profile_read("radix_pass")
+    assert launches == 2 and nbytes == n * (12 + 16)
+    _, launches, nbytes = vdf.profile_read("radix_local")
+    assert launches == 1 and nbytes == n * (8 + 4)
+    vdf.tune("sort_local", old_local)
     vdf.profile(False)
     vdf.profile_reset()
 

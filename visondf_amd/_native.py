@@ -106,7 +106,7 @@ class VdfKeyCol(C.Structure):
 
 class VdfTuneReport(C.Structure):
     _fields_ = [("trials", C.c_int), ("sort_items", C.c_int), ("sort_blocks_per_cu", C.c_int), ("sort_xcd", C.c_int),
-                ("sort_chunk", C.c_int), ("hist_rows", C.c_int), ("digit_bits", C.c_int), ("reserved", C.c_int),
+                ("sort_chunk", C.c_int), ("hist_rows", C.c_int), ("digit_bits", C.c_int), ("sort_local", C.c_int),
                 ("default_ms", C.c_double), ("best_ms", C.c_double)]
 
 

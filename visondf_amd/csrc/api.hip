@@ -189,10 +189,22 @@ void Handle::prof_collect()
             bytes = 0;
             active = false;
             for (size_t i = 0; i < pa.rows.size(); ++i) {
-                const uint32_t D = pa.passes[i];
-                if (s >= D) continue;
+                // slots from kLocalSlot0 on, and the local finish itself, run by
+                // the plan as the local finish's size check left it
+                const bool late = pa.passes2 && (s == kProfLocalSlot || s >= static_cast<uint32_t>(kLocalSlot0));
+                const uint32_t* w = (late ? pa.passes2 : pa.passes) + 2 * i;
+                const uint32_t end = w[0], mode = w[1], s0 = mode & 0xFFu;
+                const bool local = (mode & kModeLocal) != 0;
+                if (s == kProfLocalSlot) {
+                    if (!local) continue;
+                    active = true;
+                    bytes += pa.rows[i] * pa.per_row_local;
+                    continue;
+                }
+                if (s >= end || s < s0) continue;
                 active = true;
-                const int pos = D == 1 ? 0 : s == 0 ? 1 : s == D - 1 ? 3 : 2;
+                const bool first = s == s0, last = s == end - 1 && !local;
+                const int pos = first && last ? 0 : first ? 1 : last ? 3 : 2;
                 bytes += pa.rows[i] * pa.per_row[pos];
             }
         }
@@ -271,6 +283,7 @@ vdf_status vdf_create(vdf_handle** out, int device, uint64_t hbm_budget_bytes)
             VDF_HIP_CHECK(hipGetDevice(&prev));
             if (prev != device) VDF_HIP_CHECK(hipSetDevice(device));
             noalloc_scratch_setup(device);
+            (void)lds_rank_check(device);
             if (prev != device) VDF_HIP_CHECK(hipSetDevice(prev));
         }
         auto* h = new vdf_handle();
@@ -856,6 +869,7 @@ static int* tune_slot(Tuning& t, const std::string& n)
     if (n == "gather_rows") return &t.gather_rows;
     if (n == "gather_nt") return &t.gather_nt;
     if (n == "csv_stage") return &t.csv_stage;
+    if (n == "sort_local") return &t.sort_local;
     if (n == "groupby_carry") return &t.gb_carry;
     return nullptr;
 }

@@ -191,7 +191,34 @@ struct HistItem {
     uint64_t begin, end;  // local rows
 };
 
-constexpr int kPlanHdr = 1 + kMaxDigits;  // passes + digit_of[]
+// Per-segment plan header (words):
+//   kHdrP     pass slots in use: slots [first, P) scatter
+//   kHdrMode  first slot (bits 0..7); bit 8: local finish (the last global
+//             pass writes to scratch and k_local writes the outputs)
+//   kHdrDig   digit of each slot
+//   kHdrNS, kHdrS  the plain plan (non-constant digits, ascending), kept for
+//             the fallback when a local finish does not fit
+//   kHdrGa, kHdrGb  the two digits a local finish scatters globally
+constexpr int kHdrP = 0;
+constexpr int kHdrMode = 1;
+constexpr int kHdrDig = 2;
+constexpr int kHdrNS = kHdrDig + kMaxDigits;
+constexpr int kHdrS = kHdrNS + 1;
+constexpr int kHdrGa = kHdrS + 4;
+constexpr int kHdrGb = kHdrGa + 1;
+constexpr int kPlanHdr = 32;
+static_assert(kHdrGb < kPlanHdr, "plan header fields");
+
+// Local finish (4-byte keys, one segment): two global passes over the two
+// most significant non-constant digits leave every bucket of equal top digits
+// contiguous; k_local then sorts chunks of whole buckets of at most
+// kLocalRows rows in LDS.  A bucket larger than kLocalRows - kLocalMinH
+// rows makes k_local_chunks restart the plain passes at slot 2.
+constexpr uint32_t kLocalRows = 4096;
+constexpr int kLocalBlock = 512;  // threads of a k_local block (8 waves)
+constexpr uint32_t kLocalMinH = 256;  // smallest chunk stride
+constexpr uint64_t kLocalMinSortRows = 1u << 16;  // smallest sort that may finish locally
+constexpr int kLocalMaxSlots = kLocalSlot0 + 4;
 // Ticket words of a pass slot: one 64-bit claim word per XCD (kXcds), then
 // the slot's chunk counter (also the tile counter of global order).  The last
 // word of slot 0 is free: k_plan keeps the largest pass count of the batch
@@ -224,6 +251,11 @@ struct SortArgs {
     uint32_t perm64;
     uint32_t reverse;
     uint32_t norows;           // 1: no segment needs its permutation; row ids are not moved
+    uint32_t local_ok;         // k_plan may choose a local finish (one segment, 4-byte keys)
+    uint32_t lds_rank;         // LDS atomics serve lanes in lane order on this device (lds_rank_check)
+    uint32_t* lstart;          // local finish: [256][257] start of bucket (y, x) (x = 256: end of y)
+    uint32_t* lchunk;          // local finish: chunk start rows (chunks + 1 words)
+    uint32_t* lctl;            // local finish: [0] chunk count
 };
 
 // Adds one key per lane to the block histogram.  Digits flagged in `shared`
@@ -335,16 +367,18 @@ __global__ __launch_bounds__(kBlock) void k_plan(SortArgs a)
 {
     __shared__ uint32_t s_wsum[kWaves];
     __shared__ uint32_t s_triv[kMaxDigits];
+    __shared__ uint32_t s_max[kMaxDigits];  // largest bucket of each digit
     const uint32_t sgi = blockIdx.x;
     const int t = threadIdx.x;
     const uint64_t n = a.segs[sgi].n;
     const uint32_t* h = a.hist + static_cast<uint64_t>(sgi) * D * kRadix;
     uint32_t* base = a.plan_base + static_cast<uint64_t>(sgi) * D * kRadix;
-    if (t < kMaxDigits) s_triv[t] = 0;
+    if (t < kMaxDigits) s_triv[t] = s_max[t] = 0;
     __syncthreads();
     for (int d = 0; d < D; ++d) {
         const uint32_t c = h[d * kRadix + t];
         if (c == n) s_triv[d] = 1;
+        if (D == 4) atomicMax(&s_max[d], c);
         base[d * kRadix + t] = block_excl_scan(c, s_wsum);
     }
     __syncthreads();
@@ -352,12 +386,31 @@ __global__ __launch_bounds__(kBlock) void k_plan(SortArgs a)
         uint32_t* hdr = a.plan_hdr + static_cast<uint64_t>(sgi) * kPlanHdr;
         uint32_t P = 0;
         for (int d = 0; d < D; ++d)
-            if (!s_triv[d]) hdr[1 + P++] = static_cast<uint32_t>(d);
+            if (!s_triv[d]) hdr[kHdrDig + P++] = static_cast<uint32_t>(d);
         if (P == 0 && n > 0) {  // every digit constant: one pass still writes perm and keys
-            hdr[1] = 0;
+            hdr[kHdrDig] = 0;
             P = 1;
         }
-        hdr[0] = P;
+        hdr[kHdrMode] = 0;
+        // Local finish: with three or more non-constant digits, scatter the
+        // top two globally and sort the rest inside buckets of equal top
+        // digits in LDS.  Chosen when the buckets look small: the largest
+        // bucket of two independent digits is about max_ga * max_gb / n.
+        if (D == 4 && a.local_ok && P >= 3) {
+            const uint32_t ga = hdr[kHdrDig + P - 2], gb = hdr[kHdrDig + P - 1];
+            const uint64_t est = static_cast<uint64_t>(s_max[ga]) * s_max[gb] / (n ? n : 1);
+            if (est <= kLocalRows * 3 / 4) {
+                hdr[kHdrNS] = P;
+                for (uint32_t i = 0; i < P; ++i) hdr[kHdrS + i] = hdr[kHdrDig + i];
+                hdr[kHdrGa] = ga;
+                hdr[kHdrGb] = gb;
+                hdr[kHdrDig] = ga;
+                hdr[kHdrDig + 1] = gb;
+                hdr[kHdrMode] = kModeLocal;
+                P = 2;
+            }
+        }
+        hdr[kHdrP] = P;
         atomicMax(a.tickets + kMaxPassWord, P);
     }
 }
@@ -382,6 +435,41 @@ __device__ inline uint32_t block_excl_scan_lds(uint32_t x, uint32_t* s_wsum)
     for (int i = 0; i < kWaves; ++i) pre += (i < w) ? s_wsum[i] : 0u;
     lds_barrier();
     return pre + inc - x;
+}
+
+// ---------------------------------------------------------------------------
+// stable ranks inside a wave
+// ---------------------------------------------------------------------------
+// Ballot ranking: eight ballots, one per digit bit, give each lane the lanes
+// sharing its digit; every lane reads the wave's counter of its digit and the
+// lowest peer writes it back advanced by the peer count, so old + (peers below
+// the lane) is the lane's stable rank.  (LDS operations of one wave complete
+// in program order.)
+__device__ inline uint32_t rank_ballot(uint32_t d, bool v, uint32_t* wc)
+{
+    const uint64_t vm = __ballot(v);
+    uint32_t pl = static_cast<uint32_t>(vm), ph = static_cast<uint32_t>(vm >> 32);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+        // v_bitop3 truth table 0x90 = src0 & ~(src1 ^ src2): keep the lanes that agree on bit b
+        const uint32_t bm = static_cast<uint32_t>(static_cast<int32_t>(d << (31 - b)) >> 31);
+        const uint64_t mb = __ballot(bm != 0u);
+        pl = __builtin_amdgcn_bitop3_b32(pl, static_cast<uint32_t>(mb), bm, 0x90);
+        ph = __builtin_amdgcn_bitop3_b32(ph, static_cast<uint32_t>(mb >> 32), bm, 0x90);
+    }
+    const uint32_t lt = __builtin_amdgcn_mbcnt_hi(ph, __builtin_amdgcn_mbcnt_lo(pl, 0u));
+    const uint32_t cnt = static_cast<uint32_t>(__popc(pl) + __popc(ph));
+    const uint32_t old = wc[d];
+    if (v && lt == 0u) wc[d] = old + cnt;
+    return old + lt;
+}
+
+// Atomic ranking: one LDS add-with-return per lane.  Stable when the LDS
+// serves the lanes of one instruction that share an address in lane order,
+// which lds_rank_check verifies on each device before it is used.
+__device__ inline uint32_t rank_atomic(uint32_t d, bool v, uint32_t* wc)
+{
+    return v ? atomicAdd(&wc[d], 1u) : 0u;
 }
 
 // What a block knows about one claimed tile.
@@ -435,12 +523,9 @@ struct TileCtx {
 // out, so look-back waits overlap work.  Look-back never crosses the start of
 // the tile's segment.
 //
-// Ranking of item j in a wave: eight ballots, one per digit bit, give each
-// lane the lanes sharing its digit (peers).  Every lane reads the wave's
-// counter of its digit; the lowest peer writes it back advanced by the peer
-// count.  LDS operations of one wave complete in program order, so item j's
-// read sees exactly the items before j, and old + (peers below the lane) is
-// the stable rank.
+// Ranking of item j in a wave: rank_atomic (one LDS add-with-return per row)
+// where the device check passed, else rank_ballot; items are ranked in
+// order, so item j's rank counts exactly the items before it.
 // Blocks of a pass kernel that fit one CU's 160 KiB of LDS; the kernel is
 // compiled to keep that many waves per SIMD (one wave of each block per SIMD),
 // so registers never cut occupancy below what LDS allows.
@@ -548,14 +633,16 @@ void k_pass(SortArgs a, uint32_t slot)
             if (t >= a.ntiles) return false;
             const uint32_t sg = a.tile_seg ? __builtin_amdgcn_readfirstlane(a.tile_seg[t]) : 0u;
             const uint32_t* hdr = a.plan_hdr + static_cast<uint64_t>(sg) * kPlanHdr;
-            const uint32_t P = hdr[0];
-            if (slot >= P) continue;
+            const uint32_t P = hdr[kHdrP];
+            const uint32_t mode = hdr[kHdrMode];
+            const uint32_t s0 = mode & 0xFFu;
+            if (slot >= P || slot < s0) continue;
             const SegDesc& sd = a.segs[sg];
             c.t = t;
             c.t0 = sd.tile_off;
-            c.dig = hdr[1 + slot];
-            c.first = (slot == 0);
-            c.last = (slot == P - 1);
+            c.dig = hdr[kHdrDig + slot];
+            c.first = (slot == s0);
+            c.last = (slot == P - 1) && !(mode & kModeLocal);
             c.n = sd.n;
             c.row_off = sd.row_off;
             c.lrow0 = static_cast<uint64_t>(t - sd.tile_off) * TILE;
@@ -665,27 +752,14 @@ void k_pass(SortArgs a, uint32_t slot)
         lds_barrier();
         uint32_t* wc = s_wc[wave];
         const int dig = static_cast<int>(c.dig);
+        if (a.lds_rank) {
 #pragma unroll
-        for (int j = 0; j < ITEMS; ++j) {
-            const bool v = (wl + static_cast<uint64_t>(j) * 64) < c.n;
-            const uint32_t d = KO::digit(key[j], dig);
-            const uint64_t vm = __ballot(v);
-            uint32_t pl = static_cast<uint32_t>(vm), ph = static_cast<uint32_t>(vm >> 32);
+            for (int j = 0; j < ITEMS; ++j)
+                rnk[j] = rank_atomic(KO::digit(key[j], dig), (wl + static_cast<uint64_t>(j) * 64) < c.n, wc);
+        } else {
 #pragma unroll
-            for (int b = 0; b < 8; ++b) {
-                // bm: bit b of the digit, sign-extended (all ones or zero); m: lanes with
-                // the bit set; peers &= ~(m ^ bm) keeps the lanes that agree on bit b
-                // (v_bitop3 truth table 0x90 = src0 & ~(src1 ^ src2))
-                const uint32_t bm = static_cast<uint32_t>(static_cast<int32_t>(d << (31 - b)) >> 31);
-                const uint64_t m = __ballot(bm != 0u);
-                pl = __builtin_amdgcn_bitop3_b32(pl, static_cast<uint32_t>(m), bm, 0x90);
-                ph = __builtin_amdgcn_bitop3_b32(ph, static_cast<uint32_t>(m >> 32), bm, 0x90);
-            }
-            const uint32_t lt = __builtin_amdgcn_mbcnt_hi(ph, __builtin_amdgcn_mbcnt_lo(pl, 0u));
-            const uint32_t cnt = static_cast<uint32_t>(__popc(pl) + __popc(ph));
-            const uint32_t old = wc[d];
-            if (v && lt == 0u) wc[d] = old + cnt;
-            rnk[j] = old + lt;
+            for (int j = 0; j < ITEMS; ++j)
+                rnk[j] = rank_ballot(KO::digit(key[j], dig), (wl + static_cast<uint64_t>(j) * 64) < c.n, wc);
         }
         lds_barrier();
         tot = 0;
@@ -983,6 +1057,446 @@ void k_pass(SortArgs a, uint32_t slot)
     }
 }
 
+// Compares the two rankings on digit patterns with 1 .. 256 distinct values
+// per wave (block b: 2^(b mod 9)); counts the lanes whose ranks differ.
+__global__ __launch_bounds__(kBlock) void k_lds_rank_check(uint32_t* bad)
+{
+    __shared__ uint32_t s_a[kWaves][kRadix], s_b[kWaves][kRadix];
+    const int tid = threadIdx.x, wave = tid >> 6;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) s_a[w][tid] = s_b[w][tid] = 0;
+    __syncthreads();
+    const uint32_t range = 1u << (blockIdx.x % 9);
+    uint32_t miss = 0;
+    for (uint32_t j = 0; j < 32; ++j) {
+        uint32_t x = (blockIdx.x * 32u + j) * kBlock + static_cast<uint32_t>(tid);
+        x = (x ^ 61u) ^ (x >> 16);  // integer hash (Wang)
+        x *= 9u;
+        x ^= x >> 4;
+        x *= 0x27d4eb2du;
+        x ^= x >> 15;
+        const uint32_t d = x & (range - 1u);
+        const bool v = (x >> 28) != 0;  // some lanes inactive
+        const uint32_t rb = rank_ballot(d, v, s_b[wave]);
+        const uint32_t ra = rank_atomic(d, v, s_a[wave]);
+        miss += (v && ra != rb) ? 1u : 0u;
+    }
+    if (miss) atomicAdd(bad, miss);
+}
+
+static std::mutex g_lds_rank_mu;
+static std::vector<std::pair<int, bool>> g_lds_rank;  // device -> atomic ranking verified
+
+bool lds_rank_check(int device)
+{
+    std::lock_guard<std::mutex> lk(g_lds_rank_mu);
+    for (const auto& e : g_lds_rank)
+        if (e.first == device) return e.second;
+    uint32_t* d = nullptr;
+    VDF_HIP_CHECK(hipMalloc(&d, sizeof(uint32_t)));
+    uint32_t bad = 1;
+    const hipError_t e1 = hipMemset(d, 0, sizeof(uint32_t));
+    if (e1 == hipSuccess) hipLaunchKernelGGL(k_lds_rank_check, dim3(64), dim3(kBlock), 0, nullptr, d);
+    const hipError_t e2 = e1 == hipSuccess ? hipGetLastError() : e1;
+    const hipError_t e3 = e2 == hipSuccess ? hipMemcpy(&bad, d, sizeof(uint32_t), hipMemcpyDeviceToHost) : e2;
+    (void)hipFree(d);
+    VDF_HIP_CHECK(e3);
+    g_lds_rank.emplace_back(device, bad == 0);
+    return bad == 0;
+}
+
+// the check's result when it ran on `device`, else false (the ballot ranking)
+static bool lds_rank_cached(int device)
+{
+    std::lock_guard<std::mutex> lk(g_lds_rank_mu);
+    for (const auto& e : g_lds_rank)
+        if (e.first == device) return e.second;
+    return false;
+}
+
+// ---------------------------------------------------------------------------
+// local finish (4-byte keys, one segment)
+// ---------------------------------------------------------------------------
+// Bucket starts.  After the global passes over digits ga (slot 0) and gb
+// (slot 1) the rows are ordered by (gb, ga); bucket (y, x), the rows with
+// digit gb = y and digit ga = x, starts at base_gb[y] + P_y(B_x), where B_x is
+// the first row of ga-bucket x in slot 1's input (slot 0's output, ordered by
+// ga) and P_y(r) counts that input's rows before r whose digit gb is y.  At a
+// tile boundary P_y is the inclusive look-back prefix slot 1 left for the
+// tile before; the rows of B_x's own tile before it are counted here.  One
+// block per boundary x = 0 .. 256 (B_256 = n); lstart[y][x] is the start.
+__global__ __launch_bounds__(kBlock) void k_local_bounds(SortArgs a, uint32_t tile_rows)
+{
+    __shared__ uint32_t s_cnt[kRadix];
+    const uint32_t* hdr = a.plan_hdr;
+    if (!(hdr[kHdrMode] & kModeLocal)) return;
+    const int tid = threadIdx.x;
+    const uint32_t x = blockIdx.x;
+    if (x == 0 && tid == 0) a.lctl[1] = 0;  // k_local_max's maximum
+    const uint32_t ga = hdr[kHdrGa], gb = hdr[kHdrGb];
+    const uint64_t n = a.segs[0].n;
+    const uint32_t* bgb = a.plan_base + static_cast<uint64_t>(gb) * kRadix;
+    const uint64_t B = x < kRadix ? a.plan_base[static_cast<uint64_t>(ga) * kRadix + x] : n;
+    uint32_t start;
+    if (B >= n) {
+        start = bgb[tid] + a.hist[static_cast<uint64_t>(gb) * kRadix + tid];
+    } else {
+        s_cnt[tid] = 0;
+        __syncthreads();
+        const uint64_t t = B / tile_rows;
+        for (uint64_t i = t * tile_rows + tid; i < B; i += kBlock) {
+            const uint32_t k = a.norows ? gmem(static_cast<const uint32_t*>(a.kbuf[0]))[i]
+                                        : static_cast<uint32_t>(gmem(static_cast<const uint64_t*>(a.kbuf[0]))[i]);
+            atomicAdd(&s_cnt[(k >> (8 * gb)) & 0xFFu], 1u);
+        }
+        __syncthreads();
+        const uint32_t prev = t > 0 ? (gmem(a.lb[1])[(t - 1) * kRadix + tid] & kValMask) : 0u;
+        start = bgb[tid] + prev + s_cnt[tid];
+    }
+    a.lstart[static_cast<uint64_t>(tid) * (kRadix + 1) + x] = start;
+}
+
+// The largest bucket of a local finish: block y takes buckets (y, 0..255),
+// their sizes lstart[y][x + 1] - lstart[y][x], into lctl[1].
+__global__ __launch_bounds__(kBlock) void k_local_max(SortArgs a)
+{
+    __shared__ uint32_t s_m[kWaves];
+    if (!(a.plan_hdr[kHdrMode] & kModeLocal)) return;
+    const int tid = threadIdx.x;
+    const uint32_t* row = a.lstart + static_cast<uint64_t>(blockIdx.x) * (kRadix + 1);
+    uint32_t m = row[tid + 1] - row[tid];
+    for (int off = 32; off > 0; off >>= 1) m = std::max(m, static_cast<uint32_t>(__shfl_xor(m, off, 64)));
+    if ((tid & 63) == 0) s_m[tid >> 6] = m;
+    __syncthreads();
+    if (tid == 0) atomicMax(a.lctl + 1, std::max(std::max(s_m[0], s_m[1]), std::max(s_m[2], s_m[3])));
+}
+
+// Cuts the rows into chunks of whole buckets, or falls back.  With M the
+// largest bucket and H = kLocalRows - M, chunk k holds the buckets that start
+// in [kH, (k+1)H): it starts at the first bucket start >= kH and ends before
+// the first one >= (k+1)H, so it has fewer than H + M = kLocalRows rows.
+// M > kLocalRows - kLocalMinH restarts the plain passes at slot kLocalSlot0
+// (they read the caller's columns again).  Thread x of block y takes bucket
+// b = y * 256 + x; block 0 also takes the end (b = 65536, start n).
+__global__ __launch_bounds__(kBlock) void k_local_chunks(SortArgs a)
+{
+    uint32_t* hdr = a.plan_hdr;
+    if (!(hdr[kHdrMode] & kModeLocal)) return;
+    const uint32_t m = a.lctl[1];
+    const uint32_t n = static_cast<uint32_t>(a.segs[0].n);
+    const uint32_t tid = threadIdx.x, y = blockIdx.x;
+    if (m > kLocalRows - kLocalMinH) {
+        // (blocks that read the mode after this rewrite return as well)
+        if (y == 0 && tid == 0) {
+            const uint32_t np = hdr[kHdrNS];
+            for (uint32_t i = 0; i < np; ++i) hdr[kHdrDig + kLocalSlot0 + i] = hdr[kHdrS + i];
+            hdr[kHdrMode] = kLocalSlot0;
+            hdr[kHdrP] = kLocalSlot0 + np;
+            atomicMax(a.tickets + kMaxPassWord, kLocalSlot0 + np);
+            a.lctl[0] = 0;
+        }
+        return;
+    }
+    const uint32_t H = kLocalRows - m;
+    const uint32_t K = (n + H - 1) / H;
+    // chunks k with start(b - 1) < kH <= start(b) start at start(b) (each k
+    // once: an empty bucket's start equals the next one's)
+    auto cut = [&](uint32_t s, uint32_t p) {
+        for (uint32_t k = p / H + 1; k <= s / H && k < K; ++k) a.lchunk[k] = s;
+    };
+    const uint32_t* row = a.lstart + static_cast<uint64_t>(y) * (kRadix + 1);
+    const uint32_t s = row[tid];
+    // the bucket before (y, 0) is (y - 1, 255): row[-2] (row[-1] is its end)
+    if (y > 0 || tid > 0) cut(s, tid > 0 ? row[tid - 1] : row[-2]);
+    if (y == kRadix - 1 && tid == kRadix - 1) cut(n, row[tid]);
+    if (y == 0 && tid == 0) {
+        a.lchunk[0] = 0;
+        a.lchunk[K] = n;
+        a.lctl[0] = K;
+    }
+}
+
+// Sorts each chunk in LDS and writes the sort's outputs.  A chunk's rows
+// arrive ordered by their top digits (the global passes); stable 8-bit LDS
+// passes, least significant first, over every digit that varies inside the
+// chunk order it by the whole key (slots wave-major, each wave ranking its
+// rows in order, per-wave bucket counters combined in wave order).  Row ids
+// stay in the registers of the thread that loaded them; each column reaches
+// its rows' final slots through LDS and leaves in slot order, so every output
+// is written by consecutive lanes.  8 waves per block of at most 8 rows per
+// thread, 3 blocks per CU.
+template <int PB, bool ROWS>
+__global__ __launch_bounds__(kLocalBlock) __attribute__((amdgpu_waves_per_eu(4)))
+void k_local(SortArgs a)
+{
+    using KO = KeyOps<uint32_t>;
+    using P = typename std::conditional<PB >= 8, uint64_t, uint32_t>::type;
+    using P2 = typename std::conditional<PB == 12, uint32_t, uint64_t>::type;
+    constexpr bool TWO = PB == 12 || PB == 16;
+    constexpr int W = kLocalBlock / 64;         // waves
+    constexpr int IM = kLocalRows / kLocalBlock;  // rows per thread at most
+    constexpr int PI = PB ? IM : 1;
+    constexpr int PI2 = TWO ? IM : 1;
+    // 32 KiB: keys (16 KiB), their loaded slots (8 KiB) and the final slot of
+    // each loaded slot (8 KiB) while sorting; one column of 8-byte cells at write-out
+    __shared__ uint64_t s_raw[kLocalRows];
+    __shared__ uint32_t s_wc[2][W][kRadix];  // per-wave bucket counters, two sets: a pass zeroes the next one's
+    __shared__ uint32_t s_wsum[kWaves];
+    uint32_t* s_key = reinterpret_cast<uint32_t*>(s_raw);
+    uint16_t* s_org = reinterpret_cast<uint16_t*>(s_key + kLocalRows);
+    uint16_t* s_pos = s_org + kLocalRows;
+
+    const uint32_t* hdr = a.plan_hdr;
+    if (!(hdr[kHdrMode] & kModeLocal)) return;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const SegDesc& sd = a.segs[0];
+    const uint32_t n = static_cast<uint32_t>(sd.n);
+    const uint32_t K = a.lctl[0];
+    const auto* recs = gmem(static_cast<const uint64_t*>(a.kbuf[1]));
+    const auto* kin = gmem(static_cast<const uint32_t*>(a.kbuf[1]));
+    const auto* pin = gmem(static_cast<const P*>(a.pbuf[1]));
+    const auto* pin2 = gmem(static_cast<const P2*>(a.pbuf2[1]));
+    void* keys_out = sd.keys_out;
+    void* perm_out = sd.perm_out;
+    auto* pay_out = gmem(static_cast<P*>(sd.pay_out));
+    auto* pay_out2 = gmem(static_cast<P2*>(sd.pay_out2));
+    uint32_t key[IM], rid[IM], rnk[IM], org[IM];
+    P pay[PI];
+    P2 pay2[PI2];
+    // the digits every chunk sorts: the non-constant ones below the two global ones
+    uint32_t lmask = 0;
+    for (uint32_t i = 0; i < hdr[kHdrNS]; ++i) lmask |= 1u << hdr[kHdrS + i];
+    const uint32_t ga = hdr[kHdrGa], gb = hdr[kHdrGb];
+    lmask &= ~((1u << ga) | (1u << gb));
+    if (tid < kRadix) {
+#pragma unroll
+        for (int w = 0; w < W; ++w) s_wc[0][w][tid] = s_wc[1][w][tid] = 0;
+    }
+    lds_barrier();
+    uint32_t q = 0;  // passes so far (counter set q & 1)
+
+    // Software pipeline over the block's chunks k = blockIdx.x + i * gridDim.x.
+    // Iteration i issues, in this order, vector loads of: chunk i's payload,
+    // chunk i+1's (key, row id) records and first / last keys, chunk i+2's
+    // bounds -- all unconditional (slots past a chunk read its last row), so
+    // the waits before each use leave the later loads in flight.  Rows per
+    // thread: wave w holds slots [w*64*I, (w+1)*64*I); slot sl0 + 64 j is row
+    // r0 + sl0 + 64 j.
+    const uint32_t G = gridDim.x;
+    auto bound = [&](uint32_t kk) __attribute__((always_inline)) -> uint32_t {
+        return gmem(a.lchunk)[std::min(kk, K)];  // lchunk[K] = n
+    };
+    auto clampm = [](uint32_t r0x, uint32_t r1x) -> uint32_t {
+        return r1x > r0x && r1x - r0x <= kLocalRows ? r1x - r0x : 0u;  // chunks are < kLocalRows rows
+    };
+    // the row of slot s of a chunk (an empty chunk reads the table's last row)
+    auto row_at = [&](uint32_t r0x, uint32_t mx, uint32_t sx) __attribute__((always_inline)) -> uint32_t {
+        return mx ? r0x + std::min(sx, mx - 1) : n - 1;
+    };
+    auto load_recs = [&](uint32_t r0x, uint32_t mx, uint64_t* dst) __attribute__((always_inline)) {
+        const uint32_t Ix = (mx + kLocalBlock - 1) / kLocalBlock;
+        const uint32_t s0 = static_cast<uint32_t>(wave) * 64 * Ix + lane;
+#pragma unroll
+        for (int j = 0; j < IM; ++j) {
+            const uint32_t r = row_at(r0x, mx, s0 + j * 64);
+            if constexpr (ROWS)
+                dst[j] = recs[r];
+            else
+                dst[j] = kin[r];
+        }
+    };
+    uint64_t nx[IM];
+    // chunk 0 of this block: bounds, records and first / last keys, chunk 1's bounds
+    uint32_t cr0 = bound(blockIdx.x), cm = clampm(cr0, bound(blockIdx.x + 1));
+    load_recs(cr0, cm, nx);
+    uint64_t kfl = ROWS ? recs[row_at(cr0, cm, 0)] : kin[row_at(cr0, cm, 0)];
+    uint64_t kll = ROWS ? recs[row_at(cr0, cm, kLocalRows)] : kin[row_at(cr0, cm, kLocalRows)];
+    uint32_t b0 = bound(blockIdx.x + G), b1 = bound(blockIdx.x + G + 1);
+    for (uint32_t k = blockIdx.x; k < K; k += G) {
+        const uint32_t r0 = cr0, m = cm;  // this chunk
+        const uint32_t I = (m + kLocalBlock - 1) / kLocalBlock;
+        const uint32_t sl0 = static_cast<uint32_t>(wave) * 64 * I + lane;
+        auto valid = [&](int j) __attribute__((always_inline)) { return j < static_cast<int>(I) && sl0 + j * 64 < m; };
+#pragma unroll
+        for (int j = 0; j < IM; ++j) {
+            key[j] = static_cast<uint32_t>(nx[j]);
+            rid[j] = static_cast<uint32_t>(nx[j] >> 32);
+        }
+        const uint32_t kf = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<uint32_t>(kfl)));
+        const uint32_t kl = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<uint32_t>(kll)));
+        // this chunk's payload
+#pragma unroll
+        for (int j = 0; j < IM; ++j) {
+            const uint32_t r = row_at(r0, m, sl0 + j * 64);
+            if constexpr (PB != 0) pay[j] = pin[r];
+            if constexpr (TWO) pay2[j] = pin2[r];
+        }
+        // the next chunk's records and first / last keys, the one after's bounds
+        const uint32_t r0n = __builtin_amdgcn_readfirstlane(b0);
+        const uint32_t mn = clampm(r0n, __builtin_amdgcn_readfirstlane(b1));
+        load_recs(r0n, mn, nx);
+        kfl = ROWS ? recs[row_at(r0n, mn, 0)] : kin[row_at(r0n, mn, 0)];
+        kll = ROWS ? recs[row_at(r0n, mn, kLocalRows)] : kin[row_at(r0n, mn, kLocalRows)];
+        b0 = bound(k + 2 * G);
+        b1 = bound(k + 2 * G + 1);
+        cr0 = r0n;
+        cm = mn;
+        if (m == 0) continue;
+        // digits that vary inside the chunk: the low ones, and the global ones
+        // where its first and last keys differ (its rows are ordered by them)
+        uint32_t vary = lmask;
+        if (KO::digit(kf, gb) != KO::digit(kl, gb)) vary |= (1u << ga) | (1u << gb);
+        else if (KO::digit(kf, ga) != KO::digit(kl, ga)) vary |= 1u << ga;
+        if (vary == 0) vary = 1;  // one key value: one pass leaves the order and gives the final slots
+#pragma unroll
+        for (int j = 0; j < IM; ++j) org[j] = sl0 + j * 64;
+        bool from_regs = true;
+#pragma unroll 1
+        for (int d = 0; d < 4; ++d) {
+            if (!((vary >> d) & 1u)) continue;
+            const bool last = (vary >> (d + 1)) == 0;
+            if (!from_regs) {
+                // the previous pass's order
+#pragma unroll
+                for (int j = 0; j < IM; ++j) {
+                    if (j < static_cast<int>(I)) {  // slots < 512 I <= kLocalRows
+                        key[j] = s_key[sl0 + j * 64];
+                        org[j] = s_org[sl0 + j * 64];
+                    }
+                }
+            }
+            uint32_t(*cs)[kRadix] = s_wc[q & 1];
+            uint32_t* wc = cs[wave];
+            if (a.lds_rank) {
+#pragma unroll
+                for (int j = 0; j < IM; ++j)
+                    if (j < static_cast<int>(I)) rnk[j] = rank_atomic(KO::digit(key[j], d), sl0 + j * 64 < m, wc);
+            } else {
+#pragma unroll
+                for (int j = 0; j < IM; ++j)
+                    if (j < static_cast<int>(I)) rnk[j] = rank_ballot(KO::digit(key[j], d), sl0 + j * 64 < m, wc);
+            }
+            lds_barrier();
+            // bucket tid: per-wave offsets (waves in order), then the exclusive
+            // scan over buckets (threads of waves 0..3); the other set, last read
+            // before the previous barrier, is cleared for the next pass
+            uint32_t tot = 0;
+            if (tid < kRadix) {
+#pragma unroll
+                for (int w = 0; w < W; ++w) {
+                    const uint32_t cw = cs[w][tid];
+                    cs[w][tid] = tot;
+                    s_wc[(q + 1) & 1][w][tid] = 0;
+                    tot += cw;
+                }
+            }
+            uint32_t inc = tot;
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                const uint32_t y = __shfl_up(inc, off, 64);
+                if (lane >= off) inc += y;
+            }
+            if (lane == 63 && wave < kWaves) s_wsum[wave] = inc;
+            lds_barrier();
+            if (tid < kRadix) {
+                uint32_t pre = 0;
+#pragma unroll
+                for (int i = 0; i < kWaves; ++i) pre += (i < wave) ? s_wsum[i] : 0u;
+                const uint32_t lstart = pre + inc - tot;
+#pragma unroll
+                for (int w = 0; w < W; ++w) cs[w][tid] += lstart;
+            }
+            lds_barrier();
+            // the slots of the next pass; the last pass gives each loaded slot its final slot
+#pragma unroll
+            for (int j = 0; j < IM; ++j) {
+                if (valid(j)) {
+                    const uint32_t p = wc[KO::digit(key[j], d)] + rnk[j];
+                    s_key[p] = key[j];
+                    if (last)
+                        s_pos[org[j]] = static_cast<uint16_t>(p);
+                    else
+                        s_org[p] = static_cast<uint16_t>(org[j]);
+                }
+            }
+            from_regs = false;
+            ++q;
+            if (last) break;
+            lds_barrier();
+        }
+        lds_barrier();
+
+        // write-out: final slot s goes to output row r0 + s (reversed when asked)
+        auto outpos = [&](uint32_t s) __attribute__((always_inline)) -> uint32_t {
+            const uint32_t g = r0 + s;
+            return a.reverse ? n - 1 - g : g;
+        };
+        if (keys_out) {
+            uint32_t kv[IM];
+#pragma unroll
+            for (int u = 0; u < IM; ++u)
+                if (tid + u * kLocalBlock < m) kv[u] = s_key[tid + u * kLocalBlock];
+#pragma unroll
+            for (int u = 0; u < IM; ++u)
+                if (tid + u * kLocalBlock < m) KO::store(keys_out, outpos(tid + u * kLocalBlock), kv[u], a.mlo, a.mhi);
+        }
+#pragma unroll
+        for (int j = 0; j < IM; ++j)
+            if (j < static_cast<int>(I)) rnk[j] = s_pos[sl0 + j * 64];
+        // one column at a time: each row's cell to its final slot in LDS, then out in slot order
+        auto column = [&](auto* stage, auto get, auto put) __attribute__((always_inline)) {
+            lds_barrier();  // the previous phase's LDS reads are done
+#pragma unroll
+            for (int j = 0; j < IM; ++j)
+                if (valid(j)) stage[rnk[j]] = get(j);
+            lds_barrier();
+            using T = typename std::remove_reference<decltype(stage[0])>::type;
+            T v[IM];
+#pragma unroll
+            for (int u = 0; u < IM; ++u)
+                if (tid + u * kLocalBlock < m) v[u] = stage[tid + u * kLocalBlock];
+#pragma unroll
+            for (int u = 0; u < IM; ++u)
+                if (tid + u * kLocalBlock < m) put(outpos(tid + u * kLocalBlock), v[u]);
+        };
+        if constexpr (ROWS) {
+            if (perm_out) {
+                if (a.perm64) {
+                    column(s_raw, [&](int j) __attribute__((always_inline)) { return static_cast<uint64_t>(rid[j]); },
+                           [&](uint32_t p, uint64_t v) __attribute__((always_inline)) {
+                               gmem(static_cast<uint64_t*>(perm_out))[p] = v;
+                           });
+                } else {
+                    column(s_key, [&](int j) __attribute__((always_inline)) { return rid[j]; },
+                           [&](uint32_t p, uint32_t v) __attribute__((always_inline)) {
+                               gmem(static_cast<uint32_t*>(perm_out))[p] = v;
+                           });
+                }
+            }
+        }
+        if constexpr (PB != 0)
+            column(reinterpret_cast<P*>(s_raw), [&](int j) __attribute__((always_inline)) { return pay[j]; },
+                   [&](uint32_t p, P v) __attribute__((always_inline)) { pay_out[p] = v; });
+        if constexpr (TWO)
+            column(reinterpret_cast<P2*>(s_raw), [&](int j) __attribute__((always_inline)) { return pay2[j]; },
+                   [&](uint32_t p, P2 v) __attribute__((always_inline)) { pay_out2[p] = v; });
+        // (the next chunk's first LDS writes to s_raw come three barriers later)
+    }
+}
+
+using LocalFn = void (*)(SortArgs);
+static LocalFn pick_local(int pay_bytes, bool rows)
+{
+    switch (pay_bytes) {
+    case 0: return rows ? k_local<0, true> : k_local<0, false>;
+    case 4: return rows ? k_local<4, true> : k_local<4, false>;
+    case 8: return rows ? k_local<8, true> : k_local<8, false>;
+    case 12: return rows ? k_local<12, true> : k_local<12, false>;
+    case 16: return rows ? k_local<16, true> : k_local<16, false>;
+    default: return nullptr;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
@@ -1009,6 +1523,7 @@ constexpr uint64_t kMinTileRows = static_cast<uint64_t>(kBlock) * 8;  // smalles
 struct SortLayout {
     uint64_t rows = 0, tiles = 0, items = 0;
     size_t zero_words = 0;  // tickets and histograms (zeroed by every sort)
+    uint64_t lchunks = 0;   // local finish: chunk starts (0: no local finish possible)
     size_t bytes = 0;
 };
 
@@ -1051,6 +1566,13 @@ static SortLayout sort_layout(int key_type, const uint64_t* n, int nsegs, uint64
     b += 2 * align_up(L.rows * kb, 256);
     b += 2 * align_up(L.rows * 4, 256);
     b += 2 * align_up(L.rows * static_cast<uint64_t>(pay_bytes), 256);
+    if (kb == 4 && nsegs == 1 && L.rows >= kLocalMinSortRows) {
+        // local finish: bucket starts, chunk starts, control word
+        L.lchunks = L.rows / kLocalMinH + 2;
+        b += align_up(static_cast<size_t>(kRadix) * (kRadix + 1) * 4, 256);
+        b += align_up(L.lchunks * 4, 256);
+        b += 256;
+    }
     L.bytes = b + 256;
     return L;
 }
@@ -1208,6 +1730,7 @@ bool tune_supported(const char* name, int64_t v)
     if (n == "gather_nt") return v == 0 || v == 1;
     if (n == "csv_stage") return v == 0 || v == 1;
     if (n == "groupby_carry") return v == 0 || v == 1;
+    if (n == "sort_local") return v == 0 || v == 1;
     return false;
 }
 
@@ -1226,10 +1749,10 @@ static unsigned device_cus(int device)
 }
 
 // Resident blocks of a pass kernel on the whole device (persistent grid).
-static unsigned resident_blocks(PassFn fn, int device)
+static unsigned resident_blocks(const void* fn, int device, int threads = kBlock)
 {
     struct Entry {
-        PassFn fn;
+        const void* fn;
         int device;
         unsigned blocks;
     };
@@ -1239,7 +1762,7 @@ static unsigned resident_blocks(PassFn fn, int device)
     for (const Entry& e : cache)
         if (e.fn == fn && e.device == device) return e.blocks;
     int per_cu = 0, cus = 0;
-    VDF_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(fn), kBlock, 0));
+    VDF_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, 0));
     VDF_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
     const unsigned blocks = static_cast<unsigned>(std::max(1, per_cu) * std::max(1, cus));
     cache.push_back(Entry{fn, device, blocks});
@@ -1358,6 +1881,17 @@ static void launch_sort(Handle* h, const SortSeg* segs, int nsegs, uint64_t mlo,
     // a 16-byte payload is two 8-byte columns: the second in the upper half
     a.pbuf2[0] = static_cast<char*>(a.pbuf[0]) + (pay_bytes >= 12 ? L.rows * 8 : 0);
     a.pbuf2[1] = static_cast<char*>(a.pbuf[1]) + (pay_bytes >= 12 ? L.rows * 8 : 0);
+    if (L.lchunks) {
+        a.lstart = c.take<uint32_t>(static_cast<size_t>(kRadix) * (kRadix + 1));
+        a.lchunk = c.take<uint32_t>(L.lchunks);
+        a.lctl = c.take<uint32_t>(64);
+    }
+    // local finish: one segment of 4-byte keys with enough rows that two
+    // global passes and a streaming LDS pass beat the plain passes
+    LocalFn local = nullptr;
+    if (L.lchunks && h->tune.sort_local && all_rows >= kLocalMinSortRows) local = pick_local(pay_bytes, rowids);
+    a.local_ok = local ? 1u : 0u;
+    a.lds_rank = lds_rank_cached(h->device) ? 1u : 0u;
     if (devdesc) {
         const unsigned blocks = static_cast<unsigned>((nitems + kBlock - 1) / kBlock);
         hipLaunchKernelGGL(k_setup_one, dim3(blocks), dim3(kBlock), 0, s, one, dsegs, ditems, nitems, hist_rows);
@@ -1379,7 +1913,7 @@ static void launch_sort(Handle* h, const SortSeg* segs, int nsegs, uint64_t mlo,
     a.reverse = (flags & VDF_SORT_DESC_REVERSE) ? 1u : 0u;
     a.norows = rowids ? 0u : 1u;
     // persistent grid: every resident block (or the tuned blocks per CU)
-    uint64_t cap = resident_blocks(pass, h->device);
+    uint64_t cap = resident_blocks(reinterpret_cast<const void*>(pass), h->device);
     if (h->tune.sort_bpc > 0) cap = std::min<uint64_t>(cap, static_cast<uint64_t>(h->tune.sort_bpc) * device_cus(h->device));
     const unsigned grid = static_cast<unsigned>(std::min<uint64_t>(cap, ntiles));
     if (h->tune.sort_xcd != 0) {
@@ -1415,14 +1949,18 @@ static void launch_sort(Handle* h, const SortSeg* segs, int nsegs, uint64_t mlo,
     const uint64_t pb_out = segs[0].perm_out ? (a.perm64 ? 8 : 4) : 0;
     const uint64_t kout = segs[0].keys_out ? kb : 0;
     const uint64_t rid = rowids ? 4 : 0;  // row ids read / written by a pass
-    if (h->prof && h->passes_ring && static_cast<size_t>(nsegs) <= Handle::kRing) {
-        if (h->ring_pos + nsegs > Handle::kRing) h->ring_pos = 0;
+    // pinned copy of every table's (pass slots end, mode) header words
+    auto copy_plan = [&]() -> const uint32_t* {
+        if (h->ring_pos + 2 * static_cast<size_t>(nsegs) > Handle::kRing) h->ring_pos = 0;
         uint32_t* dst = h->passes_ring + h->ring_pos;
-        h->ring_pos += nsegs;
-        VDF_HIP_CHECK(hipMemcpy2DAsync(dst, sizeof(uint32_t), a.plan_hdr, kPlanHdr * sizeof(uint32_t),
-                                       sizeof(uint32_t), static_cast<size_t>(nsegs), hipMemcpyDeviceToHost, s));
+        h->ring_pos += 2 * static_cast<size_t>(nsegs);
+        VDF_HIP_CHECK(hipMemcpy2DAsync(dst, 2 * sizeof(uint32_t), a.plan_hdr, kPlanHdr * sizeof(uint32_t),
+                                       2 * sizeof(uint32_t), static_cast<size_t>(nsegs), hipMemcpyDeviceToHost, s));
+        return dst;
+    };
+    if (h->prof && h->passes_ring && 2 * static_cast<size_t>(nsegs) <= Handle::kRing / 2) {
         acct = std::make_shared<PassAcct>();
-        acct->passes = dst;
+        acct->passes = copy_plan();
         acct->rows.assign(ns, ns + nsegs);
         // keys read + keys written + row ids read + row ids written, per row, when
         // the pass is the table's only / first / middle / last one
@@ -1432,14 +1970,47 @@ static void launch_sort(Handle* h, const SortSeg* segs, int nsegs, uint64_t mlo,
         acct->per_row[1] = kb + kb + rid + pay2;
         acct->per_row[2] = 2 * kb + 2 * rid + pay2;
         acct->per_row[3] = kb + rid + kout + pb_out + pay2;
+        // a local finish reads the last global pass's keys, row ids and payload
+        // and writes the outputs
+        acct->per_row_local = kb + rid + pay_bytes + kout + pb_out + pay_bytes;
     }
     const uint64_t middle_bytes = L.rows * (2 * kb + 2 * rid + 2 * static_cast<uint64_t>(pay_bytes));
-    for (int slot = 0; slot < D; ++slot) {
+    auto launch_pass = [&](int slot) {
         LaunchTimer t(h, s, "radix_pass", middle_bytes, slot, acct);
         uint32_t sl = static_cast<uint32_t>(slot);
         void* args[] = {&a, &sl};
         VDF_HIP_CHECK(hipLaunchKernel(reinterpret_cast<const void*>(pass), dim3(grid), dim3(kBlock), args, 0, s));
+    };
+    if (!local) {
+        for (int slot = 0; slot < D; ++slot) launch_pass(slot);
+        return;
     }
+    // local finish: slots 0 and 1 scatter the top two digits, k_local sorts the
+    // buckets; slots kLocalSlot0.. run only when k_local_chunks fell back
+    launch_pass(0);
+    launch_pass(1);
+    {
+        LaunchTimer t(h, s, "radix_local_plan", 0);
+        hipLaunchKernelGGL(k_local_bounds, dim3(kRadix + 1), dim3(kBlock), 0, s, a, static_cast<uint32_t>(tile_rows));
+        check_launch("k_local_bounds");
+        hipLaunchKernelGGL(k_local_max, dim3(kRadix), dim3(kBlock), 0, s, a);
+        check_launch("k_local_max");
+        hipLaunchKernelGGL(k_local_chunks, dim3(kRadix), dim3(kBlock), 0, s, a);
+        check_launch("k_local_chunks");
+    }
+    std::shared_ptr<PassAcct> acct2;
+    if (acct) {
+        acct2 = std::make_shared<PassAcct>(*acct);
+        acct2->passes2 = copy_plan();
+        acct = acct2;
+    }
+    {
+        LaunchTimer t(h, s, "radix_local", L.rows * (acct ? acct->per_row_local : 0), kProfLocalSlot, acct);
+        const unsigned lgrid = resident_blocks(reinterpret_cast<const void*>(local), h->device, kLocalBlock);
+        void* args[] = {&a};
+        VDF_HIP_CHECK(hipLaunchKernel(reinterpret_cast<const void*>(local), dim3(lgrid), dim3(kLocalBlock), args, 0, s));
+    }
+    for (int slot = kLocalSlot0; slot < kLocalMaxSlots; ++slot) launch_pass(slot);
 }
 
 static void masks_for(int key_type, uint32_t flags, uint64_t& mlo, uint64_t& mhi)

@@ -298,6 +298,7 @@ vdf_status vdf_server_create(vdf_server** out, const vdf_server_config* cfg)
         int prev = 0;
         VDF_HIP_CHECK(hipGetDevice(&prev));
         VDF_HIP_CHECK(hipSetDevice(cfg->device));
+        (void)lds_rank_check(cfg->device);
         auto S = std::make_unique<vdf_server>();
         S->device = cfg->device;
         S->lim.max_tables = static_cast<size_t>(cfg->max_batch_tables);

@@ -103,6 +103,7 @@ int descend(Handle* h, int kb, int reps, hipStream_t s, const std::function<void
                                                  : pay_bytes == 4 ? std::vector<int>{8, 12}
                                                                   : std::vector<int>{8, 12, 16}});
     if (kb == 16) knobs.push_back({&Tuning::sort_items_128, {8, 16}});
+    if (kb == 4) knobs.push_back({&Tuning::sort_local, {1, 0}});
     knobs.push_back({&Tuning::sort_bpc, {0, 1, 2, 3}});
     knobs.push_back({&Tuning::sort_xcd, {1, 0}});
     knobs.push_back({&Tuning::sort_chunk, {4, 16, 64, 256}});
@@ -136,6 +137,7 @@ int descend(Handle* h, int kb, int reps, hipStream_t s, const std::function<void
         rep->sort_chunk = best.sort_chunk;
         rep->hist_rows = best.hist_rows;
         rep->digit_bits = kRadixBits;
+        rep->sort_local = best.sort_local;
     } catch (...) {
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);

@@ -102,10 +102,17 @@ struct SortSeg {
 // Byte accounting of the radix pass launches of one sort: each table's pass
 // count (pinned copy of the device plan) and rows, and the bytes per row of a
 // pass that is the table's only / first / middle / last one.
+// A table's plan is two words: the end of its pass slots and its mode (first
+// slot in bits 0..7, kModeLocal when a local finish writes the outputs).
+constexpr uint32_t kModeLocal = 1u << 8;
+constexpr int kLocalSlot0 = 2;       // first pass slot of the plain passes after a local finish fell back
+constexpr int kProfLocalSlot = 64;   // ProfRec.plan_slot of a local-finish launch
 struct PassAcct {
-    const uint32_t* passes = nullptr;  // nsegs words, valid once the launch's stream reached the copy
+    const uint32_t* passes = nullptr;   // 2 * nsegs words after the plan, valid once the stream reached the copy
+    const uint32_t* passes2 = nullptr;  // the same after a local finish's size check (null: none launched)
     std::vector<uint64_t> rows;
     uint64_t per_row[4] = {0, 0, 0, 0};
+    uint64_t per_row_local = 0;  // bytes per row of a local finish
 };
 
 struct ProfRec {
@@ -140,6 +147,7 @@ struct Tuning {
     int gather_nt = 0;       // non-temporal loads of the gathered column
     int csv_stage = 1;       // CSV row kernels parse each wave's rows from a copy in LDS
     int gb_carry = 1;        // group-by: value columns carried through the sort (1) or gathered (0)
+    int sort_local = 1;      // 4-byte keys: finish a sort in LDS after two global passes when its buckets fit
 };
 
 bool tune_supported(const char* name, int64_t value);
@@ -362,6 +370,13 @@ void pack_keys_enqueue(Handle* h, const int* types, const uint32_t* widths, cons
                        uint64_t n, void* comp, hipStream_t s);
 void unpack_keys_enqueue(Handle* h, const int* types, const uint32_t* widths, void* const* outs, int ncols,
                          const void* comp, const uint64_t* n_groups, uint64_t cap, hipStream_t s);
+
+// Verifies once per device (with that device current; allocates a word and
+// synchronises) that LDS atomic adds serve the lanes of one instruction in
+// lane order, which makes one atomic per row a stable rank.  Called where
+// handles are made (vdf_create, server workers); sorts on a device where it
+// has not run use the ballot ranking.
+bool lds_rank_check(int device);
 
 // fill kernel (graph-safe replacement for hipMemsetAsync)
 void fill_u32(uint32_t* p, size_t count, uint32_t value, hipStream_t s);
