@@ -445,7 +445,8 @@ __device__ inline uint32_t block_excl_scan_lds(uint32_t x, uint32_t* s_wsum)
 // lowest peer writes it back advanced by the peer count, so old + (peers below
 // the lane) is the lane's stable rank.  (LDS operations of one wave complete
 // in program order.)
-__device__ inline uint32_t rank_ballot(uint32_t d, bool v, uint32_t* wc)
+// (c: the lane's counter, the wave's counter of digit d)
+__device__ inline uint32_t rank_ballot_at(uint32_t d, bool v, uint32_t* c)
 {
     const uint64_t vm = __ballot(v);
     uint32_t pl = static_cast<uint32_t>(vm), ph = static_cast<uint32_t>(vm >> 32);
@@ -459,10 +460,11 @@ __device__ inline uint32_t rank_ballot(uint32_t d, bool v, uint32_t* wc)
     }
     const uint32_t lt = __builtin_amdgcn_mbcnt_hi(ph, __builtin_amdgcn_mbcnt_lo(pl, 0u));
     const uint32_t cnt = static_cast<uint32_t>(__popc(pl) + __popc(ph));
-    const uint32_t old = wc[d];
-    if (v && lt == 0u) wc[d] = old + cnt;
+    const uint32_t old = *c;
+    if (v && lt == 0u) *c = old + cnt;
     return old + lt;
 }
+__device__ inline uint32_t rank_ballot(uint32_t d, bool v, uint32_t* wc) { return rank_ballot_at(d, v, wc + d); }
 
 // Atomic ranking: one LDS add-with-return per lane.  Stable when the LDS
 // serves the lanes of one instruction that share an address in lane order,
@@ -1240,7 +1242,12 @@ void k_local(SortArgs a)
     // 32 KiB: keys (16 KiB), their loaded slots (8 KiB) and the final slot of
     // each loaded slot (8 KiB) while sorting; one column of 8-byte cells at write-out
     __shared__ uint64_t s_raw[kLocalRows];
-    __shared__ uint32_t s_wc[2][W][kRadix];  // per-wave bucket counters, two sets: a pass zeroes the next one's
+    // Per-wave bucket counters, two sets (a pass zeroes the next one's); the W
+    // counters of bucket b sit at b * CS .. b * CS + W - 1, a row that one
+    // thread reads and writes with paired-dword operations, and with CS odd
+    // the counters of one wave fall in distinct banks for 64 consecutive buckets.
+    constexpr int CS = W + 1;
+    __shared__ uint32_t s_wc[2][kRadix * CS];
     __shared__ uint32_t s_wsum[kWaves];
     uint32_t* s_key = reinterpret_cast<uint32_t*>(s_raw);
     uint16_t* s_org = reinterpret_cast<uint16_t*>(s_key + kLocalRows);
@@ -1268,10 +1275,7 @@ void k_local(SortArgs a)
     for (uint32_t i = 0; i < hdr[kHdrNS]; ++i) lmask |= 1u << hdr[kHdrS + i];
     const uint32_t ga = hdr[kHdrGa], gb = hdr[kHdrGb];
     lmask &= ~((1u << ga) | (1u << gb));
-    if (tid < kRadix) {
-#pragma unroll
-        for (int w = 0; w < W; ++w) s_wc[0][w][tid] = s_wc[1][w][tid] = 0;
-    }
+    for (int i = tid; i < kRadix * CS; i += kLocalBlock) s_wc[0][i] = s_wc[1][i] = 0;
     lds_barrier();
     uint32_t q = 0;  // passes so far (counter set q & 1)
 
@@ -1365,29 +1369,35 @@ void k_local(SortArgs a)
                     }
                 }
             }
-            uint32_t(*cs)[kRadix] = s_wc[q & 1];
-            uint32_t* wc = cs[wave];
+            uint32_t* cs = s_wc[q & 1];
+            uint32_t* wc = cs + wave;  // this wave's counter of bucket b: wc[b * CS]
             if (a.lds_rank) {
 #pragma unroll
                 for (int j = 0; j < IM; ++j)
-                    if (j < static_cast<int>(I)) rnk[j] = rank_atomic(KO::digit(key[j], d), sl0 + j * 64 < m, wc);
+                    if (j < static_cast<int>(I)) {
+                        const bool v = sl0 + j * 64 < m;
+                        rnk[j] = v ? atomicAdd(&wc[KO::digit(key[j], d) * CS], 1u) : 0u;
+                    }
             } else {
 #pragma unroll
                 for (int j = 0; j < IM; ++j)
-                    if (j < static_cast<int>(I)) rnk[j] = rank_ballot(KO::digit(key[j], d), sl0 + j * 64 < m, wc);
+                    if (j < static_cast<int>(I)) {
+                        const uint32_t dg = KO::digit(key[j], d);
+                        rnk[j] = rank_ballot_at(dg, sl0 + j * 64 < m, &wc[dg * CS]);
+                    }
             }
             lds_barrier();
             // bucket tid: per-wave offsets (waves in order), then the exclusive
-            // scan over buckets (threads of waves 0..3); the other set, last read
-            // before the previous barrier, is cleared for the next pass
-            uint32_t tot = 0;
+            // scan over buckets (threads of waves 0..3)
+            uint32_t tot = 0, ex[W];
             if (tid < kRadix) {
+                uint32_t c[W];
+#pragma unroll
+                for (int w = 0; w < W; ++w) c[w] = cs[tid * CS + w];
 #pragma unroll
                 for (int w = 0; w < W; ++w) {
-                    const uint32_t cw = cs[w][tid];
-                    cs[w][tid] = tot;
-                    s_wc[(q + 1) & 1][w][tid] = 0;
-                    tot += cw;
+                    ex[w] = tot;
+                    tot += c[w];
                 }
             }
             uint32_t inc = tot;
@@ -1403,15 +1413,21 @@ void k_local(SortArgs a)
 #pragma unroll
                 for (int i = 0; i < kWaves; ++i) pre += (i < wave) ? s_wsum[i] : 0u;
                 const uint32_t lstart = pre + inc - tot;
+                // the offsets, and the other set (last read before the previous
+                // barrier) cleared for the next pass
+                uint32_t* nxt = s_wc[(q + 1) & 1];
 #pragma unroll
-                for (int w = 0; w < W; ++w) cs[w][tid] += lstart;
+                for (int w = 0; w < W; ++w) {
+                    cs[tid * CS + w] = ex[w] + lstart;
+                    nxt[tid * CS + w] = 0;
+                }
             }
             lds_barrier();
             // the slots of the next pass; the last pass gives each loaded slot its final slot
 #pragma unroll
             for (int j = 0; j < IM; ++j) {
                 if (valid(j)) {
-                    const uint32_t p = wc[KO::digit(key[j], d)] + rnk[j];
+                    const uint32_t p = wc[KO::digit(key[j], d) * CS] + rnk[j];
                     s_key[p] = key[j];
                     if (last)
                         s_pos[org[j]] = static_cast<uint16_t>(p);
