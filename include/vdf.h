@@ -376,12 +376,12 @@ vdf_status vdf_groupby_batch(vdf_handle* h, int key_type, const vdf_table* table
  *                    bytes; columns 0 and 1 when both are 8 bytes) into key order, so
  *                    the aggregate kernel reads them sequentially; 0: every column is
  *                    gathered through the permutation
- *   "sort_local"     1 (default): a one-table sort of 4-byte keys with three or more
- *                    non-constant 8-bit digits scatters only its top two globally and
- *                    sorts the buckets of equal top digits (up to 4096 rows each) in
- *                    LDS, one streaming pass that writes the outputs; decided on the
- *                    device from the digit histograms, with the plain passes as the
- *                    fallback; 0: plain passes only
+ *   "sort_local"     1 (default): a one-table sort of 4-byte keys whose four 8-bit
+ *                    digits all vary scatters only its top two globally and sorts the
+ *                    buckets of equal top digits (up to 3840 rows each) in LDS, one
+ *                    streaming pass that writes the outputs; decided on the device from
+ *                    the digit histograms, with the plain passes as the fallback;
+ *                    0: plain passes only
  * No reference counterpart: the reference fixes its digit width and thread
  * count at compile time (radix_sort_uint32_mt.inl:3-5). */
 vdf_status vdf_tune_set(vdf_handle* h, const char* name, int64_t value);

@@ -107,13 +107,25 @@ def test_local_sort_by_perm64_and_without_perm(vdf):
     _check_sort_by(vdf, keys, 8, want_perm=False)
 
 
-@pytest.mark.parametrize("bits", [17, 20, 24, 28])
-def test_local_three_digit_keys(vdf, bits):
-    """Keys with three non-constant digits: one digit sorted in LDS (plus the
-    top digits that vary inside a chunk)."""
+@pytest.mark.parametrize("bits", [25, 28, 31])
+def test_local_narrower_keys(vdf, bits):
+    """Keys whose top digit holds only a few bits (buckets of equal top
+    digits cover several values of it)."""
     n = 1_200_011
     _say("local bits", bits)
     _check_sort_by(vdf, _u32(bits, n, bits), 8)
+
+
+@pytest.mark.parametrize("bits", [17, 24])
+def test_three_digit_keys_stay_plain(vdf, bits):
+    """With three non-constant digits the local finish would replace one pass
+    at the cost of more than one: the plain passes run."""
+    n = 1_200_011
+    keys = _u32(bits, n, bits)
+    (perm, out, sk), launches, passes = _run(vdf, 1, lambda: _sort_by(vdf, keys, _vals(n, 8, 2), "asc"))
+    assert launches == 0 and passes == 3
+    exp = oracle.argsort(keys, 0, 4).astype(np.int64)
+    assert np.array_equal(_perm_host(perm).astype(np.int64), exp)
 
 
 def test_local_argsort(vdf):
