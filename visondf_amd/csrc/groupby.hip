@@ -640,7 +640,7 @@ __device__ __attribute__((always_inline)) inline void gb_column(const GbArgs& a,
         ex = agg_empty();
         exf = 0;
     }
-    __syncthreads();
+    lds_barrier();
     // carry from the waves before this one, in wave order
     Agg wc = agg_empty();
     uint32_t wcf = 0;
@@ -679,7 +679,7 @@ __device__ __attribute__((always_inline)) inline void gb_column(const GbArgs& a,
         if (bf) suf[tile] = bx;
         else pre[tile] = bx;
     }
-    __syncthreads();  // every staged group of window 0 is in LDS
+    lds_barrier();  // every staged group of window 0 is in LDS
     // groups 0 .. H-2 finish in this tile (group H-1 runs into the next tiles)
     const uint32_t nfin = H ? H - 1 : 0u;
     // the row loop stages this lane's groups of tile-local ids h0 .. h0 + nheads - 2
@@ -691,7 +691,7 @@ __device__ __attribute__((always_inline)) inline void gb_column(const GbArgs& a,
             const Agg g{st->cnt[h], st->a[h], st->b[h], st->mn[h], st->mx[h]};
             write_final(a, co, c, gbase + lo + h, g);
         }
-        __syncthreads();  // s_wagg and the stage are reused (next window, next column)
+        lds_barrier();  // s_wagg and the stage are reused (next window, next column)
         lo += kGbStage;
         if (lo >= nfin) break;  // block-uniform: H is the tile's
         // only lanes with a group in this window read their values again
@@ -700,7 +700,7 @@ __device__ __attribute__((always_inline)) inline void gb_column(const GbArgs& a,
             row_loop(lo);
         }
         if (seg_fin) stage_final(st, gbase + lo, gfirst - 1, seg);
-        __syncthreads();
+        lds_barrier();
     }
 }
 
@@ -738,7 +738,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     // claim a tile, find its heads, publish its head count; false at the end
     auto begin = [&](GbTileState& t) -> bool {
         if (tid == 0) s_tile = atomicAdd(a.ticket, 1u);
-        __syncthreads();
+        lds_barrier();
         t.tile = __builtin_amdgcn_readfirstlane(s_tile);
         if (t.tile >= a.ntiles) return false;
         t.sgi = a.tile_seg ? __builtin_amdgcn_readfirstlane(a.tile_seg[t.tile]) : 0u;
@@ -761,7 +761,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             if (lane >= off) inc += y;
         }
         if (lane == 63) s_wsum[wave] = inc;
-        __syncthreads();
+        lds_barrier();
         uint32_t pre_w = 0;
 #pragma unroll
         for (int i = 0; i < kWaves; ++i) pre_w += (i < wave) ? s_wsum[i] : 0u;
@@ -822,7 +822,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                 }
             }
         }
-        __syncthreads();
+        lds_barrier();
         t.gbase = __builtin_amdgcn_readfirstlane(s_gbase);
     };
 
@@ -881,7 +881,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                         }
                     }
                 }
-                __syncthreads();
+                lds_barrier();
                 const uint32_t nst = std::min<uint32_t>(t.H - lo, kGbStage);
                 for (uint32_t h = tid; h < nst; h += kBlock) {
                     const uint32_t gg = t.gbase + lo + h;
@@ -890,7 +890,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                         if (gkeys) put_key(gkeys, gg, staged_key<K>(&s_stage, h));
                     }
                 }
-                __syncthreads();  // the stage is reused (next window, the columns)
+                lds_barrier();  // the stage is reused (next window, the columns)
             }
         }
         for (int c = 0; c < a.ncols; ++c) {

@@ -416,9 +416,6 @@ __global__ __launch_bounds__(kBlock) void k_plan(SortArgs a)
     }
 }
 
-// LDS-only block barrier: orders LDS traffic between the block's waves
-// without waiting for the block's outstanding global stores.
-__device__ inline void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
 __device__ inline uint32_t block_excl_scan_lds(uint32_t x, uint32_t* s_wsum)
 {

@@ -335,6 +335,11 @@ void groupby_aggregate_enqueue(Handle* h, int key_type, const void* skeys, const
 uint32_t key_col_width(int type, uint32_t width);
 uint32_t composite_width(const int* types, const uint32_t* widths, int ncols);
 int composite_key_type(uint32_t width);
+// LDS-only block barrier: orders LDS traffic between the block's waves
+// without waiting for the block's outstanding global loads and stores (which
+// __syncthreads waits for).
+__device__ inline void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
 // Exclusive scan of one value per thread over a 256-thread block.
 __device__ inline uint32_t block_excl_scan(uint32_t x, uint32_t* s_wsum)
 {
