@@ -309,9 +309,13 @@ def run_argsort_u32(args) -> dict:
         two = {"method": "vdf_argsort + vdf_gather", "ms_per_step": ms2 / args.steps,
                "rows_per_s": n * args.steps / (ms2 * 1e-3), "kernels": kernels2}
 
+    # the plan is chosen on the device: global passes and the LDS local finish per step
+    passes = kernels.get("radix_pass", {}).get("launches", 0) / args.steps
+    local = kernels.get("radix_local", {}).get("launches", 0) / args.steps
     rec = _base_record(args, rows_per_s, "rows/s", ms_step, "u32",
                        {"workload": "argsort_u32_gather_f64", "rows": n, "key": "uint32 uniform over 2^32",
-                        "value_column": "float64", "digit_bits": 8, "radix_passes": 4,
+                        "value_column": "float64", "digit_bits": 8, "radix_passes": passes,
+                        "local_finish_16bit": bool(local),
                         "method": "vdf_sort_by (column carried through the passes)",
                         "parallelism": "single GPU"})
     rec["hbm_gbps_step"] = step_bytes / (ms_step * 1e-3) / 1e9
