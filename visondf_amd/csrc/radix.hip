@@ -673,28 +673,33 @@ void k_pass(SortArgs a, uint32_t slot)
         // rows past the end read the segment's last row and are masked below.
         const bool full = c.lrow0 + TILE <= c.n;
         const uint64_t last = c.n - 1;
-        if constexpr (PB != 0) {
-            const auto* pp = c.first ? gmem(static_cast<const P*>(c.pay_in))
-                                     : gmem(static_cast<const P*>(a.pbuf[(slot - 1) & 1]) + c.row_off);
-            if (full) {
+        // payload loads, issued after the key loads: vmcnt counts in issue
+        // order, so the rank phase waits for the keys alone and the payload
+        // stays in flight through the rank and the look-back
+        auto load_pay = [&]() __attribute__((always_inline)) {
+            if constexpr (PB != 0) {
+                const auto* pp = c.first ? gmem(static_cast<const P*>(c.pay_in))
+                                         : gmem(static_cast<const P*>(a.pbuf[(slot - 1) & 1]) + c.row_off);
+                if (full) {
 #pragma unroll
-                for (int j = 0; j < PI; ++j) pay[j] = pp[wl + static_cast<uint64_t>(j) * 64];
-            } else {
+                    for (int j = 0; j < PI; ++j) pay[j] = pp[wl + static_cast<uint64_t>(j) * 64];
+                } else {
 #pragma unroll
-                for (int j = 0; j < PI; ++j) pay[j] = pp[std::min(wl + static_cast<uint64_t>(j) * 64, last)];
+                    for (int j = 0; j < PI; ++j) pay[j] = pp[std::min(wl + static_cast<uint64_t>(j) * 64, last)];
+                }
             }
-        }
-        if constexpr (TWO) {
-            const auto* pp = c.first ? gmem(static_cast<const P2*>(c.pay_in2))
-                                     : gmem(static_cast<const P2*>(a.pbuf2[(slot - 1) & 1]) + c.row_off);
-            if (full) {
+            if constexpr (TWO) {
+                const auto* pp = c.first ? gmem(static_cast<const P2*>(c.pay_in2))
+                                         : gmem(static_cast<const P2*>(a.pbuf2[(slot - 1) & 1]) + c.row_off);
+                if (full) {
 #pragma unroll
-                for (int j = 0; j < PI2; ++j) pay2[j] = pp[wl + static_cast<uint64_t>(j) * 64];
-            } else {
+                    for (int j = 0; j < PI2; ++j) pay2[j] = pp[wl + static_cast<uint64_t>(j) * 64];
+                } else {
 #pragma unroll
-                for (int j = 0; j < PI2; ++j) pay2[j] = pp[std::min(wl + static_cast<uint64_t>(j) * 64, last)];
+                    for (int j = 0; j < PI2; ++j) pay2[j] = pp[std::min(wl + static_cast<uint64_t>(j) * 64, last)];
+                }
             }
-        }
+        };
         if (c.first) {
             if (full) {
 #pragma unroll
@@ -704,6 +709,7 @@ void k_pass(SortArgs a, uint32_t slot)
                 for (int j = 0; j < ITEMS; ++j)
                     key[j] = KO::fetch(c.keys, std::min(wl + static_cast<uint64_t>(j) * 64, last));
             }
+            load_pay();
 #pragma unroll
             for (int j = 0; j < ITEMS; ++j) {
                 key[j] = KO::xform(key[j], a.mlo, a.mhi);
@@ -720,6 +726,7 @@ void k_pass(SortArgs a, uint32_t slot)
 #pragma unroll
                     for (int j = 0; j < ITEMS; ++j) r[j] = rp[std::min(wl + static_cast<uint64_t>(j) * 64, last)];
                 }
+                load_pay();
 #pragma unroll
                 for (int j = 0; j < ITEMS; ++j) {
                     key[j] = static_cast<K>(r[j]);
@@ -744,6 +751,7 @@ void k_pass(SortArgs a, uint32_t slot)
                     for (int j = 0; j < ITEMS; ++j) val[j] = vp[std::min(wl + static_cast<uint64_t>(j) * 64, last)];
                 }
             }
+            load_pay();
             if (!ROWS || a.norows) {
 #pragma unroll
                 for (int j = 0; j < ITEMS; ++j) val[j] = 0u;
