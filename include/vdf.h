@@ -382,6 +382,11 @@ vdf_status vdf_groupby_batch(vdf_handle* h, int key_type, const vdf_table* table
  *                    streaming pass that writes the outputs; decided on the device from
  *                    the digit histograms, with the plain passes as the fallback;
  *                    0: plain passes only
+ *   "sort_rank"      1 (default): the radix passes and the local finish rank rows with one
+ *                    LDS atomic per row where the device serves same-address lanes in lane
+ *                    order (checked once per device by vdf_create); 0: wave ballots.
+ *                    vdf_tune_get("sort_rank_atomic") reads back (only) whether the
+ *                    atomic ranking is in use on the handle's device
  * No reference counterpart: the reference fixes its digit width and thread
  * count at compile time (radix_sort_uint32_mt.inl:3-5). */
 vdf_status vdf_tune_set(vdf_handle* h, const char* name, int64_t value);

@@ -206,3 +206,50 @@ def test_local_groupby_full_range(vdf, combo):
     assert r1.n_groups == o["n_groups"]
     assert np.array_equal(r1.keys.cpu().numpy(), o["keys"])
     assert np.array_equal(r1.counts.cpu().numpy().astype(np.uint64), o["counts"])
+
+
+@pytest.mark.parametrize("case", ["sort_by_local", "sort_by_plain", "argsort_u64", "groupby_full", "zipf"])
+def test_ballot_and_atomic_ranking_identical(vdf, case):
+    """The per-row LDS atomic ranking (used where the device serves same-address
+    lanes in lane order) and the wave-ballot ranking give bit-identical
+    results on every path that ranks: plain passes, the local finish, 8-byte
+    keys, group-bys."""
+    import torch
+
+    if vdf.tune_get("sort_rank_atomic") != 1:
+        pytest.skip("the device check did not verify the atomic ranking: both runs would use ballots")
+    n = 1_000_003
+    rng = np.random.default_rng(17)
+
+    def run():
+        if case.startswith("sort_by"):
+            keys = _u32(5, n)
+            vals = _vals(n, 8, 5)
+            return _run(vdf, 1 if case == "sort_by_local" else 0,
+                        lambda: _sort_by(vdf, keys, vals, "reverse"))[0]
+        if case == "argsort_u64":
+            keys = datagen.splitmix64(6, 0, n)
+            return vdf.argsort(_dev(keys), key_type=N.KEY_U64, want_keys=True)
+        if case == "groupby_full":
+            keys = _u32(9, n).view(np.int32)
+            r = vdf.groupby(_dev(keys), [_dev(rng.standard_normal(n))])
+            return (r.keys, r.counts, r.perm, r.aggs[0]["sum"], r.aggs[0]["min"])
+        keys = datagen.scramble24(datagen.zipf_ranks(2, 0, n, 1.1, 1 << 24)).astype(np.int32)
+        r = vdf.groupby(_dev(keys), [_dev(rng.integers(-10**6, 10**6, n).astype(np.int32))])
+        return (r.keys, r.counts, r.perm, r.aggs[0]["sum"], r.aggs[0]["max"])
+
+    old = vdf.tune_get("sort_rank")
+    try:
+        vdf.tune("sort_rank", 0)
+        assert vdf.tune_get("sort_rank_atomic") == 0
+        ballot = run()
+        rng = np.random.default_rng(17)
+        vdf.tune("sort_rank", 1)
+        atomic = run()
+    finally:
+        vdf.tune("sort_rank", old)
+    for b, a in zip(ballot, atomic):
+        if b is None:
+            assert a is None
+            continue
+        assert torch.equal(b.view(torch.uint8), a.view(torch.uint8))

@@ -870,6 +870,7 @@ static int* tune_slot(Tuning& t, const std::string& n)
     if (n == "gather_nt") return &t.gather_nt;
     if (n == "csv_stage") return &t.csv_stage;
     if (n == "sort_local") return &t.sort_local;
+    if (n == "sort_rank") return &t.sort_rank;
     if (n == "groupby_carry") return &t.gb_carry;
     return nullptr;
 }
@@ -891,6 +892,10 @@ vdf_status vdf_tune_get(vdf_handle* h, const char* name, int64_t* value)
     return guarded([&] {
         require(h != nullptr && name != nullptr && value != nullptr, "vdf_tune_get: null argument");
         std::lock_guard<std::mutex> lk(h->mu);
+        if (std::string(name) == "sort_rank_atomic") {  // read-only: the ranking the sorts use
+            *value = (h->tune.sort_rank && lds_rank_verified(h->device)) ? 1 : 0;
+            return;
+        }
         int* slot = tune_slot(h->tune, name);
         require(slot != nullptr, "vdf_tune_get: unknown tuning knob");
         *value = *slot;

@@ -1114,7 +1114,7 @@ bool lds_rank_check(int device)
 }
 
 // the check's result when it ran on `device`, else false (the ballot ranking)
-static bool lds_rank_cached(int device)
+bool lds_rank_verified(int device)
 {
     std::lock_guard<std::mutex> lk(g_lds_rank_mu);
     for (const auto& e : g_lds_rank)
@@ -1753,6 +1753,7 @@ bool tune_supported(const char* name, int64_t v)
     if (n == "csv_stage") return v == 0 || v == 1;
     if (n == "groupby_carry") return v == 0 || v == 1;
     if (n == "sort_local") return v == 0 || v == 1;
+    if (n == "sort_rank") return v == 0 || v == 1;
     return false;
 }
 
@@ -1913,7 +1914,7 @@ static void launch_sort(Handle* h, const SortSeg* segs, int nsegs, uint64_t mlo,
     LocalFn local = nullptr;
     if (L.lchunks && h->tune.sort_local && all_rows >= kLocalMinSortRows) local = pick_local(pay_bytes, rowids);
     a.local_ok = local ? 1u : 0u;
-    a.lds_rank = lds_rank_cached(h->device) ? 1u : 0u;
+    a.lds_rank = (h->tune.sort_rank && lds_rank_verified(h->device)) ? 1u : 0u;
     if (devdesc) {
         const unsigned blocks = static_cast<unsigned>((nitems + kBlock - 1) / kBlock);
         hipLaunchKernelGGL(k_setup_one, dim3(blocks), dim3(kBlock), 0, s, one, dsegs, ditems, nitems, hist_rows);

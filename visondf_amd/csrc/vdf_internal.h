@@ -148,6 +148,7 @@ struct Tuning {
     int csv_stage = 1;       // CSV row kernels parse each wave's rows from a copy in LDS
     int gb_carry = 1;        // group-by: value columns carried through the sort (1) or gathered (0)
     int sort_local = 1;      // 4-byte keys: finish a sort in LDS after two global passes when its buckets fit
+    int sort_rank = 1;       // 1: one LDS atomic per row ranks (where the device check passed); 0: wave ballots
 };
 
 bool tune_supported(const char* name, int64_t value);
@@ -382,6 +383,7 @@ void unpack_keys_enqueue(Handle* h, const int* types, const uint32_t* widths, vo
 // handles are made (vdf_create, server workers); sorts on a device where it
 // has not run use the ballot ranking.
 bool lds_rank_check(int device);
+bool lds_rank_verified(int device);  // the check's result where it ran, else false
 
 // fill kernel (graph-safe replacement for hipMemsetAsync)
 void fill_u32(uint32_t* p, size_t count, uint32_t value, hipStream_t s);
