@@ -378,7 +378,8 @@ vdf_status vdf_groupby_batch(vdf_handle* h, int key_type, const vdf_table* table
  *                    gathered through the permutation
  *   "sort_local"     1 (default): a one-table sort of 4-byte keys whose four 8-bit
  *                    digits all vary scatters only its top two globally and sorts the
- *                    buckets of equal top digits (up to 3840 rows each) in LDS, one
+ *                    buckets of equal top digits (up to 3840 rows each; 4096 from 2^24
+ *                    rows on) in LDS, one
  *                    streaming pass that writes the outputs; decided on the device from
  *                    the digit histograms, with the plain passes as the fallback;
  *                    0: plain passes only

@@ -154,14 +154,60 @@ def _equal_buckets(n_per: int, seed: int) -> np.ndarray:
 
 
 def test_local_bucket_size_limit(vdf):
-    """Buckets of exactly 3840 rows (4096 - 256, the largest a chunk takes)
-    finish locally; 3841-row buckets fall back to the plain passes, which then
-    read the caller's columns again and must give the same result."""
+    """Stride rule (tables under 2^24 rows): buckets of exactly 3840 rows
+    (4096 - 256, the largest it takes) finish locally; 3841-row buckets fall
+    back to the plain passes, which then read the caller's columns again and
+    must give the same result."""
     _say("local limit 3840")
     _check_sort_by(vdf, _equal_buckets(3840, 1), 8)
     _say("local fallback 3841")
     _check_sort_by(vdf, _equal_buckets(3841, 2), 8, expect_local=False)
     _check_sort_by(vdf, _equal_buckets(3841, 3), 4, "reverse", expect_local=False)
+
+
+def _pair_buckets(n_per: int, seed: int) -> np.ndarray:
+    """4096 buckets of exactly n_per rows (at least 2^24 rows in all, so the
+    chunks are packed greedily); every value of digits 2 and 3 occurs in 16
+    of them, so each digit alone looks uniform."""
+    i = np.arange(4096, dtype=np.uint32)
+    y = i % 256
+    x = i // 256 + 16 * (i % 16)
+    top = np.repeat((y << np.uint32(8)) | x, n_per)
+    np.random.default_rng(seed).shuffle(top)
+    return (top << np.uint32(16)) | _u32(seed, top.shape[0], 16)
+
+
+def test_local_greedy_bucket_size_limit(vdf):
+    """Greedy packing (tables of 2^24 rows or more): buckets of exactly 4096
+    rows, a whole chunk each, finish locally; 4097-row buckets fall back."""
+    _say("local greedy limit 4096")
+    _check_sort_by(vdf, _pair_buckets(4096, 5), 8)
+    _say("local greedy fallback 4097")
+    _check_sort_by(vdf, _pair_buckets(4097, 6), 4, "desc", expect_local=False)
+
+
+@pytest.mark.parametrize("mode", ["asc", "reverse"])
+def test_local_greedy_uniform(vdf, mode):
+    """20M uniform keys: chunks packed greedily hold about 13 buckets each."""
+    _say("local greedy uniform", mode)
+    _check_sort_by(vdf, _u32(20, 20_000_003), 8, mode)
+
+
+def _two_bucket_keys(n: int, seed: int) -> np.ndarray:
+    """Digit 3 in [0, 48): 12,288 buckets of about 1,630 rows, so greedy
+    chunks hold two adjacent buckets and finish with the 9-bit pass."""
+    top = (datagen.splitmix64(seed, 0, n) % np.uint64(48 * 256)).astype(np.uint32)
+    return (top << np.uint32(16)) | _u32(seed + 1, n, 16)
+
+
+@pytest.mark.parametrize("mode", ["asc", "desc", "reverse"])
+def test_local_two_bucket_chunks(vdf, mode):
+    _say("local two-bucket chunks", mode)
+    keys = _two_bucket_keys(20_000_011, 7)
+    if mode != "asc":
+        keys = keys ^ np.uint32(0xFFFF0000)  # the same buckets at the top of the range
+    _check_sort_by(vdf, keys, 8, mode)
+    _check_sort_by(vdf, keys.view(np.int32), 4, mode)
 
 
 def test_local_skewed_keys_stay_plain(vdf):
@@ -208,7 +254,8 @@ def test_local_groupby_full_range(vdf, combo):
     assert np.array_equal(r1.counts.cpu().numpy().astype(np.uint64), o["counts"])
 
 
-@pytest.mark.parametrize("case", ["sort_by_local", "sort_by_plain", "argsort_u64", "groupby_full", "zipf"])
+@pytest.mark.parametrize("case", ["sort_by_local", "sort_by_plain", "sort_by_two_bucket", "argsort_u64",
+                                  "groupby_full", "zipf"])
 def test_ballot_and_atomic_ranking_identical(vdf, case):
     """The per-row LDS atomic ranking (used where the device serves same-address
     lanes in lane order) and the wave-ballot ranking give bit-identical
@@ -222,6 +269,11 @@ def test_ballot_and_atomic_ranking_identical(vdf, case):
     rng = np.random.default_rng(17)
 
     def run():
+        if case == "sort_by_two_bucket":
+            keys = _two_bucket_keys(20_000_011, 3)
+            out, launches, _ = _run(vdf, 1, lambda: _sort_by(vdf, keys, _vals(keys.shape[0], 8, 3), "asc"))
+            assert launches == 1
+            return out
         if case.startswith("sort_by"):
             keys = _u32(5, n)
             vals = _vals(n, 8, 5)

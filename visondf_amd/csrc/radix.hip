@@ -212,12 +212,19 @@ static_assert(kHdrGb < kPlanHdr, "plan header fields");
 // Local finish (4-byte keys, one segment, four non-constant digits): two
 // global passes over the two top digits leave every bucket of equal top digits
 // contiguous; k_local then sorts chunks of whole buckets of at most
-// kLocalRows rows in LDS.  A bucket larger than kLocalRows - kLocalMinH
-// rows makes k_local_chunks restart the plain passes at slot 2.
+// kLocalRows rows in LDS.  Below kLocalGreedyRows rows chunks are cut at a
+// fixed stride (a bucket larger than kLocalRows - kLocalMinH rows makes
+// k_local_chunks restart the plain passes at slot 2); from kLocalGreedyRows
+// rows on they are packed greedily inside each run of 256 buckets of equal
+// top digit (a bucket larger than kLocalRows restarts the plain passes).
 constexpr uint32_t kLocalRows = 4096;
 constexpr int kLocalBlock = 512;  // threads of a k_local block (8 waves)
 constexpr uint32_t kLocalMinH = 256;  // smallest chunk stride
 constexpr uint64_t kLocalMinSortRows = 1u << 16;  // smallest sort that may finish locally
+constexpr uint32_t kLocalGreedyRows = 1u << 24;   // greedy packing from here on
+// chunk words (start, end pairs): the stride rule makes at most n / kLocalMinH
+// + 1 chunks, the greedy one at most 2 n / kLocalRows + 256
+constexpr uint64_t local_chunk_words(uint64_t rows) { return 2 * (rows / kLocalMinH + 2) + 2 * kRadix; }
 constexpr int kLocalMaxSlots = kLocalSlot0 + 4;
 // Ticket words of a pass slot: one 64-bit claim word per XCD (kXcds), then
 // the slot's chunk counter (also the tile counter of global order).  The last
@@ -438,18 +445,19 @@ __device__ inline uint32_t block_excl_scan_lds(uint32_t x, uint32_t* s_wsum)
 // ---------------------------------------------------------------------------
 // stable ranks inside a wave
 // ---------------------------------------------------------------------------
-// Ballot ranking: eight ballots, one per digit bit, give each lane the lanes
+// Ballot ranking: one ballot per digit bit (BITS of them) gives each lane the lanes
 // sharing its digit; every lane reads the wave's counter of its digit and the
 // lowest peer writes it back advanced by the peer count, so old + (peers below
 // the lane) is the lane's stable rank.  (LDS operations of one wave complete
 // in program order.)
 // (c: the lane's counter, the wave's counter of digit d)
+template <int BITS = kRadixBits>
 __device__ inline uint32_t rank_ballot_at(uint32_t d, bool v, uint32_t* c)
 {
     const uint64_t vm = __ballot(v);
     uint32_t pl = static_cast<uint32_t>(vm), ph = static_cast<uint32_t>(vm >> 32);
 #pragma unroll
-    for (int b = 0; b < 8; ++b) {
+    for (int b = 0; b < BITS; ++b) {
         // v_bitop3 truth table 0x90 = src0 & ~(src1 ^ src2): keep the lanes that agree on bit b
         const uint32_t bm = static_cast<uint32_t>(static_cast<int32_t>(d << (31 - b)) >> 31);
         const uint64_t mb = __ballot(bm != 0u);
@@ -1140,7 +1148,7 @@ __global__ __launch_bounds__(kBlock) void k_local_bounds(SortArgs a, uint32_t ti
     if (!(hdr[kHdrMode] & kModeLocal)) return;
     const int tid = threadIdx.x;
     const uint32_t x = blockIdx.x;
-    if (x == 0 && tid == 0) a.lctl[1] = 0;  // k_local_max's maximum
+    if (x == 0 && tid == 0) a.lctl[0] = a.lctl[1] = 0;  // chunk count, k_local_max's maximum
     const uint32_t ga = hdr[kHdrGa], gb = hdr[kHdrGb];
     const uint64_t n = a.segs[0].n;
     const uint32_t* bgb = a.plan_base + static_cast<uint64_t>(gb) * kRadix;
@@ -1179,13 +1187,26 @@ __global__ __launch_bounds__(kBlock) void k_local_max(SortArgs a)
     if (tid == 0) atomicMax(a.lctl + 1, std::max(std::max(s_m[0], s_m[1]), std::max(s_m[2], s_m[3])));
 }
 
-// Cuts the rows into chunks of whole buckets, or falls back.  With M the
-// largest bucket and H = kLocalRows - M, chunk k holds the buckets that start
-// in [kH, (k+1)H): it starts at the first bucket start >= kH and ends before
-// the first one >= (k+1)H, so it has fewer than H + M = kLocalRows rows.
-// M > kLocalRows - kLocalMinH restarts the plain passes at slot kLocalSlot0
-// (they read the caller's columns again).  Thread x of block y takes bucket
-// b = y * 256 + x; block 0 also takes the end (b = 65536, start n).
+// Cuts the rows into chunks of whole buckets of at most kLocalRows rows, as
+// (start, end) pairs in lchunk with their count in lctl[0], or falls back.
+// Thread x of block y takes bucket b = y * 256 + x, which starts at
+// lstart[y][x]; lstart[y][256] is the end of the run of buckets y.
+//  - Stride rule (fewer than kLocalGreedyRows rows): with M the largest
+//    bucket and H = kLocalRows - M, chunk k holds the buckets that start in
+//    [kH, (k+1)H): it starts at the first bucket start >= kH and ends before
+//    the first one >= (k+1)H, so it has fewer than H + M = kLocalRows rows.
+//    M > kLocalRows - kLocalMinH falls back.
+//  - Greedy rule: wave 0 of block y packs run y, each chunk taking as many
+//    whole buckets as fit in kLocalRows rows (a chunk and the next hold more
+//    than kLocalRows rows together, so a run of R rows gives at most
+//    2R / kLocalRows + 1 chunks).  Each lane finds, for its buckets, where a
+//    chunk beginning there would end (binary search over the run's starts);
+//    the wave then follows that chain from bucket 0 in registers, reserves
+//    the run's slots with one atomic add and writes the chunks (slot order
+//    does not matter: every chunk writes its own rows).  M > kLocalRows
+//    falls back.
+// The fallback restarts the plain passes at slot kLocalSlot0 (they read the
+// caller's columns again).
 __global__ __launch_bounds__(kBlock) void k_local_chunks(SortArgs a)
 {
     uint32_t* hdr = a.plan_hdr;
@@ -1193,7 +1214,8 @@ __global__ __launch_bounds__(kBlock) void k_local_chunks(SortArgs a)
     const uint32_t m = a.lctl[1];
     const uint32_t n = static_cast<uint32_t>(a.segs[0].n);
     const uint32_t tid = threadIdx.x, y = blockIdx.x;
-    if (m > kLocalRows - kLocalMinH) {
+    const bool greedy = n >= kLocalGreedyRows;
+    if (m > (greedy ? kLocalRows : kLocalRows - kLocalMinH)) {
         // (blocks that read the mode after this rewrite return as well)
         if (y == 0 && tid == 0) {
             const uint32_t np = hdr[kHdrNS];
@@ -1205,21 +1227,83 @@ __global__ __launch_bounds__(kBlock) void k_local_chunks(SortArgs a)
         }
         return;
     }
+    const uint32_t* row = a.lstart + static_cast<uint64_t>(y) * (kRadix + 1);
+    uint32_t* ch = a.lchunk;
+    if (greedy) {
+        __shared__ uint32_t s_st[kRadix + 1];
+        s_st[tid] = row[tid];  // (blockDim.x == kRadix)
+        if (tid == 0) s_st[kRadix] = row[kRadix];
+        __syncthreads();
+        if (tid >= 64) return;
+        const uint32_t lane = tid;
+        // bucket x = 64 i + lane: its start, and nx = the last bucket index
+        // (256 = the run's end) whose start lies within start + kLocalRows,
+        // where a chunk that begins at x ends (a bucket has at most kLocalRows
+        // rows, so nx > x; the max() only keeps the walk finite regardless)
+        uint32_t st[4], nx[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const uint32_t x = static_cast<uint32_t>(i) * 64 + lane;
+            st[i] = s_st[x];
+            const uint32_t lim = st[i] + kLocalRows;
+            uint32_t lo = x, hi = kRadix;  // s_st[lo] <= lim; find the largest such index
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi + 1) >> 1;
+                if (s_st[mid] <= lim) lo = mid;
+                else hi = mid - 1;
+            }
+            nx[i] = std::max(lo, x + 1);
+        }
+        // the chain of chunk starts from bucket 0, marked in the lanes that hold them
+        uint32_t mark = 0;
+        if (s_st[0] < s_st[kRadix]) {
+            uint32_t x = 0;
+            while (x < kRadix) {
+                const uint32_t q = x >> 6, l = x & 63;
+                if (lane == l) mark |= 1u << q;
+                const uint32_t v = q == 0 ? nx[0] : q == 1 ? nx[1] : q == 2 ? nx[2] : nx[3];
+                x = __builtin_amdgcn_readlane(v, l);
+            }
+        }
+        // slots: one atomic add per run, chunks in bucket order inside it
+        uint32_t below[4], cnt = 0;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const uint64_t b = __ballot((mark >> i) & 1u);
+            below[i] = cnt + __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(b >> 32),
+                                                       __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(b), 0u));
+            cnt += static_cast<uint32_t>(__popcll(b));
+        }
+        if (cnt == 0) return;
+        uint32_t slot = 0;
+        if (lane == 0) slot = atomicAdd(a.lctl, cnt);
+        slot = __builtin_amdgcn_readfirstlane(slot);
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            if ((mark >> i) & 1u) {
+                const uint64_t k = slot + below[i];
+                ch[2 * k] = st[i];
+                ch[2 * k + 1] = s_st[nx[i]];
+            }
+        return;
+    }
     const uint32_t H = kLocalRows - m;
     const uint32_t K = (n + H - 1) / H;
     // chunks k with start(b - 1) < kH <= start(b) start at start(b) (each k
-    // once: an empty bucket's start equals the next one's)
+    // once: an empty bucket's start equals the next one's); chunk k - 1 ends there
     auto cut = [&](uint32_t s, uint32_t p) {
-        for (uint32_t k = p / H + 1; k <= s / H && k < K; ++k) a.lchunk[k] = s;
+        for (uint32_t k = p / H + 1; k <= s / H && k < K; ++k) {
+            ch[2 * static_cast<uint64_t>(k)] = s;
+            ch[2 * static_cast<uint64_t>(k) - 1] = s;
+        }
     };
-    const uint32_t* row = a.lstart + static_cast<uint64_t>(y) * (kRadix + 1);
     const uint32_t s = row[tid];
     // the bucket before (y, 0) is (y - 1, 255): row[-2] (row[-1] is its end)
     if (y > 0 || tid > 0) cut(s, tid > 0 ? row[tid - 1] : row[-2]);
     if (y == kRadix - 1 && tid == kRadix - 1) cut(n, row[tid]);
     if (y == 0 && tid == 0) {
-        a.lchunk[0] = 0;
-        a.lchunk[K] = n;
+        ch[0] = 0;
+        ch[2 * static_cast<uint64_t>(K) - 1] = n;
         a.lctl[0] = K;
     }
 }
@@ -1231,8 +1315,11 @@ __global__ __launch_bounds__(kBlock) void k_local_chunks(SortArgs a)
 // rows in order, per-wave bucket counters combined in wave order).  Row ids
 // stay in the registers of the thread that loaded them; each column reaches
 // its rows' final slots through LDS and leaves in slot order, so every output
-// is written by consecutive lanes.  8 waves per block of at most 8 rows per
-// thread, 3 blocks per CU.
+// is written by consecutive lanes.  A chunk of two adjacent buckets (digit ga
+// x and x + 1, one digit gb) needs the order of digit ga restored after the
+// low digits; its last low pass instead ranks a 9-bit digit, the low digit
+// plus bit 8 = (digit ga) - x, which saves that pass.  8 waves per block of
+// at most 8 rows per thread, 2 blocks per CU.
 template <int PB, bool ROWS>
 __global__ __launch_bounds__(kLocalBlock) __attribute__((amdgpu_waves_per_eu(4)))
 void k_local(SortArgs a)
@@ -1253,8 +1340,10 @@ void k_local(SortArgs a)
     // thread reads and writes with paired-dword operations, and with CS odd
     // the counters of one wave fall in distinct banks for 64 consecutive buckets.
     constexpr int CS = W + 1;
-    __shared__ uint32_t s_wc[2][kRadix * CS];
-    __shared__ uint32_t s_wsum[kWaves];
+    constexpr int NB = 2 * kRadix;  // buckets of a 9-bit pass (the one thread each)
+    static_assert(NB == kLocalBlock, "one bucket per thread in the 9-bit scan");
+    __shared__ uint32_t s_wc[2][NB * CS];
+    __shared__ uint32_t s_wsum[W];
     uint32_t* s_key = reinterpret_cast<uint32_t*>(s_raw);
     uint16_t* s_org = reinterpret_cast<uint16_t*>(s_key + kLocalRows);
     uint16_t* s_pos = s_org + kLocalRows;
@@ -1281,7 +1370,7 @@ void k_local(SortArgs a)
     for (uint32_t i = 0; i < hdr[kHdrNS]; ++i) lmask |= 1u << hdr[kHdrS + i];
     const uint32_t ga = hdr[kHdrGa], gb = hdr[kHdrGb];
     lmask &= ~((1u << ga) | (1u << gb));
-    for (int i = tid; i < kRadix * CS; i += kLocalBlock) s_wc[0][i] = s_wc[1][i] = 0;
+    for (int i = tid; i < NB * CS; i += kLocalBlock) s_wc[0][i] = s_wc[1][i] = 0;
     lds_barrier();
     uint32_t q = 0;  // passes so far (counter set q & 1)
 
@@ -1293,8 +1382,9 @@ void k_local(SortArgs a)
     // thread: wave w holds slots [w*64*I, (w+1)*64*I); slot sl0 + 64 j is row
     // r0 + sl0 + 64 j.
     const uint32_t G = gridDim.x;
-    auto bound = [&](uint32_t kk) __attribute__((always_inline)) -> uint32_t {
-        return gmem(a.lchunk)[std::min(kk, K)];  // lchunk[K] = n
+    // chunk kk's start and end (past the last chunk: empty, at n)
+    auto bound = [&](uint32_t kk, uint32_t off) __attribute__((always_inline)) -> uint32_t {
+        return kk < K ? gmem(a.lchunk)[2 * static_cast<uint64_t>(kk) + off] : n;
     };
     auto clampm = [](uint32_t r0x, uint32_t r1x) -> uint32_t {
         return r1x > r0x && r1x - r0x <= kLocalRows ? r1x - r0x : 0u;  // chunks are < kLocalRows rows
@@ -1317,11 +1407,11 @@ void k_local(SortArgs a)
     };
     uint64_t nx[IM];
     // chunk 0 of this block: bounds, records and first / last keys, chunk 1's bounds
-    uint32_t cr0 = bound(blockIdx.x), cm = clampm(cr0, bound(blockIdx.x + 1));
+    uint32_t cr0 = bound(blockIdx.x, 0), cm = clampm(cr0, bound(blockIdx.x, 1));
     load_recs(cr0, cm, nx);
     uint64_t kfl = ROWS ? recs[row_at(cr0, cm, 0)] : kin[row_at(cr0, cm, 0)];
     uint64_t kll = ROWS ? recs[row_at(cr0, cm, kLocalRows)] : kin[row_at(cr0, cm, kLocalRows)];
-    uint32_t b0 = bound(blockIdx.x + G), b1 = bound(blockIdx.x + G + 1);
+    uint32_t b0 = bound(blockIdx.x + G, 0), b1 = bound(blockIdx.x + G, 1);
     for (uint32_t k = blockIdx.x; k < K; k += G) {
         const uint32_t r0 = cr0, m = cm;  // this chunk
         const uint32_t I = (m + kLocalBlock - 1) / kLocalBlock;
@@ -1347,16 +1437,20 @@ void k_local(SortArgs a)
         load_recs(r0n, mn, nx);
         kfl = ROWS ? recs[row_at(r0n, mn, 0)] : kin[row_at(r0n, mn, 0)];
         kll = ROWS ? recs[row_at(r0n, mn, kLocalRows)] : kin[row_at(r0n, mn, kLocalRows)];
-        b0 = bound(k + 2 * G);
-        b1 = bound(k + 2 * G + 1);
+        b0 = bound(k + 2 * G, 0);
+        b1 = bound(k + 2 * G, 1);
         cr0 = r0n;
         cm = mn;
         if (m == 0) continue;
         // digits that vary inside the chunk: the low ones, and the global ones
         // where its first and last keys differ (its rows are ordered by them)
+        // (two adjacent digits ga: the last low pass also ranks bit 8 = digit ga - fa)
         uint32_t vary = lmask;
+        const uint32_t fa = KO::digit(kf, ga);
+        bool comb = false;
         if (KO::digit(kf, gb) != KO::digit(kl, gb)) vary |= (1u << ga) | (1u << gb);
-        else if (KO::digit(kf, ga) != KO::digit(kl, ga)) vary |= 1u << ga;
+        else if (KO::digit(kl, ga) == fa + 1 && lmask) comb = true;
+        else if (KO::digit(kl, ga) != fa) vary |= 1u << ga;
         if (vary == 0) vary = 1;  // one key value: one pass leaves the order and gives the final slots
 #pragma unroll
         for (int j = 0; j < IM; ++j) org[j] = sl0 + j * 64;
@@ -1377,26 +1471,34 @@ void k_local(SortArgs a)
             }
             uint32_t* cs = s_wc[q & 1];
             uint32_t* wc = cs + wave;  // this wave's counter of bucket b: wc[b * CS]
+            // the pass's digit: 8 bits, or 9 in the last pass of a two-bucket chunk
+            const bool wide = comb && last;
+            const uint32_t nb = wide ? NB : kRadix;
+            auto digit_of = [&](uint32_t kv) __attribute__((always_inline)) -> uint32_t {
+                const uint32_t g = KO::digit(kv, d);
+                return wide ? g | (((KO::digit(kv, ga) - fa) & 1u) << kRadixBits) : g;
+            };
             if (a.lds_rank) {
 #pragma unroll
                 for (int j = 0; j < IM; ++j)
                     if (j < static_cast<int>(I)) {
                         const bool v = sl0 + j * 64 < m;
-                        rnk[j] = v ? atomicAdd(&wc[KO::digit(key[j], d) * CS], 1u) : 0u;
+                        rnk[j] = v ? atomicAdd(&wc[digit_of(key[j]) * CS], 1u) : 0u;
                     }
             } else {
+                // (nine ballots: bit 8 is zero in every lane of an 8-bit pass)
 #pragma unroll
                 for (int j = 0; j < IM; ++j)
                     if (j < static_cast<int>(I)) {
-                        const uint32_t dg = KO::digit(key[j], d);
-                        rnk[j] = rank_ballot_at(dg, sl0 + j * 64 < m, &wc[dg * CS]);
+                        const uint32_t dg = digit_of(key[j]);
+                        rnk[j] = rank_ballot_at<kRadixBits + 1>(dg, sl0 + j * 64 < m, &wc[dg * CS]);
                     }
             }
             lds_barrier();
             // bucket tid: per-wave offsets (waves in order), then the exclusive
-            // scan over buckets (threads of waves 0..3)
+            // scan over the nb buckets (one per thread)
             uint32_t tot = 0, ex[W];
-            if (tid < kRadix) {
+            if (static_cast<uint32_t>(tid) < nb) {
                 uint32_t c[W];
 #pragma unroll
                 for (int w = 0; w < W; ++w) c[w] = cs[tid * CS + w];
@@ -1412,28 +1514,29 @@ void k_local(SortArgs a)
                 const uint32_t y = __shfl_up(inc, off, 64);
                 if (lane >= off) inc += y;
             }
-            if (lane == 63 && wave < kWaves) s_wsum[wave] = inc;
+            if (lane == 63) s_wsum[wave] = inc;
             lds_barrier();
-            if (tid < kRadix) {
-                uint32_t pre = 0;
-#pragma unroll
-                for (int i = 0; i < kWaves; ++i) pre += (i < wave) ? s_wsum[i] : 0u;
-                const uint32_t lstart = pre + inc - tot;
+            {
                 // the offsets, and the other set (last read before the previous
-                // barrier) cleared for the next pass
+                // barrier) cleared for the next pass, all NB buckets
                 uint32_t* nxt = s_wc[(q + 1) & 1];
+                if (static_cast<uint32_t>(tid) < nb) {
+                    uint32_t pre = 0;
 #pragma unroll
-                for (int w = 0; w < W; ++w) {
-                    cs[tid * CS + w] = ex[w] + lstart;
-                    nxt[tid * CS + w] = 0;
+                    for (int i = 0; i < W; ++i) pre += (i < wave) ? s_wsum[i] : 0u;
+                    const uint32_t lstart = pre + inc - tot;
+#pragma unroll
+                    for (int w = 0; w < W; ++w) cs[tid * CS + w] = ex[w] + lstart;
                 }
+#pragma unroll
+                for (int w = 0; w < W; ++w) nxt[tid * CS + w] = 0;
             }
             lds_barrier();
             // the slots of the next pass; the last pass gives each loaded slot its final slot
 #pragma unroll
             for (int j = 0; j < IM; ++j) {
                 if (valid(j)) {
-                    const uint32_t p = wc[KO::digit(key[j], d) * CS] + rnk[j];
+                    const uint32_t p = wc[digit_of(key[j]) * CS] + rnk[j];
                     s_key[p] = key[j];
                     if (last)
                         s_pos[org[j]] = static_cast<uint16_t>(p);
@@ -1545,7 +1648,7 @@ constexpr uint64_t kMinTileRows = static_cast<uint64_t>(kBlock) * 8;  // smalles
 struct SortLayout {
     uint64_t rows = 0, tiles = 0, items = 0;
     size_t zero_words = 0;  // tickets and histograms (zeroed by every sort)
-    uint64_t lchunks = 0;   // local finish: chunk starts (0: no local finish possible)
+    uint64_t lchunks = 0;   // local finish: chunk words (0: no local finish possible)
     size_t bytes = 0;
 };
 
@@ -1589,8 +1692,8 @@ static SortLayout sort_layout(int key_type, const uint64_t* n, int nsegs, uint64
     b += 2 * align_up(L.rows * 4, 256);
     b += 2 * align_up(L.rows * static_cast<uint64_t>(pay_bytes), 256);
     if (kb == 4 && nsegs == 1 && L.rows >= kLocalMinSortRows) {
-        // local finish: bucket starts, chunk starts, control word
-        L.lchunks = L.rows / kLocalMinH + 2;
+        // local finish: bucket starts, chunk (start, end) pairs, control words
+        L.lchunks = local_chunk_words(L.rows);
         b += align_up(static_cast<size_t>(kRadix) * (kRadix + 1) * 4, 256);
         b += align_up(L.lchunks * 4, 256);
         b += 256;
