@@ -57,11 +57,15 @@ struct KeyOps<uint32_t> {
     {
         return xform(fetch(p, i), mlo, mhi);
     }
+    // the caller's key of a transformed one
+    __device__ static uint32_t unxform(uint32_t k, uint64_t mlo, uint64_t mhi)
+    {
+        const uint32_t x = k ^ static_cast<uint32_t>(mlo);
+        return mhi ? ((x >> 31) ? (x ^ 0x80000000u) : ~x) : x;
+    }
     __device__ static void store(void* p, uint64_t i, uint32_t k, uint64_t mlo, uint64_t mhi)
     {
-        uint32_t x = k ^ static_cast<uint32_t>(mlo);
-        if (mhi) x = (x >> 31) ? (x ^ 0x80000000u) : ~x;
-        gmem(static_cast<uint32_t*>(p))[i] = x;
+        gmem(static_cast<uint32_t*>(p))[i] = unxform(k, mlo, mhi);
     }
     __device__ static uint32_t digit(uint32_t k, int d) { return (k >> (8 * d)) & 0xFFu; }
     __device__ static uint32_t zero() { return 0u; }
@@ -1308,6 +1312,21 @@ __global__ __launch_bounds__(kBlock) void k_local_chunks(SortArgs a)
     }
 }
 
+// Untyped buffer stores (MUBUF): word 3 of the resource for gfx9-family
+// GPUs (data format 32 bits); a lane whose byte offset is at or past the
+// resource's size stores nothing.
+constexpr int kBufferWord3 = 0x00020000;
+__device__ inline void buffer_store(uint32_t v, __amdgpu_buffer_rsrc_t r, int off)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(v, r, off, 0, 0);
+}
+__device__ inline void buffer_store(uint64_t v, __amdgpu_buffer_rsrc_t r, int off)
+{
+    typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+    const u32x2 w = {static_cast<uint32_t>(v), static_cast<uint32_t>(v >> 32)};
+    __builtin_amdgcn_raw_buffer_store_b64(w, r, off, 0, 0);
+}
+
 // Sorts each chunk in LDS and writes the sort's outputs.  A chunk's rows
 // arrive ordered by their top digits (the global passes); stable 8-bit LDS
 // passes, least significant first, over every digit that varies inside the
@@ -1358,10 +1377,18 @@ void k_local(SortArgs a)
     const auto* kin = gmem(static_cast<const uint32_t*>(a.kbuf[1]));
     const auto* pin = gmem(static_cast<const P*>(a.pbuf[1]));
     const auto* pin2 = gmem(static_cast<const P2*>(a.pbuf2[1]));
-    void* keys_out = sd.keys_out;
-    void* perm_out = sd.perm_out;
-    auto* pay_out = gmem(static_cast<P*>(sd.pay_out));
-    auto* pay_out2 = gmem(static_cast<P2*>(sd.pay_out2));
+    // output pointers in scalar registers (buffer resources are scalar; a
+    // pointer the compiler kept in vector registers costs a loop per store)
+    auto sgpr_ptr = [](void* p) __attribute__((always_inline)) -> void* {
+        const uint64_t v = reinterpret_cast<uint64_t>(p);
+        const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v));
+        const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v >> 32));
+        return reinterpret_cast<void*>((static_cast<uint64_t>(hi) << 32) | lo);
+    };
+    void* keys_out = sgpr_ptr(sd.keys_out);
+    void* perm_out = sgpr_ptr(sd.perm_out);
+    void* pay_out = sgpr_ptr(sd.pay_out);
+    void* pay_out2 = sgpr_ptr(sd.pay_out2);
     uint32_t key[IM], rid[IM], rnk[IM], org[IM];
     P pay[PI];
     P2 pay2[PI2];
@@ -1382,9 +1409,11 @@ void k_local(SortArgs a)
     // thread: wave w holds slots [w*64*I, (w+1)*64*I); slot sl0 + 64 j is row
     // r0 + sl0 + 64 j.
     const uint32_t G = gridDim.x;
-    // chunk kk's start and end (past the last chunk: empty, at n)
+    // chunk kk's start and end (past the last chunk: empty, at n); the load
+    // itself is unconditional (a clamped index) so every path issues it
     auto bound = [&](uint32_t kk, uint32_t off) __attribute__((always_inline)) -> uint32_t {
-        return kk < K ? gmem(a.lchunk)[2 * static_cast<uint64_t>(kk) + off] : n;
+        const uint32_t v = gmem(a.lchunk)[2 * static_cast<uint64_t>(std::min(kk, std::max(K, 1u) - 1)) + off];
+        return kk < K ? v : n;
     };
     auto clampm = [](uint32_t r0x, uint32_t r1x) -> uint32_t {
         return r1x > r0x && r1x - r0x <= kLocalRows ? r1x - r0x : 0u;  // chunks are < kLocalRows rows
@@ -1409,9 +1438,19 @@ void k_local(SortArgs a)
     // chunk 0 of this block: bounds, records and first / last keys, chunk 1's bounds
     uint32_t cr0 = bound(blockIdx.x, 0), cm = clampm(cr0, bound(blockIdx.x, 1));
     load_recs(cr0, cm, nx);
-    uint64_t kfl = ROWS ? recs[row_at(cr0, cm, 0)] : kin[row_at(cr0, cm, 0)];
-    uint64_t kll = ROWS ? recs[row_at(cr0, cm, kLocalRows)] : kin[row_at(cr0, cm, kLocalRows)];
+    // a row's key alone (32-bit loads: a 64-bit record whose upper half went
+    // unused made the compiler wait for it, and so for every earlier load)
+    auto key_at = [&](uint32_t r) __attribute__((always_inline)) -> uint32_t {
+        return ROWS ? kin[2 * static_cast<uint64_t>(r)] : kin[r];
+    };
+    uint32_t kfl = key_at(row_at(cr0, cm, 0));
+    uint32_t kll = key_at(row_at(cr0, cm, kLocalRows));
     uint32_t b0 = bound(blockIdx.x + G, 0), b1 = bound(blockIdx.x + G, 1);
+    // waited for here, so that the loop's first wait is placed for the loads
+    // of the loop alone (and leaves the previous chunk's stores in flight)
+#pragma unroll
+    for (int j = 0; j < IM; ++j) asm volatile("" ::"v"(nx[j]));
+    asm volatile("" ::"v"(kfl), "v"(kll), "v"(b0), "v"(b1));
     for (uint32_t k = blockIdx.x; k < K; k += G) {
         const uint32_t r0 = cr0, m = cm;  // this chunk
         const uint32_t I = (m + kLocalBlock - 1) / kLocalBlock;
@@ -1422,8 +1461,8 @@ void k_local(SortArgs a)
             key[j] = static_cast<uint32_t>(nx[j]);
             rid[j] = static_cast<uint32_t>(nx[j] >> 32);
         }
-        const uint32_t kf = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<uint32_t>(kfl)));
-        const uint32_t kl = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<uint32_t>(kll)));
+        const uint32_t kf = __builtin_amdgcn_readfirstlane(kfl);
+        const uint32_t kl = __builtin_amdgcn_readfirstlane(kll);
         // this chunk's payload
 #pragma unroll
         for (int j = 0; j < IM; ++j) {
@@ -1435,13 +1474,14 @@ void k_local(SortArgs a)
         const uint32_t r0n = __builtin_amdgcn_readfirstlane(b0);
         const uint32_t mn = clampm(r0n, __builtin_amdgcn_readfirstlane(b1));
         load_recs(r0n, mn, nx);
-        kfl = ROWS ? recs[row_at(r0n, mn, 0)] : kin[row_at(r0n, mn, 0)];
-        kll = ROWS ? recs[row_at(r0n, mn, kLocalRows)] : kin[row_at(r0n, mn, kLocalRows)];
+        kfl = key_at(row_at(r0n, mn, 0));
+        kll = key_at(row_at(r0n, mn, kLocalRows));
         b0 = bound(k + 2 * G, 0);
         b1 = bound(k + 2 * G, 1);
         cr0 = r0n;
         cm = mn;
-        if (m == 0) continue;
+        // (an empty chunk runs through as well: its loads read one row and its
+        // stores are all dropped, so every path issues the same stores)
         // digits that vary inside the chunk: the low ones, and the global ones
         // where its first and last keys differ (its rows are ordered by them)
         // (two adjacent digits ga: the last low pass also ranks bit 8 = digit ga - fa)
@@ -1551,60 +1591,72 @@ void k_local(SortArgs a)
         }
         lds_barrier();
 
-        // write-out: final slot s goes to output row r0 + s (reversed when asked)
-        auto outpos = [&](uint32_t s) __attribute__((always_inline)) -> uint32_t {
-            const uint32_t g = r0 + s;
-            return a.reverse ? n - 1 - g : g;
+        // Write-out: final slot s goes to output row r0 + s (n - 1 - r0 - s when
+        // reversed).  Every column leaves through buffer stores over the
+        // chunk's m output rows, issued by every lane whatever the chunk and
+        // the outputs asked for: slots past the chunk fall outside the buffer
+        // and are dropped, and so is a column that is not asked for (an empty
+        // buffer).  Vector memory operations complete in issue order; with a
+        // store count fixed on every path, the next iteration's wait for the
+        // loads issued before these stores leaves the stores in flight (with
+        // conditional stores the compiler waited for all of them).
+        const uint64_t orow = a.reverse ? static_cast<uint64_t>(n) - r0 - m : r0;
+        auto rsrc = [&](void* base, uint32_t es) __attribute__((always_inline)) {
+            return __builtin_amdgcn_make_buffer_rsrc(static_cast<char*>(base) + (base ? orow * es : 0), 0,
+                                                     base ? static_cast<int>(m * es) : 0, kBufferWord3);
         };
-        if (keys_out) {
+        auto boff = [&](int u, uint32_t es) __attribute__((always_inline)) -> int {
+            const uint32_t sl = static_cast<uint32_t>(tid + u * kLocalBlock);
+            return static_cast<int>(sl < m ? (a.reverse ? m - 1 - sl : sl) * es : 0x80000000u);
+        };
+        {
+            const auto r = rsrc(keys_out, 4);
             uint32_t kv[IM];
 #pragma unroll
-            for (int u = 0; u < IM; ++u)
-                if (tid + u * kLocalBlock < m) kv[u] = s_key[tid + u * kLocalBlock];
+            for (int u = 0; u < IM; ++u) kv[u] = 0;
+            if (keys_out) {
 #pragma unroll
-            for (int u = 0; u < IM; ++u)
-                if (tid + u * kLocalBlock < m) KO::store(keys_out, outpos(tid + u * kLocalBlock), kv[u], a.mlo, a.mhi);
+                for (int u = 0; u < IM; ++u) kv[u] = KO::unxform(s_key[tid + u * kLocalBlock], a.mlo, a.mhi);
+            }
+#pragma unroll
+            for (int u = 0; u < IM; ++u) buffer_store(kv[u], r, boff(u, 4));
         }
 #pragma unroll
         for (int j = 0; j < IM; ++j)
             if (j < static_cast<int>(I)) rnk[j] = s_pos[sl0 + j * 64];
-        // one column at a time: each row's cell to its final slot in LDS, then out in slot order
-        auto column = [&](auto* stage, auto get, auto put) __attribute__((always_inline)) {
-            lds_barrier();  // the previous phase's LDS reads are done
-#pragma unroll
-            for (int j = 0; j < IM; ++j)
-                if (valid(j)) stage[rnk[j]] = get(j);
-            lds_barrier();
+        // one column at a time: each row's cell to its final slot in LDS, then
+        // out in slot order
+        auto column = [&](auto* stage, auto get, void* base) __attribute__((always_inline)) {
             using T = typename std::remove_reference<decltype(stage[0])>::type;
+            const auto r = rsrc(base, sizeof(T));
             T v[IM];
 #pragma unroll
-            for (int u = 0; u < IM; ++u)
-                if (tid + u * kLocalBlock < m) v[u] = stage[tid + u * kLocalBlock];
+            for (int u = 0; u < IM; ++u) v[u] = 0;
+            if (base) {
+                lds_barrier();  // the previous phase's LDS reads are done
 #pragma unroll
-            for (int u = 0; u < IM; ++u)
-                if (tid + u * kLocalBlock < m) put(outpos(tid + u * kLocalBlock), v[u]);
+                for (int j = 0; j < IM; ++j)
+                    if (valid(j)) stage[rnk[j]] = get(j);
+                lds_barrier();
+#pragma unroll
+                for (int u = 0; u < IM; ++u) v[u] = stage[tid + u * kLocalBlock];  // (slots < kLocalRows)
+            }
+#pragma unroll
+            for (int u = 0; u < IM; ++u) buffer_store(v[u], r, boff(u, sizeof(T)));
         };
         if constexpr (ROWS) {
-            if (perm_out) {
-                if (a.perm64) {
-                    column(s_raw, [&](int j) __attribute__((always_inline)) { return static_cast<uint64_t>(rid[j]); },
-                           [&](uint32_t p, uint64_t v) __attribute__((always_inline)) {
-                               gmem(static_cast<uint64_t*>(perm_out))[p] = v;
-                           });
-                } else {
-                    column(s_key, [&](int j) __attribute__((always_inline)) { return rid[j]; },
-                           [&](uint32_t p, uint32_t v) __attribute__((always_inline)) {
-                               gmem(static_cast<uint32_t*>(perm_out))[p] = v;
-                           });
-                }
-            }
+            if (perm_out && a.perm64)
+                column(s_raw, [&](int j) __attribute__((always_inline)) { return static_cast<uint64_t>(rid[j]); },
+                       perm_out);
+            else
+                column(s_key, [&](int j) __attribute__((always_inline)) { return rid[j]; }, perm_out);
         }
         if constexpr (PB != 0)
             column(reinterpret_cast<P*>(s_raw), [&](int j) __attribute__((always_inline)) { return pay[j]; },
-                   [&](uint32_t p, P v) __attribute__((always_inline)) { pay_out[p] = v; });
+                   pay_out);
         if constexpr (TWO)
             column(reinterpret_cast<P2*>(s_raw), [&](int j) __attribute__((always_inline)) { return pay2[j]; },
-                   [&](uint32_t p, P2 v) __attribute__((always_inline)) { pay_out2[p] = v; });
+                   pay_out2);
         // (the next chunk's first LDS writes to s_raw come three barriers later)
     }
 }
