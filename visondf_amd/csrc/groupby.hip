@@ -754,12 +754,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         t.r0 = t.tbase + lr0;
         t.hb = head_bits<K>(keys, t.r0, t.cnt);
         const uint32_t hcount = __popc(t.hb);
-        uint32_t inc = hcount;  // inclusive scan of head counts over the wave
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const uint32_t y = __shfl_up(inc, off, 64);
-            if (lane >= off) inc += y;
-        }
+        const uint32_t inc = wave_incl_scan(hcount);  // inclusive scan of head counts over the wave
         if (lane == 63) s_wsum[wave] = inc;
         lds_barrier();
         uint32_t pre_w = 0;

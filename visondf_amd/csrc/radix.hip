@@ -431,12 +431,7 @@ __global__ __launch_bounds__(kBlock) void k_plan(SortArgs a)
 __device__ inline uint32_t block_excl_scan_lds(uint32_t x, uint32_t* s_wsum)
 {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    uint32_t inc = x;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t y = __shfl_up(inc, off, 64);
-        if (lane >= off) inc += y;
-    }
+    const uint32_t inc = wave_incl_scan(x);
     if (lane == 63) s_wsum[w] = inc;
     lds_barrier();
     uint32_t pre = 0;
@@ -1548,12 +1543,7 @@ void k_local(SortArgs a)
                     tot += c[w];
                 }
             }
-            uint32_t inc = tot;
-#pragma unroll
-            for (int off = 1; off < 64; off <<= 1) {
-                const uint32_t y = __shfl_up(inc, off, 64);
-                if (lane >= off) inc += y;
-            }
+            const uint32_t inc = wave_incl_scan(tot);
             if (lane == 63) s_wsum[wave] = inc;
             lds_barrier();
             {

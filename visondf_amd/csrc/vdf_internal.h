@@ -342,15 +342,26 @@ int composite_key_type(uint32_t width);
 __device__ inline void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
 // Exclusive scan of one value per thread over a 256-thread block.
+// Inclusive sum over lanes 0 .. lane of a wave (every lane active), by DPP:
+// VALU operand routing instead of six dependent LDS-crossbar permutes.
+// Shifts by 1, 2, 4 and 8 inside each row of 16 lanes (lanes without a
+// source add 0), then lane 15's sum into the next row for rows 1 and 3
+// (row_bcast:15), then lane 31's into rows 2 and 3 (row_bcast:31).
+__device__ inline uint32_t wave_incl_scan(uint32_t x)
+{
+    x += static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(x), 0x111, 0xF, 0xF, false));
+    x += static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(x), 0x112, 0xF, 0xF, false));
+    x += static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(x), 0x114, 0xF, 0xF, false));
+    x += static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(x), 0x118, 0xF, 0xF, false));
+    x += static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(x), 0x142, 0xA, 0xF, false));
+    x += static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(x), 0x143, 0xC, 0xF, false));
+    return x;
+}
+
 __device__ inline uint32_t block_excl_scan(uint32_t x, uint32_t* s_wsum)
 {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    uint32_t inc = x;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t y = __shfl_up(inc, off, 64);
-        if (lane >= off) inc += y;
-    }
+    const uint32_t inc = wave_incl_scan(x);
     if (lane == 63) s_wsum[w] = inc;
     __syncthreads();
     uint32_t pre = 0;
