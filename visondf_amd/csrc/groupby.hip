@@ -372,12 +372,6 @@ __device__ inline uint32_t load_relaxed(uint32_t* p)
 }
 
 // 64-bit wave shuffles as two 32-bit ones
-__device__ inline uint64_t shfl_up64(uint64_t x, int off)
-{
-    const uint32_t lo = __shfl_up(static_cast<uint32_t>(x), off, 64);
-    const uint32_t hi = __shfl_up(static_cast<uint32_t>(x >> 32), off, 64);
-    return (static_cast<uint64_t>(hi) << 32) | lo;
-}
 __device__ inline uint64_t shfl_down64(uint64_t x, int off)
 {
     const uint32_t lo = __shfl_down(static_cast<uint32_t>(x), off, 64);
@@ -385,24 +379,11 @@ __device__ inline uint64_t shfl_down64(uint64_t x, int off)
     return (static_cast<uint64_t>(hi) << 32) | lo;
 }
 
-template <int KIND>
-__device__ inline Agg shfl_up_agg(const Agg& x, int off)
-{
-    Agg y = agg_empty();
-    y.cnt = shfl_up64(x.cnt, off);
-    if (KIND != KIND_COUNT) {
-        y.a = shfl_up64(x.a, off);
-        y.b = shfl_up64(x.b, off);
-        y.mn = shfl_up64(x.mn, off);
-        y.mx = shfl_up64(x.mx, off);
-    }
-    return y;
-}
-
 // Lane moves by DPP (data-parallel primitives: VALU operand routing, no LDS
 // round trip).  CTRL: row_shr:n = 0x110 + n (within 16-lane rows),
-// row_bcast:15 = 0x142, row_bcast:31 = 0x143; RM: rows written.  Lanes a move
-// does not reach are not used by the caller.
+// wave_shr:1 = 0x138 (across the wave), row_bcast:15 = 0x142,
+// row_bcast:31 = 0x143; RM: rows written.  Lanes a move does not reach get 0
+// and are not used by the caller.
 template <int CTRL, int RM>
 __device__ inline uint32_t dpp32(uint32_t x)
 {
@@ -634,8 +615,9 @@ __device__ __attribute__((always_inline)) inline void gb_column(const GbArgs& a,
         s_wagg[wave] = x;
         s_wflag[wave] = f;
     }
-    Agg ex = shfl_up_agg<KIND>(x, 1);
-    uint32_t exf = __shfl_up(f, 1, 64);
+    // each lane's predecessor by DPP wave_shr:1 (lane 0 gets 0, replaced below)
+    Agg ex = dpp_agg<KIND, 0x138, 0xF>(x);
+    uint32_t exf = dpp32<0x138, 0xF>(f);
     if (lane == 0) {
         ex = agg_empty();
         exf = 0;
