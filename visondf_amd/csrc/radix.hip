@@ -862,10 +862,11 @@ void k_pass(SortArgs a, uint32_t slot)
         const uint32_t tn = static_cast<uint32_t>(std::min<uint64_t>(TILE, c.n - c.lrow0));
         const int dig = static_cast<int>(c.dig);
         // Rows go out U per thread at a time: the LDS reads of U rows are
-        // issued before their stores, so one LDS wait covers U rows.  Tiles
-        // are a multiple of U * kBlock rows, so every LDS index is inside the
-        // staging arrays; rows past a partial tile's end are not stored.
-        constexpr int U = 4;
+        // issued before their stores, so one LDS wait covers U rows (8 where
+        // the tile allows it, except in split phases, where 8 measured slower).
+        // Tiles are a multiple of U * kBlock rows, so every LDS index is inside
+        // the staging arrays; rows past a partial tile's end are not stored.
+        constexpr int U = (!SPLIT && ITEMS % 8 == 0) ? 8 : 4;
         static_assert(ITEMS % U == 0, "tile rows per thread must be a multiple of U");
         // staged row i: key and row id
         auto staged = [&](uint32_t i, K& kk, uint32_t& vv) {
