@@ -1355,7 +1355,7 @@ void k_local(SortArgs a)
     // thread reads and writes with paired-dword operations, and with CS odd
     // the counters of one wave fall in distinct banks for 64 consecutive buckets.
     constexpr int CS = W + 1;
-    constexpr int NB = 2 * kRadix;  // buckets of a 9-bit pass (the one thread each)
+    constexpr int NB = 2 * kRadix;  // buckets of a 9-bit pass (one thread each in the scan)
     static_assert(NB == kLocalBlock, "one bucket per thread in the 9-bit scan");
     __shared__ uint32_t s_wc[2][NB * CS];
     __shared__ uint32_t s_wsum[W];
