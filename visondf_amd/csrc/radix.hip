@@ -267,7 +267,6 @@ struct SortArgs {
     uint32_t* lstart;          // local finish: [256][257] start of bucket (y, x) (x = 256: end of y)
     uint32_t* lchunk;          // local finish: chunk start rows (chunks + 1 words)
     uint32_t* lctl;            // local finish: [0] chunk count
-    uint32_t lbw;              // predecessor words per bucket per look-back round (8, or more for few tiles)
 };
 
 // Adds one key per lane to the block histogram.  Digits flagged in `shared`
@@ -826,28 +825,22 @@ void k_pass(SortArgs a, uint32_t slot)
 
     // decoupled look-back for the staged tile (8 predecessor words per step,
     // consumed in tile order until an inclusive prefix), then write it out
-    // Look-back window: 8 predecessor words per round; kernels with register
-    // room take up to kLbwMax when the sort has few tiles, where every tile
-    // starts at once and a tile's walk goes back to the segment's first tile.
-    constexpr int kLbwMax = (!SPLIT && PassShape<K, ITEMS, PB, SPLIT, ROWS>::kBlocksPerCu == 2) ? 32 : 8;
-    const uint32_t nw = std::min<uint32_t>(a.lbw, kLbwMax);
     auto resolve_write = [&](const TileCtx& c, uint32_t tot, uint32_t lstart) {
         uint32_t excl = 0;
         if (c.t > c.t0) {
             uint32_t k = c.t - 1;
             for (;;) {
-                uint32_t w[kLbwMax];
+                uint32_t w[8];
 #pragma unroll
-                for (int q = 0; q < kLbwMax; ++q)
-                    w[q] = static_cast<uint32_t>(q) >= nw ? 0u
-                           : (k >= c.t0 + static_cast<uint32_t>(q))
+                for (int q = 0; q < 8; ++q)
+                    w[q] = (k >= c.t0 + static_cast<uint32_t>(q))
                                ? load_relaxed(lb + static_cast<uint64_t>(k - q) * kRadix + tid)
                                : kFlagP;
                 uint32_t consumed = 0;
                 bool done = false, stall = false;
 #pragma unroll
-                for (int q = 0; q < kLbwMax; ++q) {
-                    if (static_cast<uint32_t>(q) < nw && !done && !stall) {
+                for (int q = 0; q < 8; ++q) {
+                    if (!done && !stall) {
                         const uint32_t f = w[q] & ~kValMask;
                         if (f == 0u) {
                             stall = true;
@@ -1692,7 +1685,6 @@ size_t key_bytes(int key_type)
 static int key_digits(int key_type) { return static_cast<int>(key_bytes(key_type)); }
 
 constexpr uint64_t kMinTileRows = static_cast<uint64_t>(kBlock) * 8;  // smallest pass tile
-constexpr uint64_t kWideLookbackTiles = 1024;  // sorts of at most this many tiles look back in wider rounds
 
 // Scratch of a sort batch.  Tile-indexed arrays are sized for the smallest
 // pass tile, so they hold every tuning choice.
@@ -2084,7 +2076,6 @@ static void launch_sort(Handle* h, const SortSeg* segs, int nsegs, uint64_t mlo,
     a.mlo = mlo;
     a.mhi = mhi;
     a.lb_words = ntiles * kRadix;
-    a.lbw = ntiles <= kWideLookbackTiles ? 32u : 8u;
     a.nsegs = static_cast<uint32_t>(nsegs);
     a.ntiles = static_cast<uint32_t>(ntiles);
     a.perm64 = (flags & VDF_SORT_PERM64) ? 1u : 0u;
