@@ -210,6 +210,14 @@ def test_local_two_bucket_chunks(vdf, mode):
     _check_sort_by(vdf, keys.view(np.int32), 4, mode)
 
 
+def test_local_two_bucket_chunks_perm64_and_without_perm(vdf):
+    """The greedy rule and the 9-bit pass with 64-bit row ids and with the
+    row-id-free kernel (no permutation asked)."""
+    keys = _two_bucket_keys(20_000_011, 8)
+    _check_sort_by(vdf, keys, 8, perm64=True)
+    _check_sort_by(vdf, keys, 4, "reverse", want_perm=False)
+
+
 def test_local_skewed_keys_stay_plain(vdf):
     """A heavy key makes the digit histograms rule the local finish out
     before any pass runs (no wasted passes)."""
